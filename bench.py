@@ -1,0 +1,327 @@
+#!/usr/bin/env python3
+"""Substrate benchmark on one MI355X. Prints ONE JSON line on stdout.
+
+Headline step: the batched edge check of the edges workload (Panda 7-DoF,
+64 random OBBs, 16,384 straight edges of 8..256 states at <= 0.02 rad joint
+step): one persistent launch that returns every edge's any-collision verdict
+and first-hit index, with every state's verdict computed (no skipping past a
+known hit). value = states checked per second.
+
+Also measured (extra fields): first-hit early-exit mode, plain-vs-batched
+any-collision (64 requests of 4,096 configs), plain-vs-batched edges, the
+fused CHOMP adjoint vs finite-difference gradients, STOMP rollout costs
+through a hipGraph, and the Baxter ESDF adjoint; the FP32 CPU restatement
+timed on this host's cores; parity of the timed outputs vs the FP64 oracle.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-extras]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+import traceback
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+METRIC = "collision-checked configs/s + CHOMP cost+grad traj/s, batched vs plain, 1 GPU"
+PEAK_FP32 = 157.3e12          # MI355X FP32 peak (packed issue)
+PEAK_FP32_UNPACKED = 78.6e12  # without packed FP32 (this build disables packed FP32 ops)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def timed_loop(fn, steps, warmup, torch):
+    """Wall time per step (synchronized both sides) and mean HIP-event time of
+    each step on the current stream."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    ev = sum(a.elapsed_time(b) for a, b in evs) / steps / 1e3
+    return wall, ev
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name"):
+                return ln.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the substrate shards by "
+              f"independent requests, see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from prismplan_amd import substrate as S
+    from prismplan_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    props = torch.cuda.get_device_properties(dev)
+    log(f"device {props.name} CUs={props.multi_processor_count}")
+
+    # ------------------------------------------------------------------ headline
+    panda = S.Robot("panda")
+    e = W.edges()
+    ds = S.DeviceScene.from_host(e.scene)
+    plan = S.Plan.edges(panda, e.n_states)
+    frm, to = torch.from_numpy(e.frm).to(dev), torch.from_numpy(e.to).to(dev)
+    fh = torch.empty(len(e.n_states), dtype=torch.int32, device=dev)
+    total_states = e.total_states
+    log(f"edges: E={len(e.n_states)} states={total_states} sha256={e.sha256()[:16]}")
+
+    def step():
+        S.edge_check_batched(plan, ds, frm, to, skip_resolved=False, first_hit=fh)
+
+    wall, ev = timed_loop(step, args.steps, args.warmup, torch)
+    value = total_states / wall
+    log(f"headline: {wall * 1e3:.3f} ms/step wall, {ev * 1e3:.3f} ms event -> {value:.3e} states/s")
+
+    # ------------------------------------------------------------------ CPU baseline leg
+    cpu_baseline, parity, f_state = cpu_baseline_leg(e, fh.cpu().numpy(), args)
+    achieved = f_state * total_states / ev
+    roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32, "traffic": None,
+                "frac_of_unpacked_peak": achieved / PEAK_FP32_UNPACKED,
+                "flops_per_state": f_state, "flops_rule": "gated early-exit count (oracle, 20k-state sample)",
+                "exhaustive_flops_per_state": 114923, "kernel_ms": ev * 1e3}
+
+    extras = {}
+    if not args.no_extras:
+        for name, fn in (("first_hit_skip", lambda: bench_skip(plan, ds, frm, to, fh, total_states, S, torch, args)),
+                         ("any_collision", lambda: bench_any(panda, S, W, torch, dev, args)),
+                         ("edges_plain_vs_batched", lambda: bench_edges_plain(panda, e, ds, S, torch, dev, args)),
+                         ("chomp", lambda: bench_chomp(panda, S, W, torch, dev, args)),
+                         ("stomp", lambda: bench_stomp(S, W, torch, dev, args)),
+                         ("esdf", lambda: bench_esdf(S, W, torch, dev, args))):
+            try:
+                extras[name] = fn()
+                log(f"{name}: {json.dumps(extras[name])}")
+            except Exception as ex:  # keep the headline line even if an extra fails
+                traceback.print_exc()
+                extras[name] = {"error": repr(ex)}
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "configs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32",
+        "data": f"synthetic (seeded PCG64, sha256 {e.sha256()[:16]})",
+        "config": {"workload": "edges: Panda 7-DoF, 64 random OBBs, 16384 straight edges, 8-256 states at <=0.02 rad "
+                               "joint step, per-edge any-collision + first-hit, every state checked",
+                   "edges": int(len(e.n_states)), "states": int(total_states), "obbs": int(e.scene.n_obbs)},
+        "roofline": roofline, "cpu_baseline": cpu_baseline, "parity": parity,
+        "device": props.name, **extras,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_leg(e, fh_gpu, args):
+    """The only place bench.py touches the oracle: time the FP32 CPU
+    restatement on this host's cores, check the timed GPU output against the
+    FP64 oracle, and count the gated algorithmic work per state."""
+    import numpy as np
+    import pyoracle as PO
+    o = PO.Oracle(e.robot, e.scene, nthreads=min(16, os.cpu_count() or 1))
+    n_chk = 2000
+    fh_ref = o.edge_batch(e.frm[:n_chk], e.to[:n_chk], e.n_states[:n_chk])
+    parity = {"edges_checked": n_chk, "first_hit_mismatches": int(np.sum(fh_gpu[:n_chk] != fh_ref))}
+    rng = np.random.default_rng(0)
+    states = PO.edge_states(e.frm, e.to, e.n_states)
+    samp = states[rng.choice(len(states), size=min(20000, len(states)), replace=False)]
+    f_state = float(np.mean(o.gated_flops(samp)))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    threads = max(1, min(threads, len(os.sched_getaffinity(0))))
+    ocpu = PO.Oracle(e.robot, e.scene, nthreads=threads)
+    n = 20000
+    while True:
+        t0 = time.perf_counter()
+        ocpu.check_collision_f32_timed(states[:n])
+        dt = time.perf_counter() - t0
+        if dt > args.cpu_seconds / 4 or n >= len(states):
+            break
+        n = min(len(states), n * 4)
+    reps = min(5, max(1, int(args.cpu_seconds / max(dt, 1e-3))))
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ocpu.check_collision_f32_timed(states[:n])
+        times.append(time.perf_counter() - t0)
+    cpu_value = n / float(np.median(times))
+    o1 = PO.Oracle(e.robot, e.scene, nthreads=1)
+    n1 = min(n, 50000)
+    t0 = time.perf_counter()
+    o1.check_collision_f32_timed(states[:n1])
+    cpu_1 = n1 / (time.perf_counter() - t0)
+    log(f"cpu baseline: {cpu_value:.3e} states/s on {threads} threads, {cpu_1:.3e} on 1")
+    cpu = {"value": cpu_value, "unit": "configs/s", "cores": threads, "kind": "port",
+           "sample": f"first {n} states of the edges workload, FP32 two-phase check, median of {reps}",
+           "one_core_value": cpu_1, "cpu_model": cpu_model()}
+    return cpu, parity, f_state
+
+
+def bench_skip(plan, ds, frm, to, fh, total_states, S, torch, args):
+    def step():
+        S.edge_check_batched(plan, ds, frm, to, skip_resolved=True, first_hit=fh)
+    wall, ev = timed_loop(step, args.steps, args.warmup, torch)
+    return {"states_resolved_per_s": total_states / wall, "edges_per_s": len(plan.unit_states) / wall,
+            "ms_per_step": wall * 1e3}
+
+
+def bench_any(panda, S, W, torch, dev, args):
+    """64 independent any-collision requests of 4,096 configs, each with its
+    own 20 OBBs: 64 plain calls (host arrays, synchronous) vs one batched
+    launch over device-resident configs."""
+    import numpy as np
+    reqs = W.any_collision(n_requests=64)
+    scenes = [S.DeviceScene.from_host(r.scene) for r in reqs]
+    for _ in range(2):
+        for r, sc in zip(reqs, scenes):
+            S.check_collision_scene(panda, sc, r.configs)
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        outs = [S.check_collision_scene(panda, sc, r.configs) for r, sc in zip(reqs, scenes)]
+    plain = 64 * 4096 * reps / (time.perf_counter() - t0)
+    from prismplan_amd.planning_scene import HostScene
+    cfg = torch.from_numpy(np.concatenate([r.configs for r in reqs])).to(dev)
+    hs = HostScene(obb_dims=np.concatenate([r.scene.obb_dims for r in reqs]),
+                   obb_pose=np.concatenate([r.scene.obb_pose for r in reqs]),
+                   obb_enable=np.concatenate([r.scene.obb_enable for r in reqs]))
+    ds = S.DeviceScene.from_host(hs)
+    plan = S.Plan.configs(panda, 64 * 4096, problem_units=np.arange(65) * 4096, obb_offsets=np.arange(65) * 20)
+    out = torch.empty(64 * 4096, dtype=torch.uint8, device=dev)
+    wall, ev = timed_loop(lambda: S.check_collision_batched(plan, ds, cfg, out=out), args.steps, args.warmup, torch)
+    batched = 64 * 4096 / wall
+    same = bool(np.array_equal(out.cpu().numpy().astype(bool), np.concatenate(outs)))
+    # sweep of one request size (single scene) for the launch-bound regime
+    sweep = {}
+    for n in (4096, 65536, 1048576):
+        w = W.any_collision(n_configs=n, seed=0)[0]
+        p2 = S.Plan.configs(panda, n)
+        c2 = torch.from_numpy(w.configs).to(dev)
+        d2 = S.DeviceScene.from_host(w.scene)
+        o2 = torch.empty(n, dtype=torch.uint8, device=dev)
+        wl, evv = timed_loop(lambda: S.check_collision_batched(p2, d2, c2, out=o2), 20, 3, torch)
+        sweep[str(n)] = n / wl
+    return {"plain_configs_per_s": plain, "batched_configs_per_s": batched, "batched_over_plain": batched / plain,
+            "batched_equals_plain": same, "batched_kernel_ms": ev * 1e3, "single_scene_sweep_configs_per_s": sweep}
+
+
+def bench_edges_plain(panda, e, ds, S, torch, dev, args):
+    """64 plain EdgeEvalGpu calls of 256 edges (states j=1..G, G from 0.02 rad
+    resolution) vs the same edges in one batched launch (first-hit skipping on
+    in both)."""
+    import numpy as np
+    ev = S.EdgeEvaluator(panda, ds, max_edges=256)
+    Gs, _ = S.edge_granularities(e.frm, e.to, resolution=0.02)
+    for k in range(4):
+        ev.evaluate(e.frm[k * 256:(k + 1) * 256], e.to[k * 256:(k + 1) * 256], resolution=0.02)
+    t0 = time.perf_counter()
+    valid = []
+    for k in range(64):
+        v, c, _ = ev.evaluate(e.frm[k * 256:(k + 1) * 256], e.to[k * 256:(k + 1) * 256], resolution=0.02)
+        valid.append(v)
+    dt = time.perf_counter() - t0
+    plain_states = float(np.sum(Gs)) / dt
+    # batched over the same states (n = G + 1 with the start skipped)
+    plan = S.Plan.edges(panda, Gs + 1, skip_first=True)
+    frm, to = torch.from_numpy(e.frm).to(dev), torch.from_numpy(e.to).to(dev)
+    fh = torch.empty(len(Gs), dtype=torch.int32, device=dev)
+    wall, evt = timed_loop(lambda: S.edge_check_batched(plan, ds, frm, to, skip_resolved=True, first_hit=fh),
+                           args.steps, args.warmup, torch)
+    batched_states = float(np.sum(Gs)) / wall
+    same = bool(np.array_equal(np.concatenate(valid), fh.cpu().numpy() < 0))
+    return {"plain_states_per_s": plain_states, "batched_states_per_s": batched_states,
+            "batched_over_plain": batched_states / plain_states, "verdicts_equal": same,
+            "plain_ms_per_256_edge_call": dt / 64 * 1e3}
+
+
+def bench_chomp(panda, S, W, torch, dev, args):
+    w = W.chomp()
+    ds = S.DeviceScene.from_host(w.scene)
+    plan = S.Plan.trajectories(panda, w.lengths())
+    th = torch.from_numpy(w.theta).to(dev)
+    prm = S.cost_params(world_margin=0.05, weight=1.0, use_self=False)
+    wall_a, ev_a = timed_loop(lambda: S.hinge_cost(plan, ds, th, prm, grad=True), args.steps, args.warmup, torch)
+    wall_f, ev_f = timed_loop(lambda: S.chomp_fd_grad(plan, ds, th, prm, eps=1e-3), max(5, args.steps // 5), 2, torch)
+    B = w.n_traj
+    f_adj = 5.79e6
+    return {"adjoint_traj_per_s": B / wall_a, "fd_traj_per_s": B / wall_f, "adjoint_over_fd": wall_f / wall_a,
+            "adjoint_ms": wall_a * 1e3, "fd_ms": wall_f * 1e3,
+            "adjoint_roofline_frac": B * f_adj / ev_a / PEAK_FP32,
+            "fd_roofline_frac": B * 80.9e6 / ev_f / PEAK_FP32}
+
+
+def bench_stomp(S, W, torch, dev, args):
+    import numpy as np
+    wk = W.stomp()
+    fetch = S.Robot("fetch")
+    R = wk.n_rollouts
+    plan = S.Plan.trajectories(fetch, np.repeat(wk.lengths, R), problem_units=np.arange(wk.n_problems + 1) * R,
+                               obb_offsets=wk.obb_offsets)
+    ds = S.DeviceScene.from_host(wk.scene)
+    ro = torch.from_numpy(wk.rollouts).to(dev)
+    prm = S.cost_params(world_margin=0.02, self_margin=0.01, weight=20.0)
+    state = torch.empty(wk.n_states, device=dev)
+    total = torch.empty(wk.n_problems * R, device=dev)
+    g = S.capture_stomp(plan, ds, ro, prm, 0.1, 0.1, state, total)
+    wall, ev = timed_loop(lambda: g.launch(), args.steps, args.warmup, torch)
+    wall_e, _ = timed_loop(lambda: S.stomp_rollout_costs(plan, ds, ro, prm, 0.1, 0.1), args.steps, args.warmup, torch)
+    f_state = 251199.0
+    return {"rollout_states_per_s": wk.n_states / wall, "rollouts_per_s": wk.n_problems * R / wall,
+            "graph_ms": wall * 1e3, "eager_ms": wall_e * 1e3, "states": wk.n_states,
+            "roofline_frac_exhaustive": wk.n_states * f_state / ev / PEAK_FP32}
+
+
+def bench_esdf(S, W, torch, dev, args):
+    baxter = S.Robot("baxter")
+    wk = W.esdf()
+    obbs = S.DeviceScene.from_host(wk.scene.source_obbs)
+    params, pose = wk.scene.voxel_params, wk.scene.voxel_pose
+    wall_b, _ = timed_loop(lambda: S.esdf_from_obbs(obbs, params[0], pose[0]), 10, 2, torch)
+    feats = S.esdf_from_obbs(obbs, params[0], pose[0])
+    ds = S.DeviceScene(None, None, None, torch.from_numpy(params).to(dev), torch.from_numpy(pose).to(dev),
+                       torch.ones(1, dtype=torch.uint8, device=dev), feats, esdf_interp=1)
+    plan = S.Plan.trajectories(baxter, wk.lengths())
+    th = torch.from_numpy(wk.theta).to(dev)
+    prm = S.cost_params(world_margin=0.02, weight=1.0, use_obb=False, use_self=False)
+    wall, ev = timed_loop(lambda: S.hinge_cost(plan, ds, th, prm, grad=True), args.steps, args.warmup, torch)
+    return {"waypoints_per_s": wk.n_waypoints / wall, "trajectories_per_s": wk.n_traj / wall,
+            "adjoint_ms": wall * 1e3, "esdf_build_ms": wall_b * 1e3, "waypoints": wk.n_waypoints,
+            "roofline_frac": wk.n_waypoints * (8137 + 5487) / ev / PEAK_FP32}
+
+
+if __name__ == "__main__":
+    main()

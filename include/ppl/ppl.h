@@ -1,0 +1,256 @@
+/*
+ * ppl.h -- C ABI of the MI355X collision-kinematics substrate (libppl.so).
+ *
+ * Plain pointers and sizes only; every entry point returns a ppl_status and
+ * never exits or throws across the ABI. Reference paths below are relative to
+ * code/gpu_planning/cpp/src/ of PrismPlan; each declaration names the
+ * reference interface it replaces.
+ *
+ * Memory: "h_" arguments are host arrays, "d_" arguments device arrays
+ * (HBM). Streams are hipStream_t passed as void* (NULL = default stream).
+ * Calls that share one ppl_robot, ppl_edge_eval or ppl_plan must not run
+ * concurrently on different streams: each handle owns one work counter.
+ */
+#ifndef PPL_PPL_H
+#define PPL_PPL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PPL_OK = 0,
+    PPL_ERR_INVALID = 1,      /* bad argument, shape or table */
+    PPL_ERR_HIP = 2,          /* a HIP runtime call failed */
+    PPL_ERR_ALLOC = 3,        /* device or pinned allocation failed */
+    PPL_ERR_UNSUPPORTED = 4,  /* valid for the reference, not handled here */
+    PPL_ERR_CAPACITY = 5      /* above a documented size limit */
+} ppl_status;
+
+typedef void* ppl_stream;                 /* hipStream_t */
+typedef struct ppl_robot_s* ppl_robot;
+typedef struct ppl_edge_eval_s* ppl_edge_eval;
+typedef struct ppl_plan_s* ppl_plan;
+
+/* Last error message of the calling thread ("" if none). */
+const char* ppl_last_error(void);
+/* Library version string. */
+const char* ppl_version(void);
+
+/* ------------------------------------------------------------------------
+ * Runtime robot model.
+ * Replaces RobotModelDevice::create / destroy (planning/robot_model.cuh:91-122,
+ * 262-293) and the RobotModelBuilder fields (python/bindings.cpp:118-160).
+ * Host arrays, copied at create time. Topology arrays may be NULL for a serial
+ * chain (same defaults as robot_model.cuh:205-236); approx_fixed_transforms /
+ * approx_joint_types may be NULL (= full chain).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int n_dof;
+    int n_joints;
+    const float* fixed_transforms;        /* [n_joints*16] row-major 4x4 */
+    const int* joint_types;               /* [n_joints] -1 fixed, 0..2 prism x/y/z, 3..5 rot x/y/z */
+    int n_spheres;
+    const float* spheres;                 /* [n_spheres*4] x, y, z, r (link frame) */
+    const int* sphere_to_joint;           /* [n_spheres] */
+    int n_self_cc_ranges;
+    const int* self_cc_ranges;            /* [n*3] sphere_i, start, end (inclusive) */
+    int n_approx_spheres;
+    const float* approx_spheres;          /* [n_approx*4] */
+    const int* approx_sphere_to_joint;    /* [n_approx] */
+    int n_approx_self_cc_ranges;
+    const int* approx_self_cc_ranges;     /* [n*3] */
+    const float* approx_fixed_transforms; /* NULL or [n_joints*16] */
+    const int* approx_joint_types;        /* NULL or [n_joints] */
+    const float* joint_lower;             /* [n_dof] */
+    const float* joint_upper;             /* [n_dof] */
+    const int* joint_parents;             /* NULL or [n_joints] */
+    const int* joint_id_to_dof;           /* NULL or [n_joints] */
+    const int* t_memory_idx;              /* NULL or [n_joints] */
+    const int* dfs_order;                 /* NULL or [n_joints] */
+} ppl_robot_desc;
+
+typedef struct {
+    int n_dof, n_joints, n_spheres, n_approx_spheres, n_self_pairs, n_t_slots;
+} ppl_robot_info;
+
+ppl_status ppl_robot_create(const ppl_robot_desc* desc, ppl_robot* out);
+ppl_status ppl_robot_destroy(ppl_robot robot);
+ppl_status ppl_robot_get_info(ppl_robot robot, ppl_robot_info* info);
+/* RobotModelBuilder::update_joint_limits (python/bindings.cpp:180-206). */
+ppl_status ppl_robot_update_joint_limits(ppl_robot robot, const float* h_lower, const float* h_upper);
+
+/* ------------------------------------------------------------------------
+ * Scene view: non-owning DEVICE pointers with the field meanings of
+ * SceneCollisionData (collision/scene_collision_data.hh:10-47).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    const float* obb_dims;        /* [n_obbs*4] full extents dx, dy, dz, pad */
+    const float* obb_pose;        /* [n_obbs*8] inverse pose tx, ty, tz, qw, qx, qy, qz, pad */
+    const uint8_t* obb_enable;    /* [n_obbs] */
+    int n_obbs;
+    const float* voxel_params;    /* [L*4] dx, dy, dz, voxel_size (DEVICE) */
+    const float* voxel_pose;      /* [L*8] inverse grid pose (DEVICE) */
+    const uint8_t* voxel_enable;  /* [L] */
+    const float* voxel_features;  /* [L*max_voxels_per_layer], positive inside */
+    int n_voxel_layers;
+    int max_voxels_per_layer;
+    const uint8_t* sphere_acm_mask;             /* [S*L] or NULL */
+    const uint8_t* sphere_obb_acm_mask;         /* [S*O] or NULL */
+    const uint8_t* sphere_obb_acm_mask_approx;  /* [A*O] or NULL (ignored, as in the reference) */
+    int esdf_interp;              /* 0: nearest voxel (reference), 1: trilinear (value + gradient) */
+} ppl_scene_view;
+
+/* ------------------------------------------------------------------------
+ * Plain per-request forms (the drop-in).
+ * ------------------------------------------------------------------------ */
+
+/* pRRTC::check_collision_scene (planning/Planners.hh:141-148,
+ * planning/pRRTC.cu:2388-2423): two-phase environment-only any-collision of N
+ * host configurations [N*n_dof] row-major; h_results[i] = 1 on collision.
+ * Synchronous. */
+ppl_status ppl_check_collision_scene(ppl_robot robot, const ppl_scene_view* scene, const float* h_configs, int n,
+                                     float collision_margin, uint8_t* h_results);
+
+/* ppl::search::EdgeEvalParams (planning/edge_eval.cuh:37-51). */
+typedef struct {
+    int granularity;          /* states per edge when resolution <= 0 */
+    float collision_margin;
+    int check_self;
+    int compute_clearance;    /* min ESDF clearance over the edge's states */
+    int enable_mesh;          /* must be 0 */
+    float resolution;         /* > 0: ceil(len / resolution) states, clamped to [2, 1024] */
+} ppl_edge_params;
+
+/* ppl::search::EdgeEvalGpu (planning/edge_batch_bridge.hh:27-111): persistent
+ * device and pinned buffers for up to max_edges edges, private stream. */
+ppl_status ppl_edge_eval_create(ppl_robot robot, const ppl_scene_view* scene, int dim, int max_edges,
+                                ppl_edge_eval* out);
+ppl_status ppl_edge_eval_destroy(ppl_edge_eval ev);
+/* EdgeEvalGpu::evaluate (edge_batch_bridge.hh:60-96): states j = 1..G at
+ * t = j/G of each edge; valid = no collision, cost = |to - from| if valid
+ * else FLT_MAX, clearance (nullable) = min ESDF clearance if requested. */
+ppl_status ppl_edge_eval_evaluate(ppl_edge_eval ev, const float* h_from, const float* h_to, int n_edges,
+                                  const ppl_edge_params* params, uint8_t* h_valid, float* h_cost,
+                                  float* h_clearance);
+
+/* Host-side state counts of the reference granularity rule
+ * (edge_eval.cuh:54-60) for n edges: h_G[e] states (j = 1..G), h_len[e] the
+ * FP32 configuration-space length. Pure host code. */
+ppl_status ppl_edge_granularities(const float* h_from, const float* h_to, int n_edges, int dim,
+                                  const ppl_edge_params* params, int* h_G, float* h_len);
+
+/* Summed-hinge cost parameters (chomp.cu:83-188, stomp.cu:206-246). */
+typedef struct {
+    float world_margin;       /* OBB and ESDF hinge band (raw radius) */
+    float self_margin;        /* self min-clearance hinge band */
+    float weight;             /* collision_cost_weight */
+    int use_obb;
+    int use_esdf;
+    int use_self;
+} ppl_cost_params;
+
+/* chomp_state_cost_kernel (chomp.cu:151-188) for one trajectory
+ * d_theta [D][T] (dim-major): d_state_costs [D][T], D-broadcast. */
+ppl_status ppl_chomp_state_cost(ppl_robot robot, const ppl_scene_view* scene, const float* d_theta, int T,
+                                const ppl_cost_params* params, float* d_state_costs, ppl_stream stream);
+
+/* chomp_obs_grad_kernel (chomp.cu:83-144) for one trajectory: central finite
+ * difference of the summed hinge, d_grad [D][T], 0 at both ends. */
+ppl_status ppl_chomp_fd_grad(ppl_robot robot, const ppl_scene_view* scene, const float* d_theta, int T,
+                             const ppl_cost_params* params, float eps, float* d_grad, ppl_stream stream);
+
+/* ------------------------------------------------------------------------
+ * Batched forms: one persistent launch over a plan (a flattened state space
+ * cut into 64-state chunks that never span two problems). No counterpart in
+ * the reference, which solves one request per launch.
+ * ------------------------------------------------------------------------ */
+typedef enum { PPL_PLAN_CONFIGS = 0, PPL_PLAN_EDGES = 1, PPL_PLAN_TRAJ = 2 } ppl_plan_kind;
+
+typedef struct {
+    int kind;                 /* ppl_plan_kind */
+    int n_units;              /* configurations, edges or trajectories */
+    const int* unit_states;   /* host [n_units]: states per edge / waypoints per trajectory; NULL for configs */
+    int skip_first;           /* edges: state 0 (the from-configuration) is not checked */
+    int n_problems;           /* >= 1 */
+    const int* problem_units; /* host [n_problems+1] unit offsets; NULL = one problem */
+    const int* obb_offsets;   /* host [n_problems+1] OBB ranges; NULL = all OBBs for every problem */
+} ppl_plan_desc;
+
+ppl_status ppl_plan_create(ppl_robot robot, const ppl_plan_desc* desc, ppl_plan* out);
+ppl_status ppl_plan_destroy(ppl_plan plan);
+/* Total states of the plan (configs, checked edge states or waypoints). */
+long ppl_plan_states(ppl_plan plan);
+
+/* Any-collision of every configuration, d_configs [n*n_dof] row-major.
+ * check_self adds the self check (two_phase_cc check_self, MIT* batch_cc). */
+ppl_status ppl_check_collision_batched(ppl_plan plan, const ppl_scene_view* scene, const float* d_configs,
+                                       float collision_margin, int check_self, uint8_t* d_out, ppl_stream stream);
+
+typedef struct {
+    float collision_margin;
+    int check_self;
+    int skip_resolved;        /* 1: states past an edge's known first hit are not checked */
+} ppl_edge_batch_params;
+
+/* Edges of a PPL_PLAN_EDGES plan: edge e has n_e = unit_states[e] (+1 if
+ * skip_first) states at t = k/(n_e-1). d_first_hit[e] = smallest colliding
+ * k, or -1. d_state_hits (nullable, only with skip_resolved = 0): per-state
+ * verdicts in plan order. */
+ppl_status ppl_edge_check_batched(ppl_plan plan, const ppl_scene_view* scene, const float* d_from, const float* d_to,
+                                  const ppl_edge_batch_params* params, int* d_first_hit, uint8_t* d_state_hits,
+                                  ppl_stream stream);
+
+/* scene_min_clearance_runtime (collision/scene_collision.cuh:339-384): min
+ * over spheres of ESDF clearance with radius + margin, per configuration. */
+ppl_status ppl_min_clearance_batched(ppl_plan plan, const ppl_scene_view* scene, const float* d_configs,
+                                     float collision_margin, float* d_out, ppl_stream stream);
+
+/* Summed-hinge cost of every state (configs or trajectories): d_state_cost
+ * [states] weighted per-state cost, d_total (nullable, trajectory plans)
+ * per-trajectory sum. d_grad (nullable): d cost / d q in the input layout
+ * (trajectory plans: 0 at both ends), computed by the fused adjoint. */
+ppl_status ppl_hinge_cost_batched(ppl_plan plan, const ppl_scene_view* scene, const float* d_x,
+                                  const ppl_cost_params* params, float* d_state_cost, float* d_total, float* d_grad,
+                                  ppl_stream stream);
+
+/* Central-FD obstacle gradient over a trajectory plan (chomp_obs_grad_kernel
+ * semantics), the baseline for the fused adjoint. */
+ppl_status ppl_chomp_fd_grad_batched(ppl_plan plan, const ppl_scene_view* scene, const float* d_theta,
+                                     const ppl_cost_params* params, float eps, float* d_grad, ppl_stream stream);
+
+/* STOMP rollout costs (stomp.cu:206-308) over a trajectory plan whose
+ * trajectories are the rollouts of each problem: per-waypoint state cost and
+ * per-rollout total = sum_t state + w_u * sum_d theta_d^T R theta_d with R
+ * the 5-point acceleration metric of step dt (chomp_precompute.cpp:20-38). */
+ppl_status ppl_stomp_rollout_costs(ppl_plan plan, const ppl_scene_view* scene, const float* d_rollouts,
+                                   const ppl_cost_params* params, float control_weight, float dt,
+                                   float* d_state_cost, float* d_total, ppl_stream stream);
+
+/* ESDF of a box world on a node grid: value = -min_o sdf_o(node), positive
+ * inside. h_voxel_params [4] and h_voxel_pose [8] on the host; d_features
+ * holds nx*ny*nz floats, index ix*ny*nz + iy*nz + iz. */
+ppl_status ppl_esdf_from_obbs(const ppl_scene_view* obbs, const float* h_voxel_params, const float* h_voxel_pose,
+                              float* d_features, ppl_stream stream);
+
+/* ------------------------------------------------------------------------
+ * hipGraph capture of a plan's cost evaluation (one optimiser iteration's
+ * substrate work) for replay without per-launch host overhead.
+ * ------------------------------------------------------------------------ */
+typedef struct ppl_graph_s* ppl_graph;
+ppl_status ppl_graph_capture_stomp(ppl_plan plan, const ppl_scene_view* scene, const float* d_rollouts,
+                                   const ppl_cost_params* params, float control_weight, float dt,
+                                   float* d_state_cost, float* d_total, ppl_stream stream, ppl_graph* out);
+ppl_status ppl_graph_capture_hinge(ppl_plan plan, const ppl_scene_view* scene, const float* d_x,
+                                   const ppl_cost_params* params, float* d_state_cost, float* d_total,
+                                   float* d_grad, ppl_stream stream, ppl_graph* out);
+ppl_status ppl_graph_launch(ppl_graph graph, ppl_stream stream);
+ppl_status ppl_graph_destroy(ppl_graph graph);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PPL_PPL_H */

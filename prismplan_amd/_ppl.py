@@ -1,0 +1,138 @@
+"""ctypes binding of libppl.so (include/ppl/ppl.h).
+
+torch is imported first: its bundled HIP runtime carries the same SONAME as
+the one libppl.so links, so the process keeps a single runtime and device
+pointers from torch tensors are valid in the library. Loading fails loudly
+when the library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import torch  # noqa: F401  (must precede the library load, see above)
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libppl.so"
+
+PPL_OK, PPL_ERR_INVALID, PPL_ERR_HIP, PPL_ERR_ALLOC, PPL_ERR_UNSUPPORTED, PPL_ERR_CAPACITY = range(6)
+PLAN_CONFIGS, PLAN_EDGES, PLAN_TRAJ = 0, 1, 2
+
+_f32p = C.POINTER(C.c_float)
+_i32p = C.POINTER(C.c_int)
+_u8p = C.POINTER(C.c_uint8)
+_vp = C.c_void_p
+
+
+class PplError(RuntimeError):
+    def __init__(self, status: int, where: str, msg: str):
+        super().__init__(f"{where} failed with status {status}: {msg}")
+        self.status = status
+
+
+class RobotDesc(C.Structure):
+    _fields_ = [("n_dof", C.c_int), ("n_joints", C.c_int),
+                ("fixed_transforms", _f32p), ("joint_types", _i32p),
+                ("n_spheres", C.c_int), ("spheres", _f32p), ("sphere_to_joint", _i32p),
+                ("n_self_cc_ranges", C.c_int), ("self_cc_ranges", _i32p),
+                ("n_approx_spheres", C.c_int), ("approx_spheres", _f32p), ("approx_sphere_to_joint", _i32p),
+                ("n_approx_self_cc_ranges", C.c_int), ("approx_self_cc_ranges", _i32p),
+                ("approx_fixed_transforms", _f32p), ("approx_joint_types", _i32p),
+                ("joint_lower", _f32p), ("joint_upper", _f32p),
+                ("joint_parents", _i32p), ("joint_id_to_dof", _i32p), ("t_memory_idx", _i32p),
+                ("dfs_order", _i32p)]
+
+
+class RobotInfo(C.Structure):
+    _fields_ = [("n_dof", C.c_int), ("n_joints", C.c_int), ("n_spheres", C.c_int), ("n_approx_spheres", C.c_int),
+                ("n_self_pairs", C.c_int), ("n_t_slots", C.c_int)]
+
+
+class SceneView(C.Structure):
+    _fields_ = [("obb_dims", _vp), ("obb_pose", _vp), ("obb_enable", _vp), ("n_obbs", C.c_int),
+                ("voxel_params", _vp), ("voxel_pose", _vp), ("voxel_enable", _vp), ("voxel_features", _vp),
+                ("n_voxel_layers", C.c_int), ("max_voxels_per_layer", C.c_int),
+                ("sphere_acm_mask", _vp), ("sphere_obb_acm_mask", _vp), ("sphere_obb_acm_mask_approx", _vp),
+                ("esdf_interp", C.c_int)]
+
+
+class EdgeParams(C.Structure):
+    _fields_ = [("granularity", C.c_int), ("collision_margin", C.c_float), ("check_self", C.c_int),
+                ("compute_clearance", C.c_int), ("enable_mesh", C.c_int), ("resolution", C.c_float)]
+
+
+class CostParams(C.Structure):
+    _fields_ = [("world_margin", C.c_float), ("self_margin", C.c_float), ("weight", C.c_float),
+                ("use_obb", C.c_int), ("use_esdf", C.c_int), ("use_self", C.c_int)]
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("n_units", C.c_int), ("unit_states", _i32p), ("skip_first", C.c_int),
+                ("n_problems", C.c_int), ("problem_units", _i32p), ("obb_offsets", _i32p)]
+
+
+class EdgeBatchParams(C.Structure):
+    _fields_ = [("collision_margin", C.c_float), ("check_self", C.c_int), ("skip_resolved", C.c_int)]
+
+
+# (name, restype, argtypes) of every exported entry point of ppl.h
+SIGNATURES = [
+    ("ppl_last_error", C.c_char_p, []),
+    ("ppl_version", C.c_char_p, []),
+    ("ppl_robot_create", C.c_int, [C.POINTER(RobotDesc), C.POINTER(_vp)]),
+    ("ppl_robot_destroy", C.c_int, [_vp]),
+    ("ppl_robot_get_info", C.c_int, [_vp, C.POINTER(RobotInfo)]),
+    ("ppl_robot_update_joint_limits", C.c_int, [_vp, _f32p, _f32p]),
+    ("ppl_check_collision_scene", C.c_int, [_vp, C.POINTER(SceneView), _f32p, C.c_int, C.c_float, _u8p]),
+    ("ppl_edge_eval_create", C.c_int, [_vp, C.POINTER(SceneView), C.c_int, C.c_int, C.POINTER(_vp)]),
+    ("ppl_edge_eval_destroy", C.c_int, [_vp]),
+    ("ppl_edge_eval_evaluate", C.c_int, [_vp, _f32p, _f32p, C.c_int, C.POINTER(EdgeParams), _u8p, _f32p, _f32p]),
+    ("ppl_edge_granularities", C.c_int, [_f32p, _f32p, C.c_int, C.c_int, C.POINTER(EdgeParams), _i32p, _f32p]),
+    ("ppl_chomp_state_cost", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.c_int, C.POINTER(CostParams), _vp, _vp]),
+    ("ppl_chomp_fd_grad", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.c_int, C.POINTER(CostParams), C.c_float, _vp,
+                                    _vp]),
+    ("ppl_plan_create", C.c_int, [_vp, C.POINTER(PlanDesc), C.POINTER(_vp)]),
+    ("ppl_plan_destroy", C.c_int, [_vp]),
+    ("ppl_plan_states", C.c_long, [_vp]),
+    ("ppl_check_collision_batched", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.c_float, C.c_int, _vp, _vp]),
+    ("ppl_edge_check_batched", C.c_int, [_vp, C.POINTER(SceneView), _vp, _vp, C.POINTER(EdgeBatchParams), _vp, _vp,
+                                         _vp]),
+    ("ppl_min_clearance_batched", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.c_float, _vp, _vp]),
+    ("ppl_hinge_cost_batched", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), _vp, _vp, _vp, _vp]),
+    ("ppl_chomp_fd_grad_batched", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), C.c_float, _vp,
+                                            _vp]),
+    ("ppl_stomp_rollout_costs", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), C.c_float, C.c_float,
+                                          _vp, _vp, _vp]),
+    ("ppl_esdf_from_obbs", C.c_int, [C.POINTER(SceneView), _f32p, _f32p, _vp, _vp]),
+    ("ppl_graph_capture_stomp", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), C.c_float,
+                                          C.c_float, _vp, _vp, _vp, C.POINTER(_vp)]),
+    ("ppl_graph_capture_hinge", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), _vp, _vp, _vp, _vp,
+                                          C.POINTER(_vp)]),
+    ("ppl_graph_launch", C.c_int, [_vp, _vp]),
+    ("ppl_graph_destroy", C.c_int, [_vp]),
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"libppl.so not built at {LIB_PATH}; run `python -m prismplan_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int, where: str) -> None:
+    if status != PPL_OK:
+        msg = lib().ppl_last_error().decode(errors="replace")
+        raise PplError(status, where, msg)
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args), name)

@@ -1,0 +1,1212 @@
+// capi.hip -- C ABI of the substrate (include/ppl/ppl.h): validation, device
+// table construction, plans, and the plain / batched / differentiable entry
+// points. No exceptions or exits cross the ABI; every failure returns a
+// ppl_status and leaves a message for ppl_last_error().
+#include "ppl/ppl.h"
+#include "ppl_internal.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace ppl;
+
+namespace {
+
+thread_local std::string g_err;
+
+ppl_status fail(ppl_status s, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return s;
+}
+
+#define HIP_TRY(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return fail(PPL_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHECK_ARG(cond, ...)                                     \
+    do {                                                         \
+        if (!(cond)) return fail(PPL_ERR_INVALID, __VA_ARGS__); \
+    } while (0)
+
+// Growable device buffer.
+struct DBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max(bytes, (size_t)256);
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Growable pinned host buffer.
+struct HBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max(bytes, (size_t)256);
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+}  // namespace
+
+struct ppl_robot_s {
+    DRobot R{};
+    void* blob = nullptr;
+    int n_slots = 1;
+    int n_pairs = 0;
+    bool subtrees_contiguous = true;
+    std::vector<float> lower, upper;
+    hipStream_t stream = nullptr;
+    unsigned* counter = nullptr;   // plain-call work counter
+    DBuf d_cfg, d_out, d_off, d_aux, d_self;
+    HBuf h_stage;
+};
+
+struct ppl_edge_eval_s {
+    ppl_robot robot = nullptr;
+    DScene S{};
+    int dim = 0, max_edges = 0;
+    hipStream_t stream = nullptr;
+    unsigned* counter = nullptr;
+    DBuf d_from, d_to, d_off, d_fh, d_clr_state, d_clr_edge, d_self;
+    HBuf h_from, h_to, h_off, h_fh, h_clr;
+};
+
+struct ppl_plan_s {
+    ppl_robot robot = nullptr;
+    int kind = 0;
+    int n_units = 0;
+    long n_states = 0;
+    int k0 = 0;
+    int n_problems = 1;
+    std::vector<int> h_obb_off;      // empty: single scene
+    std::vector<int> h_off;          // edges / trajectories: state offsets [n_units+1]
+    int* d_off = nullptr;            // [n_units+1]
+    int4* d_tab = nullptr;
+    int n_chunks = 0;
+    int* d_obb_off = nullptr;
+    unsigned* counter = nullptr;
+    DBuf d_self, d_cost, d_fdcfg, d_fdcost;
+};
+
+struct ppl_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+ppl_status scene_from_view(const ppl_scene_view* v, DScene& S) {
+    CHECK_ARG(v != nullptr, "scene is NULL");
+    CHECK_ARG(v->n_obbs >= 0 && v->n_voxel_layers >= 0, "negative scene sizes");
+    if (v->n_obbs > 0)
+        CHECK_ARG(v->obb_dims && v->obb_pose && v->obb_enable, "scene has %d OBBs but NULL OBB arrays", v->n_obbs);
+    if (v->n_voxel_layers > 0)
+        CHECK_ARG(v->voxel_params && v->voxel_pose && v->voxel_enable && v->voxel_features && v->max_voxels_per_layer > 0,
+                  "scene has voxel layers but NULL voxel arrays");
+    S.obb_dims = v->obb_dims;
+    S.obb_pose = v->obb_pose;
+    S.obb_enable = v->obb_enable;
+    S.n_obbs = v->n_obbs;
+    S.voxel_params = v->voxel_params;
+    S.voxel_pose = v->voxel_pose;
+    S.voxel_enable = v->voxel_enable;
+    S.voxel_features = v->voxel_features;
+    S.n_voxel_layers = v->n_voxel_layers;
+    S.max_voxels_per_layer = v->max_voxels_per_layer;
+    S.sphere_acm = v->sphere_acm_mask;
+    S.sphere_obb_acm = v->sphere_obb_acm_mask;
+    S.sphere_obb_acm_approx = v->sphere_obb_acm_mask_approx;
+    S.esdf_interp = v->esdf_interp ? 1 : 0;
+    return PPL_OK;
+}
+
+ppl_status check_obb_capacity(int n) {
+    if (n > kMaxObbLds)
+        return fail(PPL_ERR_CAPACITY, "%d OBBs in one problem exceeds the staged limit of %d", n, kMaxObbLds);
+    return PPL_OK;
+}
+
+enum KernelKind { kKCollide, kKCost, kKCostGrad };
+
+// Persistent grid for a kernel family at the LDS footprint of this launch.
+int grid_for(KernelKind k, const DRobot& R, int obb_cap) {
+    const size_t lds = lds_bytes(obb_cap, R.n_dof);
+    return k == kKCollide ? grid_collide(lds) : grid_cost(R.n_joints, k == kKCostGrad, lds);
+}
+
+DChunks implicit_chunks(long n_states, unsigned* counter, KernelKind k, const DRobot& R, int obb_cap, int* grid_out) {
+    const int grid_cap = grid_for(k, R, obb_cap);
+    DChunks C{};
+    C.obb_cap = obb_cap;
+    C.tab = nullptr;
+    C.n_chunks = (int)((n_states + kWave - 1) / kWave);
+    C.obb_off = nullptr;
+    C.counter = counter;
+    const int grid = std::max(1, std::min(grid_cap, C.n_chunks));
+    C.n_waves = grid;
+    *grid_out = grid;
+    return C;
+}
+
+DChunks plan_chunks(const ppl_plan_s* P, KernelKind k, int obb_cap, int* grid_out) {
+    const int grid_cap = grid_for(k, P->robot->R, obb_cap);
+    DChunks C{};
+    C.obb_cap = obb_cap;
+    C.tab = P->d_tab;
+    C.n_chunks = P->n_chunks;
+    C.obb_off = P->d_obb_off;
+    C.counter = P->counter;
+    const int grid = std::max(1, std::min(grid_cap, C.n_chunks));
+    C.n_waves = grid;
+    *grid_out = grid;
+    return C;
+}
+
+DStates plan_states(const ppl_plan_s* P, const float* x) {
+    DStates X{};
+    X.n_states = P->n_states;
+    const int D = P->robot->R.n_dof;
+    if (P->kind == PPL_PLAN_CONFIGS) {
+        X.source = kSrcConfig;
+        X.cfg = x;
+        X.cfg_stride = D;
+    } else if (P->kind == PPL_PLAN_EDGES) {
+        X.source = kSrcEdge;
+        X.edge_off = P->d_off;
+        X.n_edges = P->n_units;
+        X.k0 = P->k0;
+    } else {
+        X.source = kSrcTraj;
+        X.theta = x;
+        X.traj_off = P->d_off;
+        X.n_traj = P->n_units;
+    }
+    return X;
+}
+
+// Reference granularity rule (edge_eval.cuh:54-60) in FP32 with fused
+// accumulation, identical to the device interpolation's inputs.
+int edge_g(const float* a, const float* b, int dim, const ppl_edge_params& p, float* len_out) {
+    float len2 = 0.0f;
+    for (int i = 0; i < dim; ++i) {
+        const float d = b[i] - a[i];
+        len2 = std::fma(d, d, len2);
+    }
+    const float len = std::sqrt(len2);
+    *len_out = len;
+    if (!(p.resolution > 0.0f)) return p.granularity;
+    int g = (int)std::ceil(len / p.resolution);
+    return std::min(std::max(g, 2), 1024);
+}
+
+ppl_status run_self(const ppl_robot_s* r, const DStates& X, int mode, uint8_t* hit_out, float* out, hipStream_t s) {
+    SelfArgs a{};
+    a.R = r->R;
+    a.X = X;
+    a.mode = mode;
+    a.hit_out = hit_out;
+    a.out = out;
+    HIP_TRY(launch_self(a, s));
+    return PPL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ppl_last_error(void) { return g_err.c_str(); }
+const char* ppl_version(void) { return "ppl 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------------------
+// Robot
+// ---------------------------------------------------------------------------
+ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
+    CHECK_ARG(d && out, "NULL argument");
+    *out = nullptr;
+    const int J = d->n_joints, D = d->n_dof, S = d->n_spheres, A = d->n_approx_spheres;
+    CHECK_ARG(D >= 1 && D <= kMaxDof, "n_dof %d outside [1, %d]", D, kMaxDof);
+    CHECK_ARG(J >= 1 && J <= kMaxJoints, "n_joints %d outside [1, %d]", J, kMaxJoints);
+    CHECK_ARG(S >= 0 && A >= 0, "negative sphere counts");
+    CHECK_ARG(d->fixed_transforms && d->joint_types && d->joint_lower && d->joint_upper, "NULL robot tables");
+    CHECK_ARG(S == 0 || (d->spheres && d->sphere_to_joint), "NULL sphere tables");
+    CHECK_ARG(A == 0 || (d->approx_spheres && d->approx_sphere_to_joint), "NULL approx sphere tables");
+    CHECK_ARG(d->n_self_cc_ranges >= 0 && (d->n_self_cc_ranges == 0 || d->self_cc_ranges), "bad self ranges");
+    CHECK_ARG(d->n_approx_self_cc_ranges >= 0 && (d->n_approx_self_cc_ranges == 0 || d->approx_self_cc_ranges),
+              "bad approx self ranges");
+    // Serial-chain defaults, robot_model.cuh:205-236.
+    std::vector<int> parents(J), id2dof(J), tmem(J), dfs(J);
+    for (int i = 0; i < J; ++i) {
+        parents[i] = d->joint_parents ? d->joint_parents[i] : (i > 0 ? i - 1 : 0);
+        id2dof[i] = d->joint_id_to_dof ? d->joint_id_to_dof[i] : i - 1;
+        tmem[i] = d->t_memory_idx ? d->t_memory_idx[i] : (i == 0 ? 0 : 1);
+        dfs[i] = d->dfs_order ? d->dfs_order[i] : i;
+    }
+    std::vector<int> pos(J, -1);
+    for (int k = 0; k < J; ++k) {
+        CHECK_ARG(dfs[k] >= 0 && dfs[k] < J && pos[dfs[k]] < 0, "dfs_order is not a permutation");
+        pos[dfs[k]] = k;
+    }
+    const int root = dfs[0];
+    CHECK_ARG(parents[root] == root, "dfs_order[0] must be the root (its own parent)");
+    int n_slots = 1;
+    for (int i = 0; i < J; ++i) {
+        CHECK_ARG(parents[i] >= 0 && parents[i] < J, "joint_parents[%d] out of range", i);
+        if (i != root) CHECK_ARG(pos[parents[i]] < pos[i], "joint %d visited before its parent", i);
+        CHECK_ARG(id2dof[i] >= -1 && id2dof[i] < D, "joint_id_to_dof[%d] out of range", i);
+        CHECK_ARG(tmem[i] >= 0, "negative t_memory_idx");
+        n_slots = std::max(n_slots, tmem[i] + 1);
+        const int t = d->joint_types[i];
+        CHECK_ARG(t >= -1 && t <= 5, "joint_types[%d] = %d unknown", i, t);
+        const float* ft = d->fixed_transforms + 16 * i;
+        CHECK_ARG(std::fabs(ft[12]) < 1e-6f && std::fabs(ft[13]) < 1e-6f && std::fabs(ft[14]) < 1e-6f &&
+                      std::fabs(ft[15] - 1.0f) < 1e-6f,
+                  "fixed_transforms[%d] last row is not [0 0 0 1]", i);
+        if (d->approx_fixed_transforms) {
+            for (int k = 0; k < 16; ++k)
+                if (d->approx_fixed_transforms[16 * i + k] != ft[k])
+                    return fail(PPL_ERR_UNSUPPORTED, "approx chain differing from the full chain is not supported");
+        }
+        if (d->approx_joint_types && d->approx_joint_types[i] != t)
+            return fail(PPL_ERR_UNSUPPORTED, "approx joint types differing from the full chain are not supported");
+    }
+    if (n_slots > kMaxSlots) return fail(PPL_ERR_UNSUPPORTED, "%d FK slots requested, %d supported", n_slots, kMaxSlots);
+    // The slot each joint reads must still hold its parent's frame.
+    {
+        std::vector<int> owner(n_slots, -1);
+        for (int k = 0; k < J; ++k) {
+            const int i = dfs[k];
+            if (k > 0) CHECK_ARG(owner[tmem[parents[i]]] == parents[i], "slot of joint %d's parent was overwritten", i);
+            owner[tmem[i]] = i;
+        }
+    }
+    for (int s = 0; s < S; ++s) CHECK_ARG(d->sphere_to_joint[s] >= 0 && d->sphere_to_joint[s] < J, "sphere_to_joint[%d]", s);
+    for (int a = 0; a < A; ++a)
+        CHECK_ARG(d->approx_sphere_to_joint[a] >= 0 && d->approx_sphere_to_joint[a] < J, "approx_sphere_to_joint[%d]", a);
+    std::vector<int2> pairs, apairs;
+    for (int e = 0; e < d->n_self_cc_ranges; ++e) {
+        const int i = d->self_cc_ranges[3 * e], a = d->self_cc_ranges[3 * e + 1], b = d->self_cc_ranges[3 * e + 2];
+        CHECK_ARG(i >= 0 && i < S && a >= 0 && b < S, "self_cc_ranges[%d] out of range", e);
+        for (int j = a; j <= b; ++j) pairs.push_back(make_int2(i, j));
+    }
+    for (int e = 0; e < d->n_approx_self_cc_ranges; ++e) {
+        const int i = d->approx_self_cc_ranges[3 * e], a = d->approx_self_cc_ranges[3 * e + 1],
+                  b = d->approx_self_cc_ranges[3 * e + 2];
+        CHECK_ARG(i >= 0 && i < A && a >= 0 && b < A, "approx_self_cc_ranges[%d] out of range", e);
+        for (int j = a; j <= b; ++j) apairs.push_back(make_int2(i, j));
+    }
+    // DFS-ordered tables.
+    std::vector<float> ft(12 * J);
+    std::vector<int4> meta(J);
+    for (int k = 0; k < J; ++k) {
+        const int i = dfs[k];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) ft[12 * k + 4 * r + c] = d->fixed_transforms[16 * i + 4 * r + c];
+        meta[k] = make_int4(d->joint_types[i], id2dof[i], tmem[i], tmem[parents[i]]);
+    }
+    auto emit = [&](int n, const float* sp, const int* s2j, std::vector<int>& off, std::vector<float4>& v,
+                    std::vector<int>& id) {
+        off.assign(J + 1, 0);
+        for (int k = 0; k < J; ++k) {
+            off[k] = (int)v.size();
+            for (int s = 0; s < n; ++s)
+                if (s2j[s] == dfs[k]) {
+                    v.push_back(make_float4(sp[4 * s], sp[4 * s + 1], sp[4 * s + 2], sp[4 * s + 3]));
+                    id.push_back(s);
+                }
+        }
+        off[J] = (int)v.size();
+    };
+    std::vector<int> sph_off, apx_off, sph_id, apx_id;
+    std::vector<float4> sph, apx;
+    emit(S, d->spheres, d->sphere_to_joint, sph_off, sph, sph_id);
+    emit(A, d->approx_spheres, d->approx_sphere_to_joint, apx_off, apx, apx_id);
+    std::vector<float4> sph_orig(S), apx_orig(A);
+    for (int s = 0; s < S; ++s) sph_orig[s] = make_float4(d->spheres[4 * s], d->spheres[4 * s + 1], d->spheres[4 * s + 2], d->spheres[4 * s + 3]);
+    for (int a = 0; a < A; ++a)
+        apx_orig[a] = make_float4(d->approx_spheres[4 * a], d->approx_spheres[4 * a + 1], d->approx_spheres[4 * a + 2],
+                                  d->approx_spheres[4 * a + 3]);
+    // Subtree of DFS position p = [p, end_p); close_mask[k] marks the p with end_p == k.
+    std::vector<unsigned> close(J + 1, 0u);
+    bool contiguous = true;
+    for (int p = 0; p < J; ++p) {
+        auto is_desc = [&](int j) {
+            for (int g = 0; g <= J; ++g) {
+                if (j == dfs[p]) return true;
+                if (j == parents[j]) return false;
+                j = parents[j];
+            }
+            return false;
+        };
+        int end = p + 1;
+        while (end < J && is_desc(dfs[end])) ++end;
+        for (int q = end; q < J; ++q)
+            if (is_desc(dfs[q])) contiguous = false;
+        close[end] |= 1u << p;
+    }
+    // One device blob for every table.
+    size_t off = 0;
+    auto place = [&](size_t bytes) {
+        const size_t o = off;
+        off = (off + bytes + 15) & ~(size_t)15;
+        return o;
+    };
+    const size_t o_ft = place(ft.size() * 4), o_meta = place(meta.size() * 16), o_soff = place(sph_off.size() * 4),
+                 o_sph = place(sph.size() * 16), o_sid = place(sph_id.size() * 4), o_aoff = place(apx_off.size() * 4),
+                 o_apx = place(apx.size() * 16), o_aid = place(apx_id.size() * 4), o_close = place(close.size() * 4),
+                 o_sorig = place(sph_orig.size() * 16), o_pairs = place(pairs.size() * 8),
+                 o_apairs = place(apairs.size() * 8), o_aorig = place(apx_orig.size() * 16);
+    std::vector<char> host(off + 16, 0);
+    auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) memcpy(host.data() + o, src, bytes); };
+    put(o_ft, ft.data(), ft.size() * 4);
+    put(o_meta, meta.data(), meta.size() * 16);
+    put(o_soff, sph_off.data(), sph_off.size() * 4);
+    put(o_sph, sph.data(), sph.size() * 16);
+    put(o_sid, sph_id.data(), sph_id.size() * 4);
+    put(o_aoff, apx_off.data(), apx_off.size() * 4);
+    put(o_apx, apx.data(), apx.size() * 16);
+    put(o_aid, apx_id.data(), apx_id.size() * 4);
+    put(o_close, close.data(), close.size() * 4);
+    put(o_sorig, sph_orig.data(), sph_orig.size() * 16);
+    put(o_pairs, pairs.data(), pairs.size() * 8);
+    put(o_apairs, apairs.data(), apairs.size() * 8);
+    put(o_aorig, apx_orig.data(), apx_orig.size() * 16);
+
+    ppl_robot_s* r = new (std::nothrow) ppl_robot_s();
+    if (!r) return fail(PPL_ERR_ALLOC, "out of host memory");
+    if (hipMalloc(&r->blob, host.size()) != hipSuccess) {
+        delete r;
+        return fail(PPL_ERR_ALLOC, "hipMalloc of %zu bytes for robot tables failed", host.size());
+    }
+    hipError_t e = hipMemcpy(r->blob, host.data(), host.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void**)&r->counter, 2 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(r->counter, 0, 2 * sizeof(unsigned));
+    if (e != hipSuccess) {
+        ppl_robot_destroy(r);
+        return fail(PPL_ERR_HIP, "robot upload failed: %s", hipGetErrorString(e));
+    }
+    char* b = static_cast<char*>(r->blob);
+    DRobot& R = r->R;
+    R.n_dof = D; R.n_joints = J; R.n_sph = S; R.n_apx = A; R.n_slots = n_slots;
+    R.n_pairs = (int)pairs.size(); R.n_apx_pairs = (int)apairs.size();
+    R.ft = reinterpret_cast<const float*>(b + o_ft);
+    R.jmeta = reinterpret_cast<const int4*>(b + o_meta);
+    R.sph_off = reinterpret_cast<const int*>(b + o_soff);
+    R.sph = reinterpret_cast<const float4*>(b + o_sph);
+    R.sph_id = reinterpret_cast<const int*>(b + o_sid);
+    R.apx_off = reinterpret_cast<const int*>(b + o_aoff);
+    R.apx = reinterpret_cast<const float4*>(b + o_apx);
+    R.apx_id = reinterpret_cast<const int*>(b + o_aid);
+    R.close_mask = reinterpret_cast<const unsigned*>(b + o_close);
+    R.sph_orig = reinterpret_cast<const float4*>(b + o_sorig);
+    R.pairs = reinterpret_cast<const int2*>(b + o_pairs);
+    R.apx_pairs = reinterpret_cast<const int2*>(b + o_apairs);
+    R.apx_orig = reinterpret_cast<const float4*>(b + o_aorig);
+    r->n_slots = n_slots;
+    r->n_pairs = (int)pairs.size();
+    r->subtrees_contiguous = contiguous;
+    r->lower.assign(d->joint_lower, d->joint_lower + D);
+    r->upper.assign(d->joint_upper, d->joint_upper + D);
+    *out = r;
+    return PPL_OK;
+}
+
+ppl_status ppl_robot_destroy(ppl_robot r) {
+    if (!r) return PPL_OK;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    if (r->blob) (void)hipFree(r->blob);
+    if (r->counter) (void)hipFree(r->counter);
+    r->d_cfg.release(); r->d_out.release(); r->d_off.release(); r->d_aux.release(); r->d_self.release();
+    r->h_stage.release();
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+    return PPL_OK;
+}
+
+ppl_status ppl_robot_get_info(ppl_robot r, ppl_robot_info* info) {
+    CHECK_ARG(r && info, "NULL argument");
+    info->n_dof = r->R.n_dof;
+    info->n_joints = r->R.n_joints;
+    info->n_spheres = r->R.n_sph;
+    info->n_approx_spheres = r->R.n_apx;
+    info->n_self_pairs = r->n_pairs;
+    info->n_t_slots = r->n_slots;
+    return PPL_OK;
+}
+
+ppl_status ppl_robot_update_joint_limits(ppl_robot r, const float* lo, const float* hi) {
+    CHECK_ARG(r && lo && hi, "NULL argument");
+    r->lower.assign(lo, lo + r->R.n_dof);
+    r->upper.assign(hi, hi + r->R.n_dof);
+    return PPL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Plain any-collision (pRRTC::check_collision_scene)
+// ---------------------------------------------------------------------------
+ppl_status ppl_check_collision_scene(ppl_robot r, const ppl_scene_view* view, const float* h_cfg, int n, float margin,
+                                     uint8_t* h_out) {
+    CHECK_ARG(r, "robot is NULL");
+    CHECK_ARG(n >= 0, "negative N");
+    if (n == 0) return PPL_OK;
+    CHECK_ARG(h_cfg && h_out, "NULL host arrays");
+    DScene S{};
+    ppl_status st = scene_from_view(view, S);
+    if (st != PPL_OK) return st;
+    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
+    const int D = r->R.n_dof;
+    const size_t cfg_bytes = (size_t)n * D * sizeof(float);
+    HIP_TRY(r->d_cfg.ensure(cfg_bytes));
+    HIP_TRY(r->d_out.ensure((size_t)n));
+    HIP_TRY(r->h_stage.ensure(std::max(cfg_bytes, (size_t)n)));
+    memcpy(r->h_stage.p, h_cfg, cfg_bytes);
+    HIP_TRY(hipMemcpyAsync(r->d_cfg.p, r->h_stage.p, cfg_bytes, hipMemcpyHostToDevice, r->stream));
+    CollideArgs a{};
+    a.R = r->R;
+    a.S = S;
+    a.X.source = kSrcConfig;
+    a.X.n_states = n;
+    a.X.cfg = r->d_cfg.as<float>();
+    a.X.cfg_stride = D;
+    int grid = 0;
+    a.C = implicit_chunks(n, r->counter, kKCollide, r->R, S.n_obbs, &grid);
+    a.margin = margin;
+    a.mode = 0;
+    a.out = r->d_out.as<uint8_t>();
+    HIP_TRY(launch_collide(a, grid, r->stream));
+    HIP_TRY(hipMemcpyAsync(r->h_stage.p, r->d_out.p, (size_t)n, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    memcpy(h_out, r->h_stage.p, (size_t)n);
+    return PPL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Plain edge evaluator (EdgeEvalGpu)
+// ---------------------------------------------------------------------------
+ppl_status ppl_edge_eval_create(ppl_robot r, const ppl_scene_view* view, int dim, int max_edges, ppl_edge_eval* out) {
+    CHECK_ARG(r && out, "NULL argument");
+    *out = nullptr;
+    CHECK_ARG(dim == r->R.n_dof, "dim %d != robot n_dof %d", dim, r->R.n_dof);
+    CHECK_ARG(max_edges > 0, "max_edges must be positive");
+    DScene S{};
+    ppl_status st = scene_from_view(view, S);
+    if (st != PPL_OK) return st;
+    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
+    ppl_edge_eval_s* ev = new (std::nothrow) ppl_edge_eval_s();
+    if (!ev) return fail(PPL_ERR_ALLOC, "out of host memory");
+    ev->robot = r;
+    ev->S = S;
+    ev->dim = dim;
+    ev->max_edges = max_edges;
+    const size_t cfg = (size_t)max_edges * dim * sizeof(float);
+    hipError_t e = hipStreamCreateWithFlags(&ev->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void**)&ev->counter, 2 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(ev->counter, 0, 2 * sizeof(unsigned));
+    if (e == hipSuccess) e = ev->d_from.ensure(cfg);
+    if (e == hipSuccess) e = ev->d_to.ensure(cfg);
+    if (e == hipSuccess) e = ev->d_off.ensure((size_t)(max_edges + 1) * 4);
+    if (e == hipSuccess) e = ev->d_fh.ensure((size_t)max_edges * 4);
+    if (e == hipSuccess) e = ev->h_from.ensure(cfg);
+    if (e == hipSuccess) e = ev->h_to.ensure(cfg);
+    if (e == hipSuccess) e = ev->h_off.ensure((size_t)(max_edges + 1) * 4);
+    if (e == hipSuccess) e = ev->h_fh.ensure((size_t)max_edges * 4);
+    if (e == hipSuccess) e = ev->h_clr.ensure((size_t)max_edges * 4);
+    if (e != hipSuccess) {
+        ppl_edge_eval_destroy(ev);
+        return fail(PPL_ERR_ALLOC, "edge evaluator buffers: %s", hipGetErrorString(e));
+    }
+    *out = ev;
+    return PPL_OK;
+}
+
+ppl_status ppl_edge_eval_destroy(ppl_edge_eval ev) {
+    if (!ev) return PPL_OK;
+    if (ev->stream) (void)hipStreamSynchronize(ev->stream);
+    if (ev->counter) (void)hipFree(ev->counter);
+    ev->d_from.release(); ev->d_to.release(); ev->d_off.release(); ev->d_fh.release();
+    ev->d_clr_state.release(); ev->d_clr_edge.release(); ev->d_self.release();
+    ev->h_from.release(); ev->h_to.release(); ev->h_off.release(); ev->h_fh.release(); ev->h_clr.release();
+    if (ev->stream) (void)hipStreamDestroy(ev->stream);
+    delete ev;
+    return PPL_OK;
+}
+
+}  // extern "C"
+
+namespace {
+__global__ void k_segment_min(const float* x, const int* off, int n_seg, float* out) {
+    const int b = blockIdx.x;
+    if (b >= n_seg) return;
+    const int lane = threadIdx.x;
+    float m = FLT_MAX;
+    for (int i = off[b] + lane; i < off[b + 1]; i += 64) m = fminf(m, x[i]);
+    for (int o = 32; o >= 1; o >>= 1) m = fminf(m, __shfl_xor(m, o));
+    if (lane == 0) out[b] = m;
+}
+}  // namespace
+
+extern "C" {
+
+ppl_status ppl_edge_granularities(const float* h_from, const float* h_to, int n, int dim, const ppl_edge_params* p,
+                                  int* h_G, float* h_len) {
+    CHECK_ARG(h_from && h_to && p && h_G, "NULL argument");
+    CHECK_ARG(n >= 0 && dim >= 1, "bad sizes");
+    for (int e = 0; e < n; ++e) {
+        float len = 0.0f;
+        h_G[e] = edge_g(h_from + (size_t)e * dim, h_to + (size_t)e * dim, dim, *p, &len);
+        if (h_len) h_len[e] = len;
+    }
+    return PPL_OK;
+}
+
+ppl_status ppl_edge_eval_evaluate(ppl_edge_eval ev, const float* h_from, const float* h_to, int n, const ppl_edge_params* p,
+                                  uint8_t* h_valid, float* h_cost, float* h_clr) {
+    CHECK_ARG(ev && p, "NULL argument");
+    CHECK_ARG(n >= 0, "negative edge count");
+    if (n == 0) return PPL_OK;
+    if (n > ev->max_edges) return fail(PPL_ERR_CAPACITY, "num_edges %d exceeds max_edges %d", n, ev->max_edges);
+    CHECK_ARG(h_from && h_to && h_valid && h_cost, "NULL host arrays");
+    if (p->enable_mesh) return fail(PPL_ERR_UNSUPPORTED, "mesh edge checks are not supported");
+    ppl_robot r = ev->robot;
+    const int D = ev->dim;
+    int* off = ev->h_off.as<int>();
+    std::vector<float> len(n);
+    off[0] = 0;
+    for (int e = 0; e < n; ++e) {
+        const int G = edge_g(h_from + (size_t)e * D, h_to + (size_t)e * D, D, *p, &len[e]);
+        off[e + 1] = off[e] + std::max(G, 0);
+    }
+    const long total = off[n];
+    const size_t cfg = (size_t)n * D * sizeof(float);
+    memcpy(ev->h_from.p, h_from, cfg);
+    memcpy(ev->h_to.p, h_to, cfg);
+    HIP_TRY(hipMemcpyAsync(ev->d_from.p, ev->h_from.p, cfg, hipMemcpyHostToDevice, ev->stream));
+    HIP_TRY(hipMemcpyAsync(ev->d_to.p, ev->h_to.p, cfg, hipMemcpyHostToDevice, ev->stream));
+    HIP_TRY(hipMemcpyAsync(ev->d_off.p, ev->h_off.p, (size_t)(n + 1) * 4, hipMemcpyHostToDevice, ev->stream));
+    HIP_TRY(launch_fill_i32(ev->d_fh.as<int>(), -1, n, ev->stream));
+    DStates X{};
+    X.source = kSrcEdge;
+    X.n_states = total;
+    X.from = ev->d_from.as<float>();
+    X.to = ev->d_to.as<float>();
+    X.edge_off = ev->d_off.as<int>();
+    X.n_edges = n;
+    X.k0 = 1;
+    // With G = granularity <= 1 the +1 state count gives t = k/G as the reference.
+    const uint8_t* self_hit = nullptr;
+    if (total > 0 && p->check_self && r->n_pairs > 0) {
+        HIP_TRY(ev->d_self.ensure((size_t)total));
+        ppl_status st = run_self(r, X, 0, ev->d_self.as<uint8_t>(), nullptr, ev->stream);
+        if (st != PPL_OK) return st;
+        self_hit = ev->d_self.as<uint8_t>();
+    }
+    if (total > 0) {
+        CollideArgs a{};
+        a.R = r->R;
+        a.S = ev->S;
+        a.X = X;
+        int grid = 0;
+        a.C = implicit_chunks(total, ev->counter, kKCollide, r->R, ev->S.n_obbs, &grid);
+        a.margin = p->collision_margin;
+        a.mode = 1;
+        a.skip_resolved = 1;
+        a.first_hit = ev->d_fh.as<int>();
+        a.self_hit = self_hit;
+        HIP_TRY(launch_collide(a, grid, ev->stream));
+    }
+    const bool want_clr = p->compute_clearance && h_clr;
+    if (want_clr && total > 0) {
+        HIP_TRY(ev->d_clr_state.ensure((size_t)total * 4));
+        HIP_TRY(ev->d_clr_edge.ensure((size_t)n * 4));
+        CostArgs c{};
+        c.R = r->R;
+        c.S = ev->S;
+        c.X = X;
+        int grid = 0;
+        c.C = implicit_chunks(total, ev->counter, kKCost, r->R, ev->S.n_obbs, &grid);
+        c.P.weight = 1.0f;
+        c.clearance_only = 1;
+        c.margin = p->collision_margin;
+        c.cost_out = ev->d_clr_state.as<float>();
+        HIP_TRY(launch_cost(c, grid, ev->stream));
+        hipLaunchKernelGGL(k_segment_min, dim3(n), dim3(64), 0, ev->stream, ev->d_clr_state.as<float>(),
+                           ev->d_off.as<int>(), n, ev->d_clr_edge.as<float>());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(ev->h_clr.p, ev->d_clr_edge.p, (size_t)n * 4, hipMemcpyDeviceToHost, ev->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(ev->h_fh.p, ev->d_fh.p, (size_t)n * 4, hipMemcpyDeviceToHost, ev->stream));
+    HIP_TRY(hipStreamSynchronize(ev->stream));
+    const int* fh = ev->h_fh.as<int>();
+    for (int e = 0; e < n; ++e) {
+        const bool ok = fh[e] < 0;
+        h_valid[e] = ok ? 1 : 0;
+        h_cost[e] = ok ? len[e] : FLT_MAX;
+    }
+    if (h_clr) {
+        for (int e = 0; e < n; ++e) {
+            const bool has = want_clr && off[e + 1] > off[e];
+            h_clr[e] = has ? ev->h_clr.as<float>()[e] : FLT_MAX;
+        }
+    }
+    return PPL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
+ppl_status ppl_plan_create(ppl_robot r, const ppl_plan_desc* d, ppl_plan* out) {
+    CHECK_ARG(r && d && out, "NULL argument");
+    *out = nullptr;
+    CHECK_ARG(d->kind >= PPL_PLAN_CONFIGS && d->kind <= PPL_PLAN_TRAJ, "unknown plan kind %d", d->kind);
+    CHECK_ARG(d->n_units >= 0, "negative n_units");
+    CHECK_ARG(d->n_problems >= 1, "n_problems must be >= 1");
+    CHECK_ARG(d->kind == PPL_PLAN_CONFIGS || d->unit_states, "unit_states required for edge/trajectory plans");
+    ppl_plan_s* P = new (std::nothrow) ppl_plan_s();
+    if (!P) return fail(PPL_ERR_ALLOC, "out of host memory");
+    P->robot = r;
+    P->kind = d->kind;
+    P->n_units = d->n_units;
+    P->k0 = (d->kind == PPL_PLAN_EDGES && d->skip_first) ? 1 : 0;
+    P->n_problems = d->n_problems;
+    // unit -> state offsets
+    std::vector<long> uoff(d->n_units + 1, 0);
+    for (int u = 0; u < d->n_units; ++u) {
+        long c = 1;
+        if (d->kind == PPL_PLAN_EDGES) c = std::max(0, d->unit_states[u] - P->k0);
+        if (d->kind == PPL_PLAN_TRAJ) c = d->unit_states[u];
+        if (c < 0 || (d->kind == PPL_PLAN_TRAJ && c < 1)) {
+            delete P;
+            return fail(PPL_ERR_INVALID, "unit %d has invalid state count", u);
+        }
+        uoff[u + 1] = uoff[u] + c;
+    }
+    if (uoff[d->n_units] > (long)INT32_MAX - 64) {
+        delete P;
+        return fail(PPL_ERR_CAPACITY, "more than 2^31 states in one plan");
+    }
+    P->n_states = uoff[d->n_units];
+    std::vector<int> pu(d->n_problems + 1);
+    for (int p = 0; p <= d->n_problems; ++p) pu[p] = d->problem_units ? d->problem_units[p] : (p == 0 ? 0 : d->n_units);
+    if (!d->problem_units && d->n_problems != 1) {
+        delete P;
+        return fail(PPL_ERR_INVALID, "problem_units required when n_problems > 1");
+    }
+    for (int p = 0; p < d->n_problems; ++p)
+        if (pu[p] < 0 || pu[p] > pu[p + 1] || pu[p + 1] > d->n_units) {
+            delete P;
+            return fail(PPL_ERR_INVALID, "problem_units not monotone in [0, n_units]");
+        }
+    if (pu[0] != 0 || pu[d->n_problems] != d->n_units) {
+        delete P;
+        return fail(PPL_ERR_INVALID, "problem_units must cover [0, n_units]");
+    }
+    if (d->obb_offsets) {
+        P->h_obb_off.assign(d->obb_offsets, d->obb_offsets + d->n_problems + 1);
+        for (int p = 0; p < d->n_problems; ++p) {
+            const int n = P->h_obb_off[p + 1] - P->h_obb_off[p];
+            if (n < 0 || n > kMaxObbLds) {
+                delete P;
+                return fail(n < 0 ? PPL_ERR_INVALID : PPL_ERR_CAPACITY, "problem %d has %d OBBs (limit %d)", p, n, kMaxObbLds);
+            }
+        }
+    }
+    std::vector<int4> tab;
+    for (int p = 0; p < d->n_problems; ++p) {
+        const long b = uoff[pu[p]], e = uoff[pu[p + 1]];
+        for (long s = b; s < e; s += kWave) tab.push_back(make_int4(p, (int)s, (int)std::min(s + kWave, e), 0));
+    }
+    P->n_chunks = (int)tab.size();
+    P->h_off.assign(uoff.size(), 0);
+    for (size_t i = 0; i < uoff.size(); ++i) P->h_off[i] = (int)uoff[i];
+    hipError_t e = hipMalloc((void**)&P->counter, 2 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(P->counter, 0, 2 * sizeof(unsigned));
+    if (e == hipSuccess && !tab.empty()) e = hipMalloc((void**)&P->d_tab, tab.size() * sizeof(int4));
+    if (e == hipSuccess && !tab.empty()) e = hipMemcpy(P->d_tab, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void**)&P->d_off, P->h_off.size() * sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(P->d_off, P->h_off.data(), P->h_off.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !P->h_obb_off.empty()) {
+        e = hipMalloc((void**)&P->d_obb_off, P->h_obb_off.size() * sizeof(int));
+        if (e == hipSuccess)
+            e = hipMemcpy(P->d_obb_off, P->h_obb_off.data(), P->h_obb_off.size() * sizeof(int), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        ppl_plan_destroy(P);
+        return fail(PPL_ERR_HIP, "plan upload: %s", hipGetErrorString(e));
+    }
+    *out = P;
+    return PPL_OK;
+}
+
+ppl_status ppl_plan_destroy(ppl_plan P) {
+    if (!P) return PPL_OK;
+    (void)hipDeviceSynchronize();
+    if (P->counter) (void)hipFree(P->counter);
+    if (P->d_tab) (void)hipFree(P->d_tab);
+    if (P->d_off) (void)hipFree(P->d_off);
+    if (P->d_obb_off) (void)hipFree(P->d_obb_off);
+    P->d_self.release(); P->d_cost.release(); P->d_fdcfg.release(); P->d_fdcost.release();
+    delete P;
+    return PPL_OK;
+}
+
+long ppl_plan_states(ppl_plan P) { return P ? P->n_states : -1; }
+
+}  // extern "C"
+
+namespace {
+
+// LDS records needed: the largest per-problem OBB range.
+int plan_obb_cap(const ppl_plan_s* P, const DScene& S) {
+    if (P->h_obb_off.empty()) return S.n_obbs;
+    int m = 0;
+    for (size_t p = 0; p + 1 < P->h_obb_off.size(); ++p) m = std::max(m, P->h_obb_off[p + 1] - P->h_obb_off[p]);
+    return m;
+}
+
+ppl_status plan_scene(const ppl_plan_s* P, const ppl_scene_view* view, DScene& S) {
+    ppl_status st = scene_from_view(view, S);
+    if (st != PPL_OK) return st;
+    if (P->h_obb_off.empty()) return check_obb_capacity(S.n_obbs);
+    if (P->h_obb_off.back() > S.n_obbs)
+        return fail(PPL_ERR_INVALID, "plan OBB offsets reach %d but the scene has %d OBBs", P->h_obb_off.back(), S.n_obbs);
+    return PPL_OK;
+}
+
+// Self pass (min clearance + argmin) for cost plans, into the plan's scratch.
+ppl_status cost_self(ppl_plan_s* P, const DStates& X, const ppl_cost_params* prm, hipStream_t s, const float** self_in) {
+    *self_in = nullptr;
+    if (!prm->use_self || P->robot->n_pairs == 0 || X.n_states == 0) return PPL_OK;
+    HIP_TRY(P->d_self.ensure((size_t)X.n_states * 8 * sizeof(float)));
+    ppl_status st = run_self(P->robot, X, 1, nullptr, P->d_self.as<float>(), s);
+    if (st != PPL_OK) return st;
+    *self_in = P->d_self.as<float>();
+    return PPL_OK;
+}
+
+ppl_status hinge_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_x, const ppl_cost_params* prm,
+                      float* d_state, float* d_total, float* d_grad, hipStream_t s) {
+    CHECK_ARG(P && prm && d_x, "NULL argument");
+    CHECK_ARG(P->kind != PPL_PLAN_EDGES, "hinge cost needs a configuration or trajectory plan");
+    if (d_grad && !P->robot->subtrees_contiguous)
+        return fail(PPL_ERR_UNSUPPORTED, "gradient needs subtrees contiguous in dfs_order");
+    DScene S{};
+    ppl_status st = plan_scene(P, view, S);
+    if (st != PPL_OK) return st;
+    if (P->n_states == 0) return PPL_OK;
+    DStates X = plan_states(P, d_x);
+    const float* self_in = nullptr;
+    if ((st = cost_self(P, X, prm, s, &self_in)) != PPL_OK) return st;
+    float* state = d_state;
+    if (!state) {
+        HIP_TRY(P->d_cost.ensure((size_t)P->n_states * 4));
+        state = P->d_cost.as<float>();
+    }
+    CostArgs a{};
+    a.R = P->robot->R;
+    a.S = S;
+    a.X = X;
+    int grid = 0;
+    a.C = plan_chunks(P, d_grad ? kKCostGrad : kKCost, plan_obb_cap(P, S), &grid);
+    a.P = {prm->world_margin, prm->self_margin, prm->weight, prm->use_obb, prm->use_esdf, prm->use_self};
+    a.grad = d_grad != nullptr;
+    a.self_in = self_in;
+    a.cost_out = state;
+    a.grad_out = d_grad;
+    HIP_TRY(launch_cost(a, grid, s));
+    if (d_total && P->kind == PPL_PLAN_TRAJ) HIP_TRY(launch_segment_sum(state, P->d_off, P->n_units, 0.0f, d_total, s));
+    return PPL_OK;
+}
+
+__global__ void k_fd_expand(const float* theta, const int* traj_off, int n_traj, long n_wp, int D, float eps, float* cfg) {
+    // one thread per (waypoint, dim, sign): cfg[((w*D + d)*2 + sgn)*D + k]
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_wp * D * 2) return;
+    const long w = i / (2 * D);
+    const int d = (int)((i / 2) % D);
+    const int sgn = (int)(i & 1);
+    int lo = 0, hi = n_traj;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (traj_off[mid] <= w) lo = mid; else hi = mid;
+    }
+    const int t0 = traj_off[lo], Tn = traj_off[lo + 1] - t0, t = (int)(w - t0);
+    const float* th = theta + (size_t)t0 * D;
+    float* q = cfg + (size_t)i * D;
+    for (int k = 0; k < D; ++k) q[k] = th[(size_t)k * Tn + t];
+    // chomp.cu:108-133: q[d] += eps; then q[d] -= 2 eps (FP32).
+    const float plus = q[d] + eps;
+    q[d] = sgn == 0 ? plus : plus - 2.0f * eps;
+}
+
+__global__ void k_fd_combine(const float* cost, const int* traj_off, int n_traj, long n_wp, int D, float eps, float w,
+                             float* grad) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_wp * D) return;
+    const long wp = i / D;
+    const int d = (int)(i % D);
+    int lo = 0, hi = n_traj;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (traj_off[mid] <= wp) lo = mid; else hi = mid;
+    }
+    const int t0 = traj_off[lo], Tn = traj_off[lo + 1] - t0, t = (int)(wp - t0);
+    const float cp = cost[2 * i], cm = cost[2 * i + 1];
+    const bool end = t == 0 || t == Tn - 1;
+    grad[(size_t)t0 * D + (size_t)d * Tn + t] = end ? 0.0f : w * (cp - cm) / (2.0f * eps);
+}
+
+ppl_status fd_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_theta, const ppl_cost_params* prm, float eps,
+                   float* d_grad, hipStream_t s) {
+    CHECK_ARG(P && prm && d_theta && d_grad, "NULL argument");
+    CHECK_ARG(P->kind == PPL_PLAN_TRAJ, "finite-difference gradient needs a trajectory plan");
+    CHECK_ARG(P->h_obb_off.empty(), "finite-difference gradient supports a single scene");
+    DScene S{};
+    ppl_status st = plan_scene(P, view, S);
+    if (st != PPL_OK) return st;
+    const long n_wp = P->n_states;
+    if (n_wp == 0) return PPL_OK;
+    const int D = P->robot->R.n_dof;
+    const long n_cfg = n_wp * D * 2;
+    HIP_TRY(P->d_fdcfg.ensure((size_t)n_cfg * D * 4));
+    HIP_TRY(P->d_fdcost.ensure((size_t)n_cfg * 4));
+    {
+        const long th = n_cfg;
+        hipLaunchKernelGGL(k_fd_expand, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, s, d_theta, P->d_off, P->n_units,
+                           n_wp, D, eps, P->d_fdcfg.as<float>());
+        HIP_TRY(hipGetLastError());
+    }
+    DStates X{};
+    X.source = kSrcConfig;
+    X.n_states = n_cfg;
+    X.cfg = P->d_fdcfg.as<float>();
+    X.cfg_stride = D;
+    const float* self_in = nullptr;
+    if (prm->use_self && P->robot->n_pairs > 0) {
+        HIP_TRY(P->d_self.ensure((size_t)n_cfg * 8 * 4));
+        if ((st = run_self(P->robot, X, 1, nullptr, P->d_self.as<float>(), s)) != PPL_OK) return st;
+        self_in = P->d_self.as<float>();
+    }
+    CostArgs a{};
+    a.R = P->robot->R;
+    a.S = S;
+    a.X = X;
+    int grid = 0;
+    a.C = implicit_chunks(n_cfg, P->counter, kKCost, a.R, S.n_obbs, &grid);
+    a.P = {prm->world_margin, prm->self_margin, 1.0f, prm->use_obb, prm->use_esdf, prm->use_self};
+    a.self_in = self_in;
+    a.cost_out = P->d_fdcost.as<float>();
+    HIP_TRY(launch_cost(a, grid, s));
+    {
+        const long th = n_wp * D;
+        hipLaunchKernelGGL(k_fd_combine, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, s, P->d_fdcost.as<float>(),
+                           P->d_off, P->n_units, n_wp, D, eps, prm->weight, d_grad);
+        HIP_TRY(hipGetLastError());
+    }
+    return PPL_OK;
+}
+
+ppl_status stomp_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_roll, const ppl_cost_params* prm, float w_u,
+                      float dt, float* d_state, float* d_total, hipStream_t s) {
+    CHECK_ARG(P && prm && d_roll && d_total, "NULL argument");
+    CHECK_ARG(P->kind == PPL_PLAN_TRAJ, "rollout costs need a trajectory plan");
+    ppl_status st = hinge_impl(P, view, d_roll, prm, d_state, d_total, nullptr, s);
+    if (st != PPL_OK) return st;
+    HIP_TRY(launch_control_cost(d_roll, P->d_off, P->n_units, P->robot->R.n_dof, dt, w_u, d_total, 1, s));
+    return PPL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ppl_status ppl_check_collision_batched(ppl_plan P, const ppl_scene_view* view, const float* d_cfg, float margin,
+                                       int check_self, uint8_t* d_out, ppl_stream stream) {
+    CHECK_ARG(P && d_cfg && d_out, "NULL argument");
+    CHECK_ARG(P->kind == PPL_PLAN_CONFIGS, "configuration plan required");
+    DScene S{};
+    ppl_status st = plan_scene(P, view, S);
+    if (st != PPL_OK) return st;
+    if (P->n_states == 0) return PPL_OK;
+    hipStream_t s = (hipStream_t)stream;
+    DStates X = plan_states(P, d_cfg);
+    const uint8_t* self_hit = nullptr;
+    if (check_self && P->robot->n_pairs > 0) {
+        HIP_TRY(P->d_self.ensure((size_t)P->n_states));
+        if ((st = run_self(P->robot, X, 0, P->d_self.as<uint8_t>(), nullptr, s)) != PPL_OK) return st;
+        self_hit = P->d_self.as<uint8_t>();
+    }
+    CollideArgs a{};
+    a.R = P->robot->R;
+    a.S = S;
+    a.X = X;
+    int grid = 0;
+    a.C = plan_chunks(P, kKCollide, plan_obb_cap(P, S), &grid);
+    a.margin = margin;
+    a.mode = 0;
+    a.out = d_out;
+    a.self_hit = self_hit;
+    HIP_TRY(launch_collide(a, grid, s));
+    return PPL_OK;
+}
+
+ppl_status ppl_edge_check_batched(ppl_plan P, const ppl_scene_view* view, const float* d_from, const float* d_to,
+                                  const ppl_edge_batch_params* prm, int* d_first_hit, uint8_t* d_state_hits,
+                                  ppl_stream stream) {
+    CHECK_ARG(P && prm && d_from && d_to && d_first_hit, "NULL argument");
+    CHECK_ARG(P->kind == PPL_PLAN_EDGES, "edge plan required");
+    CHECK_ARG(!(d_state_hits && prm->skip_resolved), "per-state verdicts need skip_resolved = 0");
+    DScene S{};
+    ppl_status st = plan_scene(P, view, S);
+    if (st != PPL_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(launch_fill_i32(d_first_hit, -1, P->n_units, s));
+    if (P->n_states == 0) return PPL_OK;
+    DStates X = plan_states(P, nullptr);
+    X.from = d_from;
+    X.to = d_to;
+    const uint8_t* self_hit = nullptr;
+    if (prm->check_self && P->robot->n_pairs > 0) {
+        HIP_TRY(P->d_self.ensure((size_t)P->n_states));
+        if ((st = run_self(P->robot, X, 0, P->d_self.as<uint8_t>(), nullptr, s)) != PPL_OK) return st;
+        self_hit = P->d_self.as<uint8_t>();
+    }
+    CollideArgs a{};
+    a.R = P->robot->R;
+    a.S = S;
+    a.X = X;
+    int grid = 0;
+    a.C = plan_chunks(P, kKCollide, plan_obb_cap(P, S), &grid);
+    a.margin = prm->collision_margin;
+    a.mode = 1;
+    a.skip_resolved = prm->skip_resolved;
+    a.out = d_state_hits;
+    a.first_hit = d_first_hit;
+    a.self_hit = self_hit;
+    HIP_TRY(launch_collide(a, grid, s));
+    return PPL_OK;
+}
+
+ppl_status ppl_min_clearance_batched(ppl_plan P, const ppl_scene_view* view, const float* d_cfg, float margin, float* d_out,
+                                     ppl_stream stream) {
+    CHECK_ARG(P && d_cfg && d_out, "NULL argument");
+    CHECK_ARG(P->kind == PPL_PLAN_CONFIGS, "configuration plan required");
+    DScene S{};
+    ppl_status st = plan_scene(P, view, S);
+    if (st != PPL_OK) return st;
+    if (P->n_states == 0) return PPL_OK;
+    CostArgs a{};
+    a.R = P->robot->R;
+    a.S = S;
+    a.X = plan_states(P, d_cfg);
+    int grid = 0;
+    a.C = plan_chunks(P, kKCost, plan_obb_cap(P, S), &grid);
+    a.P.weight = 1.0f;
+    a.clearance_only = 1;
+    a.margin = margin;
+    a.cost_out = d_out;
+    HIP_TRY(launch_cost(a, grid, (hipStream_t)stream));
+    return PPL_OK;
+}
+
+ppl_status ppl_hinge_cost_batched(ppl_plan P, const ppl_scene_view* view, const float* d_x, const ppl_cost_params* prm,
+                                  float* d_state, float* d_total, float* d_grad, ppl_stream stream) {
+    return hinge_impl(P, view, d_x, prm, d_state, d_total, d_grad, (hipStream_t)stream);
+}
+
+ppl_status ppl_chomp_fd_grad_batched(ppl_plan P, const ppl_scene_view* view, const float* d_theta,
+                                     const ppl_cost_params* prm, float eps, float* d_grad, ppl_stream stream) {
+    return fd_impl(P, view, d_theta, prm, eps, d_grad, (hipStream_t)stream);
+}
+
+ppl_status ppl_stomp_rollout_costs(ppl_plan P, const ppl_scene_view* view, const float* d_roll, const ppl_cost_params* prm,
+                                   float w_u, float dt, float* d_state, float* d_total, ppl_stream stream) {
+    return stomp_impl(P, view, d_roll, prm, w_u, dt, d_state, d_total, (hipStream_t)stream);
+}
+
+// Single-trajectory plain forms share one cached plan per (robot, T).
+}  // extern "C"
+
+namespace {
+ppl_status single_traj_plan(ppl_robot r, int T, ppl_plan* out) {
+    ppl_plan_desc d{};
+    d.kind = PPL_PLAN_TRAJ;
+    d.n_units = 1;
+    d.unit_states = &T;
+    d.n_problems = 1;
+    return ppl_plan_create(r, &d, out);
+}
+
+__global__ void k_broadcast_rows(float* x, int D, int T) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const float v = x[t];
+    for (int d = 1; d < D; ++d) x[(size_t)d * T + t] = v;
+}
+}  // namespace
+
+extern "C" {
+
+ppl_status ppl_chomp_state_cost(ppl_robot r, const ppl_scene_view* view, const float* d_theta, int T,
+                                const ppl_cost_params* prm, float* d_state_costs, ppl_stream stream) {
+    CHECK_ARG(r && prm && d_theta && d_state_costs, "NULL argument");
+    CHECK_ARG(T >= 1, "T must be >= 1");
+    ppl_plan P = nullptr;
+    ppl_status st = single_traj_plan(r, T, &P);
+    if (st != PPL_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    st = hinge_impl(P, view, d_theta, prm, d_state_costs, nullptr, nullptr, s);
+    if (st == PPL_OK) {
+        hipLaunchKernelGGL(k_broadcast_rows, dim3((T + 63) / 64), dim3(64), 0, s, d_state_costs, r->R.n_dof, T);
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = fail(PPL_ERR_HIP, "chomp_state_cost: %s", hipGetErrorString(e));
+    }
+    ppl_plan_destroy(P);
+    return st;
+}
+
+ppl_status ppl_chomp_fd_grad(ppl_robot r, const ppl_scene_view* view, const float* d_theta, int T,
+                             const ppl_cost_params* prm, float eps, float* d_grad, ppl_stream stream) {
+    CHECK_ARG(r && prm && d_theta && d_grad, "NULL argument");
+    CHECK_ARG(T >= 1, "T must be >= 1");
+    ppl_plan P = nullptr;
+    ppl_status st = single_traj_plan(r, T, &P);
+    if (st != PPL_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    st = fd_impl(P, view, d_theta, prm, eps, d_grad, s);
+    if (st == PPL_OK) {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = fail(PPL_ERR_HIP, "chomp_fd_grad: %s", hipGetErrorString(e));
+    }
+    ppl_plan_destroy(P);
+    return st;
+}
+
+ppl_status ppl_esdf_from_obbs(const ppl_scene_view* view, const float* h_params, const float* h_pose, float* d_features,
+                              ppl_stream stream) {
+    CHECK_ARG(view && h_params && h_pose && d_features, "NULL argument");
+    DScene S{};
+    ppl_status st = scene_from_view(view, S);
+    if (st != PPL_OK) return st;
+    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
+    CHECK_ARG(h_params[3] > 0.0f, "voxel size must be positive");
+    HIP_TRY(launch_esdf_build(S.obb_dims, S.obb_pose, S.n_obbs, h_params, h_pose, d_features, (hipStream_t)stream));
+    return PPL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// hipGraph capture (on a private stream; replay on any stream)
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+template <typename F>
+ppl_status capture(F&& body, ppl_graph* out) {
+    CHECK_ARG(out, "NULL argument");
+    *out = nullptr;
+    hipStream_t cs = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    ppl_graph_s* g = new (std::nothrow) ppl_graph_s();
+    if (!g) {
+        (void)hipStreamDestroy(cs);
+        return fail(PPL_ERR_ALLOC, "out of host memory");
+    }
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    ppl_status st = PPL_OK;
+    if (e == hipSuccess) {
+        st = body(cs);
+        hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
+        if (st == PPL_OK && e2 != hipSuccess) st = fail(PPL_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+    } else {
+        st = fail(PPL_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    }
+    if (st == PPL_OK) {
+        e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) st = fail(PPL_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    (void)hipStreamDestroy(cs);
+    if (st != PPL_OK) {
+        ppl_graph_destroy(g);
+        return st;
+    }
+    *out = g;
+    return PPL_OK;
+}
+}  // namespace
+
+extern "C" {
+
+ppl_status ppl_graph_capture_stomp(ppl_plan P, const ppl_scene_view* view, const float* d_roll, const ppl_cost_params* prm,
+                                   float w_u, float dt, float* d_state, float* d_total, ppl_stream stream,
+                                   ppl_graph* out) {
+    CHECK_ARG(P && prm, "NULL argument");
+    // Allocate every scratch buffer before capture (no allocation inside a graph).
+    ppl_status st = stomp_impl(P, view, d_roll, prm, w_u, dt, d_state, d_total, (hipStream_t)stream);
+    if (st != PPL_OK) return st;
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return capture([&](hipStream_t cs) { return stomp_impl(P, view, d_roll, prm, w_u, dt, d_state, d_total, cs); }, out);
+}
+
+ppl_status ppl_graph_capture_hinge(ppl_plan P, const ppl_scene_view* view, const float* d_x, const ppl_cost_params* prm,
+                                   float* d_state, float* d_total, float* d_grad, ppl_stream stream, ppl_graph* out) {
+    CHECK_ARG(P && prm, "NULL argument");
+    ppl_status st = hinge_impl(P, view, d_x, prm, d_state, d_total, d_grad, (hipStream_t)stream);
+    if (st != PPL_OK) return st;
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return capture([&](hipStream_t cs) { return hinge_impl(P, view, d_x, prm, d_state, d_total, d_grad, cs); }, out);
+}
+
+ppl_status ppl_graph_launch(ppl_graph g, ppl_stream stream) {
+    CHECK_ARG(g && g->exec, "NULL graph");
+    HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return PPL_OK;
+}
+
+ppl_status ppl_graph_destroy(ppl_graph g) {
+    if (!g) return PPL_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return PPL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,952 @@
+// kernels.hip -- gfx950 kernels of the collision-kinematics substrate.
+//
+// Execution model (wave64, CDNA4):
+//  * one lane = one robot state (configuration, edge state or waypoint);
+//    a 64-lane workgroup is one wavefront and runs persistently, pulling
+//    64-state chunks from an atomic work counter until the chunk table is
+//    exhausted (chunks never span two problems, so each wave sees one OBB
+//    set at a time);
+//  * the problem's enabled OBBs are re-packed into LDS once per problem
+//    switch; every lane then reads the same record (LDS broadcast);
+//  * FK walks the DFS-ordered joint table with wave-uniform (scalar) loads,
+//    keeping the live 3x4 transforms in VGPRs;
+//  * the any-collision predicate gates each (joint, OBB) pair on the joint's
+//    approximate bounding sphere with a wave ballot, tests the joint's exact
+//    spheres in box-local coordinates through one composed 3x4 matrix, and
+//    retires the wave as soon as every lane has a verdict;
+//  * the summed-hinge cost keeps per-joint adjoint accumulators in registers
+//    (joint loop unrolled to a compile-time bound) and differentiates through
+//    the argmin OBB of each active sphere in the same pass.
+#include "ppl_internal.hpp"
+
+#include <cfloat>
+#include <climits>
+
+namespace ppl {
+namespace {
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
+
+template <typename T>
+__device__ __forceinline__ T uniform(T v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// ---------------------------------------------------------------------------
+// 3x4 affine helpers (row-major, implicit last row 0 0 0 1)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void aff_identity(float* T) {
+#pragma unroll
+    for (int i = 0; i < 12; ++i) T[i] = 0.0f;
+    T[0] = T[5] = T[10] = 1.0f;
+}
+
+// S = FT * M(q), restating xrot_fn / yrot_fn / zrot_fn / prism_fn
+// (planning/utils.cuh:560-702). FIXED or unknown types move nothing.
+__device__ __forceinline__ void joint_step(const float* __restrict__ ft, int type, float q, float* S) {
+#pragma unroll
+    for (int i = 0; i < 12; ++i) S[i] = ft[i];
+    if (type >= kXRot) {
+        float s, c;
+        sincosf(q, &s, &c);
+        const int a = (type == kXRot) ? 1 : (type == kYRot ? 2 : 0);
+        const int b = (type == kXRot) ? 2 : (type == kYRot ? 0 : 1);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const float fa = ft[r * 4 + a], fb = ft[r * 4 + b];
+            S[r * 4 + a] = fmaf(fa, c, fb * s);
+            S[r * 4 + b] = fmaf(fb, c, -fa * s);
+        }
+    } else if (type >= kXPrism) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) S[r * 4 + 3] = fmaf(ft[r * 4 + type], q, ft[r * 4 + 3]);
+    }
+}
+
+// T = P * S
+__device__ __forceinline__ void aff_compose(const float* P, const float* S, float* T) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float acc = fmaf(P[r * 4 + 2], S[8 + c], fmaf(P[r * 4 + 1], S[4 + c], P[r * 4 + 0] * S[c]));
+            if (c == 3) acc += P[r * 4 + 3];
+            T[r * 4 + c] = acc;
+        }
+    }
+}
+
+__device__ __forceinline__ void aff_point(const float* T, float x, float y, float z, float& px, float& py, float& pz) {
+    px = fmaf(T[0], x, fmaf(T[1], y, fmaf(T[2], z, T[3])));
+    py = fmaf(T[4], x, fmaf(T[5], y, fmaf(T[6], z, T[7])));
+    pz = fmaf(T[8], x, fmaf(T[9], y, fmaf(T[10], z, T[11])));
+}
+
+// Transform slots (the reference's t_memory_idx scratch), kept in VGPRs. The
+// slot index is wave-uniform, so the switch compiles to scalar branches.
+struct Slots {
+    float s[kMaxSlots][12];
+    __device__ __forceinline__ void load(int i, float* T) const {
+        if (i == 0) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) T[k] = s[0][k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) T[k] = s[1][k];
+        }
+    }
+    __device__ __forceinline__ void store(int i, const float* T) {
+        if (i == 0) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) s[0][k] = T[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) s[1][k] = T[k];
+        }
+    }
+};
+
+// Transform of DFS position jj: identity at the root, parent * step otherwise.
+__device__ __forceinline__ void fk_joint(const DRobot& R, int jj, const float* s_q, Slots& sl, float* T, int4& meta) {
+    meta = R.jmeta[jj];
+    if (jj == 0) {
+        aff_identity(T);
+    } else {
+        float P[12], S[12];
+        sl.load(meta.w, P);
+        const float q = meta.y >= 0 ? s_q[meta.y * kWave] : 0.0f;
+        joint_step(R.ft + 12 * jj, meta.x, q, S);
+        aff_compose(P, S, T);
+    }
+    sl.store(meta.z, T);
+}
+
+// ---------------------------------------------------------------------------
+// OBB staging: enabled OBBs of [o0, o1) -> LDS records (see ppl_internal.hpp)
+// ---------------------------------------------------------------------------
+__device__ int stage_obbs(const DScene& S, int o0, int o1, float4* s_obb, int cap) {
+    const int lane = lane_id();
+    int n = 0;
+    for (int base = o0; base < o1; base += kWave) {
+        const int o = base + lane;
+        const bool ok = (o < o1) && S.obb_enable[o] != 0;
+        const unsigned long long m = __ballot(ok);
+        if (ok) {
+            const int pos = n + __popcll(m & ((1ull << lane) - 1ull));
+            if (pos < cap) {
+                const float* d = S.obb_dims + 4 * (size_t)o;
+                const float* p = S.obb_pose + 8 * (size_t)o;
+                const float w = p[3], x = p[4], y = p[5], z = p[6];
+                // Matrix of v -> v + w t + u x t, t = 2 u x v (scene_collision.cuh:36-45).
+                const float r00 = 1.0f - 2.0f * (y * y + z * z), r01 = 2.0f * (x * y - w * z), r02 = 2.0f * (x * z + w * y);
+                const float r10 = 2.0f * (x * y + w * z), r11 = 1.0f - 2.0f * (x * x + z * z), r12 = 2.0f * (y * z - w * x);
+                const float r20 = 2.0f * (x * z - w * y), r21 = 2.0f * (y * z + w * x), r22 = 1.0f - 2.0f * (x * x + y * y);
+                s_obb[4 * pos + 0] = make_float4(r00, r01, r02, p[0]);
+                s_obb[4 * pos + 1] = make_float4(r10, r11, r12, p[1]);
+                s_obb[4 * pos + 2] = make_float4(r20, r21, r22, p[2]);
+                s_obb[4 * pos + 3] = make_float4(0.5f * d[0], 0.5f * d[1], 0.5f * d[2], __int_as_float(o));
+            }
+        }
+        n += __popcll(m);
+    }
+    __syncthreads();
+    return n < cap ? n : cap;
+}
+
+__device__ __forceinline__ void to_box(const float4& r0, const float4& r1, const float4& r2, float px, float py, float pz,
+                                       float& lx, float& ly, float& lz) {
+    lx = fmaf(r0.x, px, fmaf(r0.y, py, fmaf(r0.z, pz, r0.w)));
+    ly = fmaf(r1.x, px, fmaf(r1.y, py, fmaf(r1.z, pz, r1.w)));
+    lz = fmaf(r2.x, px, fmaf(r2.y, py, fmaf(r2.z, pz, r2.w)));
+}
+
+// Collision iff squared distance to the box < r^2 (scene_collision.cuh:59-78).
+__device__ __forceinline__ bool box_hit(float lx, float ly, float lz, const float4& h, float r2) {
+    const float cx = fmaxf(fabsf(lx) - h.x, 0.0f);
+    const float cy = fmaxf(fabsf(ly) - h.y, 0.0f);
+    const float cz = fmaxf(fabsf(lz) - h.z, 0.0f);
+    return fmaf(cz, cz, fmaf(cy, cy, cx * cx)) < r2;
+}
+
+// Order-preserving key of the box signed distance (Quilez form,
+// scene_collision.cuh:88-112): key = |max(q,0)|^2 - min(max q, 0)^2 is
+// sdf*|sdf|, so the minimum needs no square root per pair.
+__device__ __forceinline__ float box_key(float lx, float ly, float lz, const float4& h) {
+    const float qx = fabsf(lx) - h.x, qy = fabsf(ly) - h.y, qz = fabsf(lz) - h.z;
+    const float ox = fmaxf(qx, 0.0f), oy = fmaxf(qy, 0.0f), oz = fmaxf(qz, 0.0f);
+    const float in = fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
+    return fmaf(-in, in, fmaf(oz, oz, fmaf(oy, oy, ox * ox)));
+}
+
+__device__ __forceinline__ float key_to_sdf(float k) { return k >= 0.0f ? sqrtf(k) : -sqrtf(-k); }
+
+// World-frame gradient of the box signed distance at world point p.
+__device__ __forceinline__ void box_sdf_grad(const float4& r0, const float4& r1, const float4& r2, const float4& h,
+                                             float px, float py, float pz, float& gx, float& gy, float& gz) {
+    float lx, ly, lz;
+    to_box(r0, r1, r2, px, py, pz, lx, ly, lz);
+    const float sx = lx >= 0.0f ? 1.0f : -1.0f, sy = ly >= 0.0f ? 1.0f : -1.0f, sz = lz >= 0.0f ? 1.0f : -1.0f;
+    const float qx = fabsf(lx) - h.x, qy = fabsf(ly) - h.y, qz = fabsf(lz) - h.z;
+    float ax, ay, az;
+    if (fmaxf(qx, fmaxf(qy, qz)) > 0.0f) {
+        const float ox = fmaxf(qx, 0.0f), oy = fmaxf(qy, 0.0f), oz = fmaxf(qz, 0.0f);
+        const float inv = 1.0f / sqrtf(fmaf(oz, oz, fmaf(oy, oy, ox * ox)));
+        ax = ox * inv * sx; ay = oy * inv * sy; az = oz * inv * sz;
+    } else {
+        const bool xb = qx >= qy && qx >= qz;
+        const bool yb = !xb && qy >= qz;
+        ax = xb ? sx : 0.0f; ay = yb ? sy : 0.0f; az = (!xb && !yb) ? sz : 0.0f;
+    }
+    // d/dp = R^T d/dl
+    gx = fmaf(r0.x, ax, fmaf(r1.x, ay, r2.x * az));
+    gy = fmaf(r0.y, ax, fmaf(r1.y, ay, r2.y * az));
+    gz = fmaf(r0.z, ax, fmaf(r1.z, ay, r2.z * az));
+}
+
+// ---------------------------------------------------------------------------
+// ESDF (scene_collision.cuh:186-331); trilinear mode adds value + gradient
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int robust_floor(float x) {
+    const float nearest = roundf(x);
+    if (x - nearest >= 1e-4f) return (int)floorf(x);
+    return (int)nearest;
+}
+
+__device__ __forceinline__ void quat_local(const float* pose, float sx, float sy, float sz, float& lx, float& ly, float& lz) {
+    const float qw = pose[3], qx = pose[4], qy = pose[5], qz = pose[6];
+    const float tx = 2.0f * (qy * sz - qz * sy);
+    const float ty = 2.0f * (qz * sx - qx * sz);
+    const float tz = 2.0f * (qx * sy - qy * sx);
+    lx = sx + qw * tx + (qy * tz - qz * ty) + pose[0];
+    ly = sy + qw * ty + (qz * tx - qx * tz) + pose[1];
+    lz = sz + qw * tz + (qx * ty - qy * tx) + pose[2];
+}
+
+// Minimum ESDF clearance (-dist - r) over enabled layers, 1000 when none
+// applies. With `want_grad` (trilinear only) returns d dist / d p (world)
+// of the winning layer in g*.
+__device__ float esdf_clearance(const DScene& S, float px, float py, float pz, float r, int sph_orig, bool want_grad,
+                                float& gx, float& gy, float& gz) {
+    float best = 1000.0f;
+    gx = gy = gz = 0.0f;
+    for (int layer = 0; layer < S.n_voxel_layers; ++layer) {
+        if (S.voxel_enable[layer] == 0) continue;
+        if (sph_orig >= 0 && S.sphere_acm && S.sphere_acm[sph_orig * S.n_voxel_layers + layer] == 0) continue;
+        const float* prm = S.voxel_params + 4 * layer;
+        const float* pose = S.voxel_pose + 8 * layer;
+        const float vs = prm[3];
+        if (vs <= 0.0f) continue;
+        const float inv_vs = 1.0f / vs;
+        const int nx = robust_floor(prm[0] * inv_vs) + 1;
+        const int ny = robust_floor(prm[1] * inv_vs) + 1;
+        const int nz = robust_floor(prm[2] * inv_vs) + 1;
+        float lx, ly, lz;
+        quat_local(pose, px, py, pz, lx, ly, lz);
+        const float ux = (lx + prm[0] * 0.5f) * inv_vs, uy = (ly + prm[1] * 0.5f) * inv_vs, uz = (lz + prm[2] * 0.5f) * inv_vs;
+        const float* feat = S.voxel_features + (size_t)layer * S.max_voxels_per_layer;
+        if (S.esdf_interp == 0) {
+            const int ix = (int)ux, iy = (int)uy, iz = (int)uz;
+            if (ix >= nx - 2 || iy >= ny - 2 || iz >= nz - 2 || ix <= 2 || iy <= 2 || iz <= 2) continue;
+            const long lin = (long)ix * ny * nz + (long)iy * nz + iz;
+            if (lin >= S.max_voxels_per_layer) continue;
+            const float d = feat[lin];
+            if (d <= -999.0f) continue;
+            const float c = -d - r;
+            if (c < best) { best = c; gx = gy = gz = 0.0f; }
+        } else {
+            const float fx0 = floorf(ux), fy0 = floorf(uy), fz0 = floorf(uz);
+            const int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
+            if (ix >= nx - 2 || iy >= ny - 2 || iz >= nz - 2 || ix <= 2 || iy <= 2 || iz <= 2) continue;
+            const long base = (long)ix * ny * nz + (long)iy * nz + iz;
+            if (base + (long)ny * nz + nz + 1 >= S.max_voxels_per_layer) continue;
+            const long sx = (long)ny * nz, sy = nz;
+            const float c000 = feat[base], c001 = feat[base + 1], c010 = feat[base + sy], c011 = feat[base + sy + 1];
+            const float c100 = feat[base + sx], c101 = feat[base + sx + 1], c110 = feat[base + sx + sy], c111 = feat[base + sx + sy + 1];
+            const float mn = fminf(fminf(fminf(c000, c001), fminf(c010, c011)), fminf(fminf(c100, c101), fminf(c110, c111)));
+            if (mn <= -999.0f) continue;
+            const float fx = ux - fx0, fy = uy - fy0, fz = uz - fz0;
+            const float c00 = fmaf(c100 - c000, fx, c000), c01 = fmaf(c101 - c001, fx, c001);
+            const float c10 = fmaf(c110 - c010, fx, c010), c11 = fmaf(c111 - c011, fx, c011);
+            const float c0 = fmaf(c10 - c00, fy, c00), c1 = fmaf(c11 - c01, fy, c01);
+            const float d = fmaf(c1 - c0, fz, c0);
+            const float c = -d - r;
+            if (c < best) {
+                best = c;
+                if (want_grad) {
+                    const float ofy = 1.0f - fy, ofz = 1.0f - fz;
+                    const float dx = ((c100 - c000) * ofy * ofz + (c101 - c001) * ofy * fz + (c110 - c010) * fy * ofz +
+                                      (c111 - c011) * fy * fz) * inv_vs;
+                    const float dy = ((c10 - c00) * ofz + (c11 - c01) * fz) * inv_vs;
+                    const float dz = (c1 - c0) * inv_vs;
+                    // grid-local -> world: inverse quaternion rotation
+                    const float qw = pose[3], qx = -pose[4], qy = -pose[5], qz = -pose[6];
+                    const float tx = 2.0f * (qy * dz - qz * dy), ty = 2.0f * (qz * dx - qx * dz), tz = 2.0f * (qx * dy - qy * dx);
+                    gx = dx + qw * tx + (qy * tz - qz * ty);
+                    gy = dy + qw * ty + (qz * tx - qx * tz);
+                    gz = dz + qw * tz + (qx * ty - qy * tx);
+                }
+            }
+        }
+    }
+    return best;
+}
+
+__device__ __forceinline__ bool esdf_hit(const DScene& S, float px, float py, float pz, float r, int sph_orig) {
+    float gx, gy, gz;
+    // min clearance < 0  <=>  some layer has dist > -r
+    return esdf_clearance(S, px, py, pz, r, sph_orig, false, gx, gy, gz) < 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// State resolution: flattened state index -> per-lane q in LDS
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int upper_bound_idx(const int* off, int n, long g) {
+    // largest e in [0, n) with off[e] <= g
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+struct StateRef {
+    int unit;      // config / edge / trajectory index
+    int k;         // edge state index or waypoint index
+    int n;         // edge states (incl. skipped) or trajectory length
+};
+
+// Writes q_d into s_q[d * 64 + lane] for d < D (inactive lanes write nothing).
+__device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, bool active, float* s_q) {
+    StateRef ref{0, 0, 0};
+    const int lane = lane_id();
+    if (!active) return ref;
+    if (X.source == kSrcConfig) {
+        ref.unit = (int)g;
+        const float* q = X.cfg + (size_t)g * X.cfg_stride;
+        for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = q[d];
+    } else if (X.source == kSrcEdge) {
+        const int e = upper_bound_idx(X.edge_off, X.n_edges, g);
+        const int cnt = X.edge_off[e + 1] - X.edge_off[e];
+        ref.unit = e;
+        ref.k = (int)(g - X.edge_off[e]) + X.k0;
+        ref.n = cnt + X.k0;
+        const float t = ref.n > 1 ? (float)ref.k / (float)(ref.n - 1) : 0.0f;
+        const float* a = X.from + (size_t)e * D;
+        const float* b = X.to + (size_t)e * D;
+        for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = fmaf(t, b[d] - a[d], a[d]);
+    } else {
+        const int b = upper_bound_idx(X.traj_off, X.n_traj, g);
+        const int t0 = X.traj_off[b];
+        const int Tn = X.traj_off[b + 1] - t0;
+        ref.unit = b;
+        ref.k = (int)(g - t0);
+        ref.n = Tn;
+        const float* th = X.theta + (size_t)t0 * D;
+        for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = th[(size_t)d * Tn + ref.k];
+    }
+    return ref;
+}
+
+// Pull the next chunk (wave-uniform). Returns false when the table is done.
+__device__ __forceinline__ bool next_chunk(const DChunks& C, long n_states, int4& ch) {
+    unsigned c = 0;
+    if (lane_id() == 0) c = atomicAdd(&C.counter[0], 1u);
+    c = uniform(__shfl(c, 0));
+    if ((int)c >= C.n_chunks) return false;
+    if (C.tab) {
+        ch = C.tab[c];
+        ch.x = uniform(ch.x); ch.y = uniform(ch.y); ch.z = uniform(ch.z);
+    } else {
+        const long b = (long)c * kWave;
+        ch = make_int4(0, (int)b, (int)(b + kWave < n_states ? b + kWave : n_states), 0);
+    }
+    return true;
+}
+
+// Last wave out resets the counters for the next launch on this plan.
+__device__ __forceinline__ void retire(const DChunks& C) {
+    if (lane_id() == 0) {
+        __threadfence();
+        const unsigned done = atomicAdd(&C.counter[1], 1u);
+        if (done == (unsigned)C.n_waves - 1u) {
+            atomicExch(&C.counter[0], 0u);
+            atomicExch(&C.counter[1], 0u);
+        }
+    }
+}
+
+__device__ __forceinline__ void problem_obbs(const DChunks& C, const DScene& S, int prob, int& o0, int& o1) {
+    if (C.obb_off) { o0 = C.obb_off[prob]; o1 = C.obb_off[prob + 1]; }
+    else { o0 = 0; o1 = S.n_obbs; }
+}
+
+// ===========================================================================
+// Any-collision: two-phase gate semantics (two_phase_cc.cuh:57-138 with the
+// check kernel's environment pass, pRRTC.cu:2337-2386).
+// ===========================================================================
+__global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
+    extern __shared__ float4 s_lds[];
+    float4* s_obb = s_lds;
+    float* s_q = reinterpret_cast<float*>(s_lds + 4 * a.C.obb_cap);
+    const DRobot& R = a.R;
+    const int lane = lane_id();
+    const int J = R.n_joints, D = R.n_dof;
+    const bool has_esdf = a.S.n_voxel_layers > 0;
+    const uint8_t* acm = a.S.sphere_obb_acm;
+    int cur_prob = -1, n_obb = 0;
+    int4 ch;
+    while (next_chunk(a.C, a.X.n_states, ch)) {
+        if (ch.x != cur_prob) {
+            int o0, o1;
+            problem_obbs(a.C, a.S, ch.x, o0, o1);
+            __syncthreads();
+            n_obb = uniform(stage_obbs(a.S, o0, o1, s_obb, a.C.obb_cap));
+            cur_prob = ch.x;
+        }
+        const long g = (long)ch.y + lane;
+        bool active = g < ch.z;
+        StateRef ref = load_state(a.X, D, g, active, s_q);
+        bool hit = false;
+        if (active && a.self_hit && a.self_hit[g]) { hit = true; active = false; }
+        if (active && a.mode == 1 && a.skip_resolved) {
+            const unsigned fh = (unsigned)a.first_hit[ref.unit];
+            if (fh < (unsigned)ref.k) active = false;
+        }
+        __syncthreads();
+        Slots sl;
+        for (int jj = 0; jj < J && wave_any(active); ++jj) {
+            float T[12];
+            int4 meta;
+            fk_joint(R, jj, s_q + lane, sl, T, meta);
+            const int s0 = R.sph_off[jj], s1 = R.sph_off[jj + 1];
+            const int a0 = R.apx_off[jj], a1 = R.apx_off[jj + 1];
+            if (s1 == s0 || a1 == a0) continue;
+            // Approximate spheres of this joint in world coordinates (usually one).
+            float apx[3], apr2;
+            {
+                const float4 as = R.apx[a0];
+                aff_point(T, as.x, as.y, as.z, apx[0], apx[1], apx[2]);
+                const float rr = as.w + a.margin;
+                apr2 = rr * rr;
+            }
+            bool flag = false;
+            for (int o = 0; o < n_obb; ++o) {
+                const float4 r0 = s_obb[4 * o], r1 = s_obb[4 * o + 1], r2 = s_obb[4 * o + 2], h = s_obb[4 * o + 3];
+                float lx, ly, lz;
+                to_box(r0, r1, r2, apx[0], apx[1], apx[2], lx, ly, lz);
+                bool ah = box_hit(lx, ly, lz, h, apr2);
+                for (int ai = a0 + 1; ai < a1; ++ai) {
+                    const float4 as = R.apx[ai];
+                    float x, y, z;
+                    aff_point(T, as.x, as.y, as.z, x, y, z);
+                    to_box(r0, r1, r2, x, y, z, lx, ly, lz);
+                    const float rr = as.w + a.margin;
+                    ah = ah || box_hit(lx, ly, lz, h, rr * rr);
+                }
+                flag = flag || ah;
+                if (!wave_any(ah && active)) continue;
+                // M = R_o * T_j: sphere-local -> box-local in one affine map.
+                float M[12];
+#pragma unroll
+                for (int rr = 0; rr < 3; ++rr) {
+                    const float4 row = rr == 0 ? r0 : (rr == 1 ? r1 : r2);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        float v = fmaf(row.z, T[8 + c], fmaf(row.y, T[4 + c], row.x * T[c]));
+                        if (c == 3) v += row.w;
+                        M[rr * 4 + c] = v;
+                    }
+                }
+                const int o_orig = __float_as_int(h.w);
+                bool h_any = false;
+                for (int s = s0; s < s1; ++s) {
+                    const float4 sp = R.sph[s];
+                    float x, y, z;
+                    aff_point(M, sp.x, sp.y, sp.z, x, y, z);
+                    const float rr = sp.w + a.margin;
+                    bool hs = box_hit(x, y, z, h, rr * rr);
+                    if (acm) hs = hs && acm[(size_t)R.sph_id[s] * a.S.n_obbs + o_orig] != 0;
+                    h_any = h_any || hs;
+                }
+                hit = hit || (active && ah && h_any);
+                active = active && !hit;
+                if (!wave_any(active)) break;
+            }
+            if (has_esdf) {
+                // Approx ESDF pass (ACM disabled, scene_collision.cuh:574-582),
+                // then the joint's exact spheres when it is flagged.
+                for (int ai = a0; ai < a1; ++ai) {
+                    const float4 as = R.apx[ai];
+                    float x, y, z;
+                    aff_point(T, as.x, as.y, as.z, x, y, z);
+                    flag = flag || (active && esdf_hit(a.S, x, y, z, as.w + a.margin, -1));
+                }
+                if (wave_any(flag && active)) {
+                    for (int s = s0; s < s1; ++s) {
+                        const float4 sp = R.sph[s];
+                        float x, y, z;
+                        aff_point(T, sp.x, sp.y, sp.z, x, y, z);
+                        const bool hs = flag && active && esdf_hit(a.S, x, y, z, sp.w + a.margin, R.sph_id[s]);
+                        hit = hit || hs;
+                    }
+                    active = active && !hit;
+                }
+            }
+        }
+        if (g < ch.z) {
+            if (a.out) a.out[g] = hit ? 1 : 0;
+            if (a.mode == 1 && hit) atomicMin((unsigned*)&a.first_hit[ref.unit], (unsigned)ref.k);
+        }
+    }
+    retire(a.C);
+}
+
+// ===========================================================================
+// Summed-hinge state cost (+ adjoint). chomp.cu:151-188 / stomp.cu:206-246
+// cost = w * (sum_s relu(m - min_o clr(s,o)) + sum_s relu(m - esdf clr(s))
+//             + relu(m_self - min_pairs clr)), raw radii.
+// Gradient: dC/dq_j = z_j . tau_sub(j) - (z_j x o_j) . F_sub(j) with
+// F = sum g_s, tau = sum p_s x g_s over the joint's subtree, built from DFS
+// prefix sums (subtrees are contiguous in DFS order).
+// ===========================================================================
+template <int MAXJ, bool GRAD>
+__global__ __launch_bounds__(64) void k_cost(CostArgs a) {
+    extern __shared__ float4 s_lds[];
+    float4* s_obb = s_lds;
+    float* s_q = reinterpret_cast<float*>(s_lds + 4 * a.C.obb_cap);
+    const DRobot& R = a.R;
+    const int lane = lane_id();
+    const int J = R.n_joints, D = R.n_dof;
+    const bool use_obb = a.P.use_obb != 0 && !a.clearance_only;
+    const bool use_esdf = (a.P.use_esdf != 0 || a.clearance_only) && a.S.n_voxel_layers > 0;
+    const float m = a.P.world_margin;
+    int cur_prob = -1, n_obb = 0;
+    int4 ch;
+    while (next_chunk(a.C, a.X.n_states, ch)) {
+        if (ch.x != cur_prob) {
+            int o0, o1;
+            problem_obbs(a.C, a.S, ch.x, o0, o1);
+            __syncthreads();
+            n_obb = uniform(stage_obbs(a.S, o0, o1, s_obb, a.C.obb_cap));
+            cur_prob = ch.x;
+        }
+        const long g = (long)ch.y + lane;
+        const bool active = g < ch.z;
+        StateRef ref = load_state(a.X, D, g, active, s_q);
+        __syncthreads();
+        // Self term from the self pass: hinge and its two point gradients.
+        float self_h = 0.0f, snx = 0.0f, sny = 0.0f, snz = 0.0f;
+        int si = -1, sj = -1;
+        if (a.self_in && active && !a.clearance_only) {
+            const float* so = a.self_in + 8 * (size_t)g;
+            self_h = fmaxf(a.P.self_margin - so[0], 0.0f);
+            if (self_h > 0.0f) { si = __float_as_int(so[1]); sj = __float_as_int(so[2]); snx = so[3]; sny = so[4]; snz = so[5]; }
+        }
+        float world = 0.0f;
+        float minclr = 1000.0f;
+        float F0 = 0.0f, F1 = 0.0f, F2 = 0.0f, t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+        float U[MAXJ][3], W[MAXJ][3], A[MAXJ];
+        int dofs[MAXJ];
+        Slots sl;
+#pragma unroll
+        for (int jj = 0; jj < MAXJ; ++jj) {
+            if (jj < J) {
+                float T[12];
+                int4 meta;
+                fk_joint(R, jj, s_q + lane, sl, T, meta);
+                dofs[jj] = meta.y;
+                if (GRAD) {
+                    const int type = meta.x;
+                    const int ax = type >= kXRot ? type - kXRot : (type >= 0 ? type : 0);
+                    const float zx = T[ax], zy = T[4 + ax], zz = T[8 + ax];
+                    const float ox = T[3], oy = T[7], oz = T[11];
+                    if (jj > 0 && type >= kXRot) {
+                        U[jj][0] = zx; U[jj][1] = zy; U[jj][2] = zz;
+                        W[jj][0] = zy * oz - zz * oy; W[jj][1] = zz * ox - zx * oz; W[jj][2] = zx * oy - zy * ox;
+                    } else if (jj > 0 && type >= 0) {
+                        U[jj][0] = U[jj][1] = U[jj][2] = 0.0f;
+                        W[jj][0] = -zx; W[jj][1] = -zy; W[jj][2] = -zz;
+                    } else {
+                        U[jj][0] = U[jj][1] = U[jj][2] = 0.0f;
+                        W[jj][0] = W[jj][1] = W[jj][2] = 0.0f;
+                    }
+                    // Subtrees that end right before this DFS position.
+                    const unsigned cm = R.close_mask[jj];
+#pragma unroll
+                    for (int p = 0; p < MAXJ; ++p) {
+                        if (p < jj && ((cm >> p) & 1u)) {
+                            A[p] += U[p][0] * t0 + U[p][1] * t1 + U[p][2] * t2 - (W[p][0] * F0 + W[p][1] * F1 + W[p][2] * F2);
+                        }
+                    }
+                    A[jj] = -(U[jj][0] * t0 + U[jj][1] * t1 + U[jj][2] * t2 - (W[jj][0] * F0 + W[jj][1] * F1 + W[jj][2] * F2));
+                }
+                const int s0 = R.sph_off[jj], s1 = R.sph_off[jj + 1];
+                for (int s = s0; s < s1; ++s) {
+                    const float4 sp = R.sph[s];
+                    float px, py, pz;
+                    aff_point(T, sp.x, sp.y, sp.z, px, py, pz);
+                    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+                    if (use_obb) {
+                        float best = FLT_MAX;
+                        int arg = -1;
+                        for (int o = 0; o < n_obb; ++o) {
+                            const float4 r0 = s_obb[4 * o], r1 = s_obb[4 * o + 1], r2 = s_obb[4 * o + 2], h = s_obb[4 * o + 3];
+                            float lx, ly, lz;
+                            to_box(r0, r1, r2, px, py, pz, lx, ly, lz);
+                            const float k = box_key(lx, ly, lz, h);
+                            if (k < best) { best = k; arg = o; }
+                        }
+                        const float clr = arg >= 0 ? key_to_sdf(best) - sp.w : 1000.0f;
+                        const float hh = m - clr;
+                        if (hh > 0.0f) {
+                            world += hh;
+                            if (GRAD) {
+                                float dx, dy, dz;
+                                box_sdf_grad(s_obb[4 * arg], s_obb[4 * arg + 1], s_obb[4 * arg + 2], s_obb[4 * arg + 3], px, py, pz, dx, dy, dz);
+                                gx -= dx; gy -= dy; gz -= dz;
+                            }
+                        }
+                    }
+                    if (use_esdf) {
+                        float dx, dy, dz;
+                        const float rr = a.clearance_only ? sp.w + a.margin : sp.w;
+                        const float c = esdf_clearance(a.S, px, py, pz, rr, R.sph_id[s], GRAD, dx, dy, dz);
+                        if (a.clearance_only) {
+                            minclr = fminf(minclr, c);
+                        } else {
+                            const float hh = m - c;
+                            if (hh > 0.0f) {
+                                world += hh;
+                                if (GRAD) { gx += dx; gy += dy; gz += dz; }
+                            }
+                        }
+                    }
+                    if (GRAD) {
+                        if (si >= 0) {
+                            const int id = R.sph_id[s];
+                            if (id == si) { gx -= snx; gy -= sny; gz -= snz; }
+                            if (id == sj) { gx += snx; gy += sny; gz += snz; }
+                        }
+                        F0 += gx; F1 += gy; F2 += gz;
+                        t0 += py * gz - pz * gy; t1 += pz * gx - px * gz; t2 += px * gy - py * gx;
+                    }
+                }
+            }
+        }
+        const float w = a.P.weight;
+        if (active && a.clearance_only) a.cost_out[g] = minclr;
+        if (active && !a.clearance_only && a.cost_out) a.cost_out[g] = w * (world + self_h);
+        if (GRAD && active && !a.clearance_only) {
+            const unsigned cm = R.close_mask[J];
+            bool endpoint = false;
+            size_t base = 0, stride = 1;
+            if (a.X.source == kSrcTraj) {
+                endpoint = (ref.k == 0 || ref.k == ref.n - 1);
+                base = (size_t)a.X.traj_off[ref.unit] * D + ref.k;
+                stride = ref.n;
+            } else {
+                base = (size_t)g * D;
+                stride = 1;
+            }
+#pragma unroll
+            for (int p = 0; p < MAXJ; ++p) {
+                if (p < J) {
+                    float v = A[p];
+                    if ((cm >> p) & 1u)
+                        v += U[p][0] * t0 + U[p][1] * t1 + U[p][2] * t2 - (W[p][0] * F0 + W[p][1] * F1 + W[p][2] * F2);
+                    const int d = dofs[p];
+                    if (p > 0 && d >= 0) a.grad_out[base + (size_t)d * stride] = endpoint ? 0.0f : w * v;
+                }
+            }
+        }
+    }
+    retire(a.C);
+}
+
+// ===========================================================================
+// Self collision: 16 lanes per state, sphere positions in LDS.
+// mode 0: any pair with |d|^2 - (ri+rj)^2 < 0 (utils.cuh:239-267,
+//         runtime_kinematics.cuh:248-283);
+// mode 1: min over pairs of |d| - ri - rj with argmin pair and unit normal
+//         (runtime_kinematics.cuh:294-329).
+// ===========================================================================
+constexpr int kSelfStates = kWave / kSelfGroup;
+
+__global__ __launch_bounds__(64) void k_self(SelfArgs a) {
+    extern __shared__ float s_dyn[];
+    const DRobot& R = a.R;
+    const int lane = lane_id();
+    const int sub = lane & (kSelfGroup - 1);
+    const int st = lane / kSelfGroup;
+    const int S = R.n_sph, D = R.n_dof, J = R.n_joints;
+    float* s_pos = s_dyn;                                  // [kSelfStates][S][3]
+    float* s_q = s_dyn + kSelfStates * S * 3;              // [kMaxDof][64]
+    const long g = (long)blockIdx.x * kSelfStates + st;
+    const bool active = g < a.X.n_states;
+    // Every lane of a state group loads the same q (lane-indexed LDS column).
+    load_state(a.X, D, active ? g : 0, true, s_q);
+    __syncthreads();
+    Slots sl;
+    float* pos = s_pos + st * S * 3;
+    for (int jj = 0; jj < J; ++jj) {
+        float T[12];
+        int4 meta;
+        fk_joint(R, jj, s_q + lane, sl, T, meta);
+        const int s0 = R.sph_off[jj], s1 = R.sph_off[jj + 1];
+        for (int s = s0 + sub; s < s1; s += kSelfGroup) {
+            const float4 sp = R.sph[s];
+            float x, y, z;
+            aff_point(T, sp.x, sp.y, sp.z, x, y, z);
+            const int id = R.sph_id[s];
+            pos[3 * id] = x; pos[3 * id + 1] = y; pos[3 * id + 2] = z;
+        }
+    }
+    __syncthreads();
+    if (a.mode == 0) {
+        bool hit = false;
+        for (int p = sub; p < R.n_pairs && !hit; p += kSelfGroup) {
+            const int2 pr = R.pairs[p];
+            const float dx = pos[3 * pr.x] - pos[3 * pr.y], dy = pos[3 * pr.x + 1] - pos[3 * pr.y + 1], dz = pos[3 * pr.x + 2] - pos[3 * pr.y + 2];
+            const float rs = R.sph_orig[pr.x].w + R.sph_orig[pr.y].w;
+            hit = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) - rs * rs < 0.0f;
+        }
+        const unsigned long long m = __ballot(hit);
+        const bool any = ((m >> (st * kSelfGroup)) & 0xFFFFull) != 0ull;
+        if (active && sub == 0) a.hit_out[g] = any ? 1 : 0;
+    } else {
+        float best = 1000.0f;
+        int bp = INT_MAX;
+        for (int p = sub; p < R.n_pairs; p += kSelfGroup) {
+            const int2 pr = R.pairs[p];
+            const float dx = pos[3 * pr.x] - pos[3 * pr.y], dy = pos[3 * pr.x + 1] - pos[3 * pr.y + 1], dz = pos[3 * pr.x + 2] - pos[3 * pr.y + 2];
+            const float c = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))) - (R.sph_orig[pr.x].w + R.sph_orig[pr.y].w);
+            if (c < best) { best = c; bp = p; }
+        }
+#pragma unroll
+        for (int off = 1; off < kSelfGroup; off <<= 1) {
+            const float ob = __shfl_xor(best, off, kSelfGroup);
+            const int op = __shfl_xor(bp, off, kSelfGroup);
+            if (ob < best || (ob == best && op < bp)) { best = ob; bp = op; }
+        }
+        if (active && sub == 0) {
+            float* o = a.out + 8 * (size_t)g;
+            o[0] = best;
+            if (bp != INT_MAX) {
+                const int2 pr = R.pairs[bp];
+                const float dx = pos[3 * pr.x] - pos[3 * pr.y], dy = pos[3 * pr.x + 1] - pos[3 * pr.y + 1], dz = pos[3 * pr.x + 2] - pos[3 * pr.y + 2];
+                const float inv = 1.0f / sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+                o[1] = __int_as_float(pr.x); o[2] = __int_as_float(pr.y);
+                o[3] = dx * inv; o[4] = dy * inv; o[5] = dz * inv;
+            } else {
+                o[1] = __int_as_float(-1); o[2] = __int_as_float(-1);
+                o[3] = o[4] = o[5] = 0.0f;
+            }
+            o[6] = o[7] = 0.0f;
+        }
+    }
+}
+
+// ===========================================================================
+// Small helpers
+// ===========================================================================
+__global__ void k_fill_i32(int* p, int v, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// out[b] = sum of x over segment b (one wave per segment, fixed order).
+__global__ __launch_bounds__(64) void k_segment_sum(const float* x, const int* off, int n_seg, float* out) {
+    const int b = blockIdx.x;
+    if (b >= n_seg) return;
+    const int lane = lane_id();
+    float acc = 0.0f;
+    for (int i = off[b] + lane; i < off[b + 1]; i += kWave) acc += x[i];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) out[b] = acc;
+}
+
+// Control cost w * sum_d theta_d^T R theta_d for the 5-point acceleration
+// metric R = dt * (A^T A)[pad..pad+T] (chomp_precompute.cpp:20-38), evaluated
+// as dt * sum_i acc_i^2 over the stencil rows that touch the trajectory,
+// acc_i = (-x[i-2] + 16x[i-1] - 30x[i] + 16x[i+1] - x[i+2]) / (12 dt^2) with
+// x = 0 outside [0, T). One wave per trajectory.
+__global__ __launch_bounds__(64) void k_control(const float* theta, const int* traj_off, int n_traj, int D, float dt,
+                                                float w, float* out, int accumulate) {
+    const int b = blockIdx.x;
+    if (b >= n_traj) return;
+    const int lane = lane_id();
+    const int t0 = traj_off[b], Tn = traj_off[b + 1] - t0;
+    const float* th = theta + (size_t)t0 * D;
+    const float inv = 1.0f / (12.0f * dt * dt);
+    float acc = 0.0f;
+    for (int d = 0; d < D; ++d) {
+        const float* x = th + (size_t)d * Tn;
+        for (int i = lane - 2; i < Tn + 2; i += kWave) {
+            auto X = [&](int k) { return (k >= 0 && k < Tn) ? x[k] : 0.0f; };
+            const float v = (-X(i - 2) + 16.0f * X(i - 1) - 30.0f * X(i) + 16.0f * X(i + 1) - X(i + 2)) * inv;
+            acc = fmaf(v, v, acc);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) {
+        const float c = w * dt * acc;
+        out[b] = accumulate ? out[b] + c : c;
+    }
+}
+
+// ESDF node values = -min_o sdf_o(node), positive inside. One thread per node.
+__global__ __launch_bounds__(256) void k_esdf_build(const float* obb_dims, const float* obb_pose, int n_obbs,
+                                                    float4 prm, float4 gp0, float4 gp1, int nx, int ny, int nz, float* out) {
+    const float gpose[8] = {gp0.x, gp0.y, gp0.z, gp0.w, gp1.x, gp1.y, gp1.z, gp1.w};
+    __shared__ float4 s_obb[4 * kMaxObbLds];
+    // stage with the first wave (same record format as the collision kernels)
+    for (int base = 0; base < n_obbs && base < kMaxObbLds; base += blockDim.x) {
+        const int o = base + threadIdx.x;
+        if (o < n_obbs && o < kMaxObbLds) {
+            const float* d = obb_dims + 4 * o;
+            const float* p = obb_pose + 8 * o;
+            const float w = p[3], x = p[4], y = p[5], z = p[6];
+            s_obb[4 * o + 0] = make_float4(1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y - w * z), 2.0f * (x * z + w * y), p[0]);
+            s_obb[4 * o + 1] = make_float4(2.0f * (x * y + w * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z - w * x), p[1]);
+            s_obb[4 * o + 2] = make_float4(2.0f * (x * z - w * y), 2.0f * (y * z + w * x), 1.0f - 2.0f * (x * x + y * y), p[2]);
+            s_obb[4 * o + 3] = make_float4(0.5f * d[0], 0.5f * d[1], 0.5f * d[2], 0.0f);
+        }
+    }
+    __syncthreads();
+    const long n = (long)nx * ny * nz;
+    const int no = n_obbs < kMaxObbLds ? n_obbs : kMaxObbLds;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int ix = (int)(i / ((long)ny * nz));
+        const int iy = (int)((i / nz) % ny);
+        const int iz = (int)(i % nz);
+        const float lx = ix * prm.w - 0.5f * prm.x - gpose[0];
+        const float ly = iy * prm.w - 0.5f * prm.y - gpose[1];
+        const float lz = iz * prm.w - 0.5f * prm.z - gpose[2];
+        // grid-local -> world with the conjugate of the stored rotation
+        const float qw = gpose[3], qx = -gpose[4], qy = -gpose[5], qz = -gpose[6];
+        const float tx = 2.0f * (qy * lz - qz * ly), ty = 2.0f * (qz * lx - qx * lz), tz = 2.0f * (qx * ly - qy * lx);
+        const float px = lx + qw * tx + (qy * tz - qz * ty);
+        const float py = ly + qw * ty + (qz * tx - qx * tz);
+        const float pz = lz + qw * tz + (qx * ty - qy * tx);
+        float best = FLT_MAX;
+        for (int o = 0; o < no; ++o) {
+            float bx, by, bz;
+            to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], px, py, pz, bx, by, bz);
+            best = fminf(best, box_key(bx, by, bz, s_obb[4 * o + 3]));
+        }
+        out[i] = -key_to_sdf(best);
+    }
+}
+
+int g_cus = 0;
+
+int occupancy_grid(const void* fn, size_t dyn_smem) {
+    if (g_cus == 0) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        hipDeviceProp_t prop;
+        (void)hipGetDeviceProperties(&prop, dev);
+        g_cus = prop.multiProcessorCount;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, dyn_smem) != hipSuccess || per_cu <= 0) per_cu = 4;
+    return per_cu * g_cus;
+}
+
+}  // namespace
+
+size_t lds_bytes(int obb_cap, int n_dof) {
+    return (size_t)obb_cap * 64 + (size_t)n_dof * kWave * sizeof(float);
+}
+
+int grid_collide(size_t lds) {
+    static size_t last = (size_t)-1;
+    static int g = 0;
+    if (lds != last) { g = occupancy_grid((const void*)k_collide, lds); last = lds; }
+    return g;
+}
+
+int grid_cost(int n_joints, bool grad, size_t lds) {
+    static size_t last[4] = {(size_t)-1, (size_t)-1, (size_t)-1, (size_t)-1};
+    static int g[4] = {0, 0, 0, 0};
+    const int idx = (n_joints <= 8 ? 0 : 2) + (grad ? 1 : 0);
+    if (lds != last[idx]) {
+        const void* fn = idx == 0 ? (const void*)k_cost<8, false> : idx == 1 ? (const void*)k_cost<8, true>
+                       : idx == 2 ? (const void*)k_cost<16, false> : (const void*)k_cost<16, true>;
+        g[idx] = occupancy_grid(fn, lds);
+        last[idx] = lds;
+    }
+    return g[idx];
+}
+
+hipError_t launch_collide(const CollideArgs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_collide, dim3(grid), dim3(64), lds_bytes(a.C.obb_cap, a.R.n_dof), s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s) {
+    const size_t lds = lds_bytes(a.C.obb_cap, a.R.n_dof);
+    if (a.R.n_joints <= 8) {
+        if (a.grad) hipLaunchKernelGGL((k_cost<8, true>), dim3(grid), dim3(64), lds, s, a);
+        else hipLaunchKernelGGL((k_cost<8, false>), dim3(grid), dim3(64), lds, s, a);
+    } else {
+        if (a.grad) hipLaunchKernelGGL((k_cost<16, true>), dim3(grid), dim3(64), lds, s, a);
+        else hipLaunchKernelGGL((k_cost<16, false>), dim3(grid), dim3(64), lds, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_self(const SelfArgs& a, hipStream_t s) {
+    const long blocks = (a.X.n_states + kSelfStates - 1) / kSelfStates;
+    if (blocks <= 0) return hipSuccess;
+    const size_t smem = sizeof(float) * ((size_t)kSelfStates * a.R.n_sph * 3 + (size_t)kMaxDof * kWave);
+    hipLaunchKernelGGL(k_self, dim3((unsigned)blocks), dim3(64), smem, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_segment_sum(const float* x, const int* off, int n_seg, float, float* out, hipStream_t s) {
+    if (n_seg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_segment_sum, dim3(n_seg), dim3(64), 0, s, x, off, n_seg, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_control_cost(const float* theta, const int* traj_off, int n_traj, int D, float dt, float w, float* out,
+                               int accumulate, hipStream_t s) {
+    if (n_traj <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_control, dim3(n_traj), dim3(64), 0, s, theta, traj_off, n_traj, D, dt, w, out, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_esdf_build(const float* obb_dims, const float* obb_pose, int n_obbs, const float* voxel_params,
+                             const float* voxel_pose, float* out, hipStream_t s) {
+    // voxel_params / voxel_pose are host arrays here
+    const float inv = 1.0f / voxel_params[3];
+    auto rf = [](float x) {
+        const float nearest = roundf(x);
+        if (x - nearest >= 1e-4f) return (int)floorf(x);
+        return (int)nearest;
+    };
+    const int nx = rf(voxel_params[0] * inv) + 1, ny = rf(voxel_params[1] * inv) + 1, nz = rf(voxel_params[2] * inv) + 1;
+    const long n = (long)nx * ny * nz;
+    long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_esdf_build, dim3((unsigned)blocks), dim3(256), 0, s, obb_dims, obb_pose, n_obbs,
+                       make_float4(voxel_params[0], voxel_params[1], voxel_params[2], voxel_params[3]),
+                       make_float4(voxel_pose[0], voxel_pose[1], voxel_pose[2], voxel_pose[3]),
+                       make_float4(voxel_pose[4], voxel_pose[5], voxel_pose[6], voxel_pose[7]), nx, ny, nz, out);
+    return hipGetLastError();
+}
+
+}  // namespace ppl

@@ -1,0 +1,143 @@
+// ppl_internal.hpp -- shared host/device declarations of the substrate library.
+//
+// Device data layout (all read-only during a launch):
+//  * robot tables are re-ordered into the kinematic tree's DFS visiting order,
+//    so the kernels walk joints with a plain counter and read every per-joint
+//    value with a wave-uniform (scalar) load;
+//  * spheres are stored in emission order (grouped by the DFS position of their
+//    joint) as float4 (x, y, z, r), with their original index kept beside them
+//    for ACM masks and outputs;
+//  * OBBs are re-packed per launch into LDS as 16-float records
+//    [R row0 | tx, R row1 | ty, R row2 | tz, hx, hy, hz, orig_index] with R the
+//    rotation of the stored inverse pose, so a world point maps to box-local
+//    coordinates with 9 FMAs;
+//  * per-state inputs (configurations, edge endpoints, trajectories) stay in
+//    the caller's layout in HBM; lanes of a wave take consecutive states.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ppl {
+
+constexpr int kMaxJoints = 16;      // per-lane unrolled joint state (Baxter: 15)
+constexpr int kMaxSlots = 2;        // FK transform slots (FK_T_SLOTS in the reference)
+constexpr int kMaxDof = 15;
+constexpr int kWave = 64;
+constexpr int kMaxObbLds = 256;     // OBBs staged per problem
+constexpr int kSelfGroup = 16;      // lanes per state in the self-collision pass
+
+enum JointType { kFixed = -1, kXPrism = 0, kYPrism = 1, kZPrism = 2, kXRot = 3, kYRot = 4, kZRot = 5 };
+
+struct DRobot {
+    int n_dof, n_joints, n_sph, n_apx, n_slots;
+    int n_pairs, n_apx_pairs;
+    const float* ft;            // [J][12] rows 0..2 of the fixed transform, DFS order
+    const int4* jmeta;          // [J] (type, dof, slot, parent_slot), DFS order
+    const int* sph_off;         // [J+1] sphere CSR by DFS position
+    const float4* sph;          // [S] emission order
+    const int* sph_id;          // [S] original index of emission slot k
+    const int* apx_off;         // [J+1]
+    const float4* apx;          // [A] emission order
+    const int* apx_id;          // [A]
+    const unsigned* close_mask; // [J+1] bit p set in close_mask[k]: subtree of DFS position p ends before k
+    const float4* sph_orig;     // [S] spheres by original index
+    const int2* pairs;          // [n_pairs] exact self pairs (original indices)
+    const int2* apx_pairs;      // [n_apx_pairs] approx self pairs (approx original indices)
+    const float4* apx_orig;     // [A] approx spheres by original index
+};
+
+// Non-owning scene pointers, SceneCollisionData semantics.
+struct DScene {
+    const float* obb_dims;      // [O][4] full extents
+    const float* obb_pose;      // [O][8] inverse pose (t, qw, qx, qy, qz, pad)
+    const uint8_t* obb_enable;  // [O]
+    int n_obbs;
+    const float* voxel_params;  // [L][4]
+    const float* voxel_pose;    // [L][8]
+    const uint8_t* voxel_enable;
+    const float* voxel_features;
+    int n_voxel_layers;
+    int max_voxels_per_layer;
+    const uint8_t* sphere_acm;           // [S][L]
+    const uint8_t* sphere_obb_acm;       // [S][O]
+    const uint8_t* sphere_obb_acm_approx;
+    int esdf_interp;            // 0 nearest, 1 trilinear
+};
+
+// Where a flattened state index comes from.
+enum StateSource { kSrcConfig = 0, kSrcEdge = 1, kSrcTraj = 2 };
+
+struct DStates {
+    int source;
+    long n_states;
+    // kSrcConfig: q(g, d) = cfg[g * cfg_stride + d]
+    const float* cfg; int cfg_stride;
+    // kSrcEdge: state g in edge e with edge_off[e] <= g < edge_off[e+1];
+    // k = g - edge_off[e] + k0, t = k / (n_e - 1) with n_e = edge_off[e+1]-edge_off[e]+k0
+    const float* from; const float* to; const int* edge_off; int n_edges; int k0;
+    // kSrcTraj: waypoint g of trajectory b (traj_off[b] <= g < traj_off[b+1]),
+    // theta(b, d, t) = theta[traj_off[b] * D + d * T_b + t]
+    const float* theta; const int* traj_off; int n_traj;
+};
+
+// Work chunks for persistent launches: a chunk never spans two problems.
+struct DChunks {
+    const int4* tab;            // (problem, begin, end, 0)
+    int n_chunks;
+    const int* obb_off;         // [P+1] OBB ranges per problem; NULL = single problem
+    unsigned* counter;          // [2] work counter + exit counter (self-resetting)
+    int n_waves;                // waves in the grid
+    int obb_cap;                // OBB records the LDS staging area holds (>= OBBs of any problem)
+};
+
+struct DCostParams {
+    float world_margin, self_margin, weight;
+    int use_obb, use_esdf, use_self;
+};
+
+// ---- launchers (kernels.hip) ----
+struct CollideArgs {
+    DRobot R; DScene S; DStates X; DChunks C;
+    float margin;
+    int mode;                   // 0 per-state verdict, 1 edge first-hit
+    int skip_resolved;          // first-hit mode: skip states past a known hit
+    uint8_t* out;               // [n_states] verdicts (nullable in mode 1)
+    int* first_hit;             // [n_edges] (mode 1), -1 = free
+    const uint8_t* self_hit;    // [n_states] precomputed self verdicts (nullable)
+};
+
+struct CostArgs {
+    DRobot R; DScene S; DStates X; DChunks C;
+    DCostParams P;
+    int grad;                   // also compute d cost / d q
+    int clearance_only;         // write min ESDF clearance (+margin) instead of cost
+    float margin;               // clearance_only: radius margin
+    const float* self_in;       // [n_states][8] self pass output (nullable)
+    float* cost_out;            // [n_states] weighted cost (or clearance)
+    float* grad_out;            // layout of X (kSrcConfig: [n][D]; kSrcTraj: theta layout)
+};
+
+struct SelfArgs {
+    DRobot R; DStates X;
+    int mode;                   // 0: any-collision bit, 1: min clearance + argmin pair + normal
+    uint8_t* hit_out;           // [n_states]
+    float* out;                 // [n_states][8]: clr, i, j (as int bits), nx, ny, nz, -, -
+};
+
+hipError_t launch_collide(const CollideArgs& a, int grid, hipStream_t s);
+hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s);
+hipError_t launch_self(const SelfArgs& a, hipStream_t s);
+hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t s);
+hipError_t launch_segment_sum(const float* x, const int* off, int n_seg, float scale_unused, float* out,
+                              hipStream_t s);
+hipError_t launch_control_cost(const float* theta, const int* traj_off, int n_traj, int D, float dt,
+                               float w, float* out, int accumulate, hipStream_t s);
+hipError_t launch_esdf_build(const float* obb_dims, const float* obb_pose, int n_obbs,
+                             const float* voxel_params, const float* voxel_pose, float* out,
+                             hipStream_t s);
+size_t lds_bytes(int obb_cap, int n_dof);
+int grid_collide(size_t lds);
+int grid_cost(int n_joints, bool grad, size_t lds);
+
+}  // namespace ppl

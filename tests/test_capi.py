@@ -1,0 +1,115 @@
+"""C ABI surface without a GPU: the library loads, exports every entry point
+include/ppl/ppl.h declares, and rejects bad inputs before any HIP call."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from prismplan_amd import _ppl
+from prismplan_amd.robot_model_loader import robot_arrays
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "ppl" / "ppl.h"
+
+
+def declared():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ppl_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared()
+    assert len(names) >= 25
+    L = _ppl.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_bindings_cover_the_header():
+    bound = {n for n, _, _ in _ppl.SIGNATURES}
+    assert set(declared()) == bound
+
+
+def test_version_and_error_string():
+    assert b"gfx950" in _ppl.lib().ppl_version()
+    assert isinstance(_ppl.lib().ppl_last_error(), bytes)
+
+
+def _desc(r, **over):
+    keep = []
+    f32 = lambda a: keep.append(np.ascontiguousarray(a, np.float32)) or keep[-1].ctypes.data_as(C.POINTER(C.c_float))
+    i32 = lambda a: keep.append(np.ascontiguousarray(a, np.int32)) or keep[-1].ctypes.data_as(C.POINTER(C.c_int))
+    d = _ppl.RobotDesc()
+    d.n_dof, d.n_joints = r.n_dof, r.n_joints
+    ft = over.get("ft", r.fixed_transforms)
+    d.fixed_transforms = f32(ft)
+    d.joint_types = i32(over.get("types", r.joint_types))
+    d.n_spheres, d.spheres, d.sphere_to_joint = r.n_spheres, f32(r.spheres), i32(r.sphere_to_joint)
+    d.n_self_cc_ranges, d.self_cc_ranges = len(r.self_ranges), i32(r.self_ranges)
+    d.n_approx_spheres, d.approx_spheres = r.n_approx, f32(r.approx_spheres)
+    d.approx_sphere_to_joint = i32(r.approx_sphere_to_joint)
+    d.n_approx_self_cc_ranges, d.approx_self_cc_ranges = len(r.approx_self_ranges), i32(r.approx_self_ranges)
+    d.joint_lower, d.joint_upper = f32(r.joint_lower), f32(r.joint_upper)
+    d.joint_parents = i32(over.get("parents", r.joint_parents))
+    d.dfs_order = i32(over.get("dfs", r.dfs_order))
+    d.joint_id_to_dof = i32(r.joint_id_to_dof)
+    d.t_memory_idx = i32(over.get("tmem", r.t_memory_idx))
+    if "n_dof" in over:
+        d.n_dof = over["n_dof"]
+    return d, keep
+
+
+@pytest.mark.parametrize("over,status", [
+    ({"n_dof": 0}, _ppl.PPL_ERR_INVALID),
+    ({"n_dof": 99}, _ppl.PPL_ERR_INVALID),
+    ({"dfs": [1, 0, 2, 3, 4, 5, 6, 7]}, _ppl.PPL_ERR_INVALID),           # root not first
+    ({"dfs": [0, 0, 2, 3, 4, 5, 6, 7]}, _ppl.PPL_ERR_INVALID),           # not a permutation
+    ({"parents": [0, 0, 1, 2, 3, 4, 7, 6]}, _ppl.PPL_ERR_INVALID),       # child before parent
+    ({"types": [3, 5, 5, 9, 5, 5, 5, 5]}, _ppl.PPL_ERR_INVALID),         # unknown joint type
+    ({"tmem": [0, 1, 2, 3, 1, 1, 1, 1]}, _ppl.PPL_ERR_UNSUPPORTED),      # more FK slots than supported
+])
+def test_robot_descriptor_validation(over, status):
+    r = robot_arrays("panda")
+    d, keep = _desc(r, **over)
+    h = C.c_void_p()
+    st = _ppl.lib().ppl_robot_create(C.byref(d), C.byref(h))
+    assert st == status
+    assert h.value is None
+    assert len(_ppl.lib().ppl_last_error()) > 0
+
+
+def test_non_affine_fixed_transform_rejected():
+    r = robot_arrays("panda")
+    ft = r.fixed_transforms.copy()
+    ft[16 + 12] = 0.5   # joint 1 bottom row
+    d, keep = _desc(r, ft=ft)
+    h = C.c_void_p()
+    assert _ppl.lib().ppl_robot_create(C.byref(d), C.byref(h)) == _ppl.PPL_ERR_INVALID
+    assert b"last row" in _ppl.lib().ppl_last_error()
+
+
+def test_null_arguments_rejected():
+    L = _ppl.lib()
+    assert L.ppl_robot_create(None, None) == _ppl.PPL_ERR_INVALID
+    assert L.ppl_check_collision_scene(None, None, None, 0, C.c_float(0), None) == _ppl.PPL_ERR_INVALID
+    assert L.ppl_plan_states(None) == -1
+    assert L.ppl_robot_destroy(None) == _ppl.PPL_OK
+    assert L.ppl_plan_destroy(None) == _ppl.PPL_OK
+    assert L.ppl_graph_destroy(None) == _ppl.PPL_OK
+
+
+def test_prrtc_builder_fields():
+    from prismplan_amd import prrtc
+    b = prrtc.RobotModelBuilder()
+    for f in ("n_dof", "n_joints", "fixed_transforms", "joint_types", "spheres_flat", "sphere_to_joint",
+              "self_cc_ranges_flat", "approx_spheres_flat", "approx_sphere_to_joint", "approx_self_cc_ranges_flat",
+              "approx_fixed_transforms", "approx_joint_types", "joint_lower", "joint_upper", "joint_parents",
+              "joint_id_to_dof", "t_memory_idx", "dfs_order"):
+        assert hasattr(b, f)
+    s = prrtc.SceneCollisionDataBuilder()
+    for f in ("obb_dims_ptr", "obb_pose_ptr", "obb_enable_ptr", "n_obbs", "voxel_params_ptr", "voxel_pose_ptr",
+              "voxel_enable_ptr", "voxel_features_ptr", "n_voxel_layers", "max_voxels_per_layer",
+              "sphere_acm_mask_ptr", "sphere_obb_acm_mask_ptr", "sphere_obb_acm_mask_approx_ptr"):
+        assert hasattr(s, f)

@@ -1,0 +1,381 @@
+"""HIP path vs the FP64 oracle, through the C ABI (needs an MI355X).
+
+Tolerances (stated per test):
+  * Boolean verdicts match exactly except where the FP64 clearance of the
+    configuration satisfies |clr| < 1e-5 m (ambiguous band, counted);
+  * costs and clearances within 1e-5 relative, denominator max(|ref|, w*m)
+    (w the cost weight, m the hinge margin: one sphere at full band depth);
+  * gradients within 1e-4 relative of the FP64 analytic adjoint, denominator
+    max(|ref|, 1e-2 * max_d |ref| of the same waypoint, 1e-3 * w), excluding
+    waypoints within 1e-5 m of a hinge kink, an argmin tie or a box-SDF kink
+    (counted).
+"""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as PO
+from prismplan_amd import substrate as S
+from prismplan_amd import workloads as W
+from prismplan_amd.planning_scene import HostScene
+
+pytestmark = pytest.mark.gpu
+BAND = 1e-5
+DEV = "cuda"
+
+
+def banner(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+def assert_verdicts(gpu, ref, clr, what):
+    gpu = np.asarray(gpu).astype(bool)
+    ref = np.asarray(ref).astype(bool)
+    amb = np.abs(clr) < BAND
+    bad = (gpu != ref) & ~amb
+    banner(f"{what}: n={gpu.size} collide={ref.mean():.3f} ambiguous={int(amb.sum())} "
+           f"mismatch_in_band={int(((gpu != ref) & amb).sum())}")
+    assert not bad.any(), f"{int(bad.sum())} verdict mismatches outside the ambiguous band"
+
+
+@pytest.fixture(scope="module")
+def robots():
+    return {n: S.Robot(n) for n in ("panda", "fetch", "baxter")}
+
+
+def test_plain_check_collision_any_collision_workload(robots):
+    w = W.any_collision()[0]
+    banner(f"plain check_collision_scene panda N={len(w.configs)} O={w.scene.n_obbs}")
+    ds = S.DeviceScene.from_host(w.scene)
+    got = S.check_collision_scene(robots["panda"], ds, w.configs)
+    ref, clr = PO.Oracle(w.robot, w.scene, nthreads=8).check_collision(w.configs)
+    assert_verdicts(got, ref, clr, "plain any-collision")
+    # margin variant
+    got = S.check_collision_scene(robots["panda"], ds, w.configs, collision_margin=0.03)
+    ref, clr = PO.Oracle(w.robot, w.scene, nthreads=8).check_collision(w.configs, margin=0.03)
+    assert_verdicts(got, ref, clr, "plain any-collision margin 0.03")
+
+
+@pytest.mark.parametrize("name", ["fetch", "baxter"])
+def test_plain_check_collision_tree_and_prismatic(robots, name):
+    rng = np.random.default_rng(21)
+    r = robots[name].arrays
+    scene = W.random_obbs(rng, 25, lo=(-0.3, -1.0, 0.0), hi=(1.1, 1.0, 1.4), half_hi=0.15)
+    q = W.uniform_configs(rng, r, 3000)
+    banner(f"plain check {name} N=3000")
+    got = S.check_collision_scene(robots[name], S.DeviceScene.from_host(scene), q)
+    ref, clr = PO.Oracle(r, scene, nthreads=8).check_collision(q)
+    assert_verdicts(got, ref, clr, f"plain any-collision {name}")
+
+
+def test_empty_inputs(robots):
+    w = W.any_collision(n_configs=64)[0]
+    ds = S.DeviceScene.from_host(w.scene)
+    assert S.check_collision_scene(robots["panda"], ds, np.zeros((0, 7), np.float32)).shape == (0,)
+    empty = S.DeviceScene.from_host(HostScene())
+    assert not S.check_collision_scene(robots["panda"], empty, w.configs).any()
+
+
+def test_batched_requests_with_own_scenes(robots):
+    reqs = W.any_collision(n_configs=700, n_obbs=20, seed=5, n_requests=6)
+    banner("batched any-collision, 6 requests with own OBB sets")
+    cfg = np.concatenate([r.configs for r in reqs])
+    scene = HostScene(obb_dims=np.concatenate([r.scene.obb_dims for r in reqs]),
+                      obb_pose=np.concatenate([r.scene.obb_pose for r in reqs]),
+                      obb_enable=np.concatenate([r.scene.obb_enable for r in reqs]))
+    pu = np.arange(7) * 700
+    oo = np.arange(7) * 20
+    plan = S.Plan.configs(robots["panda"], len(cfg), problem_units=pu, obb_offsets=oo)
+    out = S.check_collision_batched(plan, S.DeviceScene.from_host(scene), torch.from_numpy(cfg).to(DEV))
+    got = out.cpu().numpy()
+    for k, rq in enumerate(reqs):
+        ref, clr = PO.Oracle(rq.robot, rq.scene, nthreads=8).check_collision(rq.configs)
+        assert_verdicts(got[k * 700:(k + 1) * 700], ref, clr, f"request {k}")
+
+
+@pytest.mark.parametrize("name", ["panda", "fetch", "baxter"])
+def test_check_self(robots, name):
+    rng = np.random.default_rng(22)
+    r = robots[name].arrays
+    scene = W.random_obbs(rng, 10, lo=(0.4, -1.0, 0.0), hi=(1.2, 1.0, 1.4), half_hi=0.1)
+    q = W.uniform_configs(rng, r, 2000)
+    banner(f"batched any-collision + self {name}")
+    plan = S.Plan.configs(robots[name], len(q))
+    got = S.check_collision_batched(plan, S.DeviceScene.from_host(scene), torch.from_numpy(q).to(DEV),
+                                    check_self=True).cpu().numpy()
+    ref, clr = PO.Oracle(r, scene, nthreads=8).check_collision(q, check_self=True)
+    assert_verdicts(got, ref, clr, f"any-collision+self {name}")
+
+
+def esdf_scene(n=48, vs=0.02, center=(0.45, 0.0, 0.45), interp=0, with_obbs=True, seed=30):
+    rng = np.random.default_rng(seed)
+    obbs = W.random_obbs(rng, 12, lo=(0.2, -0.4, 0.1), hi=(0.7, 0.4, 0.8), half_hi=0.12)
+    params, pose = W.esdf_grid(center=center, n=n, voxel=vs)
+    vals = PO.esdf_from_obbs(obbs.obb_dims, obbs.obb_pose, params, pose).astype(np.float32)
+    extra = W.random_obbs(rng, 6) if with_obbs else HostScene()
+    s = HostScene(obb_dims=extra.obb_dims, obb_pose=extra.obb_pose, obb_enable=extra.obb_enable,
+                  voxel_params=params, voxel_pose=pose, voxel_enable=np.ones(1, np.uint8),
+                  voxel_features=vals.reshape(1, -1), esdf_interp=interp)
+    return s, obbs
+
+
+@pytest.mark.parametrize("interp", [0, 1])
+def test_esdf_any_collision_and_min_clearance(robots, interp):
+    s, _ = esdf_scene(interp=interp)
+    r = robots["panda"].arrays
+    q = W.uniform_configs(np.random.default_rng(31), r, 3000)
+    banner(f"ESDF any-collision interp={interp}")
+    ds = S.DeviceScene.from_host(s)
+    got = S.check_collision_scene(robots["panda"], ds, q)
+    o = PO.Oracle(r, s, nthreads=8)
+    ref, clr = o.check_collision(q)
+    assert_verdicts(got, ref, clr, f"ESDF any-collision interp={interp}")
+    plan = S.Plan.configs(robots["panda"], len(q))
+    mc = S.min_clearance_batched(plan, ds, torch.from_numpy(q).to(DEV), collision_margin=0.01).cpu().numpy()
+    mref = o.min_clearance(q, margin=0.01)
+    err = np.abs(mc - mref) / np.maximum(np.abs(mref), 0.01)
+    # nearest-voxel lookups can switch voxel at a boundary: those are excluded by band
+    ok = np.abs(mref) < 999
+    banner(f"min clearance max rel err {err[ok].max():.2e}")
+    assert np.mean(err[ok] < 1e-5) > 0.995
+
+
+def test_plain_edge_evaluator(robots):
+    e = W.edges(n_edges=600, n_obbs=12, seed=40)
+    ds = S.DeviceScene.from_host(e.scene)
+    ev = S.EdgeEvaluator(robots["panda"], ds, max_edges=1024)
+    o = PO.Oracle(e.robot, e.scene, nthreads=8)
+    for check_self in (False, True):
+        banner(f"EdgeEvalGpu resolution=0.025 check_self={check_self}")
+        valid, cost, _ = ev.evaluate(e.frm, e.to, granularity=32, resolution=0.025, check_self=check_self)
+        rv, rc, _ = o.edge_eval(e.frm, e.to, granularity=32, resolution=0.025, check_self=check_self)
+        mism = np.nonzero(valid != rv)[0]
+        # a mismatch must come from a state inside the ambiguous band
+        for k in mism:
+            G, _ = PO.edge_granularity(e.frm[k], e.to[k], 32, 0.025)
+            t = np.arange(1, G + 1, dtype=np.float32) / np.float32(G)
+            st = (e.frm[k] + t[:, None] * (e.to[k] - e.frm[k])).astype(np.float32)
+            _, clr = o.check_collision(st, check_self=check_self)
+            assert np.any(np.abs(clr) < BAND), f"edge {k} mismatch outside band"
+        ok = valid == rv
+        np.testing.assert_array_equal(cost[ok], rc[ok])
+        banner(f"  valid frac {rv.mean():.3f} mismatches {len(mism)}")
+    valid, cost, _ = ev.evaluate(e.frm, e.to, granularity=20, resolution=0.0)
+    rv, rc, _ = o.edge_eval(e.frm, e.to, granularity=20, resolution=0.0)
+    assert np.mean(valid == rv) > 0.995
+
+
+def test_plain_edge_clearance_with_esdf(robots):
+    s, _ = esdf_scene(interp=0, with_obbs=False)
+    e = W.edges(n_edges=200, n_obbs=1, seed=41)
+    ds = S.DeviceScene.from_host(s)
+    ev = S.EdgeEvaluator(robots["panda"], ds, max_edges=256)
+    valid, cost, clr = ev.evaluate(e.frm, e.to, resolution=0.05, compute_clearance=True, collision_margin=0.01)
+    rv, rc, rclr = PO.Oracle(e.robot, s, nthreads=8).edge_eval(e.frm, e.to, resolution=0.05, compute_clearance=True,
+                                                              margin=0.01)
+    ok = np.abs(rclr) < 999
+    err = np.abs(clr - rclr)[ok] / np.maximum(np.abs(rclr[ok]), 0.01)
+    banner(f"edge clearance max rel err {err.max():.2e}")
+    assert np.mean(err < 1e-5) > 0.99
+
+
+@pytest.mark.parametrize("n_obbs,seed", [(64, 1), (10, 42)])
+def test_batched_edges_first_hit(robots, n_obbs, seed):
+    e = W.edges(n_edges=3000, n_obbs=n_obbs, seed=seed)
+    banner(f"batched edges E=3000 O={n_obbs} states={e.total_states}")
+    ds = S.DeviceScene.from_host(e.scene)
+    plan = S.Plan.edges(robots["panda"], e.n_states)
+    frm, to = torch.from_numpy(e.frm).to(DEV), torch.from_numpy(e.to).to(DEV)
+    fh_skip = S.edge_check_batched(plan, ds, frm, to, skip_resolved=True).cpu().numpy()
+    fh_all, hits = S.edge_check_batched(plan, ds, frm, to, skip_resolved=False, state_hits=True)
+    fh_all, hits = fh_all.cpu().numpy(), hits.cpu().numpy()
+    np.testing.assert_array_equal(fh_skip, fh_all)
+    o = PO.Oracle(e.robot, e.scene, nthreads=8)
+    states = PO.edge_states(e.frm, e.to, e.n_states)
+    ref, clr = o.check_collision(states)
+    assert_verdicts(hits, ref, clr, "per-state edge verdicts")
+    off = e.offsets
+    for k in range(len(e.n_states)):
+        hk = ref[off[k]:off[k + 1]]
+        want = int(np.argmax(hk)) if hk.any() else -1
+        if fh_all[k] != want:
+            lo = min(x for x in (fh_all[k], want) if x >= 0)
+            assert abs(clr[off[k] + lo]) < BAND, f"edge {k}: first hit {fh_all[k]} vs {want}"
+
+
+def test_hinge_cost_configs(robots):
+    rng = np.random.default_rng(50)
+    for name in ("panda", "fetch", "baxter"):
+        r = robots[name].arrays
+        scene = W.random_obbs(rng, 30, lo=(-0.2, -0.9, 0.0), hi=(1.0, 0.9, 1.4))
+        q = W.uniform_configs(rng, r, 2000)
+        banner(f"hinge cost + self {name}")
+        prm_o = PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0)
+        prm = S.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0)
+        plan = S.Plan.configs(robots[name], len(q))
+        st, _, g = S.hinge_cost(plan, S.DeviceScene.from_host(scene), torch.from_numpy(q).to(DEV), prm, grad=True)
+        o = PO.Oracle(r, scene, nthreads=8)
+        c_ref, g_ref, kink = o.hinge_grad(q, prm_o)
+        c = st.cpu().numpy()
+        rel = np.abs(c - c_ref) / np.maximum(np.abs(c_ref), 2.0 * 0.05)
+        banner(f"  cost max rel err {rel.max():.2e} (active {np.mean(c_ref > 0):.2f})")
+        assert rel.max() < 1e-5
+        g = g.cpu().numpy()
+        gmax = np.abs(g_ref).max(axis=1, keepdims=True)
+        den = np.maximum(np.maximum(np.abs(g_ref), 1e-2 * gmax), 2e-3)
+        gerr = (np.abs(g - g_ref) / den)[~kink]
+        banner(f"  grad max rel err {gerr.max():.2e}, kink-excluded {int(kink.sum())}")
+        assert gerr.max() < 1e-4
+
+
+def test_chomp_adjoint_vs_oracle_and_fd(robots):
+    w = W.chomp(n_traj=48, n_wp=64)
+    banner("CHOMP adjoint: 48 trajectories x 64 waypoints")
+    ds = S.DeviceScene.from_host(w.scene)
+    plan = S.Plan.trajectories(robots["panda"], w.lengths())
+    th = torch.from_numpy(w.theta).to(DEV)
+    prm = S.cost_params(world_margin=0.05, weight=1.0, use_self=False)
+    prm_o = PO.cost_params(world_margin=0.05, weight=1.0, use_self=False)
+    st, total, g = S.hinge_cost(plan, ds, th, prm, grad=True)
+    o = PO.Oracle(w.robot, w.scene, nthreads=8)
+    tot_ref, g_ref, kink = o.traj_cost_grad(w.theta, w.offsets, prm_o)
+    total = total.cpu().numpy()
+    rel = np.abs(total - tot_ref) / np.maximum(np.abs(tot_ref), 0.05)
+    banner(f"  total max rel err {rel.max():.2e}")
+    assert rel.max() < 1e-5
+    g = g.cpu().numpy()
+    D = 7
+    kmask = np.repeat(kink.reshape(-1, 1), 1, 1)
+    errs = []
+    for b in range(w.n_traj):
+        t0, t1 = w.offsets[b], w.offsets[b + 1]
+        T = t1 - t0
+        gb, rb = g[t0 * D:t1 * D].reshape(D, T), g_ref[t0 * D:t1 * D].reshape(D, T)
+        kb = kink[t0:t1]
+        gmax = np.abs(rb).max(axis=0, keepdims=True)
+        den = np.maximum(np.maximum(np.abs(rb), 1e-2 * gmax), 1e-3)
+        e = (np.abs(gb - rb) / den)[:, ~kb]
+        errs.append(e.max() if e.size else 0.0)
+    banner(f"  grad max rel err {max(errs):.2e}, kink waypoints {int(kink.sum())}")
+    assert max(errs) < 1e-4
+    # the finite-difference kernel (chomp_obs_grad_kernel semantics) vs the FP64 oracle FD
+    gfd = S.chomp_fd_grad(plan, ds, th, prm, eps=1e-3).cpu().numpy()
+    gfd_ref = o.chomp_fd_grad(w.theta, w.offsets, prm_o, eps=1e-3)
+    # FP32 FD carries ~ulp(C)/(2 eps) of cancellation noise on top of FP64
+    tol = 1e-4 * np.abs(gfd_ref) + 2e-3
+    bad = np.abs(gfd - gfd_ref) > tol
+    banner(f"  FD kernel: max abs diff {np.abs(gfd - gfd_ref).max():.2e}, entries over tol {int(bad.sum())}")
+    assert bad.mean() < 1e-3
+
+
+def test_autograd_function(robots):
+    w = W.chomp(n_traj=8, n_wp=32, seed=7)
+    ds = S.DeviceScene.from_host(w.scene)
+    plan = S.Plan.trajectories(robots["panda"], w.lengths())
+    th = torch.from_numpy(w.theta).to(DEV).requires_grad_(True)
+    prm = S.cost_params(world_margin=0.05, weight=1.0, use_self=False)
+    total = S.HingeCostFn.apply(th, plan, ds, prm)
+    wts = torch.arange(1, 9, dtype=torch.float32, device=DEV)
+    (total * wts).sum().backward()
+    _, _, g = S.hinge_cost(plan, ds, th.detach(), prm, grad=True)
+    expect = g * torch.repeat_interleave(wts, torch.as_tensor(w.lengths(), device=DEV) * 7)
+    torch.testing.assert_close(th.grad, expect, rtol=0, atol=0)
+
+
+def test_stomp_rollout_costs_and_graph(robots):
+    wk = W.stomp(n_problems=8, n_rollouts=6, seed=3)
+    banner(f"STOMP rollouts: {wk.n_problems} problems x {wk.n_rollouts} rollouts, states {wk.n_states}")
+    R = wk.n_rollouts
+    traj_lengths = np.repeat(wk.lengths, R)
+    pu = np.arange(wk.n_problems + 1) * R
+    plan = S.Plan.trajectories(robots["fetch"], traj_lengths, problem_units=pu, obb_offsets=wk.obb_offsets)
+    ds = S.DeviceScene.from_host(wk.scene)
+    ro = torch.from_numpy(wk.rollouts).to(DEV)
+    prm = S.cost_params(world_margin=0.02, self_margin=0.01, weight=20.0)
+    state, total = S.stomp_rollout_costs(plan, ds, ro, prm, control_weight=0.1, dt=0.1)
+    o = PO.Oracle(wk.robot)
+    s_ref, t_ref = o.stomp_costs(wk.scene, wk.obb_offsets, wk.rollouts, wk.lengths, wk.roll_offsets, R,
+                                 PO.cost_params(world_margin=0.02, self_margin=0.01, weight=20.0), 0.1, 0.1)
+    st = state.cpu().numpy()
+    rel = np.abs(st - s_ref) / np.maximum(np.abs(s_ref), 20.0 * 0.02)
+    banner(f"  state cost max rel err {rel.max():.2e}")
+    assert rel.max() < 1e-5
+    tt = total.cpu().numpy()
+    trel = np.abs(tt - t_ref) / np.abs(t_ref)
+    banner(f"  rollout total max rel err {trel.max():.2e}")
+    assert trel.max() < 1e-5
+    # graph replay == eager, several replays with eager work and syncs between
+    s2 = torch.zeros_like(state)
+    t2 = torch.zeros_like(total)
+    graph = S.capture_stomp(plan, ds, ro, prm, 0.1, 0.1, s2, t2)
+    for k in range(3):
+        s2.zero_()
+        t2.zero_()
+        graph.launch()
+        torch.cuda.synchronize()
+        torch.testing.assert_close(t2, total, rtol=0, atol=0)
+        torch.testing.assert_close(s2, state, rtol=0, atol=0)
+        _ = S.stomp_rollout_costs(plan, ds, ro, prm, control_weight=0.1, dt=0.1)
+        torch.cuda.synchronize()
+
+
+def test_esdf_build_and_baxter_trilinear_adjoint(robots):
+    wk = W.esdf(n_traj=24, t_lo=16, t_hi=40, grid_n=64, voxel=0.02, seed=9)
+    obbs = wk.scene.source_obbs
+    banner("ESDF build 64^3 + Baxter trilinear adjoint")
+    params, pose = wk.scene.voxel_params, wk.scene.voxel_pose
+    feats = S.esdf_from_obbs(S.DeviceScene.from_host(obbs), params[0], pose[0]).cpu().numpy()
+    ref = PO.esdf_from_obbs(obbs.obb_dims, obbs.obb_pose, params, pose)
+    assert np.abs(feats - ref).max() < 2e-6
+    wk.scene.voxel_features = feats.reshape(1, -1)
+    ds = S.DeviceScene.from_host(wk.scene)
+    plan = S.Plan.trajectories(robots["baxter"], wk.lengths())
+    th = torch.from_numpy(wk.theta).to(DEV)
+    prm = S.cost_params(world_margin=0.05, self_margin=0.01, weight=1.0, use_obb=False, use_self=False)
+    st, total, g = S.hinge_cost(plan, ds, th, prm, grad=True)
+    o = PO.Oracle(wk.robot, wk.scene, nthreads=8)
+    prm_o = PO.cost_params(world_margin=0.05, self_margin=0.01, weight=1.0, use_obb=False, use_self=False)
+    tot_ref, g_ref, kink = o.traj_cost_grad(wk.theta, wk.offsets, prm_o)
+    rel = np.abs(total.cpu().numpy() - tot_ref) / np.maximum(np.abs(tot_ref), 0.05)
+    banner(f"  total max rel err {rel.max():.2e} active={np.mean(tot_ref > 0):.2f}")
+    assert rel.max() < 1e-5
+    g = g.cpu().numpy()
+    D = 14
+    worst = 0.0
+    for b in range(wk.n_traj):
+        t0, t1 = wk.offsets[b], wk.offsets[b + 1]
+        T = t1 - t0
+        gb, rb = g[t0 * D:t1 * D].reshape(D, T), g_ref[t0 * D:t1 * D].reshape(D, T)
+        kb = kink[t0:t1]
+        gmax = np.abs(rb).max(axis=0, keepdims=True)
+        den = np.maximum(np.maximum(np.abs(rb), 1e-2 * gmax), 1e-3)
+        e = (np.abs(gb - rb) / den)[:, ~kb]
+        worst = max(worst, e.max() if e.size else 0.0)
+    banner(f"  grad max rel err {worst:.2e}, kink waypoints {int(kink.sum())}")
+    assert worst < 1e-4
+
+
+def test_prrtc_compatible_surface():
+    from prismplan_amd import prrtc
+    from prismplan_amd.planning_scene import PlanningScene
+    from prismplan_amd.robot_model_loader import load_robot_asset
+    a = load_robot_asset("panda")
+    b = prrtc.RobotModelBuilder()
+    b.n_dof, b.n_joints = a["n_dof"], a["n_joints"]
+    for k in ("fixed_transforms", "joint_types", "sphere_to_joint", "joint_lower", "joint_upper", "joint_parents",
+              "joint_id_to_dof", "t_memory_idx", "dfs_order"):
+        setattr(b, k, a[k])
+    b.spheres_flat = a["spheres_flat"]
+    b.self_cc_ranges_flat = a["self_cc_ranges_flat"]
+    b.approx_spheres_flat = a["approx_spheres_flat"]
+    b.approx_sphere_to_joint = a["approx_sphere_to_joint"]
+    b.approx_self_cc_ranges_flat = a["approx_self_cc_ranges_flat"]
+    b.build()
+    scene = PlanningScene.from_robot_asset("panda")
+    scene.add_cuboid("table", dims=(1.2, 1.2, 0.05), position=(0.5, 0.0, 0.0))
+    ready = np.array([[0.0, -0.785, 0.0, -2.356, 0.0, 1.571, 0.785]], np.float32)
+    assert prrtc.check_collision_scene(ready, scene.build_scene_collision_data(), b)[0]
+    scene.add_cuboid("table", dims=(1.2, 1.2, 0.05), position=(0.5, 0.0, -0.1))
+    assert not prrtc.check_collision_scene(ready, scene.build_scene_collision_data(), b)[0]
+    with pytest.raises(RuntimeError):
+        prrtc.check_collision_scene(np.zeros((2, 6), np.float32), scene.build_scene_collision_data(), b)

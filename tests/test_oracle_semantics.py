@@ -1,0 +1,308 @@
+"""Robot-level oracle semantics: two-phase gate, known answers, gradients
+against FP64 finite differences, edges, ESDF and the smoothness metric (CPU)."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+from scipy.spatial.transform import Rotation
+
+import pyoracle as PO
+from prismplan_amd import workloads as W
+from prismplan_amd.planning_scene import HostScene, grid_dims
+from prismplan_amd.robot_model_loader import robot_arrays
+
+GOLDEN = Path(__file__).parent / "golden"
+READY = np.array([0.0, -0.785, 0.0, -2.356, 0.0, 1.571, 0.785], np.float32)
+
+
+def test_two_phase_equals_exhaustive_away_from_ties():
+    w = W.any_collision(n_configs=1024)[0]
+    o = PO.Oracle(w.robot, w.scene, nthreads=4)
+    h2, clr = o.check_collision(w.configs, mode=0)
+    hx, _ = o.check_collision(w.configs, mode=1)
+    ok = np.abs(clr) >= 1e-5
+    np.testing.assert_array_equal(h2[ok], hx[ok])
+    np.testing.assert_array_equal(h2[ok], clr[ok] < 0)
+    assert 0.2 < h2.mean() < 0.9
+
+
+def test_fp32_port_agrees_with_fp64_outside_band():
+    w = W.any_collision(n_configs=2048)[0]
+    o = PO.Oracle(w.robot, w.scene, nthreads=4)
+    h64, clr = o.check_collision(w.configs)
+    h32 = o.check_collision_f32_timed(w.configs)
+    ok = np.abs(clr) >= 1e-5
+    np.testing.assert_array_equal(h64[ok], h32[ok])
+
+
+def test_table_slab_known_answer():
+    """examples/bench_6way.py:60-74: a slab at z = 0 collides with the base in
+    the ready pose; lowered to z = -0.1 it does not."""
+    r = robot_arrays("panda")
+    for z, expect in ((0.0, True), (-0.1, False)):
+        s = HostScene.from_boxes([[1.2, 1.2, 0.05]], [[0.5, 0.0, z]], [[1, 0, 0, 0]])
+        hit, _ = PO.Oracle(r, s).check_collision(READY[None])
+        assert bool(hit[0]) is expect
+
+
+def mbm_box_scenes():
+    d = json.loads((GOLDEN / "sample_mbm_problems.json").read_text())
+    for prob in d["problems"]["box"]:
+        boxes = prob["box"]
+        dims = [2 * np.asarray(b["half_extents"]) for b in boxes]
+        pos = [b["position"] for b in boxes]
+        quat = []
+        for b in boxes:
+            x, y, z, w = Rotation.from_euler("xyz", b["orientation_euler_xyz"]).as_quat()
+            quat.append([w, x, y, z])
+        yield prob, HostScene.from_boxes(dims, pos, quat)
+
+
+def test_mbm_valid_start_and_goal_are_free():
+    r = robot_arrays("panda")
+    for prob, scene in mbm_box_scenes():
+        assert prob["valid"]
+        q = np.array([prob["start"], *prob["goals"]], np.float32)
+        hit, clr = PO.Oracle(r, scene).check_collision(q)
+        assert not hit.any(), clr
+
+
+def test_margin_is_monotone():
+    w = W.any_collision(n_configs=512)[0]
+    o = PO.Oracle(w.robot, w.scene)
+    h0, _ = o.check_collision(w.configs, margin=0.0)
+    h1, _ = o.check_collision(w.configs, margin=0.05)
+    assert np.all(h1 >= h0) and h1.sum() > h0.sum()
+
+
+def test_self_collision_check():
+    r = robot_arrays("panda")
+    o = PO.Oracle(r)
+    q = W.uniform_configs(np.random.default_rng(4), r, 2000)
+    hit, clr = o.check_collision(q, check_self=True, mode=0)
+    hx, _ = o.check_collision(q, check_self=True, mode=1)
+    ok = np.abs(clr) >= 1e-5
+    np.testing.assert_array_equal(hit[ok], hx[ok])
+    np.testing.assert_array_equal(hit[ok], clr[ok] < 0)
+    assert 0 < hit.mean() < 1
+
+
+# ---------------------------------------------------------------------------
+# summed hinge and its analytic gradient
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name,use_self", [("panda", False), ("panda", True), ("fetch", True), ("baxter", True)])
+def test_analytic_gradient_matches_fp64_central_fd(name, use_self):
+    rng = np.random.default_rng(12)
+    r = robot_arrays(name)
+    scene = W.random_obbs(rng, 30, lo=(-0.2, -0.9, 0.0), hi=(1.0, 0.9, 1.4))
+    o = PO.Oracle(r, scene, nthreads=4)
+    prm = PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0, use_self=use_self)
+    q = W.uniform_configs(rng, r, 200).astype(np.float64)
+    c, g, kink = o.hinge_grad(q.astype(np.float32), prm, tol=1e-5)
+    assert (c > 0).mean() > 0.3
+    # Inputs are float32, so the step is a few float32 ulps (2^-20 rad); the
+    # step h is taken from the perturbed float32 values themselves and the
+    # cost difference is evaluated in FP64, so FD round-off stays ~1e-9.
+    eps = 2.0 ** -20
+    n_checked = 0
+    for i in range(q.shape[0]):
+        if kink[i] or c[i] == 0:
+            continue
+        for d in range(r.n_dof):
+            qp = q[i].astype(np.float32).copy()
+            qm = qp.copy()
+            qp[d] = qp[d] + np.float32(eps)
+            qm[d] = qm[d] - np.float32(eps)
+            h = (float(qp[d]) - float(qm[d]))
+            cp, cm = o.hinge_cost(np.stack([qp, qm]), prm)
+            fd = (cp - cm) / h
+            # central FD error O(eps^2 * curvature); curvature <= ~1/r_min
+            assert g[i, d] == pytest.approx(fd, rel=1e-4, abs=1e-5), (i, d)
+            n_checked += 1
+    assert n_checked > 200
+
+
+def test_traj_gradient_zero_at_endpoints_and_sums_waypoints():
+    w = W.chomp(n_traj=4, n_wp=16)
+    o = PO.Oracle(w.robot, w.scene)
+    prm = PO.cost_params(world_margin=0.05, use_self=False)
+    total, grad, kink = o.traj_cost_grad(w.theta, w.offsets, prm)
+    D = w.robot.n_dof
+    for b in range(w.n_traj):
+        t0, t1 = w.offsets[b], w.offsets[b + 1]
+        T = t1 - t0
+        th = w.theta[t0 * D:t1 * D].reshape(D, T)
+        g = grad[t0 * D:t1 * D].reshape(D, T)
+        assert np.all(g[:, 0] == 0) and np.all(g[:, -1] == 0)
+        per_wp = o.hinge_cost(th.T.copy(), prm)
+        assert total[b] == pytest.approx(per_wp.sum(), rel=1e-12)
+        _, gq, _ = o.hinge_grad(th.T.copy(), prm)
+        np.testing.assert_allclose(g[:, 1:-1], gq.T[:, 1:-1], rtol=0, atol=1e-12)
+
+
+def test_fd_gradient_semantics():
+    """chomp_obs_grad_kernel: w (C(q+eps e_d) - C(q-eps e_d)) / (2 eps)."""
+    w = W.chomp(n_traj=2, n_wp=10)
+    o = PO.Oracle(w.robot, w.scene)
+    prm = PO.cost_params(world_margin=0.05, weight=3.0, use_self=False)
+    g = o.chomp_fd_grad(w.theta, w.offsets, prm, eps=1e-3)
+    D, T = w.robot.n_dof, 10
+    th = w.theta[:D * T].reshape(D, T)
+    t, d = 4, 2
+    qp = th[:, t].copy()
+    qp[d] += np.float32(1e-3)
+    qm = qp.copy()
+    qm[d] -= np.float32(2e-3)
+    prm1 = PO.cost_params(world_margin=0.05, weight=1.0, use_self=False)
+    cp, cm = o.hinge_cost(np.stack([qp, qm]), prm1)
+    eps = float(np.float32(1e-3))
+    assert g[:D * T].reshape(D, T)[d, t] == pytest.approx(3.0 * (cp - cm) / (2 * eps), rel=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# edges
+# ---------------------------------------------------------------------------
+def test_edge_granularity_rule():
+    a = np.zeros(7, np.float32)
+    assert PO.edge_granularity(a, a, 32, 0.025)[0] == 2          # zero length clamps to 2
+    b = a.copy()
+    b[0] = 1.0
+    assert PO.edge_granularity(a, b, 32, 0.025)[0] == 40
+    b[0] = 100.0
+    assert PO.edge_granularity(a, b, 32, 0.025)[0] == 1024       # upper clamp
+    assert PO.edge_granularity(a, b, 17, 0.0)[0] == 17           # fixed granularity
+
+
+def test_edge_eval_matches_state_checks():
+    e = W.edges(n_edges=64, n_obbs=12)
+    o = PO.Oracle(e.robot, e.scene)
+    valid, cost, _ = o.edge_eval(e.frm, e.to, granularity=0, resolution=0.05)
+    for k in range(16):
+        G, ln = PO.edge_granularity(e.frm[k], e.to[k], 0, 0.05)
+        t = (np.arange(1, G + 1, dtype=np.float32) / np.float32(G)).astype(np.float32)
+        q = e.frm[k] + t[:, None] * (e.to[k] - e.frm[k])
+        hit, clr = o.check_collision(q.astype(np.float32))
+        if np.all(np.abs(clr) > 1e-4):
+            assert valid[k] == (not hit.any())
+        assert cost[k] == (np.float32(ln) if valid[k] else np.finfo(np.float32).max)
+
+
+def test_first_hit_closed_form():
+    """An edge that moves the base joint into a box in front of the robot hits
+    at the first state whose configuration collides (brute force over states)."""
+    r = robot_arrays("panda")
+    scene = HostScene.from_boxes([[0.1, 0.1, 0.2]], [[0.0, 0.3, 0.55]], [[1, 0, 0, 0]])
+    o = PO.Oracle(r, scene)
+    frm = READY.copy()
+    to = READY.copy()
+    to[0] = 2.5
+    n = np.array([101], np.int32)
+    fh = o.edge_batch(frm[None], to[None], n)[0]
+    states = PO.edge_states(frm[None], to[None], n)
+    hit, _ = o.check_collision(states)
+    assert hit.any()
+    assert fh == int(np.argmax(hit))
+    assert fh > 0
+
+
+def test_edge_batch_first_hit_equals_state_scan():
+    e = W.edges(n_edges=40, n_obbs=10)
+    o = PO.Oracle(e.robot, e.scene)
+    fh = o.edge_batch(e.frm, e.to, e.n_states)
+    states = PO.edge_states(e.frm, e.to, e.n_states)
+    hit, _ = o.check_collision(states)
+    off = e.offsets
+    for k in range(40):
+        hk = hit[off[k]:off[k + 1]]
+        assert fh[k] == (int(np.argmax(hk)) if hk.any() else -1)
+
+
+# ---------------------------------------------------------------------------
+# ESDF
+# ---------------------------------------------------------------------------
+def plane_scene(n=32, vs=0.05, interp=1):
+    """Grid whose values are an analytic plane field d(p) = a.p + b."""
+    from prismplan_amd.workloads import esdf_grid
+    params, pose = esdf_grid(center=(0.3, -0.1, 0.5), n=n, voxel=vs)
+    nx, ny, nz = grid_dims(params[0])
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    local = np.stack([ii * vs, jj * vs, kk * vs], -1) - params[0, :3] / 2
+    world = local - pose[0, :3]
+    a = np.array([0.3, -0.5, 0.8])
+    vals = world @ a - 0.2
+    s = HostScene(voxel_params=params, voxel_pose=pose, voxel_enable=np.ones(1, np.uint8),
+                  voxel_features=vals.reshape(1, -1).astype(np.float32), esdf_interp=interp)
+    return s, a
+
+
+def test_trilinear_reproduces_a_plane_field():
+    s, a = plane_scene()
+    o = PO.Oracle(robot_arrays("panda"), s)
+    rng = np.random.default_rng(0)
+    for _ in range(100):
+        p = np.array([0.3, -0.1, 0.5]) + rng.uniform(-0.6, 0.6, 3)
+        c = PO.sphere_esdf_clearance(o, p, 0.02)
+        if c == 1000.0:
+            continue
+        assert c == pytest.approx(-(p @ a - 0.2) - 0.02, abs=1e-5)
+
+
+def test_nearest_lookup_uses_truncated_index():
+    s, a = plane_scene(interp=0)
+    o = PO.Oracle(robot_arrays("panda"), s)
+    vs = 0.05
+    p = np.array([0.3, -0.1, 0.5]) + np.array([0.071, -0.033, 0.12])
+    local = p - np.array([0.3, -0.1, 0.5]) + 31 * vs / 2
+    idx = np.floor(local / vs)
+    node = idx * vs - 31 * vs / 2 + np.array([0.3, -0.1, 0.5])
+    c = PO.sphere_esdf_clearance(o, p, 0.0)
+    assert c == pytest.approx(-(node @ a - 0.2), abs=1e-5)
+
+
+def test_esdf_from_obbs_is_negated_min_sdf():
+    rng = np.random.default_rng(1)
+    obbs = W.random_obbs(rng, 5, lo=(0.0, -0.2, 0.3), hi=(0.6, 0.2, 0.7), half_hi=0.1)
+    from prismplan_amd.workloads import esdf_grid
+    params, pose = esdf_grid(center=(0.3, 0.0, 0.5), n=16, voxel=0.05)
+    vals = PO.esdf_from_obbs(obbs.obb_dims, obbs.obb_pose, params, pose)
+    vs, ext = 0.05, 15 * 0.05
+    for _ in range(40):
+        i, j, k = rng.integers(0, 16, 3)
+        p = np.array([i, j, k]) * vs - ext / 2 + np.array([0.3, 0.0, 0.5])
+        m = min(PO.sphere_obb_clearance(obbs.obb_dims[o], obbs.obb_pose[o], p, 0.0) for o in range(5))
+        assert vals[i * 256 + j * 16 + k] == pytest.approx(-m, abs=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# smoothness metric
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("T", [5, 32, 64, 101])
+def test_control_R_closed_form(T):
+    R = PO.control_R(T, 0.1)
+    np.testing.assert_allclose(R, W.control_R_closed_form(T, 0.1), rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(R, R.T, atol=1e-9)
+    assert np.linalg.eigvalsh(R).min() > 0
+    # acceleration of a straight line vanishes away from the padded ends
+    x = np.linspace(-1, 2, T)
+    if T > 12:
+        assert np.abs((R @ x)[6:-6]).max() < 1e-6 * np.abs(R).max()
+
+
+def test_control_cost_sum_of_squares_identity():
+    rng = np.random.default_rng(3)
+    T, D, dt = 40, 3, 0.1
+    th = rng.standard_normal((D, T)).astype(np.float32)
+    off = np.array([0, T], np.int32)
+    c = PO.control_cost(th.reshape(-1), off, D, dt, 1.0)[0]
+    # dt * sum of squared 5-point accelerations over the zero-padded rows
+    ssq = 0.0
+    for d in range(D):
+        x = np.concatenate([np.zeros(2), th[d].astype(np.float64), np.zeros(2)])
+        acc = (-x[:-4] + 16 * x[1:-3] - 30 * x[2:-2] + 16 * x[3:-1] - x[4:]) / (12 * dt * dt)
+        ssq += np.sum(acc ** 2)
+        # rows straddling the ends
+        xp = np.concatenate([np.zeros(4), th[d].astype(np.float64), np.zeros(4)])
+        full = (-xp[:-4] + 16 * xp[1:-3] - 30 * xp[2:-2] + 16 * xp[3:-1] - xp[4:]) / (12 * dt * dt)
+        ssq += np.sum(full ** 2) - np.sum(acc ** 2)
+    assert c == pytest.approx(dt * ssq, rel=1e-10)
