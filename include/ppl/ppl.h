@@ -9,7 +9,7 @@
  * Memory: "h_" arguments are host arrays, "d_" arguments device arrays
  * (HBM). Streams are hipStream_t passed as void* (NULL = default stream).
  * Calls that share one ppl_robot, ppl_edge_eval or ppl_plan must not run
- * concurrently on different streams: each handle owns one work counter.
+ * concurrently on different streams: each handle owns its scratch buffers.
  */
 #ifndef PPL_PPL_H
 #define PPL_PPL_H

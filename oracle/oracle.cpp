@@ -201,7 +201,7 @@ inline int robust_floor(float x, float threshold = 1e-4f) {
 // within a hair of a voxel or validity boundary (decision ambiguous).
 template <typename T>
 bool esdf_layer(const orc_scene& S, int layer, T x, T y, T z, int sphere_idx, T* dist, V3<T>* grad,
-                bool* amb) {
+                bool* amb, T amb_r = 0) {
     if (S.voxel_enable[layer] == 0) return false;
     if (sphere_idx >= 0 && S.sphere_acm && S.sphere_acm[sphere_idx * S.n_voxel_layers + layer] == 0)
         return false;
@@ -221,19 +221,31 @@ bool esdf_layer(const orc_scene& S, int layer, T x, T y, T z, int sphere_idx, T*
     const float* feat = S.voxel_features + (size_t)layer * S.max_voxels_per_layer;
     if (S.esdf_interp == 0) {
         // Nearest lookup by C truncation, scene_collision.cuh:238-261.
+        auto lookup = [&](const int* idx, float* d) {
+            for (int k = 0; k < 3; ++k)
+                if (idx[k] >= n[k] - kOff || idx[k] <= kOff) return false;
+            const long lin = (long)idx[0] * ny * nz + (long)idx[1] * nz + idx[2];
+            if (lin >= S.max_voxels_per_layer) return false;
+            *d = feat[lin];
+            return *d > kUnobserved + 1.0f;
+        };
         int idx[3];
+        for (int k = 0; k < 3; ++k) idx[k] = (int)u[k];
+        float d0 = 0.0f;
+        const bool ok0 = lookup(idx, &d0);
+        // A lookup within a hair of a voxel face is ambiguous when the
+        // neighbouring voxel would flip the collision decision.
         for (int k = 0; k < 3; ++k) {
-            idx[k] = (int)u[k];
             const T fr = u[k] - std::floor(u[k]);
-            if (fr < (T)1e-4 || fr > (T)(1 - 1e-4)) *amb = true;
+            if (!(fr < (T)1e-4 || fr > (T)(1 - 1e-4))) continue;
+            int alt[3] = {idx[0], idx[1], idx[2]};
+            alt[k] += (fr < (T)0.5) ? -1 : 1;
+            float d1 = 0.0f;
+            const bool ok1 = lookup(alt, &d1);
+            if ((ok0 && d0 > -amb_r) != (ok1 && d1 > -amb_r)) *amb = true;
         }
-        for (int k = 0; k < 3; ++k)
-            if (idx[k] >= n[k] - kOff || idx[k] <= kOff) return false;
-        const long lin = (long)idx[0] * ny * nz + (long)idx[1] * nz + idx[2];
-        if (lin >= S.max_voxels_per_layer) return false;
-        const float d = feat[lin];
-        if (d <= kUnobserved + 1.0f) return false;
-        *dist = (T)d;
+        if (!ok0) return false;
+        *dist = (T)d0;
         if (grad) *grad = {0, 0, 0};
         return true;
     }
@@ -245,8 +257,9 @@ bool esdf_layer(const orc_scene& S, int layer, T x, T y, T z, int sphere_idx, T*
         const T fl = std::floor(u[k]);
         i0[k] = (int)fl;
         f[k] = u[k] - fl;
-        if ((i0[k] == kOff + 1 || i0[k] == n[k] - kOff - 1) && (f[k] < (T)1e-4 || f[k] > (T)(1 - 1e-4)))
-            *amb = true;
+        if ((i0[k] == kOff + 1 || i0[k] == kOff || i0[k] == n[k] - kOff - 1 || i0[k] == n[k] - kOff) &&
+            (f[k] < (T)1e-4 || f[k] > (T)(1 - 1e-4)))
+            *amb = true;   // the layer switches between valid and skipped here
     }
     for (int k = 0; k < 3; ++k)
         if (i0[k] >= n[k] - kOff || i0[k] <= kOff) return false;
@@ -284,7 +297,7 @@ T esdf_clearance(const orc_scene& S, T x, T y, T z, T r, int sphere_idx, V3<T>* 
     T best = (T)1000;
     for (int layer = 0; layer < S.n_voxel_layers; ++layer) {
         T d; V3<T> g;
-        if (!esdf_layer<T>(S, layer, x, y, z, sphere_idx, &d, grad ? &g : nullptr, amb)) continue;
+        if (!esdf_layer<T>(S, layer, x, y, z, sphere_idx, &d, grad ? &g : nullptr, amb, r)) continue;
         const T c = -d - r;
         if (c < best) { best = c; if (grad) *grad = {-g.x, -g.y, -g.z}; }
     }
@@ -817,6 +830,20 @@ void orc_hinge_grad_f64(const orc_robot* r, const orc_scene* s, const float* q, 
             bool k = false;
             cost[i] = hinge_one<double>(*r, *s, q + (size_t)i * r->n_dof, *p, w, grad + (size_t)i * r->n_dof, &k, tol);
             if (kink) kink[i] = k ? 1 : 0;
+        }
+    }
+}
+
+void orc_hinge_grad_f32(const orc_robot* r, const orc_scene* s, const float* q, int n, const orc_cost_params* p,
+                        float* cost, float* grad, int nthreads) {
+    set_threads(nthreads);
+#pragma omp parallel
+    {
+        Work<float> w(*r);
+#pragma omp for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            bool k = false;
+            cost[i] = hinge_one<float>(*r, *s, q + (size_t)i * r->n_dof, *p, w, grad + (size_t)i * r->n_dof, &k, 0.0f);
         }
     }
 }

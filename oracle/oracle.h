@@ -106,6 +106,9 @@ void orc_hinge_grad_f64(const orc_robot* r, const orc_scene* s, const float* q, 
                         const orc_cost_params* p, double* cost, double* grad, int* kink,
                         double tol, int nthreads);
 
+void orc_hinge_grad_f32(const orc_robot* r, const orc_scene* s, const float* q, int n,
+                        const orc_cost_params* p, float* cost, float* grad, int nthreads);
+
 /* ---- trajectories, ragged [B] x [D][T_b] at waypoint offsets off[B+1] ---- */
 /* total[b] = sum_t cost(theta_b[:, t]); grad[b][d][t] = d total / d theta, 0 at both ends */
 void orc_traj_cost_grad_f64(const orc_robot* r, const orc_scene* s, const float* theta,

@@ -96,7 +96,6 @@ struct ppl_robot_s {
     bool subtrees_contiguous = true;
     std::vector<float> lower, upper;
     hipStream_t stream = nullptr;
-    unsigned* counter = nullptr;   // plain-call work counter
     DBuf d_cfg, d_out, d_off, d_aux, d_self;
     HBuf h_stage;
 };
@@ -106,7 +105,6 @@ struct ppl_edge_eval_s {
     DScene S{};
     int dim = 0, max_edges = 0;
     hipStream_t stream = nullptr;
-    unsigned* counter = nullptr;
     DBuf d_from, d_to, d_off, d_fh, d_clr_state, d_clr_edge, d_self;
     HBuf h_from, h_to, h_off, h_fh, h_clr;
 };
@@ -121,10 +119,10 @@ struct ppl_plan_s {
     std::vector<int> h_obb_off;      // empty: single scene
     std::vector<int> h_off;          // edges / trajectories: state offsets [n_units+1]
     int* d_off = nullptr;            // [n_units+1]
-    int4* d_tab = nullptr;
-    int n_chunks = 0;
+    std::vector<long> pb, pe;        // state range of each problem
+    int4* d_tab[4] = {nullptr, nullptr, nullptr, nullptr};   // chunk tables for 64, 32, 16, 8 states
+    int n_chunks[4] = {0, 0, 0, 0};
     int* d_obb_off = nullptr;
-    unsigned* counter = nullptr;
     DBuf d_self, d_cost, d_fdcfg, d_fdcost;
 };
 
@@ -174,32 +172,57 @@ int grid_for(KernelKind k, const DRobot& R, int obb_cap) {
     return k == kKCollide ? grid_collide(lds) : grid_cost(R.n_joints, k == kKCostGrad, lds);
 }
 
-DChunks implicit_chunks(long n_states, unsigned* counter, KernelKind k, const DRobot& R, int obb_cap, int* grid_out) {
+DChunks implicit_chunks(long n_states, KernelKind k, const DRobot& R, int obb_cap, int* grid_out, int G = 1) {
     const int grid_cap = grid_for(k, R, obb_cap);
     DChunks C{};
     C.obb_cap = obb_cap;
     C.tab = nullptr;
-    C.n_chunks = (int)((n_states + kWave - 1) / kWave);
+    C.chunk = kWave / G;
+    C.n_chunks = (int)((n_states + C.chunk - 1) / C.chunk);
     C.obb_off = nullptr;
-    C.counter = counter;
-    const int grid = std::max(1, std::min(grid_cap, C.n_chunks));
-    C.n_waves = grid;
-    *grid_out = grid;
+    *grid_out = std::max(1, std::min(grid_cap, C.n_chunks));
     return C;
 }
 
-DChunks plan_chunks(const ppl_plan_s* P, KernelKind k, int obb_cap, int* grid_out) {
+int group_index(int G) { return G >= 8 ? 3 : G >= 4 ? 2 : G >= 2 ? 1 : 0; }
+
+// Chunk table of a plan for `G` lanes per state (64 / G states per chunk),
+// built on first use; chunks never cross a problem boundary.
+hipError_t ensure_table(ppl_plan_s* P, int G) {
+    const int gi = group_index(G);
+    if (P->d_tab[gi] || P->n_states == 0) return hipSuccess;
+    const long chunk = kWave / G;
+    std::vector<int4> tab;
+    for (size_t p = 0; p < P->pb.size(); ++p)
+        for (long s = P->pb[p]; s < P->pe[p]; s += chunk)
+            tab.push_back(make_int4((int)p, (int)s, (int)std::min(s + chunk, P->pe[p]), 0));
+    if (tab.empty()) return hipSuccess;
+    hipError_t e = hipMalloc((void**)&P->d_tab[gi], tab.size() * sizeof(int4));
+    if (e == hipSuccess) e = hipMemcpy(P->d_tab[gi], tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) P->n_chunks[gi] = (int)tab.size();
+    return e;
+}
+
+DChunks plan_chunks(const ppl_plan_s* P, KernelKind k, int obb_cap, int G, int* grid_out) {
+    const int gi = group_index(G);
     const int grid_cap = grid_for(k, P->robot->R, obb_cap);
     DChunks C{};
     C.obb_cap = obb_cap;
-    C.tab = P->d_tab;
-    C.n_chunks = P->n_chunks;
+    C.tab = P->d_tab[gi];
+    C.n_chunks = P->n_chunks[gi];
+    C.chunk = kWave / G;
     C.obb_off = P->d_obb_off;
-    C.counter = P->counter;
-    const int grid = std::max(1, std::min(grid_cap, C.n_chunks));
-    C.n_waves = grid;
-    *grid_out = grid;
+    *grid_out = std::max(1, std::min(grid_cap, C.n_chunks));
     return C;
+}
+
+// Lanes per state for the cost kernel: enough lanes for about two waves per
+// SIMD across the chip, at most 8.
+int pick_group(long n_states) {
+    const long want = (long)device_cus() * 4 * 2 * kWave;
+    int G = 1;
+    while (G < 8 && n_states * G < want) G <<= 1;
+    return G;
 }
 
 DStates plan_states(const ppl_plan_s* P, const float* x) {
@@ -365,23 +388,16 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
     for (int a = 0; a < A; ++a)
         apx_orig[a] = make_float4(d->approx_spheres[4 * a], d->approx_spheres[4 * a + 1], d->approx_spheres[4 * a + 2],
                                   d->approx_spheres[4 * a + 3]);
-    // Subtree of DFS position p = [p, end_p); close_mask[k] marks the p with end_p == k.
-    std::vector<unsigned> close(J + 1, 0u);
+    // Tree depth by DFS position (the adjoint keeps one live joint per depth).
+    std::vector<int> close(J, 0);
     bool contiguous = true;
-    for (int p = 0; p < J; ++p) {
-        auto is_desc = [&](int j) {
-            for (int g = 0; g <= J; ++g) {
-                if (j == dfs[p]) return true;
-                if (j == parents[j]) return false;
-                j = parents[j];
-            }
-            return false;
-        };
-        int end = p + 1;
-        while (end < J && is_desc(dfs[end])) ++end;
-        for (int q = end; q < J; ++q)
-            if (is_desc(dfs[q])) contiguous = false;
-        close[end] |= 1u << p;
+    for (int k = 0; k < J; ++k) {
+        int j = dfs[k], d = 0;
+        while (j != parents[j] && d <= J) { j = parents[j]; ++d; }
+        close[k] = d;
+        if (d >= kMaxDepth) {
+            return fail(PPL_ERR_UNSUPPORTED, "kinematic tree depth %d exceeds %d", d, kMaxDepth - 1);
+        }
     }
     // One device blob for every table.
     size_t off = 0;
@@ -419,8 +435,6 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
     }
     hipError_t e = hipMemcpy(r->blob, host.data(), host.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc((void**)&r->counter, 2 * sizeof(unsigned));
-    if (e == hipSuccess) e = hipMemset(r->counter, 0, 2 * sizeof(unsigned));
     if (e != hipSuccess) {
         ppl_robot_destroy(r);
         return fail(PPL_ERR_HIP, "robot upload failed: %s", hipGetErrorString(e));
@@ -437,7 +451,7 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
     R.apx_off = reinterpret_cast<const int*>(b + o_aoff);
     R.apx = reinterpret_cast<const float4*>(b + o_apx);
     R.apx_id = reinterpret_cast<const int*>(b + o_aid);
-    R.close_mask = reinterpret_cast<const unsigned*>(b + o_close);
+    R.depth = reinterpret_cast<const int*>(b + o_close);
     R.sph_orig = reinterpret_cast<const float4*>(b + o_sorig);
     R.pairs = reinterpret_cast<const int2*>(b + o_pairs);
     R.apx_pairs = reinterpret_cast<const int2*>(b + o_apairs);
@@ -455,7 +469,6 @@ ppl_status ppl_robot_destroy(ppl_robot r) {
     if (!r) return PPL_OK;
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     if (r->blob) (void)hipFree(r->blob);
-    if (r->counter) (void)hipFree(r->counter);
     r->d_cfg.release(); r->d_out.release(); r->d_off.release(); r->d_aux.release(); r->d_self.release();
     r->h_stage.release();
     if (r->stream) (void)hipStreamDestroy(r->stream);
@@ -509,7 +522,7 @@ ppl_status ppl_check_collision_scene(ppl_robot r, const ppl_scene_view* view, co
     a.X.cfg = r->d_cfg.as<float>();
     a.X.cfg_stride = D;
     int grid = 0;
-    a.C = implicit_chunks(n, r->counter, kKCollide, r->R, S.n_obbs, &grid);
+    a.C = implicit_chunks(n, kKCollide, r->R, S.n_obbs, &grid);
     a.margin = margin;
     a.mode = 0;
     a.out = r->d_out.as<uint8_t>();
@@ -540,8 +553,6 @@ ppl_status ppl_edge_eval_create(ppl_robot r, const ppl_scene_view* view, int dim
     ev->max_edges = max_edges;
     const size_t cfg = (size_t)max_edges * dim * sizeof(float);
     hipError_t e = hipStreamCreateWithFlags(&ev->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc((void**)&ev->counter, 2 * sizeof(unsigned));
-    if (e == hipSuccess) e = hipMemset(ev->counter, 0, 2 * sizeof(unsigned));
     if (e == hipSuccess) e = ev->d_from.ensure(cfg);
     if (e == hipSuccess) e = ev->d_to.ensure(cfg);
     if (e == hipSuccess) e = ev->d_off.ensure((size_t)(max_edges + 1) * 4);
@@ -562,7 +573,6 @@ ppl_status ppl_edge_eval_create(ppl_robot r, const ppl_scene_view* view, int dim
 ppl_status ppl_edge_eval_destroy(ppl_edge_eval ev) {
     if (!ev) return PPL_OK;
     if (ev->stream) (void)hipStreamSynchronize(ev->stream);
-    if (ev->counter) (void)hipFree(ev->counter);
     ev->d_from.release(); ev->d_to.release(); ev->d_off.release(); ev->d_fh.release();
     ev->d_clr_state.release(); ev->d_clr_edge.release(); ev->d_self.release();
     ev->h_from.release(); ev->h_to.release(); ev->h_off.release(); ev->h_fh.release(); ev->h_clr.release();
@@ -646,7 +656,7 @@ ppl_status ppl_edge_eval_evaluate(ppl_edge_eval ev, const float* h_from, const f
         a.S = ev->S;
         a.X = X;
         int grid = 0;
-        a.C = implicit_chunks(total, ev->counter, kKCollide, r->R, ev->S.n_obbs, &grid);
+        a.C = implicit_chunks(total, kKCollide, r->R, ev->S.n_obbs, &grid);
         a.margin = p->collision_margin;
         a.mode = 1;
         a.skip_resolved = 1;
@@ -663,7 +673,8 @@ ppl_status ppl_edge_eval_evaluate(ppl_edge_eval ev, const float* h_from, const f
         c.S = ev->S;
         c.X = X;
         int grid = 0;
-        c.C = implicit_chunks(total, ev->counter, kKCost, r->R, ev->S.n_obbs, &grid);
+        c.C = implicit_chunks(total, kKCost, r->R, ev->S.n_obbs, &grid);
+        c.group = 1;
         c.P.weight = 1.0f;
         c.clearance_only = 1;
         c.margin = p->collision_margin;
@@ -750,18 +761,13 @@ ppl_status ppl_plan_create(ppl_robot r, const ppl_plan_desc* d, ppl_plan* out) {
             }
         }
     }
-    std::vector<int4> tab;
     for (int p = 0; p < d->n_problems; ++p) {
-        const long b = uoff[pu[p]], e = uoff[pu[p + 1]];
-        for (long s = b; s < e; s += kWave) tab.push_back(make_int4(p, (int)s, (int)std::min(s + kWave, e), 0));
+        P->pb.push_back(uoff[pu[p]]);
+        P->pe.push_back(uoff[pu[p + 1]]);
     }
-    P->n_chunks = (int)tab.size();
     P->h_off.assign(uoff.size(), 0);
     for (size_t i = 0; i < uoff.size(); ++i) P->h_off[i] = (int)uoff[i];
-    hipError_t e = hipMalloc((void**)&P->counter, 2 * sizeof(unsigned));
-    if (e == hipSuccess) e = hipMemset(P->counter, 0, 2 * sizeof(unsigned));
-    if (e == hipSuccess && !tab.empty()) e = hipMalloc((void**)&P->d_tab, tab.size() * sizeof(int4));
-    if (e == hipSuccess && !tab.empty()) e = hipMemcpy(P->d_tab, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice);
+    hipError_t e = ensure_table(P, 1);
     if (e == hipSuccess) e = hipMalloc((void**)&P->d_off, P->h_off.size() * sizeof(int));
     if (e == hipSuccess) e = hipMemcpy(P->d_off, P->h_off.data(), P->h_off.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess && !P->h_obb_off.empty()) {
@@ -780,8 +786,8 @@ ppl_status ppl_plan_create(ppl_robot r, const ppl_plan_desc* d, ppl_plan* out) {
 ppl_status ppl_plan_destroy(ppl_plan P) {
     if (!P) return PPL_OK;
     (void)hipDeviceSynchronize();
-    if (P->counter) (void)hipFree(P->counter);
-    if (P->d_tab) (void)hipFree(P->d_tab);
+    for (int i = 0; i < 4; ++i)
+        if (P->d_tab[i]) (void)hipFree(P->d_tab[i]);
     if (P->d_off) (void)hipFree(P->d_off);
     if (P->d_obb_off) (void)hipFree(P->d_obb_off);
     P->d_self.release(); P->d_cost.release(); P->d_fdcfg.release(); P->d_fdcost.release();
@@ -827,8 +833,6 @@ ppl_status hinge_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_
                       float* d_state, float* d_total, float* d_grad, hipStream_t s) {
     CHECK_ARG(P && prm && d_x, "NULL argument");
     CHECK_ARG(P->kind != PPL_PLAN_EDGES, "hinge cost needs a configuration or trajectory plan");
-    if (d_grad && !P->robot->subtrees_contiguous)
-        return fail(PPL_ERR_UNSUPPORTED, "gradient needs subtrees contiguous in dfs_order");
     DScene S{};
     ppl_status st = plan_scene(P, view, S);
     if (st != PPL_OK) return st;
@@ -846,7 +850,10 @@ ppl_status hinge_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_
     a.S = S;
     a.X = X;
     int grid = 0;
-    a.C = plan_chunks(P, d_grad ? kKCostGrad : kKCost, plan_obb_cap(P, S), &grid);
+    const int G = pick_group(P->n_states);
+    HIP_TRY(ensure_table(P, G));
+    a.C = plan_chunks(P, d_grad ? kKCostGrad : kKCost, plan_obb_cap(P, S), G, &grid);
+    a.group = G;
     a.P = {prm->world_margin, prm->self_margin, prm->weight, prm->use_obb, prm->use_esdf, prm->use_self};
     a.grad = d_grad != nullptr;
     a.self_in = self_in;
@@ -931,7 +938,8 @@ ppl_status fd_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_the
     a.S = S;
     a.X = X;
     int grid = 0;
-    a.C = implicit_chunks(n_cfg, P->counter, kKCost, a.R, S.n_obbs, &grid);
+    a.C = implicit_chunks(n_cfg, kKCost, a.R, S.n_obbs, &grid);
+    a.group = 1;
     a.P = {prm->world_margin, prm->self_margin, 1.0f, prm->use_obb, prm->use_esdf, prm->use_self};
     a.self_in = self_in;
     a.cost_out = P->d_fdcost.as<float>();
@@ -980,7 +988,7 @@ ppl_status ppl_check_collision_batched(ppl_plan P, const ppl_scene_view* view, c
     a.S = S;
     a.X = X;
     int grid = 0;
-    a.C = plan_chunks(P, kKCollide, plan_obb_cap(P, S), &grid);
+    a.C = plan_chunks(P, kKCollide, plan_obb_cap(P, S), 1, &grid);
     a.margin = margin;
     a.mode = 0;
     a.out = d_out;
@@ -1015,7 +1023,7 @@ ppl_status ppl_edge_check_batched(ppl_plan P, const ppl_scene_view* view, const 
     a.S = S;
     a.X = X;
     int grid = 0;
-    a.C = plan_chunks(P, kKCollide, plan_obb_cap(P, S), &grid);
+    a.C = plan_chunks(P, kKCollide, plan_obb_cap(P, S), 1, &grid);
     a.margin = prm->collision_margin;
     a.mode = 1;
     a.skip_resolved = prm->skip_resolved;
@@ -1039,7 +1047,8 @@ ppl_status ppl_min_clearance_batched(ppl_plan P, const ppl_scene_view* view, con
     a.S = S;
     a.X = plan_states(P, d_cfg);
     int grid = 0;
-    a.C = plan_chunks(P, kKCost, plan_obb_cap(P, S), &grid);
+    a.C = plan_chunks(P, kKCost, plan_obb_cap(P, S), 1, &grid);
+    a.group = 1;
     a.P.weight = 1.0f;
     a.clearance_only = 1;
     a.margin = margin;

@@ -2,10 +2,9 @@
 //
 // Execution model (wave64, CDNA4):
 //  * one lane = one robot state (configuration, edge state or waypoint);
-//    a 64-lane workgroup is one wavefront and runs persistently, pulling
-//    64-state chunks from an atomic work counter until the chunk table is
-//    exhausted (chunks never span two problems, so each wave sees one OBB
-//    set at a time);
+//    a 64-lane workgroup is one wavefront and runs persistently over a
+//    static, XCD-aware sweep of 64-state chunks (chunks never span two
+//    problems, so each wave sees one OBB set at a time);
 //  * the problem's enabled OBBs are re-packed into LDS once per problem
 //    switch; every lane then reads the same record (LDS broadcast);
 //  * FK walks the DFS-ordered joint table with wave-uniform (scalar) loads,
@@ -350,32 +349,36 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
     return ref;
 }
 
-// Pull the next chunk (wave-uniform). Returns false when the table is done.
-__device__ __forceinline__ bool next_chunk(const DChunks& C, long n_states, int4& ch) {
-    unsigned c = 0;
-    if (lane_id() == 0) c = atomicAdd(&C.counter[0], 1u);
-    c = uniform(__shfl(c, 0));
-    if ((int)c >= C.n_chunks) return false;
+// Static, XCD-aware chunk schedule (no atomics). Workgroups are dispatched
+// round-robin over the 8 XCDs, so workgroup w runs on XCD w % 8; its logical
+// rank puts every XCD on a contiguous block of chunks in each sweep, which
+// keeps neighbouring states (shared edge endpoints, trajectories, OBB sets)
+// in one L2. Chunk c = rank + k * grid for k = 0, 1, ...: every wave reaches
+// the end of the table, and chunks are visited roughly in table order.
+struct ChunkCursor {
+    int next;
+    int stride;
+};
+
+__device__ __forceinline__ ChunkCursor chunk_cursor() {
+    const int W = gridDim.x, w = blockIdx.x;
+    constexpr int kXcds = 8;
+    const int rank = (W % kXcds == 0) ? (w % kXcds) * (W / kXcds) + w / kXcds : w;
+    return ChunkCursor{rank, W};
+}
+
+__device__ __forceinline__ bool next_chunk(const DChunks& C, long n_states, ChunkCursor& cur, int4& ch) {
+    const int c = cur.next;
+    if (c >= C.n_chunks) return false;
+    cur.next += cur.stride;
     if (C.tab) {
         ch = C.tab[c];
         ch.x = uniform(ch.x); ch.y = uniform(ch.y); ch.z = uniform(ch.z);
     } else {
-        const long b = (long)c * kWave;
-        ch = make_int4(0, (int)b, (int)(b + kWave < n_states ? b + kWave : n_states), 0);
+        const long b = (long)c * C.chunk;
+        ch = make_int4(0, (int)b, (int)(b + C.chunk < n_states ? b + C.chunk : n_states), 0);
     }
     return true;
-}
-
-// Last wave out resets the counters for the next launch on this plan.
-__device__ __forceinline__ void retire(const DChunks& C) {
-    if (lane_id() == 0) {
-        __threadfence();
-        const unsigned done = atomicAdd(&C.counter[1], 1u);
-        if (done == (unsigned)C.n_waves - 1u) {
-            atomicExch(&C.counter[0], 0u);
-            atomicExch(&C.counter[1], 0u);
-        }
-    }
 }
 
 __device__ __forceinline__ void problem_obbs(const DChunks& C, const DScene& S, int prob, int& o0, int& o1) {
@@ -398,7 +401,8 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
     const uint8_t* acm = a.S.sphere_obb_acm;
     int cur_prob = -1, n_obb = 0;
     int4 ch;
-    while (next_chunk(a.C, a.X.n_states, ch)) {
+    ChunkCursor cur = chunk_cursor();
+    while (next_chunk(a.C, a.X.n_states, cur, ch)) {
         if (ch.x != cur_prob) {
             int o0, o1;
             problem_obbs(a.C, a.S, ch.x, o0, o1);
@@ -501,31 +505,104 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
             if (a.mode == 1 && hit) atomicMin((unsigned*)&a.first_hit[ref.unit], (unsigned)ref.k);
         }
     }
-    retire(a.C);
 }
 
 // ===========================================================================
 // Summed-hinge state cost (+ adjoint). chomp.cu:151-188 / stomp.cu:206-246
 // cost = w * (sum_s relu(m - min_o clr(s,o)) + sum_s relu(m - esdf clr(s))
 //             + relu(m_self - min_pairs clr)), raw radii.
-// Gradient: dC/dq_j = z_j . tau_sub(j) - (z_j x o_j) . F_sub(j) with
-// F = sum g_s, tau = sum p_s x g_s over the joint's subtree, built from DFS
-// prefix sums (subtrees are contiguous in DFS order).
+// Gradient: every sphere s with a non-zero point gradient g_s adds, for each
+// ancestor joint j of its link, g_s . (z_j x (p_s - o_j)) (revolute) or
+// g_s . z_j (prismatic), with z_j, o_j the joint's world axis and origin
+// recorded during the forward walk. Lever arms are taken relative to each
+// joint, so no large world-origin moments are summed and subtracted.
 // ===========================================================================
-template <int MAXJ, bool GRAD>
+// Hinge terms of one sphere at world position p: the summed hinge (OBB +
+// ESDF) and its point gradient d hinge / d p in g*; in clearance-only mode the
+// ESDF clearance with radius + margin instead.
+template <bool GRAD>
+__device__ __forceinline__ float sphere_hinge(const CostArgs& a, const float4* __restrict__ s_obb, int n_obb, float px,
+                                              float py, float pz, float r, int sph_id, bool use_obb, bool use_esdf,
+                                              float& gx, float& gy, float& gz) {
+    const float m = a.P.world_margin;
+    float world = 0.0f;
+    gx = gy = gz = 0.0f;
+    if (use_obb) {
+        float best = FLT_MAX;
+        int arg = -1;
+        for (int o = 0; o < n_obb; ++o) {
+            const float4 r0 = s_obb[4 * o], r1 = s_obb[4 * o + 1], r2 = s_obb[4 * o + 2], h = s_obb[4 * o + 3];
+            float lx, ly, lz;
+            to_box(r0, r1, r2, px, py, pz, lx, ly, lz);
+            const float k = box_key(lx, ly, lz, h);
+            if (k < best) { best = k; arg = o; }
+        }
+        const float clr = arg >= 0 ? key_to_sdf(best) - r : 1000.0f;
+        const float hh = m - clr;
+        if (hh > 0.0f) {
+            world += hh;
+            if (GRAD) {
+                float dx, dy, dz;
+                box_sdf_grad(s_obb[4 * arg], s_obb[4 * arg + 1], s_obb[4 * arg + 2], s_obb[4 * arg + 3], px, py, pz, dx, dy, dz);
+                gx -= dx; gy -= dy; gz -= dz;
+            }
+        }
+    }
+    if (use_esdf) {
+        float dx, dy, dz;
+        const float rr = a.clearance_only ? r + a.margin : r;
+        const float c = esdf_clearance(a.S, px, py, pz, rr, sph_id, GRAD, dx, dy, dz);
+        if (a.clearance_only) return c;
+        const float hh = m - c;
+        if (hh > 0.0f) {
+            world += hh;
+            if (GRAD) { gx += dx; gy += dy; gz += dz; }
+        }
+    }
+    return world;
+}
+
+// ===========================================================================
+// Summed-hinge state cost (+ adjoint). chomp.cu:151-188 / stomp.cu:206-246
+// cost = w * (sum_s relu(m - min_o clr(s,o)) + sum_s relu(m - esdf clr(s))
+//             + relu(m_self - min_pairs clr)), raw radii.
+// Gradient: every sphere s with a non-zero point gradient g_s adds, for each
+// ancestor joint j of its link, g_s . (z_j x (p_s - o_j)) (revolute) or
+// g_s . z_j (prismatic), with z_j, o_j the joint's world axis and origin.
+// Lever arms are taken relative to each joint, so no large world-origin
+// moments are summed and subtracted. Only the joints on the current root
+// path are live (one slot per tree depth); a joint's derivative is final and
+// written out when the DFS walk leaves its subtree.
+// ===========================================================================
+// Sum / min over the G lanes of a state group (G a power of two <= 64; the
+// groups are aligned, so xor offsets below G stay inside the group).
+__device__ __forceinline__ float group_sum(float v, int G) {
+    for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ float group_min(float v, int G) {
+    for (int o = 1; o < G; o <<= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+
+template <bool GRAD>
 __global__ __launch_bounds__(64) void k_cost(CostArgs a) {
     extern __shared__ float4 s_lds[];
     float4* s_obb = s_lds;
     float* s_q = reinterpret_cast<float*>(s_lds + 4 * a.C.obb_cap);
     const DRobot& R = a.R;
     const int lane = lane_id();
+    const int G = a.group;                    // lanes per state
+    const int sub = lane & (G - 1);
+    const int per_wave = kWave / G;           // states per chunk
     const int J = R.n_joints, D = R.n_dof;
     const bool use_obb = a.P.use_obb != 0 && !a.clearance_only;
     const bool use_esdf = (a.P.use_esdf != 0 || a.clearance_only) && a.S.n_voxel_layers > 0;
-    const float m = a.P.world_margin;
+    const float wt = a.P.weight;
     int cur_prob = -1, n_obb = 0;
     int4 ch;
-    while (next_chunk(a.C, a.X.n_states, ch)) {
+    ChunkCursor cur = chunk_cursor();
+    while (next_chunk(a.C, a.X.n_states, cur, ch)) {
         if (ch.x != cur_prob) {
             int o0, o1;
             problem_obbs(a.C, a.S, ch.x, o0, o1);
@@ -533,7 +610,8 @@ __global__ __launch_bounds__(64) void k_cost(CostArgs a) {
             n_obb = uniform(stage_obbs(a.S, o0, o1, s_obb, a.C.obb_cap));
             cur_prob = ch.x;
         }
-        const long g = (long)ch.y + lane;
+        (void)per_wave;
+        const long g = (long)ch.y + lane / G;
         const bool active = g < ch.z;
         StateRef ref = load_state(a.X, D, g, active, s_q);
         __syncthreads();
@@ -545,125 +623,105 @@ __global__ __launch_bounds__(64) void k_cost(CostArgs a) {
             self_h = fmaxf(a.P.self_margin - so[0], 0.0f);
             if (self_h > 0.0f) { si = __float_as_int(so[1]); sj = __float_as_int(so[2]); snx = so[3]; sny = so[4]; snz = so[5]; }
         }
-        float world = 0.0f;
-        float minclr = 1000.0f;
-        float F0 = 0.0f, F1 = 0.0f, F2 = 0.0f, t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
-        float U[MAXJ][3], W[MAXJ][3], A[MAXJ];
-        int dofs[MAXJ];
+        // Output addressing of d cost / d q for this lane's state.
+        bool endpoint = false;
+        size_t base = 0, stride = 1;
+        if (a.X.source == kSrcTraj) {
+            endpoint = (ref.k == 0 || ref.k == ref.n - 1);
+            base = (size_t)a.X.traj_off[ref.unit] * D + ref.k;
+            stride = ref.n;
+        } else {
+            base = (size_t)g * D;
+        }
+        const bool write_grad = GRAD && active && !a.clearance_only && sub == 0;
+        float Z[kMaxDepth][3], O[kMaxDepth][3], DQ[kMaxDepth];
+        int kind[kMaxDepth], dofs[kMaxDepth];   // wave-uniform; kind 0 none, 1 revolute, 2 prismatic
+#pragma unroll
+        for (int L = 0; L < kMaxDepth; ++L) { kind[L] = 0; dofs[L] = -1; DQ[L] = 0.0f; }
+        float world = 0.0f, minclr = 1000.0f;
         Slots sl;
+        for (int jj = 0; jj < J; ++jj) {
+            float T[12];
+            int4 meta;
+            fk_joint(R, jj, s_q + lane, sl, T, meta);
+            const int dep = R.depth[jj];
+            if (GRAD) {
+                // Leave the subtrees of every live joint at depth >= dep.
 #pragma unroll
-        for (int jj = 0; jj < MAXJ; ++jj) {
-            if (jj < J) {
-                float T[12];
-                int4 meta;
-                fk_joint(R, jj, s_q + lane, sl, T, meta);
-                dofs[jj] = meta.y;
-                if (GRAD) {
-                    const int type = meta.x;
-                    const int ax = type >= kXRot ? type - kXRot : (type >= 0 ? type : 0);
-                    const float zx = T[ax], zy = T[4 + ax], zz = T[8 + ax];
-                    const float ox = T[3], oy = T[7], oz = T[11];
-                    if (jj > 0 && type >= kXRot) {
-                        U[jj][0] = zx; U[jj][1] = zy; U[jj][2] = zz;
-                        W[jj][0] = zy * oz - zz * oy; W[jj][1] = zz * ox - zx * oz; W[jj][2] = zx * oy - zy * ox;
-                    } else if (jj > 0 && type >= 0) {
-                        U[jj][0] = U[jj][1] = U[jj][2] = 0.0f;
-                        W[jj][0] = -zx; W[jj][1] = -zy; W[jj][2] = -zz;
-                    } else {
-                        U[jj][0] = U[jj][1] = U[jj][2] = 0.0f;
-                        W[jj][0] = W[jj][1] = W[jj][2] = 0.0f;
+                for (int L = 1; L < kMaxDepth; ++L) {
+                    if (L >= dep && kind[L] != 0) {
+                        const float v = group_sum(DQ[L], G);
+                        if (write_grad) a.grad_out[base + (size_t)dofs[L] * stride] = endpoint ? 0.0f : wt * v;
+                        kind[L] = 0;
                     }
-                    // Subtrees that end right before this DFS position.
-                    const unsigned cm = R.close_mask[jj];
-#pragma unroll
-                    for (int p = 0; p < MAXJ; ++p) {
-                        if (p < jj && ((cm >> p) & 1u)) {
-                            A[p] += U[p][0] * t0 + U[p][1] * t1 + U[p][2] * t2 - (W[p][0] * F0 + W[p][1] * F1 + W[p][2] * F2);
-                        }
-                    }
-                    A[jj] = -(U[jj][0] * t0 + U[jj][1] * t1 + U[jj][2] * t2 - (W[jj][0] * F0 + W[jj][1] * F1 + W[jj][2] * F2));
                 }
-                const int s0 = R.sph_off[jj], s1 = R.sph_off[jj + 1];
-                for (int s = s0; s < s1; ++s) {
-                    const float4 sp = R.sph[s];
-                    float px, py, pz;
-                    aff_point(T, sp.x, sp.y, sp.z, px, py, pz);
-                    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-                    if (use_obb) {
-                        float best = FLT_MAX;
-                        int arg = -1;
-                        for (int o = 0; o < n_obb; ++o) {
-                            const float4 r0 = s_obb[4 * o], r1 = s_obb[4 * o + 1], r2 = s_obb[4 * o + 2], h = s_obb[4 * o + 3];
-                            float lx, ly, lz;
-                            to_box(r0, r1, r2, px, py, pz, lx, ly, lz);
-                            const float k = box_key(lx, ly, lz, h);
-                            if (k < best) { best = k; arg = o; }
-                        }
-                        const float clr = arg >= 0 ? key_to_sdf(best) - sp.w : 1000.0f;
-                        const float hh = m - clr;
-                        if (hh > 0.0f) {
-                            world += hh;
-                            if (GRAD) {
-                                float dx, dy, dz;
-                                box_sdf_grad(s_obb[4 * arg], s_obb[4 * arg + 1], s_obb[4 * arg + 2], s_obb[4 * arg + 3], px, py, pz, dx, dy, dz);
-                                gx -= dx; gy -= dy; gz -= dz;
+                const int type = meta.x;
+                const int ax = type >= kXRot ? type - kXRot : (type >= 0 ? type : 0);
+                const int kd = (jj == 0 || meta.y < 0) ? 0 : (type >= kXRot ? 1 : (type >= 0 ? 2 : 0));
+#pragma unroll
+                for (int L = 1; L < kMaxDepth; ++L) {
+                    if (L == dep) {
+                        Z[L][0] = T[ax]; Z[L][1] = T[4 + ax]; Z[L][2] = T[8 + ax];
+                        O[L][0] = T[3]; O[L][1] = T[7]; O[L][2] = T[11];
+                        kind[L] = kd;
+                        dofs[L] = meta.y;
+                        DQ[L] = 0.0f;
+                    }
+                }
+            }
+            const int s0 = R.sph_off[jj], s1 = R.sph_off[jj + 1];
+            for (int s = s0 + sub; s < s1; s += G) {
+                const float4 sp = R.sph[s];
+                float px, py, pz;
+                aff_point(T, sp.x, sp.y, sp.z, px, py, pz);
+                float gx, gy, gz;
+                const float hs = sphere_hinge<GRAD>(a, s_obb, n_obb, px, py, pz, sp.w, R.sph_id[s], use_obb, use_esdf,
+                                                    gx, gy, gz);
+                if (a.clearance_only) minclr = fminf(minclr, hs);
+                else world += hs;
+                if (GRAD) {
+                    if (si >= 0) {
+                        const int id = R.sph_id[s];
+                        if (id == si) { gx -= snx; gy -= sny; gz -= snz; }
+                        if (id == sj) { gx += snx; gy += sny; gz += snz; }
+                    }
+                    const bool nz = (gx != 0.0f) || (gy != 0.0f) || (gz != 0.0f);
+                    if (wave_any(nz && active)) {
+#pragma unroll
+                        for (int L = 1; L < kMaxDepth; ++L) {
+                            if (L <= dep) {
+                                if (kind[L] == 1) {
+                                    const float dx = px - O[L][0], dy = py - O[L][1], dz = pz - O[L][2];
+                                    const float vx = Z[L][1] * dz - Z[L][2] * dy;
+                                    const float vy = Z[L][2] * dx - Z[L][0] * dz;
+                                    const float vz = Z[L][0] * dy - Z[L][1] * dx;
+                                    DQ[L] = fmaf(gx, vx, fmaf(gy, vy, fmaf(gz, vz, DQ[L])));
+                                } else if (kind[L] == 2) {
+                                    DQ[L] = fmaf(gx, Z[L][0], fmaf(gy, Z[L][1], fmaf(gz, Z[L][2], DQ[L])));
+                                }
                             }
                         }
-                    }
-                    if (use_esdf) {
-                        float dx, dy, dz;
-                        const float rr = a.clearance_only ? sp.w + a.margin : sp.w;
-                        const float c = esdf_clearance(a.S, px, py, pz, rr, R.sph_id[s], GRAD, dx, dy, dz);
-                        if (a.clearance_only) {
-                            minclr = fminf(minclr, c);
-                        } else {
-                            const float hh = m - c;
-                            if (hh > 0.0f) {
-                                world += hh;
-                                if (GRAD) { gx += dx; gy += dy; gz += dz; }
-                            }
-                        }
-                    }
-                    if (GRAD) {
-                        if (si >= 0) {
-                            const int id = R.sph_id[s];
-                            if (id == si) { gx -= snx; gy -= sny; gz -= snz; }
-                            if (id == sj) { gx += snx; gy += sny; gz += snz; }
-                        }
-                        F0 += gx; F1 += gy; F2 += gz;
-                        t0 += py * gz - pz * gy; t1 += pz * gx - px * gz; t2 += px * gy - py * gx;
                     }
                 }
             }
         }
-        const float w = a.P.weight;
-        if (active && a.clearance_only) a.cost_out[g] = minclr;
-        if (active && !a.clearance_only && a.cost_out) a.cost_out[g] = w * (world + self_h);
-        if (GRAD && active && !a.clearance_only) {
-            const unsigned cm = R.close_mask[J];
-            bool endpoint = false;
-            size_t base = 0, stride = 1;
-            if (a.X.source == kSrcTraj) {
-                endpoint = (ref.k == 0 || ref.k == ref.n - 1);
-                base = (size_t)a.X.traj_off[ref.unit] * D + ref.k;
-                stride = ref.n;
-            } else {
-                base = (size_t)g * D;
-                stride = 1;
-            }
+        if (GRAD) {
 #pragma unroll
-            for (int p = 0; p < MAXJ; ++p) {
-                if (p < J) {
-                    float v = A[p];
-                    if ((cm >> p) & 1u)
-                        v += U[p][0] * t0 + U[p][1] * t1 + U[p][2] * t2 - (W[p][0] * F0 + W[p][1] * F1 + W[p][2] * F2);
-                    const int d = dofs[p];
-                    if (p > 0 && d >= 0) a.grad_out[base + (size_t)d * stride] = endpoint ? 0.0f : w * v;
+            for (int L = 1; L < kMaxDepth; ++L) {
+                if (kind[L] != 0) {
+                    const float v = group_sum(DQ[L], G);
+                    if (write_grad) a.grad_out[base + (size_t)dofs[L] * stride] = endpoint ? 0.0f : wt * v;
                 }
             }
+        }
+        if (a.clearance_only) {
+            minclr = group_min(minclr, G);
+            if (active && sub == 0) a.cost_out[g] = minclr;
+        } else {
+            world = group_sum(world, G);
+            if (active && sub == 0 && a.cost_out) a.cost_out[g] = wt * (world + self_h);
         }
     }
-    retire(a.C);
 }
 
 // ===========================================================================
@@ -844,14 +902,23 @@ __global__ __launch_bounds__(256) void k_esdf_build(const float* obb_dims, const
 
 int g_cus = 0;
 
-int occupancy_grid(const void* fn, size_t dyn_smem) {
+}  // namespace
+
+int device_cus() {
     if (g_cus == 0) {
         int dev = 0;
         (void)hipGetDevice(&dev);
         hipDeviceProp_t prop;
         (void)hipGetDeviceProperties(&prop, dev);
-        g_cus = prop.multiProcessorCount;
+        g_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     }
+    return g_cus;
+}
+
+namespace {
+
+int occupancy_grid(const void* fn, size_t dyn_smem) {
+    (void)device_cus();
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, dyn_smem) != hipSuccess || per_cu <= 0) per_cu = 4;
     return per_cu * g_cus;
@@ -871,13 +938,12 @@ int grid_collide(size_t lds) {
 }
 
 int grid_cost(int n_joints, bool grad, size_t lds) {
-    static size_t last[4] = {(size_t)-1, (size_t)-1, (size_t)-1, (size_t)-1};
-    static int g[4] = {0, 0, 0, 0};
-    const int idx = (n_joints <= 8 ? 0 : 2) + (grad ? 1 : 0);
+    (void)n_joints;
+    static size_t last[2] = {(size_t)-1, (size_t)-1};
+    static int g[2] = {0, 0};
+    const int idx = grad ? 1 : 0;
     if (lds != last[idx]) {
-        const void* fn = idx == 0 ? (const void*)k_cost<8, false> : idx == 1 ? (const void*)k_cost<8, true>
-                       : idx == 2 ? (const void*)k_cost<16, false> : (const void*)k_cost<16, true>;
-        g[idx] = occupancy_grid(fn, lds);
+        g[idx] = occupancy_grid(grad ? (const void*)k_cost<true> : (const void*)k_cost<false>, lds);
         last[idx] = lds;
     }
     return g[idx];
@@ -890,13 +956,8 @@ hipError_t launch_collide(const CollideArgs& a, int grid, hipStream_t s) {
 
 hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s) {
     const size_t lds = lds_bytes(a.C.obb_cap, a.R.n_dof);
-    if (a.R.n_joints <= 8) {
-        if (a.grad) hipLaunchKernelGGL((k_cost<8, true>), dim3(grid), dim3(64), lds, s, a);
-        else hipLaunchKernelGGL((k_cost<8, false>), dim3(grid), dim3(64), lds, s, a);
-    } else {
-        if (a.grad) hipLaunchKernelGGL((k_cost<16, true>), dim3(grid), dim3(64), lds, s, a);
-        else hipLaunchKernelGGL((k_cost<16, false>), dim3(grid), dim3(64), lds, s, a);
-    }
+    if (a.grad) hipLaunchKernelGGL(k_cost<true>, dim3(grid), dim3(64), lds, s, a);
+    else hipLaunchKernelGGL(k_cost<false>, dim3(grid), dim3(64), lds, s, a);
     return hipGetLastError();
 }
 

@@ -21,6 +21,7 @@
 namespace ppl {
 
 constexpr int kMaxJoints = 16;      // per-lane unrolled joint state (Baxter: 15)
+constexpr int kMaxDepth = 10;      // depth slots of the adjoint's live root path (Fetch uses 9)
 constexpr int kMaxSlots = 2;        // FK transform slots (FK_T_SLOTS in the reference)
 constexpr int kMaxDof = 15;
 constexpr int kWave = 64;
@@ -40,7 +41,7 @@ struct DRobot {
     const int* apx_off;         // [J+1]
     const float4* apx;          // [A] emission order
     const int* apx_id;          // [A]
-    const unsigned* close_mask; // [J+1] bit p set in close_mask[k]: subtree of DFS position p ends before k
+    const int* depth;           // [J] tree depth of DFS position k (root 0)
     const float4* sph_orig;     // [S] spheres by original index
     const int2* pairs;          // [n_pairs] exact self pairs (original indices)
     const int2* apx_pairs;      // [n_apx_pairs] approx self pairs (approx original indices)
@@ -83,11 +84,10 @@ struct DStates {
 
 // Work chunks for persistent launches: a chunk never spans two problems.
 struct DChunks {
-    const int4* tab;            // (problem, begin, end, 0)
+    const int4* tab;            // (problem, begin, end, 0); end - begin <= chunk
     int n_chunks;
+    int chunk;                  // states per chunk (64 / lanes per state)
     const int* obb_off;         // [P+1] OBB ranges per problem; NULL = single problem
-    unsigned* counter;          // [2] work counter + exit counter (self-resetting)
-    int n_waves;                // waves in the grid
     int obb_cap;                // OBB records the LDS staging area holds (>= OBBs of any problem)
 };
 
@@ -110,6 +110,7 @@ struct CollideArgs {
 struct CostArgs {
     DRobot R; DScene S; DStates X; DChunks C;
     DCostParams P;
+    int group;                  // lanes per state (1, 2, 4, 8): a state's spheres are split over them
     int grad;                   // also compute d cost / d q
     int clearance_only;         // write min ESDF clearance (+margin) instead of cost
     float margin;               // clearance_only: radius margin
@@ -137,6 +138,7 @@ hipError_t launch_esdf_build(const float* obb_dims, const float* obb_pose, int n
                              const float* voxel_params, const float* voxel_pose, float* out,
                              hipStream_t s);
 size_t lds_bytes(int obb_cap, int n_dof);
+int device_cus();
 int grid_collide(size_t lds);
 int grid_cost(int n_joints, bool grad, size_t lds);
 
