@@ -517,25 +517,75 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
 // recorded during the forward walk. Lever arms are taken relative to each
 // joint, so no large world-origin moments are summed and subtracted.
 // ===========================================================================
+// Candidate OBBs of one joint: a wave-uniform bitmask (up to kMaxObbLds
+// OBBs) of the OBBs whose distance to the joint's approximate sphere is below
+// the hinge band for at least one lane. The approximate sphere encloses the
+// joint's spheres, so every OBB left out is at clearance >= margin from all
+// of them and cannot change a hinge, its argmin or its gradient.
+constexpr int kMaskWords = kMaxObbLds / 64;
+
+struct ObbMask {
+    unsigned long long w[kMaskWords];
+};
+
+__device__ __forceinline__ void near_obbs(const float4* __restrict__ s_obb, int n_obb, const float* T, const float4& as,
+                                          float band, bool active, int sub, int G, ObbMask& mk) {
+    float ax, ay, az;
+    aff_point(T, as.x, as.y, as.z, ax, ay, az);
+    const float thr = band + as.w;           // sdf(centre) < margin + R  <=>  key < thr^2
+    const float thr2 = thr * thr;
+#pragma unroll
+    for (int k = 0; k < kMaskWords; ++k) mk.w[k] = 0ull;
+    // lanes of a state group split the OBBs; the union is taken over the wave.
+    // rep has bit L set for every lane L = 0 (mod G).
+    const unsigned long long rep = G == 1 ? ~0ull : G == 2 ? 0x5555555555555555ull
+                                 : G == 4 ? 0x1111111111111111ull : 0x0101010101010101ull;
+    for (int base = 0; base < n_obb; base += G) {
+        const int o = base + sub;
+        bool nr = false;
+        if (o < n_obb) {
+            float lx, ly, lz;
+            to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], ax, ay, az, lx, ly, lz);
+            nr = active && box_key(lx, ly, lz, s_obb[4 * o + 3]) < thr2;
+        }
+        const unsigned long long b = __ballot(nr);
+        // lane L of the wave tested OBB base + (L & (G - 1)): fold the groups
+        unsigned long long f = 0ull;
+        for (int q = 0; q < G; ++q)
+            if (b & (rep << q)) f |= 1ull << q;
+#pragma unroll
+        for (int k = 0; k < kMaskWords; ++k) {
+            const int lo = k * 64;
+            if (base >= lo && base < lo + 64) mk.w[k] |= f << (base - lo);
+        }
+    }
+}
+
 // Hinge terms of one sphere at world position p: the summed hinge (OBB +
 // ESDF) and its point gradient d hinge / d p in g*; in clearance-only mode the
-// ESDF clearance with radius + margin instead.
+// ESDF clearance with radius + margin instead. OBBs are taken from `mk`.
 template <bool GRAD>
-__device__ __forceinline__ float sphere_hinge(const CostArgs& a, const float4* __restrict__ s_obb, int n_obb, float px,
-                                              float py, float pz, float r, int sph_id, bool use_obb, bool use_esdf,
-                                              float& gx, float& gy, float& gz) {
+__device__ __forceinline__ float sphere_hinge(const CostArgs& a, const float4* __restrict__ s_obb, const ObbMask& mk,
+                                              float px, float py, float pz, float r, int sph_id, bool use_obb,
+                                              bool use_esdf, float& gx, float& gy, float& gz) {
     const float m = a.P.world_margin;
     float world = 0.0f;
     gx = gy = gz = 0.0f;
     if (use_obb) {
         float best = FLT_MAX;
         int arg = -1;
-        for (int o = 0; o < n_obb; ++o) {
-            const float4 r0 = s_obb[4 * o], r1 = s_obb[4 * o + 1], r2 = s_obb[4 * o + 2], h = s_obb[4 * o + 3];
-            float lx, ly, lz;
-            to_box(r0, r1, r2, px, py, pz, lx, ly, lz);
-            const float k = box_key(lx, ly, lz, h);
-            if (k < best) { best = k; arg = o; }
+#pragma unroll
+        for (int k = 0; k < kMaskWords; ++k) {
+            unsigned long long bits = mk.w[k];
+            while (bits) {
+                const int o = k * 64 + (__ffsll(bits) - 1);
+                bits &= bits - 1ull;
+                const float4 r0 = s_obb[4 * o], r1 = s_obb[4 * o + 1], r2 = s_obb[4 * o + 2], h = s_obb[4 * o + 3];
+                float lx, ly, lz;
+                to_box(r0, r1, r2, px, py, pz, lx, ly, lz);
+                const float key = box_key(lx, ly, lz, h);
+                if (key < best) { best = key; arg = o; }
+            }
         }
         const float clr = arg >= 0 ? key_to_sdf(best) - r : 1000.0f;
         const float hh = m - clr;
@@ -562,18 +612,6 @@ __device__ __forceinline__ float sphere_hinge(const CostArgs& a, const float4* _
     return world;
 }
 
-// ===========================================================================
-// Summed-hinge state cost (+ adjoint). chomp.cu:151-188 / stomp.cu:206-246
-// cost = w * (sum_s relu(m - min_o clr(s,o)) + sum_s relu(m - esdf clr(s))
-//             + relu(m_self - min_pairs clr)), raw radii.
-// Gradient: every sphere s with a non-zero point gradient g_s adds, for each
-// ancestor joint j of its link, g_s . (z_j x (p_s - o_j)) (revolute) or
-// g_s . z_j (prismatic), with z_j, o_j the joint's world axis and origin.
-// Lever arms are taken relative to each joint, so no large world-origin
-// moments are summed and subtracted. Only the joints on the current root
-// path are live (one slot per tree depth); a joint's derivative is final and
-// written out when the DFS walk leaves its subtree.
-// ===========================================================================
 // Sum / min over the G lanes of a state group (G a power of two <= 64; the
 // groups are aligned, so xor offsets below G stay inside the group).
 __device__ __forceinline__ float group_sum(float v, int G) {
@@ -670,12 +708,28 @@ __global__ __launch_bounds__(64) void k_cost(CostArgs a) {
                 }
             }
             const int s0 = R.sph_off[jj], s1 = R.sph_off[jj + 1];
+            if (s1 == s0) continue;
+            ObbMask mk;
+            if (use_obb) {
+                const int a0 = R.apx_off[jj], a1 = R.apx_off[jj + 1];
+                if (a1 - a0 == 1) {
+                    near_obbs(s_obb, n_obb, T, R.apx[a0], a.P.world_margin, active, sub, G, mk);
+                } else {
+                    // no single enclosing sphere for this joint: every OBB is a candidate
+#pragma unroll
+                    for (int k = 0; k < kMaskWords; ++k) {
+                        const int lo = k * 64;
+                        const int n = n_obb - lo;
+                        mk.w[k] = n >= 64 ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
+                    }
+                }
+            }
             for (int s = s0 + sub; s < s1; s += G) {
                 const float4 sp = R.sph[s];
                 float px, py, pz;
                 aff_point(T, sp.x, sp.y, sp.z, px, py, pz);
                 float gx, gy, gz;
-                const float hs = sphere_hinge<GRAD>(a, s_obb, n_obb, px, py, pz, sp.w, R.sph_id[s], use_obb, use_esdf,
+                const float hs = sphere_hinge<GRAD>(a, s_obb, mk, px, py, pz, sp.w, R.sph_id[s], use_obb, use_esdf,
                                                     gx, gy, gz);
                 if (a.clearance_only) minclr = fminf(minclr, hs);
                 else world += hs;
