@@ -13,7 +13,11 @@ fused CHOMP adjoint vs finite-difference gradients, STOMP rollout costs
 through a hipGraph, and the Baxter ESDF adjoint; the FP32 CPU restatement
 timed on this host's cores; parity of the timed outputs vs the FP64 oracle.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-extras]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-extras] [--legs a,b,...] [--no-cpu]
+
+--legs picks the extra measurements (skip, any, edges_plain, chomp, stomp,
+esdf, copy); the headline always runs. --no-cpu drops the CPU baseline leg
+(for profiler runs, whose JSON line then carries cpu_baseline null).
 """
 from __future__ import annotations
 
@@ -33,6 +37,9 @@ sys.path.insert(0, str(ROOT / "oracle"))
 METRIC = "collision-checked configs/s + CHOMP cost+grad traj/s, batched vs plain, 1 GPU"
 PEAK_FP32 = 157.3e12          # MI355X FP32 peak (packed issue)
 PEAK_FP32_UNPACKED = 78.6e12  # without packed FP32 (this build disables packed FP32 ops)
+# Mean gated early-exit flops per edges-workload state, as the CPU leg counts
+# them on its 20,000-state sample; used when that leg is switched off.
+GATED_FLOPS_EDGES = 3191.0
 
 
 def log(msg):
@@ -75,7 +82,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--legs", default="skip,any,edges_plain,chomp,stomp,esdf")
+    ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
+    legs = set(x for x in args.legs.split(",") if x)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the substrate shards by "
               f"independent requests, see DESIGN.md).", file=sys.stderr)
@@ -110,7 +120,10 @@ def main():
     log(f"headline: {wall * 1e3:.3f} ms/step wall, {ev * 1e3:.3f} ms event -> {value:.3e} states/s")
 
     # ------------------------------------------------------------------ CPU baseline leg
-    cpu_baseline, parity, f_state = cpu_baseline_leg(e, fh.cpu().numpy(), args)
+    if args.no_cpu:
+        cpu_baseline, parity, f_state = None, None, GATED_FLOPS_EDGES
+    else:
+        cpu_baseline, parity, f_state = cpu_baseline_leg(e, fh.cpu().numpy(), args)
     achieved = f_state * total_states / ev
     roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP32, "traffic": None,
@@ -120,12 +133,16 @@ def main():
 
     extras = {}
     if not args.no_extras:
-        for name, fn in (("first_hit_skip", lambda: bench_skip(plan, ds, frm, to, fh, total_states, S, torch, args)),
-                         ("any_collision", lambda: bench_any(panda, S, W, torch, dev, args)),
-                         ("edges_plain_vs_batched", lambda: bench_edges_plain(panda, e, ds, S, torch, dev, args)),
-                         ("chomp", lambda: bench_chomp(panda, S, W, torch, dev, args)),
-                         ("stomp", lambda: bench_stomp(S, W, torch, dev, args)),
-                         ("esdf", lambda: bench_esdf(S, W, torch, dev, args))):
+        for key, name, fn in (
+                ("skip", "first_hit_skip", lambda: bench_skip(plan, ds, frm, to, fh, total_states, S, torch, args)),
+                ("any", "any_collision", lambda: bench_any(panda, S, W, torch, dev, args)),
+                ("edges_plain", "edges_plain_vs_batched", lambda: bench_edges_plain(panda, e, ds, S, torch, dev, args)),
+                ("chomp", "chomp", lambda: bench_chomp(panda, S, W, torch, dev, args)),
+                ("stomp", "stomp", lambda: bench_stomp(S, W, torch, dev, args)),
+                ("esdf", "esdf", lambda: bench_esdf(S, W, torch, dev, args)),
+                ("copy", "copy", lambda: bench_copy(torch, dev, args))):
+            if key not in legs:
+                continue
             try:
                 extras[name] = fn()
                 log(f"{name}: {json.dumps(extras[name])}")
@@ -189,6 +206,16 @@ def cpu_baseline_leg(e, fh_gpu, args):
            "sample": f"first {n} states of the edges workload, FP32 two-phase check, median of {reps}",
            "one_core_value": cpu_1, "cpu_model": cpu_model()}
     return cpu, parity, f_state
+
+
+def bench_copy(torch, dev, args):
+    """A plain 256 MiB device-to-device copy: the reference kernel whose
+    FETCH_SIZE / WRITE_SIZE readings scale the HBM counters."""
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    wall, ev = timed_loop(lambda: b.copy_(a), 10, 2, torch)
+    return {"bytes_read": 4 * n, "bytes_written": 4 * n, "GB_per_s": 8 * n / ev / 1e9}
 
 
 def bench_skip(plan, ds, frm, to, fh, total_states, S, torch, args):

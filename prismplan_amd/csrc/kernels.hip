@@ -28,9 +28,9 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
-template <typename T>
-__device__ __forceinline__ T uniform(T v) {
-    return __builtin_amdgcn_readfirstlane(v);
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
 // ---------------------------------------------------------------------------
@@ -315,21 +315,24 @@ struct StateRef {
     int unit;      // config / edge / trajectory index
     int k;         // edge state index or waypoint index
     int n;         // edge states (incl. skipped) or trajectory length
+    long u0;       // plan-order index of the unit's first state
 };
 
 // Writes q_d into s_q[d * 64 + lane] for d < D (inactive lanes write nothing).
 __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, bool active, float* s_q) {
-    StateRef ref{0, 0, 0};
+    StateRef ref{0, 0, 0, 0};
     const int lane = lane_id();
     if (!active) return ref;
     if (X.source == kSrcConfig) {
         ref.unit = (int)g;
+        ref.u0 = g;
         const float* q = X.cfg + (size_t)g * X.cfg_stride;
         for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = q[d];
     } else if (X.source == kSrcEdge) {
         const int e = upper_bound_idx(X.edge_off, X.n_edges, g);
         const int cnt = X.edge_off[e + 1] - X.edge_off[e];
         ref.unit = e;
+        ref.u0 = X.edge_off[e];
         ref.k = (int)(g - X.edge_off[e]) + X.k0;
         ref.n = cnt + X.k0;
         const float t = ref.n > 1 ? (float)ref.k / (float)(ref.n - 1) : 0.0f;
@@ -341,6 +344,7 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         const int t0 = X.traj_off[b];
         const int Tn = X.traj_off[b + 1] - t0;
         ref.unit = b;
+        ref.u0 = t0;
         ref.k = (int)(g - t0);
         ref.n = Tn;
         const float* th = X.theta + (size_t)t0 * D;
@@ -384,6 +388,51 @@ __device__ __forceinline__ bool next_chunk(const DChunks& C, long n_states, Chun
 __device__ __forceinline__ void problem_obbs(const DChunks& C, const DScene& S, int prob, int& o0, int& o1) {
     if (C.obb_off) { o0 = C.obb_off[prob]; o1 = C.obb_off[prob + 1]; }
     else { o0 = 0; o1 = S.n_obbs; }
+}
+
+// Wave-uniform OBB candidate set: bit o set for OBB o (up to kMaxObbLds).
+constexpr int kMaskWords = kMaxObbLds / 64;
+
+struct ObbMask {
+    unsigned long long w[kMaskWords];
+};
+
+__device__ __forceinline__ void mask_all(int n_obb, ObbMask& mk) {
+#pragma unroll
+    for (int k = 0; k < kMaskWords; ++k) {
+        const int n = n_obb - k * 64;
+        mk.w[k] = n >= 64 ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
+    }
+}
+
+__device__ __forceinline__ float read_lane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// OBBs within `reach` of the point (cx, cy, cz) (wave-uniform inputs): lane o
+// of the wave tests OBB base + o, one pass per 64 OBBs.
+__device__ __forceinline__ void reach_obbs(const float4* __restrict__ s_obb, int n_obb, float cx, float cy, float cz,
+                                           float reach, ObbMask& mk) {
+    const float r2 = reach * reach;
+#pragma unroll
+    for (int k = 0; k < kMaskWords; ++k) {
+        const int o = k * 64 + lane_id();
+        bool c = false;
+        if (k * 64 < n_obb) {
+            if (o < n_obb) {
+                float lx, ly, lz;
+                to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], cx, cy, cz, lx, ly, lz);
+                c = box_hit(lx, ly, lz, s_obb[4 * o + 3], r2);
+            }
+        }
+        mk.w[k] = __ballot(c);
+    }
 }
 
 // ===========================================================================
@@ -430,54 +479,85 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
             if (s1 == s0 || a1 == a0) continue;
             // Approximate spheres of this joint in world coordinates (usually one).
             float apx[3], apr2;
+            const float4 as0 = R.apx[a0];
+            aff_point(T, as0.x, as0.y, as0.z, apx[0], apx[1], apx[2]);
             {
-                const float4 as = R.apx[a0];
-                aff_point(T, as.x, as.y, as.z, apx[0], apx[1], apx[2]);
-                const float rr = as.w + a.margin;
+                const float rr = as0.w + a.margin;
                 apr2 = rr * rr;
             }
+            // Broad phase: the OBBs some active lane's approximate sphere can
+            // reach. The active lanes' centres lie within rw of the first
+            // active lane's centre c0, so every OBB an approximate sphere
+            // overlaps lies within rw + r of c0 (slack covers rounding).
+            const bool broad = a1 - a0 == 1;
+            float cx = 0.0f, cy = 0.0f, cz = 0.0f, reach2 = 0.0f;
+            if (broad) {
+                const int l0 = __ffsll(__ballot(active)) - 1;
+                cx = read_lane(apx[0], l0); cy = read_lane(apx[1], l0); cz = read_lane(apx[2], l0);
+                const float dx = apx[0] - cx, dy = apx[1] - cy, dz = apx[2] - cz;
+                const float d2 = wave_max(active ? fmaf(dz, dz, fmaf(dy, dy, dx * dx)) : 0.0f);
+                const float reach = sqrtf(d2) * 1.0001f + fabsf(as0.w + a.margin) + 1e-5f;
+                reach2 = uniform(reach * reach);
+            }
             bool flag = false;
-            for (int o = 0; o < n_obb; ++o) {
-                const float4 r0 = s_obb[4 * o], r1 = s_obb[4 * o + 1], r2 = s_obb[4 * o + 2], h = s_obb[4 * o + 3];
-                float lx, ly, lz;
-                to_box(r0, r1, r2, apx[0], apx[1], apx[2], lx, ly, lz);
-                bool ah = box_hit(lx, ly, lz, h, apr2);
-                for (int ai = a0 + 1; ai < a1; ++ai) {
-                    const float4 as = R.apx[ai];
-                    float x, y, z;
-                    aff_point(T, as.x, as.y, as.z, x, y, z);
-                    to_box(r0, r1, r2, x, y, z, lx, ly, lz);
-                    const float rr = as.w + a.margin;
-                    ah = ah || box_hit(lx, ly, lz, h, rr * rr);
-                }
-                flag = flag || ah;
-                if (!wave_any(ah && active)) continue;
-                // M = R_o * T_j: sphere-local -> box-local in one affine map.
-                float M[12];
-#pragma unroll
-                for (int rr = 0; rr < 3; ++rr) {
-                    const float4 row = rr == 0 ? r0 : (rr == 1 ? r1 : r2);
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        float v = fmaf(row.z, T[8 + c], fmaf(row.y, T[4 + c], row.x * T[c]));
-                        if (c == 3) v += row.w;
-                        M[rr * 4 + c] = v;
+            bool done = false;
+            for (int base = 0; base < n_obb && !done; base += kWave) {
+                // candidates of this block of 64 OBBs: lane i tests OBB base + i
+                unsigned long long bits;
+                {
+                    const int o = base + lane;
+                    bool c = o < n_obb;
+                    if (broad && c) {
+                        float lx, ly, lz;
+                        to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], cx, cy, cz, lx, ly, lz);
+                        c = box_hit(lx, ly, lz, s_obb[4 * o + 3], reach2);
                     }
+                    bits = __ballot(c);
                 }
-                const int o_orig = __float_as_int(h.w);
-                bool h_any = false;
-                for (int s = s0; s < s1; ++s) {
-                    const float4 sp = R.sph[s];
-                    float x, y, z;
-                    aff_point(M, sp.x, sp.y, sp.z, x, y, z);
-                    const float rr = sp.w + a.margin;
-                    bool hs = box_hit(x, y, z, h, rr * rr);
-                    if (acm) hs = hs && acm[(size_t)R.sph_id[s] * a.S.n_obbs + o_orig] != 0;
-                    h_any = h_any || hs;
+                while (bits && !done) {
+                    const int o = base + (__ffsll(bits) - 1);
+                    bits &= bits - 1ull;
+                    const float4 r0 = s_obb[4 * o], r1 = s_obb[4 * o + 1], r2 = s_obb[4 * o + 2], h = s_obb[4 * o + 3];
+                    float lx, ly, lz;
+                    to_box(r0, r1, r2, apx[0], apx[1], apx[2], lx, ly, lz);
+                    bool ah = box_hit(lx, ly, lz, h, apr2);
+                    for (int ai = a0 + 1; ai < a1; ++ai) {
+                        const float4 as = R.apx[ai];
+                        float x, y, z;
+                        aff_point(T, as.x, as.y, as.z, x, y, z);
+                        to_box(r0, r1, r2, x, y, z, lx, ly, lz);
+                        const float rr = as.w + a.margin;
+                        ah = ah || box_hit(lx, ly, lz, h, rr * rr);
+                    }
+                    flag = flag || ah;
+                    if (!wave_any(ah && active)) continue;
+                    // M = R_o * T_j: sphere-local -> box-local in one affine map.
+                    float M[12];
+#pragma unroll
+                    for (int rr = 0; rr < 3; ++rr) {
+                        const float4 row = rr == 0 ? r0 : (rr == 1 ? r1 : r2);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            float v = fmaf(row.z, T[8 + c], fmaf(row.y, T[4 + c], row.x * T[c]));
+                            if (c == 3) v += row.w;
+                            M[rr * 4 + c] = v;
+                        }
+                    }
+                    const int o_orig = __float_as_int(h.w);
+                    bool h_any = false;
+                    for (int s = s0; s < s1; ++s) {
+                        const float4 sp = R.sph[s];
+                        float x, y, z;
+                        aff_point(M, sp.x, sp.y, sp.z, x, y, z);
+                        const float rr = sp.w + a.margin;
+                        bool hs = box_hit(x, y, z, h, rr * rr);
+                        if (acm) hs = hs && acm[(size_t)R.sph_id[s] * a.S.n_obbs + o_orig] != 0;
+                        h_any = h_any || hs;
+                    }
+                    hit = hit || (active && ah && h_any);
+                    active = active && !hit;
+                    done = !wave_any(active);
                 }
-                hit = hit || (active && ah && h_any);
-                active = active && !hit;
-                if (!wave_any(active)) break;
             }
             if (has_esdf) {
                 // Approx ESDF pass (ACM disabled, scene_collision.cuh:574-582),
@@ -500,9 +580,17 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
                 }
             }
         }
-        if (g < ch.z) {
-            if (a.out) a.out[g] = hit ? 1 : 0;
-            if (a.mode == 1 && hit) atomicMin((unsigned*)&a.first_hit[ref.unit], (unsigned)ref.k);
+        if (g < ch.z && a.out) a.out[g] = hit ? 1 : 0;
+        if (a.mode == 1) {
+            // One atomic per (chunk, edge): a hit lane reports only when no
+            // lower lane of the same edge hit (lanes hold consecutive states).
+            const bool rep = hit && g < ch.z;
+            const unsigned long long hm = __ballot(rep);
+            if (rep) {
+                const int l_e0 = ref.u0 > ch.y ? (int)(ref.u0 - ch.y) : 0;
+                const unsigned long long same = ((1ull << lane) - 1ull) & ~((1ull << l_e0) - 1ull);
+                if ((hm & same) == 0ull) atomicMin((unsigned*)&a.first_hit[ref.unit], (unsigned)ref.k);
+            }
         }
     }
 }
@@ -522,12 +610,6 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
 // the hinge band for at least one lane. The approximate sphere encloses the
 // joint's spheres, so every OBB left out is at clearance >= margin from all
 // of them and cannot change a hinge, its argmin or its gradient.
-constexpr int kMaskWords = kMaxObbLds / 64;
-
-struct ObbMask {
-    unsigned long long w[kMaskWords];
-};
-
 __device__ __forceinline__ void near_obbs(const float4* __restrict__ s_obb, int n_obb, const float* T, const float4& as,
                                           float band, bool active, int sub, int G, ObbMask& mk) {
     float ax, ay, az;
@@ -715,13 +797,7 @@ __global__ __launch_bounds__(64) void k_cost(CostArgs a) {
                 if (a1 - a0 == 1) {
                     near_obbs(s_obb, n_obb, T, R.apx[a0], a.P.world_margin, active, sub, G, mk);
                 } else {
-                    // no single enclosing sphere for this joint: every OBB is a candidate
-#pragma unroll
-                    for (int k = 0; k < kMaskWords; ++k) {
-                        const int lo = k * 64;
-                        const int n = n_obb - lo;
-                        mk.w[k] = n >= 64 ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
-                    }
+                    mask_all(n_obb, mk);   // no single enclosing sphere: every OBB is a candidate
                 }
             }
             for (int s = s0 + sub; s < s1; s += G) {
