@@ -125,11 +125,14 @@ def main():
     else:
         cpu_baseline, parity, f_state = cpu_baseline_leg(e, fh.cpu().numpy(), args)
     achieved = f_state * total_states / ev
+    from prismplan_amd import _ppl
+    f_exh = panda.flops(_ppl.WORK_ANY_COLLISION, n_obbs=e.scene.n_obbs)
     roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP32, "traffic": None,
                 "frac_of_unpacked_peak": achieved / PEAK_FP32_UNPACKED,
                 "flops_per_state": f_state, "flops_rule": "gated early-exit count (oracle, 20k-state sample)",
-                "exhaustive_flops_per_state": 114923, "kernel_ms": ev * 1e3}
+                "exhaustive_flops_per_state": f_exh,
+                "exhaustive_frac": f_exh * total_states / ev / PEAK_FP32, "kernel_ms": ev * 1e3}
 
     extras = {}
     if not args.no_extras:
@@ -301,14 +304,18 @@ def bench_chomp(panda, S, W, torch, dev, args):
     plan = S.Plan.trajectories(panda, w.lengths())
     th = torch.from_numpy(w.theta).to(dev)
     prm = S.cost_params(world_margin=0.05, weight=1.0, use_self=False)
+    tuned = plan.autotune_cost(ds, th, prm, grad=True)
     wall_a, ev_a = timed_loop(lambda: S.hinge_cost(plan, ds, th, prm, grad=True), args.steps, args.warmup, torch)
     wall_f, ev_f = timed_loop(lambda: S.chomp_fd_grad(plan, ds, th, prm, eps=1e-3), max(5, args.steps // 5), 2, torch)
-    B = w.n_traj
-    f_adj = 5.79e6
+    from prismplan_amd import _ppl
+    B, T, O = w.n_traj, int(w.lengths()[0]), w.scene.n_obbs
+    f_adj = panda.flops(_ppl.WORK_CHOMP_ADJOINT, n_obbs=O, n_waypoints=T)
+    f_fd = panda.flops(_ppl.WORK_CHOMP_FD, n_obbs=O, n_waypoints=T)
     return {"adjoint_traj_per_s": B / wall_a, "fd_traj_per_s": B / wall_f, "adjoint_over_fd": wall_f / wall_a,
             "adjoint_ms": wall_a * 1e3, "fd_ms": wall_f * 1e3,
+            "adjoint_flops_per_traj": f_adj, "fd_flops_per_traj": f_fd, "adjoint_lanes_per_state": tuned[0],
             "adjoint_roofline_frac": B * f_adj / ev_a / PEAK_FP32,
-            "fd_roofline_frac": B * 80.9e6 / ev_f / PEAK_FP32}
+            "fd_roofline_frac": B * f_fd / ev_f / PEAK_FP32}
 
 
 def bench_stomp(S, W, torch, dev, args):
@@ -317,19 +324,26 @@ def bench_stomp(S, W, torch, dev, args):
     fetch = S.Robot("fetch")
     R = wk.n_rollouts
     plan = S.Plan.trajectories(fetch, np.repeat(wk.lengths, R), problem_units=np.arange(wk.n_problems + 1) * R,
-                               obb_offsets=wk.obb_offsets)
+                               obb_offsets=wk.obb_offsets, waypoint_major=True)
     ds = S.DeviceScene.from_host(wk.scene)
     ro = torch.from_numpy(wk.rollouts).to(dev)
     prm = S.cost_params(world_margin=0.02, self_margin=0.01, weight=20.0)
     state = torch.empty(wk.n_states, device=dev)
     total = torch.empty(wk.n_problems * R, device=dev)
+    tuned = plan.autotune_cost(ds, ro, prm, grad=False)
     g = S.capture_stomp(plan, ds, ro, prm, 0.1, 0.1, state, total)
     wall, ev = timed_loop(lambda: g.launch(), args.steps, args.warmup, torch)
     wall_e, _ = timed_loop(lambda: S.stomp_rollout_costs(plan, ds, ro, prm, 0.1, 0.1), args.steps, args.warmup, torch)
-    f_state = 251199.0
+    from prismplan_amd import _ppl
+    # exhaustive flops per rollout state at each problem's own OBB count, + control
+    n_obb = np.diff(wk.obb_offsets)
+    states_p = wk.lengths * R
+    flops = sum(float(sp) * (fetch.flops(_ppl.WORK_STOMP_STATE, n_obbs=int(o), use_self=True) + 18.0 * fetch.n_dof)
+                for sp, o in zip(states_p, n_obb))
     return {"rollout_states_per_s": wk.n_states / wall, "rollouts_per_s": wk.n_problems * R / wall,
             "graph_ms": wall * 1e3, "eager_ms": wall_e * 1e3, "states": wk.n_states,
-            "roofline_frac_exhaustive": wk.n_states * f_state / ev / PEAK_FP32}
+            "flops_per_state_mean": flops / wk.n_states, "lanes_per_state": tuned[0],
+            "roofline_frac_exhaustive": flops / ev / PEAK_FP32}
 
 
 def bench_esdf(S, W, torch, dev, args):
@@ -344,10 +358,14 @@ def bench_esdf(S, W, torch, dev, args):
     plan = S.Plan.trajectories(baxter, wk.lengths())
     th = torch.from_numpy(wk.theta).to(dev)
     prm = S.cost_params(world_margin=0.02, weight=1.0, use_obb=False, use_self=False)
+    tuned = plan.autotune_cost(ds, th, prm, grad=True)
     wall, ev = timed_loop(lambda: S.hinge_cost(plan, ds, th, prm, grad=True), args.steps, args.warmup, torch)
+    from prismplan_amd import _ppl
+    f_wp = baxter.flops(_ppl.WORK_ESDF_WAYPOINT)
     return {"waypoints_per_s": wk.n_waypoints / wall, "trajectories_per_s": wk.n_traj / wall,
             "adjoint_ms": wall * 1e3, "esdf_build_ms": wall_b * 1e3, "waypoints": wk.n_waypoints,
-            "roofline_frac": wk.n_waypoints * (8137 + 5487) / ev / PEAK_FP32}
+            "flops_per_waypoint": f_wp, "lanes_per_state": tuned[0],
+            "roofline_frac": wk.n_waypoints * f_wp / ev / PEAK_FP32}
 
 
 if __name__ == "__main__":

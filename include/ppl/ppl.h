@@ -83,6 +83,28 @@ ppl_status ppl_robot_get_info(ppl_robot robot, ppl_robot_info* info);
 /* RobotModelBuilder::update_joint_limits (python/bindings.cpp:180-206). */
 ppl_status ppl_robot_update_joint_limits(ppl_robot robot, const float* h_lower, const float* h_upper);
 
+/* Device-resident robot tables for planner kernels built on ppl/ppl_device.hpp:
+ * the counterpart of the RobotModel struct the reference passes by value to
+ * its kernels (planning/robot_model.cuh:27-82). Tables are in the tree's DFS
+ * order; the pointers stay valid until ppl_robot_destroy. */
+typedef struct {
+    int n_dof, n_joints, n_spheres, n_approx_spheres, n_pairs, n_approx_pairs;
+    const float* ft;          /* [n_joints][12] rows 0..2 of each fixed transform */
+    const int* jmeta;         /* [n_joints][4] joint type, dof (-1 none), FK slot, parent's FK slot */
+    const int* sph_off;       /* [n_joints+1] spheres of DFS joint k: sph[sph_off[k] .. sph_off[k+1]) */
+    const float* sph;         /* [n_spheres][4] x, y, z, r in the link frame, grouped by DFS joint */
+    const int* sph_id;        /* [n_spheres] original sphere index of each sph entry */
+    const int* apx_off;       /* [n_joints+1] the same for the approximate spheres */
+    const float* apx;         /* [n_approx_spheres][4] */
+    const int* apx_id;        /* [n_approx_spheres] */
+    const float* sph_orig;    /* [n_spheres][4] by original index */
+    const float* apx_orig;    /* [n_approx_spheres][4] by original index */
+    const int* pairs;         /* [n_pairs][2] self pairs expanded from self_cc_ranges (original indices) */
+    const int* apx_pairs;     /* [n_approx_pairs][2] */
+} ppl_robot_device_view;
+
+ppl_status ppl_robot_get_device_view(ppl_robot robot, ppl_robot_device_view* out);
+
 /* ------------------------------------------------------------------------
  * Scene view: non-owning DEVICE pointers with the field meanings of
  * SceneCollisionData (collision/scene_collision_data.hh:10-47).
@@ -178,12 +200,25 @@ typedef struct {
     int n_problems;           /* >= 1 */
     const int* problem_units; /* host [n_problems+1] unit offsets; NULL = one problem */
     const int* obb_offsets;   /* host [n_problems+1] OBB ranges; NULL = all OBBs for every problem */
+    int waypoint_major;       /* trajectory plans: 1 = work is cut waypoint-major inside each problem
+                                 (waypoint t of every trajectory, then t+1), which keeps the states a
+                                 wave holds together in space for rollout sets such as STOMP's; the
+                                 trajectories of each problem must then have equal length. Results
+                                 and layouts are the same either way. */
 } ppl_plan_desc;
 
 ppl_status ppl_plan_create(ppl_robot robot, const ppl_plan_desc* desc, ppl_plan* out);
 ppl_status ppl_plan_destroy(ppl_plan plan);
 /* Total states of the plan (configs, checked edge states or waypoints). */
 long ppl_plan_states(ppl_plan plan);
+
+/* Forward kinematics outputs of every configuration of a configuration plan
+ * (either output nullable): d_frames [n][n_joints][16], each joint's world
+ * frame column-major (out[c*4 + r] = T[r][c]) in original joint order, as
+ * fk_joint_transforms_runtime emits (planning/runtime_kinematics.cuh:97-171);
+ * d_spheres [n][n_spheres][4] world centre and radius in original sphere
+ * order (fk_runtime, planning/runtime_kinematics.cuh:16-88). */
+ppl_status ppl_fk_batched(ppl_plan plan, const float* d_configs, float* d_frames, float* d_spheres, ppl_stream stream);
 
 /* Any-collision of every configuration, d_configs [n*n_dof] row-major.
  * check_self adds the self check (two_phase_cc check_self, MIT* batch_cc). */
@@ -217,6 +252,17 @@ ppl_status ppl_hinge_cost_batched(ppl_plan plan, const ppl_scene_view* scene, co
                                   const ppl_cost_params* params, float* d_state_cost, float* d_total, float* d_grad,
                                   ppl_stream stream);
 
+/* Autotuner of the cost kernel's tile shape: times the plan's summed-hinge
+ * evaluation (with the fused adjoint if grad) on the caller's own inputs for
+ * every lane-group size G = 1, 2, 4, ..., 16 lanes per state (64 / G states
+ * per wavefront, a state's spheres split over G lanes) that fits in LDS, and
+ * keeps the fastest for this plan's later hinge / STOMP calls. reps timed
+ * runs per candidate on `stream`; outputs go to plan scratch. No counterpart
+ * in the reference. */
+ppl_status ppl_plan_autotune_cost(ppl_plan plan, const ppl_scene_view* scene, const float* d_x,
+                                  const ppl_cost_params* params, int grad, int reps, int* best_group,
+                                  float* best_ms, ppl_stream stream);
+
 /* Central-FD obstacle gradient over a trajectory plan (chomp_obs_grad_kernel
  * semantics), the baseline for the fused adjoint. */
 ppl_status ppl_chomp_fd_grad_batched(ppl_plan plan, const ppl_scene_view* scene, const float* d_theta,
@@ -249,6 +295,21 @@ ppl_status ppl_graph_capture_hinge(ppl_plan plan, const ppl_scene_view* scene, c
                                    float* d_grad, ppl_stream stream, ppl_graph* out);
 ppl_status ppl_graph_launch(ppl_graph graph, ppl_stream stream);
 ppl_status ppl_graph_destroy(ppl_graph graph);
+
+/* ------------------------------------------------------------------------
+ * Algorithmic flop counts (ppl/roofline.h) of one work unit for this robot:
+ * the numerators of the benchmark's roofline fractions. No counterpart in
+ * the reference. Returns a negative value for a bad handle or kind.
+ * ------------------------------------------------------------------------ */
+typedef enum {
+    PPL_WORK_ANY_COLLISION = 0,   /* one state, exhaustive, n_obbs OBBs */
+    PPL_WORK_HINGE_STATE = 1,     /* one state, OBB hinge (+ self min-hinge when the robot has pairs and use_self) */
+    PPL_WORK_CHOMP_ADJOINT = 2,   /* one trajectory of n_waypoints: cost + fused-adjoint gradient */
+    PPL_WORK_CHOMP_FD = 3,        /* one trajectory of n_waypoints: central-FD gradient */
+    PPL_WORK_STOMP_STATE = 4,     /* one rollout waypoint: world + self hinge */
+    PPL_WORK_ESDF_WAYPOINT = 5    /* one waypoint: trilinear ESDF hinge forward + backward */
+} ppl_work_kind;
+double ppl_algorithmic_flops(ppl_robot robot, int kind, int n_obbs, int n_waypoints, int use_self);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,14 @@ class RobotInfo(C.Structure):
                 ("n_self_pairs", C.c_int), ("n_t_slots", C.c_int)]
 
 
+class RobotDeviceView(C.Structure):
+    _fields_ = [("n_dof", C.c_int), ("n_joints", C.c_int), ("n_spheres", C.c_int), ("n_approx_spheres", C.c_int),
+                ("n_pairs", C.c_int), ("n_approx_pairs", C.c_int),
+                ("ft", _vp), ("jmeta", _vp), ("sph_off", _vp), ("sph", _vp), ("sph_id", _vp), ("apx_off", _vp),
+                ("apx", _vp), ("apx_id", _vp), ("sph_orig", _vp), ("apx_orig", _vp), ("pairs", _vp),
+                ("apx_pairs", _vp)]
+
+
 class SceneView(C.Structure):
     _fields_ = [("obb_dims", _vp), ("obb_pose", _vp), ("obb_enable", _vp), ("n_obbs", C.c_int),
                 ("voxel_params", _vp), ("voxel_pose", _vp), ("voxel_enable", _vp), ("voxel_features", _vp),
@@ -67,7 +75,8 @@ class CostParams(C.Structure):
 
 class PlanDesc(C.Structure):
     _fields_ = [("kind", C.c_int), ("n_units", C.c_int), ("unit_states", _i32p), ("skip_first", C.c_int),
-                ("n_problems", C.c_int), ("problem_units", _i32p), ("obb_offsets", _i32p)]
+                ("n_problems", C.c_int), ("problem_units", _i32p), ("obb_offsets", _i32p),
+                ("waypoint_major", C.c_int)]
 
 
 class EdgeBatchParams(C.Structure):
@@ -82,6 +91,7 @@ SIGNATURES = [
     ("ppl_robot_destroy", C.c_int, [_vp]),
     ("ppl_robot_get_info", C.c_int, [_vp, C.POINTER(RobotInfo)]),
     ("ppl_robot_update_joint_limits", C.c_int, [_vp, _f32p, _f32p]),
+    ("ppl_robot_get_device_view", C.c_int, [_vp, C.POINTER(RobotDeviceView)]),
     ("ppl_check_collision_scene", C.c_int, [_vp, C.POINTER(SceneView), _f32p, C.c_int, C.c_float, _u8p]),
     ("ppl_edge_eval_create", C.c_int, [_vp, C.POINTER(SceneView), C.c_int, C.c_int, C.POINTER(_vp)]),
     ("ppl_edge_eval_destroy", C.c_int, [_vp]),
@@ -93,11 +103,14 @@ SIGNATURES = [
     ("ppl_plan_create", C.c_int, [_vp, C.POINTER(PlanDesc), C.POINTER(_vp)]),
     ("ppl_plan_destroy", C.c_int, [_vp]),
     ("ppl_plan_states", C.c_long, [_vp]),
+    ("ppl_fk_batched", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("ppl_check_collision_batched", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.c_float, C.c_int, _vp, _vp]),
     ("ppl_edge_check_batched", C.c_int, [_vp, C.POINTER(SceneView), _vp, _vp, C.POINTER(EdgeBatchParams), _vp, _vp,
                                          _vp]),
     ("ppl_min_clearance_batched", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.c_float, _vp, _vp]),
     ("ppl_hinge_cost_batched", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), _vp, _vp, _vp, _vp]),
+    ("ppl_plan_autotune_cost", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), C.c_int, C.c_int,
+                                         _i32p, _f32p, _vp]),
     ("ppl_chomp_fd_grad_batched", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), C.c_float, _vp,
                                             _vp]),
     ("ppl_stomp_rollout_costs", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), C.c_float, C.c_float,
@@ -109,7 +122,9 @@ SIGNATURES = [
                                           C.POINTER(_vp)]),
     ("ppl_graph_launch", C.c_int, [_vp, _vp]),
     ("ppl_graph_destroy", C.c_int, [_vp]),
+    ("ppl_algorithmic_flops", C.c_double, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
 ]
+WORK_ANY_COLLISION, WORK_HINGE_STATE, WORK_CHOMP_ADJOINT, WORK_CHOMP_FD, WORK_STOMP_STATE, WORK_ESDF_WAYPOINT = range(6)
 
 _lib = None
 

@@ -36,6 +36,23 @@ def flags() -> list:
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
 
 
+def build_prrtc(verbose: bool = True) -> Path:
+    """The pybind11 module (host C++ over the C ABI), linked to libppl.so."""
+    import sysconfig
+    import pybind11
+    out = LIB_DIR / ("_prrtc" + sysconfig.get_config_var("EXT_SUFFIX"))
+    tmp = out.with_suffix(".tmp")
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-shared", "-fPIC", "-fvisibility=hidden",
+           "-I", str(ROOT / "include"), "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
+           str(CSRC / "prrtc_module.cpp"), "-L", str(LIB_DIR), "-lppl", "-Wl,-rpath,$ORIGIN",
+           "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(tmp)]
+    if verbose:
+        print("[build]", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, out)
+    return out
+
+
 def build(verbose: bool = True) -> Path:
     LIB_DIR.mkdir(exist_ok=True)
     objs = []
@@ -55,6 +72,7 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     for o in objs:
         Path(o).unlink(missing_ok=True)
+    build_prrtc(verbose)
     return LIB
 
 

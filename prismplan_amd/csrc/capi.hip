@@ -3,6 +3,7 @@
 // points. No exceptions or exits cross the ABI; every failure returns a
 // ppl_status and leaves a message for ppl_last_error().
 #include "ppl/ppl.h"
+#include "ppl/roofline.h"
 #include "ppl_internal.hpp"
 
 #include <algorithm>
@@ -93,7 +94,7 @@ struct ppl_robot_s {
     void* blob = nullptr;
     int n_slots = 1;
     int n_pairs = 0;
-    bool subtrees_contiguous = true;
+    int n_revolute = 0, n_prismatic = 0;   // joints that move with a dof
     std::vector<float> lower, upper;
     hipStream_t stream = nullptr;
     DBuf d_cfg, d_out, d_off, d_aux, d_self;
@@ -120,10 +121,13 @@ struct ppl_plan_s {
     std::vector<int> h_off;          // edges / trajectories: state offsets [n_units+1]
     int* d_off = nullptr;            // [n_units+1]
     std::vector<long> pb, pe;        // state range of each problem
-    int4* d_tab[4] = {nullptr, nullptr, nullptr, nullptr};   // chunk tables for 64, 32, 16, 8 states
-    int n_chunks[4] = {0, 0, 0, 0};
+    int4* d_tab[kGroupKinds] = {};   // chunk tables for 64, 32, 16, 8, 4, 2 states
+    int n_chunks[kGroupKinds] = {};
     int* d_obb_off = nullptr;
-    DBuf d_self, d_cost, d_fdcfg, d_fdcost;
+    int wmajor = 0;                  // waypoint-major trajectory plan
+    int tuned_group[2] = {0, 0};     // cost kernel lanes per state chosen by ppl_plan_autotune_cost (0: heuristic)
+    int4* d_prob = nullptr;          // wmajor: per problem (first trajectory, trajectories, length, first state)
+    DBuf d_self, d_cost, d_fdcfg, d_fdcost, d_tune;
 };
 
 struct ppl_graph_s {
@@ -167,13 +171,14 @@ ppl_status check_obb_capacity(int n) {
 enum KernelKind { kKCollide, kKCost, kKCostGrad };
 
 // Persistent grid for a kernel family at the LDS footprint of this launch.
-int grid_for(KernelKind k, const DRobot& R, int obb_cap) {
-    const size_t lds = lds_bytes(obb_cap, R.n_dof);
-    return k == kKCollide ? grid_collide(lds) : grid_cost(R.n_joints, k == kKCostGrad, lds);
+int grid_for(KernelKind k, const DRobot& R, int obb_cap, int G) {
+    if (k == kKCollide) return grid_collide(lds_bytes(obb_cap, R.n_dof));
+    const bool grad = k == kKCostGrad;
+    return grid_cost(R.n_joints, grad, lds_bytes_cost(obb_cap, R.n_dof, R.n_joints, G, grad));
 }
 
 DChunks implicit_chunks(long n_states, KernelKind k, const DRobot& R, int obb_cap, int* grid_out, int G = 1) {
-    const int grid_cap = grid_for(k, R, obb_cap);
+    const int grid_cap = grid_for(k, R, obb_cap, G);
     DChunks C{};
     C.obb_cap = obb_cap;
     C.tab = nullptr;
@@ -184,7 +189,7 @@ DChunks implicit_chunks(long n_states, KernelKind k, const DRobot& R, int obb_ca
     return C;
 }
 
-int group_index(int G) { return G >= 8 ? 3 : G >= 4 ? 2 : G >= 2 ? 1 : 0; }
+int group_index(int G) { return G >= 32 ? 5 : G >= 16 ? 4 : G >= 8 ? 3 : G >= 4 ? 2 : G >= 2 ? 1 : 0; }
 
 // Chunk table of a plan for `G` lanes per state (64 / G states per chunk),
 // built on first use; chunks never cross a problem boundary.
@@ -193,9 +198,15 @@ hipError_t ensure_table(ppl_plan_s* P, int G) {
     if (P->d_tab[gi] || P->n_states == 0) return hipSuccess;
     const long chunk = kWave / G;
     std::vector<int4> tab;
+    // w = the unit (edge / trajectory) holding the chunk's first state
     for (size_t p = 0; p < P->pb.size(); ++p)
-        for (long s = P->pb[p]; s < P->pe[p]; s += chunk)
-            tab.push_back(make_int4((int)p, (int)s, (int)std::min(s + chunk, P->pe[p]), 0));
+        for (long s = P->pb[p]; s < P->pe[p]; s += chunk) {
+            int u = (int)s;
+            if (P->wmajor) u = -1;   // the kernel maps waypoint-major indices arithmetically
+            else if (P->kind != PPL_PLAN_CONFIGS)
+                u = (int)(std::upper_bound(P->h_off.begin(), P->h_off.end(), (int)s) - P->h_off.begin()) - 1;
+            tab.push_back(make_int4((int)p, (int)s, (int)std::min(s + chunk, P->pe[p]), u));
+        }
     if (tab.empty()) return hipSuccess;
     hipError_t e = hipMalloc((void**)&P->d_tab[gi], tab.size() * sizeof(int4));
     if (e == hipSuccess) e = hipMemcpy(P->d_tab[gi], tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice);
@@ -205,7 +216,7 @@ hipError_t ensure_table(ppl_plan_s* P, int G) {
 
 DChunks plan_chunks(const ppl_plan_s* P, KernelKind k, int obb_cap, int G, int* grid_out) {
     const int gi = group_index(G);
-    const int grid_cap = grid_for(k, P->robot->R, obb_cap);
+    const int grid_cap = grid_for(k, P->robot->R, obb_cap, G);
     DChunks C{};
     C.obb_cap = obb_cap;
     C.tab = P->d_tab[gi];
@@ -216,12 +227,16 @@ DChunks plan_chunks(const ppl_plan_s* P, KernelKind k, int obb_cap, int G, int* 
     return C;
 }
 
-// Lanes per state for the cost kernel: enough lanes for about two waves per
-// SIMD across the chip, at most 8.
-int pick_group(long n_states) {
-    const long want = (long)device_cus() * 4 * 2 * kWave;
+// Lanes per state for the cost kernel: enough lanes for about four waves per
+// SIMD across the chip, at most kMaxGroup. The adjoint also keeps its per-state
+// records within 16 KiB of LDS per wave.
+int pick_group(long n_states, const DRobot& R, bool grad) {
+    const long want = (long)device_cus() * 4 * 4 * kWave;
     int G = 1;
-    while (G < 8 && n_states * G < want) G <<= 1;
+    while (G < kMaxGroup && n_states * G < want) G <<= 1;
+    if (grad)
+        while (G < kMaxGroup && (size_t)(kWave / G) * (rec_stride(R.n_joints) + dq_stride(R.n_dof)) * sizeof(float) > 16384)
+            G <<= 1;
     return G;
 }
 
@@ -243,6 +258,8 @@ DStates plan_states(const ppl_plan_s* P, const float* x) {
         X.theta = x;
         X.traj_off = P->d_off;
         X.n_traj = P->n_units;
+        X.prob = P->d_prob;
+        X.wmajor = P->wmajor;
     }
     return X;
 }
@@ -262,11 +279,16 @@ int edge_g(const float* a, const float* b, int dim, const ppl_edge_params& p, fl
     return std::min(std::max(g, 2), 1024);
 }
 
-ppl_status run_self(const ppl_robot_s* r, const DStates& X, int mode, uint8_t* hit_out, float* out, hipStream_t s) {
+// Self pass. mode 0: any-collision; mode 1: min clearance, exact below `band`
+// (hinge callers pass their self margin).
+ppl_status run_self(const ppl_robot_s* r, const DStates& X, int mode, uint8_t* hit_out, float* out, hipStream_t s,
+                    float band = FLT_MAX) {
     SelfArgs a{};
     a.R = r->R;
     a.X = X;
+    a.X.wmajor = 0;      // the self pass walks states in plan order
     a.mode = mode;
+    a.band = band;
     a.hit_out = hit_out;
     a.out = out;
     HIP_TRY(launch_self(a, s));
@@ -351,6 +373,22 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
         CHECK_ARG(i >= 0 && i < S && a >= 0 && b < S, "self_cc_ranges[%d] out of range", e);
         for (int j = a; j <= b; ++j) pairs.push_back(make_int2(i, j));
     }
+    // Group the pairs by the (DFS) joints of their two spheres, so the self
+    // pass can skip a whole group when the joints' enclosing spheres are apart.
+    std::vector<int4> sblk;
+    {
+        auto key = [&](const int2& p) {
+            const int a = pos[d->sphere_to_joint[p.x]], b = pos[d->sphere_to_joint[p.y]];
+            return std::make_pair(std::min(a, b), std::max(a, b));
+        };
+        std::stable_sort(pairs.begin(), pairs.end(), [&](const int2& u, const int2& v) { return key(u) < key(v); });
+        for (size_t k = 0; k < pairs.size();) {
+            size_t e = k;
+            while (e < pairs.size() && key(pairs[e]) == key(pairs[k])) ++e;
+            sblk.push_back(make_int4(key(pairs[k]).first, key(pairs[k]).second, (int)k, (int)e));
+            k = e;
+        }
+    }
     for (int e = 0; e < d->n_approx_self_cc_ranges; ++e) {
         const int i = d->approx_self_cc_ranges[3 * e], a = d->approx_self_cc_ranges[3 * e + 1],
                   b = d->approx_self_cc_ranges[3 * e + 2];
@@ -388,17 +426,9 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
     for (int a = 0; a < A; ++a)
         apx_orig[a] = make_float4(d->approx_spheres[4 * a], d->approx_spheres[4 * a + 1], d->approx_spheres[4 * a + 2],
                                   d->approx_spheres[4 * a + 3]);
-    // Tree depth by DFS position (the adjoint keeps one live joint per depth).
+    // Parent of each DFS position, as a DFS position (root: itself).
     std::vector<int> close(J, 0);
-    bool contiguous = true;
-    for (int k = 0; k < J; ++k) {
-        int j = dfs[k], d = 0;
-        while (j != parents[j] && d <= J) { j = parents[j]; ++d; }
-        close[k] = d;
-        if (d >= kMaxDepth) {
-            return fail(PPL_ERR_UNSUPPORTED, "kinematic tree depth %d exceeds %d", d, kMaxDepth - 1);
-        }
-    }
+    for (int k = 0; k < J; ++k) close[k] = k == 0 ? 0 : pos[parents[dfs[k]]];
     // One device blob for every table.
     size_t off = 0;
     auto place = [&](size_t bytes) {
@@ -410,7 +440,8 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
                  o_sph = place(sph.size() * 16), o_sid = place(sph_id.size() * 4), o_aoff = place(apx_off.size() * 4),
                  o_apx = place(apx.size() * 16), o_aid = place(apx_id.size() * 4), o_close = place(close.size() * 4),
                  o_sorig = place(sph_orig.size() * 16), o_pairs = place(pairs.size() * 8),
-                 o_apairs = place(apairs.size() * 8), o_aorig = place(apx_orig.size() * 16);
+                 o_apairs = place(apairs.size() * 8), o_aorig = place(apx_orig.size() * 16),
+                 o_sblk = place(sblk.size() * 16), o_jorig = place(J * 4);
     std::vector<char> host(off + 16, 0);
     auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) memcpy(host.data() + o, src, bytes); };
     put(o_ft, ft.data(), ft.size() * 4);
@@ -426,6 +457,8 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
     put(o_pairs, pairs.data(), pairs.size() * 8);
     put(o_apairs, apairs.data(), apairs.size() * 8);
     put(o_aorig, apx_orig.data(), apx_orig.size() * 16);
+    put(o_sblk, sblk.data(), sblk.size() * 16);
+    put(o_jorig, dfs.data(), (size_t)J * 4);
 
     ppl_robot_s* r = new (std::nothrow) ppl_robot_s();
     if (!r) return fail(PPL_ERR_ALLOC, "out of host memory");
@@ -451,17 +484,48 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
     R.apx_off = reinterpret_cast<const int*>(b + o_aoff);
     R.apx = reinterpret_cast<const float4*>(b + o_apx);
     R.apx_id = reinterpret_cast<const int*>(b + o_aid);
-    R.depth = reinterpret_cast<const int*>(b + o_close);
+    R.parent = reinterpret_cast<const int*>(b + o_close);
     R.sph_orig = reinterpret_cast<const float4*>(b + o_sorig);
     R.pairs = reinterpret_cast<const int2*>(b + o_pairs);
     R.apx_pairs = reinterpret_cast<const int2*>(b + o_apairs);
     R.apx_orig = reinterpret_cast<const float4*>(b + o_aorig);
+    R.sblk = reinterpret_cast<const int4*>(b + o_sblk);
+    R.jorig = reinterpret_cast<const int*>(b + o_jorig);
+    R.n_sblk = (int)sblk.size();
     r->n_slots = n_slots;
     r->n_pairs = (int)pairs.size();
-    r->subtrees_contiguous = contiguous;
+    for (int i = 0; i < J; ++i) {
+        if (i == root || id2dof[i] < 0) continue;
+        if (d->joint_types[i] >= kXRot) ++r->n_revolute;
+        else if (d->joint_types[i] >= kXPrism) ++r->n_prismatic;
+    }
     r->lower.assign(d->joint_lower, d->joint_lower + D);
     r->upper.assign(d->joint_upper, d->joint_upper + D);
     *out = r;
+    return PPL_OK;
+}
+
+ppl_status ppl_robot_get_device_view(ppl_robot r, ppl_robot_device_view* v) {
+    CHECK_ARG(r && v, "NULL argument");
+    const DRobot& R = r->R;
+    v->n_dof = R.n_dof;
+    v->n_joints = R.n_joints;
+    v->n_spheres = R.n_sph;
+    v->n_approx_spheres = R.n_apx;
+    v->n_pairs = R.n_pairs;
+    v->n_approx_pairs = R.n_apx_pairs;
+    v->ft = R.ft;
+    v->jmeta = reinterpret_cast<const int*>(R.jmeta);
+    v->sph_off = R.sph_off;
+    v->sph = reinterpret_cast<const float*>(R.sph);
+    v->sph_id = R.sph_id;
+    v->apx_off = R.apx_off;
+    v->apx = reinterpret_cast<const float*>(R.apx);
+    v->apx_id = R.apx_id;
+    v->sph_orig = reinterpret_cast<const float*>(R.sph_orig);
+    v->apx_orig = reinterpret_cast<const float*>(R.apx_orig);
+    v->pairs = reinterpret_cast<const int*>(R.pairs);
+    v->apx_pairs = reinterpret_cast<const int*>(R.apx_pairs);
     return PPL_OK;
 }
 
@@ -765,9 +829,31 @@ ppl_status ppl_plan_create(ppl_robot r, const ppl_plan_desc* d, ppl_plan* out) {
         P->pb.push_back(uoff[pu[p]]);
         P->pe.push_back(uoff[pu[p + 1]]);
     }
+    std::vector<int4> prob;
+    if (d->waypoint_major) {
+        if (d->kind != PPL_PLAN_TRAJ) {
+            delete P;
+            return fail(PPL_ERR_INVALID, "waypoint_major needs a trajectory plan");
+        }
+        for (int p = 0; p < d->n_problems; ++p) {
+            const int u0 = pu[p], nu = pu[p + 1] - pu[p];
+            const int T = nu > 0 ? d->unit_states[u0] : 1;
+            for (int u = u0; u < pu[p + 1]; ++u)
+                if (d->unit_states[u] != T) {
+                    delete P;
+                    return fail(PPL_ERR_INVALID, "waypoint_major: trajectories of problem %d differ in length", p);
+                }
+            prob.push_back(make_int4(u0, std::max(nu, 1), T, (int)uoff[u0]));
+        }
+        P->wmajor = 1;
+    }
     P->h_off.assign(uoff.size(), 0);
     for (size_t i = 0; i < uoff.size(); ++i) P->h_off[i] = (int)uoff[i];
     hipError_t e = ensure_table(P, 1);
+    if (e == hipSuccess && !prob.empty()) {
+        e = hipMalloc((void**)&P->d_prob, prob.size() * sizeof(int4));
+        if (e == hipSuccess) e = hipMemcpy(P->d_prob, prob.data(), prob.size() * sizeof(int4), hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess) e = hipMalloc((void**)&P->d_off, P->h_off.size() * sizeof(int));
     if (e == hipSuccess) e = hipMemcpy(P->d_off, P->h_off.data(), P->h_off.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess && !P->h_obb_off.empty()) {
@@ -786,16 +872,33 @@ ppl_status ppl_plan_create(ppl_robot r, const ppl_plan_desc* d, ppl_plan* out) {
 ppl_status ppl_plan_destroy(ppl_plan P) {
     if (!P) return PPL_OK;
     (void)hipDeviceSynchronize();
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < kGroupKinds; ++i)
         if (P->d_tab[i]) (void)hipFree(P->d_tab[i]);
     if (P->d_off) (void)hipFree(P->d_off);
     if (P->d_obb_off) (void)hipFree(P->d_obb_off);
+    if (P->d_prob) (void)hipFree(P->d_prob);
     P->d_self.release(); P->d_cost.release(); P->d_fdcfg.release(); P->d_fdcost.release();
     delete P;
     return PPL_OK;
 }
 
 long ppl_plan_states(ppl_plan P) { return P ? P->n_states : -1; }
+
+double ppl_algorithmic_flops(ppl_robot r, int kind, int n_obbs, int n_waypoints, int use_self) {
+    if (!r || n_obbs < 0) return -1.0;
+    const int nr = r->n_revolute, np = r->n_prismatic, ns = r->R.n_sph, D = r->R.n_dof;
+    const int pairs = use_self ? r->n_pairs : 0;
+    switch (kind) {
+        case PPL_WORK_ANY_COLLISION: return ppl_flops_any_collision(nr, np, ns, n_obbs);
+        case PPL_WORK_HINGE_STATE: return ppl_flops_hinge_state(nr, np, ns, n_obbs, pairs);
+        case PPL_WORK_CHOMP_ADJOINT:
+            return n_waypoints >= 2 ? ppl_flops_chomp_adjoint_traj(nr, np, ns, n_obbs, n_waypoints) : -1.0;
+        case PPL_WORK_CHOMP_FD: return n_waypoints >= 2 ? ppl_flops_chomp_fd_traj(nr, np, ns, n_obbs, n_waypoints, D) : -1.0;
+        case PPL_WORK_STOMP_STATE: return ppl_flops_stomp_state(nr, np, ns, n_obbs, pairs);
+        case PPL_WORK_ESDF_WAYPOINT: return ppl_flops_esdf_waypoint(nr, np, ns);
+        default: return -1.0;
+    }
+}
 
 }  // extern "C"
 
@@ -823,14 +926,20 @@ ppl_status cost_self(ppl_plan_s* P, const DStates& X, const ppl_cost_params* prm
     *self_in = nullptr;
     if (!prm->use_self || P->robot->n_pairs == 0 || X.n_states == 0) return PPL_OK;
     HIP_TRY(P->d_self.ensure((size_t)X.n_states * 8 * sizeof(float)));
-    ppl_status st = run_self(P->robot, X, 1, nullptr, P->d_self.as<float>(), s);
+    ppl_status st = run_self(P->robot, X, 1, nullptr, P->d_self.as<float>(), s, prm->self_margin);
     if (st != PPL_OK) return st;
     *self_in = P->d_self.as<float>();
     return PPL_OK;
 }
 
+bool group_fits(const ppl_plan_s* P, int G, bool grad, int obb_cap) {
+    const DRobot& R = P->robot->R;
+    return lds_bytes_cost(obb_cap, R.n_dof, R.n_joints, G, grad) <= 65536;
+}
+
+// force_G > 0 overrides the plan's lane-group choice (autotuning).
 ppl_status hinge_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_x, const ppl_cost_params* prm,
-                      float* d_state, float* d_total, float* d_grad, hipStream_t s) {
+                      float* d_state, float* d_total, float* d_grad, hipStream_t s, int force_G = 0) {
     CHECK_ARG(P && prm && d_x, "NULL argument");
     CHECK_ARG(P->kind != PPL_PLAN_EDGES, "hinge cost needs a configuration or trajectory plan");
     DScene S{};
@@ -850,7 +959,8 @@ ppl_status hinge_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_
     a.S = S;
     a.X = X;
     int grid = 0;
-    const int G = pick_group(P->n_states);
+    const int tuned = P->tuned_group[d_grad ? 1 : 0];
+    const int G = force_G > 0 ? force_G : tuned > 0 ? tuned : pick_group(P->n_states, P->robot->R, d_grad != nullptr);
     HIP_TRY(ensure_table(P, G));
     a.C = plan_chunks(P, d_grad ? kKCostGrad : kKCost, plan_obb_cap(P, S), G, &grid);
     a.group = G;
@@ -930,7 +1040,7 @@ ppl_status fd_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_the
     const float* self_in = nullptr;
     if (prm->use_self && P->robot->n_pairs > 0) {
         HIP_TRY(P->d_self.ensure((size_t)n_cfg * 8 * 4));
-        if ((st = run_self(P->robot, X, 1, nullptr, P->d_self.as<float>(), s)) != PPL_OK) return st;
+        if ((st = run_self(P->robot, X, 1, nullptr, P->d_self.as<float>(), s, prm->self_margin)) != PPL_OK) return st;
         self_in = P->d_self.as<float>();
     }
     CostArgs a{};
@@ -966,6 +1076,14 @@ ppl_status stomp_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_
 }  // namespace
 
 extern "C" {
+
+ppl_status ppl_fk_batched(ppl_plan P, const float* d_cfg, float* d_frames, float* d_spheres, ppl_stream stream) {
+    CHECK_ARG(P && d_cfg, "NULL argument");
+    CHECK_ARG(P->kind == PPL_PLAN_CONFIGS, "configuration plan required");
+    if (P->n_states == 0 || (!d_frames && !d_spheres)) return PPL_OK;
+    HIP_TRY(launch_fk_out(P->robot->R, plan_states(P, d_cfg), d_frames, d_spheres, (hipStream_t)stream));
+    return PPL_OK;
+}
 
 ppl_status ppl_check_collision_batched(ppl_plan P, const ppl_scene_view* view, const float* d_cfg, float margin,
                                        int check_self, uint8_t* d_out, ppl_stream stream) {
@@ -1054,6 +1172,52 @@ ppl_status ppl_min_clearance_batched(ppl_plan P, const ppl_scene_view* view, con
     a.margin = margin;
     a.cost_out = d_out;
     HIP_TRY(launch_cost(a, grid, (hipStream_t)stream));
+    return PPL_OK;
+}
+
+ppl_status ppl_plan_autotune_cost(ppl_plan P, const ppl_scene_view* view, const float* d_x, const ppl_cost_params* prm,
+                                  int grad, int reps, int* best_group, float* best_ms, ppl_stream stream) {
+    CHECK_ARG(P && view && d_x && prm, "NULL argument");
+    CHECK_ARG(P->kind != PPL_PLAN_EDGES, "cost autotuning needs a configuration or trajectory plan");
+    CHECK_ARG(reps >= 1 && reps <= 1000, "reps must be in [1, 1000]");
+    if (P->n_states == 0) return PPL_OK;
+    DScene S{};
+    ppl_status st = plan_scene(P, view, S);
+    if (st != PPL_OK) return st;
+    const int D = P->robot->R.n_dof;
+    const size_t n = (size_t)P->n_states;
+    HIP_TRY(P->d_tune.ensure((n * (grad ? D + 1 : 1) + (size_t)P->n_units) * sizeof(float)));
+    float* state = P->d_tune.as<float>();
+    float* total = state + n;
+    float* g = grad ? total + P->n_units : nullptr;
+    if (P->kind != PPL_PLAN_TRAJ) total = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    hipError_t he = hipEventCreate(&e1);
+    if (he != hipSuccess) { (void)hipEventDestroy(e0); return fail(PPL_ERR_HIP, "hipEventCreate failed"); }
+    int best = 0;
+    float best_t = 0.0f;
+    const int cap = plan_obb_cap(P, S);
+    for (int G = 1; G <= kMaxGroup && st == PPL_OK; G <<= 1) {
+        if (!group_fits(P, G, grad != 0, cap)) continue;
+        st = hinge_impl(P, view, d_x, prm, state, total, g, s, G);   // warm-up (builds the chunk table)
+        he = hipEventRecord(e0, s);
+        for (int k = 0; k < reps && st == PPL_OK && he == hipSuccess; ++k)
+            st = hinge_impl(P, view, d_x, prm, state, total, g, s, G);
+        if (he == hipSuccess) he = hipEventRecord(e1, s);
+        if (he == hipSuccess) he = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+        if (he != hipSuccess) st = fail(PPL_ERR_HIP, "autotune timing: %s", hipGetErrorString(he));
+        if (st == PPL_OK && (best == 0 || ms < best_t)) { best = G; best_t = ms; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != PPL_OK) return st;
+    P->tuned_group[grad ? 1 : 0] = best;
+    if (best_group) *best_group = best;
+    if (best_ms) *best_ms = best_t / reps;
     return PPL_OK;
 }
 

@@ -316,20 +316,60 @@ struct StateRef {
     int k;         // edge state index or waypoint index
     int n;         // edge states (incl. skipped) or trajectory length
     long u0;       // plan-order index of the unit's first state
+    long gs;       // plan-order index of the state (differs from the chunk index when waypoint-major)
 };
 
+// Ends of units u0 .. u0+63 into s_end (whole wave; no-op for configuration
+// sources or when the chunk's first unit is unknown, u0 < 0).
+__device__ __forceinline__ void stage_unit_ends(const DStates& X, int u0, int* s_end) {
+    if (X.source == kSrcConfig || u0 < 0) return;
+    const int* off = X.source == kSrcEdge ? X.edge_off : X.traj_off;
+    const int n = X.source == kSrcEdge ? X.n_edges : X.n_traj;
+    const int u = u0 + lane_id();
+    s_end[lane_id()] = u < n ? off[u + 1] : INT_MAX;
+}
+
+// Unit holding state g: the first of units u0 .. u0+63 whose end exceeds g
+// (bisection over the staged ends), else a binary search over the table.
+__device__ __forceinline__ int find_unit(const int* off, int n, long g, int u0, const int* s_end) {
+    if (u0 >= 0 && s_end[kWave - 1] > g) {
+        int lo = 0, hi = kWave - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s_end[mid] > g) hi = mid; else lo = mid + 1;
+        }
+        return u0 + lo;
+    }
+    return upper_bound_idx(off, n, g);
+}
+
 // Writes q_d into s_q[d * 64 + lane] for d < D (inactive lanes write nothing).
-__device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, bool active, float* s_q) {
-    StateRef ref{0, 0, 0, 0};
+// g is the chunk-order index; prob the chunk's problem (waypoint-major plans).
+__device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, bool active, float* s_q,
+                                               int u0 = -1, const int* s_end = nullptr, int prob = 0) {
+    StateRef ref{0, 0, 0, 0, g};
     const int lane = lane_id();
     if (!active) return ref;
+    if (X.source == kSrcTraj && X.wmajor) {
+        const int4 pr = X.prob[prob];
+        const int v = (int)(g - pr.w);
+        const int t = v / pr.y, r = v - t * pr.y;
+        ref.unit = pr.x + r;
+        ref.k = t;
+        ref.n = pr.z;
+        ref.u0 = (long)pr.w + (long)r * pr.z;
+        ref.gs = ref.u0 + t;
+        const float* th = X.theta + (size_t)ref.u0 * D;
+        for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = th[(size_t)d * pr.z + t];
+        return ref;
+    }
     if (X.source == kSrcConfig) {
         ref.unit = (int)g;
         ref.u0 = g;
         const float* q = X.cfg + (size_t)g * X.cfg_stride;
         for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = q[d];
     } else if (X.source == kSrcEdge) {
-        const int e = upper_bound_idx(X.edge_off, X.n_edges, g);
+        const int e = find_unit(X.edge_off, X.n_edges, g, u0, s_end);
         const int cnt = X.edge_off[e + 1] - X.edge_off[e];
         ref.unit = e;
         ref.u0 = X.edge_off[e];
@@ -340,7 +380,7 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         const float* b = X.to + (size_t)e * D;
         for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = fmaf(t, b[d] - a[d], a[d]);
     } else {
-        const int b = upper_bound_idx(X.traj_off, X.n_traj, g);
+        const int b = find_unit(X.traj_off, X.n_traj, g, u0, s_end);
         const int t0 = X.traj_off[b];
         const int Tn = X.traj_off[b + 1] - t0;
         ref.unit = b;
@@ -377,10 +417,10 @@ __device__ __forceinline__ bool next_chunk(const DChunks& C, long n_states, Chun
     cur.next += cur.stride;
     if (C.tab) {
         ch = C.tab[c];
-        ch.x = uniform(ch.x); ch.y = uniform(ch.y); ch.z = uniform(ch.z);
+        ch.x = uniform(ch.x); ch.y = uniform(ch.y); ch.z = uniform(ch.z); ch.w = uniform(ch.w);
     } else {
         const long b = (long)c * C.chunk;
-        ch = make_int4(0, (int)b, (int)(b + C.chunk < n_states ? b + C.chunk : n_states), 0);
+        ch = make_int4(0, (int)b, (int)(b + C.chunk < n_states ? b + C.chunk : n_states), -1);
     }
     return true;
 }
@@ -415,26 +455,6 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-// OBBs within `reach` of the point (cx, cy, cz) (wave-uniform inputs): lane o
-// of the wave tests OBB base + o, one pass per 64 OBBs.
-__device__ __forceinline__ void reach_obbs(const float4* __restrict__ s_obb, int n_obb, float cx, float cy, float cz,
-                                           float reach, ObbMask& mk) {
-    const float r2 = reach * reach;
-#pragma unroll
-    for (int k = 0; k < kMaskWords; ++k) {
-        const int o = k * 64 + lane_id();
-        bool c = false;
-        if (k * 64 < n_obb) {
-            if (o < n_obb) {
-                float lx, ly, lz;
-                to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], cx, cy, cz, lx, ly, lz);
-                c = box_hit(lx, ly, lz, s_obb[4 * o + 3], r2);
-            }
-        }
-        mk.w[k] = __ballot(c);
-    }
-}
-
 // ===========================================================================
 // Any-collision: two-phase gate semantics (two_phase_cc.cuh:57-138 with the
 // check kernel's environment pass, pRRTC.cu:2337-2386).
@@ -446,6 +466,7 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
     const DRobot& R = a.R;
     const int lane = lane_id();
     const int J = R.n_joints, D = R.n_dof;
+    int* s_end = reinterpret_cast<int*>(s_q + D * kWave);
     const bool has_esdf = a.S.n_voxel_layers > 0;
     const uint8_t* acm = a.S.sphere_obb_acm;
     int cur_prob = -1, n_obb = 0;
@@ -461,9 +482,11 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
         }
         const long g = (long)ch.y + lane;
         bool active = g < ch.z;
-        StateRef ref = load_state(a.X, D, g, active, s_q);
+        stage_unit_ends(a.X, ch.w, s_end);
+        __syncthreads();
+        StateRef ref = load_state(a.X, D, g, active, s_q, ch.w, s_end, ch.x);
         bool hit = false;
-        if (active && a.self_hit && a.self_hit[g]) { hit = true; active = false; }
+        if (active && a.self_hit && a.self_hit[ref.gs]) { hit = true; active = false; }
         if (active && a.mode == 1 && a.skip_resolved) {
             const unsigned fh = (unsigned)a.first_hit[ref.unit];
             if (fh < (unsigned)ref.k) active = false;
@@ -580,7 +603,7 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
                 }
             }
         }
-        if (g < ch.z && a.out) a.out[g] = hit ? 1 : 0;
+        if (g < ch.z && a.out) a.out[ref.gs] = hit ? 1 : 0;
         if (a.mode == 1) {
             // One atomic per (chunk, edge): a hit lane reports only when no
             // lower lane of the same edge hit (lanes hold consecutive states).
@@ -605,41 +628,42 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
 // recorded during the forward walk. Lever arms are taken relative to each
 // joint, so no large world-origin moments are summed and subtracted.
 // ===========================================================================
-// Candidate OBBs of one joint: a wave-uniform bitmask (up to kMaxObbLds
-// OBBs) of the OBBs whose distance to the joint's approximate sphere is below
-// the hinge band for at least one lane. The approximate sphere encloses the
-// joint's spheres, so every OBB left out is at clearance >= margin from all
-// of them and cannot change a hinge, its argmin or its gradient.
+// Candidate OBBs of one joint for the hinge: a wave-uniform bitmask of the
+// OBBs some active lane's spheres of this joint can come within the hinge
+// band of. Every sphere s lies inside the joint's approximate sphere (centre
+// a, radius R), so clr(s, o) >= sdf_o(a) - R; the active lanes' centres lie
+// within rw of the first active lane's centre c0, so sdf_o(a) >= sdf_o(c0) -
+// rw. An OBB with sdf_o(c0) >= band + R + rw therefore has clearance >= band
+// from every sphere: it cannot change a hinge, its argmin or its gradient.
+// One pass per 64 OBBs (lane i tests OBB base + i).
 __device__ __forceinline__ void near_obbs(const float4* __restrict__ s_obb, int n_obb, const float* T, const float4& as,
-                                          float band, bool active, int sub, int G, ObbMask& mk) {
+                                          float band, bool active, ObbMask& mk) {
     float ax, ay, az;
     aff_point(T, as.x, as.y, as.z, ax, ay, az);
-    const float thr = band + as.w;           // sdf(centre) < margin + R  <=>  key < thr^2
-    const float thr2 = thr * thr;
+    const int l0 = __ffsll(__ballot(active)) - 1;
+    if (l0 < 0) {
 #pragma unroll
-    for (int k = 0; k < kMaskWords; ++k) mk.w[k] = 0ull;
-    // lanes of a state group split the OBBs; the union is taken over the wave.
-    // rep has bit L set for every lane L = 0 (mod G).
-    const unsigned long long rep = G == 1 ? ~0ull : G == 2 ? 0x5555555555555555ull
-                                 : G == 4 ? 0x1111111111111111ull : 0x0101010101010101ull;
-    for (int base = 0; base < n_obb; base += G) {
-        const int o = base + sub;
-        bool nr = false;
-        if (o < n_obb) {
-            float lx, ly, lz;
-            to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], ax, ay, az, lx, ly, lz);
-            nr = active && box_key(lx, ly, lz, s_obb[4 * o + 3]) < thr2;
-        }
-        const unsigned long long b = __ballot(nr);
-        // lane L of the wave tested OBB base + (L & (G - 1)): fold the groups
-        unsigned long long f = 0ull;
-        for (int q = 0; q < G; ++q)
-            if (b & (rep << q)) f |= 1ull << q;
+        for (int k = 0; k < kMaskWords; ++k) mk.w[k] = 0ull;
+        return;
+    }
+    const float cx = read_lane(ax, l0), cy = read_lane(ay, l0), cz = read_lane(az, l0);
+    const float dx = ax - cx, dy = ay - cy, dz = az - cz;
+    const float d2 = wave_max(active ? fmaf(dz, dz, fmaf(dy, dy, dx * dx)) : 0.0f);
+    // sdf(c0) < reach  <=>  key(c0) < reach * |reach|; slack covers rounding
+    const float reach = band + as.w + sqrtf(d2) * 1.0001f + 1e-5f;
+    const float thr = uniform(reach * fabsf(reach));
 #pragma unroll
-        for (int k = 0; k < kMaskWords; ++k) {
-            const int lo = k * 64;
-            if (base >= lo && base < lo + 64) mk.w[k] |= f << (base - lo);
+    for (int k = 0; k < kMaskWords; ++k) {
+        bool c = false;
+        if (k * 64 < n_obb) {
+            const int o = k * 64 + lane_id();
+            if (o < n_obb) {
+                float lx, ly, lz;
+                to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], cx, cy, cz, lx, ly, lz);
+                c = box_key(lx, ly, lz, s_obb[4 * o + 3]) < thr;
+            }
         }
+        mk.w[k] = __ballot(c);
     }
 }
 
@@ -706,7 +730,7 @@ __device__ __forceinline__ float group_min(float v, int G) {
 }
 
 template <bool GRAD>
-__global__ __launch_bounds__(64) void k_cost(CostArgs a) {
+__global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
     extern __shared__ float4 s_lds[];
     float4* s_obb = s_lds;
     float* s_q = reinterpret_cast<float*>(s_lds + 4 * a.C.obb_cap);
@@ -714,8 +738,15 @@ __global__ __launch_bounds__(64) void k_cost(CostArgs a) {
     const int lane = lane_id();
     const int G = a.group;                    // lanes per state
     const int sub = lane & (G - 1);
-    const int per_wave = kWave / G;           // states per chunk
+    const int slot = lane / G;                // state slot of this lane group
     const int J = R.n_joints, D = R.n_dof;
+    // Adjoint records (GRAD): per state slot and joint, the joint origin o,
+    // axis z, and the force f / moment tau (about o) of its own spheres'
+    // hinge gradients; then d cost / d q per dof.
+    int* s_end = reinterpret_cast<int*>(s_q + D * kWave);
+    const int rs = rec_stride(J), ds = dq_stride(D);
+    float* rec = s_q + (D + 1) * kWave + slot * rs;
+    float* dqs = s_q + (D + 1) * kWave + (kWave / G) * rs + slot * ds;
     const bool use_obb = a.P.use_obb != 0 && !a.clearance_only;
     const bool use_esdf = (a.P.use_esdf != 0 || a.clearance_only) && a.S.n_voxel_layers > 0;
     const float wt = a.P.weight;
@@ -730,127 +761,115 @@ __global__ __launch_bounds__(64) void k_cost(CostArgs a) {
             n_obb = uniform(stage_obbs(a.S, o0, o1, s_obb, a.C.obb_cap));
             cur_prob = ch.x;
         }
-        (void)per_wave;
-        const long g = (long)ch.y + lane / G;
+        const long g = (long)ch.y + slot;
         const bool active = g < ch.z;
-        StateRef ref = load_state(a.X, D, g, active, s_q);
+        stage_unit_ends(a.X, ch.w, s_end);
+        __syncthreads();
+        StateRef ref = load_state(a.X, D, g, active, s_q, ch.w, s_end, ch.x);
         __syncthreads();
         // Self term from the self pass: hinge and its two point gradients.
         float self_h = 0.0f, snx = 0.0f, sny = 0.0f, snz = 0.0f;
         int si = -1, sj = -1;
         if (a.self_in && active && !a.clearance_only) {
-            const float* so = a.self_in + 8 * (size_t)g;
+            const float* so = a.self_in + 8 * (size_t)ref.gs;
             self_h = fmaxf(a.P.self_margin - so[0], 0.0f);
             if (self_h > 0.0f) { si = __float_as_int(so[1]); sj = __float_as_int(so[2]); snx = so[3]; sny = so[4]; snz = so[5]; }
         }
-        // Output addressing of d cost / d q for this lane's state.
-        bool endpoint = false;
-        size_t base = 0, stride = 1;
-        if (a.X.source == kSrcTraj) {
-            endpoint = (ref.k == 0 || ref.k == ref.n - 1);
-            base = (size_t)a.X.traj_off[ref.unit] * D + ref.k;
-            stride = ref.n;
-        } else {
-            base = (size_t)g * D;
-        }
-        const bool write_grad = GRAD && active && !a.clearance_only && sub == 0;
-        float Z[kMaxDepth][3], O[kMaxDepth][3], DQ[kMaxDepth];
-        int kind[kMaxDepth], dofs[kMaxDepth];   // wave-uniform; kind 0 none, 1 revolute, 2 prismatic
-#pragma unroll
-        for (int L = 0; L < kMaxDepth; ++L) { kind[L] = 0; dofs[L] = -1; DQ[L] = 0.0f; }
         float world = 0.0f, minclr = 1000.0f;
         Slots sl;
         for (int jj = 0; jj < J; ++jj) {
             float T[12];
             int4 meta;
             fk_joint(R, jj, s_q + lane, sl, T, meta);
-            const int dep = R.depth[jj];
-            if (GRAD) {
-                // Leave the subtrees of every live joint at depth >= dep.
-#pragma unroll
-                for (int L = 1; L < kMaxDepth; ++L) {
-                    if (L >= dep && kind[L] != 0) {
-                        const float v = group_sum(DQ[L], G);
-                        if (write_grad) a.grad_out[base + (size_t)dofs[L] * stride] = endpoint ? 0.0f : wt * v;
-                        kind[L] = 0;
-                    }
-                }
-                const int type = meta.x;
-                const int ax = type >= kXRot ? type - kXRot : (type >= 0 ? type : 0);
-                const int kd = (jj == 0 || meta.y < 0) ? 0 : (type >= kXRot ? 1 : (type >= 0 ? 2 : 0));
-#pragma unroll
-                for (int L = 1; L < kMaxDepth; ++L) {
-                    if (L == dep) {
-                        Z[L][0] = T[ax]; Z[L][1] = T[4 + ax]; Z[L][2] = T[8 + ax];
-                        O[L][0] = T[3]; O[L][1] = T[7]; O[L][2] = T[11];
-                        kind[L] = kd;
-                        dofs[L] = meta.y;
-                        DQ[L] = 0.0f;
-                    }
-                }
-            }
+            float fx = 0.0f, fy = 0.0f, fz = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
             const int s0 = R.sph_off[jj], s1 = R.sph_off[jj + 1];
-            if (s1 == s0) continue;
-            ObbMask mk;
-            if (use_obb) {
-                const int a0 = R.apx_off[jj], a1 = R.apx_off[jj + 1];
-                if (a1 - a0 == 1) {
-                    near_obbs(s_obb, n_obb, T, R.apx[a0], a.P.world_margin, active, sub, G, mk);
-                } else {
-                    mask_all(n_obb, mk);   // no single enclosing sphere: every OBB is a candidate
+            if (s1 > s0) {
+                ObbMask mk;
+                if (use_obb) {
+                    const int a0 = R.apx_off[jj], a1 = R.apx_off[jj + 1];
+                    if (a1 - a0 == 1) near_obbs(s_obb, n_obb, T, R.apx[a0], a.P.world_margin, active, mk);
+                    else mask_all(n_obb, mk);   // no single enclosing sphere: every OBB is a candidate
+                }
+                for (int s = s0 + sub; s < s1; s += G) {
+                    const float4 sp = R.sph[s];
+                    float px, py, pz;
+                    aff_point(T, sp.x, sp.y, sp.z, px, py, pz);
+                    float gx, gy, gz;
+                    const float hs = sphere_hinge<GRAD>(a, s_obb, mk, px, py, pz, sp.w, R.sph_id[s], use_obb, use_esdf,
+                                                        gx, gy, gz);
+                    if (a.clearance_only) minclr = fminf(minclr, hs);
+                    else world += hs;
+                    if (GRAD) {
+                        if (si >= 0) {
+                            const int id = R.sph_id[s];
+                            if (id == si) { gx -= snx; gy -= sny; gz -= snz; }
+                            if (id == sj) { gx += snx; gy += sny; gz += snz; }
+                        }
+                        // wrench about the joint origin: f += g, tau += (p - o) x g
+                        const float rx = px - T[3], ry = py - T[7], rz = pz - T[11];
+                        fx += gx; fy += gy; fz += gz;
+                        tx = fmaf(ry, gz, fmaf(-rz, gy, tx));
+                        ty = fmaf(rz, gx, fmaf(-rx, gz, ty));
+                        tz = fmaf(rx, gy, fmaf(-ry, gx, tz));
+                    }
                 }
             }
-            for (int s = s0 + sub; s < s1; s += G) {
-                const float4 sp = R.sph[s];
-                float px, py, pz;
-                aff_point(T, sp.x, sp.y, sp.z, px, py, pz);
-                float gx, gy, gz;
-                const float hs = sphere_hinge<GRAD>(a, s_obb, mk, px, py, pz, sp.w, R.sph_id[s], use_obb, use_esdf,
-                                                    gx, gy, gz);
-                if (a.clearance_only) minclr = fminf(minclr, hs);
-                else world += hs;
-                if (GRAD) {
-                    if (si >= 0) {
-                        const int id = R.sph_id[s];
-                        if (id == si) { gx -= snx; gy -= sny; gz -= snz; }
-                        if (id == sj) { gx += snx; gy += sny; gz += snz; }
-                    }
-                    const bool nz = (gx != 0.0f) || (gy != 0.0f) || (gz != 0.0f);
-                    if (wave_any(nz && active)) {
-#pragma unroll
-                        for (int L = 1; L < kMaxDepth; ++L) {
-                            if (L <= dep) {
-                                if (kind[L] == 1) {
-                                    const float dx = px - O[L][0], dy = py - O[L][1], dz = pz - O[L][2];
-                                    const float vx = Z[L][1] * dz - Z[L][2] * dy;
-                                    const float vy = Z[L][2] * dx - Z[L][0] * dz;
-                                    const float vz = Z[L][0] * dy - Z[L][1] * dx;
-                                    DQ[L] = fmaf(gx, vx, fmaf(gy, vy, fmaf(gz, vz, DQ[L])));
-                                } else if (kind[L] == 2) {
-                                    DQ[L] = fmaf(gx, Z[L][0], fmaf(gy, Z[L][1], fmaf(gz, Z[L][2], DQ[L])));
-                                }
-                            }
-                        }
-                    }
+            if (GRAD && !a.clearance_only) {
+                const bool nz = fx != 0.0f || fy != 0.0f || fz != 0.0f || tx != 0.0f || ty != 0.0f || tz != 0.0f;
+                if (G > 1 && wave_any(nz)) {
+                    fx = group_sum(fx, G); fy = group_sum(fy, G); fz = group_sum(fz, G);
+                    tx = group_sum(tx, G); ty = group_sum(ty, G); tz = group_sum(tz, G);
+                }
+                if (sub == 0) {
+                    const int type = meta.x;
+                    const int ax = type >= kXRot ? type - kXRot : (type >= 0 ? type : 0);
+                    float* r = rec + 12 * jj;
+                    r[0] = T[3]; r[1] = T[7]; r[2] = T[11];
+                    r[3] = T[ax]; r[4] = T[4 + ax]; r[5] = T[8 + ax];
+                    r[6] = fx; r[7] = fy; r[8] = fz;
+                    r[9] = tx; r[10] = ty; r[11] = tz;
                 }
             }
         }
-        if (GRAD) {
-#pragma unroll
-            for (int L = 1; L < kMaxDepth; ++L) {
-                if (kind[L] != 0) {
-                    const float v = group_sum(DQ[L], G);
-                    if (write_grad) a.grad_out[base + (size_t)dofs[L] * stride] = endpoint ? 0.0f : wt * v;
+        if (GRAD && !a.clearance_only && sub == 0 && active) {
+            // Reverse DFS sweep: every joint's subtree wrench is complete once
+            // its descendants (higher DFS positions) have been folded in.
+            for (int d = 0; d < D; ++d) dqs[d] = 0.0f;
+            for (int jj = J - 1; jj >= 1; --jj) {
+                const float* r = rec + 12 * jj;
+                const float ox = r[0], oy = r[1], oz = r[2];
+                const float fx = r[6], fy = r[7], fz = r[8];
+                const float tx = r[9], ty = r[10], tz = r[11];
+                const int4 meta = R.jmeta[jj];
+                if (meta.y >= 0 && meta.x >= 0) {
+                    const float dq = meta.x >= kXRot ? fmaf(r[3], tx, fmaf(r[4], ty, r[5] * tz))
+                                                     : fmaf(r[3], fx, fmaf(r[4], fy, r[5] * fz));
+                    dqs[meta.y] += dq;
                 }
+                float* q = rec + 12 * R.parent[jj];
+                const float lx = ox - q[0], ly = oy - q[1], lz = oz - q[2];
+                q[6] += fx; q[7] += fy; q[8] += fz;
+                q[9] += tx + (ly * fz - lz * fy);
+                q[10] += ty + (lz * fx - lx * fz);
+                q[11] += tz + (lx * fy - ly * fx);
             }
+            bool endpoint = false;
+            size_t base = (size_t)ref.gs * D, stride = 1;
+            if (a.X.source == kSrcTraj) {
+                endpoint = (ref.k == 0 || ref.k == ref.n - 1);
+                base = (size_t)a.X.traj_off[ref.unit] * D + ref.k;
+                stride = ref.n;
+            }
+            for (int d = 0; d < D; ++d) a.grad_out[base + (size_t)d * stride] = endpoint ? 0.0f : wt * dqs[d];
         }
         if (a.clearance_only) {
             minclr = group_min(minclr, G);
-            if (active && sub == 0) a.cost_out[g] = minclr;
+            if (active && sub == 0) a.cost_out[ref.gs] = minclr;
         } else {
             world = group_sum(world, G);
-            if (active && sub == 0 && a.cost_out) a.cost_out[g] = wt * (world + self_h);
+            if (active && sub == 0 && a.cost_out) a.cost_out[ref.gs] = wt * (world + self_h);
         }
+        __syncthreads();   // records and q of this chunk are dead before the next load_state
     }
 }
 
@@ -864,21 +883,23 @@ __global__ __launch_bounds__(64) void k_cost(CostArgs a) {
 constexpr int kSelfStates = kWave / kSelfGroup;
 
 __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
-    extern __shared__ float s_dyn[];
+    extern __shared__ float4 s_dyn4[];
     const DRobot& R = a.R;
     const int lane = lane_id();
     const int sub = lane & (kSelfGroup - 1);
     const int st = lane / kSelfGroup;
     const int S = R.n_sph, D = R.n_dof, J = R.n_joints;
-    float* s_pos = s_dyn;                                  // [kSelfStates][S][3]
-    float* s_q = s_dyn + kSelfStates * S * 3;              // [kMaxDof][64]
+    float4* s_pos = s_dyn4;                                          // [kSelfStates][S] (x, y, z, r)
+    float4* s_jnt = s_dyn4 + kSelfStates * S;                         // [kSelfStates][J] enclosing sphere, r < 0: none
+    float* s_q = reinterpret_cast<float*>(s_jnt + kSelfStates * J);   // [kMaxDof][64]
     const long g = (long)blockIdx.x * kSelfStates + st;
     const bool active = g < a.X.n_states;
     // Every lane of a state group loads the same q (lane-indexed LDS column).
     load_state(a.X, D, active ? g : 0, true, s_q);
     __syncthreads();
     Slots sl;
-    float* pos = s_pos + st * S * 3;
+    float4* pos = s_pos + st * S;
+    float4* jnt = s_jnt + st * J;
     for (int jj = 0; jj < J; ++jj) {
         float T[12];
         int4 meta;
@@ -888,30 +909,69 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
             const float4 sp = R.sph[s];
             float x, y, z;
             aff_point(T, sp.x, sp.y, sp.z, x, y, z);
-            const int id = R.sph_id[s];
-            pos[3 * id] = x; pos[3 * id + 1] = y; pos[3 * id + 2] = z;
+            pos[R.sph_id[s]] = make_float4(x, y, z, sp.w);
+        }
+        if (sub == 0) {
+            const int a0 = R.apx_off[jj], a1 = R.apx_off[jj + 1];
+            float4 e = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+            if (a1 - a0 == 1) {
+                const float4 as = R.apx[a0];
+                aff_point(T, as.x, as.y, as.z, e.x, e.y, e.z);
+                e.w = as.w;
+            }
+            jnt[jj] = e;
         }
     }
     __syncthreads();
+    // Pairs come grouped by the joints of their spheres. Each joint's
+    // enclosing sphere contains all of its spheres, so a group whose two
+    // enclosing spheres are at clearance >= band holds no pair below band
+    // (band 0 for the any-collision check: no pair there can collide).
+    const float band = a.mode == 0 ? 0.0f : a.band;
     if (a.mode == 0) {
         bool hit = false;
-        for (int p = sub; p < R.n_pairs && !hit; p += kSelfGroup) {
-            const int2 pr = R.pairs[p];
-            const float dx = pos[3 * pr.x] - pos[3 * pr.y], dy = pos[3 * pr.x + 1] - pos[3 * pr.y + 1], dz = pos[3 * pr.x + 2] - pos[3 * pr.y + 2];
-            const float rs = R.sph_orig[pr.x].w + R.sph_orig[pr.y].w;
-            hit = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) - rs * rs < 0.0f;
+        for (int b = 0; b < R.n_sblk; ++b) {
+            const int4 blk = R.sblk[b];
+            const float4 ea = jnt[blk.x], eb = jnt[blk.y];
+            const float dx = ea.x - eb.x, dy = ea.y - eb.y, dz = ea.z - eb.z;
+            const float rs = ea.w + eb.w;
+            const bool gate = active && !hit && (ea.w < 0.0f || eb.w < 0.0f ||
+                                                 fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < rs * rs);
+            if (!wave_any(gate)) continue;
+            if (gate) {
+                for (int p = blk.z + sub; p < blk.w && !hit; p += kSelfGroup) {
+                    const int2 pr = R.pairs[p];
+                    const float4 u = pos[pr.x], v = pos[pr.y];
+                    const float ux = u.x - v.x, uy = u.y - v.y, uz = u.z - v.z;
+                    const float r2 = u.w + v.w;
+                    hit = fmaf(uz, uz, fmaf(uy, uy, ux * ux)) - r2 * r2 < 0.0f;
+                }
+            }
+            // share the group's verdict so its lanes stop early
+            const unsigned long long m = __ballot(hit);
+            hit = ((m >> (st * kSelfGroup)) & 0xFFFFull) != 0ull;
+            if (__ballot(active && !hit) == 0ull) break;
         }
-        const unsigned long long m = __ballot(hit);
-        const bool any = ((m >> (st * kSelfGroup)) & 0xFFFFull) != 0ull;
-        if (active && sub == 0) a.hit_out[g] = any ? 1 : 0;
+        if (active && sub == 0) a.hit_out[g] = hit ? 1 : 0;
     } else {
         float best = 1000.0f;
         int bp = INT_MAX;
-        for (int p = sub; p < R.n_pairs; p += kSelfGroup) {
-            const int2 pr = R.pairs[p];
-            const float dx = pos[3 * pr.x] - pos[3 * pr.y], dy = pos[3 * pr.x + 1] - pos[3 * pr.y + 1], dz = pos[3 * pr.x + 2] - pos[3 * pr.y + 2];
-            const float c = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))) - (R.sph_orig[pr.x].w + R.sph_orig[pr.y].w);
-            if (c < best) { best = c; bp = p; }
+        for (int b = 0; b < R.n_sblk; ++b) {
+            const int4 blk = R.sblk[b];
+            const float4 ea = jnt[blk.x], eb = jnt[blk.y];
+            const float dx = ea.x - eb.x, dy = ea.y - eb.y, dz = ea.z - eb.z;
+            const float gap = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))) - ea.w - eb.w;
+            const bool gate = active && (ea.w < 0.0f || eb.w < 0.0f || gap < band);
+            if (!wave_any(gate)) continue;
+            if (gate) {
+                for (int p = blk.z + sub; p < blk.w; p += kSelfGroup) {
+                    const int2 pr = R.pairs[p];
+                    const float4 u = pos[pr.x], v = pos[pr.y];
+                    const float ux = u.x - v.x, uy = u.y - v.y, uz = u.z - v.z;
+                    const float c = sqrtf(fmaf(uz, uz, fmaf(uy, uy, ux * ux))) - (u.w + v.w);
+                    if (c < best) { best = c; bp = p; }
+                }
+            }
         }
 #pragma unroll
         for (int off = 1; off < kSelfGroup; off <<= 1) {
@@ -924,7 +984,8 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
             o[0] = best;
             if (bp != INT_MAX) {
                 const int2 pr = R.pairs[bp];
-                const float dx = pos[3 * pr.x] - pos[3 * pr.y], dy = pos[3 * pr.x + 1] - pos[3 * pr.y + 1], dz = pos[3 * pr.x + 2] - pos[3 * pr.y + 2];
+                const float4 u = pos[pr.x], v = pos[pr.y];
+                const float dx = u.x - v.x, dy = u.y - v.y, dz = u.z - v.z;
                 const float inv = 1.0f / sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
                 o[1] = __int_as_float(pr.x); o[2] = __int_as_float(pr.y);
                 o[3] = dx * inv; o[4] = dy * inv; o[5] = dz * inv;
@@ -933,6 +994,46 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
                 o[3] = o[4] = o[5] = 0.0f;
             }
             o[6] = o[7] = 0.0f;
+        }
+    }
+}
+
+// ===========================================================================
+// FK outputs: every joint's world frame (fk_joint_transforms_runtime,
+// planning/runtime_kinematics.cuh:97-171: column-major, out[c*4 + r] = T[r][c])
+// and every sphere's world centre (fk_runtime). One lane per configuration.
+// ===========================================================================
+__global__ __launch_bounds__(64) void k_fk_out(DRobot R, DStates X, float* frames, float* spheres) {
+    __shared__ float s_q[kMaxDof * kWave];
+    const int lane = lane_id();
+    const long g = (long)blockIdx.x * kWave + lane;
+    const bool active = g < X.n_states;
+    load_state(X, R.n_dof, g, active, s_q);
+    __syncthreads();
+    if (!active) return;
+    const int J = R.n_joints;
+    Slots sl;
+    for (int jj = 0; jj < J; ++jj) {
+        float T[12];
+        int4 meta;
+        fk_joint(R, jj, s_q + lane, sl, T, meta);
+        if (frames) {
+            float* o = frames + ((size_t)g * J + R.jorig[jj]) * 16;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                o[c * 4 + 0] = T[c];
+                o[c * 4 + 1] = T[4 + c];
+                o[c * 4 + 2] = T[8 + c];
+                o[c * 4 + 3] = c == 3 ? 1.0f : 0.0f;
+            }
+        }
+        if (spheres) {
+            for (int s = R.sph_off[jj]; s < R.sph_off[jj + 1]; ++s) {
+                const float4 sp = R.sph[s];
+                float x, y, z;
+                aff_point(T, sp.x, sp.y, sp.z, x, y, z);
+                reinterpret_cast<float4*>(spheres)[(size_t)g * R.n_sph + R.sph_id[s]] = make_float4(x, y, z, sp.w);
+            }
         }
     }
 }
@@ -1057,7 +1158,13 @@ int occupancy_grid(const void* fn, size_t dyn_smem) {
 }  // namespace
 
 size_t lds_bytes(int obb_cap, int n_dof) {
-    return (size_t)obb_cap * 64 + (size_t)n_dof * kWave * sizeof(float);
+    return (size_t)obb_cap * 64 + (size_t)(n_dof + 1) * kWave * sizeof(float);
+}
+
+size_t lds_bytes_cost(int obb_cap, int n_dof, int n_joints, int G, bool grad) {
+    size_t b = lds_bytes(obb_cap, n_dof);
+    if (grad) b += (size_t)(kWave / G) * (rec_stride(n_joints) + dq_stride(n_dof)) * sizeof(float);
+    return b;
 }
 
 int grid_collide(size_t lds) {
@@ -1085,7 +1192,7 @@ hipError_t launch_collide(const CollideArgs& a, int grid, hipStream_t s) {
 }
 
 hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s) {
-    const size_t lds = lds_bytes(a.C.obb_cap, a.R.n_dof);
+    const size_t lds = lds_bytes_cost(a.C.obb_cap, a.R.n_dof, a.R.n_joints, a.group, a.grad != 0);
     if (a.grad) hipLaunchKernelGGL(k_cost<true>, dim3(grid), dim3(64), lds, s, a);
     else hipLaunchKernelGGL(k_cost<false>, dim3(grid), dim3(64), lds, s, a);
     return hipGetLastError();
@@ -1094,8 +1201,16 @@ hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s) {
 hipError_t launch_self(const SelfArgs& a, hipStream_t s) {
     const long blocks = (a.X.n_states + kSelfStates - 1) / kSelfStates;
     if (blocks <= 0) return hipSuccess;
-    const size_t smem = sizeof(float) * ((size_t)kSelfStates * a.R.n_sph * 3 + (size_t)kMaxDof * kWave);
+    const size_t smem = sizeof(float4) * (size_t)kSelfStates * (a.R.n_sph + a.R.n_joints) +
+                        sizeof(float) * (size_t)kMaxDof * kWave;
     hipLaunchKernelGGL(k_self, dim3((unsigned)blocks), dim3(64), smem, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fk_out(const DRobot& R, const DStates& X, float* frames, float* spheres, hipStream_t s) {
+    const long blocks = (X.n_states + kWave - 1) / kWave;
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fk_out, dim3((unsigned)blocks), dim3(64), 0, s, R, X, frames, spheres);
     return hipGetLastError();
 }
 

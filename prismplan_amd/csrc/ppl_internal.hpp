@@ -21,12 +21,13 @@
 namespace ppl {
 
 constexpr int kMaxJoints = 16;      // per-lane unrolled joint state (Baxter: 15)
-constexpr int kMaxDepth = 10;      // depth slots of the adjoint's live root path (Fetch uses 9)
 constexpr int kMaxSlots = 2;        // FK transform slots (FK_T_SLOTS in the reference)
 constexpr int kMaxDof = 15;
 constexpr int kWave = 64;
 constexpr int kMaxObbLds = 256;     // OBBs staged per problem
 constexpr int kSelfGroup = 16;      // lanes per state in the self-collision pass
+constexpr int kMaxGroup = 16;       // lanes per state of the cost kernel (power of two)
+constexpr int kGroupKinds = 6;      // chunk tables for 64 / G states, G = 1 .. 32
 
 enum JointType { kFixed = -1, kXPrism = 0, kYPrism = 1, kZPrism = 2, kXRot = 3, kYRot = 4, kZRot = 5 };
 
@@ -41,9 +42,12 @@ struct DRobot {
     const int* apx_off;         // [J+1]
     const float4* apx;          // [A] emission order
     const int* apx_id;          // [A]
-    const int* depth;           // [J] tree depth of DFS position k (root 0)
+    const int* parent;          // [J] DFS position of the parent of DFS position k (root: 0)
+    const int* jorig;           // [J] original joint index of DFS position k
     const float4* sph_orig;     // [S] spheres by original index
-    const int2* pairs;          // [n_pairs] exact self pairs (original indices)
+    const int2* pairs;          // [n_pairs] exact self pairs (original indices), grouped by joint pair
+    const int4* sblk;           // [n_sblk] (DFS joint a, DFS joint b, first pair, end pair) of each group
+    int n_sblk;
     const int2* apx_pairs;      // [n_apx_pairs] approx self pairs (approx original indices)
     const float4* apx_orig;     // [A] approx spheres by original index
 };
@@ -80,11 +84,15 @@ struct DStates {
     // kSrcTraj: waypoint g of trajectory b (traj_off[b] <= g < traj_off[b+1]),
     // theta(b, d, t) = theta[traj_off[b] * D + d * T_b + t]
     const float* theta; const int* traj_off; int n_traj;
+    // Waypoint-major trajectory plans: chunk index v of problem p maps to
+    // trajectory prob[p].x + v % prob[p].y, waypoint v / prob[p].y, where
+    // prob[p] = (first trajectory, trajectories, length, first state).
+    const int4* prob; int wmajor;
 };
 
 // Work chunks for persistent launches: a chunk never spans two problems.
 struct DChunks {
-    const int4* tab;            // (problem, begin, end, 0); end - begin <= chunk
+    const int4* tab;            // (problem, begin, end, first unit or -1); end - begin <= chunk
     int n_chunks;
     int chunk;                  // states per chunk (64 / lanes per state)
     const int* obb_off;         // [P+1] OBB ranges per problem; NULL = single problem
@@ -110,7 +118,7 @@ struct CollideArgs {
 struct CostArgs {
     DRobot R; DScene S; DStates X; DChunks C;
     DCostParams P;
-    int group;                  // lanes per state (1, 2, 4, 8): a state's spheres are split over them
+    int group;                  // lanes per state (power of two <= kMaxGroup): a state's spheres are split over them
     int grad;                   // also compute d cost / d q
     int clearance_only;         // write min ESDF clearance (+margin) instead of cost
     float margin;               // clearance_only: radius margin
@@ -122,6 +130,8 @@ struct CostArgs {
 struct SelfArgs {
     DRobot R; DStates X;
     int mode;                   // 0: any-collision bit, 1: min clearance + argmin pair + normal
+    float band;                 // mode 1: pairs whose joints' enclosing spheres are >= band apart are
+                                // skipped (the minimum is exact whenever it is below band)
     uint8_t* hit_out;           // [n_states]
     float* out;                 // [n_states][8]: clr, i, j (as int bits), nx, ny, nz, -, -
 };
@@ -130,6 +140,9 @@ hipError_t launch_collide(const CollideArgs& a, int grid, hipStream_t s);
 hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s);
 hipError_t launch_self(const SelfArgs& a, hipStream_t s);
 hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t s);
+// Joint frames [n][J][16] (column-major 4x4, original joint order) and / or
+// sphere centres [n][S][4] (x, y, z, r, original order) of configurations.
+hipError_t launch_fk_out(const DRobot& R, const DStates& X, float* frames, float* spheres, hipStream_t s);
 hipError_t launch_segment_sum(const float* x, const int* off, int n_seg, float scale_unused, float* out,
                               hipStream_t s);
 hipError_t launch_control_cost(const float* theta, const int* traj_off, int n_traj, int D, float dt,
@@ -138,6 +151,11 @@ hipError_t launch_esdf_build(const float* obb_dims, const float* obb_pose, int n
                              const float* voxel_params, const float* voxel_pose, float* out,
                              hipStream_t s);
 size_t lds_bytes(int obb_cap, int n_dof);
+// Cost kernel: OBB records + q + (grad) per-state adjoint records.
+size_t lds_bytes_cost(int obb_cap, int n_dof, int n_joints, int G, bool grad);
+// Floats per state of the adjoint records: 12 per joint (odd stride), d cost / d q per dof.
+__host__ __device__ inline int rec_stride(int n_joints) { return 12 * n_joints + 1; }
+__host__ __device__ inline int dq_stride(int n_dof) { return n_dof | 1; }
 int device_cus();
 int grid_collide(size_t lds);
 int grid_cost(int n_joints, bool grad, size_t lds);

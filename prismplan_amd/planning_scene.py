@@ -207,8 +207,8 @@ class PlanningScene:
     def build_scene_collision_data(self):
         """A SceneCollisionDataBuilder over this scene's device tensors, or None
         when the world is empty (planning_scene.py:219-242)."""
-        from .prrtc import SceneCollisionDataBuilder
+        from .prrtc import scene_builder
         ds = self.device_scene()
         if self._n_obbs == 0:
             return None
-        return SceneCollisionDataBuilder.from_device_scene(ds)
+        return scene_builder(ds)

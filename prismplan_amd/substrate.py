@@ -91,6 +91,13 @@ class Robot:
         call("ppl_robot_get_info", self.handle, C.byref(i))
         return {f: getattr(i, f) for f, _ in _ppl.RobotInfo._fields_}
 
+    def flops(self, kind: int, n_obbs=0, n_waypoints=0, use_self=False) -> float:
+        """Algorithmic flops of one work unit (include/ppl/roofline.h)."""
+        v = lib().ppl_algorithmic_flops(self.handle, int(kind), int(n_obbs), int(n_waypoints), int(bool(use_self)))
+        if v < 0:
+            raise ValueError(f"no flop count for work kind {kind}")
+        return float(v)
+
     def close(self):
         if getattr(self, "handle", None):
             lib().ppl_robot_destroy(self.handle)
@@ -219,7 +226,7 @@ class Plan:
     64-state chunks, optionally with one OBB range per problem."""
 
     def __init__(self, robot: Robot, kind: int, n_units: int, unit_states=None, skip_first=False,
-                 problem_units=None, obb_offsets=None):
+                 problem_units=None, obb_offsets=None, waypoint_major=False):
         self.robot = robot
         self.kind = kind
         self.n_units = int(n_units)
@@ -228,7 +235,7 @@ class Plan:
         oo = None if obb_offsets is None else _np(obb_offsets, np.int32)
         n_prob = 1 if pu is None else int(pu.shape[0] - 1)
         d = _ppl.PlanDesc(int(kind), self.n_units, _p(us, _i32p), int(bool(skip_first)), n_prob, _p(pu, _i32p),
-                          _p(oo, _i32p))
+                          _p(oo, _i32p), int(bool(waypoint_major)))
         h = C.c_void_p()
         call("ppl_plan_create", robot.handle, C.byref(d), C.byref(h))
         self.handle = h
@@ -247,9 +254,24 @@ class Plan:
         return Plan(robot, _ppl.PLAN_EDGES, ns.shape[0], ns, skip_first, problem_units, obb_offsets)
 
     @staticmethod
-    def trajectories(robot, lengths, problem_units=None, obb_offsets=None) -> "Plan":
+    def trajectories(robot, lengths, problem_units=None, obb_offsets=None, waypoint_major=False) -> "Plan":
+        """waypoint_major: cut the work waypoint-major inside each problem
+        (rollout sets of equal length, e.g. STOMP), so a wave's states are
+        near-identical configurations. Results are the same either way."""
         ls = _np(lengths, np.int32)
-        return Plan(robot, _ppl.PLAN_TRAJ, ls.shape[0], ls, False, problem_units, obb_offsets)
+        return Plan(robot, _ppl.PLAN_TRAJ, ls.shape[0], ls, False, problem_units, obb_offsets, waypoint_major)
+
+    def autotune_cost(self, scene: "DeviceScene", x: torch.Tensor, params: _ppl.CostParams, grad=False, reps=5,
+                      stream=None):
+        """Time the cost kernel at every lane-group size on these inputs and
+        keep the fastest for this plan; returns (lanes per state, ms per call)."""
+        _dev(x)
+        g = C.c_int(0)
+        ms = C.c_float(0.0)
+        v = scene.view()
+        call("ppl_plan_autotune_cost", self.handle, C.byref(v), _dp(x), C.byref(params), int(bool(grad)), int(reps),
+             C.byref(g), C.byref(ms), _stream(stream))
+        return g.value, ms.value
 
     def close(self):
         if getattr(self, "handle", None):
@@ -294,6 +316,19 @@ def edge_check_batched(plan: Plan, scene: DeviceScene, frm: torch.Tensor, to: to
     call("ppl_edge_check_batched", plan.handle, C.byref(v), _dp(frm), _dp(to), C.byref(p), _dp(first_hit),
          _dp(hits), _stream(stream))
     return (first_hit, hits) if state_hits else first_hit
+
+
+def fk_batched(plan: Plan, configs: torch.Tensor, frames=True, spheres=True, stream=None):
+    """Joint frames [n, J, 4, 4] (row-major view of the column-major output)
+    and sphere centres + radii [n, S, 4] of every configuration."""
+    _dev(configs)
+    info = plan.robot.info()
+    f = torch.empty((plan.n_states, info["n_joints"], 16), dtype=torch.float32, device=configs.device) if frames else None
+    s = torch.empty((plan.n_states, info["n_spheres"], 4), dtype=torch.float32, device=configs.device) if spheres else None
+    call("ppl_fk_batched", plan.handle, _dp(configs), _dp(f), _dp(s), _stream(stream))
+    if f is not None:
+        f = f.view(plan.n_states, info["n_joints"], 4, 4).transpose(-1, -2)
+    return f, s
 
 
 def min_clearance_batched(plan: Plan, scene: DeviceScene, configs: torch.Tensor, collision_margin=0.0,

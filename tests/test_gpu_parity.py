@@ -13,11 +13,14 @@ Tolerances (stated per test):
     shows the same ~3e-4 element-wise worst case), so the vector is the
     unit of comparison.
 """
+from pathlib import Path
+
 import numpy as np
 import pytest
 import torch
 
 import pyoracle as PO
+from prismplan_amd import _ppl
 from prismplan_amd import substrate as S
 from prismplan_amd import workloads as W
 from prismplan_amd.planning_scene import HostScene
@@ -294,13 +297,16 @@ def test_autograd_function(robots):
     torch.testing.assert_close(th.grad, expect, rtol=0, atol=0)
 
 
-def test_stomp_rollout_costs_and_graph(robots):
+@pytest.mark.parametrize("wmajor", [False, True])
+def test_stomp_rollout_costs_and_graph(robots, wmajor):
     wk = W.stomp(n_problems=8, n_rollouts=6, seed=3)
-    banner(f"STOMP rollouts: {wk.n_problems} problems x {wk.n_rollouts} rollouts, states {wk.n_states}")
+    banner(f"STOMP rollouts: {wk.n_problems} problems x {wk.n_rollouts} rollouts, states {wk.n_states}, "
+           f"waypoint-major {wmajor}")
     R = wk.n_rollouts
     traj_lengths = np.repeat(wk.lengths, R)
     pu = np.arange(wk.n_problems + 1) * R
-    plan = S.Plan.trajectories(robots["fetch"], traj_lengths, problem_units=pu, obb_offsets=wk.obb_offsets)
+    plan = S.Plan.trajectories(robots["fetch"], traj_lengths, problem_units=pu, obb_offsets=wk.obb_offsets,
+                               waypoint_major=wmajor)
     ds = S.DeviceScene.from_host(wk.scene)
     ro = torch.from_numpy(wk.rollouts).to(DEV)
     prm = S.cost_params(world_margin=0.02, self_margin=0.01, weight=20.0)
@@ -329,6 +335,33 @@ def test_stomp_rollout_costs_and_graph(robots):
         torch.testing.assert_close(s2, state, rtol=0, atol=0)
         _ = S.stomp_rollout_costs(plan, ds, ro, prm, control_weight=0.1, dt=0.1)
         torch.cuda.synchronize()
+
+
+def test_waypoint_major_plan_matches_unit_major(robots):
+    """Waypoint-major cutting changes only which states share a wave: costs
+    and fused-adjoint gradients equal the unit-major plan's."""
+    wk = W.stomp(n_problems=5, n_rollouts=7, seed=11)
+    R = wk.n_rollouts
+    lengths = np.repeat(wk.lengths, R)
+    pu = np.arange(wk.n_problems + 1) * R
+    ds = S.DeviceScene.from_host(wk.scene)
+    ro = torch.from_numpy(wk.rollouts).to(DEV)
+    prm = S.cost_params(world_margin=0.02, self_margin=0.01, weight=20.0)
+    outs = []
+    for wm in (False, True):
+        plan = S.Plan.trajectories(robots["fetch"], lengths, problem_units=pu, obb_offsets=wk.obb_offsets,
+                                   waypoint_major=wm)
+        outs.append(S.hinge_cost(plan, ds, ro, prm, grad=True))
+    (s0, t0, g0), (s1, t1, g1) = outs
+    torch.testing.assert_close(s1, s0, rtol=1e-6, atol=1e-7)
+    torch.testing.assert_close(t1, t0, rtol=1e-6, atol=1e-6)
+    torch.testing.assert_close(g1, g0, rtol=1e-5, atol=1e-6)
+    banner(f"  waypoint-major == unit-major over {wk.n_states} states")
+    bad = lengths.copy()
+    bad[1] += 1
+    with pytest.raises(_ppl.PplError) as ei:
+        S.Plan.trajectories(robots["fetch"], bad, problem_units=pu, obb_offsets=wk.obb_offsets, waypoint_major=True)
+    assert ei.value.status == _ppl.PPL_ERR_INVALID
 
 
 def test_esdf_build_and_baxter_trilinear_adjoint(robots):
@@ -385,3 +418,114 @@ def test_prrtc_compatible_surface():
     assert not prrtc.check_collision_scene(ready, scene.build_scene_collision_data(), b)[0]
     with pytest.raises(RuntimeError):
         prrtc.check_collision_scene(np.zeros((2, 6), np.float32), scene.build_scene_collision_data(), b)
+
+
+DEVTEST = Path(__file__).resolve().parent / "devapi" / "_build" / "libppl_devtest.so"
+
+
+@pytest.mark.parametrize("name,G", [("panda", 4), ("fetch", 16), ("baxter", 1), ("panda", 64)])
+def test_device_inline_api(robots, name, G):
+    """A planner-style kernel built only on include/ppl/ppl_device.hpp:
+    two_phase_cc (+ self), fk_runtime, scene_min_clearance_runtime,
+    scene_sum_{obb,esdf}_hinge_runtime and self_min_hinge_runtime, one G-lane
+    group per configuration, against the FP64 oracle."""
+    import ctypes as C
+    assert DEVTEST.exists(), f"{DEVTEST} not built (tests/devapi/build_devtest.py)"
+    lib = C.CDLL(str(DEVTEST))
+    rob = robots[name]
+    r = rob.arrays
+    scene, _ = esdf_scene(interp=0, seed=41)
+    rng = np.random.default_rng(43)
+    q = W.uniform_configs(rng, r, 1500)
+    n = len(q)
+    banner(f"device API {name} G={G} n={n}")
+    ds = S.DeviceScene.from_host(scene)
+    view = ds.view()
+    dv = _ppl.RobotDeviceView()
+    _ppl.call("ppl_robot_get_device_view", rob.handle, C.byref(dv))
+    per = 64 // G
+    slots = (n + per - 1) // per * per
+    dq = torch.from_numpy(q).to(DEV)
+    hit = torch.zeros(n, dtype=torch.uint8, device=DEV)
+    outs = [torch.zeros(n, dtype=torch.float32, device=DEV) for _ in range(4)]
+    scratch = torch.zeros(slots * 3 * (r.n_spheres + r.n_approx), dtype=torch.float32, device=DEV)
+    vp = C.c_void_p
+    lib.ppl_devtest_run.restype = C.c_int
+    lib.ppl_devtest_run.argtypes = [C.POINTER(_ppl.RobotDeviceView), C.POINTER(_ppl.SceneView), vp, C.c_int, C.c_int,
+                                    C.c_float, C.c_int, C.c_float, C.c_float, vp, vp, vp, vp, vp, vp, vp]
+    st = lib.ppl_devtest_run(C.byref(dv), C.byref(view), dq.data_ptr(), n, G, 0.0, 1, 0.03, 0.01, hit.data_ptr(),
+                             *[o.data_ptr() for o in outs], scratch.data_ptr(), None)
+    assert st == 0
+    torch.cuda.synchronize()
+    mc, ho, he, sh = [o.cpu().numpy().astype(np.float64) for o in outs]
+    o = PO.Oracle(r, scene, nthreads=8)
+    ref, clr = o.check_collision(q, check_self=True)
+    assert_verdicts(hit.cpu().numpy(), ref, clr, f"device two_phase_cc {name}")
+    ref_mc = o.min_clearance(q)
+    np.testing.assert_allclose(mc, ref_mc, rtol=1e-5, atol=1e-5)
+    # cost tolerance 1e-5 relative, denominator max(|ref|, weight * margin) as in the other hinge tests
+    for what, prm, got, scale in (("obb hinge", PO.cost_params(world_margin=0.03, use_obb=True, use_esdf=False,
+                                                               use_self=False), ho, 0.03),
+                                  ("esdf hinge", PO.cost_params(world_margin=0.03, use_obb=False, use_esdf=True,
+                                                                use_self=False), he, 0.03),
+                                  ("self hinge", PO.cost_params(self_margin=0.01, use_obb=False, use_esdf=False,
+                                                                use_self=True), sh, 0.01)):
+        exp = o.hinge_cost(q, prm)
+        err = np.abs(got - exp) / np.maximum(np.abs(exp), scale)
+        banner(f"  {what}: max rel err {err.max():.2e} (denominator max(|ref|, {scale}))")
+        assert err.max() < 1e-5, what
+
+
+def test_algorithmic_flop_counts(robots):
+    """ppl_algorithmic_flops (include/ppl/roofline.h) reproduces the hand
+    counts of the benchmark workloads (FMA = 2, sqrt/sin/cos = 1, abs = 0)."""
+    P, F, B = robots["panda"], robots["fetch"], robots["baxter"]
+    assert P.flops(_ppl.WORK_ANY_COLLISION, n_obbs=20) == 7 * 83 + 59 * 18 + 59 * 20 * 30 == 37043
+    assert P.flops(_ppl.WORK_ANY_COLLISION, n_obbs=64) == 114923
+    fwd = 581 + 1062 + 59 * 40 * 36 + 59 * 3
+    assert P.flops(_ppl.WORK_CHOMP_ADJOINT, n_obbs=40, n_waypoints=64) == 64 * fwd + 62 * (59 * 35 + 7 * 25 + 1643)
+    assert P.flops(_ppl.WORK_CHOMP_FD, n_obbs=40, n_waypoints=64) == (2 * 7 * 62 + 64) * fwd
+    assert F.flops(_ppl.WORK_STOMP_STATE, n_obbs=55, use_self=True) == 251199
+    assert B.flops(_ppl.WORK_ESDF_WAYPOINT) == 8137 + 5487
+    with pytest.raises(ValueError):
+        P.flops(99)
+
+
+def test_autotuned_tile_keeps_results(robots):
+    """The cost-kernel autotuner picks a lane-group size that fits, and the
+    plan's results under it equal the heuristic's (summation order aside)."""
+    w = W.chomp(n_traj=24, n_wp=40, n_obbs=30, seed=5)
+    ds = S.DeviceScene.from_host(w.scene)
+    plan = S.Plan.trajectories(robots["panda"], w.lengths())
+    th = torch.from_numpy(w.theta).to(DEV)
+    prm = S.cost_params(world_margin=0.05, weight=1.0, use_self=False)
+    s0, t0, g0 = S.hinge_cost(plan, ds, th, prm, grad=True)
+    G, ms = plan.autotune_cost(ds, th, prm, grad=True, reps=3)
+    banner(f"autotuned lanes per state {G} ({ms * 1e3:.1f} us per call)")
+    assert G in (1, 2, 4, 8, 16) and ms > 0
+    s1, t1, g1 = S.hinge_cost(plan, ds, th, prm, grad=True)
+    torch.testing.assert_close(s1, s0, rtol=1e-5, atol=1e-6)
+    torch.testing.assert_close(t1, t0, rtol=1e-5, atol=1e-5)
+    torch.testing.assert_close(g1, g0, rtol=1e-4, atol=1e-5)
+
+
+@pytest.mark.parametrize("name", ["panda", "fetch", "baxter"])
+def test_fk_frames_and_spheres(robots, name):
+    """ppl_fk_batched vs the FP64 oracle: joint frames (column-major per joint,
+    original joint order) and sphere centres, within 2e-6 m / 2e-6."""
+    rob = robots[name]
+    r = rob.arrays
+    q = W.uniform_configs(np.random.default_rng(51), r, 700)
+    plan = S.Plan.configs(rob, len(q))
+    f, s = S.fk_batched(plan, torch.from_numpy(q).to(DEV))
+    o = PO.Oracle(r)
+    ref_f = o.fk_frames(q)
+    ref_s = o.fk_spheres(q)
+    got_f = f.cpu().numpy().astype(np.float64)
+    got_s = s.cpu().numpy().astype(np.float64)
+    err_f = np.abs(got_f[:, :, :3, :] - ref_f).max()
+    err_s = np.abs(got_s[..., :3] - ref_s).max()
+    banner(f"FK {name}: frame max err {err_f:.2e}, sphere max err {err_s:.2e}")
+    assert err_f < 2e-6 and err_s < 2e-6
+    np.testing.assert_array_equal(got_f[:, :, 3, :], np.broadcast_to([0, 0, 0, 1], got_f[:, :, 3, :].shape))
+    np.testing.assert_array_equal(got_s[..., 3], np.broadcast_to(r.spheres.reshape(-1, 4)[:, 3], got_s[..., 3].shape))
