@@ -307,12 +307,23 @@ def bench_chomp(panda, S, W, torch, dev, args):
     tuned = plan.autotune_cost(ds, th, prm, grad=True)
     wall_a, ev_a = timed_loop(lambda: S.hinge_cost(plan, ds, th, prm, grad=True), args.steps, args.warmup, torch)
     wall_f, ev_f = timed_loop(lambda: S.chomp_fd_grad(plan, ds, th, prm, eps=1e-3), max(5, args.steps // 5), 2, torch)
+    # the reference's call pattern: one FD launch per trajectory (plain per-trajectory form)
+    D, Tw = panda.n_dof, int(w.lengths()[0])
+    views = [th[b * D * Tw:(b + 1) * D * Tw] for b in range(w.n_traj)]
+    S.chomp_fd_grad_plain(panda, ds, views[0], Tw, prm, eps=1e-3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for v in views:
+        S.chomp_fd_grad_plain(panda, ds, v, Tw, prm, eps=1e-3)
+    torch.cuda.synchronize()
+    wall_fp = (time.perf_counter() - t0)
     from prismplan_amd import _ppl
     B, T, O = w.n_traj, int(w.lengths()[0]), w.scene.n_obbs
     f_adj = panda.flops(_ppl.WORK_CHOMP_ADJOINT, n_obbs=O, n_waypoints=T)
     f_fd = panda.flops(_ppl.WORK_CHOMP_FD, n_obbs=O, n_waypoints=T)
     return {"adjoint_traj_per_s": B / wall_a, "fd_traj_per_s": B / wall_f, "adjoint_over_fd": wall_f / wall_a,
             "adjoint_ms": wall_a * 1e3, "fd_ms": wall_f * 1e3,
+            "fd_per_trajectory_launch_traj_per_s": B / wall_fp, "adjoint_over_fd_per_trajectory": wall_fp / wall_a,
             "adjoint_flops_per_traj": f_adj, "fd_flops_per_traj": f_fd, "adjoint_lanes_per_state": tuned[0],
             "adjoint_roofline_frac": B * f_adj / ev_a / PEAK_FP32,
             "fd_roofline_frac": B * f_fd / ev_f / PEAK_FP32}

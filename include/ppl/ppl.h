@@ -263,6 +263,11 @@ ppl_status ppl_plan_autotune_cost(ppl_plan plan, const ppl_scene_view* scene, co
                                   const ppl_cost_params* params, int grad, int reps, int* best_group,
                                   float* best_ms, ppl_stream stream);
 
+/* Pin the cost kernel's lanes per state for this plan (grad: the adjoint
+ * variant); 0 returns to the built-in choice. Power of two <= 16 that fits
+ * in LDS. */
+ppl_status ppl_plan_set_cost_group(ppl_plan plan, int grad, int lanes_per_state);
+
 /* Central-FD obstacle gradient over a trajectory plan (chomp_obs_grad_kernel
  * semantics), the baseline for the fused adjoint. */
 ppl_status ppl_chomp_fd_grad_batched(ppl_plan plan, const ppl_scene_view* scene, const float* d_theta,

@@ -96,6 +96,7 @@ struct ppl_robot_s {
     int n_pairs = 0;
     int n_revolute = 0, n_prismatic = 0;   // joints that move with a dof
     std::vector<float> lower, upper;
+    std::vector<std::pair<int, ppl_plan>> traj_plans;   // single-trajectory plans of the plain CHOMP forms, by T
     hipStream_t stream = nullptr;
     DBuf d_cfg, d_out, d_off, d_aux, d_self;
     HBuf h_stage;
@@ -531,6 +532,8 @@ ppl_status ppl_robot_get_device_view(ppl_robot r, ppl_robot_device_view* v) {
 
 ppl_status ppl_robot_destroy(ppl_robot r) {
     if (!r) return PPL_OK;
+    for (auto& e : r->traj_plans) ppl_plan_destroy(e.second);
+    r->traj_plans.clear();
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     if (r->blob) (void)hipFree(r->blob);
     r->d_cfg.release(); r->d_out.release(); r->d_off.release(); r->d_aux.release(); r->d_self.release();
@@ -1221,6 +1224,21 @@ ppl_status ppl_plan_autotune_cost(ppl_plan P, const ppl_scene_view* view, const 
     return PPL_OK;
 }
 
+ppl_status ppl_plan_set_cost_group(ppl_plan P, int grad, int lanes_per_state) {
+    CHECK_ARG(P, "NULL argument");
+    CHECK_ARG(lanes_per_state == 0 || (lanes_per_state >= 1 && lanes_per_state <= kMaxGroup &&
+                                       (lanes_per_state & (lanes_per_state - 1)) == 0),
+              "lanes_per_state must be 0 or a power of two in [1, %d]", kMaxGroup);
+    if (lanes_per_state > 0) {
+        const int cap = P->h_obb_off.empty() ? kMaxObbLds : 0;
+        int need = cap;
+        for (size_t p = 0; p + 1 < P->h_obb_off.size(); ++p) need = std::max(need, P->h_obb_off[p + 1] - P->h_obb_off[p]);
+        CHECK_ARG(group_fits(P, lanes_per_state, grad != 0, need), "tile does not fit in LDS");
+    }
+    P->tuned_group[grad ? 1 : 0] = lanes_per_state;
+    return PPL_OK;
+}
+
 ppl_status ppl_hinge_cost_batched(ppl_plan P, const ppl_scene_view* view, const float* d_x, const ppl_cost_params* prm,
                                   float* d_state, float* d_total, float* d_grad, ppl_stream stream) {
     return hinge_impl(P, view, d_x, prm, d_state, d_total, d_grad, (hipStream_t)stream);
@@ -1240,13 +1258,19 @@ ppl_status ppl_stomp_rollout_costs(ppl_plan P, const ppl_scene_view* view, const
 }  // extern "C"
 
 namespace {
+// Single-trajectory plan of length T, created once per robot and T and kept
+// with the robot (the plain per-trajectory forms then allocate nothing).
 ppl_status single_traj_plan(ppl_robot r, int T, ppl_plan* out) {
+    for (auto& e : r->traj_plans)
+        if (e.first == T) { *out = e.second; return PPL_OK; }
     ppl_plan_desc d{};
     d.kind = PPL_PLAN_TRAJ;
     d.n_units = 1;
     d.unit_states = &T;
     d.n_problems = 1;
-    return ppl_plan_create(r, &d, out);
+    ppl_status st = ppl_plan_create(r, &d, out);
+    if (st == PPL_OK) r->traj_plans.emplace_back(T, *out);
+    return st;
 }
 
 __global__ void k_broadcast_rows(float* x, int D, int T) {
@@ -1273,7 +1297,6 @@ ppl_status ppl_chomp_state_cost(ppl_robot r, const ppl_scene_view* view, const f
         hipError_t e = hipStreamSynchronize(s);
         if (e != hipSuccess) st = fail(PPL_ERR_HIP, "chomp_state_cost: %s", hipGetErrorString(e));
     }
-    ppl_plan_destroy(P);
     return st;
 }
 
@@ -1290,7 +1313,6 @@ ppl_status ppl_chomp_fd_grad(ppl_robot r, const ppl_scene_view* view, const floa
         hipError_t e = hipStreamSynchronize(s);
         if (e != hipSuccess) st = fail(PPL_ERR_HIP, "chomp_fd_grad: %s", hipGetErrorString(e));
     }
-    ppl_plan_destroy(P);
     return st;
 }
 

@@ -273,6 +273,10 @@ class Plan:
              C.byref(g), C.byref(ms), _stream(stream))
         return g.value, ms.value
 
+    def set_cost_group(self, lanes_per_state: int, grad=False):
+        """Pin the cost kernel's lanes per state (0: built-in choice)."""
+        call("ppl_plan_set_cost_group", self.handle, int(bool(grad)), int(lanes_per_state))
+
     def close(self):
         if getattr(self, "handle", None):
             lib().ppl_plan_destroy(self.handle)
@@ -316,6 +320,30 @@ def edge_check_batched(plan: Plan, scene: DeviceScene, frm: torch.Tensor, to: to
     call("ppl_edge_check_batched", plan.handle, C.byref(v), _dp(frm), _dp(to), C.byref(p), _dp(first_hit),
          _dp(hits), _stream(stream))
     return (first_hit, hits) if state_hits else first_hit
+
+
+def chomp_state_cost(robot: Robot, scene: DeviceScene, theta: torch.Tensor, T: int, params: _ppl.CostParams,
+                     stream=None) -> torch.Tensor:
+    """Plain per-trajectory state cost (chomp_state_cost_kernel semantics):
+    theta [D*T] dim-major -> [D, T], the per-waypoint cost broadcast over D."""
+    _dev(theta)
+    out = torch.empty((robot.n_dof, T), dtype=torch.float32, device=theta.device)
+    v = scene.view()
+    call("ppl_chomp_state_cost", robot.handle, C.byref(v), _dp(theta), int(T), C.byref(params), _dp(out),
+         _stream(stream))
+    return out
+
+
+def chomp_fd_grad_plain(robot: Robot, scene: DeviceScene, theta: torch.Tensor, T: int, params: _ppl.CostParams,
+                        eps=1e-3, stream=None) -> torch.Tensor:
+    """Plain per-trajectory central-FD obstacle gradient (chomp_obs_grad_kernel
+    semantics): theta [D*T] dim-major -> [D, T], zero at both ends."""
+    _dev(theta)
+    out = torch.empty((robot.n_dof, T), dtype=torch.float32, device=theta.device)
+    v = scene.view()
+    call("ppl_chomp_fd_grad", robot.handle, C.byref(v), _dp(theta), int(T), C.byref(params), C.c_float(eps), _dp(out),
+         _stream(stream))
+    return out
 
 
 def fk_batched(plan: Plan, configs: torch.Tensor, frames=True, spheres=True, stream=None):

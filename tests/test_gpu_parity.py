@@ -529,3 +529,28 @@ def test_fk_frames_and_spheres(robots, name):
     assert err_f < 2e-6 and err_s < 2e-6
     np.testing.assert_array_equal(got_f[:, :, 3, :], np.broadcast_to([0, 0, 0, 1], got_f[:, :, 3, :].shape))
     np.testing.assert_array_equal(got_s[..., 3], np.broadcast_to(r.spheres.reshape(-1, 4)[:, 3], got_s[..., 3].shape))
+
+
+def test_plain_chomp_forms(robots):
+    """Per-trajectory forms (chomp_state_cost_kernel / chomp_obs_grad_kernel
+    semantics) against the FP64 oracle and the batched forms."""
+    w = W.chomp(n_traj=3, n_wp=48, n_obbs=30, seed=17)
+    panda = robots["panda"]
+    ds = S.DeviceScene.from_host(w.scene)
+    prm = S.cost_params(world_margin=0.05, weight=2.0, use_self=True)
+    o = PO.Oracle(w.robot, w.scene, nthreads=8)
+    D, T = panda.n_dof, 48
+    plan = S.Plan.trajectories(panda, w.lengths())
+    th_all = torch.from_numpy(w.theta).to(DEV)
+    fd_b = S.chomp_fd_grad(plan, ds, th_all, prm, eps=1e-3).cpu().numpy()
+    for b in range(w.n_traj):
+        th = th_all[b * D * T:(b + 1) * D * T]
+        sc = S.chomp_state_cost(panda, ds, th, T, prm).cpu().numpy()
+        q = th.cpu().numpy().reshape(D, T).T
+        ref = o.hinge_cost(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0))
+        rel = np.abs(sc - ref[None, :]) / np.maximum(np.abs(ref[None, :]), 2.0 * 0.05)
+        assert rel.max() < 1e-5
+        fd = S.chomp_fd_grad_plain(panda, ds, th, T, prm, eps=1e-3).cpu().numpy()
+        np.testing.assert_array_equal(fd.reshape(-1), fd_b[b * D * T:(b + 1) * D * T])
+        assert np.all(fd[:, 0] == 0) and np.all(fd[:, -1] == 0)
+    banner(f"plain CHOMP forms: {w.n_traj} trajectories, state cost vs oracle and FD == batched FD")
