@@ -82,7 +82,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
-    ap.add_argument("--legs", default="skip,any,edges_plain,chomp,stomp,esdf")
+    ap.add_argument("--legs", default="skip,any,edges_plain,chomp,stomp,chomp_opt,esdf")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     legs = set(x for x in args.legs.split(",") if x)
@@ -142,6 +142,7 @@ def main():
                 ("edges_plain", "edges_plain_vs_batched", lambda: bench_edges_plain(panda, e, ds, S, torch, dev, args)),
                 ("chomp", "chomp", lambda: bench_chomp(panda, S, W, torch, dev, args)),
                 ("stomp", "stomp", lambda: bench_stomp(S, W, torch, dev, args)),
+                ("chomp_opt", "chomp_optimizer", lambda: bench_chomp_opt(panda, S, W, torch, dev, args)),
                 ("esdf", "esdf", lambda: bench_esdf(S, W, torch, dev, args)),
                 ("copy", "copy", lambda: bench_copy(torch, dev, args))):
             if key not in legs:
@@ -327,6 +328,31 @@ def bench_chomp(panda, S, W, torch, dev, args):
             "adjoint_flops_per_traj": f_adj, "fd_flops_per_traj": f_fd, "adjoint_lanes_per_state": tuned[0],
             "adjoint_roofline_frac": B * f_adj / ev_a / PEAK_FP32,
             "fd_roofline_frac": B * f_fd / ev_f / PEAK_FP32}
+
+
+def bench_chomp_opt(panda, S, W, torch, dev, args):
+    """The batched CHOMP optimiser on the CHOMP workload (256 trajectories x 64
+    waypoints, 40 OBBs): iterations of adjoint gradient, covariant step,
+    cost, densified validity and bookkeeping, all on the device."""
+    from prismplan_amd import _ppl
+    w = W.chomp()
+    ds = S.DeviceScene.from_host(w.scene)
+    plan = S.Plan.trajectories(panda, w.lengths())
+    st = S.chomp_settings(world_margin=0.05)
+    th = torch.from_numpy(w.theta.copy()).to(dev)
+    opt = S.ChompOptimizer(plan, ds, th, st).step(0)
+    opt.step(2)
+    torch.cuda.synchronize()
+    iters = 20
+    t0 = time.perf_counter()
+    opt.step(iters)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rec = opt.record.cpu()
+    return {"iterations": iters, "ms_per_iteration": dt / iters * 1e3,
+            "trajectory_iterations_per_s": w.n_traj * iters / dt,
+            "solved": int(rec[:, _ppl.CHOMP_SOLVED].sum()), "trajectories": w.n_traj,
+            "done": int(rec[:, _ppl.CHOMP_DONE].sum())}
 
 
 def bench_stomp(S, W, torch, dev, args):

@@ -281,6 +281,58 @@ ppl_status ppl_stomp_rollout_costs(ppl_plan plan, const ppl_scene_view* scene, c
                                    const ppl_cost_params* params, float control_weight, float dt,
                                    float* d_state_cost, float* d_total, ppl_stream stream);
 
+/* ------------------------------------------------------------------------
+ * Batched CHOMP optimiser on the substrate (planning/chomp.cu:358-610
+ * semantics, every trajectory of a trajectory plan at once, no host
+ * round trip per iteration). Trajectories are the plan's [D][T] blocks; both
+ * endpoints stay fixed. Per iteration and trajectory:
+ *   obstacle gradient (fused adjoint, or central FD as chomp.cu:83-144),
+ *   covariant step theta_d -= step_size * A^-1 (g_d + smoothness_weight R theta_d)
+ *     with A = R + 1e-8 I (chomp_precompute.cpp:20-64; A^-1 is formed in FP64
+ *     on the host, stored FP32), joint-limit clamp (chomp.cu:261-290);
+ *   cost = sum_t weight (world + self hinge) + smoothness_weight theta^T R theta;
+ *   validity = two-phase any-collision with self check at every waypoint and
+ *     ceil(max segment / collision_check_step) sub-states per segment
+ *     (chomp.cu:297-351, 459-481);
+ *   best-valid tracking, convergence streak and polish count (chomp.cu:484-557).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    float step_size;               /* 0.3 */
+    float smoothness_weight;       /* lambda, 1.0 */
+    float collision_cost_weight;   /* 20 */
+    float world_margin;            /* OBB / ESDF hinge band, 0.02 */
+    float self_margin;             /* self hinge band, 0.01 */
+    float dt;                      /* 0.1 */
+    float collision_check_step;    /* validity densification step (rad), 0.05; <= 0: waypoints only */
+    float collision_margin;        /* validity check margin, 0 */
+    float convergence_eps;         /* 1e-4 */
+    int iterations_after_valid;    /* polish iterations once a valid trajectory exists, 5 */
+    int use_self;                  /* self hinge in the cost */
+    int fd_gradient;               /* 0: fused adjoint; 1: central finite differences */
+    float fd_eps;                  /* 1e-3 */
+} ppl_chomp_settings;
+
+/* Per-trajectory record of the optimiser, 8 floats per trajectory. */
+enum {
+    PPL_CHOMP_BEST_COST = 0,       /* best valid total cost (+inf until valid) */
+    PPL_CHOMP_PREV_COST = 1,
+    PPL_CHOMP_SOLVED = 2,          /* 1: d_best holds a valid trajectory */
+    PPL_CHOMP_STREAK = 3,
+    PPL_CHOMP_POLISH_LEFT = 4,
+    PPL_CHOMP_DONE = 5,            /* 1: converged or polished, no further updates */
+    PPL_CHOMP_ITERS = 6,           /* iterations run */
+    PPL_CHOMP_LAST_COST = 7        /* total cost of the current d_theta */
+};
+
+/* Runs `iterations` CHOMP iterations on every trajectory of `plan`. d_theta
+ * [plan layout] is updated in place; d_best [plan layout] receives the best
+ * valid trajectory of each; d_record [n_traj][8] as above. first_call = 1
+ * initialises the records and evaluates d_theta before the first step (the
+ * reference's initial cost / validity pass); later calls continue. */
+ppl_status ppl_chomp_optimize(ppl_plan plan, const ppl_scene_view* scene, const ppl_chomp_settings* settings,
+                              float* d_theta, float* d_best, float* d_record, int iterations, int first_call,
+                              ppl_stream stream);
+
 /* ESDF of a box world on a node grid: value = -min_o sdf_o(node), positive
  * inside. h_voxel_params [4] and h_voxel_pose [8] on the host; d_features
  * holds nx*ny*nz floats, index ix*ny*nz + iy*nz + iz. */

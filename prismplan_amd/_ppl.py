@@ -79,6 +79,18 @@ class PlanDesc(C.Structure):
                 ("waypoint_major", C.c_int)]
 
 
+class ChompSettings(C.Structure):
+    _fields_ = [("step_size", C.c_float), ("smoothness_weight", C.c_float), ("collision_cost_weight", C.c_float),
+                ("world_margin", C.c_float), ("self_margin", C.c_float), ("dt", C.c_float),
+                ("collision_check_step", C.c_float), ("collision_margin", C.c_float), ("convergence_eps", C.c_float),
+                ("iterations_after_valid", C.c_int), ("use_self", C.c_int), ("fd_gradient", C.c_int),
+                ("fd_eps", C.c_float)]
+
+
+CHOMP_BEST_COST, CHOMP_PREV_COST, CHOMP_SOLVED, CHOMP_STREAK, CHOMP_POLISH_LEFT, CHOMP_DONE, CHOMP_ITERS, \
+    CHOMP_LAST_COST = range(8)
+
+
 class EdgeBatchParams(C.Structure):
     _fields_ = [("collision_margin", C.c_float), ("check_self", C.c_int), ("skip_resolved", C.c_int)]
 
@@ -116,6 +128,8 @@ SIGNATURES = [
                                             _vp]),
     ("ppl_stomp_rollout_costs", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), C.c_float, C.c_float,
                                           _vp, _vp, _vp]),
+    ("ppl_chomp_optimize", C.c_int, [_vp, C.POINTER(SceneView), C.POINTER(ChompSettings), _vp, _vp, _vp, C.c_int,
+                                     C.c_int, _vp]),
     ("ppl_esdf_from_obbs", C.c_int, [C.POINTER(SceneView), _f32p, _f32p, _vp, _vp]),
     ("ppl_graph_capture_stomp", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), C.c_float,
                                           C.c_float, _vp, _vp, _vp, C.POINTER(_vp)]),

@@ -129,6 +129,12 @@ struct ppl_plan_s {
     int tuned_group[2] = {0, 0};     // cost kernel lanes per state chosen by ppl_plan_autotune_cost (0: heuristic)
     int4* d_prob = nullptr;          // wmajor: per problem (first trajectory, trajectories, length, first state)
     DBuf d_self, d_cost, d_fdcfg, d_fdcost, d_tune;
+    // CHOMP optimiser: A^-1 (FP32, T x T) per trajectory length, the
+    // per-trajectory pointers to them, and scratch for gradient, costs, flags
+    std::vector<std::pair<int, float*>> ainv;
+    const float** d_ainv_ptr = nullptr;
+    float ainv_dt = 0.0f;
+    DBuf d_opt;
 };
 
 struct ppl_graph_s {
@@ -141,8 +147,11 @@ namespace {
 ppl_status scene_from_view(const ppl_scene_view* v, DScene& S) {
     CHECK_ARG(v != nullptr, "scene is NULL");
     CHECK_ARG(v->n_obbs >= 0 && v->n_voxel_layers >= 0, "negative scene sizes");
-    if (v->n_obbs > 0)
+    if (v->n_obbs > 0) {
         CHECK_ARG(v->obb_dims && v->obb_pose && v->obb_enable, "scene has %d OBBs but NULL OBB arrays", v->n_obbs);
+        CHECK_ARG(((uintptr_t)v->obb_dims & 15) == 0 && ((uintptr_t)v->obb_pose & 15) == 0,
+                  "obb_dims / obb_pose must be 16-byte aligned (rows are read as float4)");
+    }
     if (v->n_voxel_layers > 0)
         CHECK_ARG(v->voxel_params && v->voxel_pose && v->voxel_enable && v->voxel_features && v->max_voxels_per_layer > 0,
                   "scene has voxel layers but NULL voxel arrays");
@@ -283,13 +292,14 @@ int edge_g(const float* a, const float* b, int dim, const ppl_edge_params& p, fl
 // Self pass. mode 0: any-collision; mode 1: min clearance, exact below `band`
 // (hinge callers pass their self margin).
 ppl_status run_self(const ppl_robot_s* r, const DStates& X, int mode, uint8_t* hit_out, float* out, hipStream_t s,
-                    float band = FLT_MAX) {
+                    float band = FLT_MAX, int* unit_first_hit = nullptr) {
     SelfArgs a{};
     a.R = r->R;
     a.X = X;
     a.X.wmajor = 0;      // the self pass walks states in plan order
     a.mode = mode;
     a.band = band;
+    a.unit_first_hit = unit_first_hit;
     a.hit_out = hit_out;
     a.out = out;
     HIP_TRY(launch_self(a, s));
@@ -442,7 +452,7 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
                  o_apx = place(apx.size() * 16), o_aid = place(apx_id.size() * 4), o_close = place(close.size() * 4),
                  o_sorig = place(sph_orig.size() * 16), o_pairs = place(pairs.size() * 8),
                  o_apairs = place(apairs.size() * 8), o_aorig = place(apx_orig.size() * 16),
-                 o_sblk = place(sblk.size() * 16), o_jorig = place(J * 4);
+                 o_sblk = place(sblk.size() * 16), o_jorig = place(J * 4), o_lim = place((size_t)D * 8);
     std::vector<char> host(off + 16, 0);
     auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) memcpy(host.data() + o, src, bytes); };
     put(o_ft, ft.data(), ft.size() * 4);
@@ -460,6 +470,9 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
     put(o_aorig, apx_orig.data(), apx_orig.size() * 16);
     put(o_sblk, sblk.data(), sblk.size() * 16);
     put(o_jorig, dfs.data(), (size_t)J * 4);
+    std::vector<float2> lim(D);
+    for (int k = 0; k < D; ++k) lim[k] = make_float2(d->joint_lower[k], d->joint_upper[k]);
+    put(o_lim, lim.data(), (size_t)D * 8);
 
     ppl_robot_s* r = new (std::nothrow) ppl_robot_s();
     if (!r) return fail(PPL_ERR_ALLOC, "out of host memory");
@@ -492,6 +505,7 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
     R.apx_orig = reinterpret_cast<const float4*>(b + o_aorig);
     R.sblk = reinterpret_cast<const int4*>(b + o_sblk);
     R.jorig = reinterpret_cast<const int*>(b + o_jorig);
+    R.lim = reinterpret_cast<const float2*>(b + o_lim);
     R.n_sblk = (int)sblk.size();
     r->n_slots = n_slots;
     r->n_pairs = (int)pairs.size();
@@ -556,8 +570,12 @@ ppl_status ppl_robot_get_info(ppl_robot r, ppl_robot_info* info) {
 
 ppl_status ppl_robot_update_joint_limits(ppl_robot r, const float* lo, const float* hi) {
     CHECK_ARG(r && lo && hi, "NULL argument");
-    r->lower.assign(lo, lo + r->R.n_dof);
-    r->upper.assign(hi, hi + r->R.n_dof);
+    const int D = r->R.n_dof;
+    r->lower.assign(lo, lo + D);
+    r->upper.assign(hi, hi + D);
+    std::vector<float2> lim(D);
+    for (int k = 0; k < D; ++k) lim[k] = make_float2(lo[k], hi[k]);
+    HIP_TRY(hipMemcpy(const_cast<float2*>(r->R.lim), lim.data(), (size_t)D * sizeof(float2), hipMemcpyHostToDevice));
     return PPL_OK;
 }
 
@@ -880,6 +898,9 @@ ppl_status ppl_plan_destroy(ppl_plan P) {
     if (P->d_off) (void)hipFree(P->d_off);
     if (P->d_obb_off) (void)hipFree(P->d_obb_off);
     if (P->d_prob) (void)hipFree(P->d_prob);
+    for (auto& e : P->ainv) (void)hipFree(e.second);
+    if (P->d_ainv_ptr) (void)hipFree(P->d_ainv_ptr);
+    P->d_opt.release();
     P->d_self.release(); P->d_cost.release(); P->d_fdcfg.release(); P->d_fdcost.release();
     delete P;
     return PPL_OK;
@@ -1076,9 +1097,198 @@ ppl_status stomp_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_
     return PPL_OK;
 }
 
+// (R + 1e-8 I)^-1 in FP64 for the 5-point acceleration metric R = dt (A^T A)
+// restricted to T x T from a grid padded by 6 (chomp_precompute.cpp:20-64),
+// by Cholesky factorisation; stored FP32 row-major.
+bool chomp_ainv(int T, float dt, std::vector<float>& out) {
+    const int pad = 6, N = T + 2 * pad;
+    const double c[5] = {-1.0 / 12.0, 16.0 / 12.0, -30.0 / 12.0, 16.0 / 12.0, -1.0 / 12.0};
+    const double inv_dt2 = 1.0 / ((double)dt * (double)dt);
+    // R[i][j] = dt * sum_r A[r][i + pad] A[r][j + pad], A[r][k] = inv_dt2 c[k - r + 2]
+    std::vector<double> R((size_t)T * T, 0.0);
+    for (int i = 0; i < T; ++i)
+        for (int j = 0; j < T; ++j) {
+            double acc = 0.0;
+            for (int r = 0; r < N; ++r) {
+                const int a = i + pad - r + 2, b = j + pad - r + 2;
+                if (a < 0 || a > 4 || b < 0 || b > 4) continue;
+                acc += c[a] * c[b];
+            }
+            R[(size_t)i * T + j] = (double)dt * inv_dt2 * inv_dt2 * acc + (i == j ? 1e-8 : 0.0);
+        }
+    // Cholesky R = L L^T, then the columns of R^-1 by two triangular solves
+    std::vector<double> L((size_t)T * T, 0.0);
+    for (int j = 0; j < T; ++j) {
+        double d = R[(size_t)j * T + j];
+        for (int k = 0; k < j; ++k) d -= L[(size_t)j * T + k] * L[(size_t)j * T + k];
+        if (!(d > 0.0)) return false;
+        const double ljj = std::sqrt(d);
+        L[(size_t)j * T + j] = ljj;
+        for (int i = j + 1; i < T; ++i) {
+            double v = R[(size_t)i * T + j];
+            for (int k = 0; k < j; ++k) v -= L[(size_t)i * T + k] * L[(size_t)j * T + k];
+            L[(size_t)i * T + j] = v / ljj;
+        }
+    }
+    out.assign((size_t)T * T, 0.0f);
+    std::vector<double> y(T), x(T);
+    for (int col = 0; col < T; ++col) {
+        for (int i = 0; i < T; ++i) {
+            double v = i == col ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) v -= L[(size_t)i * T + k] * y[k];
+            y[i] = v / L[(size_t)i * T + i];
+        }
+        for (int i = T - 1; i >= 0; --i) {
+            double v = y[i];
+            for (int k = i + 1; k < T; ++k) v -= L[(size_t)k * T + i] * x[k];
+            x[i] = v / L[(size_t)i * T + i];
+        }
+        for (int i = 0; i < T; ++i) out[(size_t)i * T + col] = (float)x[i];
+    }
+    return true;
+}
+
+// A^-1 of every trajectory length of the plan for step dt, and the
+// per-trajectory pointer table the update kernel reads.
+ppl_status ensure_ainv(ppl_plan_s* P, float dt) {
+    if (P->d_ainv_ptr && P->ainv_dt == dt) return PPL_OK;
+    for (auto& e : P->ainv) (void)hipFree(e.second);
+    P->ainv.clear();
+    std::vector<const float*> ptr(P->n_units);
+    for (int u = 0; u < P->n_units; ++u) {
+        const int T = P->h_off[u + 1] - P->h_off[u];
+        float* d = nullptr;
+        for (auto& e : P->ainv)
+            if (e.first == T) d = e.second;
+        if (!d) {
+            std::vector<float> h;
+            if (!chomp_ainv(T, dt, h))
+                return fail(PPL_ERR_INVALID, "smoothness metric of length %d is not positive definite", T);
+            HIP_TRY(hipMalloc((void**)&d, h.size() * sizeof(float)));
+            P->ainv.emplace_back(T, d);
+            HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
+        ptr[u] = d;
+    }
+    if (!P->d_ainv_ptr) HIP_TRY(hipMalloc((void**)&P->d_ainv_ptr, (size_t)P->n_units * sizeof(float*)));
+    HIP_TRY(hipMemcpy(P->d_ainv_ptr, ptr.data(), (size_t)P->n_units * sizeof(float*), hipMemcpyHostToDevice));
+    P->ainv_dt = dt;
+    return PPL_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+ppl_status ppl_chomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_chomp_settings* cs, float* d_theta,
+                              float* d_best, float* d_record, int iterations, int first_call, ppl_stream stream) {
+    CHECK_ARG(P && view && cs && d_theta && d_best && d_record, "NULL argument");
+    CHECK_ARG(P->kind == PPL_PLAN_TRAJ && !P->wmajor, "CHOMP needs a (unit-major) trajectory plan");
+    CHECK_ARG(P->h_obb_off.empty(), "CHOMP optimises trajectories of one scene (no per-problem OBB ranges)");
+    CHECK_ARG(iterations >= 0, "negative iteration count");
+    CHECK_ARG(cs->dt > 0.0f, "dt must be positive");
+    if (P->n_units == 0) return PPL_OK;
+    int max_t = 0;
+    for (int u = 0; u < P->n_units; ++u) {
+        const int T = P->h_off[u + 1] - P->h_off[u];
+        CHECK_ARG(T >= 2, "trajectory %d has fewer than 2 waypoints", u);
+        max_t = std::max(max_t, T);
+    }
+    CHECK_ARG(max_t <= 4096, "trajectories longer than 4096 waypoints are not supported");
+    ppl_status st = ensure_ainv(P, cs->dt);
+    if (st != PPL_OK) return st;
+    DScene S{};
+    if ((st = plan_scene(P, view, S)) != PPL_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    const ppl_robot_s* r = P->robot;
+    const int D = r->R.n_dof;
+    const size_t n = (size_t)P->n_states, B = (size_t)P->n_units;
+    // Densification cap: no segment of a trajectory inside the joint limits is
+    // longer than the limit box's diagonal.
+    double diag2 = 0.0;
+    for (int k = 0; k < D; ++k) diag2 += (double)(r->upper[k] - r->lower[k]) * (r->upper[k] - r->lower[k]);
+    const int nsub_max = cs->collision_check_step > 0.0f
+                             ? std::max(1, (int)std::ceil(std::sqrt(diag2) / cs->collision_check_step)) : 1;
+    size_t n_dense = 0;
+    for (int u = 0; u < P->n_units; ++u) n_dense += (size_t)(P->h_off[u + 1] - P->h_off[u] - 1) * nsub_max + 1;
+    CHECK_ARG(n_dense < (size_t)INT32_MAX, "densified validity states exceed 2^31");
+    // scratch: gradient [n D], state cost [n], totals [B], first hit [B], nsub [B], dense offsets [B + 1]
+    HIP_TRY(P->d_opt.ensure((n * D + n + B) * sizeof(float) + (3 * B + 1) * sizeof(int)));
+    float* grad = P->d_opt.as<float>();
+    float* state = grad + n * D;
+    float* total = state + n;
+    int* first_hit = reinterpret_cast<int*>(total + B);
+    int* nsub = first_hit + B;
+    int* dense_off = nsub + B;
+    ppl_cost_params cp{cs->world_margin, cs->self_margin, cs->collision_cost_weight, 1, 1, cs->use_self};
+    ChompArgs a{};
+    a.R = P->robot->R;
+    a.X = plan_states(P, d_theta);
+    a.step_size = cs->step_size;
+    a.smoothness_weight = cs->smoothness_weight;
+    a.check_step = cs->collision_check_step;
+    a.margin = cs->collision_margin;
+    a.convergence_eps = cs->convergence_eps;
+    a.dt = cs->dt;
+    a.polish0 = cs->iterations_after_valid;
+    a.nsub_max = nsub_max;
+    a.ainv = P->d_ainv_ptr;
+    a.theta = d_theta;
+    a.best = d_best;
+    a.record = d_record;
+    a.grad = grad;
+    a.total = total;
+    a.first_hit = first_hit;
+    a.nsub = nsub;
+    a.dense_off = dense_off;
+    // validity states: every waypoint plus nsub - 1 interpolants per segment
+    DStates Xd = a.X;
+    Xd.source = kSrcDense;
+    Xd.dense_off = dense_off;
+    Xd.nsub = nsub;
+    Xd.n_dyn = dense_off + B;
+    Xd.n_states = (long)n_dense;
+    CollideArgs va{};
+    va.R = r->R;
+    va.S = S;
+    va.X = Xd;
+    int vgrid = 0;
+    va.C = implicit_chunks((long)n_dense, kKCollide, r->R, S.n_obbs, &vgrid);
+    va.margin = cs->collision_margin;
+    va.mode = 1;
+    va.skip_resolved = 1;
+    va.first_hit = first_hit;
+    va.self_hit = nullptr;
+    // cost (+ gradient) of the current theta, its totals and its validity
+    auto evaluate = [&]() -> ppl_status {
+        ppl_status e = cs->fd_gradient ? fd_impl(P, view, d_theta, &cp, cs->fd_eps, grad, s) : PPL_OK;
+        if (e != PPL_OK) return e;
+        e = hinge_impl(P, view, d_theta, &cp, state, total, cs->fd_gradient ? nullptr : grad, s);
+        if (e != PPL_OK) return e;
+        HIP_TRY(launch_control_cost(d_theta, P->d_off, P->n_units, D, cs->dt, cs->smoothness_weight, total, 1, s));
+        HIP_TRY(launch_chomp_nsub(a, s));
+        HIP_TRY(launch_fill_i32(first_hit, -1, (long)B, s));
+        // environment first; the self check then only visits trajectories still free
+        // (the order of two_phase_cc, collision/two_phase_cc.cuh:89-131)
+        HIP_TRY(launch_collide(va, vgrid, s));
+        if (r->n_pairs > 0) {
+            const ppl_status e2 = run_self(r, Xd, 0, nullptr, nullptr, s, FLT_MAX, first_hit);
+            if (e2 != PPL_OK) return e2;
+        }
+        return PPL_OK;
+    };
+    // the gradient at the current theta feeds the first step of this call
+    if (first_call || iterations > 0) {
+        if ((st = evaluate()) != PPL_OK) return st;
+    }
+    if (first_call) HIP_TRY(launch_chomp_book(a, 1, s));
+    for (int it = 0; it < iterations; ++it) {
+        HIP_TRY(launch_chomp_update(a, max_t, s));
+        if ((st = evaluate()) != PPL_OK) return st;
+        HIP_TRY(launch_chomp_book(a, 0, s));
+    }
+    return PPL_OK;
+}
 
 ppl_status ppl_fk_batched(ppl_plan P, const float* d_cfg, float* d_frames, float* d_spheres, ppl_stream stream) {
     CHECK_ARG(P && d_cfg, "NULL argument");

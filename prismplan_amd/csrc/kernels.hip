@@ -18,6 +18,7 @@
 //    the argmin OBB of each active sphere in the same pass.
 #include "ppl_internal.hpp"
 
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 
@@ -130,14 +131,22 @@ __device__ int stage_obbs(const DScene& S, int o0, int o1, float4* s_obb, int ca
     int n = 0;
     for (int base = o0; base < o1; base += kWave) {
         const int o = base + lane;
-        const bool ok = (o < o1) && S.obb_enable[o] != 0;
+        // enable flag, extents and pose in one round trip, then compaction
+        float4 d4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p0 = d4, p1 = d4;
+        bool ok = false;
+        if (o < o1) {
+            ok = S.obb_enable[o] != 0;
+            d4 = *reinterpret_cast<const float4*>(S.obb_dims + 4 * (size_t)o);
+            p0 = *reinterpret_cast<const float4*>(S.obb_pose + 8 * (size_t)o);
+            p1 = *reinterpret_cast<const float4*>(S.obb_pose + 8 * (size_t)o + 4);
+        }
         const unsigned long long m = __ballot(ok);
         if (ok) {
             const int pos = n + __popcll(m & ((1ull << lane) - 1ull));
             if (pos < cap) {
-                const float* d = S.obb_dims + 4 * (size_t)o;
-                const float* p = S.obb_pose + 8 * (size_t)o;
-                const float w = p[3], x = p[4], y = p[5], z = p[6];
+                const float d[3] = {d4.x, d4.y, d4.z};
+                const float p[3] = {p0.x, p0.y, p0.z};
+                const float w = p0.w, x = p1.x, y = p1.y, z = p1.z;
                 // Matrix of v -> v + w t + u x t, t = 2 u x v (scene_collision.cuh:36-45).
                 const float r00 = 1.0f - 2.0f * (y * y + z * z), r01 = 2.0f * (x * y - w * z), r02 = 2.0f * (x * z + w * y);
                 const float r10 = 2.0f * (x * y + w * z), r11 = 1.0f - 2.0f * (x * x + z * z), r12 = 2.0f * (y * z - w * x);
@@ -319,28 +328,38 @@ struct StateRef {
     long gs;       // plan-order index of the state (differs from the chunk index when waypoint-major)
 };
 
-// Ends of units u0 .. u0+63 into s_end (whole wave; no-op for configuration
-// sources or when the chunk's first unit is unknown, u0 < 0).
+// State ranges of units u0 .. u0+63 into LDS: s_end[i] = off[u0+i+1] and
+// s_end[64+i] = off[u0+i] (whole wave; no-op for configuration sources or
+// when the chunk's first unit is unknown, u0 < 0).
 __device__ __forceinline__ void stage_unit_ends(const DStates& X, int u0, int* s_end) {
     if (X.source == kSrcConfig || u0 < 0) return;
     const int* off = X.source == kSrcEdge ? X.edge_off : X.traj_off;
     const int n = X.source == kSrcEdge ? X.n_edges : X.n_traj;
     const int u = u0 + lane_id();
-    s_end[lane_id()] = u < n ? off[u + 1] : INT_MAX;
+    const bool in = u < n;
+    s_end[lane_id()] = in ? off[u + 1] : INT_MAX;
+    s_end[kWave + lane_id()] = in ? off[u] : INT_MAX;
 }
 
-// Unit holding state g: the first of units u0 .. u0+63 whose end exceeds g
-// (bisection over the staged ends), else a binary search over the table.
-__device__ __forceinline__ int find_unit(const int* off, int n, long g, int u0, const int* s_end) {
+// Unit holding state g and its state range [*u_begin, *u_end): the first of
+// units u0 .. u0+63 whose end exceeds g (bisection over the staged ends),
+// else a binary search over the offset table.
+__device__ __forceinline__ int find_unit(const int* off, int n, long g, int u0, const int* s_end, int* u_begin,
+                                         int* u_end) {
     if (u0 >= 0 && s_end[kWave - 1] > g) {
         int lo = 0, hi = kWave - 1;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             if (s_end[mid] > g) hi = mid; else lo = mid + 1;
         }
+        *u_begin = s_end[kWave + lo];
+        *u_end = s_end[lo];
         return u0 + lo;
     }
-    return upper_bound_idx(off, n, g);
+    const int u = upper_bound_idx(off, n, g);
+    *u_begin = off[u];
+    *u_end = off[u + 1];
+    return u;
 }
 
 // Writes q_d into s_q[d * 64 + lane] for d < D (inactive lanes write nothing).
@@ -350,6 +369,29 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
     StateRef ref{0, 0, 0, 0, g};
     const int lane = lane_id();
     if (!active) return ref;
+    if (X.source == kSrcDense) {
+        const int b = upper_bound_idx(X.dense_off, X.n_traj, g);
+        const int d0 = X.dense_off[b];
+        const int t0 = X.traj_off[b], T = X.traj_off[b + 1] - t0;
+        const int ns = X.nsub[b];
+        const int f = (int)(g - d0);
+        ref.unit = b;
+        ref.k = f;
+        ref.n = (T - 1) * ns + 1;
+        ref.u0 = d0;
+        const float* th = X.theta + (size_t)t0 * D;
+        if (f >= (T - 1) * ns) {
+            for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = th[(size_t)d * T + T - 1];
+        } else {
+            const int seg = f / ns, sub = f - seg * ns;
+            const float frac = __fdiv_rn((float)sub, (float)ns);
+            for (int d = 0; d < D; ++d) {
+                const float x0 = th[(size_t)d * T + seg], x1 = th[(size_t)d * T + seg + 1];
+                s_q[d * kWave + lane] = __fadd_rn(x0, __fmul_rn(__fsub_rn(x1, x0), frac));
+            }
+        }
+        return ref;
+    }
     if (X.source == kSrcTraj && X.wmajor) {
         const int4 pr = X.prob[prob];
         const int v = (int)(g - pr.w);
@@ -369,20 +411,21 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         const float* q = X.cfg + (size_t)g * X.cfg_stride;
         for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = q[d];
     } else if (X.source == kSrcEdge) {
-        const int e = find_unit(X.edge_off, X.n_edges, g, u0, s_end);
-        const int cnt = X.edge_off[e + 1] - X.edge_off[e];
+        int b0, b1;
+        const int e = find_unit(X.edge_off, X.n_edges, g, u0, s_end, &b0, &b1);
+        const int cnt = b1 - b0;
         ref.unit = e;
-        ref.u0 = X.edge_off[e];
-        ref.k = (int)(g - X.edge_off[e]) + X.k0;
+        ref.u0 = b0;
+        ref.k = (int)(g - b0) + X.k0;
         ref.n = cnt + X.k0;
         const float t = ref.n > 1 ? (float)ref.k / (float)(ref.n - 1) : 0.0f;
         const float* a = X.from + (size_t)e * D;
         const float* b = X.to + (size_t)e * D;
         for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = fmaf(t, b[d] - a[d], a[d]);
     } else {
-        const int b = find_unit(X.traj_off, X.n_traj, g, u0, s_end);
-        const int t0 = X.traj_off[b];
-        const int Tn = X.traj_off[b + 1] - t0;
+        int t0, t1;
+        const int b = find_unit(X.traj_off, X.n_traj, g, u0, s_end, &t0, &t1);
+        const int Tn = t1 - t0;
         ref.unit = b;
         ref.u0 = t0;
         ref.k = (int)(g - t0);
@@ -411,9 +454,17 @@ __device__ __forceinline__ ChunkCursor chunk_cursor() {
     return ChunkCursor{rank, W};
 }
 
-__device__ __forceinline__ bool next_chunk(const DChunks& C, long n_states, ChunkCursor& cur, int4& ch) {
+// Chunk count of a launch: the table's, or ceil(states / chunk) for implicit
+// chunks (the state count may come from device memory).
+__device__ __forceinline__ int chunk_count(const DChunks& C, long n_states) {
+    return C.tab ? C.n_chunks : (int)((n_states + C.chunk - 1) / C.chunk);
+}
+
+__device__ __forceinline__ long state_count(const DStates& X) { return X.n_dyn ? (long)uniform(*X.n_dyn) : X.n_states; }
+
+__device__ __forceinline__ bool next_chunk(const DChunks& C, long n_states, int n_chunks, ChunkCursor& cur, int4& ch) {
     const int c = cur.next;
-    if (c >= C.n_chunks) return false;
+    if (c >= n_chunks) return false;
     cur.next += cur.stride;
     if (C.tab) {
         ch = C.tab[c];
@@ -472,7 +523,9 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
     int cur_prob = -1, n_obb = 0;
     int4 ch;
     ChunkCursor cur = chunk_cursor();
-    while (next_chunk(a.C, a.X.n_states, cur, ch)) {
+    const long n_states = state_count(a.X);
+    const int n_chunks = chunk_count(a.C, n_states);
+    while (next_chunk(a.C, n_states, n_chunks, cur, ch)) {
         if (ch.x != cur_prob) {
             int o0, o1;
             problem_obbs(a.C, a.S, ch.x, o0, o1);
@@ -745,15 +798,17 @@ __global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
     // hinge gradients; then d cost / d q per dof.
     int* s_end = reinterpret_cast<int*>(s_q + D * kWave);
     const int rs = rec_stride(J), ds = dq_stride(D);
-    float* rec = s_q + (D + 1) * kWave + slot * rs;
-    float* dqs = s_q + (D + 1) * kWave + (kWave / G) * rs + slot * ds;
+    float* rec = s_q + (D + 2) * kWave + slot * rs;
+    float* dqs = s_q + (D + 2) * kWave + (kWave / G) * rs + slot * ds;
     const bool use_obb = a.P.use_obb != 0 && !a.clearance_only;
     const bool use_esdf = (a.P.use_esdf != 0 || a.clearance_only) && a.S.n_voxel_layers > 0;
     const float wt = a.P.weight;
     int cur_prob = -1, n_obb = 0;
     int4 ch;
     ChunkCursor cur = chunk_cursor();
-    while (next_chunk(a.C, a.X.n_states, cur, ch)) {
+    const long n_states = state_count(a.X);
+    const int n_chunks = chunk_count(a.C, n_states);
+    while (next_chunk(a.C, n_states, n_chunks, cur, ch)) {
         if (ch.x != cur_prob) {
             int o0, o1;
             problem_obbs(a.C, a.S, ch.x, o0, o1);
@@ -892,10 +947,14 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
     float4* s_pos = s_dyn4;                                          // [kSelfStates][S] (x, y, z, r)
     float4* s_jnt = s_dyn4 + kSelfStates * S;                         // [kSelfStates][J] enclosing sphere, r < 0: none
     float* s_q = reinterpret_cast<float*>(s_jnt + kSelfStates * J);   // [kMaxDof][64]
-    const long g = (long)blockIdx.x * kSelfStates + st;
-    const bool active = g < a.X.n_states;
+    const long n_states = state_count(a.X);
+    for (long base = (long)blockIdx.x * kSelfStates; base < n_states; base += (long)gridDim.x * kSelfStates) {
+    const long g = base + st;
+    bool active = g < n_states;
     // Every lane of a state group loads the same q (lane-indexed LDS column).
-    load_state(a.X, D, active ? g : 0, true, s_q);
+    const StateRef ref = load_state(a.X, D, active ? g : base, true, s_q);
+    if (active && a.unit_first_hit && a.unit_first_hit[ref.unit] >= 0) active = false;   // unit already invalid
+    if (a.unit_first_hit && !wave_any(active)) continue;      // nothing left to decide in this round
     __syncthreads();
     Slots sl;
     float4* pos = s_pos + st * S;
@@ -952,7 +1011,13 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
             hit = ((m >> (st * kSelfGroup)) & 0xFFFFull) != 0ull;
             if (__ballot(active && !hit) == 0ull) break;
         }
-        if (active && sub == 0) a.hit_out[g] = hit ? 1 : 0;
+        if (active && sub == 0) {
+            if (a.unit_first_hit) {
+                if (hit) atomicMin((unsigned*)&a.unit_first_hit[ref.unit], (unsigned)ref.k);
+            } else {
+                a.hit_out[g] = hit ? 1 : 0;
+            }
+        }
     } else {
         float best = 1000.0f;
         int bp = INT_MAX;
@@ -996,6 +1061,8 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
             o[6] = o[7] = 0.0f;
         }
     }
+    __syncthreads();   // positions of this round are dead before the next load
+    }
 }
 
 // ===========================================================================
@@ -1035,6 +1102,160 @@ __global__ __launch_bounds__(64) void k_fk_out(DRobot R, DStates X, float* frame
                 reinterpret_cast<float4*>(spheres)[(size_t)g * R.n_sph + R.sph_id[s]] = make_float4(x, y, z, sp.w);
             }
         }
+    }
+}
+
+// ===========================================================================
+// CHOMP optimiser (planning/chomp.cu:261-351, 459-557 semantics, batched)
+// ===========================================================================
+// Sub-states per segment of each trajectory: ceil(max segment length / step)
+// when the longest segment exceeds the step, else 1 (chomp.cu:459-475, FP32 as
+// the host code: squares summed in dof order, no contraction), capped at
+// nsub_max (the joint-limit box diagonal over the step, which no clamped
+// trajectory exceeds). One wave per trajectory.
+__global__ __launch_bounds__(64) void k_chomp_nsub(ChompArgs a) {
+    const int b = blockIdx.x;
+    if (b >= a.X.n_traj) return;
+    const int lane = lane_id();
+    const int D = a.R.n_dof;
+    const int t0 = a.X.traj_off[b], T = a.X.traj_off[b + 1] - t0;
+    const float* th = a.theta + (size_t)t0 * D;
+    float m = 0.0f;
+    for (int t = lane; t < T - 1; t += kWave) {
+        float d2 = 0.0f;
+        for (int d = 0; d < D; ++d) {
+            const float diff = __fsub_rn(th[(size_t)d * T + t + 1], th[(size_t)d * T + t]);
+            d2 = __fadd_rn(d2, __fmul_rn(diff, diff));
+        }
+        m = fmaxf(m, __fsqrt_rn(d2));
+    }
+    m = wave_max(m);
+    if (lane == 0) {
+        int n = 1;
+        if (a.check_step > 0.0f && m > a.check_step)
+            n = (int)fminf(ceilf(__fdiv_rn(m, a.check_step)), (float)a.nsub_max);
+        a.nsub[b] = n;
+    }
+}
+
+// dense_off = exclusive scan of (T_b - 1) nsub[b] + 1 (dense_off[B] = total).
+// One workgroup of 256 threads, each scanning a contiguous slice.
+__global__ __launch_bounds__(256) void k_chomp_scan(ChompArgs a) {
+    __shared__ int part[256];
+    const int B = a.X.n_traj, tid = threadIdx.x;
+    const int per = (B + 255) / 256, lo = tid * per, hi = min(B, lo + per);
+    int acc = 0;
+    for (int b = lo; b < hi; ++b) acc += (a.X.traj_off[b + 1] - a.X.traj_off[b] - 1) * a.nsub[b] + 1;
+    part[tid] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int k = 0; k < 256; ++k) {
+            const int v = part[k];
+            part[k] = run;
+            run += v;
+        }
+        a.dense_off[B] = run;
+    }
+    __syncthreads();
+    int run = part[tid];
+    for (int b = lo; b < hi; ++b) {
+        a.dense_off[b] = run;
+        run += (a.X.traj_off[b + 1] - a.X.traj_off[b] - 1) * a.nsub[b] + 1;
+    }
+}
+
+// Per-trajectory bookkeeping after an evaluation (chomp.cu:484-557): best
+// valid snapshot, convergence streak on the best cost, polish count, done.
+// One wave per trajectory (the snapshot copy uses every lane).
+__global__ __launch_bounds__(64) void k_chomp_book(ChompArgs a, int initial) {
+    const int b = blockIdx.x;
+    if (b >= a.X.n_traj) return;
+    const int lane = lane_id();
+    const int D = a.R.n_dof;
+    float* rec = a.record + 8 * (size_t)b;
+    const float total = a.total[b];
+    const bool valid = a.first_hit[b] < 0;
+    bool copy = false;
+    if (initial) {
+        copy = valid;
+    } else {
+        copy = rec[PPL_CHOMP_DONE] == 0.0f && valid && total < rec[PPL_CHOMP_BEST_COST];
+    }
+    __syncthreads();
+    if (lane == 0) {
+        if (initial) {
+            rec[PPL_CHOMP_BEST_COST] = valid ? total : INFINITY;
+            rec[PPL_CHOMP_PREV_COST] = INFINITY;
+            rec[PPL_CHOMP_SOLVED] = valid ? 1.0f : 0.0f;
+            rec[PPL_CHOMP_STREAK] = 0.0f;
+            rec[PPL_CHOMP_POLISH_LEFT] = (float)a.polish0;
+            rec[PPL_CHOMP_DONE] = 0.0f;
+            rec[PPL_CHOMP_ITERS] = 0.0f;
+        } else if (rec[PPL_CHOMP_DONE] == 0.0f) {
+            rec[PPL_CHOMP_ITERS] += 1.0f;
+            if (copy) {
+                rec[PPL_CHOMP_BEST_COST] = total;
+                rec[PPL_CHOMP_SOLVED] = 1.0f;
+            }
+            if (rec[PPL_CHOMP_SOLVED] != 0.0f) {
+                const float impr = rec[PPL_CHOMP_PREV_COST] - rec[PPL_CHOMP_BEST_COST];
+                rec[PPL_CHOMP_STREAK] = (impr >= 0.0f && impr < a.convergence_eps) ? rec[PPL_CHOMP_STREAK] + 1.0f : 0.0f;
+                rec[PPL_CHOMP_PREV_COST] = rec[PPL_CHOMP_BEST_COST];
+                rec[PPL_CHOMP_POLISH_LEFT] -= 1.0f;
+                if (rec[PPL_CHOMP_STREAK] >= 2.0f) rec[PPL_CHOMP_DONE] = 1.0f;
+            }
+        }
+        // the next iteration starts only while no valid result is polished out
+        if (rec[PPL_CHOMP_SOLVED] != 0.0f && rec[PPL_CHOMP_POLISH_LEFT] <= 0.0f) rec[PPL_CHOMP_DONE] = 1.0f;
+        rec[PPL_CHOMP_LAST_COST] = total;
+    }
+    if (copy) {
+        const size_t o = (size_t)a.X.traj_off[b] * D;
+        const int n = (a.X.traj_off[b + 1] - a.X.traj_off[b]) * D;
+        for (int i = lane; i < n; i += kWave) a.best[o + i] = a.theta[o + i];
+    }
+}
+
+// Covariant step for one (dof, trajectory) (chomp.cu:261-290):
+//   g = grad_d + lambda R theta_d with R theta = dt A^T (A theta) for the
+//   5-point stencil A on the zero-padded grid, step = A^-1 g,
+//   theta_d[t] -= step_size step[t] for 0 < t < T-1, clamped to the limits.
+__global__ __launch_bounds__(256) void k_chomp_update(ChompArgs a) {
+    extern __shared__ float s_u[];
+    const int d = blockIdx.x, b = blockIdx.y;
+    const float* rec = a.record + 8 * (size_t)b;
+    if (rec[PPL_CHOMP_DONE] != 0.0f) return;
+    const int D = a.R.n_dof;
+    const int t0 = a.X.traj_off[b], T = a.X.traj_off[b + 1] - t0;
+    float* x = s_u;             // [T + 8], theta with 4 zeros each side
+    float* v = x + T + 8;       // [T + 4], A theta on [-2, T + 2)
+    float* g = v + T + 4;       // [T]
+    float* row = a.theta + (size_t)t0 * D + (size_t)d * T;
+    const float* gr = a.grad + (size_t)t0 * D + (size_t)d * T;
+    const float dt = a.dt;
+    const float inv = 1.0f / (12.0f * dt * dt);
+    for (int i = threadIdx.x; i < T + 8; i += blockDim.x) x[i] = (i >= 4 && i < T + 4) ? row[i - 4] : 0.0f;
+    __syncthreads();
+    for (int i = threadIdx.x; i < T + 4; i += blockDim.x) {
+        const float* c = x + i + 2;   // centre at grid index i - 2
+        v[i] = (-c[-2] + 16.0f * c[-1] - 30.0f * c[0] + 16.0f * c[1] - c[2]) * inv;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        const float* c = v + t + 2;
+        const float rt = dt * ((-c[-2] + 16.0f * c[-1] - 30.0f * c[0] + 16.0f * c[1] - c[2]) * inv);
+        g[t] = gr[t] + a.smoothness_weight * rt;
+    }
+    __syncthreads();
+    const float* ai = a.ainv[b];
+    const float2 lim = a.R.lim[d];
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        float step = 0.0f;
+        for (int u = 0; u < T; ++u) step = fmaf(ai[(size_t)u * T + t], g[u], step);   // A^-1 symmetric
+        if (t == 0 || t == T - 1) continue;
+        const float nv = x[t + 4] - a.step_size * step;
+        row[t] = fmaxf(lim.x, fminf(lim.y, nv));
     }
 }
 
@@ -1158,7 +1379,7 @@ int occupancy_grid(const void* fn, size_t dyn_smem) {
 }  // namespace
 
 size_t lds_bytes(int obb_cap, int n_dof) {
-    return (size_t)obb_cap * 64 + (size_t)(n_dof + 1) * kWave * sizeof(float);
+    return (size_t)obb_cap * 64 + (size_t)(n_dof + 2) * kWave * sizeof(float);
 }
 
 size_t lds_bytes_cost(int obb_cap, int n_dof, int n_joints, int G, bool grad) {
@@ -1199,7 +1420,9 @@ hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s) {
 }
 
 hipError_t launch_self(const SelfArgs& a, hipStream_t s) {
-    const long blocks = (a.X.n_states + kSelfStates - 1) / kSelfStates;
+    // X.n_states bounds the state count (a device-side count may be smaller)
+    long blocks = (a.X.n_states + kSelfStates - 1) / kSelfStates;
+    if (blocks > 16384) blocks = 16384;
     if (blocks <= 0) return hipSuccess;
     const size_t smem = sizeof(float4) * (size_t)kSelfStates * (a.R.n_sph + a.R.n_joints) +
                         sizeof(float) * (size_t)kMaxDof * kWave;
@@ -1211,6 +1434,26 @@ hipError_t launch_fk_out(const DRobot& R, const DStates& X, float* frames, float
     const long blocks = (X.n_states + kWave - 1) / kWave;
     if (blocks <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fk_out, dim3((unsigned)blocks), dim3(64), 0, s, R, X, frames, spheres);
+    return hipGetLastError();
+}
+
+hipError_t launch_chomp_nsub(const ChompArgs& a, hipStream_t s) {
+    if (a.X.n_traj <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_chomp_nsub, dim3(a.X.n_traj), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_chomp_scan, dim3(1), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_chomp_book(const ChompArgs& a, int initial, hipStream_t s) {
+    if (a.X.n_traj <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_chomp_book, dim3(a.X.n_traj), dim3(64), 0, s, a, initial);
+    return hipGetLastError();
+}
+
+hipError_t launch_chomp_update(const ChompArgs& a, int max_t, hipStream_t s) {
+    if (a.X.n_traj <= 0) return hipSuccess;
+    const size_t smem = sizeof(float) * (3 * (size_t)max_t + 12);
+    hipLaunchKernelGGL(k_chomp_update, dim3(a.R.n_dof, a.X.n_traj), dim3(256), smem, s, a);
     return hipGetLastError();
 }
 

@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ppl/ppl.h"
+
 namespace ppl {
 
 constexpr int kMaxJoints = 16;      // per-lane unrolled joint state (Baxter: 15)
@@ -44,6 +46,7 @@ struct DRobot {
     const int* apx_id;          // [A]
     const int* parent;          // [J] DFS position of the parent of DFS position k (root: 0)
     const int* jorig;           // [J] original joint index of DFS position k
+    const float2* lim;          // [D] joint limits (lower, upper)
     const float4* sph_orig;     // [S] spheres by original index
     const int2* pairs;          // [n_pairs] exact self pairs (original indices), grouped by joint pair
     const int4* sblk;           // [n_sblk] (DFS joint a, DFS joint b, first pair, end pair) of each group
@@ -71,7 +74,7 @@ struct DScene {
 };
 
 // Where a flattened state index comes from.
-enum StateSource { kSrcConfig = 0, kSrcEdge = 1, kSrcTraj = 2 };
+enum StateSource { kSrcConfig = 0, kSrcEdge = 1, kSrcTraj = 2, kSrcDense = 3 };
 
 struct DStates {
     int source;
@@ -88,6 +91,13 @@ struct DStates {
     // trajectory prob[p].x + v % prob[p].y, waypoint v / prob[p].y, where
     // prob[p] = (first trajectory, trajectories, length, first state).
     const int4* prob; int wmajor;
+    // kSrcDense: densified trajectories (theta, traj_off, n_traj as kSrcTraj):
+    // state g of trajectory b (dense_off[b] <= g < dense_off[b+1]) is
+    // f = g - dense_off[b]; f < (T_b - 1) nsub[b] lies at segment f / nsub[b],
+    // fraction (f % nsub[b]) / nsub[b]; the last one is waypoint T_b - 1.
+    const int* dense_off; const int* nsub;
+    // when set, the state count is read from device memory (*n_dyn) at launch
+    const int* n_dyn;
 };
 
 // Work chunks for persistent launches: a chunk never spans two problems.
@@ -133,6 +143,8 @@ struct SelfArgs {
     float band;                 // mode 1: pairs whose joints' enclosing spheres are >= band apart are
                                 // skipped (the minimum is exact whenever it is below band)
     uint8_t* hit_out;           // [n_states]
+    int* unit_first_hit;        // mode 0, nullable: per-unit first hit (-1 free); states of units with a
+                                // hit are skipped and a self collision lowers the unit's entry instead of hit_out
     float* out;                 // [n_states][8]: clr, i, j (as int bits), nx, ny, nz, -, -
 };
 
@@ -140,6 +152,21 @@ hipError_t launch_collide(const CollideArgs& a, int grid, hipStream_t s);
 hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s);
 hipError_t launch_self(const SelfArgs& a, hipStream_t s);
 hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t s);
+// CHOMP optimiser pieces (trajectory plans, dim-major [D][T] blocks).
+struct ChompArgs {
+    DRobot R; DStates X;        // X: trajectory source over theta
+    float step_size, smoothness_weight, check_step, margin, convergence_eps, dt;
+    int polish0;                // iterations after valid
+    int nsub_max;               // densification cap (joint-limit box diagonal / step)
+    const float* const* ainv;   // [n_traj] A^-1 (T x T) of each trajectory's length
+    float* theta; float* best; float* record; const float* grad; const float* total;
+    const int* first_hit;       // [n_traj] validity pass result, -1 = collision-free
+    int* nsub; int* dense_off;  // [n_traj], [n_traj + 1]
+};
+hipError_t launch_chomp_nsub(const ChompArgs& a, hipStream_t s);   // nsub + dense_off (scan)
+hipError_t launch_chomp_book(const ChompArgs& a, int initial, hipStream_t s);
+hipError_t launch_chomp_update(const ChompArgs& a, int max_t, hipStream_t s);
+
 // Joint frames [n][J][16] (column-major 4x4, original joint order) and / or
 // sphere centres [n][S][4] (x, y, z, r, original order) of configurations.
 hipError_t launch_fk_out(const DRobot& R, const DStates& X, float* frames, float* spheres, hipStream_t s);

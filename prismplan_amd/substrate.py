@@ -346,6 +346,42 @@ def chomp_fd_grad_plain(robot: Robot, scene: DeviceScene, theta: torch.Tensor, T
     return out
 
 
+def chomp_settings(step_size=0.3, smoothness_weight=1.0, collision_cost_weight=20.0, world_margin=0.02,
+                   self_margin=0.01, dt=0.1, collision_check_step=0.05, collision_margin=0.0, convergence_eps=1e-4,
+                   iterations_after_valid=5, use_self=True, fd_gradient=False, fd_eps=1e-3) -> _ppl.ChompSettings:
+    """CHOMP optimiser settings; defaults are the reference's CHOMP_settings."""
+    return _ppl.ChompSettings(step_size, smoothness_weight, collision_cost_weight, world_margin, self_margin, dt,
+                              collision_check_step, collision_margin, convergence_eps, int(iterations_after_valid),
+                              int(bool(use_self)), int(bool(fd_gradient)), fd_eps)
+
+
+class ChompOptimizer:
+    """Batched CHOMP over the trajectories of a plan, all on the device.
+
+    theta is optimised in place; best holds each trajectory's best valid
+    snapshot and record the per-trajectory state (see _ppl.CHOMP_*)."""
+
+    def __init__(self, plan: Plan, scene: DeviceScene, theta: torch.Tensor, settings: _ppl.ChompSettings):
+        _dev(theta)
+        self.plan, self.scene, self.theta, self.settings = plan, scene, theta, settings
+        self.best = theta.clone()
+        self.record = torch.zeros((plan.n_units, 8), dtype=torch.float32, device=theta.device)
+        self._started = False
+
+    def step(self, iterations: int, stream=None):
+        v = self.scene.view()
+        call("ppl_chomp_optimize", self.plan.handle, C.byref(v), C.byref(self.settings), _dp(self.theta),
+             _dp(self.best), _dp(self.record), int(iterations), int(not self._started), _stream(stream))
+        self._started = True
+        return self
+
+    def solved(self) -> torch.Tensor:
+        return self.record[:, _ppl.CHOMP_SOLVED] != 0
+
+    def done(self) -> torch.Tensor:
+        return self.record[:, _ppl.CHOMP_DONE] != 0
+
+
 def fk_batched(plan: Plan, configs: torch.Tensor, frames=True, spheres=True, stream=None):
     """Joint frames [n, J, 4, 4] (row-major view of the column-major output)
     and sphere centres + radii [n, S, 4] of every configuration."""
