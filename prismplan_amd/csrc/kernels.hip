@@ -720,32 +720,37 @@ __device__ __forceinline__ void near_obbs(const float4* __restrict__ s_obb, int 
     }
 }
 
-// Hinge terms of one sphere at world position p: the summed hinge (OBB +
-// ESDF) and its point gradient d hinge / d p in g*; in clearance-only mode the
-// ESDF clearance with radius + margin instead. OBBs are taken from `mk`.
+// Smallest box key over the candidate OBBs of `mk` at world point p, and its
+// OBB (LDS record index; -1 when there is none). Ties keep the lowest index.
+__device__ __forceinline__ void obb_argmin(const float4* __restrict__ s_obb, const ObbMask& mk, float px, float py,
+                                           float pz, float& best, int& arg) {
+    best = FLT_MAX;
+    arg = -1;
+#pragma unroll
+    for (int k = 0; k < kMaskWords; ++k) {
+        unsigned long long bits = mk.w[k];
+        while (bits) {
+            const int o = k * 64 + (__ffsll(bits) - 1);
+            bits &= bits - 1ull;
+            float lx, ly, lz;
+            to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], px, py, pz, lx, ly, lz);
+            const float key = box_key(lx, ly, lz, s_obb[4 * o + 3]);
+            if (key < best) { best = key; arg = o; }
+        }
+    }
+}
+
+// Hinge terms of one sphere at world position p given its OBB argmin (best
+// key, arg): the summed hinge (OBB + ESDF) and its point gradient d hinge / d p
+// in g*; in clearance-only mode the ESDF clearance with radius + margin.
 template <bool GRAD>
-__device__ __forceinline__ float sphere_hinge(const CostArgs& a, const float4* __restrict__ s_obb, const ObbMask& mk,
-                                              float px, float py, float pz, float r, int sph_id, bool use_obb,
-                                              bool use_esdf, float& gx, float& gy, float& gz) {
+__device__ __forceinline__ float sphere_hinge_at(const CostArgs& a, const float4* __restrict__ s_obb, float best,
+                                                 int arg, float px, float py, float pz, float r, int sph_id,
+                                                 bool use_obb, bool use_esdf, float& gx, float& gy, float& gz) {
     const float m = a.P.world_margin;
     float world = 0.0f;
     gx = gy = gz = 0.0f;
     if (use_obb) {
-        float best = FLT_MAX;
-        int arg = -1;
-#pragma unroll
-        for (int k = 0; k < kMaskWords; ++k) {
-            unsigned long long bits = mk.w[k];
-            while (bits) {
-                const int o = k * 64 + (__ffsll(bits) - 1);
-                bits &= bits - 1ull;
-                const float4 r0 = s_obb[4 * o], r1 = s_obb[4 * o + 1], r2 = s_obb[4 * o + 2], h = s_obb[4 * o + 3];
-                float lx, ly, lz;
-                to_box(r0, r1, r2, px, py, pz, lx, ly, lz);
-                const float key = box_key(lx, ly, lz, h);
-                if (key < best) { best = key; arg = o; }
-            }
-        }
         const float clr = arg >= 0 ? key_to_sdf(best) - r : 1000.0f;
         const float hh = m - clr;
         if (hh > 0.0f) {
@@ -849,9 +854,12 @@ __global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
                     const float4 sp = R.sph[s];
                     float px, py, pz;
                     aff_point(T, sp.x, sp.y, sp.z, px, py, pz);
+                    float best = FLT_MAX;
+                    int arg = -1;
+                    if (use_obb) obb_argmin(s_obb, mk, px, py, pz, best, arg);
                     float gx, gy, gz;
-                    const float hs = sphere_hinge<GRAD>(a, s_obb, mk, px, py, pz, sp.w, R.sph_id[s], use_obb, use_esdf,
-                                                        gx, gy, gz);
+                    const float hs = sphere_hinge_at<GRAD>(a, s_obb, best, arg, px, py, pz, sp.w, R.sph_id[s], use_obb,
+                                                           use_esdf, gx, gy, gz);
                     if (a.clearance_only) minclr = fminf(minclr, hs);
                     else world += hs;
                     if (GRAD) {
@@ -1420,9 +1428,10 @@ hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s) {
 }
 
 hipError_t launch_self(const SelfArgs& a, hipStream_t s) {
-    // X.n_states bounds the state count (a device-side count may be smaller)
+    // X.n_states bounds the state count (a device-side count may be smaller,
+    // then a bounded grid strides over it)
     long blocks = (a.X.n_states + kSelfStates - 1) / kSelfStates;
-    if (blocks > 16384) blocks = 16384;
+    if (a.X.n_dyn && blocks > 16384) blocks = 16384;
     if (blocks <= 0) return hipSuccess;
     const size_t smem = sizeof(float4) * (size_t)kSelfStates * (a.R.n_sph + a.R.n_joints) +
                         sizeof(float) * (size_t)kMaxDof * kWave;
