@@ -40,6 +40,11 @@ PEAK_FP32_UNPACKED = 78.6e12  # without packed FP32 (this build disables packed 
 # Mean gated early-exit flops per edges-workload state, as the CPU leg counts
 # them on its 20,000-state sample; used when that leg is switched off.
 GATED_FLOPS_EDGES = 3191.0
+# HBM bytes one headline launch moves (k_collide), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE scaled by a 256 MiB copy (profiles/edges_collide_hbm_traffic.txt);
+# the algorithmic bytes are 16,384 edges x 69 B.
+TRAFFIC_BYTES_EDGES = 2_101_488
+ALGORITHMIC_BYTES_EDGES = 16384 * 69
 
 
 def log(msg):
@@ -128,7 +133,10 @@ def main():
     from prismplan_amd import _ppl
     f_exh = panda.flops(_ppl.WORK_ANY_COLLISION, n_obbs=e.scene.n_obbs)
     roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32, "traffic": None,
+                "frac": achieved / PEAK_FP32, "traffic": TRAFFIC_BYTES_EDGES,
+                "traffic_note": "HBM bytes per launch (counters, copy-scaled); algorithmic bytes "
+                                f"{ALGORITHMIC_BYTES_EDGES}; achieved HBM GB/s "
+                                f"{TRAFFIC_BYTES_EDGES / ev / 1e9:.1f} of 8000",
                 "frac_of_unpacked_peak": achieved / PEAK_FP32_UNPACKED,
                 "flops_per_state": f_state, "flops_rule": "gated early-exit count (oracle, 20k-state sample)",
                 "exhaustive_flops_per_state": f_exh,
