@@ -333,6 +333,101 @@ ppl_status ppl_chomp_optimize(ppl_plan plan, const ppl_scene_view* scene, const 
                               float* d_theta, float* d_best, float* d_record, int iterations, int first_call,
                               ppl_stream stream);
 
+/* Single-goal CHOMP solve from host start / goal configurations, the
+ * reference's CHOMP::solve_runtime_scene (planning/chomp.cu:358-610) and its
+ * pybind entry chomp_solve_scene (python/bindings.cpp:872-902):
+ *   theta = clamp(linear interpolation start -> goal) over num_timesteps
+ *     (chomp.cu:57-69),
+ *   up to num_iterations iterations of ppl_chomp_optimize, stopping when the
+ *     record is done or, with time_limit_ms > 0, once the wall time since the
+ *     call began exceeds it before an iteration (chomp.cu:491-495),
+ *   h_path [num_timesteps][n_dof] = the best valid trajectory, else the
+ *     working one, with its first and last rows set to start and goal.
+ * final_state_cost / final_control_cost are those of the returned trajectory;
+ * final_total_cost is the optimiser's recorded total. `bufs` (may be NULL) are
+ * the reused device buffers of the reference's CHOMPBuffers (chomp.cu:371-414);
+ * without them the call allocates and frees its own. Synchronous. */
+typedef struct ppl_opt_buffers_s* ppl_chomp_buffers;
+ppl_status ppl_chomp_buffers_create(int n_dof, int num_timesteps, ppl_chomp_buffers* out);
+ppl_status ppl_chomp_buffers_destroy(ppl_chomp_buffers bufs);
+
+typedef struct {
+    int solved;
+    int iterations_run;
+    float final_state_cost;
+    float final_control_cost;
+    float final_total_cost;
+    double wall_time_ms;
+} ppl_opt_result;
+typedef ppl_opt_result ppl_chomp_result;
+
+ppl_status ppl_chomp_solve(ppl_robot robot, ppl_chomp_buffers bufs, const ppl_scene_view* scene, const float* h_start,
+                           const float* h_goal, int num_timesteps, int num_iterations, float time_limit_ms,
+                           const ppl_chomp_settings* settings, float* h_path, ppl_chomp_result* result,
+                           ppl_stream stream);
+
+/* ------------------------------------------------------------------------
+ * Batched STOMP optimiser on the substrate (planning/stomp.cu:565-930
+ * semantics, every trajectory of a trajectory plan at once). Per iteration
+ * and trajectory, with R = dt A^T A the 5-point metric:
+ *   population: num_rollouts_new fresh samples x = clamp(theta + L z) with the
+ *     endpoints pinned (num_rollouts_new + num_rollouts_old on the first
+ *     iteration), the num_rollouts_old cheapest rollouts of the previous
+ *     population, and theta itself (stomp.cu:100-189);
+ *   cost of a row per (dof, waypoint): state cost (weighted world + self
+ *     hinge) + control_cost_weight x_t (R x)_t (stomp.cu:206-303);
+ *   weights p = exp(-h (c - min) / max(max - min, 1e-8)) normalised over the
+ *     population, per (dof, waypoint) (stomp.cu:310-350);
+ *   theta += M (sum_r p_r (x_r - theta)) with both endpoints pinned and no
+ *     clamp (stomp.cu:358-460);
+ *   validity, best-valid tracking, streak and polish as the CHOMP optimiser.
+ * L is the Cholesky factor of R^-1 rescaled to max entry 1/T, times
+ * noise_scale; M is R^-1 with each column scaled to max 1/T
+ * (stomp_precompute.cpp:48-101); both are formed in FP64, stored FP32.
+ * z ~ N(0, 1) comes from Philox4x32-10 (Salmon et al., SC'11) with the
+ * Box-Muller transform, counter (t >> 2, r | dof << 16, trajectory,
+ * iteration), key = seed; the reference draws from cuRAND XORWOW instead,
+ * so sampled rollouts match it in distribution, not value.
+ * d_record [n_traj][8] uses the PPL_CHOMP_* layout. Limits: rollouts
+ * (new + old + 1) <= 1024 and (rollouts + 1) * longest trajectory <= 16384.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int num_rollouts_new;                 /* 30 */
+    int num_rollouts_old;                 /* 20 */
+    float control_cost_weight;            /* 0.1 */
+    float collision_cost_weight;          /* 20 */
+    float exponentiated_cost_sensitivity; /* h, 2 */
+    float world_margin;                   /* 0.02 */
+    float self_margin;                    /* 0.01 */
+    float dt;                             /* 0.1 */
+    float noise_scale;                    /* 1 */
+    float collision_check_step;           /* 0.05; <= 0: waypoints only */
+    float collision_margin;               /* 0 */
+    float convergence_eps;                /* 1e-4 */
+    int iterations_after_valid;           /* 3 */
+    int use_self;                         /* self hinge in the state cost */
+    unsigned long long seed;              /* 0x5be5 */
+} ppl_stomp_settings;
+
+/* first_call = 1 initialises the records and evaluates d_theta; later calls
+ * continue the same population (settings must then be unchanged). */
+ppl_status ppl_stomp_optimize(ppl_plan plan, const ppl_scene_view* scene, const ppl_stomp_settings* settings,
+                              float* d_theta, float* d_best, float* d_record, int iterations, int first_call,
+                              ppl_stream stream);
+
+/* Single-goal STOMP solve, the reference's STOMP::solve_runtime_scene
+ * (planning/stomp.cu:565-930) and pybind stomp_solve_scene
+ * (python/bindings.cpp:801-831), with the same contract as ppl_chomp_solve. */
+typedef struct ppl_opt_buffers_s* ppl_stomp_buffers;
+ppl_status ppl_stomp_buffers_create(int n_dof, int num_timesteps, int num_rollouts_new, int num_rollouts_old,
+                                    ppl_stomp_buffers* out);
+ppl_status ppl_stomp_buffers_destroy(ppl_stomp_buffers bufs);
+typedef ppl_opt_result ppl_stomp_result;
+ppl_status ppl_stomp_solve(ppl_robot robot, ppl_stomp_buffers bufs, const ppl_scene_view* scene, const float* h_start,
+                           const float* h_goal, int num_timesteps, int num_iterations, float time_limit_ms,
+                           const ppl_stomp_settings* settings, float* h_path, ppl_stomp_result* result,
+                           ppl_stream stream);
+
 /* ESDF of a box world on a node grid: value = -min_o sdf_o(node), positive
  * inside. h_voxel_params [4] and h_voxel_pose [8] on the host; d_features
  * holds nx*ny*nz floats, index ix*ny*nz + iy*nz + iz. */

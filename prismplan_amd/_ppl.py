@@ -87,6 +87,19 @@ class ChompSettings(C.Structure):
                 ("fd_eps", C.c_float)]
 
 
+class StompSettings(C.Structure):
+    _fields_ = [("num_rollouts_new", C.c_int), ("num_rollouts_old", C.c_int), ("control_cost_weight", C.c_float),
+                ("collision_cost_weight", C.c_float), ("exponentiated_cost_sensitivity", C.c_float),
+                ("world_margin", C.c_float), ("self_margin", C.c_float), ("dt", C.c_float), ("noise_scale", C.c_float),
+                ("collision_check_step", C.c_float), ("collision_margin", C.c_float), ("convergence_eps", C.c_float),
+                ("iterations_after_valid", C.c_int), ("use_self", C.c_int), ("seed", C.c_ulonglong)]
+
+
+class ChompResult(C.Structure):
+    _fields_ = [("solved", C.c_int), ("iterations_run", C.c_int), ("final_state_cost", C.c_float),
+                ("final_control_cost", C.c_float), ("final_total_cost", C.c_float), ("wall_time_ms", C.c_double)]
+
+
 CHOMP_BEST_COST, CHOMP_PREV_COST, CHOMP_SOLVED, CHOMP_STREAK, CHOMP_POLISH_LEFT, CHOMP_DONE, CHOMP_ITERS, \
     CHOMP_LAST_COST = range(8)
 
@@ -130,6 +143,16 @@ SIGNATURES = [
                                           _vp, _vp, _vp]),
     ("ppl_chomp_optimize", C.c_int, [_vp, C.POINTER(SceneView), C.POINTER(ChompSettings), _vp, _vp, _vp, C.c_int,
                                      C.c_int, _vp]),
+    ("ppl_chomp_buffers_create", C.c_int, [C.c_int, C.c_int, C.POINTER(_vp)]),
+    ("ppl_chomp_buffers_destroy", C.c_int, [_vp]),
+    ("ppl_chomp_solve", C.c_int, [_vp, _vp, C.POINTER(SceneView), _f32p, _f32p, C.c_int, C.c_int, C.c_float,
+                                  C.POINTER(ChompSettings), _f32p, C.POINTER(ChompResult), _vp]),
+    ("ppl_stomp_optimize", C.c_int, [_vp, C.POINTER(SceneView), C.POINTER(StompSettings), _vp, _vp, _vp, C.c_int,
+                                     C.c_int, _vp]),
+    ("ppl_stomp_buffers_create", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    ("ppl_stomp_buffers_destroy", C.c_int, [_vp]),
+    ("ppl_stomp_solve", C.c_int, [_vp, _vp, C.POINTER(SceneView), _f32p, _f32p, C.c_int, C.c_int, C.c_float,
+                                  C.POINTER(StompSettings), _f32p, C.POINTER(ChompResult), _vp]),
     ("ppl_esdf_from_obbs", C.c_int, [C.POINTER(SceneView), _f32p, _f32p, _vp, _vp]),
     ("ppl_graph_capture_stomp", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), C.c_float,
                                           C.c_float, _vp, _vp, _vp, C.POINTER(_vp)]),

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -111,6 +112,25 @@ struct ppl_edge_eval_s {
     HBuf h_from, h_to, h_off, h_fh, h_clr;
 };
 
+// STOMP optimiser state kept with a trajectory plan between calls: rollout
+// plans over the fresh samples, per-length sampling / projection matrices,
+// the population buffers, and where the run is.
+struct StompState {
+    int n_new = 0, n_old = 0;
+    float dt = 0.0f, noise_scale = 0.0f;
+    ppl_plan first_plan = nullptr;   // n_new + n_old fresh rows per trajectory (first iteration)
+    ppl_plan fresh_plan = nullptr;   // n_new fresh rows per trajectory
+    std::vector<std::pair<int, float*>> mats;   // per length T: [L^T | M^T]
+    const float** d_lt = nullptr;   // [2 B]: L^T of each trajectory, then M^T
+    const float** d_mt = nullptr;
+    DBuf buf;
+    float *fresh = nullptr, *fresh_cost = nullptr, *keep[2] = {}, *keep_cost[2] = {};
+    float *slot = nullptr, *slot_cost = nullptr, *theta_cost = nullptr, *ctrl_sum = nullptr, *totals = nullptr;
+    int *sel = nullptr, *first_hit = nullptr, *nsub = nullptr, *dense_off = nullptr;
+    unsigned iter = 0;
+    int parity = 0, nf_prev = 0, started = 0;
+};
+
 struct ppl_plan_s {
     ppl_robot robot = nullptr;
     int kind = 0;
@@ -135,6 +155,7 @@ struct ppl_plan_s {
     const float** d_ainv_ptr = nullptr;
     float ainv_dt = 0.0f;
     DBuf d_opt;
+    StompState* stomp = nullptr;
 };
 
 struct ppl_graph_s {
@@ -900,6 +921,14 @@ ppl_status ppl_plan_destroy(ppl_plan P) {
     if (P->d_prob) (void)hipFree(P->d_prob);
     for (auto& e : P->ainv) (void)hipFree(e.second);
     if (P->d_ainv_ptr) (void)hipFree(P->d_ainv_ptr);
+    if (StompState* z = P->stomp) {
+        ppl_plan_destroy(z->first_plan);
+        ppl_plan_destroy(z->fresh_plan);
+        for (auto& e : z->mats) (void)hipFree(e.second);
+        if (z->d_lt) (void)hipFree((void*)z->d_lt);
+        z->buf.release();
+        delete z;
+    }
     P->d_opt.release();
     P->d_self.release(); P->d_cost.release(); P->d_fdcfg.release(); P->d_fdcost.release();
     delete P;
@@ -1097,15 +1126,15 @@ ppl_status stomp_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_
     return PPL_OK;
 }
 
-// (R + 1e-8 I)^-1 in FP64 for the 5-point acceleration metric R = dt (A^T A)
-// restricted to T x T from a grid padded by 6 (chomp_precompute.cpp:20-64),
-// by Cholesky factorisation; stored FP32 row-major.
-bool chomp_ainv(int T, float dt, std::vector<float>& out) {
+// R + 1e-8 I in FP64 for the 5-point acceleration metric R = dt (A^T A)
+// restricted to T x T from a grid padded by 6 (chomp_precompute.cpp:20-64,
+// stomp_precompute.cpp:24-44), row-major.
+void smooth_metric64(int T, float dt, std::vector<double>& R) {
     const int pad = 6, N = T + 2 * pad;
     const double c[5] = {-1.0 / 12.0, 16.0 / 12.0, -30.0 / 12.0, 16.0 / 12.0, -1.0 / 12.0};
     const double inv_dt2 = 1.0 / ((double)dt * (double)dt);
     // R[i][j] = dt * sum_r A[r][i + pad] A[r][j + pad], A[r][k] = inv_dt2 c[k - r + 2]
-    std::vector<double> R((size_t)T * T, 0.0);
+    R.assign((size_t)T * T, 0.0);
     for (int i = 0; i < T; ++i)
         for (int j = 0; j < T; ++j) {
             double acc = 0.0;
@@ -1116,21 +1145,32 @@ bool chomp_ainv(int T, float dt, std::vector<float>& out) {
             }
             R[(size_t)i * T + j] = (double)dt * inv_dt2 * inv_dt2 * acc + (i == j ? 1e-8 : 0.0);
         }
-    // Cholesky R = L L^T, then the columns of R^-1 by two triangular solves
-    std::vector<double> L((size_t)T * T, 0.0);
+}
+
+// Lower Cholesky factor of a symmetric positive definite T x T matrix (FP64).
+bool cholesky64(const std::vector<double>& A, int T, std::vector<double>& L) {
+    L.assign((size_t)T * T, 0.0);
     for (int j = 0; j < T; ++j) {
-        double d = R[(size_t)j * T + j];
+        double d = A[(size_t)j * T + j];
         for (int k = 0; k < j; ++k) d -= L[(size_t)j * T + k] * L[(size_t)j * T + k];
         if (!(d > 0.0)) return false;
         const double ljj = std::sqrt(d);
         L[(size_t)j * T + j] = ljj;
         for (int i = j + 1; i < T; ++i) {
-            double v = R[(size_t)i * T + j];
+            double v = A[(size_t)i * T + j];
             for (int k = 0; k < j; ++k) v -= L[(size_t)i * T + k] * L[(size_t)j * T + k];
             L[(size_t)i * T + j] = v / ljj;
         }
     }
-    out.assign((size_t)T * T, 0.0f);
+    return true;
+}
+
+// Inverse of a symmetric positive definite matrix: Cholesky, then two
+// triangular solves per column (FP64).
+bool spd_inverse64(const std::vector<double>& A, int T, std::vector<double>& inv) {
+    std::vector<double> L;
+    if (!cholesky64(A, T, L)) return false;
+    inv.assign((size_t)T * T, 0.0);
     std::vector<double> y(T), x(T);
     for (int col = 0; col < T; ++col) {
         for (int i = 0; i < T; ++i) {
@@ -1143,7 +1183,47 @@ bool chomp_ainv(int T, float dt, std::vector<float>& out) {
             for (int k = i + 1; k < T; ++k) v -= L[(size_t)k * T + i] * x[k];
             x[i] = v / L[(size_t)i * T + i];
         }
-        for (int i = 0; i < T; ++i) out[(size_t)i * T + col] = (float)x[i];
+        for (int i = 0; i < T; ++i) inv[(size_t)i * T + col] = x[i];
+    }
+    return true;
+}
+
+// (R + 1e-8 I)^-1, stored FP32 row-major (chomp_precompute.cpp:40-64).
+bool chomp_ainv(int T, float dt, std::vector<float>& out) {
+    std::vector<double> R, inv;
+    smooth_metric64(T, dt, R);
+    if (!spd_inverse64(R, T, inv)) return false;
+    out.assign(inv.begin(), inv.end());
+    return true;
+}
+
+// STOMP's sampling factor and projection (stomp_precompute.cpp:48-101), out =
+// [L^T | M^T] (2 T^2 floats, entry k * T + t of each = L[t][k] / M[t][k]):
+// L = chol(S) * noise_scale, S = R^-1 scaled so its largest entry is 1/T and
+// symmetrised; M = R^-1 with column j scaled so its largest entry is 1/T.
+bool stomp_mats(int T, float dt, float noise_scale, std::vector<float>& out) {
+    std::vector<double> R, inv, S, L;
+    smooth_metric64(T, dt, R);
+    if (!spd_inverse64(R, T, inv)) return false;
+    double mx = inv[0];
+    for (double v : inv) mx = std::max(mx, v);
+    const double sc = (1.0 / T) / std::max(mx, 1e-12);
+    S.resize(inv.size());
+    for (int i = 0; i < T; ++i)
+        for (int j = 0; j < T; ++j)
+            S[(size_t)i * T + j] = 0.5 * (inv[(size_t)i * T + j] * sc + inv[(size_t)j * T + i] * sc);
+    if (!cholesky64(S, T, L)) return false;
+    const double ns = noise_scale > 0.0f ? (double)noise_scale : 1.0;
+    out.assign(2 * (size_t)T * T, 0.0f);
+    float* Lt = out.data();
+    float* Mt = Lt + (size_t)T * T;
+    for (int t = 0; t < T; ++t)
+        for (int k = 0; k <= t; ++k) Lt[(size_t)k * T + t] = (float)(L[(size_t)t * T + k] * ns);
+    for (int j = 0; j < T; ++j) {
+        double cm = inv[j];
+        for (int i = 1; i < T; ++i) cm = std::max(cm, inv[(size_t)i * T + j]);
+        const double cs = (1.0 / T) / std::max(cm, 1e-12);
+        for (int i = 0; i < T; ++i) Mt[(size_t)j * T + i] = (float)(inv[(size_t)i * T + j] * cs);
     }
     return true;
 }
@@ -1176,6 +1256,174 @@ ppl_status ensure_ainv(ppl_plan_s* P, float dt) {
     return PPL_OK;
 }
 
+// Every trajectory has >= 2 waypoints and at most 4096; *max_t the longest.
+ppl_status traj_lengths(const ppl_plan_s* P, int* max_t) {
+    *max_t = 0;
+    for (int u = 0; u < P->n_units; ++u) {
+        const int T = P->h_off[u + 1] - P->h_off[u];
+        CHECK_ARG(T >= 2, "trajectory %d has fewer than 2 waypoints", u);
+        *max_t = std::max(*max_t, T);
+    }
+    CHECK_ARG(*max_t <= 4096, "trajectories longer than 4096 waypoints are not supported");
+    return PPL_OK;
+}
+
+// Densification cap: no segment of a trajectory inside the joint limits is
+// longer than the limit box's diagonal (times `slack`).
+int nsub_cap(const ppl_robot_s* r, float step, double slack) {
+    if (!(step > 0.0f)) return 1;
+    double diag2 = 0.0;
+    for (int k = 0; k < r->R.n_dof; ++k) diag2 += (double)(r->upper[k] - r->lower[k]) * (r->upper[k] - r->lower[k]);
+    return std::max(1, (int)std::ceil(slack * std::sqrt(diag2) / step));
+}
+
+// Densified validity of every trajectory of a plan (chomp.cu:297-351,
+// 459-481; stomp.cu:462-560, 662-686): every waypoint plus nsub - 1
+// interpolants per segment, nsub = ceil(max segment / step) from the current
+// theta on the device; environment first, then the self check on the
+// trajectories still free (the order of two_phase_cc,
+// collision/two_phase_cc.cuh:89-131). first_hit[b] = -1 when valid.
+struct DenseCheck {
+    CollideArgs va{};
+    DStates Xd{};
+    int vgrid = 0;
+    int* first_hit = nullptr;
+    long n_traj = 0;
+};
+
+ppl_status dense_check_setup(const ppl_plan_s* P, const DScene& S, float step, float margin, int nsub_max,
+                             int* first_hit, int* nsub, int* dense_off, ChompArgs& a, DenseCheck& dc) {
+    size_t n_dense = 0;
+    for (int u = 0; u < P->n_units; ++u) n_dense += (size_t)(P->h_off[u + 1] - P->h_off[u] - 1) * nsub_max + 1;
+    CHECK_ARG(n_dense < (size_t)INT32_MAX, "densified validity states exceed 2^31");
+    a.check_step = step;
+    a.margin = margin;
+    a.nsub_max = nsub_max;
+    a.first_hit = first_hit;
+    a.nsub = nsub;
+    a.dense_off = dense_off;
+    dc.Xd = a.X;
+    dc.Xd.source = kSrcDense;
+    dc.Xd.dense_off = dense_off;
+    dc.Xd.nsub = nsub;
+    dc.Xd.n_dyn = dense_off + P->n_units;
+    dc.Xd.n_states = (long)n_dense;
+    dc.va.R = P->robot->R;
+    dc.va.S = S;
+    dc.va.X = dc.Xd;
+    dc.va.C = implicit_chunks((long)n_dense, kKCollide, P->robot->R, S.n_obbs, &dc.vgrid);
+    dc.va.margin = margin;
+    dc.va.mode = 1;
+    dc.va.skip_resolved = 1;
+    dc.va.first_hit = first_hit;
+    dc.va.self_hit = nullptr;
+    dc.first_hit = first_hit;
+    dc.n_traj = P->n_units;
+    return PPL_OK;
+}
+
+ppl_status dense_check_run(const ppl_robot_s* r, const ChompArgs& a, DenseCheck& dc, hipStream_t s) {
+    HIP_TRY(launch_chomp_nsub(a, s));
+    HIP_TRY(launch_fill_i32(dc.first_hit, -1, dc.n_traj, s));
+    HIP_TRY(launch_collide(dc.va, dc.vgrid, s));
+    if (r->n_pairs > 0) return run_self(r, dc.Xd, 0, nullptr, nullptr, s, FLT_MAX, dc.first_hit);
+    return PPL_OK;
+}
+
+// Rollout plan over the fresh rows: nf rollouts of each trajectory, b-major,
+// cut waypoint-major inside each trajectory's rollout set.
+ppl_status stomp_rollout_plan(ppl_plan_s* P, int nf, ppl_plan* out) {
+    const int B = P->n_units;
+    std::vector<int> len((size_t)B * nf), pu(B + 1);
+    for (int b = 0; b < B; ++b) {
+        pu[b] = b * nf;
+        for (int k = 0; k < nf; ++k) len[(size_t)b * nf + k] = P->h_off[b + 1] - P->h_off[b];
+    }
+    pu[B] = B * nf;
+    ppl_plan_desc d{};
+    d.kind = PPL_PLAN_TRAJ;
+    d.n_units = B * nf;
+    d.unit_states = len.data();
+    d.n_problems = B;
+    d.problem_units = pu.data();
+    d.waypoint_major = 1;
+    return ppl_plan_create(P->robot, &d, out);
+}
+
+// STOMP state of a plan for these settings: rollout plans and buffers for
+// the rollout counts, matrices for (dt, noise_scale), reused when unchanged.
+ppl_status ensure_stomp(ppl_plan_s* P, const ppl_stomp_settings* ss) {
+    if (!P->stomp) {
+        P->stomp = new (std::nothrow) StompState;
+        CHECK_ARG(P->stomp, "out of host memory");
+    }
+    StompState* z = P->stomp;
+    const int B = P->n_units, D = P->robot->R.n_dof;
+    const size_t n = (size_t)P->n_states;
+    const int nn = ss->num_rollouts_new, no = ss->num_rollouts_old, n_roll = nn + no + 1;
+    if (!z->fresh_plan || z->n_new != nn || z->n_old != no) {
+        ppl_plan_destroy(z->first_plan);
+        ppl_plan_destroy(z->fresh_plan);
+        z->first_plan = z->fresh_plan = nullptr;
+        z->started = 0;
+        ppl_status st = stomp_rollout_plan(P, nn + no, &z->first_plan);
+        if (st != PPL_OK) return st;
+        if ((st = stomp_rollout_plan(P, nn, &z->fresh_plan)) != PPL_OK) return st;
+        const size_t nf = (size_t)(nn + no);
+        const size_t floats = nf * n * D + nf * n + 2 * ((size_t)no * n * D + (size_t)no * n) + n * D + 2 * n +
+                              (size_t)B * n_roll * D + B;
+        const size_t ints = (size_t)B * no + 3 * (size_t)B + 1;
+        HIP_TRY(z->buf.ensure((floats + ints) * sizeof(float) + 64));
+        float* f = z->buf.as<float>();
+        z->fresh = f; f += nf * n * D;
+        z->fresh_cost = f; f += nf * n;
+        for (int k = 0; k < 2; ++k) {
+            z->keep[k] = f; f += (size_t)no * n * D;
+            z->keep_cost[k] = f; f += (size_t)no * n;
+        }
+        z->slot = f; f += n * D;
+        z->slot_cost = f; f += n;
+        z->theta_cost = f; f += n;
+        z->ctrl_sum = f; f += (size_t)B * n_roll * D;
+        z->totals = f; f += B;
+        int* i = reinterpret_cast<int*>(f);
+        z->sel = i; i += (size_t)B * no;
+        z->first_hit = i; i += B;
+        z->nsub = i; i += B;
+        z->dense_off = i;
+        z->n_new = nn;
+        z->n_old = no;
+    }
+    if (!z->d_lt || z->dt != ss->dt || z->noise_scale != ss->noise_scale) {
+        for (auto& e : z->mats) (void)hipFree(e.second);
+        z->mats.clear();
+        std::vector<const float*> ptr(2 * (size_t)B);
+        for (int b = 0; b < B; ++b) {
+            const int T = P->h_off[b + 1] - P->h_off[b];
+            float* d = nullptr;
+            for (auto& e : z->mats)
+                if (e.first == T) d = e.second;
+            if (!d) {
+                std::vector<float> h;
+                if (!stomp_mats(T, ss->dt, ss->noise_scale, h))
+                    return fail(PPL_ERR_INVALID, "smoothness metric of length %d is not positive definite", T);
+                HIP_TRY(hipMalloc((void**)&d, h.size() * sizeof(float)));
+                z->mats.emplace_back(T, d);
+                HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+            }
+            ptr[b] = d;
+            ptr[(size_t)B + b] = d + (size_t)T * T;
+        }
+        if (!z->d_lt) HIP_TRY(hipMalloc((void**)&z->d_lt, 2 * (size_t)B * sizeof(float*)));
+        z->d_mt = z->d_lt + B;
+        HIP_TRY(hipMemcpy((void*)z->d_lt, ptr.data(), 2 * (size_t)B * sizeof(float*), hipMemcpyHostToDevice));
+        z->dt = ss->dt;
+        z->noise_scale = ss->noise_scale;
+        z->started = 0;
+    }
+    return PPL_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1189,76 +1437,41 @@ ppl_status ppl_chomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_
     CHECK_ARG(cs->dt > 0.0f, "dt must be positive");
     if (P->n_units == 0) return PPL_OK;
     int max_t = 0;
-    for (int u = 0; u < P->n_units; ++u) {
-        const int T = P->h_off[u + 1] - P->h_off[u];
-        CHECK_ARG(T >= 2, "trajectory %d has fewer than 2 waypoints", u);
-        max_t = std::max(max_t, T);
-    }
-    CHECK_ARG(max_t <= 4096, "trajectories longer than 4096 waypoints are not supported");
-    ppl_status st = ensure_ainv(P, cs->dt);
+    ppl_status st = traj_lengths(P, &max_t);
     if (st != PPL_OK) return st;
+    if ((st = ensure_ainv(P, cs->dt)) != PPL_OK) return st;
     DScene S{};
     if ((st = plan_scene(P, view, S)) != PPL_OK) return st;
     hipStream_t s = (hipStream_t)stream;
     const ppl_robot_s* r = P->robot;
     const int D = r->R.n_dof;
     const size_t n = (size_t)P->n_states, B = (size_t)P->n_units;
-    // Densification cap: no segment of a trajectory inside the joint limits is
-    // longer than the limit box's diagonal.
-    double diag2 = 0.0;
-    for (int k = 0; k < D; ++k) diag2 += (double)(r->upper[k] - r->lower[k]) * (r->upper[k] - r->lower[k]);
-    const int nsub_max = cs->collision_check_step > 0.0f
-                             ? std::max(1, (int)std::ceil(std::sqrt(diag2) / cs->collision_check_step)) : 1;
-    size_t n_dense = 0;
-    for (int u = 0; u < P->n_units; ++u) n_dense += (size_t)(P->h_off[u + 1] - P->h_off[u] - 1) * nsub_max + 1;
-    CHECK_ARG(n_dense < (size_t)INT32_MAX, "densified validity states exceed 2^31");
+    const int nsub_max = nsub_cap(r, cs->collision_check_step, 1.0);
     // scratch: gradient [n D], state cost [n], totals [B], first hit [B], nsub [B], dense offsets [B + 1]
     HIP_TRY(P->d_opt.ensure((n * D + n + B) * sizeof(float) + (3 * B + 1) * sizeof(int)));
     float* grad = P->d_opt.as<float>();
     float* state = grad + n * D;
     float* total = state + n;
     int* first_hit = reinterpret_cast<int*>(total + B);
-    int* nsub = first_hit + B;
-    int* dense_off = nsub + B;
     ppl_cost_params cp{cs->world_margin, cs->self_margin, cs->collision_cost_weight, 1, 1, cs->use_self};
     ChompArgs a{};
     a.R = P->robot->R;
     a.X = plan_states(P, d_theta);
     a.step_size = cs->step_size;
     a.smoothness_weight = cs->smoothness_weight;
-    a.check_step = cs->collision_check_step;
-    a.margin = cs->collision_margin;
     a.convergence_eps = cs->convergence_eps;
     a.dt = cs->dt;
     a.polish0 = cs->iterations_after_valid;
-    a.nsub_max = nsub_max;
     a.ainv = P->d_ainv_ptr;
     a.theta = d_theta;
     a.best = d_best;
     a.record = d_record;
     a.grad = grad;
     a.total = total;
-    a.first_hit = first_hit;
-    a.nsub = nsub;
-    a.dense_off = dense_off;
-    // validity states: every waypoint plus nsub - 1 interpolants per segment
-    DStates Xd = a.X;
-    Xd.source = kSrcDense;
-    Xd.dense_off = dense_off;
-    Xd.nsub = nsub;
-    Xd.n_dyn = dense_off + B;
-    Xd.n_states = (long)n_dense;
-    CollideArgs va{};
-    va.R = r->R;
-    va.S = S;
-    va.X = Xd;
-    int vgrid = 0;
-    va.C = implicit_chunks((long)n_dense, kKCollide, r->R, S.n_obbs, &vgrid);
-    va.margin = cs->collision_margin;
-    va.mode = 1;
-    va.skip_resolved = 1;
-    va.first_hit = first_hit;
-    va.self_hit = nullptr;
+    DenseCheck dc{};
+    if ((st = dense_check_setup(P, S, cs->collision_check_step, cs->collision_margin, nsub_max, first_hit,
+                                first_hit + B, first_hit + 2 * B, a, dc)) != PPL_OK)
+        return st;
     // cost (+ gradient) of the current theta, its totals and its validity
     auto evaluate = [&]() -> ppl_status {
         ppl_status e = cs->fd_gradient ? fd_impl(P, view, d_theta, &cp, cs->fd_eps, grad, s) : PPL_OK;
@@ -1266,16 +1479,7 @@ ppl_status ppl_chomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_
         e = hinge_impl(P, view, d_theta, &cp, state, total, cs->fd_gradient ? nullptr : grad, s);
         if (e != PPL_OK) return e;
         HIP_TRY(launch_control_cost(d_theta, P->d_off, P->n_units, D, cs->dt, cs->smoothness_weight, total, 1, s));
-        HIP_TRY(launch_chomp_nsub(a, s));
-        HIP_TRY(launch_fill_i32(first_hit, -1, (long)B, s));
-        // environment first; the self check then only visits trajectories still free
-        // (the order of two_phase_cc, collision/two_phase_cc.cuh:89-131)
-        HIP_TRY(launch_collide(va, vgrid, s));
-        if (r->n_pairs > 0) {
-            const ppl_status e2 = run_self(r, Xd, 0, nullptr, nullptr, s, FLT_MAX, first_hit);
-            if (e2 != PPL_OK) return e2;
-        }
-        return PPL_OK;
+        return dense_check_run(r, a, dc, s);
     };
     // the gradient at the current theta feeds the first step of this call
     if (first_call || iterations > 0) {
@@ -1286,6 +1490,115 @@ ppl_status ppl_chomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_
         HIP_TRY(launch_chomp_update(a, max_t, s));
         if ((st = evaluate()) != PPL_OK) return st;
         HIP_TRY(launch_chomp_book(a, 0, s));
+    }
+    return PPL_OK;
+}
+
+ppl_status ppl_stomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_stomp_settings* ss, float* d_theta,
+                              float* d_best, float* d_record, int iterations, int first_call, ppl_stream stream) {
+    CHECK_ARG(P && view && ss && d_theta && d_best && d_record, "NULL argument");
+    CHECK_ARG(P->kind == PPL_PLAN_TRAJ && !P->wmajor, "STOMP needs a (unit-major) trajectory plan");
+    CHECK_ARG(P->h_obb_off.empty(), "STOMP optimises trajectories of one scene (no per-problem OBB ranges)");
+    CHECK_ARG(iterations >= 0, "negative iteration count");
+    CHECK_ARG(ss->dt > 0.0f, "dt must be positive");
+    CHECK_ARG(ss->num_rollouts_new >= 1 && ss->num_rollouts_old >= 0, "need >= 1 new and >= 0 kept rollouts");
+    const int n_roll = ss->num_rollouts_new + ss->num_rollouts_old + 1;
+    CHECK_ARG(n_roll <= 1024, "at most 1024 rollouts per trajectory (new + old + 1 = %d)", n_roll);
+    CHECK_ARG(P->n_units <= 65535, "at most 65535 trajectories per STOMP plan");
+    if (P->n_units == 0) return PPL_OK;
+    int max_t = 0;
+    ppl_status st = traj_lengths(P, &max_t);
+    if (st != PPL_OK) return st;
+    if (stomp_update_lds(n_roll, max_t) > 65536)
+        return fail(PPL_ERR_CAPACITY, "(rollouts + 1) x longest trajectory = %d x %d exceeds 16384", n_roll + 1, max_t);
+    StompState* z = P->stomp;
+    if (!first_call) {
+        CHECK_ARG(z && z->started, "the first STOMP call on a plan must have first_call = 1");
+        CHECK_ARG(z->n_new == ss->num_rollouts_new && z->n_old == ss->num_rollouts_old && z->dt == ss->dt &&
+                      z->noise_scale == ss->noise_scale,
+                  "rollout counts, dt and noise_scale cannot change between calls of one run");
+    }
+    if ((st = ensure_stomp(P, ss)) != PPL_OK) return st;
+    z = P->stomp;
+    DScene S{};
+    if ((st = plan_scene(P, view, S)) != PPL_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    const ppl_robot_s* r = P->robot;
+    const int D = r->R.n_dof;
+    const int B = P->n_units;
+    ppl_cost_params cp{ss->world_margin, ss->self_margin, ss->collision_cost_weight, 1, 1, ss->use_self};
+    // theta may leave the joint-limit box (the update is not clamped): twice the diagonal caps a segment
+    const int nsub_max = nsub_cap(r, ss->collision_check_step, 2.0);
+    ChompArgs a{};
+    a.R = r->R;
+    a.X = plan_states(P, d_theta);
+    a.convergence_eps = ss->convergence_eps;
+    a.dt = ss->dt;
+    a.polish0 = ss->iterations_after_valid;
+    a.theta = d_theta;
+    a.best = d_best;
+    a.record = d_record;
+    a.total = z->totals;
+    DenseCheck dc{};
+    if ((st = dense_check_setup(P, S, ss->collision_check_step, ss->collision_margin, nsub_max, z->first_hit, z->nsub,
+                                z->dense_off, a, dc)) != PPL_OK)
+        return st;
+    auto evaluate = [&]() -> ppl_status {
+        ppl_status e = hinge_impl(P, view, d_theta, &cp, z->theta_cost, z->totals, nullptr, s);
+        if (e != PPL_OK) return e;
+        HIP_TRY(launch_control_cost(d_theta, P->d_off, B, D, ss->dt, ss->control_cost_weight, z->totals, 1, s));
+        return dense_check_run(r, a, dc, s);
+    };
+    if (first_call) {
+        z->iter = 0;
+        z->parity = 0;
+        z->nf_prev = 0;
+        z->started = 1;
+        if ((st = evaluate()) != PPL_OK) return st;
+        HIP_TRY(launch_chomp_book(a, 1, s));
+    }
+    StompArgs g{};
+    g.R = r->R;
+    g.B = B;
+    g.traj_off = P->d_off;
+    g.n_new = ss->num_rollouts_new;
+    g.n_old = ss->num_rollouts_old;
+    g.n_roll = n_roll;
+    g.w_u = ss->control_cost_weight;
+    g.h = ss->exponentiated_cost_sensitivity;
+    g.dt = ss->dt;
+    g.key0 = (unsigned)(ss->seed & 0xffffffffull);
+    g.key1 = (unsigned)(ss->seed >> 32);
+    g.Lt = z->d_lt;
+    g.Mt = z->d_mt;
+    g.theta = d_theta;
+    g.theta_cost = z->theta_cost;
+    g.record = d_record;
+    g.fresh = z->fresh;
+    g.slot = z->slot;
+    g.slot_cost = z->slot_cost;
+    g.ctrl_sum = z->ctrl_sum;
+    g.sel = z->sel;
+    for (int it = 0; it < iterations; ++it) {
+        const bool first = z->iter == 0;
+        g.iter = z->iter;
+        g.keep = z->keep[z->parity];
+        g.keep_cost = z->keep_cost[z->parity];
+        g.cur = StompPop{z->fresh, z->fresh_cost, first ? g.n_new + g.n_old : g.n_new, g.keep, g.keep_cost,
+                         first ? 0 : g.n_old, z->slot, z->slot_cost};
+        g.prev = StompPop{z->fresh, z->fresh_cost, z->nf_prev, z->keep[z->parity ^ 1], z->keep_cost[z->parity ^ 1],
+                          z->nf_prev == g.n_new + g.n_old ? 0 : g.n_old, z->slot, z->slot_cost};
+        if (!first) HIP_TRY(launch_stomp_reuse(g, max_t, s));
+        HIP_TRY(launch_stomp_sample(g, max_t, s));
+        if ((st = hinge_impl(first ? z->first_plan : z->fresh_plan, view, z->fresh, &cp, z->fresh_cost, nullptr,
+                             nullptr, s)) != PPL_OK)
+            return st;
+        HIP_TRY(launch_stomp_update(g, max_t, s));
+        if ((st = evaluate()) != PPL_OK) return st;
+        HIP_TRY(launch_chomp_book(a, 0, s));
+        z->nf_prev = g.cur.nf;
+        z->parity ^= 1;
+        ++z->iter;
     }
     return PPL_OK;
 }
@@ -1524,6 +1837,187 @@ ppl_status ppl_chomp_fd_grad(ppl_robot r, const ppl_scene_view* view, const floa
         if (e != hipSuccess) st = fail(PPL_ERR_HIP, "chomp_fd_grad: %s", hipGetErrorString(e));
     }
     return st;
+}
+
+}  // extern "C"
+
+struct ppl_opt_buffers_s {
+    int n_dof = 0, T = 0;
+    DBuf dev;    // theta [D T], best [D T], record [8], state / control totals [2]
+    HBuf host;   // record [8], totals [2], path [D T]
+    float* theta() const { return dev.as<float>(); }
+    float* best() const { return theta() + (size_t)n_dof * T; }
+    float* record() const { return best() + (size_t)n_dof * T; }
+    float* totals() const { return record() + 8; }
+    float* h_record() const { return host.as<float>(); }
+    float* h_totals() const { return h_record() + 8; }
+    float* h_path() const { return h_totals() + 2; }
+};
+
+namespace {
+ppl_status opt_buffers_alloc(int n_dof, int T, ppl_opt_buffers_s* b) {
+    b->n_dof = n_dof;
+    b->T = T;
+    const size_t floats = 2 * (size_t)n_dof * T + 10;
+    HIP_TRY(b->dev.ensure(floats * sizeof(float)));
+    HIP_TRY(b->host.ensure((10 + (size_t)n_dof * T) * sizeof(float)));
+    return PPL_OK;
+}
+
+ppl_status opt_buffers_create(int n_dof, int T, ppl_opt_buffers_s** out) {
+    CHECK_ARG(out, "NULL argument");
+    *out = nullptr;
+    CHECK_ARG(n_dof >= 1 && n_dof <= kMaxDof, "n_dof must be in [1, %d]", kMaxDof);
+    CHECK_ARG(T >= 2 && T <= 4096, "num_timesteps must be in [2, 4096]");
+    auto* b = new (std::nothrow) ppl_opt_buffers_s;
+    CHECK_ARG(b, "out of host memory");
+    const ppl_status st = opt_buffers_alloc(n_dof, T, b);
+    if (st != PPL_OK) {
+        b->dev.release();
+        b->host.release();
+        delete b;
+        return st;
+    }
+    *out = b;
+    return PPL_OK;
+}
+
+// Single-goal solve shared by CHOMP and STOMP (chomp.cu:358-610,
+// stomp.cu:565-930): clamped straight line, iterations of `optimize` in
+// chunks until the record is done (one by one under a time limit, checked
+// before each iteration), then the best valid trajectory or the working one
+// with its costs, endpoints set to start and goal.
+template <typename Optimize>
+ppl_status solve_single(ppl_robot r, ppl_opt_buffers_s* bufs, const ppl_scene_view* view, const float* h_start,
+                        const float* h_goal, int T, int num_iterations, float time_limit_ms,
+                        const ppl_cost_params& cp, float ctrl_weight, float dt, Optimize&& optimize, float* h_path,
+                        ppl_opt_result* res, hipStream_t s) {
+    const auto wall0 = std::chrono::steady_clock::now();
+    CHECK_ARG(r && view && h_start && h_goal && h_path && res, "NULL argument");
+    const int D = r->R.n_dof;
+    CHECK_ARG(T >= 2 && T <= 4096, "num_timesteps must be in [2, 4096]");
+    CHECK_ARG(num_iterations >= 0, "negative iteration count");
+    CHECK_ARG(!bufs || (bufs->n_dof == D && bufs->T == T),
+              "buffers were created for n_dof %d, T %d; the solve needs n_dof %d, T %d", bufs ? bufs->n_dof : 0,
+              bufs ? bufs->T : 0, D, T);
+    ppl_opt_buffers_s local;
+    ppl_opt_buffers_s* b = bufs;
+    ppl_status st = PPL_OK;
+    if (!b) {
+        if ((st = opt_buffers_alloc(D, T, &local)) != PPL_OK) {
+            local.dev.release();
+            local.host.release();
+            return st;
+        }
+        b = &local;
+    }
+    // the temporary buffers of a call without `bufs` live until the call returns
+    struct Release {
+        ppl_opt_buffers_s* b;
+        ~Release() {
+            if (b) {
+                b->dev.release();
+                b->host.release();
+            }
+        }
+    } release{bufs ? nullptr : &local};
+    ppl_plan P = nullptr;
+    if ((st = single_traj_plan(r, T, &P)) != PPL_OK) return st;
+    // chomp.cu:57-69 / stomp.cu:58-70: joint-space line start -> goal, clamped
+    float* h_theta = b->h_path();
+    for (int d = 0; d < D; ++d)
+        for (int t = 0; t < T; ++t) {
+            const float frac = (float)t / (float)(T - 1);
+            const float v = fmaf(h_goal[d], frac, h_start[d] * (1.0f - frac));
+            h_theta[(size_t)d * T + t] = std::max(r->lower[d], std::min(r->upper[d], v));
+        }
+    HIP_TRY(hipMemcpyAsync(b->theta(), h_theta, (size_t)D * T * sizeof(float), hipMemcpyHostToDevice, s));
+    // The device stops updating a finished trajectory by itself; the chunks
+    // only bound the launches queued after it finished.
+    const int chunk = time_limit_ms > 0.0f ? 1 : 8;
+    int done_iters = 0;
+    for (bool first = true;; first = false) {
+        const int k = std::min(chunk, num_iterations - done_iters);
+        if (!first && k <= 0) break;
+        if (!first && time_limit_ms > 0.0f) {
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+            if (ms > time_limit_ms) break;
+        }
+        if ((st = optimize(P, b->theta(), b->best(), b->record(), k, first ? 1 : 0, s)) != PPL_OK) return st;
+        done_iters += k;
+        HIP_TRY(hipMemcpyAsync(b->h_record(), b->record(), 8 * sizeof(float), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (b->h_record()[PPL_CHOMP_DONE] != 0.0f || done_iters >= num_iterations) break;
+    }
+    // chomp.cu:560-600, stomp.cu:889-925
+    const bool solved = b->h_record()[PPL_CHOMP_SOLVED] != 0.0f;
+    const float* d_res = solved ? b->best() : b->theta();
+    if ((st = hinge_impl(P, view, d_res, &cp, nullptr, b->totals(), nullptr, s)) != PPL_OK) return st;
+    HIP_TRY(launch_control_cost(d_res, P->d_off, 1, D, dt, ctrl_weight, b->totals() + 1, 0, s));
+    HIP_TRY(hipMemcpyAsync(b->h_totals(), b->totals(), 2 * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(b->h_path(), d_res, (size_t)D * T * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int t = 0; t < T; ++t)
+        for (int d = 0; d < D; ++d) h_path[(size_t)t * D + d] = b->h_path()[(size_t)d * T + t];
+    for (int d = 0; d < D; ++d) {
+        h_path[d] = h_start[d];
+        h_path[(size_t)(T - 1) * D + d] = h_goal[d];
+    }
+    res->solved = solved ? 1 : 0;
+    res->iterations_run = (int)b->h_record()[PPL_CHOMP_ITERS];
+    res->final_state_cost = b->h_totals()[0];
+    res->final_control_cost = b->h_totals()[1];
+    res->final_total_cost = solved ? b->h_record()[PPL_CHOMP_BEST_COST] : b->h_record()[PPL_CHOMP_LAST_COST];
+    res->wall_time_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    return PPL_OK;
+}
+}  // namespace
+
+extern "C" {
+
+ppl_status ppl_chomp_buffers_create(int n_dof, int num_timesteps, ppl_chomp_buffers* out) {
+    return opt_buffers_create(n_dof, num_timesteps, out);
+}
+
+ppl_status ppl_stomp_buffers_create(int n_dof, int num_timesteps, int num_rollouts_new, int num_rollouts_old,
+                                    ppl_stomp_buffers* out) {
+    CHECK_ARG(num_rollouts_new >= 1 && num_rollouts_old >= 0 && num_rollouts_new + num_rollouts_old + 1 <= 1024,
+              "need 1 <= new, 0 <= old, new + old + 1 <= 1024 rollouts");
+    return opt_buffers_create(n_dof, num_timesteps, out);
+}
+
+ppl_status ppl_chomp_buffers_destroy(ppl_chomp_buffers b) {
+    if (!b) return PPL_OK;
+    b->dev.release();
+    b->host.release();
+    delete b;
+    return PPL_OK;
+}
+
+ppl_status ppl_stomp_buffers_destroy(ppl_stomp_buffers b) { return ppl_chomp_buffers_destroy(b); }
+
+ppl_status ppl_chomp_solve(ppl_robot r, ppl_chomp_buffers bufs, const ppl_scene_view* view, const float* h_start,
+                           const float* h_goal, int num_timesteps, int num_iterations, float time_limit_ms,
+                           const ppl_chomp_settings* cs, float* h_path, ppl_chomp_result* res, ppl_stream stream) {
+    CHECK_ARG(cs, "NULL argument");
+    const ppl_cost_params cp{cs->world_margin, cs->self_margin, cs->collision_cost_weight, 1, 1, cs->use_self};
+    auto opt = [&](ppl_plan P, float* th, float* best, float* rec, int k, int first, hipStream_t s) {
+        return ppl_chomp_optimize(P, view, cs, th, best, rec, k, first, s);
+    };
+    return solve_single(r, bufs, view, h_start, h_goal, num_timesteps, num_iterations, time_limit_ms, cp,
+                        cs->smoothness_weight, cs->dt, opt, h_path, res, (hipStream_t)stream);
+}
+
+ppl_status ppl_stomp_solve(ppl_robot r, ppl_stomp_buffers bufs, const ppl_scene_view* view, const float* h_start,
+                           const float* h_goal, int num_timesteps, int num_iterations, float time_limit_ms,
+                           const ppl_stomp_settings* ss, float* h_path, ppl_stomp_result* res, ppl_stream stream) {
+    CHECK_ARG(ss, "NULL argument");
+    const ppl_cost_params cp{ss->world_margin, ss->self_margin, ss->collision_cost_weight, 1, 1, ss->use_self};
+    auto opt = [&](ppl_plan P, float* th, float* best, float* rec, int k, int first, hipStream_t s) {
+        return ppl_stomp_optimize(P, view, ss, th, best, rec, k, first, s);
+    };
+    return solve_single(r, bufs, view, h_start, h_goal, num_timesteps, num_iterations, time_limit_ms, cp,
+                        ss->control_cost_weight, ss->dt, opt, h_path, res, (hipStream_t)stream);
 }
 
 ppl_status ppl_esdf_from_obbs(const ppl_scene_view* view, const float* h_params, const float* h_pose, float* d_features,

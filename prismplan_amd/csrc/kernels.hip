@@ -506,6 +506,12 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
 // ===========================================================================
 // Any-collision: two-phase gate semantics (two_phase_cc.cuh:57-138 with the
 // check kernel's environment pass, pRRTC.cu:2337-2386).
@@ -1268,6 +1274,201 @@ __global__ __launch_bounds__(256) void k_chomp_update(ChompArgs a) {
 }
 
 // ===========================================================================
+// STOMP optimiser (planning/stomp.cu:565-930). Population rows are [D][T_b]
+// blocks; every kernel finds a slot's row through stomp_row.
+// ===========================================================================
+__device__ __forceinline__ void stomp_row(const StompPop& p, int off_b, int T, int D, int r, int n_roll,
+                                          const float** x, const float** cost) {
+    size_t base;
+    if (r < p.nf) {
+        base = (size_t)p.nf * off_b + (size_t)r * T;
+        *x = p.fresh + base * D;
+        *cost = p.fresh_cost + base;
+    } else if (r < n_roll - 1) {
+        base = (size_t)p.nkeep * off_b + (size_t)(r - p.nf) * T;
+        *x = p.keep + base * D;
+        *cost = p.keep_cost + base;
+    } else {
+        *x = p.slot + (size_t)off_b * D;
+        *cost = p.slot_cost + off_b;
+    }
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): ten rounds of two
+// 32 x 32 -> 64-bit products, the key advanced by the Weyl constants.
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, unsigned k0, unsigned k1) {
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        if (i) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const unsigned hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        const unsigned hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    }
+    return c;
+}
+
+// z ~ N(0, 1) for waypoint t of row (b, r, d): Box-Muller on the Philox words
+// of counter (t >> 2, r | d << 16, b, iter); t % 4 picks the pair and cos / sin.
+__device__ __forceinline__ float stomp_normal(const StompArgs& a, int b, int r, int d, int t) {
+    const uint4 w = philox4x32_10(make_uint4((unsigned)t >> 2, (unsigned)r | ((unsigned)d << 16), (unsigned)b, a.iter),
+                                  a.key0, a.key1);
+    const int j = t & 3;
+    const unsigned w1 = j < 2 ? w.x : w.z, w2 = j < 2 ? w.y : w.w;
+    const float u1 = ((float)(w1 >> 9) + 0.5f) * 1.1920928955078125e-7f;   // (0, 1), exact
+    const float u2 = ((float)(w2 >> 9) + 0.5f) * 1.1920928955078125e-7f;
+    const float rad = sqrtf(-2.0f * logf(u1));
+    float sn, cs;
+    sincospif(2.0f * u2, &sn, &cs);
+    return rad * ((j & 1) ? sn : cs);
+}
+
+// Kept rollouts (stomp.cu:131-170): the n_old cheapest slots of the previous
+// population by total cost (state sum + control sum), ties to the lower slot,
+// copied with their state costs into slots nf.. of the current one.
+__global__ __launch_bounds__(64) void k_stomp_select(StompArgs a) {
+    extern __shared__ float s_tot[];
+    const int b = blockIdx.x;
+    if (b >= a.B) return;
+    const int lane = lane_id();
+    const int D = a.R.n_dof, off = a.traj_off[b], T = a.traj_off[b + 1] - off;
+    for (int r = lane; r < a.n_roll; r += kWave) {
+        const float *x, *c;
+        stomp_row(a.prev, off, T, D, r, a.n_roll, &x, &c);
+        float s = 0.0f;
+        for (int t = 0; t < T; ++t) s += c[t];
+        const float* cs = a.ctrl_sum + ((size_t)b * a.n_roll + r) * D;
+        for (int d = 0; d < D; ++d) s += cs[d];
+        s_tot[r] = s == s ? s : INFINITY;
+    }
+    __syncthreads();
+    for (int r = lane; r < a.n_roll; r += kWave) {
+        const float v = s_tot[r];
+        int rank = 0;
+        for (int j = 0; j < a.n_roll; ++j) {
+            const float u = s_tot[j];
+            rank += (u < v || (u == v && j < r)) ? 1 : 0;
+        }
+        if (rank < a.n_old) a.sel[(size_t)b * a.n_old + rank] = r;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_stomp_gather(StompArgs a) {
+    const int k = blockIdx.x, b = blockIdx.y;
+    const int D = a.R.n_dof, off = a.traj_off[b], T = a.traj_off[b + 1] - off;
+    const int src = a.sel[(size_t)b * a.n_old + k];
+    const float *x, *c;
+    stomp_row(a.prev, off, T, D, src, a.n_roll, &x, &c);
+    const size_t base = (size_t)a.n_old * off + (size_t)k * T;
+    float* dx = a.keep + base * D;
+    float* dc = a.keep_cost + base;
+    for (int i = threadIdx.x; i < D * T; i += blockDim.x) dx[i] = x[i];
+    for (int i = threadIdx.x; i < T; i += blockDim.x) dc[i] = c[i];
+}
+
+// The optimum as the last slot (stomp.cu:172-189).
+__global__ __launch_bounds__(256) void k_stomp_seed(StompArgs a) {
+    const int b = blockIdx.x;
+    const int D = a.R.n_dof, off = a.traj_off[b], T = a.traj_off[b + 1] - off;
+    const size_t o = (size_t)off * D;
+    for (int i = threadIdx.x; i < D * T; i += blockDim.x) a.slot[o + i] = a.theta[o + i];
+    for (int i = threadIdx.x; i < T; i += blockDim.x) a.slot_cost[off + i] = a.theta_cost[off + i];
+}
+
+// Fresh sample rows (stomp.cu:100-127): x = clamp(theta + L z) with the
+// endpoints pinned; one wave per (dof, slot, trajectory).
+__global__ __launch_bounds__(64) void k_stomp_sample(StompArgs a) {
+    extern __shared__ float s_z[];
+    const int d = blockIdx.x, r = blockIdx.y, b = blockIdx.z;
+    const int lane = lane_id();
+    const int D = a.R.n_dof, off = a.traj_off[b], T = a.traj_off[b + 1] - off;
+    for (int t = lane; t < T; t += kWave) s_z[t] = stomp_normal(a, b, r, d, t);
+    __syncthreads();
+    const float* Lt = a.Lt[b];
+    const float* th = a.theta + (size_t)off * D + (size_t)d * T;
+    float* out = a.fresh + ((size_t)a.cur.nf * off + (size_t)r * T) * D + (size_t)d * T;
+    const float2 lim = a.R.lim[d];
+    for (int t = lane; t < T; t += kWave) {
+        float e = 0.0f;
+        for (int k = 0; k <= t; ++k) e += Lt[(size_t)k * T + t] * s_z[k];
+        if (t == 0 || t == T - 1) e = 0.0f;
+        out[t] = fmaxf(lim.x, fminf(lim.y, th[t] + e));
+    }
+}
+
+// Per (dof, trajectory): control cost of every population row, the
+// exponentiated-cost probabilities per waypoint, the weighted noise average,
+// its M projection and the unconditional step (stomp.cu:255-403, 406-460).
+// Control: w x_t (R x)_t with R = dt A^T A, (A^T A x)_t the 9-point
+// autocorrelation stencil of the 5-point acceleration rule, zero beyond the
+// ends (the padded-grid block of stomp_precompute.cpp:24-44).
+__global__ __launch_bounds__(256) void k_stomp_update(StompArgs a) {
+    extern __shared__ float s_u[];
+    const int d = blockIdx.x, b = blockIdx.y;
+    const int D = a.R.n_dof, off = a.traj_off[b], T = a.traj_off[b + 1] - off;
+    const int NR = a.n_roll;
+    float* tot = s_u;             // [NR][T]: total cost, then the unnormalised weight
+    float* delta = tot + NR * T;  // [T]
+    const int lane = lane_id(), wave = threadIdx.x / kWave, n_waves = blockDim.x / kWave;
+    const float scale = 1.0f / (144.0f * a.dt * a.dt * a.dt);
+    for (int r = wave; r < NR; r += n_waves) {
+        const float *x, *c;
+        stomp_row(a.cur, off, T, D, r, NR, &x, &c);
+        x += (size_t)d * T;
+        auto X = [&](int k) { return (k >= 0 && k < T) ? x[k] : 0.0f; };
+        float part = 0.0f;
+        for (int t = lane; t < T; t += kWave) {
+            const float xt = x[t];
+            // the weights sum to zero, so the stencil is taken over x_{t+k} - x_t:
+            // FP32 error then scales with the local variation, not with |x|
+            auto P2 = [&](int k) { return (X(t - k) - xt) + (X(t + k) - xt); };
+            const float st = P2(4) - 32.0f * P2(3) + 316.0f * P2(2) - 992.0f * P2(1);
+            const float ctrl = a.w_u * xt * (st * scale);
+            tot[r * T + t] = c[t] + ctrl;
+            part += ctrl;
+        }
+        part = wave_sum(part);
+        if (lane == 0) a.ctrl_sum[((size_t)b * NR + r) * D + d] = part;
+    }
+    __syncthreads();
+    const float* th = a.theta + (size_t)off * D + (size_t)d * T;
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        float mn = tot[t], mx = tot[t];
+        for (int r = 1; r < NR; ++r) {
+            mn = fminf(mn, tot[r * T + t]);
+            mx = fmaxf(mx, tot[r * T + t]);
+        }
+        const float denom = fmaxf(mx - mn, 1.0e-8f);
+        float sum = 0.0f;
+        for (int r = 0; r < NR; ++r) {
+            const float u = expf(-a.h * (tot[r * T + t] - mn) / denom);
+            tot[r * T + t] = u;
+            sum += u;
+        }
+        sum = fmaxf(sum, 1.0e-12f);
+        const float tht = th[t];
+        float acc = 0.0f;
+        for (int r = 0; r < NR; ++r) {
+            const float *x, *c;
+            stomp_row(a.cur, off, T, D, r, NR, &x, &c);
+            acc += (x[(size_t)d * T + t] - tht) * (tot[r * T + t] / sum);
+        }
+        delta[t] = (t == 0 || t == T - 1) ? 0.0f : acc;
+    }
+    __syncthreads();
+    if (a.record[8 * (size_t)b + PPL_CHOMP_DONE] != 0.0f) return;
+    const float* Mt = a.Mt[b];
+    float* row = a.theta + (size_t)off * D + (size_t)d * T;
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        float acc = 0.0f;
+        for (int u = 0; u < T; ++u) acc += Mt[(size_t)u * T + t] * delta[u];
+        if (t != 0 && t != T - 1) row[t] = th[t] + acc;
+    }
+}
+
+// ===========================================================================
 // Small helpers
 // ===========================================================================
 __global__ void k_fill_i32(int* p, int v, long n) {
@@ -1463,6 +1664,28 @@ hipError_t launch_chomp_update(const ChompArgs& a, int max_t, hipStream_t s) {
     if (a.X.n_traj <= 0) return hipSuccess;
     const size_t smem = sizeof(float) * (3 * (size_t)max_t + 12);
     hipLaunchKernelGGL(k_chomp_update, dim3(a.R.n_dof, a.X.n_traj), dim3(256), smem, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_stomp_reuse(const StompArgs& a, int, hipStream_t s) {
+    if (a.B <= 0 || a.n_old <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stomp_select, dim3(a.B), dim3(64), sizeof(float) * (size_t)a.n_roll, s, a);
+    hipLaunchKernelGGL(k_stomp_gather, dim3(a.n_old, a.B), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_stomp_sample(const StompArgs& a, int max_t, hipStream_t s) {
+    if (a.B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stomp_seed, dim3(a.B), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_stomp_sample, dim3(a.R.n_dof, a.cur.nf, a.B), dim3(64), sizeof(float) * (size_t)max_t, s, a);
+    return hipGetLastError();
+}
+
+size_t stomp_update_lds(int n_roll, int max_t) { return sizeof(float) * ((size_t)n_roll + 1) * (size_t)max_t; }
+
+hipError_t launch_stomp_update(const StompArgs& a, int max_t, hipStream_t s) {
+    if (a.B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stomp_update, dim3(a.R.n_dof, a.B), dim3(256), stomp_update_lds(a.n_roll, max_t), s, a);
     return hipGetLastError();
 }
 

@@ -167,6 +167,33 @@ hipError_t launch_chomp_nsub(const ChompArgs& a, hipStream_t s);   // nsub + den
 hipError_t launch_chomp_book(const ChompArgs& a, int initial, hipStream_t s);
 hipError_t launch_chomp_update(const ChompArgs& a, int max_t, hipStream_t s);
 
+// STOMP optimiser population of trajectory b (waypoint offset off_b, length
+// T_b): slot r < nf is a fresh sample, nf <= r < n_roll - 1 a rollout kept
+// from the previous population, n_roll - 1 the optimum it was drawn around.
+struct StompPop {
+    const float* fresh; const float* fresh_cost; int nf;   // [B][nf][D][T_b] and [B][nf][T_b], b-major
+    const float* keep; const float* keep_cost; int nkeep;  // [B][nkeep][D][T_b] and [B][nkeep][T_b]
+    const float* slot; const float* slot_cost;             // optimum snapshot, plan layout
+};
+struct StompArgs {
+    DRobot R;
+    int B; const int* traj_off;       // trajectory plan: [B + 1] waypoint offsets
+    int n_new, n_old, n_roll;         // n_roll = n_new + n_old + 1
+    float w_u, h, dt;                 // control weight, cost sensitivity, metric step
+    unsigned key0, key1, iter;        // Philox key and the iteration word of the counter
+    const float* const* Lt;           // [B] sampling factor L^T (T x T, entry k * T + t = L[t][k])
+    const float* const* Mt;           // [B] projection M^T (entry u * T + t = M[t][u])
+    float* theta; const float* theta_cost; const float* record;
+    StompPop cur, prev;
+    float* fresh; float* keep; float* keep_cost; float* slot; float* slot_cost;   // writable views of cur
+    float* ctrl_sum;                  // [B][n_roll][D]: sum_t of each row's control cost
+    int* sel;                         // [B][n_old]: previous-population slots kept, cheapest first
+};
+hipError_t launch_stomp_reuse(const StompArgs& a, int max_t, hipStream_t s);    // select + gather kept rollouts
+hipError_t launch_stomp_sample(const StompArgs& a, int max_t, hipStream_t s);   // optimum snapshot + fresh samples
+hipError_t launch_stomp_update(const StompArgs& a, int max_t, hipStream_t s);   // weights, projection, step
+size_t stomp_update_lds(int n_roll, int max_t);
+
 // Joint frames [n][J][16] (column-major 4x4, original joint order) and / or
 // sphere centres [n][S][4] (x, y, z, r, original order) of configurations.
 hipError_t launch_fk_out(const DRobot& R, const DStates& X, float* frames, float* spheres, hipStream_t s);

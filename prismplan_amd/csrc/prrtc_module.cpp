@@ -2,7 +2,9 @@
 //
 // Mirrors the builder / check surface of the reference's pybind module
 // (python/bindings.cpp:118-222 RobotModelBuilder, 363-402 and 992-1006
-// SceneCollisionDataBuilder, 493-530 and 1033-1037 check_collision_scene):
+// SceneCollisionDataBuilder, 493-530 and 1033-1037 check_collision_scene,
+// 764-902 and 1191-1282 STOMP / CHOMP settings, results, buffers and
+// stomp_solve_scene / chomp_solve_scene):
 // the same class and field names, int64 device-pointer fields filled from
 // torch.Tensor.data_ptr(), and shape errors raised as RuntimeError. Host C++
 // only: every device operation goes through libppl.so (include/ppl/ppl.h).
@@ -151,6 +153,210 @@ py::array_t<bool> check_collision_scene(py::array_t<float, py::array::c_style | 
     return out;
 }
 
+// CHOMP_settings (planning/CHOMP_settings.hh:21-57) with the two fields the
+// solver takes from pRRTC_settings (time_limit_ms, collision_margin), same
+// names and defaults.
+struct CHOMPSettings {
+    float time_limit_ms = 4000.0f;
+    float collision_margin = 0.0f;
+    int num_timesteps = 51;
+    int num_iterations = 100;
+    int num_iterations_after_valid = 5;
+    float step_size = 0.3f;
+    float smoothness_weight = 1.0f;
+    float collision_cost_weight = 20.0f;
+    float fd_epsilon = 1.0e-3f;
+    float world_collision_margin = 0.02f;
+    float self_collision_margin = 0.01f;
+    float delta_t = 0.1f;
+    float convergence_eps = 1.0e-4f;
+    float collision_free_tol = 1.0e-6f;
+    float collision_check_step = 0.05f;
+    bool fd_gradient = false;   // true: the reference's central-FD obstacle gradient; false: the fused adjoint
+};
+
+// STOMP_settings (planning/STOMP_settings.hh:17-62) with time_limit_ms and
+// collision_margin from pRRTC_settings, same names and defaults; `seed` keys
+// the sampler.
+struct STOMPSettings {
+    float time_limit_ms = 4000.0f;
+    float collision_margin = 0.0f;
+    int num_batch = 1;
+    int num_timesteps = 51;
+    int num_iterations = 20;
+    int num_iterations_after_valid = 3;
+    int num_rollouts_new = 30;
+    int num_rollouts_old = 20;
+    float delta_t = 0.1f;
+    float control_cost_weight = 0.1f;
+    float collision_cost_weight = 20.0f;
+    float exponentiated_cost_sensitivity = 2.0f;
+    float world_collision_margin = 0.02f;
+    float self_collision_margin = 0.01f;
+    float convergence_eps = 1.0e-4f;
+    float collision_free_tol = 1.0e-6f;
+    float noise_scale = 1.0f;
+    float collision_check_step = 0.05f;
+    unsigned long long seed = 0x5be5ull;
+};
+
+// CHOMPResult / STOMPResult (python/bindings.cpp:764-797, 837-868).
+struct OptResult {
+    bool solved = false;
+    py::array_t<float> path;   // (T, n_dof), start -> goal
+    int iterations_run = 0;
+    float final_state_cost = 0.0f, final_control_cost = 0.0f, final_total_cost = 0.0f;
+    double wall_time_ms = 0.0;
+};
+struct CHOMPResult : OptResult {};
+struct STOMPResult : OptResult {};
+
+// CHOMPBuffers / STOMPBuffers (planning/chomp_buffers.hh:20-80,
+// planning/stomp_buffers.hh:9-140): device buffers reused across solves of
+// one shape.
+struct OptBuffers {
+    ppl_chomp_buffers h = nullptr;
+    int n_dof = 0, num_timesteps = 0;
+    bool owns_memory() const { return h != nullptr; }
+    void destroy() {
+        if (h) ppl_chomp_buffers_destroy(h);
+        h = nullptr;
+    }
+    ~OptBuffers() { destroy(); }
+};
+struct CHOMPBuffers : OptBuffers {
+    static CHOMPBuffers* create(int n_dof, int num_timesteps) {
+        auto* b = new CHOMPBuffers;
+        const ppl_status st = ppl_chomp_buffers_create(n_dof, num_timesteps, &b->h);
+        if (st != PPL_OK) {
+            delete b;
+            check(st, "CHOMPBuffers.create");
+        }
+        b->n_dof = n_dof;
+        b->num_timesteps = num_timesteps;
+        return b;
+    }
+};
+struct STOMPBuffers : OptBuffers {
+    int num_rollouts_new = 0, num_rollouts_old = 0, num_batch = 1;
+    int num_rollouts_all() const { return num_rollouts_new + num_rollouts_old + 1; }
+    static STOMPBuffers* create(int n_dof, int num_timesteps, int num_rollouts_new, int num_rollouts_old,
+                                int num_batch) {
+        if (num_batch != 1) throw std::runtime_error("STOMPBuffers.create: num_batch must be 1 (single goal)");
+        auto* b = new STOMPBuffers;
+        const ppl_status st =
+            ppl_stomp_buffers_create(n_dof, num_timesteps, num_rollouts_new, num_rollouts_old, &b->h);
+        if (st != PPL_OK) {
+            delete b;
+            check(st, "STOMPBuffers.create");
+        }
+        b->n_dof = n_dof;
+        b->num_timesteps = num_timesteps;
+        b->num_rollouts_new = num_rollouts_new;
+        b->num_rollouts_old = num_rollouts_old;
+        return b;
+    }
+};
+
+using F32Array = py::array_t<float, py::array::c_style | py::array::forcecast>;
+
+template <typename Result, typename Solve>
+Result solve_scene(F32Array start, F32Array goal, RobotModelBuilder& robot, int T, Solve&& solve, const char* what) {
+    robot.build();
+    const int D = robot.n_dof;
+    if (start.ndim() != 1 || (int)start.shape(0) != D) throw std::runtime_error("start size mismatch with robot n_dof");
+    if (goal.ndim() != 1 || (int)goal.shape(0) != D) throw std::runtime_error("goal size mismatch with robot n_dof");
+    if (T < 2) throw std::runtime_error("num_timesteps must be >= 2");
+    Result res;
+    res.path = py::array_t<float>({T, D});
+    ppl_opt_result r{};
+    float* path = res.path.mutable_data();
+    {
+        py::gil_scoped_release nogil;
+        check(solve(start.data(), goal.data(), path, &r), what);
+    }
+    res.solved = r.solved != 0;
+    res.iterations_run = r.iterations_run;
+    res.final_state_cost = r.final_state_cost;
+    res.final_control_cost = r.final_control_cost;
+    res.final_total_cost = r.final_total_cost;
+    res.wall_time_ms = r.wall_time_ms;
+    return res;
+}
+
+// destroyed or absent buffers: the call allocates its own
+template <typename Buffers>
+ppl_chomp_buffers buffers_of(const py::object& o) {
+    return o.is_none() ? nullptr : o.cast<Buffers*>()->h;
+}
+
+// chomp_solve_scene (python/bindings.cpp:872-902): single-goal CHOMP from start to goal.
+CHOMPResult chomp_solve_scene(F32Array start, F32Array goal, const SceneCollisionDataBuilder& scene,
+                              const CHOMPSettings& st, RobotModelBuilder& robot, py::object bufs_obj) {
+    ppl_chomp_settings cs{};
+    cs.step_size = st.step_size;
+    cs.smoothness_weight = st.smoothness_weight;
+    cs.collision_cost_weight = st.collision_cost_weight;
+    cs.world_margin = st.world_collision_margin;
+    cs.self_margin = st.self_collision_margin;
+    cs.dt = st.delta_t;
+    cs.collision_check_step = st.collision_check_step;
+    cs.collision_margin = st.collision_margin;
+    cs.convergence_eps = st.convergence_eps;
+    cs.iterations_after_valid = st.num_iterations_after_valid;
+    cs.use_self = 1;
+    cs.fd_gradient = st.fd_gradient ? 1 : 0;
+    cs.fd_eps = st.fd_epsilon;
+    const ppl_scene_view v = scene.view();
+    ppl_chomp_buffers h = buffers_of<CHOMPBuffers>(bufs_obj);
+    auto solve = [&](const float* a, const float* b, float* path, ppl_opt_result* r) {
+        return ppl_chomp_solve(robot.robot, h, &v, a, b, st.num_timesteps, st.num_iterations, st.time_limit_ms, &cs,
+                               path, r, nullptr);
+    };
+    return solve_scene<CHOMPResult>(start, goal, robot, st.num_timesteps, solve, "chomp_solve_scene");
+}
+
+// stomp_solve_scene (python/bindings.cpp:801-831): single-goal STOMP from start to goal.
+STOMPResult stomp_solve_scene(F32Array start, F32Array goal, const SceneCollisionDataBuilder& scene,
+                              const STOMPSettings& st, RobotModelBuilder& robot, py::object bufs_obj) {
+    if (st.num_batch != 1) throw std::runtime_error("stomp_solve_scene: num_batch must be 1 (single goal)");
+    ppl_stomp_settings ss{};
+    ss.num_rollouts_new = st.num_rollouts_new;
+    ss.num_rollouts_old = st.num_rollouts_old;
+    ss.control_cost_weight = st.control_cost_weight;
+    ss.collision_cost_weight = st.collision_cost_weight;
+    ss.exponentiated_cost_sensitivity = st.exponentiated_cost_sensitivity;
+    ss.world_margin = st.world_collision_margin;
+    ss.self_margin = st.self_collision_margin;
+    ss.dt = st.delta_t;
+    ss.noise_scale = st.noise_scale;
+    ss.collision_check_step = st.collision_check_step;
+    ss.collision_margin = st.collision_margin;
+    ss.convergence_eps = st.convergence_eps;
+    ss.iterations_after_valid = st.num_iterations_after_valid;
+    ss.use_self = 1;
+    ss.seed = st.seed;
+    const ppl_scene_view v = scene.view();
+    ppl_chomp_buffers h = buffers_of<STOMPBuffers>(bufs_obj);
+    auto solve = [&](const float* a, const float* b, float* path, ppl_opt_result* r) {
+        return ppl_stomp_solve(robot.robot, h, &v, a, b, st.num_timesteps, st.num_iterations, st.time_limit_ms, &ss,
+                               path, r, nullptr);
+    };
+    return solve_scene<STOMPResult>(start, goal, robot, st.num_timesteps, solve, "stomp_solve_scene");
+}
+
+template <typename R>
+void bind_result(py::module_& m, const char* name) {
+    py::class_<R>(m, name)
+        .def_readonly("solved", &R::solved)
+        .def_readonly("path", &R::path)
+        .def_readonly("iterations_run", &R::iterations_run)
+        .def_readonly("final_state_cost", &R::final_state_cost)
+        .def_readonly("final_control_cost", &R::final_control_cost)
+        .def_readonly("final_total_cost", &R::final_total_cost)
+        .def_readonly("wall_time_ms", &R::wall_time_ms);
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_prrtc, m) {
@@ -199,4 +405,70 @@ PYBIND11_MODULE(_prrtc, m) {
     m.def("check_collision_scene", &check_collision_scene, py::arg("configs"), py::arg("scene"), py::arg("robot"),
           py::arg("collision_margin") = 0.0f,
           "Batch environment any-collision (two-phase gate) of configs [N, n_dof] -> bool[N]");
+    py::class_<CHOMPSettings>(m, "CHOMPSettings")
+        .def(py::init<>())
+        .def_readwrite("time_limit_ms", &CHOMPSettings::time_limit_ms)
+        .def_readwrite("collision_margin", &CHOMPSettings::collision_margin)
+        .def_readwrite("num_timesteps", &CHOMPSettings::num_timesteps)
+        .def_readwrite("num_iterations", &CHOMPSettings::num_iterations)
+        .def_readwrite("num_iterations_after_valid", &CHOMPSettings::num_iterations_after_valid)
+        .def_readwrite("step_size", &CHOMPSettings::step_size)
+        .def_readwrite("smoothness_weight", &CHOMPSettings::smoothness_weight)
+        .def_readwrite("collision_cost_weight", &CHOMPSettings::collision_cost_weight)
+        .def_readwrite("fd_epsilon", &CHOMPSettings::fd_epsilon)
+        .def_readwrite("world_collision_margin", &CHOMPSettings::world_collision_margin)
+        .def_readwrite("self_collision_margin", &CHOMPSettings::self_collision_margin)
+        .def_readwrite("delta_t", &CHOMPSettings::delta_t)
+        .def_readwrite("convergence_eps", &CHOMPSettings::convergence_eps)
+        .def_readwrite("collision_free_tol", &CHOMPSettings::collision_free_tol)
+        .def_readwrite("collision_check_step", &CHOMPSettings::collision_check_step)
+        .def_readwrite("fd_gradient", &CHOMPSettings::fd_gradient);
+    bind_result<CHOMPResult>(m, "CHOMPResult");
+    bind_result<STOMPResult>(m, "STOMPResult");
+    py::class_<CHOMPBuffers>(m, "CHOMPBuffers")
+        .def_property_readonly("owns_memory", &CHOMPBuffers::owns_memory)
+        .def_readonly("n_dof", &CHOMPBuffers::n_dof)
+        .def_readonly("num_timesteps", &CHOMPBuffers::num_timesteps)
+        .def_static("create", &CHOMPBuffers::create, py::arg("n_dof"), py::arg("num_timesteps"),
+                    py::return_value_policy::take_ownership, "Allocate device buffers for chomp_solve_scene calls.")
+        .def("destroy", &CHOMPBuffers::destroy, "Free the device buffers. Safe to call more than once.");
+    py::class_<STOMPBuffers>(m, "STOMPBuffers")
+        .def_property_readonly("owns_memory", &STOMPBuffers::owns_memory)
+        .def_readonly("n_dof", &STOMPBuffers::n_dof)
+        .def_readonly("num_timesteps", &STOMPBuffers::num_timesteps)
+        .def_readonly("num_rollouts_new", &STOMPBuffers::num_rollouts_new)
+        .def_readonly("num_rollouts_old", &STOMPBuffers::num_rollouts_old)
+        .def_property_readonly("num_rollouts_all", &STOMPBuffers::num_rollouts_all)
+        .def_readonly("num_batch", &STOMPBuffers::num_batch)
+        .def_static("create", &STOMPBuffers::create, py::arg("n_dof"), py::arg("num_timesteps"),
+                    py::arg("num_rollouts_new"), py::arg("num_rollouts_old"), py::arg("num_batch") = 1,
+                    py::return_value_policy::take_ownership, "Allocate device buffers for stomp_solve_scene calls.")
+        .def("destroy", &STOMPBuffers::destroy, "Free the device buffers. Safe to call more than once.");
+    py::class_<STOMPSettings>(m, "STOMPSettings")
+        .def(py::init<>())
+        .def_readwrite("time_limit_ms", &STOMPSettings::time_limit_ms)
+        .def_readwrite("collision_margin", &STOMPSettings::collision_margin)
+        .def_readwrite("num_batch", &STOMPSettings::num_batch)
+        .def_readwrite("num_timesteps", &STOMPSettings::num_timesteps)
+        .def_readwrite("num_iterations", &STOMPSettings::num_iterations)
+        .def_readwrite("num_iterations_after_valid", &STOMPSettings::num_iterations_after_valid)
+        .def_readwrite("num_rollouts_new", &STOMPSettings::num_rollouts_new)
+        .def_readwrite("num_rollouts_old", &STOMPSettings::num_rollouts_old)
+        .def_readwrite("delta_t", &STOMPSettings::delta_t)
+        .def_readwrite("control_cost_weight", &STOMPSettings::control_cost_weight)
+        .def_readwrite("collision_cost_weight", &STOMPSettings::collision_cost_weight)
+        .def_readwrite("exponentiated_cost_sensitivity", &STOMPSettings::exponentiated_cost_sensitivity)
+        .def_readwrite("world_collision_margin", &STOMPSettings::world_collision_margin)
+        .def_readwrite("self_collision_margin", &STOMPSettings::self_collision_margin)
+        .def_readwrite("convergence_eps", &STOMPSettings::convergence_eps)
+        .def_readwrite("collision_free_tol", &STOMPSettings::collision_free_tol)
+        .def_readwrite("collision_check_step", &STOMPSettings::collision_check_step)
+        .def_readwrite("noise_scale", &STOMPSettings::noise_scale)
+        .def_readwrite("seed", &STOMPSettings::seed);
+    m.def("stomp_solve_scene", &stomp_solve_scene, py::arg("start"), py::arg("goal"), py::arg("scene"),
+          py::arg("settings"), py::arg("robot"), py::arg("bufs") = py::none(),
+          "Single-goal STOMP against the scene; returns STOMPResult with path [T, n_dof].");
+    m.def("chomp_solve_scene", &chomp_solve_scene, py::arg("start"), py::arg("goal"), py::arg("scene"),
+          py::arg("settings"), py::arg("robot"), py::arg("bufs") = py::none(),
+          "Single-goal CHOMP against the scene; returns CHOMPResult with path [T, n_dof].");
 }

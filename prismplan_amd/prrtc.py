@@ -2,10 +2,12 @@
 
 `prismplan_amd/_lib/_prrtc*.so` (prismplan_amd/csrc/prrtc_module.cpp) mirrors
 the builder and check surface of the reference's pybind module `prrtc`
-(cpp/src/python/bindings.cpp:118-222, 363-402, 493-530, 960-1037):
-RobotModelBuilder with the same fields, SceneCollisionDataBuilder with int64
-device-pointer fields taken from torch tensors, and
-check_collision_scene(configs, scene, robot, collision_margin) -> bool[N].
+(cpp/src/python/bindings.cpp:118-222, 363-402, 493-530, 764-902, 960-1037,
+1191-1282): RobotModelBuilder with the same fields, SceneCollisionDataBuilder
+with int64 device-pointer fields taken from torch tensors,
+check_collision_scene(configs, scene, robot, collision_margin) -> bool[N], and
+chomp_solve_scene / stomp_solve_scene(start, goal, scene, settings, robot,
+bufs=None) -> CHOMPResult / STOMPResult with their settings and buffers.
 Errors surface as RuntimeError. Importing fails if the module is not built.
 """
 from __future__ import annotations
@@ -28,6 +30,14 @@ _spec.loader.exec_module(_m)
 RobotModelBuilder = _m.RobotModelBuilder
 SceneCollisionDataBuilder = _m.SceneCollisionDataBuilder
 check_collision_scene = _m.check_collision_scene
+CHOMPSettings = _m.CHOMPSettings
+CHOMPResult = _m.CHOMPResult
+CHOMPBuffers = _m.CHOMPBuffers
+chomp_solve_scene = _m.chomp_solve_scene
+STOMPSettings = _m.STOMPSettings
+STOMPResult = _m.STOMPResult
+STOMPBuffers = _m.STOMPBuffers
+stomp_solve_scene = _m.stomp_solve_scene
 
 
 def scene_builder(ds) -> SceneCollisionDataBuilder:

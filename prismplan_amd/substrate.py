@@ -382,6 +382,29 @@ class ChompOptimizer:
         return self.record[:, _ppl.CHOMP_DONE] != 0
 
 
+def stomp_settings(num_rollouts_new=30, num_rollouts_old=20, control_cost_weight=0.1, collision_cost_weight=20.0,
+                   exponentiated_cost_sensitivity=2.0, world_margin=0.02, self_margin=0.01, dt=0.1, noise_scale=1.0,
+                   collision_check_step=0.05, collision_margin=0.0, convergence_eps=1e-4, iterations_after_valid=3,
+                   use_self=True, seed=0x5BE5) -> _ppl.StompSettings:
+    """STOMP optimiser settings; defaults are the reference's STOMP_settings."""
+    return _ppl.StompSettings(int(num_rollouts_new), int(num_rollouts_old), control_cost_weight, collision_cost_weight,
+                              exponentiated_cost_sensitivity, world_margin, self_margin, dt, noise_scale,
+                              collision_check_step, collision_margin, convergence_eps, int(iterations_after_valid),
+                              int(bool(use_self)), int(seed))
+
+
+class StompOptimizer(ChompOptimizer):
+    """Batched STOMP over the trajectories of a plan, all on the device; the
+    same theta / best / record contract as ChompOptimizer."""
+
+    def step(self, iterations: int, stream=None):
+        v = self.scene.view()
+        call("ppl_stomp_optimize", self.plan.handle, C.byref(v), C.byref(self.settings), _dp(self.theta),
+             _dp(self.best), _dp(self.record), int(iterations), int(not self._started), _stream(stream))
+        self._started = True
+        return self
+
+
 def fk_batched(plan: Plan, configs: torch.Tensor, frames=True, spheres=True, stream=None):
     """Joint frames [n, J, 4, 4] (row-major view of the column-major output)
     and sphere centres + radii [n, S, 4] of every configuration."""

@@ -171,12 +171,13 @@ def _perturbed_lines(rng, rob: RobotArrays, lengths: np.ndarray, amp: float) -> 
 
 
 def chomp(n_traj: int = 256, n_wp: int = 64, n_obbs: int = 40, seed: int = 2, amp: float = 0.3,
-          robot: str = "panda") -> TrajWorkload:
-    """Straight start->goal lines plus 0.3 rad sinusoids that vanish at both ends."""
+          robot: str = "panda", lengths=None) -> TrajWorkload:
+    """Straight start->goal lines plus 0.3 rad sinusoids that vanish at both
+    ends; `lengths` (optional) gives each trajectory its own waypoint count."""
     rng = np.random.Generator(np.random.PCG64(seed))
     rob = robot_arrays(robot)
     scene = random_obbs(rng, n_obbs)
-    lengths = np.full(n_traj, n_wp, np.int64)
+    lengths = np.full(n_traj, n_wp, np.int64) if lengths is None else np.asarray(lengths, np.int64)
     theta = _perturbed_lines(rng, rob, lengths, amp)
     off = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int32)
     return TrajWorkload(rob, scene, theta, off)

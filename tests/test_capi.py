@@ -113,3 +113,33 @@ def test_prrtc_builder_fields():
               "voxel_enable_ptr", "voxel_features_ptr", "n_voxel_layers", "max_voxels_per_layer",
               "sphere_acm_mask_ptr", "sphere_obb_acm_mask_ptr", "sphere_obb_acm_mask_approx_ptr"):
         assert hasattr(s, f)
+
+
+def test_prrtc_chomp_surface_and_defaults():
+    """CHOMPSettings carries the reference's field names and defaults
+    (planning/CHOMP_settings.hh:21-57, pRRTC_settings.hh:6,25)."""
+    from prismplan_amd import prrtc
+    st = prrtc.CHOMPSettings()
+    want = dict(time_limit_ms=4000.0, collision_margin=0.0, num_timesteps=51, num_iterations=100,
+                num_iterations_after_valid=5, step_size=0.3, smoothness_weight=1.0, collision_cost_weight=20.0,
+                fd_epsilon=1e-3, world_collision_margin=0.02, self_collision_margin=0.01, delta_t=0.1,
+                convergence_eps=1e-4, collision_free_tol=1e-6, collision_check_step=0.05)
+    for k, v in want.items():
+        assert abs(getattr(st, k) - v) <= 1e-7 * max(1.0, abs(v)), k
+    for f in ("solved", "path", "iterations_run", "final_state_cost", "final_control_cost", "final_total_cost",
+              "wall_time_ms"):
+        assert hasattr(prrtc.CHOMPResult, f)
+    for f in ("owns_memory", "n_dof", "num_timesteps", "create", "destroy"):
+        assert hasattr(prrtc.CHOMPBuffers, f)
+    assert callable(prrtc.chomp_solve_scene)
+
+
+def test_chomp_solve_rejects_bad_arguments():
+    L = _ppl.lib()
+    out = C.c_void_p()
+    assert L.ppl_chomp_buffers_create(0, 51, C.byref(out)) == _ppl.PPL_ERR_INVALID
+    assert L.ppl_chomp_buffers_create(7, 1, C.byref(out)) == _ppl.PPL_ERR_INVALID
+    assert L.ppl_chomp_buffers_destroy(None) == _ppl.PPL_OK
+    res = _ppl.ChompResult()
+    assert L.ppl_chomp_solve(None, None, None, None, None, 51, 10, C.c_float(0), None, None, C.byref(res),
+                             None) == _ppl.PPL_ERR_INVALID

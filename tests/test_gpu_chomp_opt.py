@@ -16,6 +16,7 @@ import pyoracle as PO
 from prismplan_amd import _ppl
 from prismplan_amd import substrate as S
 from prismplan_amd import workloads as W
+from opt_common import densified, prrtc_panda
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
@@ -24,26 +25,6 @@ BAND = 1e-5
 
 def banner(msg):
     print(f"[gpu-test] {msg}", flush=True)
-
-
-def densified(th_td: np.ndarray, step: float):
-    """FP32 densification exactly as the validity pass (chomp.cu:459-475, 297-351)."""
-    th = th_td.astype(np.float32)
-    T, D = th.shape
-    d2 = np.zeros(T - 1, np.float32)
-    for d in range(D):
-        diff = (th[1:, d] - th[:-1, d]).astype(np.float32)
-        d2 = (d2 + (diff * diff).astype(np.float32)).astype(np.float32)
-    m = np.float32(np.sqrt(d2).max()) if T > 1 else np.float32(0)
-    n_sub = int(np.ceil(np.float32(m / np.float32(step)))) if (step > 0 and m > np.float32(step)) else 1
-    out = []
-    for seg in range(T - 1):
-        a, b = th[seg], th[seg + 1]
-        for sub in range(n_sub):
-            frac = np.float32(np.float32(sub) / np.float32(n_sub))
-            out.append((a + ((b - a).astype(np.float32) * frac).astype(np.float32)).astype(np.float32))
-    out.append(th[T - 1])
-    return np.stack(out), n_sub
 
 
 def setup(n_traj, n_wp, seed, **kw):
@@ -153,3 +134,72 @@ def test_optimize_loop_bookkeeping(fd):
         a1 = cur[off[b] * D: off[b + 1] * D].reshape(D, T)
         np.testing.assert_array_equal(a1[:, [0, -1]], a0[:, [0, -1]])
         assert np.all(a1 >= w.robot.joint_lower[:, None]) and np.all(a1 <= w.robot.joint_upper[:, None])
+
+
+def test_chomp_solve_scene():
+    """chomp_solve_scene (planning/chomp.cu:358-610 through the pybind surface):
+    the same trajectory as the batched optimiser run from the same straight
+    line, endpoints pinned, and reported costs / validity equal to the
+    oracle's for the returned path."""
+    from prismplan_amd import prrtc
+    w, panda, plan, ds, th, _ = setup(1, 32, 24)
+    D, T = w.robot.n_dof, 32
+    x0 = w.theta[:D * T].reshape(D, T)
+    start, goal = x0[:, 0].copy(), x0[:, -1].copy()
+    robot = prrtc_panda()
+    scene = prrtc.scene_builder(ds)
+    ps = prrtc.CHOMPSettings()
+    ps.num_timesteps, ps.num_iterations, ps.time_limit_ms = T, 0, 0.0
+    ps.world_collision_margin = 0.05
+    line = prrtc.chomp_solve_scene(start, goal, scene, ps, robot)
+    assert line.iterations_run == 0 and line.path.shape == (T, D)
+    ps.num_iterations = 40
+    res = prrtc.chomp_solve_scene(start, goal, scene, ps, robot)
+    banner(f"chomp_solve_scene: solved {res.solved}, iterations {res.iterations_run}, cost "
+           f"{res.final_state_cost:.4f} + {res.final_control_cost:.4f} = {res.final_total_cost:.4f}, "
+           f"{res.wall_time_ms:.2f} ms")
+    np.testing.assert_array_equal(res.path[0], start)
+    np.testing.assert_array_equal(res.path[-1], goal)
+    assert 0 < res.iterations_run <= 40
+    # the batched optimiser from the same straight line
+    st = S.chomp_settings(world_margin=0.05)
+    theta = torch.from_numpy(np.ascontiguousarray(line.path.T).reshape(-1)).to(DEV)
+    opt = S.ChompOptimizer(S.Plan.trajectories(panda, [T]), ds, theta, st).step(0).step(40)
+    rec = opt.record.cpu().numpy()[0]
+    assert bool(rec[_ppl.CHOMP_SOLVED]) == res.solved
+    assert int(rec[_ppl.CHOMP_ITERS]) == res.iterations_run
+    want = (opt.best if res.solved else opt.theta).cpu().numpy().reshape(D, T).T
+    np.testing.assert_allclose(res.path[1:-1], want[1:-1], rtol=0, atol=1e-6)
+    # reported costs are those of the returned path
+    o = PO.Oracle(w.robot, w.scene, nthreads=8)
+    flat = np.ascontiguousarray(res.path.T).reshape(-1).astype(np.float32)
+    off = np.array([0, T], np.int32)
+    prm = PO.cost_params(world_margin=0.05, self_margin=st.self_margin, weight=st.collision_cost_weight,
+                         use_self=True)
+    state, _, _ = o.traj_cost_grad(flat, off, prm)
+    ctrl = PO.control_cost(flat, off, D, st.dt, st.smoothness_weight)
+    for got, ref in ((res.final_state_cost, state[0]), (res.final_control_cost, ctrl[0]),
+                     (res.final_total_cost, state[0] + ctrl[0])):
+        assert abs(got - ref) <= 1e-5 * max(abs(ref), st.collision_cost_weight * 0.05)
+    if res.solved:
+        states, _ = densified(res.path, st.collision_check_step)
+        hit, clr = o.check_collision(states, check_self=True)
+        assert not hit.any() or (np.abs(clr) < BAND).any()
+    # reused buffers give the same solve; mismatched buffers are an error
+    bufs = prrtc.CHOMPBuffers.create(D, T)
+    assert bufs.owns_memory and bufs.n_dof == D and bufs.num_timesteps == T
+    for _ in range(2):
+        again = prrtc.chomp_solve_scene(start, goal, scene, ps, robot, bufs)
+        np.testing.assert_array_equal(again.path, res.path)
+        assert again.iterations_run == res.iterations_run and again.final_total_cost == res.final_total_cost
+    bufs.destroy()
+    assert not bufs.owns_memory
+    small = prrtc.CHOMPBuffers.create(D, T // 2)
+    with pytest.raises(RuntimeError):
+        prrtc.chomp_solve_scene(start, goal, scene, ps, robot, small)
+    small.destroy()
+    # a time limit is checked before every iteration
+    ps.time_limit_ms = 1e-3
+    assert prrtc.chomp_solve_scene(start, goal, scene, ps, robot).iterations_run <= 1
+    with pytest.raises(RuntimeError):
+        prrtc.chomp_solve_scene(start[:3], goal, scene, ps, robot)
