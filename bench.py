@@ -87,7 +87,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
-    ap.add_argument("--legs", default="skip,any,edges_plain,chomp,stomp,chomp_opt,esdf")
+    ap.add_argument("--legs", default="skip,any,edges_plain,chomp,stomp,chomp_opt,stomp_opt,esdf")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     legs = set(x for x in args.legs.split(",") if x)
@@ -151,6 +151,7 @@ def main():
                 ("chomp", "chomp", lambda: bench_chomp(panda, S, W, torch, dev, args)),
                 ("stomp", "stomp", lambda: bench_stomp(S, W, torch, dev, args)),
                 ("chomp_opt", "chomp_optimizer", lambda: bench_chomp_opt(panda, S, W, torch, dev, args)),
+                ("stomp_opt", "stomp_optimizer", lambda: bench_stomp_opt(panda, S, W, torch, dev, args)),
                 ("esdf", "esdf", lambda: bench_esdf(S, W, torch, dev, args)),
                 ("copy", "copy", lambda: bench_copy(torch, dev, args))):
             if key not in legs:
@@ -361,6 +362,42 @@ def bench_chomp_opt(panda, S, W, torch, dev, args):
             "trajectory_iterations_per_s": w.n_traj * iters / dt,
             "solved": int(rec[:, _ppl.CHOMP_SOLVED].sum()), "trajectories": w.n_traj,
             "done": int(rec[:, _ppl.CHOMP_DONE].sum())}
+
+
+def bench_stomp_opt(panda, S, W, torch, dev, args):
+    """The batched STOMP optimiser: 128 Panda trajectories x 51 waypoints, 40
+    OBBs, 30 new + 20 kept rollouts (the reference's STOMP_settings defaults):
+    sampling, rollout costs, weighting, projection, optimum cost, densified
+    validity and bookkeeping, all on the device. Beside it, the single-goal
+    solve (stomp_solve_scene's contract) per iteration."""
+    import numpy as np
+    from prismplan_amd import _ppl
+    w = W.chomp(n_traj=128, n_wp=51, seed=5)
+    ds = S.DeviceScene.from_host(w.scene)
+    plan = S.Plan.trajectories(panda, w.lengths())
+    st = S.stomp_settings(world_margin=0.05, iterations_after_valid=1000)
+    th = torch.from_numpy(w.theta.copy()).to(dev)
+    opt = S.StompOptimizer(plan, ds, th, st).step(0)
+    opt.step(2)
+    torch.cuda.synchronize()
+    iters = 20
+    t0 = time.perf_counter()
+    opt.step(iters)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rec = opt.record.cpu()
+    n_fresh = w.n_traj * st.num_rollouts_new * 51
+    # one request at a time through the single-goal solve (one trajectory, same settings)
+    x0 = w.theta[:7 * 51].reshape(7, 51)
+    start, goal = np.ascontiguousarray(x0[:, 0]), np.ascontiguousarray(x0[:, -1])
+    single = S.stomp_solve(panda, ds, start, goal, 51, 10, st)
+    single = S.stomp_solve(panda, ds, start, goal, 51, 10, st)
+    return {"iterations": iters, "ms_per_iteration": dt / iters * 1e3, "trajectories": w.n_traj,
+            "trajectory_iterations_per_s": w.n_traj * iters / dt, "fresh_rollout_states_per_s": n_fresh * iters / dt,
+            "solved": int(rec[:, _ppl.CHOMP_SOLVED].sum()), "done": int(rec[:, _ppl.CHOMP_DONE].sum()),
+            "single_goal_ms_per_iteration": single["wall_time_ms"] / max(single["iterations_run"], 1),
+            "batched_over_single_goal": (w.n_traj * iters / dt) /
+                                        (max(single["iterations_run"], 1) / (single["wall_time_ms"] * 1e-3))}
 
 
 def bench_stomp(S, W, torch, dev, args):

@@ -405,6 +405,38 @@ class StompOptimizer(ChompOptimizer):
         return self
 
 
+def _solve(fn: str, robot: Robot, scene: DeviceScene, start, goal, num_timesteps: int, num_iterations: int, settings,
+           time_limit_ms: float = 0.0):
+    D = robot.n_dof
+    a = np.ascontiguousarray(start, np.float32)
+    b = np.ascontiguousarray(goal, np.float32)
+    if a.shape != (D,) or b.shape != (D,):
+        raise ValueError(f"start and goal must have shape ({D},)")
+    path = np.empty((num_timesteps, D), np.float32)
+    res = _ppl.ChompResult()
+    v = scene.view()
+    call(fn, robot.handle, None, C.byref(v), a.ctypes.data_as(_ppl._f32p), b.ctypes.data_as(_ppl._f32p),
+         int(num_timesteps), int(num_iterations), C.c_float(time_limit_ms), C.byref(settings),
+         path.ctypes.data_as(_ppl._f32p), C.byref(res), None)
+    return {"solved": bool(res.solved), "path": path, "iterations_run": res.iterations_run,
+            "final_state_cost": res.final_state_cost, "final_control_cost": res.final_control_cost,
+            "final_total_cost": res.final_total_cost, "wall_time_ms": res.wall_time_ms}
+
+
+def chomp_solve(robot: Robot, scene: DeviceScene, start, goal, num_timesteps=51, num_iterations=100, settings=None,
+                time_limit_ms=0.0):
+    """Single-goal CHOMP (ppl_chomp_solve): path [T, D] from start to goal."""
+    return _solve("ppl_chomp_solve", robot, scene, start, goal, num_timesteps, num_iterations,
+                  settings if settings is not None else chomp_settings(), time_limit_ms)
+
+
+def stomp_solve(robot: Robot, scene: DeviceScene, start, goal, num_timesteps=51, num_iterations=20, settings=None,
+                time_limit_ms=0.0):
+    """Single-goal STOMP (ppl_stomp_solve): path [T, D] from start to goal."""
+    return _solve("ppl_stomp_solve", robot, scene, start, goal, num_timesteps, num_iterations,
+                  settings if settings is not None else stomp_settings(), time_limit_ms)
+
+
 def fk_batched(plan: Plan, configs: torch.Tensor, frames=True, spheres=True, stream=None):
     """Joint frames [n, J, 4, 4] (row-major view of the column-major output)
     and sphere centres + radii [n, S, 4] of every configuration."""
