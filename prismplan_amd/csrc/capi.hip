@@ -7,6 +7,7 @@
 #include "ppl_internal.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -96,6 +97,7 @@ struct ppl_robot_s {
     int n_slots = 1;
     int n_pairs = 0;
     int n_revolute = 0, n_prismatic = 0;   // joints that move with a dof
+    bool self_gate_implied = true;          // exact self hits imply approximate ones
     std::vector<float> lower, upper;
     std::vector<std::pair<int, ppl_plan>> traj_plans;   // single-trajectory plans of the plain CHOMP forms, by T
     hipStream_t stream = nullptr;
@@ -313,11 +315,23 @@ int edge_g(const float* a, const float* b, int dim, const ppl_edge_params& p, fl
 // Self pass. mode 0: any-collision; mode 1: min clearance, exact below `band`
 // (hinge callers pass their self margin).
 ppl_status run_self(const ppl_robot_s* r, const DStates& X, int mode, uint8_t* hit_out, float* out, hipStream_t s,
-                    float band = FLT_MAX, int* unit_first_hit = nullptr) {
+                    float band = FLT_MAX, int* unit_first_hit = nullptr, ppl_plan_s* P = nullptr) {
+    if (mode == 0 && !r->self_gate_implied)
+        return fail(PPL_ERR_UNSUPPORTED,
+                    "self any-collision: the approximate model does not cover every exact self pair (an exact "
+                    "pair's spheres must lie in approximate spheres that form an approximate pair)");
     SelfArgs a{};
     a.R = r->R;
     a.X = X;
-    a.X.wmajor = 0;      // the self pass walks states in plan order
+    if (P && X.source != kSrcDense) {
+        // the plan's 4-state chunks: unit hints, and waypoint-major order where the plan has it
+        HIP_TRY(ensure_table(P, kSelfGroup));
+        a.C.tab = P->d_tab[group_index(kSelfGroup)];
+        a.C.n_chunks = P->n_chunks[group_index(kSelfGroup)];
+        a.C.chunk = kWave / kSelfGroup;
+    } else {
+        a.X.wmajor = 0;  // flat plan order
+    }
     a.mode = mode;
     a.band = band;
     a.unit_first_hit = unit_first_hit;
@@ -405,27 +419,39 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
         CHECK_ARG(i >= 0 && i < S && a >= 0 && b < S, "self_cc_ranges[%d] out of range", e);
         for (int j = a; j <= b; ++j) pairs.push_back(make_int2(i, j));
     }
-    // Group the pairs by the (DFS) joints of their two spheres, so the self
-    // pass can skip a whole group when the joints' enclosing spheres are apart.
-    std::vector<int4> sblk;
-    {
-        auto key = [&](const int2& p) {
-            const int a = pos[d->sphere_to_joint[p.x]], b = pos[d->sphere_to_joint[p.y]];
-            return std::make_pair(std::min(a, b), std::max(a, b));
-        };
-        std::stable_sort(pairs.begin(), pairs.end(), [&](const int2& u, const int2& v) { return key(u) < key(v); });
-        for (size_t k = 0; k < pairs.size();) {
-            size_t e = k;
-            while (e < pairs.size() && key(pairs[e]) == key(pairs[k])) ++e;
-            sblk.push_back(make_int4(key(pairs[k]).first, key(pairs[k]).second, (int)k, (int)e));
-            k = e;
-        }
-    }
     for (int e = 0; e < d->n_approx_self_cc_ranges; ++e) {
         const int i = d->approx_self_cc_ranges[3 * e], a = d->approx_self_cc_ranges[3 * e + 1],
                   b = d->approx_self_cc_ranges[3 * e + 2];
         CHECK_ARG(i >= 0 && i < A && a >= 0 && b < A, "approx_self_cc_ranges[%d] out of range", e);
         for (int j = a; j <= b; ++j) apairs.push_back(make_int2(i, j));
+    }
+    // The reference checks self collision in two phases, exact pairs only
+    // after an approximate pair hits (two_phase_cc.cuh:114-131). When every
+    // exact pair's spheres lie inside approximate spheres of their joints that
+    // form an approximate pair (the loader's models), an exact hit implies an
+    // approximate one and the self pass checks exact pairs alone.
+    bool self_gate_implied = true;
+    {
+        std::vector<char> apair((size_t)A * A, 0);
+        for (const int2& p : apairs) apair[(size_t)p.x * A + p.y] = apair[(size_t)p.y * A + p.x] = 1;
+        auto inside = [&](int s, int a) {
+            const float* f = d->spheres + 4 * s;
+            const float* g = d->approx_spheres + 4 * a;
+            const double dx = (double)f[0] - g[0], dy = (double)f[1] - g[1], dz = (double)f[2] - g[2];
+            return d->approx_sphere_to_joint[a] == d->sphere_to_joint[s] &&
+                   std::sqrt(dx * dx + dy * dy + dz * dz) + f[3] <= (double)g[3] + 1e-6;
+        };
+        for (const int2& p : pairs) {
+            bool ok = false;
+            for (int a = 0; a < A && !ok; ++a) {
+                if (!inside(p.x, a)) continue;
+                for (int b = 0; b < A && !ok; ++b) ok = apair[(size_t)a * A + b] && inside(p.y, b);
+            }
+            if (!ok) {
+                self_gate_implied = false;
+                break;
+            }
+        }
     }
     // DFS-ordered tables.
     std::vector<float> ft(12 * J);
@@ -453,11 +479,89 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
     std::vector<float4> sph, apx;
     emit(S, d->spheres, d->sphere_to_joint, sph_off, sph, sph_id);
     emit(A, d->approx_spheres, d->approx_sphere_to_joint, apx_off, apx, apx_id);
+    // Self-pair blocks for the self pass: the pairs of DFS joints (A, B),
+    // A <= B, cut into at most 64 x 64 sphere tiles in emission order, with a
+    // 64-bit partner mask per row. Emission order keeps each joint's spheres
+    // contiguous, so a block names its rows and columns by first slot.
+    std::vector<int4> sblk;
+    std::vector<unsigned long long> rowmask;
+    {
+        std::vector<int> slot_of(S);
+        for (int k = 0; k < S; ++k) slot_of[sph_id[k]] = k;
+        std::vector<int> joint_of_slot(S);
+        for (int k = 0; k < J; ++k)
+            for (int t = sph_off[k]; t < sph_off[k + 1]; ++t) joint_of_slot[t] = k;
+        // key (A, B, row chunk, column chunk) -> (row slot, column slot) bits
+        std::vector<std::pair<std::array<int, 4>, std::pair<int, int>>> ent;
+        ent.reserve(pairs.size());
+        for (const int2& p : pairs) {
+            int u = slot_of[p.x], v = slot_of[p.y];
+            if (joint_of_slot[u] > joint_of_slot[v] || (joint_of_slot[u] == joint_of_slot[v] && u > v)) std::swap(u, v);
+            const int ja = joint_of_slot[u], jb = joint_of_slot[v];
+            const int ca = (u - sph_off[ja]) / 64, cb = (v - sph_off[jb]) / 64;
+            ent.push_back({{ja, jb, ca, cb}, {u, v}});
+        }
+        std::stable_sort(ent.begin(), ent.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        for (size_t k = 0; k < ent.size();) {
+            size_t e = k;
+            while (e < ent.size() && ent[e].first == ent[k].first) ++e;
+            const auto& key = ent[k].first;
+            const int ja = key[0], jb = key[1];
+            const int a0 = sph_off[ja] + 64 * key[2], b0 = sph_off[jb] + 64 * key[3];
+            const int na = std::min(64, sph_off[ja + 1] - a0), nb = std::min(64, sph_off[jb + 1] - b0);
+            const int moff = (int)rowmask.size();
+            rowmask.resize(rowmask.size() + na, 0ull);
+            for (size_t t = k; t < e; ++t)
+                rowmask[moff + (ent[t].second.first - a0)] |= 1ull << (ent[t].second.second - b0);
+            sblk.push_back(make_int4(ja, jb, a0, na));
+            sblk.push_back(make_int4(b0, nb, moff, 0));
+            k = e;
+        }
+    }
     std::vector<float4> sph_orig(S), apx_orig(A);
     for (int s = 0; s < S; ++s) sph_orig[s] = make_float4(d->spheres[4 * s], d->spheres[4 * s + 1], d->spheres[4 * s + 2], d->spheres[4 * s + 3]);
     for (int a = 0; a < A; ++a)
         apx_orig[a] = make_float4(d->approx_spheres[4 * a], d->approx_spheres[4 * a + 1], d->approx_spheres[4 * a + 2],
                                   d->approx_spheres[4 * a + 3]);
+    // Sphere around each joint's spheres for the self pass's group culling:
+    // the joint's single approximate sphere when it contains them all (to
+    // 1e-6 m), else a centroid sphere that does; 1e-5 m larger so rounding in
+    // the FP32 transforms cannot make a contained sphere poke out. w < 0: no
+    // spheres.
+    std::vector<float4> jenc(J, make_float4(0.0f, 0.0f, 0.0f, -1.0f));
+    for (int k = 0; k < J; ++k) {
+        const int s0 = sph_off[k], s1 = sph_off[k + 1];
+        if (s0 == s1) continue;
+        auto reach = [&](double cx, double cy, double cz) {
+            double m = 0.0;
+            for (int s = s0; s < s1; ++s) {
+                const double dx = sph[s].x - cx, dy = sph[s].y - cy, dz = sph[s].z - cz;
+                m = std::max(m, std::sqrt(dx * dx + dy * dy + dz * dz) + sph[s].w);
+            }
+            return m;
+        };
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        for (int s = s0; s < s1; ++s) {
+            cx += sph[s].x;
+            cy += sph[s].y;
+            cz += sph[s].z;
+        }
+        cx /= (s1 - s0);
+        cy /= (s1 - s0);
+        cz /= (s1 - s0);
+        double rad = reach(cx, cy, cz);
+        if (apx_off[k + 1] - apx_off[k] == 1) {
+            const float4 a = apx[apx_off[k]];
+            const double ra = reach(a.x, a.y, a.z);   // may exceed a.w by FP32 rounding of the tables
+            if (ra <= (double)a.w + 1e-6 && ra <= rad) {
+                cx = a.x;
+                cy = a.y;
+                cz = a.z;
+                rad = std::max(ra, (double)a.w);
+            }
+        }
+        jenc[k] = make_float4((float)cx, (float)cy, (float)cz, (float)(rad + 1e-5));
+    }
     // Parent of each DFS position, as a DFS position (root: itself).
     std::vector<int> close(J, 0);
     for (int k = 0; k < J; ++k) close[k] = k == 0 ? 0 : pos[parents[dfs[k]]];
@@ -473,7 +577,9 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
                  o_apx = place(apx.size() * 16), o_aid = place(apx_id.size() * 4), o_close = place(close.size() * 4),
                  o_sorig = place(sph_orig.size() * 16), o_pairs = place(pairs.size() * 8),
                  o_apairs = place(apairs.size() * 8), o_aorig = place(apx_orig.size() * 16),
-                 o_sblk = place(sblk.size() * 16), o_jorig = place(J * 4), o_lim = place((size_t)D * 8);
+                 o_sblk = place(sblk.size() * 16), o_rmask = place(rowmask.size() * 8), o_jorig = place(J * 4),
+                 o_lim = place((size_t)D * 8),
+                 o_jenc = place((size_t)J * 16);
     std::vector<char> host(off + 16, 0);
     auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) memcpy(host.data() + o, src, bytes); };
     put(o_ft, ft.data(), ft.size() * 4);
@@ -490,10 +596,12 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
     put(o_apairs, apairs.data(), apairs.size() * 8);
     put(o_aorig, apx_orig.data(), apx_orig.size() * 16);
     put(o_sblk, sblk.data(), sblk.size() * 16);
+    put(o_rmask, rowmask.data(), rowmask.size() * 8);
     put(o_jorig, dfs.data(), (size_t)J * 4);
     std::vector<float2> lim(D);
     for (int k = 0; k < D; ++k) lim[k] = make_float2(d->joint_lower[k], d->joint_upper[k]);
     put(o_lim, lim.data(), (size_t)D * 8);
+    put(o_jenc, jenc.data(), (size_t)J * 16);
 
     ppl_robot_s* r = new (std::nothrow) ppl_robot_s();
     if (!r) return fail(PPL_ERR_ALLOC, "out of host memory");
@@ -525,11 +633,14 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
     R.apx_pairs = reinterpret_cast<const int2*>(b + o_apairs);
     R.apx_orig = reinterpret_cast<const float4*>(b + o_aorig);
     R.sblk = reinterpret_cast<const int4*>(b + o_sblk);
+    R.rowmask = reinterpret_cast<const unsigned long long*>(b + o_rmask);
     R.jorig = reinterpret_cast<const int*>(b + o_jorig);
     R.lim = reinterpret_cast<const float2*>(b + o_lim);
-    R.n_sblk = (int)sblk.size();
+    R.jenc = reinterpret_cast<const float4*>(b + o_jenc);
+    R.n_sblk = (int)sblk.size() / 2;
     r->n_slots = n_slots;
     r->n_pairs = (int)pairs.size();
+    r->self_gate_implied = self_gate_implied;
     for (int i = 0; i < J; ++i) {
         if (i == root || id2dof[i] < 0) continue;
         if (d->joint_types[i] >= kXRot) ++r->n_revolute;
@@ -979,7 +1090,7 @@ ppl_status cost_self(ppl_plan_s* P, const DStates& X, const ppl_cost_params* prm
     *self_in = nullptr;
     if (!prm->use_self || P->robot->n_pairs == 0 || X.n_states == 0) return PPL_OK;
     HIP_TRY(P->d_self.ensure((size_t)X.n_states * 8 * sizeof(float)));
-    ppl_status st = run_self(P->robot, X, 1, nullptr, P->d_self.as<float>(), s, prm->self_margin);
+    ppl_status st = run_self(P->robot, X, 1, nullptr, P->d_self.as<float>(), s, prm->self_margin, nullptr, P);
     if (st != PPL_OK) return st;
     *self_in = P->d_self.as<float>();
     return PPL_OK;
@@ -1624,7 +1735,8 @@ ppl_status ppl_check_collision_batched(ppl_plan P, const ppl_scene_view* view, c
     const uint8_t* self_hit = nullptr;
     if (check_self && P->robot->n_pairs > 0) {
         HIP_TRY(P->d_self.ensure((size_t)P->n_states));
-        if ((st = run_self(P->robot, X, 0, P->d_self.as<uint8_t>(), nullptr, s)) != PPL_OK) return st;
+        if ((st = run_self(P->robot, X, 0, P->d_self.as<uint8_t>(), nullptr, s, FLT_MAX, nullptr, P)) != PPL_OK)
+            return st;
         self_hit = P->d_self.as<uint8_t>();
     }
     CollideArgs a{};
@@ -1659,7 +1771,8 @@ ppl_status ppl_edge_check_batched(ppl_plan P, const ppl_scene_view* view, const 
     const uint8_t* self_hit = nullptr;
     if (prm->check_self && P->robot->n_pairs > 0) {
         HIP_TRY(P->d_self.ensure((size_t)P->n_states));
-        if ((st = run_self(P->robot, X, 0, P->d_self.as<uint8_t>(), nullptr, s)) != PPL_OK) return st;
+        if ((st = run_self(P->robot, X, 0, P->d_self.as<uint8_t>(), nullptr, s, FLT_MAX, nullptr, P)) != PPL_OK)
+            return st;
         self_hit = P->d_self.as<uint8_t>();
     }
     CollideArgs a{};

@@ -961,12 +961,33 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
     float4* s_pos = s_dyn4;                                          // [kSelfStates][S] (x, y, z, r)
     float4* s_jnt = s_dyn4 + kSelfStates * S;                         // [kSelfStates][J] enclosing sphere, r < 0: none
     float* s_q = reinterpret_cast<float*>(s_jnt + kSelfStates * J);   // [kMaxDof][64]
+    int* s_end = reinterpret_cast<int*>(s_q + kMaxDof * kWave);       // [2][64] unit ends / starts of a chunk
     const long n_states = state_count(a.X);
-    for (long base = (long)blockIdx.x * kSelfStates; base < n_states; base += (long)gridDim.x * kSelfStates) {
-    const long g = base + st;
-    bool active = g < n_states;
+    // With the plan's chunk table a chunk names its first unit (staged unit
+    // ends make the state -> unit lookup an LDS bisection) or, waypoint-major,
+    // its problem; without it states go in flat order.
+    const bool chunked = a.C.tab != nullptr;
+    const long n_work = chunked ? (long)a.C.n_chunks : (n_states + kSelfStates - 1) / kSelfStates;
+    for (long w = blockIdx.x; w < n_work; w += gridDim.x) {
+    long base, g;
+    bool active;
+    int u0 = -1, prob = 0;
+    if (chunked) {
+        const int4 ch = a.C.tab[w];
+        base = ch.y;
+        g = base + st;
+        active = g < ch.z;
+        u0 = ch.w;
+        prob = ch.x;
+        stage_unit_ends(a.X, u0, s_end);
+        __syncthreads();
+    } else {
+        base = w * kSelfStates;
+        g = base + st;
+        active = g < n_states;
+    }
     // Every lane of a state group loads the same q (lane-indexed LDS column).
-    const StateRef ref = load_state(a.X, D, active ? g : base, true, s_q);
+    const StateRef ref = load_state(a.X, D, active ? g : base, true, s_q, u0, s_end, prob);
     if (active && a.unit_first_hit && a.unit_first_hit[ref.unit] >= 0) active = false;   // unit already invalid
     if (a.unit_first_hit && !wave_any(active)) continue;      // nothing left to decide in this round
     __syncthreads();
@@ -982,91 +1003,116 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
             const float4 sp = R.sph[s];
             float x, y, z;
             aff_point(T, sp.x, sp.y, sp.z, x, y, z);
-            pos[R.sph_id[s]] = make_float4(x, y, z, sp.w);
+            pos[s] = make_float4(x, y, z, sp.w);   // emission slot: a joint's spheres are contiguous
         }
         if (sub == 0) {
-            const int a0 = R.apx_off[jj], a1 = R.apx_off[jj + 1];
-            float4 e = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-            if (a1 - a0 == 1) {
-                const float4 as = R.apx[a0];
-                aff_point(T, as.x, as.y, as.z, e.x, e.y, e.z);
-                e.w = as.w;
-            }
+            const float4 je = R.jenc[jj];
+            float4 e = make_float4(0.0f, 0.0f, 0.0f, je.w);
+            aff_point(T, je.x, je.y, je.z, e.x, e.y, e.z);
             jnt[jj] = e;
         }
     }
     __syncthreads();
-    // Pairs come grouped by the joints of their spheres. Each joint's
-    // enclosing sphere contains all of its spheres, so a group whose two
-    // enclosing spheres are at clearance >= band holds no pair below band
-    // (band 0 for the any-collision check: no pair there can collide).
+    // Pairs come in blocks: rows = up to 64 spheres of joint A, columns = up
+    // to 64 spheres of joint B, a 64-bit mask of partners per row. Each
+    // joint's enclosing sphere contains all of its spheres, which culls at
+    // two levels for a state: a block whose two enclosing spheres are at
+    // clearance >= band holds no pair below band; within a block, a row
+    // sphere at clearance >= band from B's enclosing sphere (a column sphere
+    // from A's) has no partner below band. band = 0 for the any-collision
+    // check, where "below" is the strict d^2 < (r_i + r_j)^2 test. The 16
+    // lanes of a state test the spheres, then each surviving row walks its
+    // surviving partners.
     const float band = a.mode == 0 ? 0.0f : a.band;
-    if (a.mode == 0) {
-        bool hit = false;
-        for (int b = 0; b < R.n_sblk; ++b) {
-            const int4 blk = R.sblk[b];
-            const float4 ea = jnt[blk.x], eb = jnt[blk.y];
-            const float dx = ea.x - eb.x, dy = ea.y - eb.y, dz = ea.z - eb.z;
-            const float rs = ea.w + eb.w;
-            const bool gate = active && !hit && (ea.w < 0.0f || eb.w < 0.0f ||
-                                                 fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < rs * rs);
-            if (!wave_any(gate)) continue;
-            if (gate) {
-                for (int p = blk.z + sub; p < blk.w && !hit; p += kSelfGroup) {
-                    const int2 pr = R.pairs[p];
-                    const float4 u = pos[pr.x], v = pos[pr.y];
-                    const float ux = u.x - v.x, uy = u.y - v.y, uz = u.z - v.z;
+    // near: clearance of (p, r) to the enclosing sphere e is below band
+    auto near = [&](float4 p, float4 e) {
+        const float dx = p.x - e.x, dy = p.y - e.y, dz = p.z - e.z;
+        const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float rs = p.w + e.w;
+        return e.w < 0.0f || (a.mode == 0 ? d2 < rs * rs : sqrtf(d2) - rs < band);
+    };
+    // 64-bit mask of the spheres [s0, s0 + n) of this state near e
+    auto near_mask = [&](int s0, int n, float4 e) {
+        unsigned long long m = 0ull;
+        for (int r0 = 0; r0 < n; r0 += kSelfGroup) {
+            const int r = r0 + sub;
+            const bool nb = r < n && near(pos[s0 + r], e);
+            m |= ((__ballot(nb) >> (st * kSelfGroup)) & 0xFFFFull) << r0;
+        }
+        return m;
+    };
+    float best = 1000.0f;
+    int bkey = INT_MAX;   // (block, row, column) of the best pair: ties go to the first
+    bool hit = false;
+    for (int b = 0; b < R.n_sblk; ++b) {
+        const int4 h = R.sblk[2 * b], c = R.sblk[2 * b + 1];   // (jA, jB, a0, na), (b0, nb, mask offset, -)
+        const float4 ea = jnt[h.x], eb = jnt[h.y];
+        const bool gate = active && !hit && near(make_float4(ea.x, ea.y, ea.z, ea.w < 0.0f ? 0.0f : ea.w), eb);
+        if (!wave_any(gate)) continue;
+        const unsigned long long ma = gate ? near_mask(h.z, h.w, eb) : 0ull;
+        const unsigned long long mb = gate && ma ? near_mask(c.x, c.y, ea) : 0ull;
+        for (int r0 = 0; r0 < h.w; r0 += kSelfGroup) {
+            const int r = r0 + sub;
+            unsigned long long cols = 0ull;
+            if (r < h.w && ((ma >> r) & 1ull)) cols = R.rowmask[c.z + r] & mb;
+            if (!wave_any(cols != 0ull)) continue;
+            const float4 u = pos[h.z + (r < h.w ? r : 0)];
+            while (cols) {
+                const int j = __builtin_ctzll(cols);
+                cols &= cols - 1ull;
+                const float4 v = pos[c.x + j];
+                const float ux = u.x - v.x, uy = u.y - v.y, uz = u.z - v.z;
+                const float e2 = fmaf(uz, uz, fmaf(uy, uy, ux * ux));
+                if (a.mode == 0) {
                     const float r2 = u.w + v.w;
-                    hit = fmaf(uz, uz, fmaf(uy, uy, ux * ux)) - r2 * r2 < 0.0f;
+                    if (e2 - r2 * r2 < 0.0f) {
+                        hit = true;
+                        cols = 0ull;
+                    }
+                } else {
+                    const float cl = sqrtf(e2) - (u.w + v.w);
+                    const int key = (b << 12) | (r << 6) | j;
+                    if (cl < best || (cl == best && key < bkey)) {
+                        best = cl;
+                        bkey = key;
+                    }
                 }
             }
-            // share the group's verdict so its lanes stop early
-            const unsigned long long m = __ballot(hit);
-            hit = ((m >> (st * kSelfGroup)) & 0xFFFFull) != 0ull;
+        }
+        if (a.mode == 0) {
+            // share the state's verdict so its lanes stop
+            hit = ((__ballot(hit) >> (st * kSelfGroup)) & 0xFFFFull) != 0ull;
             if (__ballot(active && !hit) == 0ull) break;
         }
+    }
+    if (a.mode == 0) {
         if (active && sub == 0) {
             if (a.unit_first_hit) {
                 if (hit) atomicMin((unsigned*)&a.unit_first_hit[ref.unit], (unsigned)ref.k);
             } else {
-                a.hit_out[g] = hit ? 1 : 0;
+                a.hit_out[ref.gs] = hit ? 1 : 0;
             }
         }
     } else {
-        float best = 1000.0f;
-        int bp = INT_MAX;
-        for (int b = 0; b < R.n_sblk; ++b) {
-            const int4 blk = R.sblk[b];
-            const float4 ea = jnt[blk.x], eb = jnt[blk.y];
-            const float dx = ea.x - eb.x, dy = ea.y - eb.y, dz = ea.z - eb.z;
-            const float gap = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))) - ea.w - eb.w;
-            const bool gate = active && (ea.w < 0.0f || eb.w < 0.0f || gap < band);
-            if (!wave_any(gate)) continue;
-            if (gate) {
-                for (int p = blk.z + sub; p < blk.w; p += kSelfGroup) {
-                    const int2 pr = R.pairs[p];
-                    const float4 u = pos[pr.x], v = pos[pr.y];
-                    const float ux = u.x - v.x, uy = u.y - v.y, uz = u.z - v.z;
-                    const float c = sqrtf(fmaf(uz, uz, fmaf(uy, uy, ux * ux))) - (u.w + v.w);
-                    if (c < best) { best = c; bp = p; }
-                }
-            }
-        }
 #pragma unroll
         for (int off = 1; off < kSelfGroup; off <<= 1) {
             const float ob = __shfl_xor(best, off, kSelfGroup);
-            const int op = __shfl_xor(bp, off, kSelfGroup);
-            if (ob < best || (ob == best && op < bp)) { best = ob; bp = op; }
+            const int ok = __shfl_xor(bkey, off, kSelfGroup);
+            if (ob < best || (ob == best && ok < bkey)) {
+                best = ob;
+                bkey = ok;
+            }
         }
         if (active && sub == 0) {
-            float* o = a.out + 8 * (size_t)g;
+            float* o = a.out + 8 * (size_t)ref.gs;
             o[0] = best;
-            if (bp != INT_MAX) {
-                const int2 pr = R.pairs[bp];
-                const float4 u = pos[pr.x], v = pos[pr.y];
+            if (bkey != INT_MAX) {
+                const int b = bkey >> 12, r = (bkey >> 6) & 63, j = bkey & 63;
+                const int si = R.sblk[2 * b].z + r, sj = R.sblk[2 * b + 1].x + j;
+                const float4 u = pos[si], v = pos[sj];
                 const float dx = u.x - v.x, dy = u.y - v.y, dz = u.z - v.z;
                 const float inv = 1.0f / sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-                o[1] = __int_as_float(pr.x); o[2] = __int_as_float(pr.y);
+                o[1] = __int_as_float(R.sph_id[si]); o[2] = __int_as_float(R.sph_id[sj]);
                 o[3] = dx * inv; o[4] = dy * inv; o[5] = dz * inv;
             } else {
                 o[1] = __int_as_float(-1); o[2] = __int_as_float(-1);
@@ -1631,11 +1677,11 @@ hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s) {
 hipError_t launch_self(const SelfArgs& a, hipStream_t s) {
     // X.n_states bounds the state count (a device-side count may be smaller,
     // then a bounded grid strides over it)
-    long blocks = (a.X.n_states + kSelfStates - 1) / kSelfStates;
+    long blocks = a.C.tab ? (long)a.C.n_chunks : (a.X.n_states + kSelfStates - 1) / kSelfStates;
     if (a.X.n_dyn && blocks > 16384) blocks = 16384;
     if (blocks <= 0) return hipSuccess;
     const size_t smem = sizeof(float4) * (size_t)kSelfStates * (a.R.n_sph + a.R.n_joints) +
-                        sizeof(float) * (size_t)kMaxDof * kWave;
+                        sizeof(float) * (size_t)kMaxDof * kWave + sizeof(int) * 2 * kWave;
     hipLaunchKernelGGL(k_self, dim3((unsigned)blocks), dim3(64), smem, s, a);
     return hipGetLastError();
 }

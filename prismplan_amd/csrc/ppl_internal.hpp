@@ -49,10 +49,13 @@ struct DRobot {
     const float2* lim;          // [D] joint limits (lower, upper)
     const float4* sph_orig;     // [S] spheres by original index
     const int2* pairs;          // [n_pairs] exact self pairs (original indices), grouped by joint pair
-    const int4* sblk;           // [n_sblk] (DFS joint a, DFS joint b, first pair, end pair) of each group
+    const int4* sblk;           // [n_sblk][2] self-pair blocks: (DFS joint A, DFS joint B, first slot a0,
+                                // rows na), (first slot b0, columns nb, row-mask offset, -); na, nb <= 64
+    const unsigned long long* rowmask;   // per block row: bit j = partner b0 + j (slots = emission order)
     int n_sblk;
     const int2* apx_pairs;      // [n_apx_pairs] approx self pairs (approx original indices)
     const float4* apx_orig;     // [A] approx spheres by original index
+    const float4* jenc;         // [J] link-frame sphere containing every sphere of DFS joint k (w < 0: none)
 };
 
 // Non-owning scene pointers, SceneCollisionData semantics.
@@ -146,6 +149,8 @@ struct SelfArgs {
     int* unit_first_hit;        // mode 0, nullable: per-unit first hit (-1 free); states of units with a
                                 // hit are skipped and a self collision lowers the unit's entry instead of hit_out
     float* out;                 // [n_states][8]: clr, i, j (as int bits), nx, ny, nz, -, -
+    DChunks C;                  // the plan's 4-state chunks (tab NULL: states in flat order); outputs are
+                                // indexed by plan order either way
 };
 
 hipError_t launch_collide(const CollideArgs& a, int grid, hipStream_t s);

@@ -225,6 +225,38 @@ def test_batched_edges_first_hit(robots, n_obbs, seed):
             assert abs(clr[off[k] + lo]) < BAND, f"edge {k}: first hit {fh_all[k]} vs {want}"
 
 
+def test_self_culling_without_enclosing_approx_spheres():
+    """The self pass culls pair groups with spheres it builds around each
+    joint's spheres, not with the approximate model: with approximate spheres
+    shrunk so they no longer enclose, the min-hinge self cost still equals the
+    oracle's minimum over every pair, and the any-collision self check, whose
+    two-phase gate would then differ, is refused."""
+    from prismplan_amd.robot_model_loader import load_robot_asset
+    from prismplan_amd.robot_model_loader import RobotArrays
+    a = dict(load_robot_asset("panda"))
+    ap = np.array(a["approx_spheres_flat"], np.float64).reshape(-1, 4)
+    ap[:, 3] *= 0.3
+    a["approx_spheres_flat"] = ap.reshape(-1).tolist()
+    rob = S.Robot(a)
+    r = RobotArrays(a)
+    rng = np.random.default_rng(51)
+    scene = W.random_obbs(rng, 0)
+    q = W.uniform_configs(rng, r, 3000)
+    prm = S.cost_params(world_margin=0.05, self_margin=0.05, weight=1.0)
+    plan = S.Plan.configs(rob, len(q))
+    ds = S.DeviceScene.from_host(scene)
+    c = S.hinge_cost(plan, ds, torch.from_numpy(q).to(DEV), prm)[0].cpu().numpy()
+    c_ref = PO.Oracle(r, scene, nthreads=8).hinge_cost(q, PO.cost_params(world_margin=0.05, self_margin=0.05,
+                                                                         weight=1.0))
+    rel = np.abs(c - c_ref) / np.maximum(np.abs(c_ref), 0.05)
+    banner(f"self hinge with shrunk approx spheres: max rel err {rel.max():.2e}, active {np.mean(c_ref > 0):.2f}")
+    assert np.mean(c_ref > 0) > 0.05
+    assert rel.max() < 1e-5
+    with pytest.raises(_ppl.PplError) as err:
+        S.check_collision_batched(plan, ds, torch.from_numpy(q).to(DEV), check_self=True)
+    assert err.value.status == _ppl.PPL_ERR_UNSUPPORTED
+
+
 def test_hinge_cost_configs(robots):
     rng = np.random.default_rng(50)
     for name in ("panda", "fetch", "baxter"):
