@@ -449,6 +449,43 @@ ppl_status ppl_graph_launch(ppl_graph graph, ppl_stream stream);
 ppl_status ppl_graph_destroy(ppl_graph graph);
 
 /* ------------------------------------------------------------------------
+ * MIT*'s substrate calls over device-resident data (planning/MITStar.cu). The
+ * planner's own bookkeeping (adjacency, checks-done slots, tree freebies,
+ * whitelists, tags) stays with the caller; these run the collision part.
+ * ------------------------------------------------------------------------ */
+
+/* batch_cc_kernel (MITStar.cu:212-276): d_cc[i] = two-phase any-collision of
+ * d_samples [n][n_dof] with the self check (1 = collision). With d_clearance,
+ * each FREE sample gets its min ESDF clearance (scene_min_clearance_runtime,
+ * collision_margin included) when clearance_weight > 0, else 1000; colliding
+ * samples keep their d_clearance entry. */
+ppl_status ppl_mitstar_batch_cc(ppl_robot robot, const ppl_scene_view* scene, const float* d_samples, int n,
+                                float collision_margin, uint8_t* d_cc, float* d_clearance, float clearance_weight,
+                                ppl_stream stream);
+
+/* compute_node_clearance_kernel (MITStar.cu:283-329): for i < n_nodes, node
+ * = d_node_indices ? d_node_indices[i] : node_start + i, and
+ * d_clearance[node] = min ESDF clearance of d_nodes[node] (+ margin). */
+ppl_status ppl_mitstar_node_clearance(ppl_robot robot, const ppl_scene_view* scene, const float* d_nodes,
+                                      const int* d_node_indices, int node_start, int n_nodes, float collision_margin,
+                                      float* d_clearance, ppl_stream stream);
+
+/* sparse_cc_kernel / full_cc_kernel (MITStar.cu:832-1033, 1197-1375): edge e
+ * joins d_nodes[d_src[e]] and d_nodes[d_dst[e]] ([node][dim], dim = n_dof).
+ * With len the FP32 joint-space length, total = max(1, ceil(len /
+ * valid_segment_length)) and target = min(num_sparse_checks, total - 1)
+ * (num_sparse_checks < 0: total - 1, the full check), the breadth-first
+ * midpoints k = checks_done[e] .. target - 1 (t = 1/2, 1/4, 3/4, 1/8, ...;
+ * d_checks_done NULL = 0) are checked with the two-phase test and the self
+ * check at q = q_src (1 - t) + q_dst t. d_cc[e] = 1 if any collides;
+ * d_new_checks[e] (nullable) = the midpoints verified free (0 on collision).
+ * Synchronises once on `stream` to size the launch. */
+ppl_status ppl_mitstar_edge_cc(ppl_robot robot, const ppl_scene_view* scene, const float* d_nodes, int dim,
+                               const int* d_src, const int* d_dst, const int* d_checks_done, int n_edges,
+                               float valid_segment_length, int num_sparse_checks, float collision_margin,
+                               uint8_t* d_cc, int* d_new_checks, ppl_stream stream);
+
+/* ------------------------------------------------------------------------
  * Algorithmic flop counts (ppl/roofline.h) of one work unit for this robot:
  * the numerators of the benchmark's roofline fractions. No counterpart in
  * the reference. Returns a negative value for a bad handle or kind.

@@ -153,6 +153,12 @@ SIGNATURES = [
     ("ppl_stomp_buffers_destroy", C.c_int, [_vp]),
     ("ppl_stomp_solve", C.c_int, [_vp, _vp, C.POINTER(SceneView), _f32p, _f32p, C.c_int, C.c_int, C.c_float,
                                   C.POINTER(StompSettings), _f32p, C.POINTER(ChompResult), _vp]),
+    ("ppl_mitstar_batch_cc", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.c_int, C.c_float, _vp, _vp, C.c_float,
+                                       _vp]),
+    ("ppl_mitstar_node_clearance", C.c_int, [_vp, C.POINTER(SceneView), _vp, _vp, C.c_int, C.c_int, C.c_float, _vp,
+                                             _vp]),
+    ("ppl_mitstar_edge_cc", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_float,
+                                      C.c_int, C.c_float, _vp, _vp, _vp]),
     ("ppl_esdf_from_obbs", C.c_int, [C.POINTER(SceneView), _f32p, _f32p, _vp, _vp]),
     ("ppl_graph_capture_stomp", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), C.c_float,
                                           C.c_float, _vp, _vp, _vp, C.POINTER(_vp)]),

@@ -101,8 +101,8 @@ struct ppl_robot_s {
     std::vector<float> lower, upper;
     std::vector<std::pair<int, ppl_plan>> traj_plans;   // single-trajectory plans of the plain CHOMP forms, by T
     hipStream_t stream = nullptr;
-    DBuf d_cfg, d_out, d_off, d_aux, d_self;
-    HBuf h_stage;
+    DBuf d_cfg, d_out, d_off, d_aux, d_self, d_mit;
+    HBuf h_stage, h_mit;
 };
 
 struct ppl_edge_eval_s {
@@ -1811,6 +1811,157 @@ ppl_status ppl_min_clearance_batched(ppl_plan P, const ppl_scene_view* view, con
     a.margin = margin;
     a.cost_out = d_out;
     HIP_TRY(launch_cost(a, grid, (hipStream_t)stream));
+    return PPL_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Min ESDF clearance (+ margin) of n configurations d_cfg [n][D] into d_out,
+// scene_min_clearance_runtime semantics (the clearance mode of the cost kernel).
+ppl_status configs_clearance(const ppl_robot_s* r, const DScene& S, const float* d_cfg, int n, float margin,
+                             float* d_out, hipStream_t s) {
+    if (n <= 0) return PPL_OK;
+    CostArgs c{};
+    c.R = r->R;
+    c.S = S;
+    c.X.source = kSrcConfig;
+    c.X.n_states = n;
+    c.X.cfg = d_cfg;
+    c.X.cfg_stride = r->R.n_dof;
+    int grid = 0;
+    c.C = implicit_chunks(n, kKCost, r->R, S.n_obbs, &grid, 1);
+    c.group = 1;
+    c.P.weight = 1.0f;
+    c.clearance_only = 1;
+    c.margin = margin;
+    c.cost_out = d_out;
+    HIP_TRY(launch_cost(c, grid, s));
+    return PPL_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+extern "C" {
+
+ppl_status ppl_mitstar_batch_cc(ppl_robot r, const ppl_scene_view* view, const float* d_samples, int n, float margin,
+                                uint8_t* d_cc, float* d_clearance, float clearance_weight, ppl_stream stream) {
+    CHECK_ARG(r && d_samples && d_cc, "NULL argument");
+    CHECK_ARG(n >= 0, "negative sample count");
+    if (n == 0) return PPL_OK;
+    DScene S{};
+    ppl_status st = scene_from_view(view, S);
+    if (st != PPL_OK) return st;
+    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    // scratch: self verdicts [n] bytes, clearances [n] floats
+    HIP_TRY(r->d_mit.ensure(align256((size_t)n) + (size_t)n * sizeof(float)));
+    uint8_t* self = r->d_mit.as<uint8_t>();
+    float* clr = reinterpret_cast<float*>(self + align256((size_t)n));
+    CollideArgs a{};
+    a.R = r->R;
+    a.S = S;
+    a.X.source = kSrcConfig;
+    a.X.n_states = n;
+    a.X.cfg = d_samples;
+    a.X.cfg_stride = r->R.n_dof;
+    if (r->n_pairs > 0) {
+        if ((st = run_self(r, a.X, 0, self, nullptr, s)) != PPL_OK) return st;
+        a.self_hit = self;
+    }
+    int grid = 0;
+    a.C = implicit_chunks(n, kKCollide, r->R, S.n_obbs, &grid);
+    a.margin = margin;
+    a.mode = 0;
+    a.out = d_cc;
+    HIP_TRY(launch_collide(a, grid, s));
+    if (d_clearance) {
+        if (clearance_weight > 0.0f && (st = configs_clearance(r, S, d_samples, n, margin, clr, s)) != PPL_OK) return st;
+        HIP_TRY(launch_sample_clearance(d_cc, clr, n, clearance_weight, d_clearance, s));
+    }
+    return PPL_OK;
+}
+
+ppl_status ppl_mitstar_node_clearance(ppl_robot r, const ppl_scene_view* view, const float* d_nodes,
+                                      const int* d_node_indices, int node_start, int n_nodes, float margin,
+                                      float* d_clearance, ppl_stream stream) {
+    CHECK_ARG(r && d_nodes && d_clearance, "NULL argument");
+    CHECK_ARG(n_nodes >= 0 && (d_node_indices || node_start >= 0), "bad node range");
+    if (n_nodes == 0) return PPL_OK;
+    DScene S{};
+    ppl_status st = scene_from_view(view, S);
+    if (st != PPL_OK) return st;
+    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    const int D = r->R.n_dof;
+    const size_t cfg_bytes = align256((size_t)n_nodes * D * sizeof(float));
+    HIP_TRY(r->d_mit.ensure(cfg_bytes + (size_t)n_nodes * sizeof(float)));
+    float* cfg = r->d_mit.as<float>();
+    float* clr = reinterpret_cast<float*>(r->d_mit.as<char>() + cfg_bytes);
+    HIP_TRY(launch_gather_rows(d_nodes, D, d_node_indices, node_start, n_nodes, cfg, s));
+    if ((st = configs_clearance(r, S, cfg, n_nodes, margin, clr, s)) != PPL_OK) return st;
+    HIP_TRY(launch_scatter(clr, d_node_indices, node_start, n_nodes, d_clearance, s));
+    return PPL_OK;
+}
+
+ppl_status ppl_mitstar_edge_cc(ppl_robot r, const ppl_scene_view* view, const float* d_nodes, int dim, const int* d_src,
+                               const int* d_dst, const int* d_checks_done, int n_edges, float valid_segment_length,
+                               int num_sparse_checks, float margin, uint8_t* d_cc, int* d_new_checks,
+                               ppl_stream stream) {
+    CHECK_ARG(r && d_nodes && d_src && d_dst && d_cc, "NULL argument");
+    CHECK_ARG(dim == r->R.n_dof, "dim %d != robot n_dof %d", dim, r->R.n_dof);
+    CHECK_ARG(n_edges >= 0, "negative edge count");
+    CHECK_ARG(valid_segment_length > 0.0f, "valid_segment_length must be positive");
+    if (n_edges == 0) return PPL_OK;
+    DScene S{};
+    ppl_status st = scene_from_view(view, S);
+    if (st != PPL_OK) return st;
+    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t E = (size_t)n_edges;
+    // scratch: first midpoint [E], count [E], offsets [E + 1], first hit [E], 64-bit total
+    HIP_TRY(r->d_mit.ensure(align256((4 * E + 1) * sizeof(int)) + sizeof(long long)));
+    int* k0 = r->d_mit.as<int>();
+    int* cnt = k0 + E;
+    int* off = cnt + E;
+    int* first_hit = off + E + 1;
+    long long* d_total = reinterpret_cast<long long*>(r->d_mit.as<char>() + align256((4 * E + 1) * sizeof(int)));
+    HIP_TRY(r->h_mit.ensure(sizeof(long long)));
+    HIP_TRY(launch_bfs_count(d_nodes, dim, d_src, d_dst, d_checks_done, n_edges, valid_segment_length,
+                             num_sparse_checks, k0, cnt, off, d_total, s));
+    HIP_TRY(hipMemcpyAsync(r->h_mit.p, d_total, sizeof(long long), hipMemcpyDeviceToHost, s));
+    HIP_TRY(launch_fill_i32(first_hit, -1, n_edges, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const long long total = *r->h_mit.as<long long>();
+    if (total >= (long long)INT32_MAX)
+        return fail(PPL_ERR_CAPACITY, "%lld edge midpoints exceed 2^31 in one call", total);
+    if (total > 0) {
+        DStates X{};
+        X.source = kSrcBfs;
+        X.n_states = (long)total;
+        X.cfg = d_nodes;
+        X.cfg_stride = dim;
+        X.node_src = d_src;
+        X.node_dst = d_dst;
+        X.bfs_k0 = k0;
+        X.dense_off = off;
+        X.n_traj = n_edges;
+        CollideArgs a{};
+        a.R = r->R;
+        a.S = S;
+        a.X = X;
+        int grid = 0;
+        a.C = implicit_chunks((long)total, kKCollide, r->R, S.n_obbs, &grid);
+        a.margin = margin;
+        a.mode = 1;
+        a.skip_resolved = 1;
+        a.first_hit = first_hit;
+        HIP_TRY(launch_collide(a, grid, s));
+        // self after the environment, on the edges still free (two_phase_cc's order)
+        if (r->n_pairs > 0 && (st = run_self(r, X, 0, nullptr, nullptr, s, FLT_MAX, first_hit)) != PPL_OK) return st;
+    }
+    HIP_TRY(launch_bfs_finish(first_hit, cnt, n_edges, d_cc, d_new_checks, s));
     return PPL_OK;
 }
 

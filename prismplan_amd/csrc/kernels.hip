@@ -392,6 +392,26 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         }
         return ref;
     }
+    if (X.source == kSrcBfs) {
+        // MIT*'s edge midpoints (MITStar.cu:938-965, 1316-1330): level
+        // L = floor(log2(k + 1)), t = (2 (k + 1 - 2^L) + 1) / 2^(L + 1), and
+        // q = q_src (1 - t) + q_dst t as written (no contraction)
+        const int e = upper_bound_idx(X.dense_off, X.n_traj, g);
+        const int f = (int)(g - X.dense_off[e]);
+        const int k = X.bfs_k0[e] + f;
+        ref.unit = e;
+        ref.k = f;
+        ref.n = X.dense_off[e + 1] - X.dense_off[e];
+        ref.u0 = X.dense_off[e];
+        const int kk = k + 1;
+        const int lvl = 31 - __clz(kk);
+        const float t = __fdiv_rn((float)(2 * (kk - (1 << lvl)) + 1), (float)(1 << (lvl + 1)));
+        const float* a = X.cfg + (size_t)X.node_src[e] * X.cfg_stride;
+        const float* b = X.cfg + (size_t)X.node_dst[e] * X.cfg_stride;
+        for (int d = 0; d < D; ++d)
+            s_q[d * kWave + lane] = __fadd_rn(__fmul_rn(a[d], 1.0f - t), __fmul_rn(b[d], t));
+        return ref;
+    }
     if (X.source == kSrcTraj && X.wmajor) {
         const int4 pr = X.prob[prob];
         const int v = (int)(g - pr.w);
@@ -1515,6 +1535,88 @@ __global__ __launch_bounds__(256) void k_stomp_update(StompArgs a) {
 }
 
 // ===========================================================================
+// MIT* substrate consumers (planning/MITStar.cu:212-329, 832-1375)
+// ===========================================================================
+// Per edge: FP32 length in the kernels' order, segment count
+// max(1, ceil(len / seg)), and the breadth-first midpoints still to check:
+// [done, min(sparse, total - 1)), or [done, total - 1) when sparse < 0.
+__global__ void k_bfs_count(const float* nodes, int dim, const int* src, const int* dst, const int* done, int n_edges,
+                            float seg_len, int sparse, int* k0, int* cnt) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_edges) return;
+    const float* a = nodes + (size_t)src[e] * dim;
+    const float* b = nodes + (size_t)dst[e] * dim;
+    float len = 0.0f;
+    for (int i = 0; i < dim; ++i) {
+        const float diff = __fsub_rn(a[i], b[i]);
+        len = fmaf(diff, diff, len);
+    }
+    const int total = max(1, (int)ceilf(__fdiv_rn(__fsqrt_rn(len), seg_len)));
+    const int target = sparse >= 0 ? min(sparse, total - 1) : total - 1;
+    const int start = done ? done[e] : 0;
+    k0[e] = start;
+    cnt[e] = max(0, target - start);
+}
+
+// off = exclusive scan of cnt (off[n] = total), one workgroup of 256 threads,
+// each scanning a contiguous slice; 64-bit running sums, *total64 for the host.
+__global__ __launch_bounds__(256) void k_scan_i32(const int* cnt, int n, int* off, long long* total64) {
+    __shared__ long long part[256];
+    const int tid = threadIdx.x;
+    const int per = (n + 255) / 256, lo = tid * per, hi = min(n, lo + per);
+    long long acc = 0;
+    for (int i = lo; i < hi; ++i) acc += cnt[i];
+    part[tid] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        long long run = 0;
+        for (int k = 0; k < 256; ++k) {
+            const long long v = part[k];
+            part[k] = run;
+            run += v;
+        }
+        *total64 = run;
+        off[n] = (int)min(run, (long long)INT_MAX);
+    }
+    __syncthreads();
+    long long run = part[tid];
+    for (int i = lo; i < hi; ++i) {
+        off[i] = (int)min(run, (long long)INT_MAX);
+        run += cnt[i];
+    }
+}
+
+__global__ void k_bfs_finish(const int* first_hit, const int* cnt, int n, uint8_t* cc, int* new_checks) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const bool hit = first_hit[e] >= 0;
+    cc[e] = hit ? 1 : 0;
+    if (new_checks) new_checks[e] = hit ? 0 : cnt[e];   // only verified-free midpoints count
+}
+
+// batch_cc: clearance of the free samples only (weight > 0: the min
+// clearance, else 1000); colliding samples keep their entry.
+__global__ void k_sample_clearance(const uint8_t* cc, const float* clr, int n, float weight, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || cc[i]) return;
+    out[i] = weight > 0.0f ? clr[i] : 1000.0f;
+}
+
+__global__ void k_gather_rows(const float* src, int dim, const int* idx, int base, int n, float* out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n * dim) return;
+    const int r = (int)(i / dim), d = (int)(i % dim);
+    const int node = idx ? idx[r] : base + r;
+    out[i] = src[(size_t)node * dim + d];
+}
+
+__global__ void k_scatter(const float* v, const int* idx, int base, int n, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[idx ? idx[i] : base + i] = v[i];
+}
+
+// ===========================================================================
 // Small helpers
 // ===========================================================================
 __global__ void k_fill_i32(int* p, int v, long n) {
@@ -1732,6 +1834,40 @@ size_t stomp_update_lds(int n_roll, int max_t) { return sizeof(float) * ((size_t
 hipError_t launch_stomp_update(const StompArgs& a, int max_t, hipStream_t s) {
     if (a.B <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_stomp_update, dim3(a.R.n_dof, a.B), dim3(256), stomp_update_lds(a.n_roll, max_t), s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bfs_count(const float* nodes, int dim, const int* src, const int* dst, const int* done, int n,
+                            float seg_len, int sparse, int* k0, int* cnt, int* off, long long* total, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bfs_count, dim3((n + 255) / 256), dim3(256), 0, s, nodes, dim, src, dst, done, n, seg_len,
+                       sparse, k0, cnt);
+    hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(256), 0, s, cnt, n, off, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_bfs_finish(const int* first_hit, const int* cnt, int n, uint8_t* cc, int* new_checks, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bfs_finish, dim3((n + 255) / 256), dim3(256), 0, s, first_hit, cnt, n, cc, new_checks);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_clearance(const uint8_t* cc, const float* clr, int n, float weight, float* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sample_clearance, dim3((n + 255) / 256), dim3(256), 0, s, cc, clr, n, weight, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(const float* src, int dim, const int* idx, int base, int n, float* out, hipStream_t s) {
+    const long total = (long)n * dim;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, dim, idx, base, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const float* v, const int* idx, int base, int n, float* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, s, v, idx, base, n, out);
     return hipGetLastError();
 }
 

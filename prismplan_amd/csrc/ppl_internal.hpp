@@ -77,7 +77,7 @@ struct DScene {
 };
 
 // Where a flattened state index comes from.
-enum StateSource { kSrcConfig = 0, kSrcEdge = 1, kSrcTraj = 2, kSrcDense = 3 };
+enum StateSource { kSrcConfig = 0, kSrcEdge = 1, kSrcTraj = 2, kSrcDense = 3, kSrcBfs = 4 };
 
 struct DStates {
     int source;
@@ -99,6 +99,11 @@ struct DStates {
     // f = g - dense_off[b]; f < (T_b - 1) nsub[b] lies at segment f / nsub[b],
     // fraction (f % nsub[b]) / nsub[b]; the last one is waypoint T_b - 1.
     const int* dense_off; const int* nsub;
+    // kSrcBfs: graph edges node_src[e] -> node_dst[e] over node configurations
+    // cfg[node * cfg_stride + d]; state g of edge e (dense_off[e] <= g <
+    // dense_off[e+1]) is breadth-first midpoint k = bfs_k0[e] + g - dense_off[e]
+    // (t = 1/2, 1/4, 3/4, 1/8, ...), n_traj holds the edge count.
+    const int* node_src; const int* node_dst; const int* bfs_k0;
     // when set, the state count is read from device memory (*n_dyn) at launch
     const int* n_dyn;
 };
@@ -198,6 +203,15 @@ hipError_t launch_stomp_reuse(const StompArgs& a, int max_t, hipStream_t s);    
 hipError_t launch_stomp_sample(const StompArgs& a, int max_t, hipStream_t s);   // optimum snapshot + fresh samples
 hipError_t launch_stomp_update(const StompArgs& a, int max_t, hipStream_t s);   // weights, projection, step
 size_t stomp_update_lds(int n_roll, int max_t);
+
+// MIT* consumers: per-edge breadth-first midpoint counts + their scan (*total
+// is the 64-bit sum), the edge verdicts, sample clearances, gather / scatter.
+hipError_t launch_bfs_count(const float* nodes, int dim, const int* src, const int* dst, const int* done, int n,
+                            float seg_len, int sparse, int* k0, int* cnt, int* off, long long* total, hipStream_t s);
+hipError_t launch_bfs_finish(const int* first_hit, const int* cnt, int n, uint8_t* cc, int* new_checks, hipStream_t s);
+hipError_t launch_sample_clearance(const uint8_t* cc, const float* clr, int n, float weight, float* out, hipStream_t s);
+hipError_t launch_gather_rows(const float* src, int dim, const int* idx, int base, int n, float* out, hipStream_t s);
+hipError_t launch_scatter(const float* v, const int* idx, int base, int n, float* out, hipStream_t s);
 
 // Joint frames [n][J][16] (column-major 4x4, original joint order) and / or
 // sphere centres [n][S][4] (x, y, z, r, original order) of configurations.

@@ -437,6 +437,47 @@ def stomp_solve(robot: Robot, scene: DeviceScene, start, goal, num_timesteps=51,
                   settings if settings is not None else stomp_settings(), time_limit_ms)
 
 
+def mitstar_batch_cc(robot: Robot, scene: DeviceScene, samples: torch.Tensor, collision_margin=0.0,
+                     clearance: Optional[torch.Tensor] = None, clearance_weight=0.0, stream=None) -> torch.Tensor:
+    """MIT*'s batch_cc: uint8 verdicts (1 = collision, self check included);
+    `clearance` (float32 [n], optional) receives free samples' clearances."""
+    _dev(samples)
+    n = samples.shape[0]
+    cc = torch.empty(n, dtype=torch.uint8, device=samples.device)
+    v = scene.view()
+    call("ppl_mitstar_batch_cc", robot.handle, C.byref(v), _dp(samples), int(n), C.c_float(collision_margin), _dp(cc),
+         _dp(clearance), C.c_float(clearance_weight), _stream(stream))
+    return cc
+
+
+def mitstar_node_clearance(robot: Robot, scene: DeviceScene, nodes: torch.Tensor, clearance: torch.Tensor,
+                           node_indices: Optional[torch.Tensor] = None, node_start=0, n_nodes=None,
+                           collision_margin=0.0, stream=None) -> torch.Tensor:
+    """MIT*'s compute_node_clearance: clearance[node] for the listed nodes."""
+    _dev(nodes)
+    n = int(node_indices.shape[0]) if node_indices is not None else int(n_nodes)
+    v = scene.view()
+    call("ppl_mitstar_node_clearance", robot.handle, C.byref(v), _dp(nodes), _dp(node_indices), int(node_start), n,
+         C.c_float(collision_margin), _dp(clearance), _stream(stream))
+    return clearance
+
+
+def mitstar_edge_cc(robot: Robot, scene: DeviceScene, nodes: torch.Tensor, src: torch.Tensor, dst: torch.Tensor,
+                    valid_segment_length: float, num_sparse_checks=-1, checks_done: Optional[torch.Tensor] = None,
+                    collision_margin=0.0, stream=None):
+    """MIT*'s sparse_cc / full_cc (num_sparse_checks < 0): per-edge uint8
+    verdicts and the midpoints verified free."""
+    _dev(nodes)
+    E = int(src.shape[0])
+    cc = torch.empty(E, dtype=torch.uint8, device=nodes.device)
+    new = torch.empty(E, dtype=torch.int32, device=nodes.device)
+    v = scene.view()
+    call("ppl_mitstar_edge_cc", robot.handle, C.byref(v), _dp(nodes), int(nodes.shape[1]), _dp(src), _dp(dst),
+         _dp(checks_done), E, C.c_float(valid_segment_length), int(num_sparse_checks), C.c_float(collision_margin),
+         _dp(cc), _dp(new), _stream(stream))
+    return cc, new
+
+
 def fk_batched(plan: Plan, configs: torch.Tensor, frames=True, spheres=True, stream=None):
     """Joint frames [n, J, 4, 4] (row-major view of the column-major output)
     and sphere centres + radii [n, S, 4] of every configuration."""
