@@ -16,7 +16,7 @@ timed on this host's cores; parity of the timed outputs vs the FP64 oracle.
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-extras] [--legs a,b,...] [--no-cpu]
 
 --legs picks the extra measurements (skip, any, edges_plain, chomp, stomp,
-esdf, copy); the headline always runs. --no-cpu drops the CPU baseline leg
+chomp_opt, stomp_opt, mitstar, esdf, copy); the headline always runs. --no-cpu drops the CPU baseline leg
 (for profiler runs, whose JSON line then carries cpu_baseline null).
 """
 from __future__ import annotations
@@ -87,7 +87,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
-    ap.add_argument("--legs", default="skip,any,edges_plain,chomp,stomp,chomp_opt,stomp_opt,esdf")
+    ap.add_argument("--legs", default="skip,any,edges_plain,chomp,stomp,chomp_opt,stomp_opt,mitstar,esdf")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     legs = set(x for x in args.legs.split(",") if x)
@@ -152,6 +152,7 @@ def main():
                 ("stomp", "stomp", lambda: bench_stomp(S, W, torch, dev, args)),
                 ("chomp_opt", "chomp_optimizer", lambda: bench_chomp_opt(panda, S, W, torch, dev, args)),
                 ("stomp_opt", "stomp_optimizer", lambda: bench_stomp_opt(panda, S, W, torch, dev, args)),
+                ("mitstar", "mitstar", lambda: bench_mitstar(panda, S, W, torch, dev, args)),
                 ("esdf", "esdf", lambda: bench_esdf(S, W, torch, dev, args)),
                 ("copy", "copy", lambda: bench_copy(torch, dev, args))):
             if key not in legs:
@@ -398,6 +399,35 @@ def bench_stomp_opt(panda, S, W, torch, dev, args):
             "single_goal_ms_per_iteration": single["wall_time_ms"] / max(single["iterations_run"], 1),
             "batched_over_single_goal": (w.n_traj * iters / dt) /
                                         (max(single["iterations_run"], 1) / (single["wall_time_ms"] * 1e-3))}
+
+
+def bench_mitstar(panda, S, W, torch, dev, args):
+    """MIT*'s substrate calls on a device-resident graph: 4,096 Panda nodes
+    (half uniform, half within ~0.15 rad of them), 16,384 edges, a 20-OBB
+    world, self checks on: batch_cc of the nodes, then full_cc (every
+    breadth-first midpoint at 0.02 rad) and sparse_cc (3 midpoints) of the
+    edges."""
+    import numpy as np
+    ds = S.DeviceScene.from_host(W.any_collision(n_configs=16)[0].scene)
+    rng = np.random.default_rng(7)
+    r = panda.arrays
+    nodes = W.uniform_configs(rng, r, 4096)
+    nodes[2048:] = nodes[:2048] + rng.normal(scale=0.15, size=(2048, r.n_dof)).astype(np.float32)
+    nodes = np.clip(nodes, r.joint_lower, r.joint_upper).astype(np.float32)
+    src = rng.integers(0, 2048, 16384).astype(np.int32)
+    dst = np.where(rng.random(16384) < 0.8, src + 2048, rng.integers(0, 4096, 16384)).astype(np.int32)
+    xn, xs, xd = (torch.from_numpy(a).to(dev) for a in (nodes, src, dst))
+    clr = torch.empty(4096, device=dev)
+    wall_b, _ = timed_loop(lambda: S.mitstar_batch_cc(panda, ds, xn, 0.0, clr, 1.0), args.steps, args.warmup, torch)
+    cc, new = S.mitstar_edge_cc(panda, ds, xn, xs, xd, 0.02)
+    mids = int(new.sum().item())
+    wall_f, _ = timed_loop(lambda: S.mitstar_edge_cc(panda, ds, xn, xs, xd, 0.02), args.steps, args.warmup, torch)
+    wall_s, _ = timed_loop(lambda: S.mitstar_edge_cc(panda, ds, xn, xs, xd, 0.02, num_sparse_checks=3),
+                           args.steps, args.warmup, torch)
+    return {"batch_cc_samples_per_s": 4096 / wall_b, "batch_cc_ms": wall_b * 1e3,
+            "full_cc_edges_per_s": 16384 / wall_f, "full_cc_ms": wall_f * 1e3,
+            "full_cc_free_edges": int((cc == 0).sum().item()), "full_cc_verified_midpoints": mids,
+            "sparse_cc_edges_per_s": 16384 / wall_s, "sparse_cc_ms": wall_s * 1e3}
 
 
 def bench_stomp(S, W, torch, dev, args):

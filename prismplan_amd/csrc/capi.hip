@@ -101,7 +101,7 @@ struct ppl_robot_s {
     std::vector<float> lower, upper;
     std::vector<std::pair<int, ppl_plan>> traj_plans;   // single-trajectory plans of the plain CHOMP forms, by T
     hipStream_t stream = nullptr;
-    DBuf d_cfg, d_out, d_off, d_aux, d_self, d_mit;
+    DBuf d_cfg, d_out, d_off, d_aux, d_self, d_mit, d_hint;
     HBuf h_stage, h_mit;
 };
 
@@ -125,7 +125,7 @@ struct StompState {
     std::vector<std::pair<int, float*>> mats;   // per length T: [L^T | M^T]
     const float** d_lt = nullptr;   // [2 B]: L^T of each trajectory, then M^T
     const float** d_mt = nullptr;
-    DBuf buf;
+    DBuf buf, hint;
     float *fresh = nullptr, *fresh_cost = nullptr, *keep[2] = {}, *keep_cost[2] = {};
     float *slot = nullptr, *slot_cost = nullptr, *theta_cost = nullptr, *ctrl_sum = nullptr, *totals = nullptr;
     int *sel = nullptr, *first_hit = nullptr, *nsub = nullptr, *dense_off = nullptr;
@@ -315,7 +315,8 @@ int edge_g(const float* a, const float* b, int dim, const ppl_edge_params& p, fl
 // Self pass. mode 0: any-collision; mode 1: min clearance, exact below `band`
 // (hinge callers pass their self margin).
 ppl_status run_self(const ppl_robot_s* r, const DStates& X, int mode, uint8_t* hit_out, float* out, hipStream_t s,
-                    float band = FLT_MAX, int* unit_first_hit = nullptr, ppl_plan_s* P = nullptr) {
+                    float band = FLT_MAX, int* unit_first_hit = nullptr, ppl_plan_s* P = nullptr,
+                    const int* unit_hint = nullptr) {
     if (mode == 0 && !r->self_gate_implied)
         return fail(PPL_ERR_UNSUPPORTED,
                     "self any-collision: the approximate model does not cover every exact self pair (an exact "
@@ -331,6 +332,7 @@ ppl_status run_self(const ppl_robot_s* r, const DStates& X, int mode, uint8_t* h
         a.C.chunk = kWave / kSelfGroup;
     } else {
         a.X.wmajor = 0;  // flat plan order
+        a.C.unit_hint = unit_hint;
     }
     a.mode = mode;
     a.band = band;
@@ -683,6 +685,7 @@ ppl_status ppl_robot_destroy(ppl_robot r) {
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     if (r->blob) (void)hipFree(r->blob);
     r->d_cfg.release(); r->d_out.release(); r->d_off.release(); r->d_aux.release(); r->d_self.release();
+    r->d_mit.release(); r->d_hint.release(); r->h_mit.release();
     r->h_stage.release();
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
@@ -1038,6 +1041,7 @@ ppl_status ppl_plan_destroy(ppl_plan P) {
         for (auto& e : z->mats) (void)hipFree(e.second);
         if (z->d_lt) (void)hipFree((void*)z->d_lt);
         z->buf.release();
+        z->hint.release();
         delete z;
     }
     P->d_opt.release();
@@ -1399,13 +1403,21 @@ struct DenseCheck {
     DStates Xd{};
     int vgrid = 0;
     int* first_hit = nullptr;
+    int* hint = nullptr;        // unit holding each 64-state block (built on the device per evaluation)
+    int n_hint = 0;
     long n_traj = 0;
 };
 
-ppl_status dense_check_setup(const ppl_plan_s* P, const DScene& S, float step, float margin, int nsub_max,
-                             int* first_hit, int* nsub, int* dense_off, ChompArgs& a, DenseCheck& dc) {
+// Upper bound of the densified state count, and the hint entries it needs.
+size_t dense_bound(const ppl_plan_s* P, int nsub_max) {
     size_t n_dense = 0;
     for (int u = 0; u < P->n_units; ++u) n_dense += (size_t)(P->h_off[u + 1] - P->h_off[u] - 1) * nsub_max + 1;
+    return n_dense;
+}
+
+ppl_status dense_check_setup(const ppl_plan_s* P, const DScene& S, float step, float margin, int nsub_max,
+                             int* first_hit, int* nsub, int* dense_off, int* hint, ChompArgs& a, DenseCheck& dc) {
+    const size_t n_dense = dense_bound(P, nsub_max);
     CHECK_ARG(n_dense < (size_t)INT32_MAX, "densified validity states exceed 2^31");
     a.check_step = step;
     a.margin = margin;
@@ -1428,16 +1440,20 @@ ppl_status dense_check_setup(const ppl_plan_s* P, const DScene& S, float step, f
     dc.va.skip_resolved = 1;
     dc.va.first_hit = first_hit;
     dc.va.self_hit = nullptr;
+    dc.va.C.unit_hint = hint;
     dc.first_hit = first_hit;
+    dc.hint = hint;
+    dc.n_hint = (int)(n_dense / 64 + 1);
     dc.n_traj = P->n_units;
     return PPL_OK;
 }
 
 ppl_status dense_check_run(const ppl_robot_s* r, const ChompArgs& a, DenseCheck& dc, hipStream_t s) {
     HIP_TRY(launch_chomp_nsub(a, s));
+    HIP_TRY(launch_unit_hint(a.dense_off, (int)dc.n_traj, dc.hint, dc.n_hint, s));
     HIP_TRY(launch_fill_i32(dc.first_hit, -1, dc.n_traj, s));
     HIP_TRY(launch_collide(dc.va, dc.vgrid, s));
-    if (r->n_pairs > 0) return run_self(r, dc.Xd, 0, nullptr, nullptr, s, FLT_MAX, dc.first_hit);
+    if (r->n_pairs > 0) return run_self(r, dc.Xd, 0, nullptr, nullptr, s, FLT_MAX, dc.first_hit, nullptr, dc.hint);
     return PPL_OK;
 }
 
@@ -1558,8 +1574,10 @@ ppl_status ppl_chomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_
     const int D = r->R.n_dof;
     const size_t n = (size_t)P->n_states, B = (size_t)P->n_units;
     const int nsub_max = nsub_cap(r, cs->collision_check_step, 1.0);
-    // scratch: gradient [n D], state cost [n], totals [B], first hit [B], nsub [B], dense offsets [B + 1]
-    HIP_TRY(P->d_opt.ensure((n * D + n + B) * sizeof(float) + (3 * B + 1) * sizeof(int)));
+    // scratch: gradient [n D], state cost [n], totals [B], first hit [B], nsub [B], dense offsets [B + 1],
+    // block hints [n_dense / 64 + 1]
+    const size_t n_hint = dense_bound(P, nsub_max) / 64 + 1;
+    HIP_TRY(P->d_opt.ensure((n * D + n + B) * sizeof(float) + (3 * B + 1 + n_hint) * sizeof(int)));
     float* grad = P->d_opt.as<float>();
     float* state = grad + n * D;
     float* total = state + n;
@@ -1581,7 +1599,7 @@ ppl_status ppl_chomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_
     a.total = total;
     DenseCheck dc{};
     if ((st = dense_check_setup(P, S, cs->collision_check_step, cs->collision_margin, nsub_max, first_hit,
-                                first_hit + B, first_hit + 2 * B, a, dc)) != PPL_OK)
+                                first_hit + B, first_hit + 2 * B, first_hit + 3 * B + 1, a, dc)) != PPL_OK)
         return st;
     // cost (+ gradient) of the current theta, its totals and its validity
     auto evaluate = [&]() -> ppl_status {
@@ -1651,8 +1669,9 @@ ppl_status ppl_stomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_
     a.record = d_record;
     a.total = z->totals;
     DenseCheck dc{};
+    HIP_TRY(z->hint.ensure((dense_bound(P, nsub_max) / 64 + 1) * sizeof(int)));
     if ((st = dense_check_setup(P, S, ss->collision_check_step, ss->collision_margin, nsub_max, z->first_hit, z->nsub,
-                                z->dense_off, a, dc)) != PPL_OK)
+                                z->dense_off, z->hint.as<int>(), a, dc)) != PPL_OK)
         return st;
     auto evaluate = [&]() -> ppl_status {
         ppl_status e = hinge_impl(P, view, d_theta, &cp, z->theta_cost, z->totals, nullptr, s);
@@ -1937,6 +1956,9 @@ ppl_status ppl_mitstar_edge_cc(ppl_robot r, const ppl_scene_view* view, const fl
     if (total >= (long long)INT32_MAX)
         return fail(PPL_ERR_CAPACITY, "%lld edge midpoints exceed 2^31 in one call", total);
     if (total > 0) {
+        HIP_TRY(r->d_hint.ensure((size_t)(total / 64 + 1) * sizeof(int)));
+        int* hint = r->d_hint.as<int>();
+        HIP_TRY(launch_unit_hint(off, n_edges, hint, (int)(total / 64 + 1), s));
         DStates X{};
         X.source = kSrcBfs;
         X.n_states = (long)total;
@@ -1953,13 +1975,16 @@ ppl_status ppl_mitstar_edge_cc(ppl_robot r, const ppl_scene_view* view, const fl
         a.X = X;
         int grid = 0;
         a.C = implicit_chunks((long)total, kKCollide, r->R, S.n_obbs, &grid);
+        a.C.unit_hint = hint;
         a.margin = margin;
         a.mode = 1;
         a.skip_resolved = 1;
         a.first_hit = first_hit;
         HIP_TRY(launch_collide(a, grid, s));
         // self after the environment, on the edges still free (two_phase_cc's order)
-        if (r->n_pairs > 0 && (st = run_self(r, X, 0, nullptr, nullptr, s, FLT_MAX, first_hit)) != PPL_OK) return st;
+        if (r->n_pairs > 0 &&
+            (st = run_self(r, X, 0, nullptr, nullptr, s, FLT_MAX, first_hit, nullptr, hint)) != PPL_OK)
+            return st;
     }
     HIP_TRY(launch_bfs_finish(first_hit, cnt, n_edges, d_cc, d_new_checks, s));
     return PPL_OK;

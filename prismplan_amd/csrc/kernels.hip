@@ -333,7 +333,8 @@ struct StateRef {
 // when the chunk's first unit is unknown, u0 < 0).
 __device__ __forceinline__ void stage_unit_ends(const DStates& X, int u0, int* s_end) {
     if (X.source == kSrcConfig || u0 < 0) return;
-    const int* off = X.source == kSrcEdge ? X.edge_off : X.traj_off;
+    const bool dense = X.source == kSrcDense || X.source == kSrcBfs;
+    const int* off = X.source == kSrcEdge ? X.edge_off : dense ? X.dense_off : X.traj_off;
     const int n = X.source == kSrcEdge ? X.n_edges : X.n_traj;
     const int u = u0 + lane_id();
     const bool in = u < n;
@@ -370,8 +371,8 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
     const int lane = lane_id();
     if (!active) return ref;
     if (X.source == kSrcDense) {
-        const int b = upper_bound_idx(X.dense_off, X.n_traj, g);
-        const int d0 = X.dense_off[b];
+        int d0, d1;
+        const int b = find_unit(X.dense_off, X.n_traj, g, u0, s_end, &d0, &d1);
         const int t0 = X.traj_off[b], T = X.traj_off[b + 1] - t0;
         const int ns = X.nsub[b];
         const int f = (int)(g - d0);
@@ -396,13 +397,14 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         // MIT*'s edge midpoints (MITStar.cu:938-965, 1316-1330): level
         // L = floor(log2(k + 1)), t = (2 (k + 1 - 2^L) + 1) / 2^(L + 1), and
         // q = q_src (1 - t) + q_dst t as written (no contraction)
-        const int e = upper_bound_idx(X.dense_off, X.n_traj, g);
-        const int f = (int)(g - X.dense_off[e]);
+        int e0, e1;
+        const int e = find_unit(X.dense_off, X.n_traj, g, u0, s_end, &e0, &e1);
+        const int f = (int)(g - e0);
         const int k = X.bfs_k0[e] + f;
         ref.unit = e;
         ref.k = f;
-        ref.n = X.dense_off[e + 1] - X.dense_off[e];
-        ref.u0 = X.dense_off[e];
+        ref.n = e1 - e0;
+        ref.u0 = e0;
         const int kk = k + 1;
         const int lvl = 31 - __clz(kk);
         const float t = __fdiv_rn((float)(2 * (kk - (1 << lvl)) + 1), (float)(1 << (lvl + 1)));
@@ -491,7 +493,8 @@ __device__ __forceinline__ bool next_chunk(const DChunks& C, long n_states, int 
         ch.x = uniform(ch.x); ch.y = uniform(ch.y); ch.z = uniform(ch.z); ch.w = uniform(ch.w);
     } else {
         const long b = (long)c * C.chunk;
-        ch = make_int4(0, (int)b, (int)(b + C.chunk < n_states ? b + C.chunk : n_states), -1);
+        const int u = C.unit_hint ? uniform(C.unit_hint[b >> 6]) : -1;
+        ch = make_int4(0, (int)b, (int)(b + C.chunk < n_states ? b + C.chunk : n_states), u);
     }
     return true;
 }
@@ -1005,11 +1008,27 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
         base = w * kSelfStates;
         g = base + st;
         active = g < n_states;
+        if (a.C.unit_hint) {
+            u0 = uniform(a.C.unit_hint[base >> 6]);
+            stage_unit_ends(a.X, u0, s_end);
+            __syncthreads();
+        }
+    }
+    // units already decided need no configuration, FK or pairs (the unit
+    // first-hit mode runs over densified trajectories and edge midpoints)
+    const bool by_unit = a.unit_first_hit && (a.X.source == kSrcDense || a.X.source == kSrcBfs);
+    if (by_unit) {
+        int b0, b1;
+        const int u = find_unit(a.X.dense_off, a.X.n_traj, active ? g : base, u0, s_end, &b0, &b1);
+        if (active && a.unit_first_hit[u] >= 0) active = false;
+        if (!wave_any(active)) continue;
     }
     // Every lane of a state group loads the same q (lane-indexed LDS column).
     const StateRef ref = load_state(a.X, D, active ? g : base, true, s_q, u0, s_end, prob);
-    if (active && a.unit_first_hit && a.unit_first_hit[ref.unit] >= 0) active = false;   // unit already invalid
-    if (a.unit_first_hit && !wave_any(active)) continue;      // nothing left to decide in this round
+    if (a.unit_first_hit && !by_unit) {
+        if (active && a.unit_first_hit[ref.unit] >= 0) active = false;
+        if (!wave_any(active)) continue;
+    }
     __syncthreads();
     Slots sl;
     float4* pos = s_pos + st * S;
@@ -1558,32 +1577,63 @@ __global__ void k_bfs_count(const float* nodes, int dim, const int* src, const i
     cnt[e] = max(0, target - start);
 }
 
-// off = exclusive scan of cnt (off[n] = total), one workgroup of 256 threads,
-// each scanning a contiguous slice; 64-bit running sums, *total64 for the host.
-__global__ __launch_bounds__(256) void k_scan_i32(const int* cnt, int n, int* off, long long* total64) {
-    __shared__ long long part[256];
-    const int tid = threadIdx.x;
-    const int per = (n + 255) / 256, lo = tid * per, hi = min(n, lo + per);
+// off = exclusive scan of cnt (off[n] = total): one workgroup of 1024 threads
+// walks tiles of 1024 counts with wave scans. Offsets are only used when the
+// 64-bit total (*total64, summed separately) is below 2^31, so every partial
+// sum then fits 32 bits.
+__global__ __launch_bounds__(1024) void k_scan_i32(const int* cnt, int n, int* off, long long* total64) {
+    __shared__ int wsum[16];
+    __shared__ long long part[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int carry = 0;
     long long acc = 0;
-    for (int i = lo; i < hi; ++i) acc += cnt[i];
-    part[tid] = acc;
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + tid;
+        const int c = i < n ? cnt[i] : 0;
+        acc += c;
+        int v = c;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int t = __shfl_up(v, o);
+            if (lane >= o) v += t;
+        }
+        if (lane == kWave - 1) wsum[wave] = v;
+        __syncthreads();
+        int before = 0, tile = 0;
+        for (int w = 0; w < 16; ++w) {
+            const int x = wsum[w];
+            before += w < wave ? x : 0;
+            tile += x;
+        }
+        if (i < n) off[i] = carry + before + v - c;
+        carry += tile;
+        __syncthreads();
+    }
+    // 64-bit total, reduced from 32-bit halves
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const int lo = __shfl_xor((int)(unsigned)(acc & 0xffffffffll), o);
+        const int hi = __shfl_xor((int)(acc >> 32), o);
+        acc += ((long long)hi << 32) | (unsigned)lo;
+    }
+    if (lane == 0) part[wave] = acc;
     __syncthreads();
     if (tid == 0) {
-        long long run = 0;
-        for (int k = 0; k < 256; ++k) {
-            const long long v = part[k];
-            part[k] = run;
-            run += v;
-        }
-        *total64 = run;
-        off[n] = (int)min(run, (long long)INT_MAX);
+        long long t = 0;
+        for (int w = 0; w < 16; ++w) t += part[w];
+        *total64 = t;
+        off[n] = (int)min(t, (long long)INT_MAX);
     }
-    __syncthreads();
-    long long run = part[tid];
-    for (int i = lo; i < hi; ++i) {
-        off[i] = (int)min(run, (long long)INT_MAX);
-        run += cnt[i];
-    }
+}
+
+// hint[c] = the unit holding state 64 c, for every 64-state block below the
+// total (one thread per block; c < n_hint, the buffer's capacity)
+__global__ void k_unit_hint(const int* off, int n, int* hint, int n_hint) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_hint) return;
+    const long g = (long)c << 6;
+    if (g >= off[n]) return;
+    hint[c] = upper_bound_idx(off, n, g);
 }
 
 __global__ void k_bfs_finish(const int* first_hit, const int* cnt, int n, uint8_t* cc, int* new_checks) {
@@ -1842,7 +1892,13 @@ hipError_t launch_bfs_count(const float* nodes, int dim, const int* src, const i
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_bfs_count, dim3((n + 255) / 256), dim3(256), 0, s, nodes, dim, src, dst, done, n, seg_len,
                        sparse, k0, cnt);
-    hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(256), 0, s, cnt, n, off, total);
+    hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, s, cnt, n, off, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_unit_hint(const int* off, int n, int* hint, int n_hint, hipStream_t s) {
+    if (n <= 0 || n_hint <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unit_hint, dim3((n_hint + 255) / 256), dim3(256), 0, s, off, n, hint, n_hint);
     return hipGetLastError();
 }
 

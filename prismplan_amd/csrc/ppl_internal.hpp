@@ -115,6 +115,7 @@ struct DChunks {
     int chunk;                  // states per chunk (64 / lanes per state)
     const int* obb_off;         // [P+1] OBB ranges per problem; NULL = single problem
     int obb_cap;                // OBB records the LDS staging area holds (>= OBBs of any problem)
+    const int* unit_hint;       // implicit chunks (tab NULL), nullable: unit holding state 64 k, for each k
 };
 
 struct DCostParams {
@@ -208,6 +209,8 @@ size_t stomp_update_lds(int n_roll, int max_t);
 // is the 64-bit sum), the edge verdicts, sample clearances, gather / scatter.
 hipError_t launch_bfs_count(const float* nodes, int dim, const int* src, const int* dst, const int* done, int n,
                             float seg_len, int sparse, int* k0, int* cnt, int* off, long long* total, hipStream_t s);
+// hint[k] = the unit holding state 64 k, for the units' offsets off[0..n]; k < n_hint
+hipError_t launch_unit_hint(const int* off, int n, int* hint, int n_hint, hipStream_t s);
 hipError_t launch_bfs_finish(const int* first_hit, const int* cnt, int n, uint8_t* cc, int* new_checks, hipStream_t s);
 hipError_t launch_sample_clearance(const uint8_t* cc, const float* clr, int n, float weight, float* out, hipStream_t s);
 hipError_t launch_gather_rows(const float* src, int dim, const int* idx, int base, int n, float* out, hipStream_t s);
