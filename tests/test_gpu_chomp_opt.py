@@ -27,9 +27,9 @@ def banner(msg):
     print(f"[gpu-test] {msg}", flush=True)
 
 
-def setup(n_traj, n_wp, seed, **kw):
-    w = W.chomp(n_traj=n_traj, n_wp=n_wp, n_obbs=30, seed=seed)
-    panda = S.Robot("panda")
+def setup(n_traj, n_wp, seed, robot="panda", lengths=None, **kw):
+    w = W.chomp(n_traj=n_traj, n_wp=n_wp, n_obbs=30, seed=seed, robot=robot, lengths=lengths)
+    panda = S.Robot(robot)
     plan = S.Plan.trajectories(panda, w.lengths())
     ds = S.DeviceScene.from_host(w.scene)
     th = torch.from_numpy(w.theta.copy()).to(DEV)
@@ -203,3 +203,39 @@ def test_chomp_solve_scene():
     assert prrtc.chomp_solve_scene(start, goal, scene, ps, robot).iterations_run <= 1
     with pytest.raises(RuntimeError):
         prrtc.chomp_solve_scene(start[:3], goal, scene, ps, robot)
+
+
+@pytest.mark.parametrize("robot", ["fetch", "baxter"])
+def test_covariant_step_other_robots_and_ragged(robot):
+    """The covariant step on Fetch and Baxter, ragged lengths including a
+    two-waypoint trajectory (no interior: unchanged)."""
+    lengths = [24, 2, 40, 17]
+    w, rob, plan, ds, th, st = setup(len(lengths), 0, 36, robot=robot, lengths=lengths, world_margin=0.05,
+                                     step_size=0.05)
+    D = w.robot.n_dof
+    prm = S.cost_params(world_margin=st.world_margin, self_margin=st.self_margin, weight=st.collision_cost_weight,
+                        use_self=True)
+    _, _, g = S.hinge_cost(S.Plan.trajectories(rob, w.lengths()), ds, th, prm, grad=True)
+    g = g.cpu().numpy().astype(np.float64)
+    opt = S.ChompOptimizer(plan, ds, th, st).step(1)
+    got = opt.theta.cpu().numpy().astype(np.float64)
+    r = w.robot
+    off = np.concatenate([[0], np.cumsum(lengths)])
+    worst = 0.0
+    for b, T in enumerate(lengths):
+        x0 = w.theta[off[b] * D: off[b + 1] * D].reshape(D, T).astype(np.float64)
+        x1g = got[off[b] * D: off[b + 1] * D].reshape(D, T)
+        if T == 2:
+            np.testing.assert_array_equal(x1g, x0)
+            continue
+        gb = g[off[b] * D: off[b + 1] * D].reshape(D, T)
+        R = PO.control_R(T, st.dt)
+        Ai = np.linalg.inv(R + 1e-8 * np.eye(T))
+        step = (gb + st.smoothness_weight * x0 @ R.T) @ Ai.T
+        x1 = x0 - st.step_size * step
+        x1[:, 0], x1[:, -1] = x0[:, 0], x0[:, -1]
+        x1 = np.clip(x1, r.joint_lower[:, None], r.joint_upper[:, None])
+        scale = max(np.abs(st.step_size * step).max(), 1e-3)
+        worst = max(worst, np.abs(x1g - x1).max() / scale)
+    banner(f"covariant step {robot}: max err {worst:.2e} (relative to the largest step)")
+    assert worst < 1e-5

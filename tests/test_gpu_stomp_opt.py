@@ -28,9 +28,9 @@ def banner(msg):
     print(f"[gpu-test] {msg}", flush=True)
 
 
-def setup(lengths, seed, **kw):
-    w = W.chomp(n_traj=len(lengths), n_obbs=30, seed=seed, lengths=lengths)
-    panda = S.Robot("panda")
+def setup(lengths, seed, robot="panda", **kw):
+    w = W.chomp(n_traj=len(lengths), n_obbs=30, seed=seed, lengths=lengths, robot=robot)
+    panda = S.Robot(robot)
     plan = S.Plan.trajectories(panda, w.lengths())
     ds = S.DeviceScene.from_host(w.scene)
     th = torch.from_numpy(w.theta.copy()).to(DEV)
@@ -178,3 +178,74 @@ def test_stomp_solve_scene():
     ps.num_batch = 2
     with pytest.raises(RuntimeError):
         prrtc.stomp_solve_scene(start, goal, scene, ps, robot)
+
+
+@pytest.mark.parametrize("robot,n_old", [("fetch", 6), ("baxter", 0)])
+def test_stomp_other_robots_and_edge_cases(robot, n_old):
+    """Fetch (prismatic torso, tree) and Baxter (two arms, 14 dof); no kept
+    rollouts; a two-waypoint trajectory (no interior: it never moves)."""
+    lengths = [20, 2, 33]
+    w, rob, plan, ds, th, st = setup(lengths, 35, robot=robot, num_rollouts_new=9, num_rollouts_old=n_old,
+                                     world_margin=0.05, iterations_after_valid=50)
+    D = w.robot.n_dof
+    off = np.concatenate([[0], np.cumsum(lengths)])
+    o = PO.Oracle(w.robot, w.scene, nthreads=8)
+    cost, check = oracle_fns(o, st)
+    orc = [SO.StompOracle(x.astype(np.float64), w.robot.joint_lower, w.robot.joint_upper, cost,
+                          lambda t: check(t)[0], Rn=9, Ro=n_old, dt=st.dt, w_u=st.control_cost_weight,
+                          h=st.exponentiated_cost_sensitivity, seed=st.seed, b=b, polish=50)
+           for b, x in enumerate(blocks(w.theta, off, D))]
+    opt = S.StompOptimizer(plan, ds, th, st).step(0)
+    for it in range(3):
+        opt.step(1)
+        for x in orc:
+            x.iterate()
+        got = blocks(opt.theta.cpu().numpy().astype(np.float64), off, D)
+        err = max(np.abs(g - x.th).max() for g, x in zip(got, orc))
+        banner(f"STOMP {robot} (old {n_old}) iteration {it + 1}: max |theta - oracle| {err:.2e}")
+        assert err < STEP_TOL
+    np.testing.assert_array_equal(got[1], w.theta[off[1] * D: off[2] * D].reshape(D, 2))
+
+
+@pytest.mark.parametrize("polish,eps", [(3, 1e-3), (50, 1e9)])
+def test_stomp_solved_bookkeeping_matches_oracle(polish, eps):
+    """A world with nothing to hit: trajectories are valid from the start, so
+    best snapshots, the convergence streak, the polish count and the done
+    flag all act -- done by polish (3 iterations) or, with polish to spare
+    and a huge eps, by a streak of two flat iterations; every record field
+    follows the oracle's loop (stomp.cu:838-887)."""
+    lengths = [24, 30, 24, 18, 27, 24]
+    w = W.chomp(n_traj=len(lengths), n_obbs=0, seed=37, lengths=lengths, amp=0.05)
+    rob = S.Robot("panda")
+    plan = S.Plan.trajectories(rob, w.lengths())
+    ds = S.DeviceScene.from_host(w.scene)
+    th = torch.from_numpy(w.theta.copy()).to(DEV)
+    st = S.stomp_settings(num_rollouts_new=8, num_rollouts_old=4, iterations_after_valid=polish, convergence_eps=eps)
+    D = w.robot.n_dof
+    off = np.concatenate([[0], np.cumsum(lengths)])
+    o = PO.Oracle(w.robot, w.scene, nthreads=8)
+    cost, check = oracle_fns(o, st)
+    orc = [SO.StompOracle(x.astype(np.float64), w.robot.joint_lower, w.robot.joint_upper, cost,
+                          lambda t: check(t)[0], Rn=8, Ro=4, dt=st.dt, w_u=st.control_cost_weight,
+                          h=st.exponentiated_cost_sensitivity, seed=st.seed, b=b, polish=polish, eps=eps)
+           for b, x in enumerate(blocks(w.theta, off, D))]
+    opt = S.StompOptimizer(plan, ds, th, st).step(0)
+    for _ in range(8):
+        opt.step(1)
+        for x in orc:
+            x.iterate()
+    rec = opt.record.cpu().numpy()
+    best = blocks(opt.best.cpu().numpy().astype(np.float64), off, D)
+    n_solved = 0
+    for b, x in enumerate(orc):
+        solved = x.best is not None
+        n_solved += solved
+        assert bool(rec[b, _ppl.CHOMP_SOLVED]) == solved, b
+        assert int(rec[b, _ppl.CHOMP_ITERS]) == x.iters, (b, rec[b], x.iters)
+        assert bool(rec[b, _ppl.CHOMP_DONE]) == x.done, b
+        if solved:
+            assert abs(rec[b, _ppl.CHOMP_BEST_COST] - x.best_cost) <= 1e-4 * max(abs(x.best_cost), 1.0)
+            assert np.abs(best[b] - x.best).max() < STEP_TOL
+    banner(f"STOMP solved bookkeeping: {n_solved}/{len(lengths)} solved, done {int(rec[:, _ppl.CHOMP_DONE].sum())}, "
+           f"iterations {rec[:, _ppl.CHOMP_ITERS].astype(int).tolist()}")
+    assert n_solved >= 4 and rec[:, _ppl.CHOMP_DONE].sum() >= 4
