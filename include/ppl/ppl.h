@@ -175,6 +175,20 @@ typedef struct {
     int use_self;
 } ppl_cost_params;
 
+/* Plain forms of the other two per-configuration reductions, with
+ * check_collision_scene's contract (host arrays [N*n_dof] in, host array out,
+ * synchronous):
+ *   h_clearance[i] = scene_min_clearance_runtime of configuration i: the min
+ *     ESDF clearance, collision_margin included (collision/
+ *     scene_collision.cuh:339-384);
+ *   h_cost[i] = weight * (summed ESDF and OBB hinge + self min-hinge), the
+ *     state cost the STOMP and CHOMP kernels read (scene_collision.cuh:
+ *     394-495, planning/runtime_kinematics.cuh:294-329). */
+ppl_status ppl_min_clearance_scene(ppl_robot robot, const ppl_scene_view* scene, const float* h_configs, int n,
+                                   float collision_margin, float* h_clearance);
+ppl_status ppl_hinge_cost_scene(ppl_robot robot, const ppl_scene_view* scene, const float* h_configs, int n,
+                                const ppl_cost_params* params, float* h_cost);
+
 /* chomp_state_cost_kernel (chomp.cu:151-188) for one trajectory
  * d_theta [D][T] (dim-major): d_state_costs [D][T], D-broadcast. */
 ppl_status ppl_chomp_state_cost(ppl_robot robot, const ppl_scene_view* scene, const float* d_theta, int T,

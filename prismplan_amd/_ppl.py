@@ -153,6 +153,8 @@ SIGNATURES = [
     ("ppl_stomp_buffers_destroy", C.c_int, [_vp]),
     ("ppl_stomp_solve", C.c_int, [_vp, _vp, C.POINTER(SceneView), _f32p, _f32p, C.c_int, C.c_int, C.c_float,
                                   C.POINTER(StompSettings), _f32p, C.POINTER(ChompResult), _vp]),
+    ("ppl_min_clearance_scene", C.c_int, [_vp, C.POINTER(SceneView), _f32p, C.c_int, C.c_float, _f32p]),
+    ("ppl_hinge_cost_scene", C.c_int, [_vp, C.POINTER(SceneView), _f32p, C.c_int, C.POINTER(CostParams), _f32p]),
     ("ppl_mitstar_batch_cc", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.c_int, C.c_float, _vp, _vp, C.c_float,
                                        _vp]),
     ("ppl_mitstar_node_clearance", C.c_int, [_vp, C.POINTER(SceneView), _vp, _vp, C.c_int, C.c_int, C.c_float, _vp,

@@ -1864,6 +1864,78 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 extern "C" {
 
+// Stage n host configurations into the robot's device buffer (synchronous
+// plain forms); returns the device copy.
+namespace {
+ppl_status stage_configs(ppl_robot_s* r, const float* h_cfg, int n, float** d_cfg) {
+    const size_t bytes = (size_t)n * r->R.n_dof * sizeof(float);
+    HIP_TRY(r->d_cfg.ensure(bytes));
+    HIP_TRY(r->h_stage.ensure(std::max(bytes, (size_t)n * sizeof(float))));
+    memcpy(r->h_stage.p, h_cfg, bytes);
+    HIP_TRY(hipMemcpyAsync(r->d_cfg.p, r->h_stage.p, bytes, hipMemcpyHostToDevice, r->stream));
+    *d_cfg = r->d_cfg.as<float>();
+    return PPL_OK;
+}
+
+ppl_status fetch_floats(ppl_robot_s* r, const float* d_out, int n, float* h_out) {
+    HIP_TRY(hipMemcpyAsync(r->h_stage.p, d_out, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    memcpy(h_out, r->h_stage.p, (size_t)n * sizeof(float));
+    return PPL_OK;
+}
+}  // namespace
+
+ppl_status ppl_min_clearance_scene(ppl_robot r, const ppl_scene_view* view, const float* h_configs, int n,
+                                   float margin, float* h_clearance) {
+    CHECK_ARG(r, "robot is NULL");
+    CHECK_ARG(n >= 0, "negative N");
+    if (n == 0) return PPL_OK;
+    CHECK_ARG(h_configs && h_clearance, "NULL host arrays");
+    DScene S{};
+    ppl_status st = scene_from_view(view, S);
+    if (st != PPL_OK) return st;
+    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
+    float* d_cfg = nullptr;
+    if ((st = stage_configs(r, h_configs, n, &d_cfg)) != PPL_OK) return st;
+    HIP_TRY(r->d_out.ensure((size_t)n * sizeof(float)));
+    if ((st = configs_clearance(r, S, d_cfg, n, margin, r->d_out.as<float>(), r->stream)) != PPL_OK) return st;
+    return fetch_floats(r, r->d_out.as<float>(), n, h_clearance);
+}
+
+ppl_status ppl_hinge_cost_scene(ppl_robot r, const ppl_scene_view* view, const float* h_configs, int n,
+                                const ppl_cost_params* prm, float* h_cost) {
+    CHECK_ARG(r && prm, "NULL argument");
+    CHECK_ARG(n >= 0, "negative N");
+    if (n == 0) return PPL_OK;
+    CHECK_ARG(h_configs && h_cost, "NULL host arrays");
+    DScene S{};
+    ppl_status st = scene_from_view(view, S);
+    if (st != PPL_OK) return st;
+    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
+    float* d_cfg = nullptr;
+    if ((st = stage_configs(r, h_configs, n, &d_cfg)) != PPL_OK) return st;
+    CostArgs c{};
+    c.R = r->R;
+    c.S = S;
+    c.X.source = kSrcConfig;
+    c.X.n_states = n;
+    c.X.cfg = d_cfg;
+    c.X.cfg_stride = r->R.n_dof;
+    if (prm->use_self && r->n_pairs > 0) {
+        HIP_TRY(r->d_self.ensure((size_t)n * 8 * sizeof(float)));
+        if ((st = run_self(r, c.X, 1, nullptr, r->d_self.as<float>(), r->stream, prm->self_margin)) != PPL_OK) return st;
+        c.self_in = r->d_self.as<float>();
+    }
+    HIP_TRY(r->d_out.ensure((size_t)n * sizeof(float)));
+    int grid = 0;
+    c.C = implicit_chunks(n, kKCost, r->R, S.n_obbs, &grid, 1);
+    c.group = 1;
+    c.P = {prm->world_margin, prm->self_margin, prm->weight, prm->use_obb, prm->use_esdf, prm->use_self};
+    c.cost_out = r->d_out.as<float>();
+    HIP_TRY(launch_cost(c, grid, r->stream));
+    return fetch_floats(r, r->d_out.as<float>(), n, h_cost);
+}
+
 ppl_status ppl_mitstar_batch_cc(ppl_robot r, const ppl_scene_view* view, const float* d_samples, int n, float margin,
                                 uint8_t* d_cc, float* d_clearance, float clearance_weight, ppl_stream stream) {
     CHECK_ARG(r && d_samples && d_cc, "NULL argument");

@@ -437,6 +437,26 @@ def stomp_solve(robot: Robot, scene: DeviceScene, start, goal, num_timesteps=51,
                   settings if settings is not None else stomp_settings(), time_limit_ms)
 
 
+def min_clearance_scene(robot: Robot, scene: DeviceScene, configs, collision_margin=0.0) -> np.ndarray:
+    """Plain form: min ESDF clearance (+ margin) of host configurations."""
+    q = _np(configs, np.float32).reshape(-1, robot.n_dof)
+    out = np.empty(q.shape[0], np.float32)
+    v = scene.view()
+    call("ppl_min_clearance_scene", robot.handle, C.byref(v), _p(q, _f32p), q.shape[0],
+         C.c_float(collision_margin), _p(out, _f32p))
+    return out
+
+
+def hinge_cost_scene(robot: Robot, scene: DeviceScene, configs, params: _ppl.CostParams) -> np.ndarray:
+    """Plain form: weighted summed-hinge state cost of host configurations."""
+    q = _np(configs, np.float32).reshape(-1, robot.n_dof)
+    out = np.empty(q.shape[0], np.float32)
+    v = scene.view()
+    call("ppl_hinge_cost_scene", robot.handle, C.byref(v), _p(q, _f32p), q.shape[0], C.byref(params),
+         _p(out, _f32p))
+    return out
+
+
 def mitstar_batch_cc(robot: Robot, scene: DeviceScene, samples: torch.Tensor, collision_margin=0.0,
                      clearance: Optional[torch.Tensor] = None, clearance_weight=0.0, stream=None) -> torch.Tensor:
     """MIT*'s batch_cc: uint8 verdicts (1 = collision, self check included);

@@ -94,6 +94,8 @@ def test_null_arguments_rejected():
     L = _ppl.lib()
     assert L.ppl_robot_create(None, None) == _ppl.PPL_ERR_INVALID
     assert L.ppl_check_collision_scene(None, None, None, 0, C.c_float(0), None) == _ppl.PPL_ERR_INVALID
+    assert L.ppl_min_clearance_scene(None, None, None, 0, C.c_float(0), None) == _ppl.PPL_ERR_INVALID
+    assert L.ppl_hinge_cost_scene(None, None, None, 0, None, None) == _ppl.PPL_ERR_INVALID
     assert L.ppl_plan_states(None) == -1
     assert L.ppl_robot_destroy(None) == _ppl.PPL_OK
     assert L.ppl_plan_destroy(None) == _ppl.PPL_OK

@@ -162,6 +162,40 @@ def test_esdf_any_collision_and_min_clearance(robots, interp):
     assert np.mean(err[ok] < 1e-5) > 0.995
 
 
+def test_plain_clearance_and_hinge_forms(robots):
+    """Host-array forms of min clearance and the summed hinge cost
+    (scene_collision.cuh:339-384, 394-495): the FP64 oracle's values to the
+    batched bar, and the device-resident batched forms' values."""
+    s, _ = esdf_scene(interp=1, seed=33)
+    ds = S.DeviceScene.from_host(s)
+    for name in ("panda", "fetch"):
+        r = robots[name].arrays
+        q = W.uniform_configs(np.random.default_rng(34), r, 1500)
+        banner(f"plain min_clearance / hinge_cost forms {name} N={len(q)}")
+        o = PO.Oracle(r, s, nthreads=8)
+        mc = S.min_clearance_scene(robots[name], ds, q, collision_margin=0.01)
+        mref = o.min_clearance(q, margin=0.01)
+        ok = np.abs(mref) < 999
+        err = np.abs(mc - mref) / np.maximum(np.abs(mref), 0.01)
+        banner(f"  clearance max rel err {err[ok].max():.2e}")
+        assert np.mean(err[ok] < 1e-5) > 0.995
+        plan = S.Plan.configs(robots[name], len(q))
+        x = torch.from_numpy(q).to(DEV)
+        np.testing.assert_array_equal(mc, S.min_clearance_batched(plan, ds, x, collision_margin=0.01).cpu().numpy())
+        for use_self in (False, True):
+            prm = S.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0, use_self=use_self)
+            c = S.hinge_cost_scene(robots[name], ds, q, prm)
+            c_ref = o.hinge_cost(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0,
+                                                   use_self=use_self))
+            rel = np.abs(c - c_ref) / np.maximum(np.abs(c_ref), 2.0 * 0.05)
+            banner(f"  hinge use_self={use_self}: max rel err {rel.max():.2e} (active {np.mean(c_ref > 0):.2f})")
+            assert np.mean(rel < 1e-5) > 0.995
+            # the batched form may split a state's spheres over lanes: same terms, other summation order
+            np.testing.assert_allclose(c, S.hinge_cost(plan, ds, x, prm)[0].cpu().numpy(), rtol=2e-6, atol=1e-7)
+    assert S.min_clearance_scene(robots["panda"], ds, np.zeros((0, 7), np.float32)).shape == (0,)
+    assert S.hinge_cost_scene(robots["panda"], ds, np.zeros((0, 7), np.float32), S.cost_params()).shape == (0,)
+
+
 def test_plain_edge_evaluator(robots):
     e = W.edges(n_edges=600, n_obbs=12, seed=40)
     ds = S.DeviceScene.from_host(e.scene)
