@@ -101,6 +101,7 @@ typedef struct {
     const float* apx_orig;    /* [n_approx_spheres][4] by original index */
     const int* pairs;         /* [n_pairs][2] self pairs expanded from self_cc_ranges (original indices) */
     const int* apx_pairs;     /* [n_approx_pairs][2] */
+    const float* lim;         /* [n_dof][2] joint lower, upper */
 } ppl_robot_device_view;
 
 ppl_status ppl_robot_get_device_view(ppl_robot robot, ppl_robot_device_view* out);
@@ -498,6 +499,102 @@ ppl_status ppl_mitstar_edge_cc(ppl_robot robot, const ppl_scene_view* scene, con
                                const int* d_src, const int* d_dst, const int* d_checks_done, int n_edges,
                                float valid_segment_length, int num_sparse_checks, float collision_margin,
                                uint8_t* d_cc, int* d_new_checks, ppl_stream stream);
+
+/* ------------------------------------------------------------------------
+ * pRRTC: GPU-parallel RRT-Connect on the substrate, the reference's
+ * pRRTC::solve_runtime_scene (planning/pRRTC.cu:2016-2223, kernel
+ * rrtc_runtime_scene at :1556-2010; settings planning/pRRTC_settings.hh:3-32;
+ * pybind solve_scene python/bindings.cpp:405-437, 1023-1027).
+ *
+ * num_new_configs workgroups grow a start tree and a goal tree (one root per
+ * goal). Per iteration a workgroup draws a Halton sample scaled to the joint
+ * limits, steers from the nearest node of its tree by at most `range`, checks
+ * the edge at `granularity` states nearest + k (new - nearest) / granularity
+ * (k = 1..granularity, two-phase scene test + self check, collision_margin),
+ * adds the node, then greedily extends towards the nearest node of the other
+ * tree in ceil(dist / range) steps, each checked the same way. The first
+ * workgroup whose extension reaches the other tree solves the problem.
+ * Tree choice follows `balance` (0: halves; 1: proportional / tree_ratio;
+ * 2: swap when the size gap is small); dynamic_domain keeps a per-node radius
+ * (FLT_MAX until a failed extension, then dd_radius shrinking by dd_alpha to
+ * dd_min_radius; grown by dd_alpha on success) and skips samples outside it.
+ * The run stops at max_iters per workgroup, a full tree (max_samples nodes)
+ * or time_limit_ms (> 0) of the device wall clock.
+ *
+ * Halton bases: workgroup 0 uses the first n_dof of the primes 3..59;
+ * workgroup w > 0 a Fisher-Yates shuffle of the 16 primes drawn from
+ * splitmix64 (state seed ^ (w * 0xD1B54A32D192ED03), j = next % (i + 1) for
+ * i = 15..1); the reference shuffles with curand, so its per-workgroup
+ * bases differ. ppl_rrtc_halton_bases returns the table. The ESDF lookup is
+ * the reference's nearest voxel (esdf_interp must be 0).
+ *
+ * h_path receives the solution start -> goal (the pybind order), n_waypoints
+ * rows of n_dof; cost sums the tree edges walked (the reference's `cost`).
+ * With shortcut_path the path is shortcut by ppl_shortcut_path from the goal
+ * end (margin collision_margin + shortcut_collision_margin), as the
+ * reference does before its pybind layer reverses it. Synchronous on
+ * `stream`. Nodes, parents and radii live in `bufs` (NULL: allocated for the
+ * call); concurrent solves need separate buffers.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int max_samples;          /* nodes per tree */
+    int max_iters;            /* iterations per workgroup */
+    float time_limit_ms;      /* <= 0: none */
+    int num_new_configs;      /* workgroups */
+    int granularity;          /* states per edge check, 1..64 */
+    float range;
+    int balance;              /* 0, 1 or 2 */
+    float tree_ratio;
+    int dynamic_domain;
+    float dd_alpha, dd_radius, dd_min_radius;
+    int shortcut_path;
+    float valid_segment_length;
+    float collision_margin;
+    float shortcut_collision_margin;
+    unsigned long long seed;  /* Halton base shuffle */
+} ppl_rrtc_settings;
+
+/* pRRTC_settings defaults (planning/pRRTC_settings.hh:3-32), seed 1. */
+void ppl_rrtc_default_settings(ppl_rrtc_settings* s);
+
+typedef struct {
+    int solved;
+    int iters;                /* the solving workgroup's iterations, else the most any workgroup ran */
+    int start_tree_size, goal_tree_size;
+    int path_length;          /* tree nodes on the path (the reference's path_length) */
+    int n_waypoints;          /* rows written to h_path */
+    int reached_goal;         /* index of the goal root the path ends at */
+    float cost;
+    double wall_ms, kernel_ms;
+} ppl_rrtc_result;
+
+/* SolverBuffers (planning/solver_buffers.hh:15-54): device trees for up to
+ * max_samples nodes each, num_new_configs workgroups of granularity states. */
+typedef struct ppl_rrtc_buffers_s* ppl_rrtc_buffers;
+ppl_status ppl_rrtc_buffers_create(int n_dof, int max_samples, int num_new_configs, int granularity,
+                                   ppl_rrtc_buffers* out);
+ppl_status ppl_rrtc_buffers_destroy(ppl_rrtc_buffers bufs);
+
+/* h_goals [n_goals][n_dof]; h_path [max_waypoints][n_dof]. PPL_ERR_CAPACITY
+ * when the solution has more than max_waypoints rows (result->n_waypoints
+ * then holds the count needed). */
+ppl_status ppl_rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const float* h_start, const float* h_goals,
+                          int n_goals, const ppl_rrtc_settings* settings, ppl_rrtc_buffers bufs, float* h_path,
+                          int max_waypoints, ppl_rrtc_result* result, ppl_stream stream);
+
+/* The Halton bases [n_workgroups][n_dof] the solve uses for `seed`. */
+ppl_status ppl_rrtc_halton_bases(unsigned long long seed, int n_workgroups, int n_dof, float* h_bases);
+
+/* ppln::shortcut::shortcut_path (planning/shortcut.cuh:183-307): every skip
+ * edge (i, j >= i + 2) of the n-waypoint path is checked at its interior
+ * states t = s / total, s = 1 .. total - 1, total = max(1, ceil(|q_j - q_i| /
+ * valid_segment_length)), q = q_i (1 - t) + q_j t (FP32, no contraction),
+ * with the scene and self checks at collision_margin; then greedily from
+ * waypoint 0 the farthest free j is taken (else i + 1). h_out [n][n_dof]
+ * receives *n_out <= n waypoints. Synchronous on `stream`. */
+ppl_status ppl_shortcut_path(ppl_robot robot, const ppl_scene_view* scene, const float* h_path, int n,
+                             float valid_segment_length, float collision_margin, float* h_out, int* n_out,
+                             ppl_stream stream);
 
 /* ------------------------------------------------------------------------
  * Algorithmic flop counts (ppl/roofline.h) of one work unit for this robot:

@@ -332,6 +332,26 @@ __device__ bool self_collide(const ppl_robot_device_view& R, const float* pos, i
 }
 
 // ---------------------------------------------------------------------------
+// Halton sampling (planning/pRRTC.cu:955-995)
+// ---------------------------------------------------------------------------
+// One dimension of the incremental Halton generator: base b, numerator n and
+// denominator d as exact small integers in FP32 (start n = 0, d = 1). The k-th
+// call returns the radical inverse of k in base b, n / d, and advances (n, d).
+// Exact while d < 2^24.
+__device__ __forceinline__ float halton_step(float b, float& n, float& d) {
+    const float x = d - n;
+    if (x == 1.0f) {
+        d = floorf(d * b);
+        n = 1.0f;
+    } else {
+        float y = floorf(d / b);
+        while (x <= y) y = floorf(y / b);
+        n = floorf((b + 1.0f) * y) - x;
+    }
+    return n / d;
+}
+
+// ---------------------------------------------------------------------------
 // Two-phase any-collision (collision/two_phase_cc.cuh:57-138)
 // ---------------------------------------------------------------------------
 struct TwoPhaseResult {

@@ -52,7 +52,7 @@ class RobotDeviceView(C.Structure):
                 ("n_pairs", C.c_int), ("n_approx_pairs", C.c_int),
                 ("ft", _vp), ("jmeta", _vp), ("sph_off", _vp), ("sph", _vp), ("sph_id", _vp), ("apx_off", _vp),
                 ("apx", _vp), ("apx_id", _vp), ("sph_orig", _vp), ("apx_orig", _vp), ("pairs", _vp),
-                ("apx_pairs", _vp)]
+                ("apx_pairs", _vp), ("lim", _vp)]
 
 
 class SceneView(C.Structure):
@@ -102,6 +102,21 @@ class ChompResult(C.Structure):
 
 CHOMP_BEST_COST, CHOMP_PREV_COST, CHOMP_SOLVED, CHOMP_STREAK, CHOMP_POLISH_LEFT, CHOMP_DONE, CHOMP_ITERS, \
     CHOMP_LAST_COST = range(8)
+
+
+class RrtcSettings(C.Structure):
+    _fields_ = [("max_samples", C.c_int), ("max_iters", C.c_int), ("time_limit_ms", C.c_float),
+                ("num_new_configs", C.c_int), ("granularity", C.c_int), ("range", C.c_float), ("balance", C.c_int),
+                ("tree_ratio", C.c_float), ("dynamic_domain", C.c_int), ("dd_alpha", C.c_float),
+                ("dd_radius", C.c_float), ("dd_min_radius", C.c_float), ("shortcut_path", C.c_int),
+                ("valid_segment_length", C.c_float), ("collision_margin", C.c_float),
+                ("shortcut_collision_margin", C.c_float), ("seed", C.c_ulonglong)]
+
+
+class RrtcResult(C.Structure):
+    _fields_ = [("solved", C.c_int), ("iters", C.c_int), ("start_tree_size", C.c_int), ("goal_tree_size", C.c_int),
+                ("path_length", C.c_int), ("n_waypoints", C.c_int), ("reached_goal", C.c_int), ("cost", C.c_float),
+                ("wall_ms", C.c_double), ("kernel_ms", C.c_double)]
 
 
 class EdgeBatchParams(C.Structure):
@@ -161,6 +176,14 @@ SIGNATURES = [
                                              _vp]),
     ("ppl_mitstar_edge_cc", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_float,
                                       C.c_int, C.c_float, _vp, _vp, _vp]),
+    ("ppl_rrtc_default_settings", None, [C.POINTER(RrtcSettings)]),
+    ("ppl_rrtc_halton_bases", C.c_int, [C.c_ulonglong, C.c_int, C.c_int, _f32p]),
+    ("ppl_rrtc_buffers_create", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    ("ppl_rrtc_buffers_destroy", C.c_int, [_vp]),
+    ("ppl_rrtc_solve", C.c_int, [_vp, C.POINTER(SceneView), _f32p, _f32p, C.c_int, C.POINTER(RrtcSettings), _vp,
+                                 _f32p, C.c_int, C.POINTER(RrtcResult), _vp]),
+    ("ppl_shortcut_path", C.c_int, [_vp, C.POINTER(SceneView), _f32p, C.c_int, C.c_float, C.c_float, _f32p, _i32p,
+                                    _vp]),
     ("ppl_esdf_from_obbs", C.c_int, [C.POINTER(SceneView), _f32p, _f32p, _vp, _vp]),
     ("ppl_graph_capture_stomp", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), C.c_float,
                                           C.c_float, _vp, _vp, _vp, C.POINTER(_vp)]),

@@ -20,7 +20,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB_DIR = PKG / "_lib"
 LIB = LIB_DIR / "libppl.so"
-SOURCES = ["kernels.hip", "capi.hip"]
+SOURCES = ["kernels.hip", "capi.hip", "rrtc.hip"]
 ARCH = os.environ.get("PPL_OFFLOAD_ARCH", "gfx950")
 
 

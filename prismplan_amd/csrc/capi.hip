@@ -345,6 +345,19 @@ ppl_status run_self(const ppl_robot_s* r, const DStates& X, int mode, uint8_t* h
 
 }  // namespace
 
+namespace ppl {
+ppl_status fail_msg(ppl_status s, const char* msg) {
+    g_err = msg;
+    return s;
+}
+ppl_status scene_view_check(const ppl_scene_view* v) {
+    DScene S{};
+    ppl_status st = scene_from_view(v, S);
+    return st != PPL_OK ? st : check_obb_capacity(S.n_obbs);
+}
+bool robot_self_gate_implied(ppl_robot r) { return r && r->self_gate_implied; }
+}  // namespace ppl
+
 extern "C" {
 
 const char* ppl_last_error(void) { return g_err.c_str(); }
@@ -675,6 +688,7 @@ ppl_status ppl_robot_get_device_view(ppl_robot r, ppl_robot_device_view* v) {
     v->apx_orig = reinterpret_cast<const float*>(R.apx_orig);
     v->pairs = reinterpret_cast<const int*>(R.pairs);
     v->apx_pairs = reinterpret_cast<const int*>(R.apx_pairs);
+    v->lim = reinterpret_cast<const float*>(R.lim);
     return PPL_OK;
 }
 

@@ -1,0 +1,848 @@
+// rrtc.hip -- pRRTC (GPU-parallel RRT-Connect) as a planner kernel on the
+// device-inline substrate API (ppl/ppl_device.hpp) and the public C ABI,
+// plus the path shortcut pass.
+//
+// Reference: planning/pRRTC.cu:1556-2010 (kernel rrtc_runtime_scene),
+// :2016-2223 (solve_runtime_scene), :955-995 (Halton), planning/shortcut.cuh:
+// 16-307 (shortcut_path), planning/pRRTC_settings.hh:3-32 (settings).
+//
+// Layout in HBM (one ppl_rrtc_buffers):
+//   nodes   [2][max_samples][n_dof] FP32; rows not yet written hold -9999, so a
+//           row that is not (fully) visible to a reader is never its nearest
+//   parents [2][max_samples] int32; a root is its own parent, every other
+//           node's parent has a smaller index
+//   radii   [2][max_samples] FP32 dynamic-domain radius, FLT_MAX = unbounded
+//   ctl     RrtcCtl: run status, per-tree counters, the solving workgroup,
+//           the path walk's results
+//   bases   [workgroups][16] FP32 Halton bases
+//   path    [2][kPathCap][n_dof] FP32 the two tree walks of the solution
+// One workgroup (64 * ceil(4 * granularity / 64) lanes) owns one new
+// configuration per iteration; each aligned 4-lane group checks one state of
+// an edge with two_phase_cc<4>. Counters move with agent-scope atomics; node
+// rows are published with a release fence before `completed` counts them and
+// read after an acquire fence, so a reader sees a row either whole or as the
+// -9999 sentinel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "ppl/ppl.h"
+#include "ppl/ppl_device.hpp"
+
+namespace ppl {   // capi.hip
+ppl_status fail_msg(ppl_status s, const char* msg);          // sets ppl_last_error
+ppl_status scene_view_check(const ppl_scene_view* v);        // pointers, alignment, OBB limit
+bool robot_self_gate_implied(ppl_robot r);                   // exact self pairs covered by approximate ones
+}
+
+namespace {
+
+using namespace ppl::device;
+
+constexpr int kMaxDim = 16;
+constexpr int kG = 4;                // lanes per edge state
+constexpr int kMaxGran = 64;         // states per edge check
+constexpr int kMaxThreads = kG * kMaxGran;
+constexpr int kMaxWaves = kMaxThreads / 64;
+constexpr int kPathCap = 4096;       // nodes per tree walk
+constexpr float kUnwritten = -9999.0f;
+constexpr size_t kMaxDynLds = 56 * 1024;
+constexpr long long kMaxShortcutStates = 1ll << 26;
+
+struct RrtcCtl {
+    int status;            // 0 running, 1 solved, -1 stopped (limit reached)
+    int free_idx[2];       // next node index per tree (may run past max_samples)
+    int completed[2];      // rows written per tree
+    int win_tree, win_t, win_o, win_iter;
+    int max_iter;
+    int path_n[2];
+    int reached_goal;
+    int path_ok;
+    float cost;
+};
+
+struct RrtcArgs {
+    ppl_robot_device_view R;
+    ppl_scene_view S;
+    float* nodes[2];
+    int* parents[2];
+    float* radii[2];
+    RrtcCtl* ctl;
+    const float* bases;    // [workgroups][16]
+    float* path;           // [2][kPathCap][dim]
+    int dim, max_samples, max_iters, granularity, balance, dynamic_domain, n_blocks;
+    float range, tree_ratio, dd_alpha, dd_radius, dd_min_radius, margin;
+    long long time_ticks;  // device wall-clock ticks, 0 = no limit
+};
+
+__device__ __forceinline__ int ld_agent(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ float ld_agent_f(const float* p) {
+    return __int_as_float(ld_agent(reinterpret_cast<const int*>(p)));
+}
+
+// Dynamic-domain radius update by compare-and-swap on the value last seen
+// (pRRTC.cu:1788-1799 grow on success, :1991-2004 shrink on failure).
+__device__ void radius_update(float* r, bool grow, float alpha, float dd_radius, float dd_min) {
+    int* p = reinterpret_cast<int*>(r);
+    int cur = ld_agent(p);
+    for (;;) {
+        const float old = __int_as_float(cur);
+        float nv;
+        if (grow) {
+            if (old == FLT_MAX) return;
+            nv = old * (1.0f + alpha);
+        } else {
+            nv = old == FLT_MAX ? dd_radius : fmaxf(old * (1.0f - alpha), dd_min);
+        }
+        const int prev = atomicCAS(p, cur, __float_as_int(nv));
+        if (prev == cur) return;
+        cur = prev;
+    }
+}
+
+// Workgroup (distance, index) minimum: smaller distance, then smaller index.
+// Leaves the result in red_d[0], red_i[0] for every lane.
+__device__ void block_argmin(float d, int i, float* red_d, int* red_i) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float od = __shfl_xor(d, o);
+        const int oi = __shfl_xor(i, o);
+        if (od < d || (od == d && oi < i)) {
+            d = od;
+            i = oi;
+        }
+    }
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        red_d[w] = d;
+        red_i[w] = i;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k)
+            if (red_d[k] < red_d[0] || (red_d[k] == red_d[0] && red_i[k] < red_i[0])) {
+                red_d[0] = red_d[k];
+                red_i[0] = red_i[k];
+            }
+    }
+    __syncthreads();
+}
+
+// Nearest of the first `size` rows of a tree to q (squared L2, brute force
+// over the workgroup's lanes: pRRTC.cu:1637-1666).
+__device__ void nearest(const float* nodes, int size, const float* q, int dim, float* red_d, int* red_i) {
+    float best = FLT_MAX;
+    int bi = 0;
+    for (int i = threadIdx.x; i < size; i += blockDim.x) {
+        const float* x = nodes + (size_t)i * dim;
+        float s = 0.0f;
+        for (int k = 0; k < dim; ++k) {
+            const float t = x[k] - q[k];
+            s = fmaf(t, t, s);
+        }
+        if (s < best) {
+            best = s;
+            bi = i;
+        }
+    }
+    block_argmin(best, bi, red_d, red_i);
+}
+
+// Whether any of the states base + (k + 1) * step, k = 0 .. granularity - 1
+// (one per 4-lane group), collides: the two-phase scene test and the self
+// check (pRRTC.cu:1680-1781, 1867-1948).
+__device__ bool edge_hit(const RrtcArgs& A, const float* base, const float* step, float* qbuf, float* pos,
+                         float* apos) {
+    const int g = threadIdx.x / kG;
+    bool hit = false;
+    if (g < A.granularity) {
+        float* q = qbuf + g * kMaxDim;
+        for (int k = threadIdx.x & (kG - 1); k < A.dim; k += kG) q[k] = fmaf((float)(g + 1), step[k], base[k]);
+        group_sync();
+        const TwoPhaseResult r = two_phase_cc<kG>(A.R, A.S, q, pos + (size_t)g * 3 * A.R.n_spheres,
+                                                  apos + (size_t)g * 3 * A.R.n_approx_spheres, threadIdx.x,
+                                                  A.margin, true);
+        hit = r.collision;
+    }
+    return __syncthreads_or(hit ? 1 : 0) != 0;
+}
+
+// Publish row `idx` of tree t (written by lanes < dim) and count it.
+__device__ void publish(const RrtcArgs& A, int t) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&A.ctl->completed[t], 1);
+}
+
+__device__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
+// Tree a workgroup grows this iteration (pRRTC.cu:1602-1627).
+__device__ int choose_tree(const RrtcArgs& A, int iter, int prev_t) {
+    const int bid = blockIdx.x;
+    if (A.balance == 0 || iter == 1) return bid < A.n_blocks / 2 ? 0 : 1;
+    const int f0 = ld_agent(&A.ctl->free_idx[0]), f1 = ld_agent(&A.ctl->free_idx[1]);
+    if (A.balance == 1) {
+        const float ratio = (float)f0 / (float)(f0 + f1);
+        if ((float)abs(f0 - f1) < 1.5f * (float)A.n_blocks) return bid < (float)A.n_blocks * (1.0f - ratio) ? 0 : 1;
+        return ratio < A.tree_ratio ? 0 : 1;
+    }
+    if (A.balance == 2) {
+        const int ft = prev_t == 0 ? f0 : f1, fo = prev_t == 0 ? f1 : f0;
+        const float ratio = (float)abs(ft - fo) / (float)ft;
+        return ratio < A.tree_ratio ? 1 - prev_t : prev_t;
+    }
+    return prev_t;
+}
+
+__global__ __launch_bounds__(kMaxThreads) void k_rrtc(RrtcArgs A) {
+    extern __shared__ float lds[];   // [groups][3 * n_spheres] then [groups][3 * n_approx]
+    __shared__ float s_cfg[kMaxDim], s_near[kMaxDim], s_step[kMaxDim], s_vec[kMaxDim];
+    __shared__ float s_hn[kMaxDim], s_hd[kMaxDim];
+    __shared__ float s_q[kMaxGran * kMaxDim];
+    __shared__ float red_d[kMaxWaves];
+    __shared__ int red_i[kMaxWaves];
+    __shared__ int s_t, s_stop, s_skip, s_size, s_index, s_next;
+    __shared__ float s_dist;
+    const int tid = threadIdx.x, dim = A.dim;
+    const int groups = blockDim.x / kG;
+    float* pos = lds;
+    float* apos = lds + (size_t)groups * 3 * A.R.n_spheres;
+    const float* hb = A.bases + (size_t)blockIdx.x * kMaxDim;
+    if (tid < dim) {
+        s_hn[tid] = 0.0f;
+        s_hd[tid] = 1.0f;
+    }
+    if (tid == 0) s_t = 0;
+    const long long t0 = wall_clock64();
+    int iter = 0;
+    for (;;) {
+        ++iter;
+        __syncthreads();
+        if (tid == 0) {
+            int stop = ld_agent(&A.ctl->status) != 0;
+            if (!stop && (iter > A.max_iters || (A.time_ticks > 0 && wall_clock64() - t0 > A.time_ticks))) {
+                atomicCAS(&A.ctl->status, 0, -1);
+                stop = 1;
+            }
+            s_stop = stop;
+            if (!stop) s_t = choose_tree(A, iter, s_t);
+        }
+        if (tid < dim) {   // Halton sample scaled to the joint limits (pRRTC.cu:1634-1635)
+            const float h = halton_step(hb[tid], s_hn[tid], s_hd[tid]);
+            const float lo = A.R.lim[2 * tid], hi = A.R.lim[2 * tid + 1];
+            s_cfg[tid] = fmaf(h, hi - lo, lo);
+        }
+        __syncthreads();
+        if (s_stop) break;
+        const int t = s_t, o = 1 - t;
+        if (tid == 0) s_size = min(ld_agent(&A.ctl->free_idx[t]), ld_agent(&A.ctl->completed[t]));
+        __syncthreads();
+        acquire_agent();
+        nearest(A.nodes[t], s_size, s_cfg, dim, red_d, red_i);
+        const int near = red_i[0];
+        if (tid == 0) {
+            const float dist = sqrtf(red_d[0]);
+            s_dist = dist;
+            s_skip = A.dynamic_domain && ld_agent_f(&A.radii[t][near]) < dist;
+        }
+        if (tid < dim) s_near[tid] = A.nodes[t][(size_t)near * dim + tid];
+        __syncthreads();
+        if (s_skip) continue;
+        if (tid < dim) {   // steer by at most `range` (pRRTC.cu:1668-1677)
+            const float scale = fminf(1.0f, A.range / s_dist);
+            const float nq = s_near[tid];
+            const float c = fmaf(s_cfg[tid] - nq, scale, nq);
+            s_cfg[tid] = c;
+            s_step[tid] = (c - nq) / (float)A.granularity;
+        }
+        __syncthreads();
+        if (edge_hit(A, s_near, s_step, s_q, pos, apos)) {
+            if (A.dynamic_domain && tid == 0) radius_update(&A.radii[t][near], false, A.dd_alpha, A.dd_radius, A.dd_min_radius);
+            continue;
+        }
+        // add the node (pRRTC.cu:1783-1812)
+        if (tid == 0) {
+            int idx = atomicAdd(&A.ctl->free_idx[t], 1);
+            if (idx >= A.max_samples) {
+                atomicCAS(&A.ctl->status, 0, -1);
+                idx = -1;
+            } else {
+                A.parents[t][idx] = near;
+                if (A.dynamic_domain) radius_update(&A.radii[t][near], true, A.dd_alpha, A.dd_radius, A.dd_min_radius);
+            }
+            s_index = idx;
+        }
+        __syncthreads();
+        if (s_index < 0) break;
+        int last = s_index;
+        if (tid < dim) A.nodes[t][(size_t)last * dim + tid] = s_cfg[tid];
+        publish(A, t);
+        // connect: greedy extension towards the nearest node of the other tree (pRRTC.cu:1814-1960)
+        if (tid == 0) s_size = min(ld_agent(&A.ctl->free_idx[o]), ld_agent(&A.ctl->completed[o]));
+        __syncthreads();
+        acquire_agent();
+        nearest(A.nodes[o], s_size, s_cfg, dim, red_d, red_i);
+        const int onear = red_i[0];
+        if (tid == 0) s_next = (int)ceilf(sqrtf(red_d[0]) / A.range);
+        if (tid < dim) s_near[tid] = A.nodes[o][(size_t)onear * dim + tid];
+        __syncthreads();
+        const int n_ext = s_next;
+        if (tid < dim) {
+            const float v = (s_near[tid] - s_cfg[tid]) / (float)n_ext;
+            s_vec[tid] = v;
+            s_step[tid] = v / (float)A.granularity;
+        }
+        __syncthreads();
+        int e = 0;
+        bool full = false;
+        for (; e < n_ext; ++e) {
+            if (edge_hit(A, s_cfg, s_step, s_q, pos, apos)) break;
+            if (tid == 0) {
+                int idx = -1;
+                if (A.time_ticks > 0 && wall_clock64() - t0 > A.time_ticks) atomicCAS(&A.ctl->status, 0, -1);
+                if (ld_agent(&A.ctl->status) == 0) {   // another workgroup may have finished the run
+                    idx = atomicAdd(&A.ctl->free_idx[t], 1);
+                    if (idx >= A.max_samples) {
+                        atomicCAS(&A.ctl->status, 0, -1);
+                        idx = -1;
+                    } else {
+                        A.parents[t][idx] = last;
+                    }
+                }
+                s_index = idx;
+            }
+            __syncthreads();
+            if (s_index < 0) {
+                full = true;
+                break;
+            }
+            last = s_index;
+            if (tid < dim) {
+                const float c = s_cfg[tid] + s_vec[tid];
+                s_cfg[tid] = c;
+                A.nodes[t][(size_t)last * dim + tid] = c;
+            }
+            publish(A, t);
+        }
+        if (full) break;
+        if (e == n_ext && tid == 0 && atomicCAS(&A.ctl->status, 0, 1) == 0) {
+            A.ctl->win_tree = t;
+            A.ctl->win_t = last;
+            A.ctl->win_o = onear;
+            A.ctl->win_iter = iter;
+        }
+        __syncthreads();
+        if (tid == 0) s_stop = ld_agent(&A.ctl->status) != 0;
+        __syncthreads();
+        if (s_stop) break;
+    }
+    if (tid == 0) atomicMax(&A.ctl->max_iter, iter);
+}
+
+// Walk both trees from the connection to their roots (pRRTC.cu:1962-1988):
+// path[tree] lists the walked nodes (roots excluded), cost sums the walked
+// edges' L2 lengths, the solving tree first. A walk that leaves [0, free) or
+// does not descend to smaller indices leaves path_ok = 0.
+__global__ void k_rrtc_path(RrtcArgs A) {
+    if (threadIdx.x != 0) return;
+    RrtcCtl* c = A.ctl;
+    c->path_ok = 0;
+    c->path_n[0] = c->path_n[1] = 0;
+    c->reached_goal = 0;
+    c->cost = 0.0f;
+    if (c->status != 1) return;
+    const int dim = A.dim;
+    float cost = 0.0f;
+    for (int side = 0; side < 2; ++side) {
+        const int tree = side == 0 ? c->win_tree : 1 - c->win_tree;
+        int cur = side == 0 ? c->win_t : c->win_o;
+        const float* N = A.nodes[tree];
+        const int* P = A.parents[tree];
+        const int lim = min(c->free_idx[tree], A.max_samples);
+        float* out = A.path + (size_t)tree * kPathCap * dim;
+        int n = 0;
+        for (;;) {
+            if (cur < 0 || cur >= lim) return;
+            const int p = P[cur];
+            if (p == cur) break;
+            if (p < 0 || p > cur || n >= kPathCap) return;
+            float s = 0.0f;
+            for (int k = 0; k < dim; ++k) {
+                const float d = N[(size_t)cur * dim + k] - N[(size_t)p * dim + k];
+                s = fmaf(d, d, s);
+                out[(size_t)n * dim + k] = N[(size_t)cur * dim + k];
+            }
+            cost += sqrtf(s);
+            ++n;
+            cur = p;
+        }
+        c->path_n[tree] = n;
+        if (tree == 1) c->reached_goal = cur;
+    }
+    c->cost = cost;
+    c->path_ok = 1;
+}
+
+__global__ void k_fill_f32(float* p, size_t n, float v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Shortcut skip-edge states (shortcut.cuh:85-92): state k of pair e (pairs
+// sorted by `first`) is s = k - first + 1 of total, t = s / total,
+// q = q_i (1 - t) + q_j t with separately rounded products and sum.
+__global__ void k_seg_expand(const float* path, int dim, const int4* pairs, int n_pairs, int n_states, float* out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_states) return;
+    int lo = 0, hi = n_pairs - 1;
+    while (lo < hi) {   // last pair with first <= k
+        const int mid = (lo + hi + 1) >> 1;
+        if (pairs[mid].w <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    const int4 e = pairs[lo];
+    const float t = (float)(k - e.w + 1) / (float)e.z;
+    const float* a = path + (size_t)e.x * dim;
+    const float* b = path + (size_t)e.y * dim;
+    for (int d = 0; d < dim; ++d) out[(size_t)k * dim + d] = __fadd_rn(__fmul_rn(a[d], 1.0f - t), __fmul_rn(b[d], t));
+}
+
+__global__ void k_seg_reduce(const int4* pairs, int n_pairs, const uint8_t* hit, uint8_t* blocked) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_pairs) return;
+    const int4 p = pairs[e];
+    uint8_t b = 0;
+    for (int k = p.w; k < p.w + p.z - 1 && !b; ++k) b = hit[k];
+    blocked[e] = b;
+}
+
+// Growable device allocation.
+struct Grow {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, std::max(bytes, (size_t)256));
+        if (e == hipSuccess) cap = std::max(bytes, (size_t)256);
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+ppl_status hip_fail(hipError_t e, const char* what) {
+    static thread_local char buf[256];
+    snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    return ppl::fail_msg(PPL_ERR_HIP, buf);
+}
+
+#define RR_TRY(expr)                                       \
+    do {                                                   \
+        hipError_t e_ = (expr);                            \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr);  \
+    } while (0)
+
+#define RR_ARG(cond, msg)                                          \
+    do {                                                           \
+        if (!(cond)) return ppl::fail_msg(PPL_ERR_INVALID, msg);  \
+    } while (0)
+
+uint64_t splitmix64_next(uint64_t& x) {
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// The 16 Halton bases of workgroup w (pRRTC.cu:955-966: workgroup 0 keeps
+// the primes in order, the others shuffle them).
+void halton_bases(uint64_t seed, int w, float* out16) {
+    static const float primes[16] = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59};
+    std::memcpy(out16, primes, sizeof primes);
+    if (w == 0) return;
+    uint64_t x = seed ^ ((uint64_t)w * 0xD1B54A32D192ED03ull);
+    for (int i = 15; i > 0; --i) {
+        const int j = (int)(splitmix64_next(x) % (uint64_t)(i + 1));
+        std::swap(out16[i], out16[j]);
+    }
+}
+
+struct Shortcut {
+    Grow d_path, d_pairs, d_states, d_hit, d_blocked;
+    void release() {
+        d_path.release();
+        d_pairs.release();
+        d_states.release();
+        d_hit.release();
+        d_blocked.release();
+    }
+};
+
+// Shortcut a path given on the host (shortcut.cuh:190-274). out may alias nothing.
+ppl_status shortcut_run(ppl_robot robot, const ppl_scene_view* scene, const float* path, int n, int dim, float vsl,
+                        float margin, std::vector<float>& out, Shortcut& ws, hipStream_t s) {
+    out.assign(path, path + (size_t)n * dim);
+    if (n <= 2) return PPL_OK;
+    std::vector<int4> pairs;
+    pairs.reserve((size_t)(n - 2) * (n - 1) / 2);
+    long long n_states = 0;
+    for (int i = 0; i < n - 2; ++i)
+        for (int j = i + 2; j < n; ++j) {
+            float len2 = 0.0f;
+            for (int d = 0; d < dim; ++d) {
+                const float df = path[(size_t)i * dim + d] - path[(size_t)j * dim + d];
+                len2 += df * df;
+            }
+            const float total_f = std::ceil(std::sqrt(len2) / vsl);
+            if (!(total_f < 2.0e9f)) return ppl::fail_msg(PPL_ERR_CAPACITY, "shortcut edge has too many states");
+            const int total = std::max(1, (int)total_f);
+            pairs.push_back(make_int4(i, j, total, (int)n_states));
+            n_states += total - 1;
+            if (n_states > kMaxShortcutStates) return ppl::fail_msg(PPL_ERR_CAPACITY, "shortcut needs more than 2^26 states");
+        }
+    const int n_pairs = (int)pairs.size();
+    std::vector<uint8_t> blocked(n_pairs, 0);
+    if (n_states > 0) {
+        RR_TRY(ws.d_path.ensure((size_t)n * dim * sizeof(float)));
+        RR_TRY(ws.d_pairs.ensure((size_t)n_pairs * sizeof(int4)));
+        RR_TRY(ws.d_states.ensure((size_t)n_states * dim * sizeof(float)));
+        RR_TRY(ws.d_hit.ensure((size_t)n_states));
+        RR_TRY(ws.d_blocked.ensure((size_t)n_pairs));
+        RR_TRY(hipMemcpyAsync(ws.d_path.p, path, (size_t)n * dim * sizeof(float), hipMemcpyHostToDevice, s));
+        RR_TRY(hipMemcpyAsync(ws.d_pairs.p, pairs.data(), (size_t)n_pairs * sizeof(int4), hipMemcpyHostToDevice, s));
+        const int ns = (int)n_states;
+        hipLaunchKernelGGL(k_seg_expand, dim3((ns + 255) / 256), dim3(256), 0, s, ws.d_path.as<float>(), dim,
+                           ws.d_pairs.as<int4>(), n_pairs, ns, ws.d_states.as<float>());
+        RR_TRY(hipGetLastError());
+        ppl_status st = ppl_mitstar_batch_cc(robot, scene, ws.d_states.as<float>(), ns, margin, ws.d_hit.as<uint8_t>(),
+                                             nullptr, 0.0f, (ppl_stream)s);
+        if (st != PPL_OK) return st;
+        hipLaunchKernelGGL(k_seg_reduce, dim3((n_pairs + 255) / 256), dim3(256), 0, s, ws.d_pairs.as<int4>(), n_pairs,
+                           ws.d_hit.as<uint8_t>(), ws.d_blocked.as<uint8_t>());
+        RR_TRY(hipGetLastError());
+        RR_TRY(hipMemcpyAsync(blocked.data(), ws.d_blocked.p, (size_t)n_pairs, hipMemcpyDeviceToHost, s));
+        RR_TRY(hipStreamSynchronize(s));
+    }
+    // greedy: from waypoint i jump to the farthest j > i + 1 with a free skip edge, else i + 1
+    auto pair_index = [n](int i, int j) { return i * (2 * n - i - 3) / 2 + (j - i - 2); };
+    out.clear();
+    out.insert(out.end(), path, path + dim);
+    int i = 0;
+    while (i < n - 1) {
+        int far = i + 1;
+        for (int j = n - 1; j > i + 1; --j)
+            if (!blocked[pair_index(i, j)]) {
+                far = j;
+                break;
+            }
+        out.insert(out.end(), path + (size_t)far * dim, path + (size_t)(far + 1) * dim);
+        i = far;
+    }
+    return PPL_OK;
+}
+
+}  // namespace
+
+struct ppl_rrtc_buffers_s {
+    int dim = 0, max_samples = 0, blocks = 0, gran = 0;
+    Grow nodes, parents, radii, ctl, bases, path;
+    RrtcCtl* h_ctl = nullptr;   // pinned
+    Shortcut sc;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    void release() {
+        nodes.release();
+        parents.release();
+        radii.release();
+        ctl.release();
+        bases.release();
+        path.release();
+        sc.release();
+        if (h_ctl) (void)hipHostFree(h_ctl);
+        h_ctl = nullptr;
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        ev[0] = ev[1] = nullptr;
+    }
+};
+
+namespace {
+
+ppl_status rrtc_alloc(ppl_rrtc_buffers_s* b, int dim, int max_samples, int blocks, int gran) {
+    b->dim = dim;
+    b->max_samples = max_samples;
+    b->blocks = blocks;
+    b->gran = gran;
+    const size_t ms = (size_t)max_samples;
+    RR_TRY(b->nodes.ensure(2 * ms * dim * sizeof(float)));
+    RR_TRY(b->parents.ensure(2 * ms * sizeof(int)));
+    RR_TRY(b->radii.ensure(2 * ms * sizeof(float)));
+    RR_TRY(b->ctl.ensure(sizeof(RrtcCtl)));
+    RR_TRY(b->bases.ensure((size_t)blocks * kMaxDim * sizeof(float)));
+    RR_TRY(b->path.ensure((size_t)2 * kPathCap * dim * sizeof(float)));
+    RR_TRY(hipHostMalloc((void**)&b->h_ctl, sizeof(RrtcCtl), hipHostMallocDefault));
+    RR_TRY(hipEventCreate(&b->ev[0]));
+    RR_TRY(hipEventCreate(&b->ev[1]));
+    return PPL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ppl_rrtc_default_settings(ppl_rrtc_settings* s) {
+    if (!s) return;
+    s->max_samples = 1000000;
+    s->max_iters = 1000000;
+    s->time_limit_ms = 4000.0f;
+    s->num_new_configs = 600;
+    s->granularity = 16;
+    s->range = 0.5f;
+    s->balance = 1;
+    s->tree_ratio = 0.5f;
+    s->dynamic_domain = 1;
+    s->dd_alpha = 0.0001f;
+    s->dd_radius = 4.0f;
+    s->dd_min_radius = 1.0f;
+    s->shortcut_path = 0;
+    s->valid_segment_length = 0.025f;
+    s->collision_margin = 0.0f;
+    s->shortcut_collision_margin = 0.0005f;
+    s->seed = 1;
+}
+
+ppl_status ppl_rrtc_halton_bases(unsigned long long seed, int n_workgroups, int n_dof, float* h_bases) {
+    RR_ARG(h_bases && n_workgroups >= 0 && n_dof >= 1 && n_dof <= kMaxDim, "bad Halton base request");
+    float b[kMaxDim];
+    for (int w = 0; w < n_workgroups; ++w) {
+        halton_bases(seed, w, b);
+        std::memcpy(h_bases + (size_t)w * n_dof, b, (size_t)n_dof * sizeof(float));
+    }
+    return PPL_OK;
+}
+
+ppl_status ppl_rrtc_buffers_create(int n_dof, int max_samples, int num_new_configs, int granularity,
+                                   ppl_rrtc_buffers* out) {
+    RR_ARG(out, "NULL output");
+    *out = nullptr;
+    RR_ARG(n_dof >= 1 && n_dof <= kMaxDim, "n_dof must be 1..16");
+    RR_ARG(max_samples >= 2 && num_new_configs >= 1 && num_new_configs <= 65535, "bad buffer sizes");
+    RR_ARG(granularity >= 1 && granularity <= kMaxGran, "granularity must be 1..64");
+    auto* b = new (std::nothrow) ppl_rrtc_buffers_s;
+    if (!b) return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
+    ppl_status st = rrtc_alloc(b, n_dof, max_samples, num_new_configs, granularity);
+    if (st != PPL_OK) {
+        b->release();
+        delete b;
+        return st;
+    }
+    *out = b;
+    return PPL_OK;
+}
+
+ppl_status ppl_rrtc_buffers_destroy(ppl_rrtc_buffers b) {
+    if (!b) return PPL_OK;
+    (void)hipDeviceSynchronize();
+    b->release();
+    delete b;
+    return PPL_OK;
+}
+
+ppl_status ppl_shortcut_path(ppl_robot robot, const ppl_scene_view* scene, const float* h_path, int n,
+                             float valid_segment_length, float collision_margin, float* h_out, int* n_out,
+                             ppl_stream stream) {
+    RR_ARG(robot && scene && h_out && n_out && (h_path || n == 0) && n >= 0, "NULL argument");
+    RR_ARG(valid_segment_length > 0.0f, "valid_segment_length must be > 0");
+    ppl_robot_info info{};
+    ppl_status st = ppl_robot_get_info(robot, &info);
+    if (st != PPL_OK) return st;
+    Shortcut ws;
+    std::vector<float> out;
+    st = shortcut_run(robot, scene, h_path, n, info.n_dof, valid_segment_length, collision_margin, out, ws,
+                      (hipStream_t)stream);
+    ws.release();
+    if (st != PPL_OK) return st;
+    std::memcpy(h_out, out.data(), out.size() * sizeof(float));
+    *n_out = (int)(out.size() / std::max(1, info.n_dof));
+    return PPL_OK;
+}
+
+ppl_status ppl_rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const float* h_start, const float* h_goals,
+                          int n_goals, const ppl_rrtc_settings* cfg, ppl_rrtc_buffers bufs, float* h_path,
+                          int max_waypoints, ppl_rrtc_result* res, ppl_stream stream) {
+    const auto wall0 = std::chrono::steady_clock::now();
+    RR_ARG(robot && scene && h_start && h_goals && cfg && res, "NULL argument");
+    RR_ARG(n_goals >= 1, "need at least one goal");
+    RR_ARG(max_waypoints >= 0 && (h_path || max_waypoints == 0), "NULL path buffer");
+    std::memset(res, 0, sizeof *res);
+    ppl_robot_device_view R{};
+    ppl_status st = ppl_robot_get_device_view(robot, &R);
+    if (st != PPL_OK) return st;
+    const int dim = R.n_dof;
+    RR_ARG(dim >= 1 && dim <= kMaxDim, "pRRTC needs 1..16 degrees of freedom (one Halton base each)");
+    RR_ARG(cfg->granularity >= 1 && cfg->granularity <= kMaxGran, "granularity must be 1..64");
+    RR_ARG(cfg->num_new_configs >= 1 && cfg->num_new_configs <= 65535, "num_new_configs must be 1..65535");
+    RR_ARG(cfg->range > 0.0f && std::isfinite(cfg->range), "range must be a positive number");
+    RR_ARG(cfg->max_samples > n_goals, "max_samples must exceed the number of goals");
+    RR_ARG(cfg->max_iters >= 1, "max_iters must be >= 1");
+    RR_ARG(cfg->balance >= 0 && cfg->balance <= 2, "balance must be 0, 1 or 2");
+    RR_ARG(!cfg->shortcut_path || cfg->valid_segment_length > 0.0f, "valid_segment_length must be > 0");
+    for (int i = 0; i < dim; ++i) RR_ARG(std::isfinite(h_start[i]), "start has a non-finite coordinate");
+    for (int i = 0; i < n_goals * dim; ++i) RR_ARG(std::isfinite(h_goals[i]), "a goal has a non-finite coordinate");
+    if ((st = ppl::scene_view_check(scene)) != PPL_OK) return st;
+    if (scene->esdf_interp != 0)
+        return ppl::fail_msg(PPL_ERR_UNSUPPORTED, "pRRTC checks use the nearest-voxel ESDF lookup (esdf_interp = 0)");
+    if (R.n_pairs > 0 && !ppl::robot_self_gate_implied(robot))
+        return ppl::fail_msg(PPL_ERR_UNSUPPORTED,
+                             "the approximate self pairs do not cover the exact ones: the gated self check would miss hits");
+    const int threads = std::max(64, (kG * cfg->granularity + 63) / 64 * 64);
+    const size_t lds = (size_t)(threads / kG) * 3 * (R.n_spheres + R.n_approx_spheres) * sizeof(float);
+    if (lds > kMaxDynLds) return ppl::fail_msg(PPL_ERR_CAPACITY, "too many spheres x granularity for the LDS staging");
+
+    ppl_rrtc_buffers_s local;
+    ppl_rrtc_buffers_s* b = bufs;
+    struct Guard {
+        ppl_rrtc_buffers_s* p;
+        ~Guard() {
+            if (p) p->release();
+        }
+    } guard{nullptr};
+    if (!b) {
+        b = &local;
+        guard.p = &local;
+        if ((st = rrtc_alloc(b, dim, cfg->max_samples, cfg->num_new_configs, cfg->granularity)) != PPL_OK) return st;
+    } else {
+        RR_ARG(b->dim == dim, "buffers were created for another n_dof");
+        RR_ARG(b->max_samples >= cfg->max_samples && b->blocks >= cfg->num_new_configs && b->gran >= cfg->granularity,
+               "buffers are smaller than the settings ask for");
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const size_t ms = (size_t)b->max_samples;
+    RrtcArgs A{};
+    A.R = R;
+    A.S = *scene;
+    for (int t = 0; t < 2; ++t) {
+        A.nodes[t] = b->nodes.as<float>() + t * ms * dim;
+        A.parents[t] = b->parents.as<int>() + t * ms;
+        A.radii[t] = b->radii.as<float>() + t * ms;
+    }
+    A.ctl = b->ctl.as<RrtcCtl>();
+    A.bases = b->bases.as<float>();
+    A.path = b->path.as<float>();
+    A.dim = dim;
+    A.max_samples = cfg->max_samples;
+    A.max_iters = cfg->max_iters;
+    A.granularity = cfg->granularity;
+    A.balance = cfg->balance;
+    A.dynamic_domain = cfg->dynamic_domain ? 1 : 0;
+    A.n_blocks = cfg->num_new_configs;
+    A.range = cfg->range;
+    A.tree_ratio = cfg->tree_ratio;
+    A.dd_alpha = cfg->dd_alpha;
+    A.dd_radius = cfg->dd_radius;
+    A.dd_min_radius = cfg->dd_min_radius;
+    A.margin = cfg->collision_margin;
+    A.time_ticks = 0;
+    if (cfg->time_limit_ms > 0.0f) {
+        int dev = 0, khz = 0;
+        RR_TRY(hipGetDevice(&dev));
+        RR_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        A.time_ticks = std::max(1ll, (long long)((double)cfg->time_limit_ms * (double)khz));
+    }
+    // trees: -9999 rows, unbounded radii, roots (pRRTC.cu:2070-2128)
+    const size_t fill_n = (size_t)cfg->max_samples * dim;
+    for (int t = 0; t < 2; ++t) {
+        hipLaunchKernelGGL(k_fill_f32, dim3(1024), dim3(256), 0, s, A.nodes[t], fill_n, kUnwritten);
+        hipLaunchKernelGGL(k_fill_f32, dim3(256), dim3(256), 0, s, A.radii[t], (size_t)cfg->max_samples, FLT_MAX);
+    }
+    RR_TRY(hipGetLastError());
+    std::vector<float> bases((size_t)cfg->num_new_configs * kMaxDim);
+    for (int w = 0; w < cfg->num_new_configs; ++w) halton_bases(cfg->seed, w, bases.data() + (size_t)w * kMaxDim);
+    std::vector<int> goal_parents(n_goals);
+    for (int i = 0; i < n_goals; ++i) goal_parents[i] = i;
+    const int zero = 0;
+    RrtcCtl* hc = b->h_ctl;
+    std::memset(hc, 0, sizeof *hc);
+    hc->free_idx[0] = hc->completed[0] = 1;
+    hc->free_idx[1] = hc->completed[1] = n_goals;
+    RR_TRY(hipMemcpyAsync(A.ctl, hc, sizeof *hc, hipMemcpyHostToDevice, s));
+    RR_TRY(hipMemcpyAsync(b->bases.p, bases.data(), bases.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    RR_TRY(hipMemcpyAsync(A.nodes[0], h_start, (size_t)dim * sizeof(float), hipMemcpyHostToDevice, s));
+    RR_TRY(hipMemcpyAsync(A.nodes[1], h_goals, (size_t)n_goals * dim * sizeof(float), hipMemcpyHostToDevice, s));
+    RR_TRY(hipMemcpyAsync(A.parents[0], &zero, sizeof(int), hipMemcpyHostToDevice, s));
+    RR_TRY(hipMemcpyAsync(A.parents[1], goal_parents.data(), (size_t)n_goals * sizeof(int), hipMemcpyHostToDevice, s));
+    RR_TRY(hipEventRecord(b->ev[0], s));
+    hipLaunchKernelGGL(k_rrtc, dim3(cfg->num_new_configs), dim3(threads), lds, s, A);
+    RR_TRY(hipGetLastError());
+    RR_TRY(hipEventRecord(b->ev[1], s));
+    hipLaunchKernelGGL(k_rrtc_path, dim3(1), dim3(64), 0, s, A);
+    RR_TRY(hipGetLastError());
+    RR_TRY(hipMemcpyAsync(hc, A.ctl, sizeof *hc, hipMemcpyDeviceToHost, s));
+    RR_TRY(hipStreamSynchronize(s));
+    float kms = 0.0f;
+    RR_TRY(hipEventElapsedTime(&kms, b->ev[0], b->ev[1]));
+    res->kernel_ms = kms;
+    res->start_tree_size = std::min(hc->free_idx[0], cfg->max_samples);
+    res->goal_tree_size = std::min(hc->free_idx[1], cfg->max_samples);
+    res->solved = hc->status == 1 && hc->path_ok;
+    res->iters = res->solved ? hc->win_iter : hc->max_iter;
+    if (res->solved) {
+        const int n0 = hc->path_n[0], n1 = hc->path_n[1];
+        std::vector<float> p0((size_t)n0 * dim), p1((size_t)n1 * dim);
+        if (n0) RR_TRY(hipMemcpyAsync(p0.data(), A.path, p0.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+        if (n1)
+            RR_TRY(hipMemcpyAsync(p1.data(), A.path + (size_t)kPathCap * dim, p1.size() * sizeof(float),
+                                  hipMemcpyDeviceToHost, s));
+        RR_TRY(hipStreamSynchronize(s));
+        // goal -> start, as the reference assembles it (pRRTC.cu:2167-2186)
+        std::vector<float> g2s;
+        g2s.reserve((size_t)(n0 + n1 + 2) * dim);
+        const float* goal = h_goals + (size_t)hc->reached_goal * dim;
+        g2s.insert(g2s.end(), goal, goal + dim);
+        for (int i = n1 - 1; i >= 0; --i) g2s.insert(g2s.end(), p1.begin() + (size_t)i * dim, p1.begin() + (size_t)(i + 1) * dim);
+        for (int i = 0; i < n0; ++i) g2s.insert(g2s.end(), p0.begin() + (size_t)i * dim, p0.begin() + (size_t)(i + 1) * dim);
+        g2s.insert(g2s.end(), h_start, h_start + dim);
+        res->cost = hc->cost;
+        res->path_length = n0 + n1;
+        res->reached_goal = hc->reached_goal;
+        int nw = n0 + n1 + 2;
+        if (cfg->shortcut_path && nw > 2) {
+            std::vector<float> sc;
+            st = shortcut_run(robot, scene, g2s.data(), nw, dim, cfg->valid_segment_length,
+                              cfg->collision_margin + cfg->shortcut_collision_margin, sc, b->sc, s);
+            if (st != PPL_OK) return st;
+            g2s.swap(sc);
+            nw = (int)(g2s.size() / dim);
+        }
+        res->n_waypoints = nw;
+        if (nw > max_waypoints) {
+            res->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+            return ppl::fail_msg(PPL_ERR_CAPACITY, "the solution has more waypoints than max_waypoints");
+        }
+        for (int i = 0; i < nw; ++i)   // start -> goal (python/bindings.cpp:260-272)
+            std::memcpy(h_path + (size_t)i * dim, g2s.data() + (size_t)(nw - 1 - i) * dim, (size_t)dim * sizeof(float));
+    }
+    res->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    return PPL_OK;
+}
+
+}  // extern "C"

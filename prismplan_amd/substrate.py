@@ -437,6 +437,76 @@ def stomp_solve(robot: Robot, scene: DeviceScene, start, goal, num_timesteps=51,
                   settings if settings is not None else stomp_settings(), time_limit_ms)
 
 
+def rrtc_settings(**kw) -> _ppl.RrtcSettings:
+    """pRRTC settings: the reference defaults (pRRTC_settings.hh), overridden by keyword."""
+    s = _ppl.RrtcSettings()
+    _ppl.lib().ppl_rrtc_default_settings(C.byref(s))
+    for k, v in kw.items():
+        if not hasattr(s, k):
+            raise AttributeError(f"unknown pRRTC setting {k!r}")
+        setattr(s, k, v)
+    return s
+
+
+def rrtc_halton_bases(seed: int, n_workgroups: int, n_dof: int) -> np.ndarray:
+    """Halton bases [n_workgroups, n_dof] each pRRTC workgroup samples with."""
+    out = np.empty((n_workgroups, n_dof), np.float32)
+    call("ppl_rrtc_halton_bases", C.c_ulonglong(seed), n_workgroups, n_dof, _p(out, _f32p))
+    return out
+
+
+class RrtcBuffers:
+    """Reusable pRRTC trees (ppl_rrtc_buffers): SolverBuffers' role."""
+
+    def __init__(self, n_dof: int, max_samples=1000000, num_new_configs=600, granularity=16):
+        self.handle = C.c_void_p()
+        call("ppl_rrtc_buffers_create", int(n_dof), int(max_samples), int(num_new_configs), int(granularity),
+             C.byref(self.handle))
+        self.n_dof, self.max_samples = int(n_dof), int(max_samples)
+
+    def close(self):
+        if self.handle:
+            _ppl.lib().ppl_rrtc_buffers_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rrtc_solve(robot: Robot, scene: DeviceScene, start, goals, settings: Optional[_ppl.RrtcSettings] = None,
+               bufs: Optional[RrtcBuffers] = None, max_waypoints=16384) -> dict:
+    """pRRTC (ppl_rrtc_solve): path [N, D] start -> goal when solved."""
+    D = robot.n_dof
+    a = np.ascontiguousarray(start, np.float32).reshape(-1)
+    g = np.ascontiguousarray(goals, np.float32).reshape(-1, D)
+    if a.shape != (D,):
+        raise ValueError(f"start must have shape ({D},)")
+    st = settings if settings is not None else rrtc_settings()
+    path = np.empty((max_waypoints, D), np.float32)
+    res = _ppl.RrtcResult()
+    v = scene.view()
+    call("ppl_rrtc_solve", robot.handle, C.byref(v), _p(a, _f32p), _p(g, _f32p), g.shape[0], C.byref(st),
+         bufs.handle if bufs is not None else None, _p(path, _f32p), max_waypoints, C.byref(res), None)
+    return {"solved": bool(res.solved), "path": path[:res.n_waypoints].copy() if res.solved else path[:0].copy(),
+            "iters": res.iters, "cost": res.cost, "start_tree_size": res.start_tree_size,
+            "goal_tree_size": res.goal_tree_size, "path_length": res.path_length, "reached_goal": res.reached_goal,
+            "wall_time_ms": res.wall_ms, "kernel_ms": res.kernel_ms}
+
+
+def shortcut_path(robot: Robot, scene: DeviceScene, path, valid_segment_length=0.025, collision_margin=0.0):
+    """Greedy shortcut of a host path [N, D] (ppl_shortcut_path)."""
+    q = _np(path, np.float32).reshape(-1, robot.n_dof)
+    out = np.empty_like(q)
+    n = C.c_int(0)
+    v = scene.view()
+    call("ppl_shortcut_path", robot.handle, C.byref(v), _p(q, _f32p), q.shape[0], C.c_float(valid_segment_length),
+         C.c_float(collision_margin), _p(out, _f32p), C.byref(n), None)
+    return out[:n.value].copy()
+
+
 def min_clearance_scene(robot: Robot, scene: DeviceScene, configs, collision_margin=0.0) -> np.ndarray:
     """Plain form: min ESDF clearance (+ margin) of host configurations."""
     q = _np(configs, np.float32).reshape(-1, robot.n_dof)

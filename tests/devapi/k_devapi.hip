@@ -44,9 +44,24 @@ hipError_t launch(const ppl_robot_device_view& R, const ppl_scene_view& S, const
     return hipGetLastError();
 }
 
+// One lane per dimension: n_steps samples of the incremental Halton generator.
+__global__ void k_halton(const float* bases, int dim, int n_steps, float* out) {
+    const int i = threadIdx.x;
+    if (i >= dim) return;
+    float n = 0.0f, d = 1.0f;
+    for (int k = 0; k < n_steps; ++k) out[(size_t)k * dim + i] = ppl::device::halton_step(bases[i], n, d);
+}
+
 }  // namespace
 
 extern "C" {
+
+// d_out [n_steps][dim]; dim <= 64.
+int ppl_devtest_halton(const float* d_bases, int dim, int n_steps, float* d_out, void* stream) {
+    if (!d_bases || !d_out || dim < 1 || dim > 64 || n_steps < 0) return 1;
+    hipLaunchKernelGGL(k_halton, dim3(1), dim3(64), 0, (hipStream_t)stream, d_bases, dim, n_steps, d_out);
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
 
 // Scratch: (ceil(n / (64 / G)) * (64 / G)) * 3 * (n_spheres + n_approx_spheres) floats.
 int ppl_devtest_run(const ppl_robot_device_view* R, const ppl_scene_view* S, const float* d_q, int n, int G,

@@ -1,0 +1,230 @@
+"""pRRTC (ppl_rrtc_solve) and the shortcut pass (ppl_shortcut_path) against
+the restatement in oracle/rrtc_oracle.py and the FP64 collision oracle.
+
+The planner is concurrent, so a solution is checked for what the reference's
+algorithm guarantees (planning/pRRTC.cu:1556-2010, 2167-2186): it runs from
+the start to one of the goals; every segment is at most `range` long; every
+segment is collision-free at the states pRRTC's edge checks cover (the
+points a + k (b - a) / granularity, k = 0..granularity, in the FP64 oracle
+with the planner's margin and the self check; states within 1e-5 m of
+contact are ambiguous and counted); `cost` sums the tree edges walked. The
+shortcut pass is deterministic given its input and must keep exactly the
+waypoints the oracle's greedy pass keeps (shortcut.cuh:183-274) unless a
+skip edge has an ambiguous state.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as PO
+import rrtc_oracle as RO
+from prismplan_amd import _ppl
+from prismplan_amd import substrate as S
+from prismplan_amd import workloads as W
+from prismplan_amd.planning_scene import HostScene
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+BAND = 1e-5
+DEVTEST = Path(__file__).resolve().parent / "devapi" / "_build" / "libppl_devtest.so"
+
+
+def banner(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def robots():
+    return {n: S.Robot(n) for n in ("panda", "fetch")}
+
+
+def line_blocked(o, a, b, margin, n=64):
+    t = np.linspace(0.0, 1.0, n)[:, None]
+    q = ((1 - t) * a[None, :] + t * b[None, :]).astype(np.float32)
+    hit, _ = o.check_collision(q, margin=margin, check_self=True)
+    return bool(hit.any())
+
+
+def problem(robot, seed, n_obbs=10, margin=0.0, n_goals=1):
+    """A box world with a free start and free goals whose straight line is blocked."""
+    r = robot.arrays
+    rng = np.random.default_rng(seed)
+    # Fetch's base and torso occupy x < 0.4 m near the floor: its boxes start further out
+    lo, hi = ((0.5, -0.7, 0.3), (1.1, 0.7, 1.3)) if robot.n_dof == 8 else ((0.25, -0.6, 0.0), (0.9, 0.6, 1.1))
+    scene = W.random_obbs(rng, n_obbs, lo=lo, hi=hi, half_lo=0.03, half_hi=0.12)
+    o = PO.Oracle(r, scene, nthreads=8)
+    q = W.uniform_configs(rng, r, 400)
+    hit, clr = o.check_collision(q, margin=margin, check_self=True)
+    free = q[~hit & (np.abs(clr) > 0.01)]
+    start = free[0]
+    goals = [g for g in free[1:] if line_blocked(o, start, g, margin)][:n_goals]
+    assert len(goals) == n_goals, "no blocked start-goal pair in this world"
+    return scene, o, start, np.stack(goals)
+
+
+def check_solution(o, res, start, goals, st, tag):
+    path = res["path"]
+    assert res["solved"], f"{tag}: not solved ({res})"
+    np.testing.assert_array_equal(path[0], start)
+    np.testing.assert_array_equal(path[-1], goals[res["reached_goal"]])
+    seg = np.linalg.norm(np.diff(path.astype(np.float64), axis=0), axis=1)
+    assert seg.max() <= st.range * (1 + 1e-5) + 1e-6, f"{tag}: segment {seg.max()} > range {st.range}"
+    states = np.concatenate([RO.segment_states(path[i], path[i + 1], st.granularity) for i in range(len(path) - 1)])
+    hit, clr = o.check_collision(states, margin=st.collision_margin, check_self=True)
+    amb = np.abs(clr) < BAND
+    bad = hit & ~amb
+    banner(f"{tag}: {len(path)} waypoints, cost {res['cost']:.4f}, iters {res['iters']}, trees "
+           f"{res['start_tree_size']}+{res['goal_tree_size']}, kernel {res['kernel_ms']:.2f} ms, "
+           f"{len(states)} states checked, {int(amb.sum())} ambiguous, {int(bad.sum())} colliding")
+    assert not bad.any(), f"{tag}: colliding path states"
+    # cost: every tree edge on the path; the one non-tree segment (the connection) has length ~0
+    assert abs(seg.sum() - res["cost"]) <= 1e-4 * max(1.0, seg.sum()) + 1e-5, (seg.sum(), res["cost"])
+    assert res["path_length"] == len(path) - 2
+    assert res["start_tree_size"] >= 1 and res["goal_tree_size"] >= len(goals)
+    assert res["iters"] >= 1
+
+
+def test_halton_device_matches_oracle():
+    lib = C.CDLL(str(DEVTEST))
+    lib.ppl_devtest_halton.restype = C.c_int
+    lib.ppl_devtest_halton.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    n = 4000
+    bases = torch.tensor(RO.PRIMES, dtype=torch.float32, device=DEV)
+    out = torch.empty((n, 16), dtype=torch.float32, device=DEV)
+    banner(f"Halton steps on the device: 16 bases x {n}")
+    assert lib.ppl_devtest_halton(bases.data_ptr(), 16, n, out.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), RO.halton_sequence(RO.PRIMES, n))
+
+
+@pytest.mark.parametrize("name,seed", [("panda", 1), ("panda", 2), ("fetch", 3)])
+def test_rrtc_solution_is_valid(robots, name, seed):
+    rob = robots[name]
+    scene, o, start, goals = problem(rob, 100 + seed, margin=0.0)
+    ds = S.DeviceScene.from_host(scene)
+    st = S.rrtc_settings(num_new_configs=256, max_samples=200000, time_limit_ms=3000.0, seed=seed)
+    banner(f"pRRTC {name} seed {seed}: 256 workgroups, granularity 16, range 0.5")
+    res = S.rrtc_solve(rob, ds, start, goals, st)
+    check_solution(o, res, start, goals, st, f"{name}/{seed}")
+
+
+@pytest.mark.parametrize("balance,dd,gran,margin", [(0, 0, 16, 0.0), (2, 1, 8, 0.01), (1, 1, 32, 0.005)])
+def test_rrtc_modes_and_goals(robots, balance, dd, gran, margin):
+    rob = robots["panda"]
+    scene, o, start, goals = problem(rob, 200 + balance, margin=margin, n_goals=3)
+    ds = S.DeviceScene.from_host(scene)
+    st = S.rrtc_settings(num_new_configs=192, granularity=gran, balance=balance,
+                         tree_ratio=1.0 if balance == 2 else 0.5, dynamic_domain=dd, collision_margin=margin,
+                         max_samples=200000, time_limit_ms=3000.0, range=0.4, seed=9)
+    banner(f"pRRTC panda balance={balance} dd={dd} granularity={gran} margin={margin}, 3 goals")
+    res = S.rrtc_solve(rob, ds, start, goals, st)
+    check_solution(o, res, start, goals, st, f"balance{balance}")
+
+
+def test_rrtc_limits_and_buffers(robots):
+    rob = robots["panda"]
+    scene, o, start, goals = problem(rob, 300)
+    ds = S.DeviceScene.from_host(scene)
+    # a full tree stops the run without a solution
+    st = S.rrtc_settings(num_new_configs=64, max_samples=4, time_limit_ms=2000.0)
+    banner("pRRTC max_samples=4 on a blocked problem")
+    res = S.rrtc_solve(rob, ds, start, goals, st)
+    assert not res["solved"] and res["path"].shape == (0, 7)
+    assert res["start_tree_size"] <= 4 and res["goal_tree_size"] <= 4
+    # the iteration cap
+    st = S.rrtc_settings(num_new_configs=32, max_iters=2, time_limit_ms=0.0)
+    res = S.rrtc_solve(rob, ds, start, goals, st)
+    banner(f"pRRTC max_iters=2: solved={res['solved']} iters={res['iters']}")
+    assert res["iters"] <= 3
+    # the device-clock time limit
+    st = S.rrtc_settings(num_new_configs=64, time_limit_ms=1.0, max_samples=100000)
+    res = S.rrtc_solve(rob, ds, start, goals, st)
+    banner(f"pRRTC time_limit_ms=1: solved={res['solved']} kernel {res['kernel_ms']:.2f} ms")
+    assert res["kernel_ms"] < 500.0
+    # reusable buffers, three solves in a row; smaller buffers are refused
+    bufs = S.RrtcBuffers(7, max_samples=100000, num_new_configs=128, granularity=16)
+    st = S.rrtc_settings(num_new_configs=128, max_samples=100000, time_limit_ms=3000.0)
+    for k in range(3):
+        st.seed = 11 + k
+        res = S.rrtc_solve(rob, ds, start, goals, st, bufs=bufs)
+        check_solution(o, res, start, goals, st, f"buffers/{k}")
+    st.max_samples = 200000
+    with pytest.raises(_ppl.PplError) as err:
+        S.rrtc_solve(rob, ds, start, goals, st, bufs=bufs)
+    assert err.value.status == _ppl.PPL_ERR_INVALID
+    bufs.close()
+    # the planner kernels read the nearest-voxel ESDF only
+    params, pose = W.esdf_grid(center=(0.5, 0.0, 0.5), n=16, voxel=0.05)
+    tri = HostScene(voxel_params=params, voxel_pose=pose, voxel_enable=np.ones(1, np.uint8),
+                    voxel_features=np.zeros((1, 16 ** 3), np.float32), esdf_interp=1)
+    with pytest.raises(_ppl.PplError) as err:
+        S.rrtc_solve(rob, S.DeviceScene.from_host(tri), start, goals, S.rrtc_settings(num_new_configs=8))
+    assert err.value.status == _ppl.PPL_ERR_UNSUPPORTED
+
+
+def shortcut_expected(o, path, vsl, margin):
+    free, amb_pairs = {}, 0
+    for i, j, states in RO.shortcut_pairs(path, vsl):
+        if len(states) == 0:
+            free[(i, j)] = True
+            continue
+        hit, clr = o.check_collision(states, margin=margin, check_self=True)
+        free[(i, j)] = not hit.any()
+        if (np.abs(clr) < BAND).any():
+            amb_pairs += 1
+    return RO.shortcut_greedy(len(path), free), amb_pairs
+
+
+@pytest.mark.parametrize("name", ["panda", "fetch"])
+def test_shortcut_matches_oracle(robots, name):
+    rob = robots[name]
+    scene, o, start, goals = problem(rob, 400 if name == "panda" else 401)
+    ds = S.DeviceScene.from_host(scene)
+    st = S.rrtc_settings(num_new_configs=128, max_samples=100000, time_limit_ms=3000.0, range=0.25, seed=5)
+    res = S.rrtc_solve(rob, ds, start, goals, st)
+    assert res["solved"]
+    path = res["path"]
+    for vsl, margin in ((0.025, 0.0005), (0.1, 0.02)):
+        out = S.shortcut_path(rob, ds, path, valid_segment_length=vsl, collision_margin=margin)
+        keep, amb = shortcut_expected(o, path, vsl, margin)
+        banner(f"shortcut {name}: {len(path)} -> {len(out)} waypoints (oracle {len(keep)}), vsl={vsl} "
+               f"margin={margin}, {amb} ambiguous skip edges")
+        np.testing.assert_array_equal(out[0], path[0])
+        np.testing.assert_array_equal(out[-1], path[-1])
+        same = out.shape == path[keep].shape and np.array_equal(out, path[keep])
+        assert same or amb > 0, "shortcut differs from the oracle's greedy pass without an ambiguous skip edge"
+    # the solve's own shortcut pass works from the goal end (pRRTC.cu:2190-2200)
+    st.shortcut_path = 1
+    res2 = S.rrtc_solve(rob, ds, start, goals, st)
+    assert res2["solved"]
+    p2 = res2["path"]
+    np.testing.assert_array_equal(p2[0], start)
+    np.testing.assert_array_equal(p2[-1], goals[res2["reached_goal"]])
+    m = st.collision_margin + st.shortcut_collision_margin
+    bad = 0
+    for a, b in zip(p2[:-1], p2[1:]):
+        states = RO.segment_interior(a, b, st.valid_segment_length)
+        if len(states):
+            hit, clr = o.check_collision(states, margin=m, check_self=True)
+            bad += int((hit & (np.abs(clr) >= BAND)).sum())
+    banner(f"solve with shortcut_path: {res2['path_length'] + 2} -> {len(p2)} waypoints, "
+           f"{bad} colliding interior states")
+    assert len(p2) <= res2["path_length"] + 2
+    assert bad == 0
+
+
+def test_shortcut_edge_cases(robots):
+    rob = robots["panda"]
+    ds = S.DeviceScene.from_host(HostScene())
+    q = W.uniform_configs(np.random.default_rng(5), rob.arrays, 6)
+    np.testing.assert_array_equal(S.shortcut_path(rob, ds, q[:2]), q[:2])
+    np.testing.assert_array_equal(S.shortcut_path(rob, ds, q[:1]), q[:1])
+    # no obstacles: every skip edge that is self-collision free is taken
+    o = PO.Oracle(rob.arrays, HostScene(), nthreads=8)
+    keep, amb = shortcut_expected(o, q, 0.05, 0.0)
+    out = S.shortcut_path(rob, ds, q, valid_segment_length=0.05)
+    same = out.shape == q[keep].shape and np.array_equal(out, q[keep])
+    assert same or amb > 0
