@@ -10,13 +10,15 @@ known hit). value = states checked per second.
 Also measured (extra fields): first-hit early-exit mode, plain-vs-batched
 any-collision (64 requests of 4,096 configs), plain-vs-batched edges, the
 fused CHOMP adjoint vs finite-difference gradients, STOMP rollout costs
-through a hipGraph, and the Baxter ESDF adjoint; the FP32 CPU restatement
+through a hipGraph, the STOMP / CHOMP optimisers, MIT*'s collision calls,
+pRRTC solves on the MotionBenchMaker sample problems, and the Baxter ESDF
+adjoint; the FP32 CPU restatement
 timed on this host's cores; parity of the timed outputs vs the FP64 oracle.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-extras] [--legs a,b,...] [--no-cpu]
 
 --legs picks the extra measurements (skip, any, edges_plain, chomp, stomp,
-chomp_opt, stomp_opt, mitstar, esdf, copy); the headline always runs. --no-cpu drops the CPU baseline leg
+chomp_opt, stomp_opt, mitstar, rrtc, esdf, copy); the headline always runs. --no-cpu drops the CPU baseline leg
 (for profiler runs, whose JSON line then carries cpu_baseline null).
 """
 from __future__ import annotations
@@ -87,7 +89,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
-    ap.add_argument("--legs", default="skip,any,edges_plain,chomp,stomp,chomp_opt,stomp_opt,mitstar,esdf")
+    ap.add_argument("--legs", default="skip,any,edges_plain,chomp,stomp,chomp_opt,stomp_opt,mitstar,rrtc,esdf")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     legs = set(x for x in args.legs.split(",") if x)
@@ -153,6 +155,7 @@ def main():
                 ("chomp_opt", "chomp_optimizer", lambda: bench_chomp_opt(panda, S, W, torch, dev, args)),
                 ("stomp_opt", "stomp_optimizer", lambda: bench_stomp_opt(panda, S, W, torch, dev, args)),
                 ("mitstar", "mitstar", lambda: bench_mitstar(panda, S, W, torch, dev, args)),
+                ("rrtc", "prrtc", lambda: bench_rrtc(panda, S, W, torch, dev, args)),
                 ("esdf", "esdf", lambda: bench_esdf(S, W, torch, dev, args)),
                 ("copy", "copy", lambda: bench_copy(torch, dev, args))):
             if key not in legs:
@@ -428,6 +431,37 @@ def bench_mitstar(panda, S, W, torch, dev, args):
             "full_cc_edges_per_s": 16384 / wall_f, "full_cc_ms": wall_f * 1e3,
             "full_cc_free_edges": int((cc == 0).sum().item()), "full_cc_verified_midpoints": mids,
             "sparse_cc_edges_per_s": 16384 / wall_s, "sparse_cc_ms": wall_s * 1e3}
+
+
+def bench_rrtc(panda, S, W, torch, dev, args):
+    """pRRTC solves on the bundled MotionBenchMaker sample problems (Panda,
+    box and cage scenes, primitives as containing OBBs), five seeds each,
+    the reference's default settings (600 workgroups, granularity 16, range
+    0.5, dynamic domain, balance 1) with reusable trees; wall and kernel time
+    per solve."""
+    import numpy as np
+    from prismplan_amd import problems as P
+    from prismplan_amd.planning_scene import HostScene
+    probs = P.load_vamp_problems(str(ROOT / "tests" / "golden" / "sample_mbm_problems.json"))
+    bufs = S.RrtcBuffers(7, max_samples=1000000, num_new_configs=600, granularity=16)
+    walls, kernels, iters, wps, solved = [], [], [], [], 0
+    for p in probs:
+        obbs = P.obstacles_to_obbs(p.obstacles)
+        ds = S.DeviceScene.from_host(HostScene.from_boxes([o["dims"] for o in obbs], [o["position"] for o in obbs],
+                                                          [o["quaternion"] for o in obbs]))
+        S.rrtc_solve(panda, ds, p.start, p.goal[None], S.rrtc_settings(seed=99), bufs=bufs)   # warm-up
+        for seed in range(5):
+            r = S.rrtc_solve(panda, ds, p.start, p.goal[None], S.rrtc_settings(seed=seed + 1), bufs=bufs)
+            solved += int(r["solved"])
+            walls.append(r["wall_time_ms"])
+            kernels.append(r["kernel_ms"])
+            iters.append(r["iters"])
+            wps.append(len(r["path"]))
+    bufs.close()
+    return {"problems": len(probs), "solves": len(walls), "solved": solved,
+            "wall_ms_median": float(np.median(walls)), "wall_ms_max": float(np.max(walls)),
+            "kernel_ms_median": float(np.median(kernels)), "kernel_ms_mean": float(np.mean(kernels)),
+            "iters_median": float(np.median(iters)), "waypoints_median": float(np.median(wps))}
 
 
 def bench_stomp(S, W, torch, dev, args):

@@ -4,7 +4,9 @@
 // (python/bindings.cpp:118-222 RobotModelBuilder, 363-402 and 992-1006
 // SceneCollisionDataBuilder, 493-530 and 1033-1037 check_collision_scene,
 // 764-902 and 1191-1282 STOMP / CHOMP settings, results, buffers and
-// stomp_solve_scene / chomp_solve_scene):
+// stomp_solve_scene / chomp_solve_scene, 225-233, 405-437, 910-938 and
+// 1023-1027, 1163-1176 pRRTC's Settings, PlannerResult, SolverBuffers and
+// solve_scene):
 // the same class and field names, int64 device-pointer fields filled from
 // torch.Tensor.data_ptr(), and shape errors raised as RuntimeError. Host C++
 // only: every device operation goes through libppl.so (include/ppl/ppl.h).
@@ -345,6 +347,128 @@ STOMPResult stomp_solve_scene(F32Array start, F32Array goal, const SceneCollisio
     return solve_scene<STOMPResult>(start, goal, robot, st.num_timesteps, solve, "stomp_solve_scene");
 }
 
+// pRRTC_settings (planning/pRRTC_settings.hh:3-32, python/bindings.cpp:919-938):
+// the same field names and defaults; `seed` picks the Halton base shuffle.
+struct Settings {
+    int max_samples = 1000000;
+    int max_iters = 1000000;
+    float time_limit_ms = 4000.0f;
+    int gpu_clock_rate_khz = 0;   // kept for scripts that set it; the device wall clock has a fixed rate
+    int num_new_configs = 600;
+    int granularity = 16;
+    float range = 0.5f;
+    int balance = 1;
+    float tree_ratio = 0.5f;
+    bool dynamic_domain = true;
+    float dd_alpha = 0.0001f;
+    float dd_radius = 4.0f;
+    float dd_min_radius = 1.0f;
+    bool enable_mesh_collision = false;
+    bool shortcut_path = false;
+    float valid_segment_length = 0.025f;
+    float collision_margin = 0.0f;
+    float shortcut_collision_margin = 0.0005f;
+    unsigned long long seed = 1;
+};
+
+// PlannerResult (python/bindings.cpp:225-233, 910-916): path (N, n_dof) start -> goal.
+struct PlannerResult {
+    bool solved = false;
+    py::array_t<float> path;
+    int iters = 0;
+    float cost = 0.0f;
+    double wall_time_ms = 0.0;
+    int start_tree_size = 0, goal_tree_size = 0;
+};
+
+// SolverBuffers (planning/solver_buffers.hh:15-54, python/bindings.cpp:1163-1176).
+struct SolverBuffers {
+    ppl_rrtc_buffers h = nullptr;
+    int max_samples = 0, n_dof = 0, num_new_configs = 0, granularity = 0;
+    bool owns_memory() const { return h != nullptr; }
+    void destroy() {
+        if (h) ppl_rrtc_buffers_destroy(h);
+        h = nullptr;
+    }
+    ~SolverBuffers() { destroy(); }
+    static SolverBuffers* create(int max_samples, int n_dof, int num_new_configs, int granularity, int n_joints,
+                                 bool enable_mesh) {
+        (void)n_joints;
+        if (enable_mesh) throw std::runtime_error("SolverBuffers.create: mesh collision is not supported");
+        auto* b = new SolverBuffers;
+        const ppl_status st = ppl_rrtc_buffers_create(n_dof, max_samples, num_new_configs, granularity, &b->h);
+        if (st != PPL_OK) {
+            delete b;
+            check(st, "SolverBuffers.create");
+        }
+        b->max_samples = max_samples;
+        b->n_dof = n_dof;
+        b->num_new_configs = num_new_configs;
+        b->granularity = granularity;
+        return b;
+    }
+};
+
+// solve_scene (python/bindings.cpp:405-437): pRRTC from start to any of goals [G, n_dof].
+PlannerResult solve_rrtc_scene(F32Array start, F32Array goals, const SceneCollisionDataBuilder& scene,
+                               const Settings& st, RobotModelBuilder& robot, py::object bufs_obj) {
+    if (st.enable_mesh_collision) throw std::runtime_error("solve_scene: mesh collision is not supported");
+    robot.build();
+    const int D = robot.n_dof;
+    if (start.ndim() != 1 || (int)start.shape(0) != D) throw std::runtime_error("start size mismatch with robot n_dof");
+    if (goals.ndim() != 2 || (int)goals.shape(1) != D) throw std::runtime_error("goals columns mismatch with robot n_dof");
+    const int G = (int)goals.shape(0);
+    if (G < 1) throw std::runtime_error("goals must hold at least one configuration");
+    ppl_rrtc_settings cs{};
+    cs.max_samples = st.max_samples;
+    cs.max_iters = st.max_iters;
+    cs.time_limit_ms = st.time_limit_ms;
+    cs.num_new_configs = st.num_new_configs;
+    cs.granularity = st.granularity;
+    cs.range = st.range;
+    cs.balance = st.balance;
+    cs.tree_ratio = st.tree_ratio;
+    cs.dynamic_domain = st.dynamic_domain ? 1 : 0;
+    cs.dd_alpha = st.dd_alpha;
+    cs.dd_radius = st.dd_radius;
+    cs.dd_min_radius = st.dd_min_radius;
+    cs.shortcut_path = st.shortcut_path ? 1 : 0;
+    cs.valid_segment_length = st.valid_segment_length;
+    cs.collision_margin = st.collision_margin;
+    cs.shortcut_collision_margin = st.shortcut_collision_margin;
+    cs.seed = st.seed;
+    const ppl_scene_view v = scene.view();
+    ppl_rrtc_buffers h = bufs_obj.is_none() ? nullptr : bufs_obj.cast<SolverBuffers*>()->h;
+    std::vector<float> path;
+    ppl_rrtc_result r{};
+    {
+        py::gil_scoped_release nogil;
+        int cap = 1024;
+        for (;;) {   // a longer solution reports the rows it needs; solve again with room for them
+            path.assign((size_t)cap * D, 0.0f);
+            const ppl_status s =
+                ppl_rrtc_solve(robot.robot, &v, start.data(), goals.data(), G, &cs, h, path.data(), cap, &r, nullptr);
+            if (s == PPL_ERR_CAPACITY && r.n_waypoints > cap) {
+                cap = r.n_waypoints;
+                continue;
+            }
+            check(s, "solve_scene");
+            break;
+        }
+    }
+    PlannerResult res;
+    res.solved = r.solved != 0;
+    const int n = res.solved ? r.n_waypoints : 0;
+    res.path = py::array_t<float>({n, D});
+    if (n) std::copy(path.begin(), path.begin() + (size_t)n * D, res.path.mutable_data());
+    res.iters = r.iters;
+    res.cost = r.cost;
+    res.wall_time_ms = r.wall_ms;
+    res.start_tree_size = r.start_tree_size;
+    res.goal_tree_size = r.goal_tree_size;
+    return res;
+}
+
 template <typename R>
 void bind_result(py::module_& m, const char* name) {
     py::class_<R>(m, name)
@@ -465,6 +589,46 @@ PYBIND11_MODULE(_prrtc, m) {
         .def_readwrite("collision_check_step", &STOMPSettings::collision_check_step)
         .def_readwrite("noise_scale", &STOMPSettings::noise_scale)
         .def_readwrite("seed", &STOMPSettings::seed);
+    py::class_<Settings>(m, "Settings")
+        .def(py::init<>())
+        .def_readwrite("max_samples", &Settings::max_samples)
+        .def_readwrite("max_iters", &Settings::max_iters)
+        .def_readwrite("num_new_configs", &Settings::num_new_configs)
+        .def_readwrite("granularity", &Settings::granularity)
+        .def_readwrite("range", &Settings::range)
+        .def_readwrite("balance", &Settings::balance)
+        .def_readwrite("tree_ratio", &Settings::tree_ratio)
+        .def_readwrite("dynamic_domain", &Settings::dynamic_domain)
+        .def_readwrite("dd_alpha", &Settings::dd_alpha)
+        .def_readwrite("dd_radius", &Settings::dd_radius)
+        .def_readwrite("dd_min_radius", &Settings::dd_min_radius)
+        .def_readwrite("enable_mesh_collision", &Settings::enable_mesh_collision)
+        .def_readwrite("time_limit_ms", &Settings::time_limit_ms)
+        .def_readwrite("gpu_clock_rate_khz", &Settings::gpu_clock_rate_khz)
+        .def_readwrite("shortcut_path", &Settings::shortcut_path)
+        .def_readwrite("valid_segment_length", &Settings::valid_segment_length)
+        .def_readwrite("collision_margin", &Settings::collision_margin)
+        .def_readwrite("shortcut_collision_margin", &Settings::shortcut_collision_margin)
+        .def_readwrite("seed", &Settings::seed);
+    py::class_<PlannerResult>(m, "PlannerResult")
+        .def_readonly("solved", &PlannerResult::solved)
+        .def_readonly("path", &PlannerResult::path)
+        .def_readonly("iters", &PlannerResult::iters)
+        .def_readonly("cost", &PlannerResult::cost)
+        .def_readonly("wall_time_ms", &PlannerResult::wall_time_ms)
+        .def_readonly("start_tree_size", &PlannerResult::start_tree_size)
+        .def_readonly("goal_tree_size", &PlannerResult::goal_tree_size);
+    py::class_<SolverBuffers>(m, "SolverBuffers")
+        .def_property_readonly("owns_memory", &SolverBuffers::owns_memory)
+        .def_readonly("max_samples", &SolverBuffers::max_samples)
+        .def_readonly("n_dof", &SolverBuffers::n_dof)
+        .def_static("create", &SolverBuffers::create, py::arg("max_samples"), py::arg("n_dof"),
+                    py::arg("num_new_configs"), py::arg("granularity"), py::arg("n_joints"), py::arg("enable_mesh"),
+                    py::return_value_policy::take_ownership, "Allocate device trees for solve_scene calls.")
+        .def("destroy", &SolverBuffers::destroy, "Free the device buffers. Safe to call more than once.");
+    m.def("solve_scene", &solve_rrtc_scene, py::arg("start"), py::arg("goals"), py::arg("scene"),
+          py::arg("settings"), py::arg("robot"), py::arg("bufs") = py::none(),
+          "pRRTC (GPU-parallel RRT-Connect) against the scene; returns PlannerResult with path [N, n_dof].");
     m.def("stomp_solve_scene", &stomp_solve_scene, py::arg("start"), py::arg("goal"), py::arg("scene"),
           py::arg("settings"), py::arg("robot"), py::arg("bufs") = py::none(),
           "Single-goal STOMP against the scene; returns STOMPResult with path [T, n_dof].");

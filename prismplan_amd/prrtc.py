@@ -2,12 +2,14 @@
 
 `prismplan_amd/_lib/_prrtc*.so` (prismplan_amd/csrc/prrtc_module.cpp) mirrors
 the builder and check surface of the reference's pybind module `prrtc`
-(cpp/src/python/bindings.cpp:118-222, 363-402, 493-530, 764-902, 960-1037,
-1191-1282): RobotModelBuilder with the same fields, SceneCollisionDataBuilder
-with int64 device-pointer fields taken from torch tensors,
-check_collision_scene(configs, scene, robot, collision_margin) -> bool[N], and
-chomp_solve_scene / stomp_solve_scene(start, goal, scene, settings, robot,
-bufs=None) -> CHOMPResult / STOMPResult with their settings and buffers.
+(cpp/src/python/bindings.cpp:118-222, 225-233, 363-437, 493-530, 764-902,
+910-1037, 1163-1176, 1191-1282): RobotModelBuilder with the same fields,
+SceneCollisionDataBuilder with int64 device-pointer fields taken from torch
+tensors, check_collision_scene(configs, scene, robot, collision_margin) ->
+bool[N], pRRTC's solve_scene(start, goals, scene, settings, robot, bufs=None)
+-> PlannerResult with Settings and SolverBuffers, and chomp_solve_scene /
+stomp_solve_scene(start, goal, scene, settings, robot, bufs=None) ->
+CHOMPResult / STOMPResult with their settings and buffers.
 Errors surface as RuntimeError. Importing fails if the module is not built.
 """
 from __future__ import annotations
@@ -38,6 +40,10 @@ STOMPSettings = _m.STOMPSettings
 STOMPResult = _m.STOMPResult
 STOMPBuffers = _m.STOMPBuffers
 stomp_solve_scene = _m.stomp_solve_scene
+Settings = _m.Settings
+PlannerResult = _m.PlannerResult
+SolverBuffers = _m.SolverBuffers
+solve_scene = _m.solve_scene
 
 
 def scene_builder(ds) -> SceneCollisionDataBuilder:

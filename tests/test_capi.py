@@ -136,6 +136,30 @@ def test_prrtc_chomp_surface_and_defaults():
     assert callable(prrtc.chomp_solve_scene)
 
 
+def test_prrtc_planner_surface_and_defaults():
+    """pRRTC's Settings carry the reference's names and defaults
+    (planning/pRRTC_settings.hh:3-32, python/bindings.cpp:919-938)."""
+    from prismplan_amd import prrtc
+    st = prrtc.Settings()
+    want = dict(max_samples=1000000, max_iters=1000000, time_limit_ms=4000.0, gpu_clock_rate_khz=0,
+                num_new_configs=600, granularity=16, range=0.5, balance=1, tree_ratio=0.5, dynamic_domain=True,
+                dd_alpha=1e-4, dd_radius=4.0, dd_min_radius=1.0, enable_mesh_collision=False, shortcut_path=False,
+                valid_segment_length=0.025, collision_margin=0.0, shortcut_collision_margin=5e-4)
+    for k, v in want.items():
+        assert abs(getattr(st, k) - v) <= 1e-7 * max(1.0, abs(v)), k
+    for f in ("solved", "path", "iters", "cost", "wall_time_ms"):
+        assert hasattr(prrtc.PlannerResult, f)
+    for f in ("owns_memory", "max_samples", "n_dof", "create", "destroy"):
+        assert hasattr(prrtc.SolverBuffers, f)
+    with pytest.raises(RuntimeError):
+        prrtc.SolverBuffers.create(max_samples=100, n_dof=7, num_new_configs=8, granularity=16, n_joints=9,
+                                   enable_mesh=True)
+    with pytest.raises(RuntimeError):
+        prrtc.SolverBuffers.create(max_samples=100, n_dof=0, num_new_configs=8, granularity=16, n_joints=9,
+                                   enable_mesh=False)
+    assert callable(prrtc.solve_scene)
+
+
 def test_chomp_solve_rejects_bad_arguments():
     L = _ppl.lib()
     out = C.c_void_p()

@@ -228,3 +228,38 @@ def test_shortcut_edge_cases(robots):
     out = S.shortcut_path(rob, ds, q, valid_segment_length=0.05)
     same = out.shape == q[keep].shape and np.array_equal(out, q[keep])
     assert same or amb > 0
+
+
+def test_prrtc_solve_scene_surface(robots):
+    """pRRTC through the pybind names the reference's plugin calls
+    (python/gpu_planning/plugins/prrtc_plugin.py: Settings, SolverBuffers.create,
+    solve_scene(start, goals, scene, settings, robot, bufs=...))."""
+    from prismplan_amd import prrtc
+    from opt_common import prrtc_panda
+    rob = robots["panda"]
+    scene, o, start, goals = problem(rob, 500)
+    ds = S.DeviceScene.from_host(scene)
+    robot = prrtc_panda()
+    sb = prrtc.scene_builder(ds)
+    bufs = prrtc.SolverBuffers.create(max_samples=100000, n_dof=7, num_new_configs=200, granularity=16, n_joints=9,
+                                      enable_mesh=False)
+    st = prrtc.Settings()
+    st.num_new_configs, st.max_samples, st.time_limit_ms = 200, 100000, 3000.0
+    for k in range(2):
+        st.seed = 21 + k
+        res = prrtc.solve_scene(start, goals, sb, st, robot, bufs=bufs)
+        banner(f"prrtc.solve_scene: solved {res.solved}, {res.path.shape[0]} waypoints, cost {res.cost:.4f}, "
+               f"iters {res.iters}, {res.wall_time_ms:.2f} ms")
+        d = {"solved": res.solved, "path": res.path, "iters": res.iters, "cost": res.cost, "reached_goal": 0,
+             "start_tree_size": res.start_tree_size, "goal_tree_size": res.goal_tree_size,
+             "path_length": res.path.shape[0] - 2, "kernel_ms": 0.0}
+        ws = S.rrtc_settings(num_new_configs=200, max_samples=100000, granularity=st.granularity, range=st.range)
+        check_solution(o, d, start, goals, ws, f"pybind/{k}")
+    st.shortcut_path = True
+    res = prrtc.solve_scene(start, goals, sb, st, robot)
+    assert res.solved and res.path.shape[0] >= 2
+    np.testing.assert_array_equal(res.path[0], start)
+    np.testing.assert_array_equal(res.path[-1], goals[0])
+    bufs.destroy()
+    with pytest.raises(RuntimeError):
+        prrtc.solve_scene(start[:3], goals, sb, st, robot)
