@@ -16,9 +16,9 @@
 //           the path walk's results
 //   bases   [workgroups][16] FP32 Halton bases
 //   path    [2][kPathCap][n_dof] FP32 the two tree walks of the solution
-// One workgroup (64 * ceil(4 * granularity / 64) lanes) owns one new
-// configuration per iteration; each aligned 4-lane group checks one state of
-// an edge with two_phase_cc<4>. Counters move with agent-scope atomics; node
+// One workgroup of 256 lanes (four waves) owns one new configuration per
+// iteration; each aligned group of G = 256 / granularity lanes (a power of
+// two, at most 64) checks one state of an edge with two_phase_cc<G>. Counters move with agent-scope atomics; node
 // rows are published with a release fence before `completed` counts them and
 // read after an acquire fence, so a reader sees a row either whole or as the
 // -9999 sentinel.
@@ -46,9 +46,9 @@ namespace {
 using namespace ppl::device;
 
 constexpr int kMaxDim = 16;
-constexpr int kG = 4;                // lanes per edge state
 constexpr int kMaxGran = 64;         // states per edge check
-constexpr int kMaxThreads = kG * kMaxGran;
+constexpr int kThreads = 256;        // four waves per workgroup
+constexpr int kMaxThreads = kThreads;
 constexpr int kMaxWaves = kMaxThreads / 64;
 constexpr int kPathCap = 4096;       // nodes per tree walk
 constexpr float kUnwritten = -9999.0f;
@@ -67,6 +67,17 @@ struct RrtcCtl {
     float cost;
 };
 
+// Word offsets (into the dynamic LDS, after the sphere positions) of the LDS
+// copies of the robot tables and the scene's OBBs. The edge checks read
+// every table once per sphere or pair, so a copy in LDS turns dependent
+// global-memory latency into LDS latency. A group that does not fit the
+// budget stays in global memory (flag 0).
+struct StageLayout {
+    int robot, pairs, obbs;
+    int ft, jmeta, sph_off, sph, sph_id, apx_off, apx, apx_id, sph_orig, apx_orig, apx_pairs, pairs_at;
+    int obb_dims, obb_pose, obb_enable;
+};
+
 struct RrtcArgs {
     ppl_robot_device_view R;
     ppl_scene_view S;
@@ -79,6 +90,7 @@ struct RrtcArgs {
     int dim, max_samples, max_iters, granularity, balance, dynamic_domain, n_blocks;
     float range, tree_ratio, dd_alpha, dd_radius, dd_min_radius, margin;
     long long time_ticks;  // device wall-clock ticks, 0 = no limit
+    StageLayout L;         // LDS copies of the robot tables and the OBBs
 };
 
 __device__ __forceinline__ int ld_agent(const int* p) {
@@ -89,24 +101,24 @@ __device__ __forceinline__ float ld_agent_f(const float* p) {
     return __int_as_float(ld_agent(reinterpret_cast<const int*>(p)));
 }
 
-// Dynamic-domain radius update by compare-and-swap on the value last seen
-// (pRRTC.cu:1788-1799 grow on success, :1991-2004 shrink on failure).
+// Dynamic-domain radius update (pRRTC.cu:1788-1799 grow on success,
+// :1991-2004 shrink on failure) as one compare-and-swap on the value just
+// read: an update that loses a race is dropped instead of retried. Early in a
+// run hundreds of workgroups fail against the same root, and a retry loop
+// serialises them on one address; a dropped update moves a radius by
+// dd_alpha (1e-4 by default) at most.
 __device__ void radius_update(float* r, bool grow, float alpha, float dd_radius, float dd_min) {
     int* p = reinterpret_cast<int*>(r);
-    int cur = ld_agent(p);
-    for (;;) {
-        const float old = __int_as_float(cur);
-        float nv;
-        if (grow) {
-            if (old == FLT_MAX) return;
-            nv = old * (1.0f + alpha);
-        } else {
-            nv = old == FLT_MAX ? dd_radius : fmaxf(old * (1.0f - alpha), dd_min);
-        }
-        const int prev = atomicCAS(p, cur, __float_as_int(nv));
-        if (prev == cur) return;
-        cur = prev;
+    const int cur = ld_agent(p);
+    const float old = __int_as_float(cur);
+    float nv;
+    if (grow) {
+        if (old == FLT_MAX) return;
+        nv = old * (1.0f + alpha);
+    } else {
+        nv = old == FLT_MAX ? dd_radius : fmaxf(old * (1.0f - alpha), dd_min);
     }
+    if (nv != old) atomicCAS(p, cur, __float_as_int(nv));
 }
 
 // Workgroup (distance, index) minimum: smaller distance, then smaller index.
@@ -161,20 +173,57 @@ __device__ void nearest(const float* nodes, int size, const float* q, int dim, f
 // Whether any of the states base + (k + 1) * step, k = 0 .. granularity - 1
 // (one per 4-lane group), collides: the two-phase scene test and the self
 // check (pRRTC.cu:1680-1781, 1867-1948).
-__device__ bool edge_hit(const RrtcArgs& A, const float* base, const float* step, float* qbuf, float* pos,
-                         float* apos) {
-    const int g = threadIdx.x / kG;
+template <int G>
+__device__ bool edge_hit(const RrtcArgs& A, const ppl_robot_device_view& R, const ppl_scene_view& S, const float* base,
+                         const float* step, float* qbuf, float* pos, float* apos) {
+    const int g = threadIdx.x / G;
     bool hit = false;
     if (g < A.granularity) {
         float* q = qbuf + g * kMaxDim;
-        for (int k = threadIdx.x & (kG - 1); k < A.dim; k += kG) q[k] = fmaf((float)(g + 1), step[k], base[k]);
+        for (int k = threadIdx.x & (G - 1); k < A.dim; k += G) q[k] = fmaf((float)(g + 1), step[k], base[k]);
         group_sync();
-        const TwoPhaseResult r = two_phase_cc<kG>(A.R, A.S, q, pos + (size_t)g * 3 * A.R.n_spheres,
-                                                  apos + (size_t)g * 3 * A.R.n_approx_spheres, threadIdx.x,
-                                                  A.margin, true);
+        const TwoPhaseResult r = two_phase_cc<G>(R, S, q, pos + (size_t)g * 3 * R.n_spheres,
+                                                  apos + (size_t)g * 3 * R.n_approx_spheres, threadIdx.x, A.margin,
+                                                  true);
         hit = r.collision;
     }
     return __syncthreads_or(hit ? 1 : 0) != 0;
+}
+
+template <class T>
+__device__ const T* stage_words(uint32_t* lds_words, int off, const T* src, int n_words) {
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+    for (int i = threadIdx.x; i < n_words; i += blockDim.x) lds_words[off + i] = s[i];
+    return reinterpret_cast<const T*>(lds_words + off);
+}
+
+// Copy the robot tables and OBBs the layout selects into LDS and point the
+// views at the copies.
+__device__ void stage_tables(const RrtcArgs& A, uint32_t* W, ppl_robot_device_view& R, ppl_scene_view& S) {
+    const StageLayout& L = A.L;
+    const int J = R.n_joints, Sn = R.n_spheres, An = R.n_approx_spheres;
+    if (L.robot) {
+        R.ft = stage_words(W, L.ft, R.ft, J * 12);
+        R.jmeta = stage_words(W, L.jmeta, R.jmeta, J * 4);
+        R.sph_off = stage_words(W, L.sph_off, R.sph_off, J + 1);
+        R.sph = stage_words(W, L.sph, R.sph, Sn * 4);
+        R.sph_id = stage_words(W, L.sph_id, R.sph_id, Sn);
+        R.apx_off = stage_words(W, L.apx_off, R.apx_off, J + 1);
+        R.apx = stage_words(W, L.apx, R.apx, An * 4);
+        R.apx_id = stage_words(W, L.apx_id, R.apx_id, An);
+        R.sph_orig = stage_words(W, L.sph_orig, R.sph_orig, Sn * 4);
+        R.apx_orig = stage_words(W, L.apx_orig, R.apx_orig, An * 4);
+        R.apx_pairs = stage_words(W, L.apx_pairs, R.apx_pairs, R.n_approx_pairs * 2);
+    }
+    if (L.pairs) R.pairs = stage_words(W, L.pairs_at, R.pairs, R.n_pairs * 2);
+    if (L.obbs && S.n_obbs > 0) {
+        S.obb_dims = stage_words(W, L.obb_dims, S.obb_dims, S.n_obbs * 4);
+        S.obb_pose = stage_words(W, L.obb_pose, S.obb_pose, S.n_obbs * 8);
+        uint8_t* en = reinterpret_cast<uint8_t*>(W + L.obb_enable);
+        for (int i = threadIdx.x; i < S.n_obbs; i += blockDim.x) en[i] = S.obb_enable[i];
+        S.obb_enable = en;
+    }
+    __syncthreads();
 }
 
 // Publish row `idx` of tree t (written by lanes < dim) and count it.
@@ -204,7 +253,8 @@ __device__ int choose_tree(const RrtcArgs& A, int iter, int prev_t) {
     return prev_t;
 }
 
-__global__ __launch_bounds__(kMaxThreads) void k_rrtc(RrtcArgs A) {
+template <int G>
+__global__ __launch_bounds__(kThreads) void k_rrtc(RrtcArgs A) {
     extern __shared__ float lds[];   // [groups][3 * n_spheres] then [groups][3 * n_approx]
     __shared__ float s_cfg[kMaxDim], s_near[kMaxDim], s_step[kMaxDim], s_vec[kMaxDim];
     __shared__ float s_hn[kMaxDim], s_hd[kMaxDim];
@@ -214,9 +264,11 @@ __global__ __launch_bounds__(kMaxThreads) void k_rrtc(RrtcArgs A) {
     __shared__ int s_t, s_stop, s_skip, s_size, s_index, s_next;
     __shared__ float s_dist;
     const int tid = threadIdx.x, dim = A.dim;
-    const int groups = blockDim.x / kG;
     float* pos = lds;
-    float* apos = lds + (size_t)groups * 3 * A.R.n_spheres;
+    float* apos = lds + (size_t)A.granularity * 3 * A.R.n_spheres;
+    ppl_robot_device_view R = A.R;
+    ppl_scene_view S = A.S;
+    stage_tables(A, reinterpret_cast<uint32_t*>(lds), R, S);
     const float* hb = A.bases + (size_t)blockIdx.x * kMaxDim;
     if (tid < dim) {
         s_hn[tid] = 0.0f;
@@ -266,7 +318,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_rrtc(RrtcArgs A) {
             s_step[tid] = (c - nq) / (float)A.granularity;
         }
         __syncthreads();
-        if (edge_hit(A, s_near, s_step, s_q, pos, apos)) {
+        if (edge_hit<G>(A, R, S, s_near, s_step, s_q, pos, apos)) {
             if (A.dynamic_domain && tid == 0) radius_update(&A.radii[t][near], false, A.dd_alpha, A.dd_radius, A.dd_min_radius);
             continue;
         }
@@ -306,7 +358,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_rrtc(RrtcArgs A) {
         int e = 0;
         bool full = false;
         for (; e < n_ext; ++e) {
-            if (edge_hit(A, s_cfg, s_step, s_q, pos, apos)) break;
+            if (edge_hit<G>(A, R, S, s_cfg, s_step, s_q, pos, apos)) break;
             if (tid == 0) {
                 int idx = -1;
                 if (A.time_ticks > 0 && wall_clock64() - t0 > A.time_ticks) atomicCAS(&A.ctl->status, 0, -1);
@@ -448,6 +500,75 @@ struct Grow {
 };
 
 size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Where the LDS copies go: the robot tables (without the exact pairs) first,
+// then the OBBs, then the exact pairs, each only while the whole dynamic LDS
+// stays within kMaxDynLds. Offsets in 4-byte words, 4-word aligned.
+StageLayout stage_layout(const ppl_robot_device_view& R, int n_obbs, size_t pos_words) {
+    StageLayout L{};
+    const size_t cap = kMaxDynLds / 4;
+    size_t at = (pos_words + 3) & ~(size_t)3;
+    auto take = [&at](int n) {
+        const int o = (int)at;
+        at += ((size_t)n + 3) & ~(size_t)3;
+        return o;
+    };
+    const int J = R.n_joints, Sn = R.n_spheres, An = R.n_approx_spheres;
+    {
+        const size_t save = at;
+        StageLayout t = L;
+        t.ft = take(J * 12);
+        t.jmeta = take(J * 4);
+        t.sph_off = take(J + 1);
+        t.sph = take(Sn * 4);
+        t.sph_id = take(Sn);
+        t.apx_off = take(J + 1);
+        t.apx = take(An * 4);
+        t.apx_id = take(An);
+        t.sph_orig = take(Sn * 4);
+        t.apx_orig = take(An * 4);
+        t.apx_pairs = take(R.n_approx_pairs * 2);
+        if (at <= cap) {
+            L = t;
+            L.robot = 1;
+        } else {
+            at = save;
+        }
+    }
+    if (n_obbs > 0) {
+        const size_t save = at;
+        const int d = take(n_obbs * 4), p = take(n_obbs * 8), e = take((n_obbs + 3) / 4);
+        if (at <= cap) {
+            L.obbs = 1;
+            L.obb_dims = d;
+            L.obb_pose = p;
+            L.obb_enable = e;
+        } else {
+            at = save;
+        }
+    }
+    if (L.robot && R.n_pairs > 0) {
+        const size_t save = at;
+        const int o = take(R.n_pairs * 2);
+        if (at <= cap) {
+            L.pairs = 1;
+            L.pairs_at = o;
+        } else {
+            at = save;
+        }
+    }
+    return L;
+}
+
+// Dynamic LDS words the layout uses (sphere positions included).
+size_t lay_words(const StageLayout& L, const ppl_robot_device_view& R, int n_obbs, size_t pos_words) {
+    size_t end = pos_words;
+    auto upto = [&end](int off, int n) { end = std::max(end, (size_t)off + (size_t)n); };
+    if (L.robot) upto(L.apx_pairs, R.n_approx_pairs * 2), upto(L.apx_orig, R.n_approx_spheres * 4);
+    if (L.obbs) upto(L.obb_enable, (n_obbs + 3) / 4);
+    if (L.pairs) upto(L.pairs_at, R.n_pairs * 2);
+    return end;
+}
 
 ppl_status hip_fail(hipError_t e, const char* what) {
     static thread_local char buf[256];
@@ -714,9 +835,13 @@ ppl_status ppl_rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const fl
     if (R.n_pairs > 0 && !ppl::robot_self_gate_implied(robot))
         return ppl::fail_msg(PPL_ERR_UNSUPPORTED,
                              "the approximate self pairs do not cover the exact ones: the gated self check would miss hits");
-    const int threads = std::max(64, (kG * cfg->granularity + 63) / 64 * 64);
-    const size_t lds = (size_t)(threads / kG) * 3 * (R.n_spheres + R.n_approx_spheres) * sizeof(float);
-    if (lds > kMaxDynLds) return ppl::fail_msg(PPL_ERR_CAPACITY, "too many spheres x granularity for the LDS staging");
+    // lanes per edge state: the widest power of two that gives every state its own group
+    int G = 64;
+    while (G > 1 && G * cfg->granularity > kThreads) G >>= 1;
+    const size_t pos_words = (size_t)cfg->granularity * 3 * (R.n_spheres + R.n_approx_spheres);
+    if (pos_words * 4 > kMaxDynLds) return ppl::fail_msg(PPL_ERR_CAPACITY, "too many spheres x granularity for the LDS staging");
+    StageLayout lay = stage_layout(R, scene->n_obbs, pos_words);
+    const size_t lds = (size_t)lay_words(lay, R, scene->n_obbs, pos_words) * 4;
 
     ppl_rrtc_buffers_s local;
     ppl_rrtc_buffers_s* b = bufs;
@@ -762,6 +887,7 @@ ppl_status ppl_rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const fl
     A.dd_min_radius = cfg->dd_min_radius;
     A.margin = cfg->collision_margin;
     A.time_ticks = 0;
+    A.L = lay;
     if (cfg->time_limit_ms > 0.0f) {
         int dev = 0, khz = 0;
         RR_TRY(hipGetDevice(&dev));
@@ -791,7 +917,13 @@ ppl_status ppl_rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const fl
     RR_TRY(hipMemcpyAsync(A.parents[0], &zero, sizeof(int), hipMemcpyHostToDevice, s));
     RR_TRY(hipMemcpyAsync(A.parents[1], goal_parents.data(), (size_t)n_goals * sizeof(int), hipMemcpyHostToDevice, s));
     RR_TRY(hipEventRecord(b->ev[0], s));
-    hipLaunchKernelGGL(k_rrtc, dim3(cfg->num_new_configs), dim3(threads), lds, s, A);
+    switch (G) {
+        case 64: hipLaunchKernelGGL(k_rrtc<64>, dim3(cfg->num_new_configs), dim3(kThreads), lds, s, A); break;
+        case 32: hipLaunchKernelGGL(k_rrtc<32>, dim3(cfg->num_new_configs), dim3(kThreads), lds, s, A); break;
+        case 16: hipLaunchKernelGGL(k_rrtc<16>, dim3(cfg->num_new_configs), dim3(kThreads), lds, s, A); break;
+        case 8: hipLaunchKernelGGL(k_rrtc<8>, dim3(cfg->num_new_configs), dim3(kThreads), lds, s, A); break;
+        default: hipLaunchKernelGGL(k_rrtc<4>, dim3(cfg->num_new_configs), dim3(kThreads), lds, s, A); break;
+    }
     RR_TRY(hipGetLastError());
     RR_TRY(hipEventRecord(b->ev[1], s));
     hipLaunchKernelGGL(k_rrtc_path, dim3(1), dim3(64), 0, s, A);
