@@ -208,8 +208,11 @@ __device__ __forceinline__ bool sphere_scene_collide(const ppl_scene_view& S, fl
 // Walks the DFS-ordered tree with the live transforms in registers (every lane
 // of the group walks it; lanes split the spheres) and writes world sphere
 // centres to pos[3 * original index]. approx selects the approximate spheres.
+// frames (nullable, [n_joints][12]) receives every joint's world frame, rows
+// 0..2 of the 4x4, for spheres_from_frames.
 template <int G>
-__device__ void fk_spheres(const ppl_robot_device_view& R, const float* q, float* pos, int tid, bool approx) {
+__device__ void fk_spheres(const ppl_robot_device_view& R, const float* q, float* pos, int tid, bool approx,
+                           float* frames = nullptr) {
     const int lane = tid & (G - 1);
     float slot[2][12];
     aff_identity(slot[0]);
@@ -229,6 +232,10 @@ __device__ void fk_spheres(const ppl_robot_device_view& R, const float* q, float
             if (sl == 0) slot[0][k] = T[k];
             else slot[1][k] = T[k];
         }
+        if (frames && lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) frames[12 * jj + k] = T[k];
+        }
         const int* off = approx ? R.apx_off : R.sph_off;
         const float* tab = approx ? R.apx : R.sph;
         const int* id = approx ? R.apx_id : R.sph_id;
@@ -238,6 +245,27 @@ __device__ void fk_spheres(const ppl_robot_device_view& R, const float* q, float
             const int o = id[s];
             pos[3 * o] = x; pos[3 * o + 1] = y; pos[3 * o + 2] = z;
         }
+    }
+    group_sync();
+}
+
+// World sphere centres from stored joint frames (fk_spheres' `frames`): no
+// tree walk. Lanes take spheres in turn across all joints.
+template <int G>
+__device__ void spheres_from_frames(const ppl_robot_device_view& R, const float* frames, float* pos, int tid,
+                                    bool approx) {
+    const int lane = tid & (G - 1);
+    const int n = approx ? R.n_approx_spheres : R.n_spheres;
+    const int* off = approx ? R.apx_off : R.sph_off;
+    const float* tab = approx ? R.apx : R.sph;
+    const int* id = approx ? R.apx_id : R.sph_id;
+    int jj = 0;
+    for (int s = lane; s < n; s += G) {
+        while (off[jj + 1] <= s) ++jj;
+        float x, y, z;
+        aff_point(frames + 12 * jj, tab[4 * s], tab[4 * s + 1], tab[4 * s + 2], x, y, z);
+        const int o = id[s];
+        pos[3 * o] = x; pos[3 * o + 1] = y; pos[3 * o + 2] = z;
     }
     group_sync();
 }
@@ -397,6 +425,52 @@ __device__ TwoPhaseResult two_phase_cc(const ppl_robot_device_view& R, const ppl
     if (check_self && !out.collision && self_collide<G>(R, approx_sphere_pos, tid, true)) {
         if (!out.did_full_fk) {
             fk_runtime<G>(R, q, sphere_pos, tid);
+            out.did_full_fk = true;
+        }
+        out.collision = self_collide<G>(R, sphere_pos, tid, false);
+    }
+    return out;
+}
+
+// The same verdict with one tree walk: T ([n_joints][12] per configuration,
+// the reference's FK scratch `T`) keeps the joint frames of the approximate
+// pass, and the exact spheres are placed from them. Lanes take the
+// approximate spheres, then the exact spheres of flagged joints, in turn
+// across all joints.
+template <int G>
+__device__ TwoPhaseResult two_phase_cc(const ppl_robot_device_view& R, const ppl_scene_view& S, const float* q,
+                                       float* sphere_pos, float* approx_sphere_pos, float* T, int tid,
+                                       float collision_margin, bool check_self) {
+    const int lane = tid & (G - 1);
+    TwoPhaseResult out{false, false, false};
+    fk_spheres<G>(R, q, approx_sphere_pos, tid, true, T);
+    unsigned flags = 0u;
+    int jj = 0;
+    for (int a = lane; a < R.n_approx_spheres; a += G) {
+        while (R.apx_off[jj + 1] <= a) ++jj;
+        const int o = R.apx_id[a];
+        if (sphere_scene_collide(S, approx_sphere_pos[3 * o], approx_sphere_pos[3 * o + 1], approx_sphere_pos[3 * o + 2],
+                                 R.apx[4 * a + 3] + collision_margin, -1))
+            flags |= 1u << jj;
+    }
+    flags = group_or<G>(flags);
+    if (flags) {
+        spheres_from_frames<G>(R, T, sphere_pos, tid, false);
+        out.did_full_fk = true;
+        bool hit = false;
+        int js = 0;
+        for (int s = lane; s < R.n_spheres && !hit; s += G) {
+            while (R.sph_off[js + 1] <= s) ++js;
+            if (!(flags & (1u << js))) continue;
+            const int o = R.sph_id[s];
+            hit = sphere_scene_collide(S, sphere_pos[3 * o], sphere_pos[3 * o + 1], sphere_pos[3 * o + 2],
+                                       R.sph[4 * s + 3] + collision_margin, o);
+        }
+        out.collision = out.scene_hit = group_any<G>(hit, tid);
+    }
+    if (check_self && !out.collision && self_collide<G>(R, approx_sphere_pos, tid, true)) {
+        if (!out.did_full_fk) {
+            spheres_from_frames<G>(R, T, sphere_pos, tid, false);
             out.did_full_fk = true;
         }
         out.collision = self_collide<G>(R, sphere_pos, tid, false);

@@ -18,10 +18,12 @@
 //   path    [2][kPathCap][n_dof] FP32 the two tree walks of the solution
 // One workgroup of 256 lanes (four waves) owns one new configuration per
 // iteration; each aligned group of G = 256 / granularity lanes (a power of
-// two, at most 64) checks one state of an edge with two_phase_cc<G>. Counters move with agent-scope atomics; node
-// rows are published with a release fence before `completed` counts them and
-// read after an acquire fence, so a reader sees a row either whole or as the
-// -9999 sentinel.
+// two, at most 64) checks one state of an edge with two_phase_cc<G>. Every
+// access one workgroup makes to another's data (counters, node rows, radii,
+// the status word) is an agent-scope atomic, coherent across the XCDs' L2s
+// without fences. Each row element goes from -9999 to its value once, so a
+// reader sees a row whole or with a sentinel element that keeps it from
+// being nearest; parents are read only after the kernel ends.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,6 +56,7 @@ constexpr int kPathCap = 4096;       // nodes per tree walk
 constexpr float kUnwritten = -9999.0f;
 constexpr size_t kMaxDynLds = 56 * 1024;
 constexpr long long kMaxShortcutStates = 1ll << 26;
+constexpr int kHeadRows = 64;        // path rows read back with the control block
 
 struct RrtcCtl {
     int status;            // 0 running, 1 solved, -1 stopped (limit reached)
@@ -65,7 +68,9 @@ struct RrtcCtl {
     int reached_goal;
     int path_ok;
     float cost;
+    int n_waypoints;       // rows of the assembled start -> goal path
 };
+static_assert(sizeof(RrtcCtl) <= 256, "control block must fit its 256-byte slot");
 
 // Word offsets (into the dynamic LDS, after the sphere positions) of the LDS
 // copies of the robot tables and the scene's OBBs. The edge checks read
@@ -86,7 +91,10 @@ struct RrtcArgs {
     float* radii[2];
     RrtcCtl* ctl;
     const float* bases;    // [workgroups][16]
-    float* path;           // [2][kPathCap][dim]
+    float* path;           // [2][kPathCap][dim] tree walks
+    float* out;            // [2 * kPathCap + 2][dim] assembled path, start -> goal
+    const float* roots;    // [1 + n_goals][dim] start, then the goals
+    int n_goals;
     int dim, max_samples, max_iters, granularity, balance, dynamic_domain, n_blocks;
     float range, tree_ratio, dd_alpha, dd_radius, dd_min_radius, margin;
     long long time_ticks;  // device wall-clock ticks, 0 = no limit
@@ -99,6 +107,10 @@ __device__ __forceinline__ int ld_agent(const int* p) {
 
 __device__ __forceinline__ float ld_agent_f(const float* p) {
     return __int_as_float(ld_agent(reinterpret_cast<const int*>(p)));
+}
+
+__device__ __forceinline__ void st_agent_f(float* p, float v) {
+    __hip_atomic_store(reinterpret_cast<int*>(p), __float_as_int(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Dynamic-domain radius update (pRRTC.cu:1788-1799 grow on success,
@@ -159,7 +171,7 @@ __device__ void nearest(const float* nodes, int size, const float* q, int dim, f
         const float* x = nodes + (size_t)i * dim;
         float s = 0.0f;
         for (int k = 0; k < dim; ++k) {
-            const float t = x[k] - q[k];
+            const float t = ld_agent_f(x + k) - q[k];
             s = fmaf(t, t, s);
         }
         if (s < best) {
@@ -175,7 +187,7 @@ __device__ void nearest(const float* nodes, int size, const float* q, int dim, f
 // check (pRRTC.cu:1680-1781, 1867-1948).
 template <int G>
 __device__ bool edge_hit(const RrtcArgs& A, const ppl_robot_device_view& R, const ppl_scene_view& S, const float* base,
-                         const float* step, float* qbuf, float* pos, float* apos) {
+                         const float* step, float* qbuf, float* pos, float* apos, float* frames) {
     const int g = threadIdx.x / G;
     bool hit = false;
     if (g < A.granularity) {
@@ -183,8 +195,8 @@ __device__ bool edge_hit(const RrtcArgs& A, const ppl_robot_device_view& R, cons
         for (int k = threadIdx.x & (G - 1); k < A.dim; k += G) q[k] = fmaf((float)(g + 1), step[k], base[k]);
         group_sync();
         const TwoPhaseResult r = two_phase_cc<G>(R, S, q, pos + (size_t)g * 3 * R.n_spheres,
-                                                  apos + (size_t)g * 3 * R.n_approx_spheres, threadIdx.x, A.margin,
-                                                  true);
+                                                  apos + (size_t)g * 3 * R.n_approx_spheres,
+                                                  frames + (size_t)g * 12 * R.n_joints, threadIdx.x, A.margin, true);
         hit = r.collision;
     }
     return __syncthreads_or(hit ? 1 : 0) != 0;
@@ -226,14 +238,14 @@ __device__ void stage_tables(const RrtcArgs& A, uint32_t* W, ppl_robot_device_vi
     __syncthreads();
 }
 
-// Publish row `idx` of tree t (written by lanes < dim) and count it.
+// Count a row of tree t once its lanes have stored it. The count only bounds
+// the readers' scans: rows are read and written with agent-scope atomics, so
+// no fence orders them, and a row that is not yet (fully) visible reads as
+// the -9999 sentinel.
 __device__ void publish(const RrtcArgs& A, int t) {
-    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) atomicAdd(&A.ctl->completed[t], 1);
 }
-
-__device__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
 // Tree a workgroup grows this iteration (pRRTC.cu:1602-1627).
 __device__ int choose_tree(const RrtcArgs& A, int iter, int prev_t) {
@@ -264,8 +276,9 @@ __global__ __launch_bounds__(kThreads) void k_rrtc(RrtcArgs A) {
     __shared__ int s_t, s_stop, s_skip, s_size, s_index, s_next;
     __shared__ float s_dist;
     const int tid = threadIdx.x, dim = A.dim;
-    float* pos = lds;
+    float* pos = lds;   // [granularity][3 * n_spheres], then the approximate spheres, then the joint frames
     float* apos = lds + (size_t)A.granularity * 3 * A.R.n_spheres;
+    float* frames = apos + (size_t)A.granularity * 3 * A.R.n_approx_spheres;
     ppl_robot_device_view R = A.R;
     ppl_scene_view S = A.S;
     stage_tables(A, reinterpret_cast<uint32_t*>(lds), R, S);
@@ -299,7 +312,6 @@ __global__ __launch_bounds__(kThreads) void k_rrtc(RrtcArgs A) {
         const int t = s_t, o = 1 - t;
         if (tid == 0) s_size = min(ld_agent(&A.ctl->free_idx[t]), ld_agent(&A.ctl->completed[t]));
         __syncthreads();
-        acquire_agent();
         nearest(A.nodes[t], s_size, s_cfg, dim, red_d, red_i);
         const int near = red_i[0];
         if (tid == 0) {
@@ -307,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void k_rrtc(RrtcArgs A) {
             s_dist = dist;
             s_skip = A.dynamic_domain && ld_agent_f(&A.radii[t][near]) < dist;
         }
-        if (tid < dim) s_near[tid] = A.nodes[t][(size_t)near * dim + tid];
+        if (tid < dim) s_near[tid] = ld_agent_f(A.nodes[t] + (size_t)near * dim + tid);
         __syncthreads();
         if (s_skip) continue;
         if (tid < dim) {   // steer by at most `range` (pRRTC.cu:1668-1677)
@@ -318,7 +330,7 @@ __global__ __launch_bounds__(kThreads) void k_rrtc(RrtcArgs A) {
             s_step[tid] = (c - nq) / (float)A.granularity;
         }
         __syncthreads();
-        if (edge_hit<G>(A, R, S, s_near, s_step, s_q, pos, apos)) {
+        if (edge_hit<G>(A, R, S, s_near, s_step, s_q, pos, apos, frames)) {
             if (A.dynamic_domain && tid == 0) radius_update(&A.radii[t][near], false, A.dd_alpha, A.dd_radius, A.dd_min_radius);
             continue;
         }
@@ -337,16 +349,15 @@ __global__ __launch_bounds__(kThreads) void k_rrtc(RrtcArgs A) {
         __syncthreads();
         if (s_index < 0) break;
         int last = s_index;
-        if (tid < dim) A.nodes[t][(size_t)last * dim + tid] = s_cfg[tid];
+        if (tid < dim) st_agent_f(A.nodes[t] + (size_t)last * dim + tid, s_cfg[tid]);
         publish(A, t);
         // connect: greedy extension towards the nearest node of the other tree (pRRTC.cu:1814-1960)
         if (tid == 0) s_size = min(ld_agent(&A.ctl->free_idx[o]), ld_agent(&A.ctl->completed[o]));
         __syncthreads();
-        acquire_agent();
         nearest(A.nodes[o], s_size, s_cfg, dim, red_d, red_i);
         const int onear = red_i[0];
         if (tid == 0) s_next = (int)ceilf(sqrtf(red_d[0]) / A.range);
-        if (tid < dim) s_near[tid] = A.nodes[o][(size_t)onear * dim + tid];
+        if (tid < dim) s_near[tid] = ld_agent_f(A.nodes[o] + (size_t)onear * dim + tid);
         __syncthreads();
         const int n_ext = s_next;
         if (tid < dim) {
@@ -358,7 +369,7 @@ __global__ __launch_bounds__(kThreads) void k_rrtc(RrtcArgs A) {
         int e = 0;
         bool full = false;
         for (; e < n_ext; ++e) {
-            if (edge_hit<G>(A, R, S, s_cfg, s_step, s_q, pos, apos)) break;
+            if (edge_hit<G>(A, R, S, s_cfg, s_step, s_q, pos, apos, frames)) break;
             if (tid == 0) {
                 int idx = -1;
                 if (A.time_ticks > 0 && wall_clock64() - t0 > A.time_ticks) atomicCAS(&A.ctl->status, 0, -1);
@@ -382,7 +393,7 @@ __global__ __launch_bounds__(kThreads) void k_rrtc(RrtcArgs A) {
             if (tid < dim) {
                 const float c = s_cfg[tid] + s_vec[tid];
                 s_cfg[tid] = c;
-                A.nodes[t][(size_t)last * dim + tid] = c;
+                st_agent_f(A.nodes[t] + (size_t)last * dim + tid, c);
             }
             publish(A, t);
         }
@@ -412,6 +423,7 @@ __global__ void k_rrtc_path(RrtcArgs A) {
     c->path_n[0] = c->path_n[1] = 0;
     c->reached_goal = 0;
     c->cost = 0.0f;
+    c->n_waypoints = 0;
     if (c->status != 1) return;
     const int dim = A.dim;
     float cost = 0.0f;
@@ -441,8 +453,36 @@ __global__ void k_rrtc_path(RrtcArgs A) {
         c->path_n[tree] = n;
         if (tree == 1) c->reached_goal = cur;
     }
+    // start, the start-tree walk reversed, the goal-tree walk, the goal reached
+    // (pRRTC.cu:2167-2186 builds goal -> start; python/bindings.cpp:260-272 reverses it)
+    const int n0 = c->path_n[0], n1 = c->path_n[1];
+    float* o = A.out;
+    for (int k = 0; k < dim; ++k) o[k] = A.roots[k];
+    for (int i = 0; i < n0; ++i)
+        for (int k = 0; k < dim; ++k) o[(size_t)(1 + i) * dim + k] = A.path[(size_t)(n0 - 1 - i) * dim + k];
+    for (int i = 0; i < n1; ++i)
+        for (int k = 0; k < dim; ++k)
+            o[(size_t)(1 + n0 + i) * dim + k] = A.path[(size_t)(kPathCap + i) * dim + k];
+    for (int k = 0; k < dim; ++k) o[(size_t)(1 + n0 + n1) * dim + k] = A.roots[(size_t)(1 + c->reached_goal) * dim + k];
+    c->n_waypoints = n0 + n1 + 2;
     c->cost = cost;
     c->path_ok = 1;
+}
+
+// Fresh run state: the control block, the roots (start in tree 0, goals in
+// tree 1, each its own parent), pRRTC.cu:2096-2128.
+__global__ void k_rrtc_init(RrtcArgs A) {
+    const int dim = A.dim;
+    if (threadIdx.x == 0) {
+        RrtcCtl c{};
+        c.free_idx[0] = c.completed[0] = 1;
+        c.free_idx[1] = c.completed[1] = A.n_goals;
+        *A.ctl = c;
+        A.parents[0][0] = 0;
+    }
+    for (int i = threadIdx.x; i < A.n_goals; i += blockDim.x) A.parents[1][i] = i;
+    for (int k = threadIdx.x; k < dim; k += blockDim.x) A.nodes[0][k] = A.roots[k];
+    for (int k = threadIdx.x; k < A.n_goals * dim; k += blockDim.x) A.nodes[1][k] = A.roots[dim + k];
 }
 
 __global__ void k_fill_f32(float* p, size_t n, float v) {
@@ -686,20 +726,26 @@ ppl_status shortcut_run(ppl_robot robot, const ppl_scene_view* scene, const floa
 
 struct ppl_rrtc_buffers_s {
     int dim = 0, max_samples = 0, blocks = 0, gran = 0;
-    Grow nodes, parents, radii, ctl, bases, path;
-    RrtcCtl* h_ctl = nullptr;   // pinned
+    Grow nodes, parents, radii, bases, path;
+    Grow io;                    // [ctl 256 B][assembled path][roots]
+    char* h_io = nullptr;       // pinned: [ctl 256 B][kHeadRows rows] then the roots staging
+    size_t h_io_cap = 0;
+    int hwm[2] = {0, 0};        // rows of each tree that may hold values from an earlier run
+    unsigned long long bases_seed = 0;
+    int bases_n = 0;            // workgroups the device table holds for bases_seed
     Shortcut sc;
     hipEvent_t ev[2] = {nullptr, nullptr};
     void release() {
         nodes.release();
         parents.release();
         radii.release();
-        ctl.release();
         bases.release();
         path.release();
+        io.release();
         sc.release();
-        if (h_ctl) (void)hipHostFree(h_ctl);
-        h_ctl = nullptr;
+        if (h_io) (void)hipHostFree(h_io);
+        h_io = nullptr;
+        h_io_cap = 0;
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         ev[0] = ev[1] = nullptr;
@@ -717,10 +763,10 @@ ppl_status rrtc_alloc(ppl_rrtc_buffers_s* b, int dim, int max_samples, int block
     RR_TRY(b->nodes.ensure(2 * ms * dim * sizeof(float)));
     RR_TRY(b->parents.ensure(2 * ms * sizeof(int)));
     RR_TRY(b->radii.ensure(2 * ms * sizeof(float)));
-    RR_TRY(b->ctl.ensure(sizeof(RrtcCtl)));
     RR_TRY(b->bases.ensure((size_t)blocks * kMaxDim * sizeof(float)));
     RR_TRY(b->path.ensure((size_t)2 * kPathCap * dim * sizeof(float)));
-    RR_TRY(hipHostMalloc((void**)&b->h_ctl, sizeof(RrtcCtl), hipHostMallocDefault));
+    b->hwm[0] = b->hwm[1] = max_samples;   // fresh memory: every row is unknown
+    b->bases_n = 0;
     RR_TRY(hipEventCreate(&b->ev[0]));
     RR_TRY(hipEventCreate(&b->ev[1]));
     return PPL_OK;
@@ -838,7 +884,7 @@ ppl_status ppl_rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const fl
     // lanes per edge state: the widest power of two that gives every state its own group
     int G = 64;
     while (G > 1 && G * cfg->granularity > kThreads) G >>= 1;
-    const size_t pos_words = (size_t)cfg->granularity * 3 * (R.n_spheres + R.n_approx_spheres);
+    const size_t pos_words = (size_t)cfg->granularity * (3 * (R.n_spheres + R.n_approx_spheres) + 12 * R.n_joints);
     if (pos_words * 4 > kMaxDynLds) return ppl::fail_msg(PPL_ERR_CAPACITY, "too many spheres x granularity for the LDS staging");
     StageLayout lay = stage_layout(R, scene->n_obbs, pos_words);
     const size_t lds = (size_t)lay_words(lay, R, scene->n_obbs, pos_words) * 4;
@@ -870,7 +916,6 @@ ppl_status ppl_rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const fl
         A.parents[t] = b->parents.as<int>() + t * ms;
         A.radii[t] = b->radii.as<float>() + t * ms;
     }
-    A.ctl = b->ctl.as<RrtcCtl>();
     A.bases = b->bases.as<float>();
     A.path = b->path.as<float>();
     A.dim = dim;
@@ -894,28 +939,51 @@ ppl_status ppl_rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const fl
         RR_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
         A.time_ticks = std::max(1ll, (long long)((double)cfg->time_limit_ms * (double)khz));
     }
-    // trees: -9999 rows, unbounded radii, roots (pRRTC.cu:2070-2128)
-    const size_t fill_n = (size_t)cfg->max_samples * dim;
+    // I/O: [ctl 256 B][assembled path][roots]; pinned mirror of the first two parts plus the roots staging
+    const size_t out_rows = 2 * (size_t)kPathCap + 2;
+    const size_t out_off = 256, roots_off = out_off + a256(out_rows * dim * sizeof(float));
+    const size_t roots_bytes = (size_t)(1 + n_goals) * dim * sizeof(float);
+    RR_TRY(b->io.ensure(roots_off + roots_bytes));
+    const size_t head_bytes = 256 + (size_t)kHeadRows * dim * sizeof(float);
+    const size_t h_need = a256(head_bytes) + roots_bytes;
+    if (b->h_io_cap < h_need) {
+        if (b->h_io) (void)hipHostFree(b->h_io);
+        b->h_io = nullptr;
+        b->h_io_cap = 0;
+        RR_TRY(hipHostMalloc((void**)&b->h_io, h_need, hipHostMallocDefault));
+        b->h_io_cap = h_need;
+    }
+    char* io = b->io.as<char>();
+    A.ctl = reinterpret_cast<RrtcCtl*>(io);
+    A.out = reinterpret_cast<float*>(io + out_off);
+    A.roots = reinterpret_cast<const float*>(io + roots_off);
+    A.n_goals = n_goals;
+    float* h_roots = reinterpret_cast<float*>(b->h_io + a256(head_bytes));
+    std::memcpy(h_roots, h_start, (size_t)dim * sizeof(float));
+    std::memcpy(h_roots + dim, h_goals, (size_t)n_goals * dim * sizeof(float));
+    // rows an earlier run may have written go back to -9999 / FLT_MAX (pRRTC.cu:2070-2083 refills everything)
     for (int t = 0; t < 2; ++t) {
-        hipLaunchKernelGGL(k_fill_f32, dim3(1024), dim3(256), 0, s, A.nodes[t], fill_n, kUnwritten);
-        hipLaunchKernelGGL(k_fill_f32, dim3(256), dim3(256), 0, s, A.radii[t], (size_t)cfg->max_samples, FLT_MAX);
+        const size_t rows = (size_t)std::min(b->hwm[t], cfg->max_samples);
+        if (rows == 0) continue;
+        const int grid = (int)std::min<size_t>(1024, (rows * dim + 255) / 256);
+        hipLaunchKernelGGL(k_fill_f32, dim3(grid), dim3(256), 0, s, A.nodes[t], rows * dim, kUnwritten);
+        hipLaunchKernelGGL(k_fill_f32, dim3((int)std::min<size_t>(256, (rows + 255) / 256)), dim3(256), 0, s,
+                           A.radii[t], rows, FLT_MAX);
     }
     RR_TRY(hipGetLastError());
-    std::vector<float> bases((size_t)cfg->num_new_configs * kMaxDim);
-    for (int w = 0; w < cfg->num_new_configs; ++w) halton_bases(cfg->seed, w, bases.data() + (size_t)w * kMaxDim);
-    std::vector<int> goal_parents(n_goals);
-    for (int i = 0; i < n_goals; ++i) goal_parents[i] = i;
-    const int zero = 0;
-    RrtcCtl* hc = b->h_ctl;
-    std::memset(hc, 0, sizeof *hc);
-    hc->free_idx[0] = hc->completed[0] = 1;
-    hc->free_idx[1] = hc->completed[1] = n_goals;
-    RR_TRY(hipMemcpyAsync(A.ctl, hc, sizeof *hc, hipMemcpyHostToDevice, s));
-    RR_TRY(hipMemcpyAsync(b->bases.p, bases.data(), bases.size() * sizeof(float), hipMemcpyHostToDevice, s));
-    RR_TRY(hipMemcpyAsync(A.nodes[0], h_start, (size_t)dim * sizeof(float), hipMemcpyHostToDevice, s));
-    RR_TRY(hipMemcpyAsync(A.nodes[1], h_goals, (size_t)n_goals * dim * sizeof(float), hipMemcpyHostToDevice, s));
-    RR_TRY(hipMemcpyAsync(A.parents[0], &zero, sizeof(int), hipMemcpyHostToDevice, s));
-    RR_TRY(hipMemcpyAsync(A.parents[1], goal_parents.data(), (size_t)n_goals * sizeof(int), hipMemcpyHostToDevice, s));
+    for (int t = 0; t < 2; ++t)   // until the run's counts come back, every row it may reach counts as written
+        b->hwm[t] = std::max(b->hwm[t], cfg->max_samples);
+    if (b->bases_n < cfg->num_new_configs || b->bases_seed != cfg->seed) {
+        std::vector<float> bases((size_t)cfg->num_new_configs * kMaxDim);
+        for (int w = 0; w < cfg->num_new_configs; ++w) halton_bases(cfg->seed, w, bases.data() + (size_t)w * kMaxDim);
+        RR_TRY(hipMemcpyAsync(b->bases.p, bases.data(), bases.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        RR_TRY(hipStreamSynchronize(s));   // the staging vector goes out of scope
+        b->bases_seed = cfg->seed;
+        b->bases_n = cfg->num_new_configs;
+    }
+    RR_TRY(hipMemcpyAsync(io + roots_off, h_roots, roots_bytes, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_rrtc_init, dim3(1), dim3(256), 0, s, A);
+    RR_TRY(hipGetLastError());
     RR_TRY(hipEventRecord(b->ev[0], s));
     switch (G) {
         case 64: hipLaunchKernelGGL(k_rrtc<64>, dim3(cfg->num_new_configs), dim3(kThreads), lds, s, A); break;
@@ -928,50 +996,50 @@ ppl_status ppl_rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const fl
     RR_TRY(hipEventRecord(b->ev[1], s));
     hipLaunchKernelGGL(k_rrtc_path, dim3(1), dim3(64), 0, s, A);
     RR_TRY(hipGetLastError());
-    RR_TRY(hipMemcpyAsync(hc, A.ctl, sizeof *hc, hipMemcpyDeviceToHost, s));
+    RR_TRY(hipMemcpyAsync(b->h_io, io, head_bytes, hipMemcpyDeviceToHost, s));
     RR_TRY(hipStreamSynchronize(s));
+    const RrtcCtl* hc = reinterpret_cast<const RrtcCtl*>(b->h_io);
     float kms = 0.0f;
     RR_TRY(hipEventElapsedTime(&kms, b->ev[0], b->ev[1]));
+    for (int t = 0; t < 2; ++t)
+        if (b->hwm[t] <= cfg->max_samples) b->hwm[t] = std::min(std::max(hc->free_idx[t], 1), cfg->max_samples);
     res->kernel_ms = kms;
     res->start_tree_size = std::min(hc->free_idx[0], cfg->max_samples);
     res->goal_tree_size = std::min(hc->free_idx[1], cfg->max_samples);
     res->solved = hc->status == 1 && hc->path_ok;
     res->iters = res->solved ? hc->win_iter : hc->max_iter;
     if (res->solved) {
-        const int n0 = hc->path_n[0], n1 = hc->path_n[1];
-        std::vector<float> p0((size_t)n0 * dim), p1((size_t)n1 * dim);
-        if (n0) RR_TRY(hipMemcpyAsync(p0.data(), A.path, p0.size() * sizeof(float), hipMemcpyDeviceToHost, s));
-        if (n1)
-            RR_TRY(hipMemcpyAsync(p1.data(), A.path + (size_t)kPathCap * dim, p1.size() * sizeof(float),
-                                  hipMemcpyDeviceToHost, s));
-        RR_TRY(hipStreamSynchronize(s));
-        // goal -> start, as the reference assembles it (pRRTC.cu:2167-2186)
-        std::vector<float> g2s;
-        g2s.reserve((size_t)(n0 + n1 + 2) * dim);
-        const float* goal = h_goals + (size_t)hc->reached_goal * dim;
-        g2s.insert(g2s.end(), goal, goal + dim);
-        for (int i = n1 - 1; i >= 0; --i) g2s.insert(g2s.end(), p1.begin() + (size_t)i * dim, p1.begin() + (size_t)(i + 1) * dim);
-        for (int i = 0; i < n0; ++i) g2s.insert(g2s.end(), p0.begin() + (size_t)i * dim, p0.begin() + (size_t)(i + 1) * dim);
-        g2s.insert(g2s.end(), h_start, h_start + dim);
+        int nw = hc->n_waypoints;
+        std::vector<float> path((size_t)nw * dim);
+        const float* head = reinterpret_cast<const float*>(b->h_io + 256);
+        std::memcpy(path.data(), head, (size_t)std::min(nw, kHeadRows) * dim * sizeof(float));
+        if (nw > kHeadRows) {
+            RR_TRY(hipMemcpyAsync(path.data() + (size_t)kHeadRows * dim, A.out + (size_t)kHeadRows * dim,
+                                  (size_t)(nw - kHeadRows) * dim * sizeof(float), hipMemcpyDeviceToHost, s));
+            RR_TRY(hipStreamSynchronize(s));
+        }
         res->cost = hc->cost;
-        res->path_length = n0 + n1;
+        res->path_length = nw - 2;
         res->reached_goal = hc->reached_goal;
-        int nw = n0 + n1 + 2;
         if (cfg->shortcut_path && nw > 2) {
-            std::vector<float> sc;
+            // from the goal end, as the reference shortcuts its goal -> start path (pRRTC.cu:2190-2200)
+            std::vector<float> g2s((size_t)nw * dim), sc;
+            for (int i = 0; i < nw; ++i)
+                std::memcpy(g2s.data() + (size_t)i * dim, path.data() + (size_t)(nw - 1 - i) * dim, (size_t)dim * sizeof(float));
             st = shortcut_run(robot, scene, g2s.data(), nw, dim, cfg->valid_segment_length,
                               cfg->collision_margin + cfg->shortcut_collision_margin, sc, b->sc, s);
             if (st != PPL_OK) return st;
-            g2s.swap(sc);
-            nw = (int)(g2s.size() / dim);
+            nw = (int)(sc.size() / dim);
+            path.resize((size_t)nw * dim);
+            for (int i = 0; i < nw; ++i)
+                std::memcpy(path.data() + (size_t)i * dim, sc.data() + (size_t)(nw - 1 - i) * dim, (size_t)dim * sizeof(float));
         }
         res->n_waypoints = nw;
         if (nw > max_waypoints) {
             res->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
             return ppl::fail_msg(PPL_ERR_CAPACITY, "the solution has more waypoints than max_waypoints");
         }
-        for (int i = 0; i < nw; ++i)   // start -> goal (python/bindings.cpp:260-272)
-            std::memcpy(h_path + (size_t)i * dim, g2s.data() + (size_t)(nw - 1 - i) * dim, (size_t)dim * sizeof(float));
+        std::memcpy(h_path, path.data(), path.size() * sizeof(float));
     }
     res->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
     return PPL_OK;

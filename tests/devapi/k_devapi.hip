@@ -15,10 +15,14 @@ __global__ __launch_bounds__(64) void k_devapi(ppl_robot_device_view R, ppl_scen
     const int tid = threadIdx.x;
     const int slot = blockIdx.x * (64 / G) + tid / G;   // one scratch slot per group, padded grid
     const int cfg = slot < n ? slot : n - 1;            // tail groups recompute the last configuration
-    float* pos = scratch + (size_t)slot * 3 * (R.n_spheres + R.n_approx_spheres);
+    float* pos = scratch + (size_t)slot * (3 * (R.n_spheres + R.n_approx_spheres) + 12 * R.n_joints);
     float* apos = pos + 3 * R.n_spheres;
+    float* frames = apos + 3 * R.n_approx_spheres;
     const float* qc = q + (size_t)cfg * R.n_dof;
-    const TwoPhaseResult r = two_phase_cc<G>(R, S, qc, pos, apos, tid, margin, check_self != 0);
+    // check_self bit 0: self check; bit 1: the one-walk overload with joint frames
+    const TwoPhaseResult r = (check_self & 2)
+                                 ? two_phase_cc<G>(R, S, qc, pos, apos, frames, tid, margin, (check_self & 1) != 0)
+                                 : two_phase_cc<G>(R, S, qc, pos, apos, tid, margin, (check_self & 1) != 0);
     fk_runtime<G>(R, qc, pos, tid);
     const float mc = scene_min_clearance_runtime<G>(R, pos, S, tid, margin);
     const float ho = scene_sum_obb_hinge_runtime<G>(R, pos, S, tid, world_margin);
@@ -63,7 +67,7 @@ int ppl_devtest_halton(const float* d_bases, int dim, int n_steps, float* d_out,
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
-// Scratch: (ceil(n / (64 / G)) * (64 / G)) * 3 * (n_spheres + n_approx_spheres) floats.
+// Scratch: (ceil(n / (64 / G)) * (64 / G)) * (3 * (n_spheres + n_approx_spheres) + 12 * n_joints) floats.
 int ppl_devtest_run(const ppl_robot_device_view* R, const ppl_scene_view* S, const float* d_q, int n, int G,
                     float margin, int check_self, float world_margin, float self_margin, uint8_t* d_hit,
                     float* d_minclr, float* d_hinge_obb, float* d_hinge_esdf, float* d_self_hinge, float* d_scratch,
@@ -76,6 +80,10 @@ int ppl_devtest_run(const ppl_robot_device_view* R, const ppl_scene_view* S, con
                               d_hinge_obb, d_hinge_esdf, d_self_hinge, d_scratch, s); break;
         case 4: e = launch<4>(*R, *S, d_q, n, margin, check_self, world_margin, self_margin, d_hit, d_minclr,
                               d_hinge_obb, d_hinge_esdf, d_self_hinge, d_scratch, s); break;
+        case 8: e = launch<8>(*R, *S, d_q, n, margin, check_self, world_margin, self_margin, d_hit, d_minclr,
+                              d_hinge_obb, d_hinge_esdf, d_self_hinge, d_scratch, s); break;
+        case 32: e = launch<32>(*R, *S, d_q, n, margin, check_self, world_margin, self_margin, d_hit, d_minclr,
+                                d_hinge_obb, d_hinge_esdf, d_self_hinge, d_scratch, s); break;
         case 16: e = launch<16>(*R, *S, d_q, n, margin, check_self, world_margin, self_margin, d_hit, d_minclr,
                                 d_hinge_obb, d_hinge_esdf, d_self_hinge, d_scratch, s); break;
         case 64: e = launch<64>(*R, *S, d_q, n, margin, check_self, world_margin, self_margin, d_hit, d_minclr,

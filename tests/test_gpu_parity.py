@@ -489,8 +489,9 @@ def test_prrtc_compatible_surface():
 DEVTEST = Path(__file__).resolve().parent / "devapi" / "_build" / "libppl_devtest.so"
 
 
-@pytest.mark.parametrize("name,G", [("panda", 4), ("fetch", 16), ("baxter", 1), ("panda", 64)])
-def test_device_inline_api(robots, name, G):
+@pytest.mark.parametrize("name,G,frames", [("panda", 4, 0), ("fetch", 16, 0), ("baxter", 1, 0), ("panda", 64, 0),
+                                            ("panda", 16, 1), ("fetch", 4, 1), ("baxter", 32, 1)])
+def test_device_inline_api(robots, name, G, frames):
     """A planner-style kernel built only on include/ppl/ppl_device.hpp:
     two_phase_cc (+ self), fk_runtime, scene_min_clearance_runtime,
     scene_sum_{obb,esdf}_hinge_runtime and self_min_hinge_runtime, one G-lane
@@ -504,7 +505,7 @@ def test_device_inline_api(robots, name, G):
     rng = np.random.default_rng(43)
     q = W.uniform_configs(rng, r, 1500)
     n = len(q)
-    banner(f"device API {name} G={G} n={n}")
+    banner(f"device API {name} G={G} n={n} frames-overload={frames}")
     ds = S.DeviceScene.from_host(scene)
     view = ds.view()
     dv = _ppl.RobotDeviceView()
@@ -514,12 +515,13 @@ def test_device_inline_api(robots, name, G):
     dq = torch.from_numpy(q).to(DEV)
     hit = torch.zeros(n, dtype=torch.uint8, device=DEV)
     outs = [torch.zeros(n, dtype=torch.float32, device=DEV) for _ in range(4)]
-    scratch = torch.zeros(slots * 3 * (r.n_spheres + r.n_approx), dtype=torch.float32, device=DEV)
+    scratch = torch.zeros(slots * (3 * (r.n_spheres + r.n_approx) + 12 * r.n_joints), dtype=torch.float32, device=DEV)
     vp = C.c_void_p
     lib.ppl_devtest_run.restype = C.c_int
     lib.ppl_devtest_run.argtypes = [C.POINTER(_ppl.RobotDeviceView), C.POINTER(_ppl.SceneView), vp, C.c_int, C.c_int,
                                     C.c_float, C.c_int, C.c_float, C.c_float, vp, vp, vp, vp, vp, vp, vp]
-    st = lib.ppl_devtest_run(C.byref(dv), C.byref(view), dq.data_ptr(), n, G, 0.0, 1, 0.03, 0.01, hit.data_ptr(),
+    st = lib.ppl_devtest_run(C.byref(dv), C.byref(view), dq.data_ptr(), n, G, 0.0, 1 | (2 * frames), 0.03, 0.01,
+                             hit.data_ptr(),
                              *[o.data_ptr() for o in outs], scratch.data_ptr(), None)
     assert st == 0
     torch.cuda.synchronize()
