@@ -415,58 +415,77 @@ __global__ __launch_bounds__(kThreads) void k_rrtc(RrtcArgs A) {
 // Walk both trees from the connection to their roots (pRRTC.cu:1962-1988):
 // path[tree] lists the walked nodes (roots excluded), cost sums the walked
 // edges' L2 lengths, the solving tree first. A walk that leaves [0, free) or
-// does not descend to smaller indices leaves path_ok = 0.
+// does not descend to smaller indices leaves path_ok = 0. Lane 0 walks; the
+// whole wave then writes the assembled start -> goal rows.
 __global__ void k_rrtc_path(RrtcArgs A) {
-    if (threadIdx.x != 0) return;
+    __shared__ int s_ok, s_n0, s_n1, s_goal;
     RrtcCtl* c = A.ctl;
-    c->path_ok = 0;
-    c->path_n[0] = c->path_n[1] = 0;
-    c->reached_goal = 0;
-    c->cost = 0.0f;
-    c->n_waypoints = 0;
-    if (c->status != 1) return;
     const int dim = A.dim;
-    float cost = 0.0f;
-    for (int side = 0; side < 2; ++side) {
-        const int tree = side == 0 ? c->win_tree : 1 - c->win_tree;
-        int cur = side == 0 ? c->win_t : c->win_o;
-        const float* N = A.nodes[tree];
-        const int* P = A.parents[tree];
-        const int lim = min(c->free_idx[tree], A.max_samples);
-        float* out = A.path + (size_t)tree * kPathCap * dim;
-        int n = 0;
-        for (;;) {
-            if (cur < 0 || cur >= lim) return;
-            const int p = P[cur];
-            if (p == cur) break;
-            if (p < 0 || p > cur || n >= kPathCap) return;
-            float s = 0.0f;
-            for (int k = 0; k < dim; ++k) {
-                const float d = N[(size_t)cur * dim + k] - N[(size_t)p * dim + k];
-                s = fmaf(d, d, s);
-                out[(size_t)n * dim + k] = N[(size_t)cur * dim + k];
+    if (threadIdx.x == 0) {
+        s_ok = 0;
+        c->path_ok = 0;
+        c->path_n[0] = c->path_n[1] = 0;
+        c->reached_goal = 0;
+        c->cost = 0.0f;
+        c->n_waypoints = 0;
+        float cost = 0.0f;
+        bool ok = c->status == 1;
+        for (int side = 0; side < 2 && ok; ++side) {
+            const int tree = side == 0 ? c->win_tree : 1 - c->win_tree;
+            int cur = side == 0 ? c->win_t : c->win_o;
+            const float* N = A.nodes[tree];
+            const int* P = A.parents[tree];
+            const int lim = min(c->free_idx[tree], A.max_samples);
+            float* out = A.path + (size_t)tree * kPathCap * dim;
+            int n = 0;
+            for (;;) {
+                if (cur < 0 || cur >= lim) {
+                    ok = false;
+                    break;
+                }
+                const int p = P[cur];
+                if (p == cur) break;
+                if (p < 0 || p > cur || n >= kPathCap) {
+                    ok = false;
+                    break;
+                }
+                float s = 0.0f;
+                for (int k = 0; k < dim; ++k) {
+                    const float d = N[(size_t)cur * dim + k] - N[(size_t)p * dim + k];
+                    s = fmaf(d, d, s);
+                    out[(size_t)n * dim + k] = N[(size_t)cur * dim + k];
+                }
+                cost += sqrtf(s);
+                ++n;
+                cur = p;
             }
-            cost += sqrtf(s);
-            ++n;
-            cur = p;
+            c->path_n[tree] = n;
+            if (tree == 1) c->reached_goal = cur;
         }
-        c->path_n[tree] = n;
-        if (tree == 1) c->reached_goal = cur;
+        if (ok) {
+            s_ok = 1;
+            s_n0 = c->path_n[0];
+            s_n1 = c->path_n[1];
+            s_goal = c->reached_goal;
+            c->n_waypoints = s_n0 + s_n1 + 2;
+            c->cost = cost;
+            c->path_ok = 1;
+        }
     }
+    __syncthreads();
+    if (!s_ok) return;
     // start, the start-tree walk reversed, the goal-tree walk, the goal reached
     // (pRRTC.cu:2167-2186 builds goal -> start; python/bindings.cpp:260-272 reverses it)
-    const int n0 = c->path_n[0], n1 = c->path_n[1];
-    float* o = A.out;
-    for (int k = 0; k < dim; ++k) o[k] = A.roots[k];
-    for (int i = 0; i < n0; ++i)
-        for (int k = 0; k < dim; ++k) o[(size_t)(1 + i) * dim + k] = A.path[(size_t)(n0 - 1 - i) * dim + k];
-    for (int i = 0; i < n1; ++i)
-        for (int k = 0; k < dim; ++k)
-            o[(size_t)(1 + n0 + i) * dim + k] = A.path[(size_t)(kPathCap + i) * dim + k];
-    for (int k = 0; k < dim; ++k) o[(size_t)(1 + n0 + n1) * dim + k] = A.roots[(size_t)(1 + c->reached_goal) * dim + k];
-    c->n_waypoints = n0 + n1 + 2;
-    c->cost = cost;
-    c->path_ok = 1;
+    const int n0 = s_n0, n1 = s_n1, rows = n0 + n1 + 2;
+    for (int e = threadIdx.x; e < rows * dim; e += blockDim.x) {
+        const int r = e / dim, k = e - r * dim;
+        float v;
+        if (r == 0) v = A.roots[k];
+        else if (r <= n0) v = A.path[(size_t)(n0 - r) * dim + k];
+        else if (r <= n0 + n1) v = A.path[(size_t)(kPathCap + r - 1 - n0) * dim + k];
+        else v = A.roots[(size_t)(1 + s_goal) * dim + k];
+        A.out[(size_t)r * dim + k] = v;
+    }
 }
 
 // Fresh run state: the control block, the roots (start in tree 0, goals in

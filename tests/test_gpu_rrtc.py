@@ -203,15 +203,24 @@ def test_shortcut_matches_oracle(robots, name):
     p2 = res2["path"]
     np.testing.assert_array_equal(p2[0], start)
     np.testing.assert_array_equal(p2[-1], goals[res2["reached_goal"]])
+    # a kept segment is either a skip edge the pass verified (interior states at
+    # valid_segment_length, margin + shortcut margin) or an original tree edge the
+    # planner verified (its granularity states at the planner's margin)
     m = st.collision_margin + st.shortcut_collision_margin
-    bad = 0
+    bad, skips = 0, 0
     for a, b in zip(p2[:-1], p2[1:]):
         states = RO.segment_interior(a, b, st.valid_segment_length)
+        skip_ok = True
         if len(states):
             hit, clr = o.check_collision(states, margin=m, check_self=True)
-            bad += int((hit & (np.abs(clr) >= BAND)).sum())
-    banner(f"solve with shortcut_path: {res2['path_length'] + 2} -> {len(p2)} waypoints, "
-           f"{bad} colliding interior states")
+            skip_ok = not (hit & (np.abs(clr) >= BAND)).any()
+        hit, clr = o.check_collision(RO.segment_states(a, b, st.granularity), margin=st.collision_margin,
+                                     check_self=True)
+        edge_ok = not (hit & (np.abs(clr) >= BAND)).any()
+        skips += int(skip_ok)
+        bad += int(not (skip_ok or edge_ok))
+    banner(f"solve with shortcut_path: {res2['path_length'] + 2} -> {len(p2)} waypoints, {skips} segments free at "
+           f"the shortcut resolution, {bad} segments free under neither check")
     assert len(p2) <= res2["path_length"] + 2
     assert bad == 0
 
