@@ -286,7 +286,8 @@ def bench_any(panda, S, W, torch, dev, args):
 def bench_edges_plain(panda, e, ds, S, torch, dev, args):
     """64 plain EdgeEvalGpu calls of 256 edges (states j=1..G, G from 0.02 rad
     resolution) vs the same edges in one batched launch (first-hit skipping on
-    in both)."""
+    in both); also the latency of the 14-edge calls an eager MHA* / wPA*SE
+    expansion makes (2 * dim successors at +-0.08 rad)."""
     import numpy as np
     ev = S.EdgeEvaluator(panda, ds, max_edges=256)
     Gs, _ = S.edge_granularities(e.frm, e.to, resolution=0.02)
@@ -307,9 +308,22 @@ def bench_edges_plain(panda, e, ds, S, torch, dev, args):
                            args.steps, args.warmup, torch)
     batched_states = float(np.sum(Gs)) / wall
     same = bool(np.array_equal(np.concatenate(valid), fh.cpu().numpy() < 0))
+    # MHA* / wPA*SE eager expansion: one synchronous call per state for its 2 * dim successor edges
+    q0 = e.frm[:64]
+    succ_to = np.repeat(q0, 14, axis=0).reshape(64, 14, 7)
+    for d in range(7):
+        succ_to[:, 2 * d, d] += 0.08
+        succ_to[:, 2 * d + 1, d] -= 0.08
+    succ_frm = np.repeat(q0, 14, axis=0).reshape(64, 14, 7)
+    for k in range(4):
+        ev.evaluate(succ_frm[k], succ_to[k], resolution=0.02)
+    t0 = time.perf_counter()
+    for k in range(64):
+        ev.evaluate(succ_frm[k], succ_to[k], resolution=0.02)
+    dt14 = time.perf_counter() - t0
     return {"plain_states_per_s": plain_states, "batched_states_per_s": batched_states,
             "batched_over_plain": batched_states / plain_states, "verdicts_equal": same,
-            "plain_ms_per_256_edge_call": dt / 64 * 1e3}
+            "plain_ms_per_256_edge_call": dt / 64 * 1e3, "plain_ms_per_14_edge_call": dt14 / 64 * 1e3}
 
 
 def bench_chomp(panda, S, W, torch, dev, args):

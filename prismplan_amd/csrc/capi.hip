@@ -110,8 +110,23 @@ struct ppl_edge_eval_s {
     DScene S{};
     int dim = 0, max_edges = 0;
     hipStream_t stream = nullptr;
-    DBuf d_from, d_to, d_off, d_fh, d_clr_state, d_clr_edge, d_self;
-    HBuf h_from, h_to, h_off, h_fh, h_clr;
+    // one call's inputs and first hits, packed [offsets][first hits][from][to]
+    // (256-byte aligned parts) in one pinned block and one device block
+    DBuf d_io, d_clr_state, d_clr_edge, d_self;
+    HBuf h_io, h_clr;
+};
+
+// Byte offsets of the packed parts for n edges of dim joints.
+struct EdgeIoLayout {
+    size_t fh, from, to, end;
+    EdgeIoLayout(int n, int dim) {
+        auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        const size_t cfg = (size_t)n * dim * sizeof(float);
+        fh = up((size_t)(n + 1) * 4);
+        from = fh + up((size_t)n * 4);
+        to = from + up(cfg);
+        end = to + cfg;
+    }
 };
 
 // STOMP optimiser state kept with a trajectory plan between calls: rollout
@@ -785,16 +800,10 @@ ppl_status ppl_edge_eval_create(ppl_robot r, const ppl_scene_view* view, int dim
     ev->S = S;
     ev->dim = dim;
     ev->max_edges = max_edges;
-    const size_t cfg = (size_t)max_edges * dim * sizeof(float);
+    const EdgeIoLayout io(max_edges, dim);
     hipError_t e = hipStreamCreateWithFlags(&ev->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = ev->d_from.ensure(cfg);
-    if (e == hipSuccess) e = ev->d_to.ensure(cfg);
-    if (e == hipSuccess) e = ev->d_off.ensure((size_t)(max_edges + 1) * 4);
-    if (e == hipSuccess) e = ev->d_fh.ensure((size_t)max_edges * 4);
-    if (e == hipSuccess) e = ev->h_from.ensure(cfg);
-    if (e == hipSuccess) e = ev->h_to.ensure(cfg);
-    if (e == hipSuccess) e = ev->h_off.ensure((size_t)(max_edges + 1) * 4);
-    if (e == hipSuccess) e = ev->h_fh.ensure((size_t)max_edges * 4);
+    if (e == hipSuccess) e = ev->d_io.ensure(io.end);
+    if (e == hipSuccess) e = ev->h_io.ensure(io.end);
     if (e == hipSuccess) e = ev->h_clr.ensure((size_t)max_edges * 4);
     if (e != hipSuccess) {
         ppl_edge_eval_destroy(ev);
@@ -807,9 +816,8 @@ ppl_status ppl_edge_eval_create(ppl_robot r, const ppl_scene_view* view, int dim
 ppl_status ppl_edge_eval_destroy(ppl_edge_eval ev) {
     if (!ev) return PPL_OK;
     if (ev->stream) (void)hipStreamSynchronize(ev->stream);
-    ev->d_from.release(); ev->d_to.release(); ev->d_off.release(); ev->d_fh.release();
-    ev->d_clr_state.release(); ev->d_clr_edge.release(); ev->d_self.release();
-    ev->h_from.release(); ev->h_to.release(); ev->h_off.release(); ev->h_fh.release(); ev->h_clr.release();
+    ev->d_io.release(); ev->d_clr_state.release(); ev->d_clr_edge.release(); ev->d_self.release();
+    ev->h_io.release(); ev->h_clr.release();
     if (ev->stream) (void)hipStreamDestroy(ev->stream);
     delete ev;
     return PPL_OK;
@@ -853,7 +861,10 @@ ppl_status ppl_edge_eval_evaluate(ppl_edge_eval ev, const float* h_from, const f
     if (p->enable_mesh) return fail(PPL_ERR_UNSUPPORTED, "mesh edge checks are not supported");
     ppl_robot r = ev->robot;
     const int D = ev->dim;
-    int* off = ev->h_off.as<int>();
+    const EdgeIoLayout io(n, D);
+    char* hio = ev->h_io.as<char>();
+    char* dio = ev->d_io.as<char>();
+    int* off = reinterpret_cast<int*>(hio);
     std::vector<float> len(n);
     off[0] = 0;
     for (int e = 0; e < n; ++e) {
@@ -862,18 +873,19 @@ ppl_status ppl_edge_eval_evaluate(ppl_edge_eval ev, const float* h_from, const f
     }
     const long total = off[n];
     const size_t cfg = (size_t)n * D * sizeof(float);
-    memcpy(ev->h_from.p, h_from, cfg);
-    memcpy(ev->h_to.p, h_to, cfg);
-    HIP_TRY(hipMemcpyAsync(ev->d_from.p, ev->h_from.p, cfg, hipMemcpyHostToDevice, ev->stream));
-    HIP_TRY(hipMemcpyAsync(ev->d_to.p, ev->h_to.p, cfg, hipMemcpyHostToDevice, ev->stream));
-    HIP_TRY(hipMemcpyAsync(ev->d_off.p, ev->h_off.p, (size_t)(n + 1) * 4, hipMemcpyHostToDevice, ev->stream));
-    HIP_TRY(launch_fill_i32(ev->d_fh.as<int>(), -1, n, ev->stream));
+    int* h_fh = reinterpret_cast<int*>(hio + io.fh);
+    for (int e = 0; e < n; ++e) h_fh[e] = -1;   // first hits start at "none"
+    memcpy(hio + io.from, h_from, cfg);
+    memcpy(hio + io.to, h_to, cfg);
+    HIP_TRY(hipMemcpyAsync(dio, hio, io.end, hipMemcpyHostToDevice, ev->stream));
+    int* d_off = reinterpret_cast<int*>(dio);
+    int* d_fh = reinterpret_cast<int*>(dio + io.fh);
     DStates X{};
     X.source = kSrcEdge;
     X.n_states = total;
-    X.from = ev->d_from.as<float>();
-    X.to = ev->d_to.as<float>();
-    X.edge_off = ev->d_off.as<int>();
+    X.from = reinterpret_cast<const float*>(dio + io.from);
+    X.to = reinterpret_cast<const float*>(dio + io.to);
+    X.edge_off = d_off;
     X.n_edges = n;
     X.k0 = 1;
     // With G = granularity <= 1 the +1 state count gives t = k/G as the reference.
@@ -894,7 +906,7 @@ ppl_status ppl_edge_eval_evaluate(ppl_edge_eval ev, const float* h_from, const f
         a.margin = p->collision_margin;
         a.mode = 1;
         a.skip_resolved = 1;
-        a.first_hit = ev->d_fh.as<int>();
+        a.first_hit = d_fh;
         a.self_hit = self_hit;
         HIP_TRY(launch_collide(a, grid, ev->stream));
     }
@@ -914,14 +926,14 @@ ppl_status ppl_edge_eval_evaluate(ppl_edge_eval ev, const float* h_from, const f
         c.margin = p->collision_margin;
         c.cost_out = ev->d_clr_state.as<float>();
         HIP_TRY(launch_cost(c, grid, ev->stream));
-        hipLaunchKernelGGL(k_segment_min, dim3(n), dim3(64), 0, ev->stream, ev->d_clr_state.as<float>(),
-                           ev->d_off.as<int>(), n, ev->d_clr_edge.as<float>());
+        hipLaunchKernelGGL(k_segment_min, dim3(n), dim3(64), 0, ev->stream, ev->d_clr_state.as<float>(), d_off, n,
+                           ev->d_clr_edge.as<float>());
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(ev->h_clr.p, ev->d_clr_edge.p, (size_t)n * 4, hipMemcpyDeviceToHost, ev->stream));
     }
-    HIP_TRY(hipMemcpyAsync(ev->h_fh.p, ev->d_fh.p, (size_t)n * 4, hipMemcpyDeviceToHost, ev->stream));
+    HIP_TRY(hipMemcpyAsync(h_fh, d_fh, (size_t)n * 4, hipMemcpyDeviceToHost, ev->stream));
     HIP_TRY(hipStreamSynchronize(ev->stream));
-    const int* fh = ev->h_fh.as<int>();
+    const int* fh = h_fh;
     for (int e = 0; e < n; ++e) {
         const bool ok = fh[e] < 0;
         h_valid[e] = ok ? 1 : 0;

@@ -221,6 +221,57 @@ def test_plain_edge_evaluator(robots):
     assert np.mean(valid == rv) > 0.995
 
 
+def test_edge_eval_flush_pattern(robots):
+    """The MHA* / wPA*SE callers' use of EdgeEvalGpu (mhastar_search.hh:256-345,
+    wpase_search.hh:192-240): one evaluator, many synchronous calls of varying
+    size -- lazy flushes of the pending batch (up to batch_edges) and eager
+    expansions of 2 * dim successor edges from one state -- each answered
+    before the next, and an over-size batch refused without side effects."""
+    e = W.edges(n_edges=400, n_obbs=16, seed=42)
+    ds = S.DeviceScene.from_host(e.scene)
+    ev = S.EdgeEvaluator(robots["panda"], ds, max_edges=256)
+    o = PO.Oracle(e.robot, e.scene, nthreads=8)
+    rng = np.random.default_rng(43)
+    at, calls, bad = 0, 0, 0
+    sizes = [1, 14, 2, 256, 14, 7, 64, 14, 1, 30]
+    banner(f"EdgeEvalGpu flush pattern: {len(sizes)} calls of {sizes} edges, max_edges 256")
+    for m in sizes:
+        if m == 14:   # eager expansion: 2 * dim successors of one state at +-step along each joint
+            q = e.frm[rng.integers(len(e.frm))]
+            step = np.float32(0.08)
+            frm = np.repeat(q[None], 14, axis=0)
+            to = frm.copy()
+            for d in range(7):
+                to[2 * d, d] += step
+                to[2 * d + 1, d] -= step
+            to = np.clip(to, e.robot.joint_lower, e.robot.joint_upper).astype(np.float32)
+        else:
+            idx = (np.arange(m) + at) % len(e.frm)
+            at += m
+            frm, to = e.frm[idx], e.to[idx]
+        for check_self in (False, True):
+            valid, cost, _ = ev.evaluate(frm, to, granularity=32, resolution=0.025, check_self=check_self)
+            rv, rc, _ = o.edge_eval(frm, to, granularity=32, resolution=0.025, check_self=check_self)
+            for k in np.nonzero(valid != rv)[0]:
+                G, _ = PO.edge_granularity(frm[k], to[k], 32, 0.025)
+                t = np.arange(1, G + 1, dtype=np.float32) / np.float32(G)
+                st = (frm[k] + t[:, None] * (to[k] - frm[k])).astype(np.float32)
+                _, clr = o.check_collision(st, check_self=check_self)
+                bad += int(not np.any(np.abs(clr) < BAND))
+            ok = valid == rv
+            np.testing.assert_array_equal(cost[ok], rc[ok])
+            calls += 1
+    assert bad == 0, f"{bad} verdict mismatches outside the ambiguous band"
+    with pytest.raises(_ppl.PplError) as err:
+        ev.evaluate(e.frm[:257], e.to[:257], resolution=0.025)
+    assert err.value.status in (_ppl.PPL_ERR_INVALID, _ppl.PPL_ERR_CAPACITY)
+    # the evaluator still answers after the refused call
+    valid, cost, _ = ev.evaluate(e.frm[:5], e.to[:5], granularity=32, resolution=0.025)
+    rv, rc, _ = o.edge_eval(e.frm[:5], e.to[:5], granularity=32, resolution=0.025)
+    assert np.mean(valid == rv) >= 0.8
+    banner(f"  {calls} calls checked against the oracle")
+
+
 def test_plain_edge_clearance_with_esdf(robots):
     s, _ = esdf_scene(interp=0, with_obbs=False)
     e = W.edges(n_edges=200, n_obbs=1, seed=41)
