@@ -95,7 +95,7 @@ def main():
     legs = set(x for x in args.legs.split(",") if x)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the substrate shards by "
-              f"independent requests, see DESIGN.md).", file=sys.stderr)
+              f"independent requests: each GPU would take whole problems, edges or trajectories).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
