@@ -1578,34 +1578,49 @@ __global__ void k_bfs_count(const float* nodes, int dim, const int* src, const i
 }
 
 // off = exclusive scan of cnt (off[n] = total): one workgroup of 1024 threads
-// walks tiles of 1024 counts with wave scans. Offsets are only used when the
-// 64-bit total (*total64, summed separately) is below 2^31, so every partial
-// sum then fits 32 bits.
+// takes passes of 16,384 counts, 16 consecutive counts per thread loaded
+// together (one memory latency per pass), scanned in registers, then across
+// the workgroup by wave scans of the thread totals. Offsets are only used when
+// the 64-bit total (*total64, summed separately) is below 2^31, so every
+// partial sum then fits 32 bits.
+constexpr int kScanPer = 16;
 __global__ __launch_bounds__(1024) void k_scan_i32(const int* cnt, int n, int* off, long long* total64) {
     __shared__ int wsum[16];
     __shared__ long long part[16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int carry = 0;
     long long acc = 0;
-    for (int base = 0; base < n; base += 1024) {
-        const int i = base + tid;
-        const int c = i < n ? cnt[i] : 0;
-        acc += c;
-        int v = c;
+    for (int base = 0; base < n; base += 1024 * kScanPer) {
+        const int i0 = base + tid * kScanPer;
+        int v[kScanPer];
+        int local = 0;
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) v[k] = i0 + k < n ? cnt[i0 + k] : 0;
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            local += v[k];
+            acc += v[k];
+        }
+        int x = local;
 #pragma unroll
         for (int o = 1; o < kWave; o <<= 1) {
-            const int t = __shfl_up(v, o);
-            if (lane >= o) v += t;
+            const int t = __shfl_up(x, o);
+            if (lane >= o) x += t;
         }
-        if (lane == kWave - 1) wsum[wave] = v;
+        if (lane == kWave - 1) wsum[wave] = x;
         __syncthreads();
         int before = 0, tile = 0;
         for (int w = 0; w < 16; ++w) {
-            const int x = wsum[w];
-            before += w < wave ? x : 0;
-            tile += x;
+            const int y = wsum[w];
+            before += w < wave ? y : 0;
+            tile += y;
         }
-        if (i < n) off[i] = carry + before + v - c;
+        int run = carry + before + x - local;
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            if (i0 + k < n) off[i0 + k] = run;
+            run += v[k];
+        }
         carry += tile;
         __syncthreads();
     }
