@@ -540,6 +540,24 @@ def test_prrtc_compatible_surface():
 DEVTEST = Path(__file__).resolve().parent / "devapi" / "_build" / "libppl_devtest.so"
 
 
+def devapi_scene(name):
+    """An ESDF + OBB world in each robot's reach where its configurations
+    collide some of the time (Panda: the shared ESDF world; Fetch and Baxter:
+    obstacles moved clear of their bases)."""
+    if name == "panda":
+        return esdf_scene(interp=0, seed=41)[0]
+    center, lo, hi = {"fetch": ((0.85, 0.0, 0.8), (0.6, -0.4, 0.5), (1.1, 0.4, 1.1)),
+                      "baxter": ((0.75, 0.0, 0.2), (0.5, -0.5, -0.1), (1.0, 0.5, 0.5))}[name]
+    rng = np.random.default_rng(41)
+    obbs = W.random_obbs(rng, 12, lo=lo, hi=hi, half_hi=0.12)
+    params, pose = W.esdf_grid(center=center, n=48, voxel=0.02)
+    vals = PO.esdf_from_obbs(obbs.obb_dims, obbs.obb_pose, params, pose).astype(np.float32)
+    extra = W.random_obbs(rng, 6, lo=lo, hi=hi, half_hi=0.1)
+    return HostScene(obb_dims=extra.obb_dims, obb_pose=extra.obb_pose, obb_enable=extra.obb_enable,
+                     voxel_params=params, voxel_pose=pose, voxel_enable=np.ones(1, np.uint8),
+                     voxel_features=vals.reshape(1, -1), esdf_interp=0)
+
+
 @pytest.mark.parametrize("name,G,frames", [("panda", 4, 0), ("fetch", 16, 0), ("baxter", 1, 0), ("panda", 64, 0),
                                             ("panda", 16, 1), ("fetch", 4, 1), ("baxter", 32, 1)])
 def test_device_inline_api(robots, name, G, frames):
@@ -552,7 +570,7 @@ def test_device_inline_api(robots, name, G, frames):
     lib = C.CDLL(str(DEVTEST))
     rob = robots[name]
     r = rob.arrays
-    scene, _ = esdf_scene(interp=0, seed=41)
+    scene = devapi_scene(name)
     rng = np.random.default_rng(43)
     q = W.uniform_configs(rng, r, 1500)
     n = len(q)
@@ -579,10 +597,13 @@ def test_device_inline_api(robots, name, G, frames):
     mc, ho, he, sh = [o.cpu().numpy().astype(np.float64) for o in outs]
     o = PO.Oracle(r, scene, nthreads=8)
     ref, clr = o.check_collision(q, check_self=True)
+    assert 0.05 < ref.mean() < 0.95, "the world should make some configurations collide and leave others free"
     assert_verdicts(hit.cpu().numpy(), ref, clr, f"device two_phase_cc {name}")
     ref_mc = o.min_clearance(q)
     np.testing.assert_allclose(mc, ref_mc, rtol=1e-5, atol=1e-5)
-    # cost tolerance 1e-5 relative, denominator max(|ref|, weight * margin) as in the other hinge tests
+    # cost tolerance 1e-5 relative, denominator max(|ref|, weight * margin) as in the other hinge tests. Sums
+    # of near-margin hinges over many spheres sit at the FP32 noise floor: there the bar is the FP32 CPU
+    # restatement's own worst error on the same inputs (x3), whichever is larger.
     for what, prm, got, scale in (("obb hinge", PO.cost_params(world_margin=0.03, use_obb=True, use_esdf=False,
                                                                use_self=False), ho, 0.03),
                                   ("esdf hinge", PO.cost_params(world_margin=0.03, use_obb=False, use_esdf=True,
@@ -591,8 +612,11 @@ def test_device_inline_api(robots, name, G, frames):
                                                                 use_self=True), sh, 0.01)):
         exp = o.hinge_cost(q, prm)
         err = np.abs(got - exp) / np.maximum(np.abs(exp), scale)
-        banner(f"  {what}: max rel err {err.max():.2e} (denominator max(|ref|, {scale}))")
-        assert err.max() < 1e-5, what
+        err32 = np.abs(o.hinge_cost(q, prm, precision="f32") - exp) / np.maximum(np.abs(exp), scale)
+        bar = max(1e-5, 3.0 * float(err32.max()))
+        banner(f"  {what}: max rel err {err.max():.2e}, FP32 restatement {err32.max():.2e} "
+               f"(denominator max(|ref|, {scale}), active {np.mean(exp > 0):.2f}, bar {bar:.2e})")
+        assert err.max() < bar, what
 
 
 def test_algorithmic_flop_counts(robots):
