@@ -82,11 +82,14 @@ def test_plain_check_collision_any_collision_workload(robots):
 def test_plain_check_collision_tree_and_prismatic(robots, name):
     rng = np.random.default_rng(21)
     r = robots[name].arrays
-    scene = W.random_obbs(rng, 25, lo=(-0.3, -1.0, 0.0), hi=(1.1, 1.0, 1.4), half_hi=0.15)
+    # boxes in front of the base and torso, so some configurations collide and some do not
+    lo, hi = ((0.45, -1.0, 0.3), (1.2, 1.0, 1.4)) if name == "fetch" else ((0.45, -1.0, -0.2), (1.2, 1.0, 0.9))
+    scene = W.random_obbs(rng, 25, lo=lo, hi=hi, half_hi=0.15)
     q = W.uniform_configs(rng, r, 3000)
     banner(f"plain check {name} N=3000")
     got = S.check_collision_scene(robots[name], S.DeviceScene.from_host(scene), q)
     ref, clr = PO.Oracle(r, scene, nthreads=8).check_collision(q)
+    assert 0.05 < ref.mean() < 0.95
     assert_verdicts(got, ref, clr, f"plain any-collision {name}")
 
 
@@ -129,12 +132,12 @@ def test_check_self(robots, name):
     assert_verdicts(got, ref, clr, f"any-collision+self {name}")
 
 
-def esdf_scene(n=48, vs=0.02, center=(0.45, 0.0, 0.45), interp=0, with_obbs=True, seed=30):
+def esdf_scene(n=48, vs=0.02, center=(0.45, 0.0, 0.45), interp=0, with_obbs=True, seed=30, extra_half=0.3):
     rng = np.random.default_rng(seed)
     obbs = W.random_obbs(rng, 12, lo=(0.2, -0.4, 0.1), hi=(0.7, 0.4, 0.8), half_hi=0.12)
     params, pose = W.esdf_grid(center=center, n=n, voxel=vs)
     vals = PO.esdf_from_obbs(obbs.obb_dims, obbs.obb_pose, params, pose).astype(np.float32)
-    extra = W.random_obbs(rng, 6) if with_obbs else HostScene()
+    extra = W.random_obbs(rng, 6, half_hi=extra_half) if with_obbs else HostScene()
     s = HostScene(obb_dims=extra.obb_dims, obb_pose=extra.obb_pose, obb_enable=extra.obb_enable,
                   voxel_params=params, voxel_pose=pose, voxel_enable=np.ones(1, np.uint8),
                   voxel_features=vals.reshape(1, -1), esdf_interp=interp)
@@ -143,7 +146,7 @@ def esdf_scene(n=48, vs=0.02, center=(0.45, 0.0, 0.45), interp=0, with_obbs=True
 
 @pytest.mark.parametrize("interp", [0, 1])
 def test_esdf_any_collision_and_min_clearance(robots, interp):
-    s, _ = esdf_scene(interp=interp)
+    s, _ = esdf_scene(interp=interp, extra_half=0.1)   # ~25 % of the configurations collide
     r = robots["panda"].arrays
     q = W.uniform_configs(np.random.default_rng(31), r, 3000)
     banner(f"ESDF any-collision interp={interp}")
