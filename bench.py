@@ -125,12 +125,17 @@ def main():
     wall, ev = timed_loop(step, args.steps, args.warmup, torch)
     value = total_states / wall
     log(f"headline: {wall * 1e3:.3f} ms/step wall, {ev * 1e3:.3f} ms event -> {value:.3e} states/s")
+    fh_host = fh.cpu().numpy()
+    hit_edges = fh_host >= 0
+    # the workload's 64 boxes of up to 0.6 m fill the Panda's reach: edges collide early
+    workload_stats = {"colliding_edge_frac": float(hit_edges.mean()),
+                      "first_hit_state_mean": float(fh_host[hit_edges].mean()) if hit_edges.any() else None}
 
     # ------------------------------------------------------------------ CPU baseline leg
     if args.no_cpu:
         cpu_baseline, parity, f_state = None, None, GATED_FLOPS_EDGES
     else:
-        cpu_baseline, parity, f_state = cpu_baseline_leg(e, fh.cpu().numpy(), args)
+        cpu_baseline, parity, f_state = cpu_baseline_leg(e, fh_host, args)
     achieved = f_state * total_states / ev
     from prismplan_amd import _ppl
     f_exh = panda.flops(_ppl.WORK_ANY_COLLISION, n_obbs=e.scene.n_obbs)
@@ -174,7 +179,8 @@ def main():
         "data": f"synthetic (seeded PCG64, sha256 {e.sha256()[:16]})",
         "config": {"workload": "edges: Panda 7-DoF, 64 random OBBs, 16384 straight edges, 8-256 states at <=0.02 rad "
                                "joint step, per-edge any-collision + first-hit, every state checked",
-                   "edges": int(len(e.n_states)), "states": int(total_states), "obbs": int(e.scene.n_obbs)},
+                   "edges": int(len(e.n_states)), "states": int(total_states), "obbs": int(e.scene.n_obbs),
+                   **workload_stats},
         "roofline": roofline, "cpu_baseline": cpu_baseline, "parity": parity,
         "device": props.name, **extras,
     }
