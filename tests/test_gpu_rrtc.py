@@ -38,7 +38,7 @@ def banner(msg):
 
 @pytest.fixture(scope="module")
 def robots():
-    return {n: S.Robot(n) for n in ("panda", "fetch")}
+    return {n: S.Robot(n) for n in ("panda", "fetch", "baxter")}
 
 
 def line_blocked(o, a, b, margin, n=64):
@@ -52,8 +52,10 @@ def problem(robot, seed, n_obbs=10, margin=0.0, n_goals=1):
     """A box world with a free start and free goals whose straight line is blocked."""
     r = robot.arrays
     rng = np.random.default_rng(seed)
-    # Fetch's base and torso occupy x < 0.4 m near the floor: its boxes start further out
-    lo, hi = ((0.5, -0.7, 0.3), (1.1, 0.7, 1.3)) if robot.n_dof == 8 else ((0.25, -0.6, 0.0), (0.9, 0.6, 1.1))
+    # Fetch's base and torso occupy x < 0.4 m near the floor, Baxter's pedestal and torso the same: their
+    # boxes start further out
+    lo, hi = {8: ((0.5, -0.7, 0.3), (1.1, 0.7, 1.3)), 14: ((0.5, -0.9, -0.1), (1.1, 0.9, 0.7))}.get(
+        robot.n_dof, ((0.25, -0.6, 0.0), (0.9, 0.6, 1.1)))
     scene = W.random_obbs(rng, n_obbs, lo=lo, hi=hi, half_lo=0.03, half_hi=0.12)
     o = PO.Oracle(r, scene, nthreads=8)
     q = W.uniform_configs(rng, r, 400)
@@ -100,7 +102,7 @@ def test_halton_device_matches_oracle():
     np.testing.assert_array_equal(out.cpu().numpy(), RO.halton_sequence(RO.PRIMES, n))
 
 
-@pytest.mark.parametrize("name,seed", [("panda", 1), ("panda", 2), ("fetch", 3)])
+@pytest.mark.parametrize("name,seed", [("panda", 1), ("panda", 2), ("fetch", 3), ("baxter", 4)])
 def test_rrtc_solution_is_valid(robots, name, seed):
     rob = robots[name]
     scene, o, start, goals = problem(rob, 100 + seed, margin=0.0)
@@ -178,10 +180,10 @@ def shortcut_expected(o, path, vsl, margin):
     return RO.shortcut_greedy(len(path), free), amb_pairs
 
 
-@pytest.mark.parametrize("name", ["panda", "fetch"])
+@pytest.mark.parametrize("name", ["panda", "fetch", "baxter"])
 def test_shortcut_matches_oracle(robots, name):
     rob = robots[name]
-    scene, o, start, goals = problem(rob, 400 if name == "panda" else 401)
+    scene, o, start, goals = problem(rob, {"panda": 400, "fetch": 401, "baxter": 402}[name])
     ds = S.DeviceScene.from_host(scene)
     st = S.rrtc_settings(num_new_configs=128, max_samples=100000, time_limit_ms=3000.0, range=0.25, seed=5)
     res = S.rrtc_solve(rob, ds, start, goals, st)
