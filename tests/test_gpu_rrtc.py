@@ -167,6 +167,22 @@ def test_rrtc_limits_and_buffers(robots):
     assert err.value.status == _ppl.PPL_ERR_UNSUPPORTED
 
 
+def test_rrtc_time_limit_ends_a_long_run(robots):
+    """With range 1e-3 a connection needs thousands of checked steps, so the
+    device-clock limit, read in every iteration and every extension step, ends
+    the run: the kernel returns close to the limit, unsolved."""
+    rob = robots["panda"]
+    scene, o, start, goals = problem(rob, 310)
+    ds = S.DeviceScene.from_host(scene)
+    for limit in (20.0, 60.0):
+        st = S.rrtc_settings(num_new_configs=128, range=1e-3, time_limit_ms=limit, max_samples=1000000)
+        res = S.rrtc_solve(rob, ds, start, goals, st)
+        banner(f"pRRTC range 1e-3, time limit {limit} ms: solved={res['solved']} kernel {res['kernel_ms']:.1f} ms, "
+               f"iters {res['iters']}, trees {res['start_tree_size']}+{res['goal_tree_size']}")
+        assert not res["solved"]
+        assert limit <= res["kernel_ms"] < limit + 15.0
+
+
 def shortcut_expected(o, path, vsl, margin):
     free, amb_pairs = {}, 0
     for i, j, states in RO.shortcut_pairs(path, vsl):
