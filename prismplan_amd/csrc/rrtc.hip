@@ -267,7 +267,7 @@ __device__ int choose_tree(const RrtcArgs& A, int iter, int prev_t) {
 
 template <int G>
 __global__ __launch_bounds__(kThreads) void k_rrtc(RrtcArgs A) {
-    extern __shared__ float lds[];   // [groups][3 * n_spheres] then [groups][3 * n_approx]
+    extern __shared__ float lds[];   // per-state sphere positions and frames, then the staged tables (StageLayout)
     __shared__ float s_cfg[kMaxDim], s_near[kMaxDim], s_step[kMaxDim], s_vec[kMaxDim];
     __shared__ float s_hn[kMaxDim], s_hd[kMaxDim];
     __shared__ float s_q[kMaxGran * kMaxDim];
