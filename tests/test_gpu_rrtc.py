@@ -167,6 +167,44 @@ def test_rrtc_limits_and_buffers(robots):
     assert err.value.status == _ppl.PPL_ERR_UNSUPPORTED
 
 
+def esdf_problem(robot, seed):
+    """An ESDF layer (nearest-voxel lookup, built from boxes) plus a few boxes,
+    with a free start and a free goal whose straight line is blocked."""
+    r = robot.arrays
+    rng = np.random.default_rng(seed)
+    src = W.random_obbs(rng, 10, lo=(0.3, -0.5, 0.1), hi=(0.8, 0.5, 0.9), half_hi=0.1)
+    params, pose = W.esdf_grid(center=(0.55, 0.0, 0.5), n=48, voxel=0.02)
+    vals = PO.esdf_from_obbs(src.obb_dims, src.obb_pose, params, pose).astype(np.float32)
+    extra = W.random_obbs(rng, 4, lo=(0.3, -0.6, 0.0), hi=(0.9, 0.6, 1.0), half_hi=0.08)
+    scene = HostScene(obb_dims=extra.obb_dims, obb_pose=extra.obb_pose, obb_enable=extra.obb_enable,
+                      voxel_params=params, voxel_pose=pose, voxel_enable=np.ones(1, np.uint8),
+                      voxel_features=vals.reshape(1, -1), esdf_interp=0)
+    o = PO.Oracle(r, scene, nthreads=8)
+    q = W.uniform_configs(rng, r, 400)
+    hit, clr = o.check_collision(q, check_self=True)
+    free = q[~hit & (np.abs(clr) > 0.01)]
+    start = free[0]
+    goals = [g for g in free[1:] if line_blocked(o, start, g, 0.0)][:1]
+    assert goals, "no blocked start-goal pair in this world"
+    return scene, o, start, np.stack(goals), float(hit.mean())
+
+
+@pytest.mark.parametrize("seed", [600, 601])
+def test_rrtc_in_an_esdf_world(robots, seed):
+    rob = robots["panda"]
+    scene, o, start, goals, rate = esdf_problem(rob, seed)
+    ds = S.DeviceScene.from_host(scene)
+    st = S.rrtc_settings(num_new_configs=256, max_samples=200000, time_limit_ms=3000.0, seed=seed)
+    banner(f"pRRTC panda in an ESDF + box world ({rate:.2f} of uniform configurations collide)")
+    res = S.rrtc_solve(rob, ds, start, goals, st)
+    check_solution(o, res, start, goals, st, f"esdf/{seed}")
+    out = S.shortcut_path(rob, ds, res["path"], valid_segment_length=0.025, collision_margin=0.0005)
+    keep, amb = shortcut_expected(o, res["path"], 0.025, 0.0005)
+    same = out.shape == res["path"][keep].shape and np.array_equal(out, res["path"][keep])
+    banner(f"  shortcut {len(res['path'])} -> {len(out)} waypoints (oracle {len(keep)}), {amb} ambiguous skip edges")
+    assert same or amb > 0
+
+
 def test_rrtc_time_limit_ends_a_long_run(robots):
     """With range 1e-3 a connection needs thousands of checked steps, so the
     device-clock limit, read in every iteration and every extension step, ends
