@@ -751,8 +751,10 @@ __device__ __forceinline__ void near_obbs(const float4* __restrict__ s_obb, int 
 
 // Smallest box key over the candidate OBBs of `mk` at world point p, and its
 // OBB (LDS record index; -1 when there is none). Ties keep the lowest index.
+// A (sphere, OBB) pair the scene's ACM masks out is skipped
+// (collision/scene_collision.cuh:140-167: sphere_obb_clearance by sphere index).
 __device__ __forceinline__ void obb_argmin(const float4* __restrict__ s_obb, const ObbMask& mk, float px, float py,
-                                           float pz, float& best, int& arg) {
+                                           float pz, const uint8_t* __restrict__ acm_row, float& best, int& arg) {
     best = FLT_MAX;
     arg = -1;
 #pragma unroll
@@ -761,6 +763,7 @@ __device__ __forceinline__ void obb_argmin(const float4* __restrict__ s_obb, con
         while (bits) {
             const int o = k * 64 + (__ffsll(bits) - 1);
             bits &= bits - 1ull;
+            if (acm_row && acm_row[__float_as_int(s_obb[4 * o + 3].w)] == 0) continue;
             float lx, ly, lz;
             to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], px, py, pz, lx, ly, lz);
             const float key = box_key(lx, ly, lz, s_obb[4 * o + 3]);
@@ -885,7 +888,9 @@ __global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
                     aff_point(T, sp.x, sp.y, sp.z, px, py, pz);
                     float best = FLT_MAX;
                     int arg = -1;
-                    if (use_obb) obb_argmin(s_obb, mk, px, py, pz, best, arg);
+                    const uint8_t* acm_row =
+                        a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)R.sph_id[s] * a.S.n_obbs : nullptr;
+                    if (use_obb) obb_argmin(s_obb, mk, px, py, pz, acm_row, best, arg);
                     float gx, gy, gz;
                     const float hs = sphere_hinge_at<GRAD>(a, s_obb, best, arg, px, py, pz, sp.w, R.sph_id[s], use_obb,
                                                            use_esdf, gx, gy, gz);

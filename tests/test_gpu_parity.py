@@ -622,6 +622,81 @@ def test_device_inline_api(robots, name, G, frames):
         assert err.max() < bar, what
 
 
+def test_allowed_collision_masks(robots):
+    """Scene ACM masks (SceneCollisionData's sphere_obb_acm_mask [S x O] and
+    sphere_acm_mask [S x L]; collision/scene_collision.cuh:118-167, 186-331):
+    a masked (sphere, box) or (sphere, layer) pair is skipped by the exact
+    checks, clearances and hinges; the approximate pass ignores the masks.
+    Plain, batched, cost and device-inline paths against the FP64 oracle."""
+    import ctypes as C
+    rob = robots["panda"]
+    r = rob.arrays
+    base = devapi_scene("panda")
+    rng = np.random.default_rng(77)
+    scene = HostScene(obb_dims=base.obb_dims, obb_pose=base.obb_pose, obb_enable=base.obb_enable,
+                      voxel_params=base.voxel_params, voxel_pose=base.voxel_pose, voxel_enable=base.voxel_enable,
+                      voxel_features=base.voxel_features, esdf_interp=0,
+                      sphere_obb_acm=(rng.random((r.n_spheres, base.n_obbs)) > 0.4).astype(np.uint8),
+                      sphere_acm=(rng.random((r.n_spheres, base.n_voxel_layers)) > 0.4).astype(np.uint8))
+    q = W.uniform_configs(np.random.default_rng(78), r, 2000)
+    o = PO.Oracle(r, scene, nthreads=8)
+    o_open = PO.Oracle(r, base, nthreads=8)
+    ds = S.DeviceScene.from_host(scene)
+    banner(f"ACM masks: {int((scene.sphere_obb_acm == 0).sum())} masked sphere-box pairs, "
+           f"{int((scene.sphere_acm == 0).sum())} masked sphere-layer pairs")
+    ref, clr = o.check_collision(q)
+    ref_open, _ = o_open.check_collision(q)
+    assert ref.mean() < ref_open.mean(), "masking pairs should free some configurations"
+    assert_verdicts(S.check_collision_scene(rob, ds, q), ref, clr, "plain any-collision with ACM")
+    plan = S.Plan.configs(rob, len(q))
+    x = torch.from_numpy(q).to(DEV)
+    ref_s, clr_s = o.check_collision(q, check_self=True)
+    got = S.check_collision_batched(plan, ds, x, check_self=True).cpu().numpy()
+    assert_verdicts(got, ref_s, clr_s, "batched any-collision + self with ACM")
+    mc = S.min_clearance_batched(plan, ds, x, collision_margin=0.01).cpu().numpy()
+    mref = o.min_clearance(q, margin=0.01)
+    ok = np.abs(mref) < 999
+    err = np.abs(mc - mref)[ok] / np.maximum(np.abs(mref[ok]), 0.01)
+    banner(f"  min clearance with ACM: max rel err {err.max():.2e}")
+    assert np.mean(err < 1e-5) > 0.995
+    prm = S.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0, use_self=False)
+    c = S.hinge_cost(plan, ds, x, prm)[0].cpu().numpy()
+    c_ref = o.hinge_cost(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0, use_self=False))
+    c_open = o_open.hinge_cost(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0, use_self=False))
+    rel = np.abs(c - c_ref) / np.maximum(np.abs(c_ref), 2.0 * 0.05)
+    banner(f"  hinge with ACM: max rel err {rel.max():.2e} (cost lowered on {np.mean(c_ref < c_open - 1e-6):.2f})")
+    assert rel.max() < 1e-5
+    # the device-inline API, both two_phase_cc overloads
+    lib = C.CDLL(str(DEVTEST))
+    dv = _ppl.RobotDeviceView()
+    _ppl.call("ppl_robot_get_device_view", rob.handle, C.byref(dv))
+    view = ds.view()
+    n, G = len(q), 16
+    slots = (n + 64 // G - 1) // (64 // G) * (64 // G)
+    vp = C.c_void_p
+    lib.ppl_devtest_run.restype = C.c_int
+    lib.ppl_devtest_run.argtypes = [C.POINTER(_ppl.RobotDeviceView), C.POINTER(_ppl.SceneView), vp, C.c_int, C.c_int,
+                                    C.c_float, C.c_int, C.c_float, C.c_float, vp, vp, vp, vp, vp, vp, vp]
+    for frames in (0, 1):
+        hit = torch.zeros(n, dtype=torch.uint8, device=DEV)
+        outs = [torch.zeros(n, dtype=torch.float32, device=DEV) for _ in range(4)]
+        scratch = torch.zeros(slots * (3 * (r.n_spheres + r.n_approx) + 12 * r.n_joints), dtype=torch.float32,
+                              device=DEV)
+        assert lib.ppl_devtest_run(C.byref(dv), C.byref(view), x.data_ptr(), n, G, 0.0, 1 | (2 * frames), 0.05,
+                                   0.01, hit.data_ptr(), *[t.data_ptr() for t in outs], scratch.data_ptr(), None) == 0
+        torch.cuda.synchronize()
+        assert_verdicts(hit.cpu().numpy(), ref_s, clr_s, f"device two_phase_cc with ACM (frames overload {frames})")
+        for what, got, prm in (("obb", outs[1], PO.cost_params(world_margin=0.05, use_obb=True, use_esdf=False,
+                                                                use_self=False)),
+                               ("esdf", outs[2], PO.cost_params(world_margin=0.05, use_obb=False, use_esdf=True,
+                                                                use_self=False))):
+            exp = o.hinge_cost(q, prm)
+            err = np.abs(got.cpu().numpy().astype(np.float64) - exp) / np.maximum(np.abs(exp), 0.05)
+            err32 = np.abs(o.hinge_cost(q, prm, precision="f32") - exp) / np.maximum(np.abs(exp), 0.05)
+            banner(f"  device {what} hinge with ACM: max rel err {err.max():.2e} (FP32 restatement {err32.max():.2e})")
+            assert err.max() < max(1e-5, 3.0 * float(err32.max()))
+
+
 def test_algorithmic_flop_counts(robots):
     """ppl_algorithmic_flops (include/ppl/roofline.h) reproduces the hand
     counts of the benchmark workloads (FMA = 2, sqrt/sin/cos = 1, abs = 0)."""
