@@ -697,6 +697,70 @@ def test_allowed_collision_masks(robots):
             assert err.max() < max(1e-5, 3.0 * float(err32.max()))
 
 
+def varied_scene(interp, layer1_enabled, seed=90):
+    """Disabled boxes, two ESDF layers (the second on a grid rotated 30 deg
+    about z, coarser, possibly disabled), a slab of unobserved voxels (-1000)
+    in the first, layers of different sizes in one feature table."""
+    from prismplan_amd.planning_scene import inverse_pose
+    rng = np.random.default_rng(seed)
+    boxes = W.random_obbs(rng, 12, lo=(0.3, -0.6, 0.0), hi=(0.9, 0.6, 1.0), half_hi=0.1)
+    enable = (rng.random(12) > 0.3).astype(np.uint8)
+    src0 = W.random_obbs(rng, 8, lo=(0.3, -0.4, 0.2), hi=(0.8, 0.4, 0.8), half_hi=0.1)
+    src1 = W.random_obbs(rng, 6, lo=(0.2, -0.3, 0.1), hi=(0.7, 0.5, 0.7), half_hi=0.08)
+    p0, pose0 = W.esdf_grid(center=(0.5, 0.0, 0.5), n=48, voxel=0.02)
+    n1, v1 = 40, 0.025
+    ext1 = (n1 - 1) * v1
+    p1 = np.array([[ext1, ext1, ext1, v1]], np.float32)
+    ang = np.deg2rad(30.0)
+    t1, q1 = inverse_pose(np.array([0.45, 0.1, 0.4]), np.array([np.cos(ang / 2), 0.0, 0.0, np.sin(ang / 2)]))
+    pose1 = np.zeros((1, 8), np.float32)
+    pose1[0, :3], pose1[0, 3:7] = t1, q1
+    f0 = PO.esdf_from_obbs(src0.obb_dims, src0.obb_pose, p0, pose0).astype(np.float32)
+    f0.reshape(48, 48, 48)[20:28] = -1000.0          # unobserved slab
+    f1 = PO.esdf_from_obbs(src1.obb_dims, src1.obb_pose, p1, pose1).astype(np.float32)
+    feats = np.full((2, f0.size), -1000.0, np.float32)
+    feats[0] = f0
+    feats[1, :f1.size] = f1
+    return HostScene(obb_dims=boxes.obb_dims, obb_pose=boxes.obb_pose, obb_enable=enable,
+                     voxel_params=np.concatenate([p0, p1]), voxel_pose=np.concatenate([pose0, pose1]),
+                     voxel_enable=np.array([1, layer1_enabled], np.uint8), voxel_features=feats, esdf_interp=interp)
+
+
+@pytest.mark.parametrize("interp,layer1", [(0, 1), (1, 1), (0, 0)])
+def test_scene_variants(robots, interp, layer1):
+    """Disabled boxes, two ESDF layers (one rotated, one disabled or not),
+    unobserved voxels: plain, batched, clearance and cost paths against the
+    FP64 oracle (scene_collision.cuh:118-331)."""
+    rob = robots["panda"]
+    r = rob.arrays
+    scene = varied_scene(interp, layer1)
+    q = W.uniform_configs(np.random.default_rng(91), r, 2000)
+    o = PO.Oracle(r, scene, nthreads=8)
+    ds = S.DeviceScene.from_host(scene)
+    banner(f"scene variants interp={interp} layer1={layer1}: {int(scene.obb_enable.sum())}/12 boxes enabled")
+    ref, clr = o.check_collision(q)
+    assert 0.05 < ref.mean() < 0.95
+    assert_verdicts(S.check_collision_scene(rob, ds, q), ref, clr, "plain any-collision, varied scene")
+    plan = S.Plan.configs(rob, len(q))
+    x = torch.from_numpy(q).to(DEV)
+    ref_s, clr_s = o.check_collision(q, check_self=True)
+    assert_verdicts(S.check_collision_batched(plan, ds, x, check_self=True).cpu().numpy(), ref_s, clr_s,
+                    "batched any-collision + self, varied scene")
+    mc = S.min_clearance_batched(plan, ds, x, collision_margin=0.01).cpu().numpy()
+    mref = o.min_clearance(q, margin=0.01)
+    ok = np.abs(mref) < 999
+    err = np.abs(mc - mref)[ok] / np.maximum(np.abs(mref[ok]), 0.01)
+    banner(f"  min clearance: max rel err {err.max():.2e}, within 1e-5 on {np.mean(err < 1e-5):.4f}")
+    assert np.mean(err < 1e-5) > 0.995
+    prm = S.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0, use_self=True)
+    c = S.hinge_cost(plan, ds, x, prm)[0].cpu().numpy()
+    c_ref = o.hinge_cost(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0, use_self=True))
+    rel = np.abs(c - c_ref) / np.maximum(np.abs(c_ref), 2.0 * 0.05)
+    banner(f"  hinge: max rel err {rel.max():.2e}, within 1e-5 on {np.mean(rel < 1e-5):.4f} (active {np.mean(c_ref > 0):.2f})")
+    # nearest-voxel lookups may switch voxel at a boundary (as for min clearance)
+    assert np.mean(rel < 1e-5) > (0.995 if interp == 0 else 0.999)
+
+
 def test_algorithmic_flop_counts(robots):
     """ppl_algorithmic_flops (include/ppl/roofline.h) reproduces the hand
     counts of the benchmark workloads (FMA = 2, sqrt/sin/cos = 1, abs = 0)."""
