@@ -666,6 +666,13 @@ def test_allowed_collision_masks(robots):
     rel = np.abs(c - c_ref) / np.maximum(np.abs(c_ref), 2.0 * 0.05)
     banner(f"  hinge with ACM: max rel err {rel.max():.2e} (cost lowered on {np.mean(c_ref < c_open - 1e-6):.2f})")
     assert rel.max() < 1e-5
+    # the fused adjoint's gradient follows the masked argmin
+    _, _, g = S.hinge_cost(plan, ds, x, prm, grad=True)
+    c_g, g_ref, kink = o.hinge_grad(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0,
+                                                      use_self=False))
+    gerr = grad_rel_err(g.cpu().numpy(), g_ref, 2.0)[~kink]
+    banner(f"  hinge gradient with ACM: max rel err {gerr.max():.2e} (norm-wise), kink-excluded {int(kink.sum())}")
+    assert gerr.max() < 1e-4
     # the device-inline API, both two_phase_cc overloads
     lib = C.CDLL(str(DEVTEST))
     dv = _ppl.RobotDeviceView()
