@@ -575,9 +575,15 @@ ppl_status ppl_rrtc_buffers_create(int n_dof, int max_samples, int num_new_confi
                                    ppl_rrtc_buffers* out);
 ppl_status ppl_rrtc_buffers_destroy(ppl_rrtc_buffers bufs);
 
+/* Each tree walk of a solution holds at most 4096 nodes, so a path has at
+ * most PPL_RRTC_MAX_WAYPOINTS rows; a connection whose walk is longer ends
+ * the run unsolved. */
+#define PPL_RRTC_MAX_WAYPOINTS 8194
+
 /* h_goals [n_goals][n_dof]; h_path [max_waypoints][n_dof]. PPL_ERR_CAPACITY
  * when the solution has more than max_waypoints rows (result->n_waypoints
- * then holds the count needed). */
+ * then holds the count needed); max_waypoints = PPL_RRTC_MAX_WAYPOINTS
+ * always suffices. */
 ppl_status ppl_rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const float* h_start, const float* h_goals,
                           int n_goals, const ppl_rrtc_settings* settings, ppl_rrtc_buffers bufs, float* h_path,
                           int max_waypoints, ppl_rrtc_result* result, ppl_stream stream);
@@ -591,7 +597,8 @@ ppl_status ppl_rrtc_halton_bases(unsigned long long seed, int n_workgroups, int 
  * valid_segment_length)), q = q_i (1 - t) + q_j t (FP32, no contraction),
  * with the scene and self checks at collision_margin; then greedily from
  * waypoint 0 the farthest free j is taken (else i + 1). h_out [n][n_dof]
- * receives *n_out <= n waypoints. Synchronous on `stream`. */
+ * receives *n_out <= n waypoints. n <= PPL_RRTC_MAX_WAYPOINTS, the longest
+ * path a solve returns (PPL_ERR_CAPACITY beyond). Synchronous on `stream`. */
 ppl_status ppl_shortcut_path(ppl_robot robot, const ppl_scene_view* scene, const float* h_path, int n,
                              float valid_segment_length, float collision_margin, float* h_out, int* n_out,
                              ppl_stream stream);

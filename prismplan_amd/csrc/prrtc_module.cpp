@@ -443,18 +443,10 @@ PlannerResult solve_rrtc_scene(F32Array start, F32Array goals, const SceneCollis
     ppl_rrtc_result r{};
     {
         py::gil_scoped_release nogil;
-        int cap = 1024;
-        for (;;) {   // a longer solution reports the rows it needs; solve again with room for them
-            path.assign((size_t)cap * D, 0.0f);
-            const ppl_status s =
-                ppl_rrtc_solve(robot.robot, &v, start.data(), goals.data(), G, &cs, h, path.data(), cap, &r, nullptr);
-            if (s == PPL_ERR_CAPACITY && r.n_waypoints > cap) {
-                cap = r.n_waypoints;
-                continue;
-            }
-            check(s, "solve_scene");
-            break;
-        }
+        path.assign((size_t)PPL_RRTC_MAX_WAYPOINTS * D, 0.0f);   // room for any solution
+        check(ppl_rrtc_solve(robot.robot, &v, start.data(), goals.data(), G, &cs, h, path.data(),
+                             PPL_RRTC_MAX_WAYPOINTS, &r, nullptr),
+              "solve_scene");
     }
     PlannerResult res;
     res.solved = r.solved != 0;

@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "ppl/ppl.h"
@@ -57,6 +58,7 @@ constexpr float kUnwritten = -9999.0f;
 constexpr size_t kMaxDynLds = 56 * 1024;
 constexpr long long kMaxShortcutStates = 1ll << 26;
 constexpr int kHeadRows = 64;        // path rows read back with the control block
+static_assert(PPL_RRTC_MAX_WAYPOINTS == 2 * kPathCap + 2, "two tree walks plus both roots");
 
 struct RrtcCtl {
     int status;            // 0 running, 1 solved, -1 stopped (limit reached)
@@ -858,23 +860,34 @@ ppl_status ppl_shortcut_path(ppl_robot robot, const ppl_scene_view* scene, const
                              ppl_stream stream) {
     RR_ARG(robot && scene && h_out && n_out && (h_path || n == 0) && n >= 0, "NULL argument");
     RR_ARG(valid_segment_length > 0.0f, "valid_segment_length must be > 0");
+    if (n > PPL_RRTC_MAX_WAYPOINTS)
+        return ppl::fail_msg(PPL_ERR_CAPACITY, "shortcut paths hold at most PPL_RRTC_MAX_WAYPOINTS (8194) waypoints");
     ppl_robot_info info{};
     ppl_status st = ppl_robot_get_info(robot, &info);
     if (st != PPL_OK) return st;
     Shortcut ws;
-    std::vector<float> out;
-    st = shortcut_run(robot, scene, h_path, n, info.n_dof, valid_segment_length, collision_margin, out, ws,
-                      (hipStream_t)stream);
+    try {
+        std::vector<float> out;
+        st = shortcut_run(robot, scene, h_path, n, info.n_dof, valid_segment_length, collision_margin, out, ws,
+                          (hipStream_t)stream);
+        if (st == PPL_OK) {
+            std::memcpy(h_out, out.data(), out.size() * sizeof(float));
+            *n_out = (int)(out.size() / std::max(1, info.n_dof));
+        }
+    } catch (const std::bad_alloc&) {
+        st = ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
+    }
     ws.release();
-    if (st != PPL_OK) return st;
-    std::memcpy(h_out, out.data(), out.size() * sizeof(float));
-    *n_out = (int)(out.size() / std::max(1, info.n_dof));
-    return PPL_OK;
+    return st;
 }
 
-ppl_status ppl_rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const float* h_start, const float* h_goals,
-                          int n_goals, const ppl_rrtc_settings* cfg, ppl_rrtc_buffers bufs, float* h_path,
-                          int max_waypoints, ppl_rrtc_result* res, ppl_stream stream) {
+}  // extern "C"
+
+namespace {
+
+ppl_status rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const float* h_start, const float* h_goals,
+                      int n_goals, const ppl_rrtc_settings* cfg, ppl_rrtc_buffers bufs, float* h_path,
+                      int max_waypoints, ppl_rrtc_result* res, ppl_stream stream) {
     const auto wall0 = std::chrono::steady_clock::now();
     RR_ARG(robot && scene && h_start && h_goals && cfg && res, "NULL argument");
     RR_ARG(n_goals >= 1, "need at least one goal");
@@ -959,7 +972,7 @@ ppl_status ppl_rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const fl
         A.time_ticks = std::max(1ll, (long long)((double)cfg->time_limit_ms * (double)khz));
     }
     // I/O: [ctl 256 B][assembled path][roots]; pinned mirror of the first two parts plus the roots staging
-    const size_t out_rows = 2 * (size_t)kPathCap + 2;
+    const size_t out_rows = PPL_RRTC_MAX_WAYPOINTS;
     const size_t out_off = 256, roots_off = out_off + a256(out_rows * dim * sizeof(float));
     const size_t roots_bytes = (size_t)(1 + n_goals) * dim * sizeof(float);
     RR_TRY(b->io.ensure(roots_off + roots_bytes));
@@ -1064,4 +1077,15 @@ ppl_status ppl_rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const fl
     return PPL_OK;
 }
 
-}  // extern "C"
+}  // namespace
+
+extern "C" ppl_status ppl_rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const float* h_start,
+                                     const float* h_goals, int n_goals, const ppl_rrtc_settings* cfg,
+                                     ppl_rrtc_buffers bufs, float* h_path, int max_waypoints, ppl_rrtc_result* res,
+                                     ppl_stream stream) {
+    try {
+        return rrtc_solve(robot, scene, h_start, h_goals, n_goals, cfg, bufs, h_path, max_waypoints, res, stream);
+    } catch (const std::bad_alloc&) {
+        return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
+    }
+}

@@ -477,7 +477,7 @@ class RrtcBuffers:
 
 
 def rrtc_solve(robot: Robot, scene: DeviceScene, start, goals, settings: Optional[_ppl.RrtcSettings] = None,
-               bufs: Optional[RrtcBuffers] = None, max_waypoints=16384) -> dict:
+               bufs: Optional[RrtcBuffers] = None, max_waypoints=8194) -> dict:
     """pRRTC (ppl_rrtc_solve): path [N, D] start -> goal when solved."""
     D = robot.n_dof
     a = np.ascontiguousarray(start, np.float32).reshape(-1)

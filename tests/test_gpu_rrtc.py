@@ -293,6 +293,11 @@ def test_shortcut_edge_cases(robots):
     out = S.shortcut_path(rob, ds, q, valid_segment_length=0.05)
     same = out.shape == q[keep].shape and np.array_equal(out, q[keep])
     assert same or amb > 0
+    # the longest path a solve returns is 2 * 4096 + 2 waypoints; longer inputs are refused before any work
+    long = np.repeat(q[:1], 8195, axis=0)
+    with pytest.raises(_ppl.PplError) as err:
+        S.shortcut_path(rob, ds, long)
+    assert err.value.status == _ppl.PPL_ERR_CAPACITY
 
 
 def test_prrtc_solve_scene_surface(robots):
