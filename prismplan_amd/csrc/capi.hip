@@ -381,7 +381,7 @@ const char* ppl_version(void) { return "ppl 0.1 (gfx950)"; }
 // ---------------------------------------------------------------------------
 // Robot
 // ---------------------------------------------------------------------------
-ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
+ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     CHECK_ARG(d && out, "NULL argument");
     *out = nullptr;
     const int J = d->n_joints, D = d->n_dof, S = d->n_spheres, A = d->n_approx_spheres;
@@ -680,9 +680,11 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) {
     r->upper.assign(d->joint_upper, d->joint_upper + D);
     *out = r;
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
-ppl_status ppl_robot_get_device_view(ppl_robot r, ppl_robot_device_view* v) {
+ppl_status ppl_robot_get_device_view(ppl_robot r, ppl_robot_device_view* v) try {
     CHECK_ARG(r && v, "NULL argument");
     const DRobot& R = r->R;
     v->n_dof = R.n_dof;
@@ -705,9 +707,11 @@ ppl_status ppl_robot_get_device_view(ppl_robot r, ppl_robot_device_view* v) {
     v->apx_pairs = reinterpret_cast<const int*>(R.apx_pairs);
     v->lim = reinterpret_cast<const float*>(R.lim);
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
-ppl_status ppl_robot_destroy(ppl_robot r) {
+ppl_status ppl_robot_destroy(ppl_robot r) try {
     if (!r) return PPL_OK;
     for (auto& e : r->traj_plans) ppl_plan_destroy(e.second);
     r->traj_plans.clear();
@@ -719,9 +723,11 @@ ppl_status ppl_robot_destroy(ppl_robot r) {
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
-ppl_status ppl_robot_get_info(ppl_robot r, ppl_robot_info* info) {
+ppl_status ppl_robot_get_info(ppl_robot r, ppl_robot_info* info) try {
     CHECK_ARG(r && info, "NULL argument");
     info->n_dof = r->R.n_dof;
     info->n_joints = r->R.n_joints;
@@ -730,9 +736,11 @@ ppl_status ppl_robot_get_info(ppl_robot r, ppl_robot_info* info) {
     info->n_self_pairs = r->n_pairs;
     info->n_t_slots = r->n_slots;
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
-ppl_status ppl_robot_update_joint_limits(ppl_robot r, const float* lo, const float* hi) {
+ppl_status ppl_robot_update_joint_limits(ppl_robot r, const float* lo, const float* hi) try {
     CHECK_ARG(r && lo && hi, "NULL argument");
     const int D = r->R.n_dof;
     r->lower.assign(lo, lo + D);
@@ -741,13 +749,15 @@ ppl_status ppl_robot_update_joint_limits(ppl_robot r, const float* lo, const flo
     for (int k = 0; k < D; ++k) lim[k] = make_float2(lo[k], hi[k]);
     HIP_TRY(hipMemcpy(const_cast<float2*>(r->R.lim), lim.data(), (size_t)D * sizeof(float2), hipMemcpyHostToDevice));
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 // ---------------------------------------------------------------------------
 // Plain any-collision (pRRTC::check_collision_scene)
 // ---------------------------------------------------------------------------
 ppl_status ppl_check_collision_scene(ppl_robot r, const ppl_scene_view* view, const float* h_cfg, int n, float margin,
-                                     uint8_t* h_out) {
+                                     uint8_t* h_out) try {
     CHECK_ARG(r, "robot is NULL");
     CHECK_ARG(n >= 0, "negative N");
     if (n == 0) return PPL_OK;
@@ -780,12 +790,14 @@ ppl_status ppl_check_collision_scene(ppl_robot r, const ppl_scene_view* view, co
     HIP_TRY(hipStreamSynchronize(r->stream));
     memcpy(h_out, r->h_stage.p, (size_t)n);
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 // ---------------------------------------------------------------------------
 // Plain edge evaluator (EdgeEvalGpu)
 // ---------------------------------------------------------------------------
-ppl_status ppl_edge_eval_create(ppl_robot r, const ppl_scene_view* view, int dim, int max_edges, ppl_edge_eval* out) {
+ppl_status ppl_edge_eval_create(ppl_robot r, const ppl_scene_view* view, int dim, int max_edges, ppl_edge_eval* out) try {
     CHECK_ARG(r && out, "NULL argument");
     *out = nullptr;
     CHECK_ARG(dim == r->R.n_dof, "dim %d != robot n_dof %d", dim, r->R.n_dof);
@@ -811,9 +823,11 @@ ppl_status ppl_edge_eval_create(ppl_robot r, const ppl_scene_view* view, int dim
     }
     *out = ev;
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
-ppl_status ppl_edge_eval_destroy(ppl_edge_eval ev) {
+ppl_status ppl_edge_eval_destroy(ppl_edge_eval ev) try {
     if (!ev) return PPL_OK;
     if (ev->stream) (void)hipStreamSynchronize(ev->stream);
     ev->d_io.release(); ev->d_clr_state.release(); ev->d_clr_edge.release(); ev->d_self.release();
@@ -821,6 +835,8 @@ ppl_status ppl_edge_eval_destroy(ppl_edge_eval ev) {
     if (ev->stream) (void)hipStreamDestroy(ev->stream);
     delete ev;
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 }  // extern "C"
@@ -840,7 +856,7 @@ __global__ void k_segment_min(const float* x, const int* off, int n_seg, float* 
 extern "C" {
 
 ppl_status ppl_edge_granularities(const float* h_from, const float* h_to, int n, int dim, const ppl_edge_params* p,
-                                  int* h_G, float* h_len) {
+                                  int* h_G, float* h_len) try {
     CHECK_ARG(h_from && h_to && p && h_G, "NULL argument");
     CHECK_ARG(n >= 0 && dim >= 1, "bad sizes");
     for (int e = 0; e < n; ++e) {
@@ -849,10 +865,12 @@ ppl_status ppl_edge_granularities(const float* h_from, const float* h_to, int n,
         if (h_len) h_len[e] = len;
     }
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_edge_eval_evaluate(ppl_edge_eval ev, const float* h_from, const float* h_to, int n, const ppl_edge_params* p,
-                                  uint8_t* h_valid, float* h_cost, float* h_clr) {
+                                  uint8_t* h_valid, float* h_cost, float* h_clr) try {
     CHECK_ARG(ev && p, "NULL argument");
     CHECK_ARG(n >= 0, "negative edge count");
     if (n == 0) return PPL_OK;
@@ -946,12 +964,14 @@ ppl_status ppl_edge_eval_evaluate(ppl_edge_eval ev, const float* h_from, const f
         }
     }
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 // ---------------------------------------------------------------------------
 // Plans
 // ---------------------------------------------------------------------------
-ppl_status ppl_plan_create(ppl_robot r, const ppl_plan_desc* d, ppl_plan* out) {
+ppl_status ppl_plan_create(ppl_robot r, const ppl_plan_desc* d, ppl_plan* out) try {
     CHECK_ARG(r && d && out, "NULL argument");
     *out = nullptr;
     CHECK_ARG(d->kind >= PPL_PLAN_CONFIGS && d->kind <= PPL_PLAN_TRAJ, "unknown plan kind %d", d->kind);
@@ -1049,9 +1069,11 @@ ppl_status ppl_plan_create(ppl_robot r, const ppl_plan_desc* d, ppl_plan* out) {
     }
     *out = P;
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
-ppl_status ppl_plan_destroy(ppl_plan P) {
+ppl_status ppl_plan_destroy(ppl_plan P) try {
     if (!P) return PPL_OK;
     (void)hipDeviceSynchronize();
     for (int i = 0; i < kGroupKinds; ++i)
@@ -1074,6 +1096,8 @@ ppl_status ppl_plan_destroy(ppl_plan P) {
     P->d_self.release(); P->d_cost.release(); P->d_fdcfg.release(); P->d_fdcost.release();
     delete P;
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 long ppl_plan_states(ppl_plan P) { return P ? P->n_states : -1; }
@@ -1582,7 +1606,7 @@ ppl_status ensure_stomp(ppl_plan_s* P, const ppl_stomp_settings* ss) {
 extern "C" {
 
 ppl_status ppl_chomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_chomp_settings* cs, float* d_theta,
-                              float* d_best, float* d_record, int iterations, int first_call, ppl_stream stream) {
+                              float* d_best, float* d_record, int iterations, int first_call, ppl_stream stream) try {
     CHECK_ARG(P && view && cs && d_theta && d_best && d_record, "NULL argument");
     CHECK_ARG(P->kind == PPL_PLAN_TRAJ && !P->wmajor, "CHOMP needs a (unit-major) trajectory plan");
     CHECK_ARG(P->h_obb_off.empty(), "CHOMP optimises trajectories of one scene (no per-problem OBB ranges)");
@@ -1647,10 +1671,12 @@ ppl_status ppl_chomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_
         HIP_TRY(launch_chomp_book(a, 0, s));
     }
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_stomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_stomp_settings* ss, float* d_theta,
-                              float* d_best, float* d_record, int iterations, int first_call, ppl_stream stream) {
+                              float* d_best, float* d_record, int iterations, int first_call, ppl_stream stream) try {
     CHECK_ARG(P && view && ss && d_theta && d_best && d_record, "NULL argument");
     CHECK_ARG(P->kind == PPL_PLAN_TRAJ && !P->wmajor, "STOMP needs a (unit-major) trajectory plan");
     CHECK_ARG(P->h_obb_off.empty(), "STOMP optimises trajectories of one scene (no per-problem OBB ranges)");
@@ -1757,18 +1783,22 @@ ppl_status ppl_stomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_
         ++z->iter;
     }
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
-ppl_status ppl_fk_batched(ppl_plan P, const float* d_cfg, float* d_frames, float* d_spheres, ppl_stream stream) {
+ppl_status ppl_fk_batched(ppl_plan P, const float* d_cfg, float* d_frames, float* d_spheres, ppl_stream stream) try {
     CHECK_ARG(P && d_cfg, "NULL argument");
     CHECK_ARG(P->kind == PPL_PLAN_CONFIGS, "configuration plan required");
     if (P->n_states == 0 || (!d_frames && !d_spheres)) return PPL_OK;
     HIP_TRY(launch_fk_out(P->robot->R, plan_states(P, d_cfg), d_frames, d_spheres, (hipStream_t)stream));
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_check_collision_batched(ppl_plan P, const ppl_scene_view* view, const float* d_cfg, float margin,
-                                       int check_self, uint8_t* d_out, ppl_stream stream) {
+                                       int check_self, uint8_t* d_out, ppl_stream stream) try {
     CHECK_ARG(P && d_cfg && d_out, "NULL argument");
     CHECK_ARG(P->kind == PPL_PLAN_CONFIGS, "configuration plan required");
     DScene S{};
@@ -1796,11 +1826,13 @@ ppl_status ppl_check_collision_batched(ppl_plan P, const ppl_scene_view* view, c
     a.self_hit = self_hit;
     HIP_TRY(launch_collide(a, grid, s));
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_edge_check_batched(ppl_plan P, const ppl_scene_view* view, const float* d_from, const float* d_to,
                                   const ppl_edge_batch_params* prm, int* d_first_hit, uint8_t* d_state_hits,
-                                  ppl_stream stream) {
+                                  ppl_stream stream) try {
     CHECK_ARG(P && prm && d_from && d_to && d_first_hit, "NULL argument");
     CHECK_ARG(P->kind == PPL_PLAN_EDGES, "edge plan required");
     CHECK_ARG(!(d_state_hits && prm->skip_resolved), "per-state verdicts need skip_resolved = 0");
@@ -1834,10 +1866,12 @@ ppl_status ppl_edge_check_batched(ppl_plan P, const ppl_scene_view* view, const 
     a.self_hit = self_hit;
     HIP_TRY(launch_collide(a, grid, s));
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_min_clearance_batched(ppl_plan P, const ppl_scene_view* view, const float* d_cfg, float margin, float* d_out,
-                                     ppl_stream stream) {
+                                     ppl_stream stream) try {
     CHECK_ARG(P && d_cfg && d_out, "NULL argument");
     CHECK_ARG(P->kind == PPL_PLAN_CONFIGS, "configuration plan required");
     DScene S{};
@@ -1857,6 +1891,8 @@ ppl_status ppl_min_clearance_batched(ppl_plan P, const ppl_scene_view* view, con
     a.cost_out = d_out;
     HIP_TRY(launch_cost(a, grid, (hipStream_t)stream));
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 }  // extern "C"
@@ -1912,7 +1948,7 @@ ppl_status fetch_floats(ppl_robot_s* r, const float* d_out, int n, float* h_out)
 }  // namespace
 
 ppl_status ppl_min_clearance_scene(ppl_robot r, const ppl_scene_view* view, const float* h_configs, int n,
-                                   float margin, float* h_clearance) {
+                                   float margin, float* h_clearance) try {
     CHECK_ARG(r, "robot is NULL");
     CHECK_ARG(n >= 0, "negative N");
     if (n == 0) return PPL_OK;
@@ -1926,10 +1962,12 @@ ppl_status ppl_min_clearance_scene(ppl_robot r, const ppl_scene_view* view, cons
     HIP_TRY(r->d_out.ensure((size_t)n * sizeof(float)));
     if ((st = configs_clearance(r, S, d_cfg, n, margin, r->d_out.as<float>(), r->stream)) != PPL_OK) return st;
     return fetch_floats(r, r->d_out.as<float>(), n, h_clearance);
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_hinge_cost_scene(ppl_robot r, const ppl_scene_view* view, const float* h_configs, int n,
-                                const ppl_cost_params* prm, float* h_cost) {
+                                const ppl_cost_params* prm, float* h_cost) try {
     CHECK_ARG(r && prm, "NULL argument");
     CHECK_ARG(n >= 0, "negative N");
     if (n == 0) return PPL_OK;
@@ -1960,10 +1998,12 @@ ppl_status ppl_hinge_cost_scene(ppl_robot r, const ppl_scene_view* view, const f
     c.cost_out = r->d_out.as<float>();
     HIP_TRY(launch_cost(c, grid, r->stream));
     return fetch_floats(r, r->d_out.as<float>(), n, h_cost);
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_mitstar_batch_cc(ppl_robot r, const ppl_scene_view* view, const float* d_samples, int n, float margin,
-                                uint8_t* d_cc, float* d_clearance, float clearance_weight, ppl_stream stream) {
+                                uint8_t* d_cc, float* d_clearance, float clearance_weight, ppl_stream stream) try {
     CHECK_ARG(r && d_samples && d_cc, "NULL argument");
     CHECK_ARG(n >= 0, "negative sample count");
     if (n == 0) return PPL_OK;
@@ -1998,11 +2038,13 @@ ppl_status ppl_mitstar_batch_cc(ppl_robot r, const ppl_scene_view* view, const f
         HIP_TRY(launch_sample_clearance(d_cc, clr, n, clearance_weight, d_clearance, s));
     }
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_mitstar_node_clearance(ppl_robot r, const ppl_scene_view* view, const float* d_nodes,
                                       const int* d_node_indices, int node_start, int n_nodes, float margin,
-                                      float* d_clearance, ppl_stream stream) {
+                                      float* d_clearance, ppl_stream stream) try {
     CHECK_ARG(r && d_nodes && d_clearance, "NULL argument");
     CHECK_ARG(n_nodes >= 0 && (d_node_indices || node_start >= 0), "bad node range");
     if (n_nodes == 0) return PPL_OK;
@@ -2020,12 +2062,14 @@ ppl_status ppl_mitstar_node_clearance(ppl_robot r, const ppl_scene_view* view, c
     if ((st = configs_clearance(r, S, cfg, n_nodes, margin, clr, s)) != PPL_OK) return st;
     HIP_TRY(launch_scatter(clr, d_node_indices, node_start, n_nodes, d_clearance, s));
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_mitstar_edge_cc(ppl_robot r, const ppl_scene_view* view, const float* d_nodes, int dim, const int* d_src,
                                const int* d_dst, const int* d_checks_done, int n_edges, float valid_segment_length,
                                int num_sparse_checks, float margin, uint8_t* d_cc, int* d_new_checks,
-                               ppl_stream stream) {
+                               ppl_stream stream) try {
     CHECK_ARG(r && d_nodes && d_src && d_dst && d_cc, "NULL argument");
     CHECK_ARG(dim == r->R.n_dof, "dim %d != robot n_dof %d", dim, r->R.n_dof);
     CHECK_ARG(n_edges >= 0, "negative edge count");
@@ -2086,10 +2130,12 @@ ppl_status ppl_mitstar_edge_cc(ppl_robot r, const ppl_scene_view* view, const fl
     }
     HIP_TRY(launch_bfs_finish(first_hit, cnt, n_edges, d_cc, d_new_checks, s));
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_plan_autotune_cost(ppl_plan P, const ppl_scene_view* view, const float* d_x, const ppl_cost_params* prm,
-                                  int grad, int reps, int* best_group, float* best_ms, ppl_stream stream) {
+                                  int grad, int reps, int* best_group, float* best_ms, ppl_stream stream) try {
     CHECK_ARG(P && view && d_x && prm, "NULL argument");
     CHECK_ARG(P->kind != PPL_PLAN_EDGES, "cost autotuning needs a configuration or trajectory plan");
     CHECK_ARG(reps >= 1 && reps <= 1000, "reps must be in [1, 1000]");
@@ -2132,9 +2178,11 @@ ppl_status ppl_plan_autotune_cost(ppl_plan P, const ppl_scene_view* view, const 
     if (best_group) *best_group = best;
     if (best_ms) *best_ms = best_t / reps;
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
-ppl_status ppl_plan_set_cost_group(ppl_plan P, int grad, int lanes_per_state) {
+ppl_status ppl_plan_set_cost_group(ppl_plan P, int grad, int lanes_per_state) try {
     CHECK_ARG(P, "NULL argument");
     CHECK_ARG(lanes_per_state == 0 || (lanes_per_state >= 1 && lanes_per_state <= kMaxGroup &&
                                        (lanes_per_state & (lanes_per_state - 1)) == 0),
@@ -2147,21 +2195,29 @@ ppl_status ppl_plan_set_cost_group(ppl_plan P, int grad, int lanes_per_state) {
     }
     P->tuned_group[grad ? 1 : 0] = lanes_per_state;
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_hinge_cost_batched(ppl_plan P, const ppl_scene_view* view, const float* d_x, const ppl_cost_params* prm,
-                                  float* d_state, float* d_total, float* d_grad, ppl_stream stream) {
+                                  float* d_state, float* d_total, float* d_grad, ppl_stream stream) try {
     return hinge_impl(P, view, d_x, prm, d_state, d_total, d_grad, (hipStream_t)stream);
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_chomp_fd_grad_batched(ppl_plan P, const ppl_scene_view* view, const float* d_theta,
-                                     const ppl_cost_params* prm, float eps, float* d_grad, ppl_stream stream) {
+                                     const ppl_cost_params* prm, float eps, float* d_grad, ppl_stream stream) try {
     return fd_impl(P, view, d_theta, prm, eps, d_grad, (hipStream_t)stream);
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_stomp_rollout_costs(ppl_plan P, const ppl_scene_view* view, const float* d_roll, const ppl_cost_params* prm,
-                                   float w_u, float dt, float* d_state, float* d_total, ppl_stream stream) {
+                                   float w_u, float dt, float* d_state, float* d_total, ppl_stream stream) try {
     return stomp_impl(P, view, d_roll, prm, w_u, dt, d_state, d_total, (hipStream_t)stream);
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 // Single-trajectory plain forms share one cached plan per (robot, T).
@@ -2194,7 +2250,7 @@ __global__ void k_broadcast_rows(float* x, int D, int T) {
 extern "C" {
 
 ppl_status ppl_chomp_state_cost(ppl_robot r, const ppl_scene_view* view, const float* d_theta, int T,
-                                const ppl_cost_params* prm, float* d_state_costs, ppl_stream stream) {
+                                const ppl_cost_params* prm, float* d_state_costs, ppl_stream stream) try {
     CHECK_ARG(r && prm && d_theta && d_state_costs, "NULL argument");
     CHECK_ARG(T >= 1, "T must be >= 1");
     ppl_plan P = nullptr;
@@ -2208,10 +2264,12 @@ ppl_status ppl_chomp_state_cost(ppl_robot r, const ppl_scene_view* view, const f
         if (e != hipSuccess) st = fail(PPL_ERR_HIP, "chomp_state_cost: %s", hipGetErrorString(e));
     }
     return st;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_chomp_fd_grad(ppl_robot r, const ppl_scene_view* view, const float* d_theta, int T,
-                             const ppl_cost_params* prm, float eps, float* d_grad, ppl_stream stream) {
+                             const ppl_cost_params* prm, float eps, float* d_grad, ppl_stream stream) try {
     CHECK_ARG(r && prm && d_theta && d_grad, "NULL argument");
     CHECK_ARG(T >= 1, "T must be >= 1");
     ppl_plan P = nullptr;
@@ -2224,6 +2282,8 @@ ppl_status ppl_chomp_fd_grad(ppl_robot r, const ppl_scene_view* view, const floa
         if (e != hipSuccess) st = fail(PPL_ERR_HIP, "chomp_fd_grad: %s", hipGetErrorString(e));
     }
     return st;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 }  // extern "C"
@@ -2362,30 +2422,36 @@ ppl_status solve_single(ppl_robot r, ppl_opt_buffers_s* bufs, const ppl_scene_vi
 
 extern "C" {
 
-ppl_status ppl_chomp_buffers_create(int n_dof, int num_timesteps, ppl_chomp_buffers* out) {
+ppl_status ppl_chomp_buffers_create(int n_dof, int num_timesteps, ppl_chomp_buffers* out) try {
     return opt_buffers_create(n_dof, num_timesteps, out);
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_stomp_buffers_create(int n_dof, int num_timesteps, int num_rollouts_new, int num_rollouts_old,
-                                    ppl_stomp_buffers* out) {
+                                    ppl_stomp_buffers* out) try {
     CHECK_ARG(num_rollouts_new >= 1 && num_rollouts_old >= 0 && num_rollouts_new + num_rollouts_old + 1 <= 1024,
               "need 1 <= new, 0 <= old, new + old + 1 <= 1024 rollouts");
     return opt_buffers_create(n_dof, num_timesteps, out);
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
-ppl_status ppl_chomp_buffers_destroy(ppl_chomp_buffers b) {
+ppl_status ppl_chomp_buffers_destroy(ppl_chomp_buffers b) try {
     if (!b) return PPL_OK;
     b->dev.release();
     b->host.release();
     delete b;
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_stomp_buffers_destroy(ppl_stomp_buffers b) { return ppl_chomp_buffers_destroy(b); }
 
 ppl_status ppl_chomp_solve(ppl_robot r, ppl_chomp_buffers bufs, const ppl_scene_view* view, const float* h_start,
                            const float* h_goal, int num_timesteps, int num_iterations, float time_limit_ms,
-                           const ppl_chomp_settings* cs, float* h_path, ppl_chomp_result* res, ppl_stream stream) {
+                           const ppl_chomp_settings* cs, float* h_path, ppl_chomp_result* res, ppl_stream stream) try {
     CHECK_ARG(cs, "NULL argument");
     const ppl_cost_params cp{cs->world_margin, cs->self_margin, cs->collision_cost_weight, 1, 1, cs->use_self};
     auto opt = [&](ppl_plan P, float* th, float* best, float* rec, int k, int first, hipStream_t s) {
@@ -2393,11 +2459,13 @@ ppl_status ppl_chomp_solve(ppl_robot r, ppl_chomp_buffers bufs, const ppl_scene_
     };
     return solve_single(r, bufs, view, h_start, h_goal, num_timesteps, num_iterations, time_limit_ms, cp,
                         cs->smoothness_weight, cs->dt, opt, h_path, res, (hipStream_t)stream);
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_stomp_solve(ppl_robot r, ppl_stomp_buffers bufs, const ppl_scene_view* view, const float* h_start,
                            const float* h_goal, int num_timesteps, int num_iterations, float time_limit_ms,
-                           const ppl_stomp_settings* ss, float* h_path, ppl_stomp_result* res, ppl_stream stream) {
+                           const ppl_stomp_settings* ss, float* h_path, ppl_stomp_result* res, ppl_stream stream) try {
     CHECK_ARG(ss, "NULL argument");
     const ppl_cost_params cp{ss->world_margin, ss->self_margin, ss->collision_cost_weight, 1, 1, ss->use_self};
     auto opt = [&](ppl_plan P, float* th, float* best, float* rec, int k, int first, hipStream_t s) {
@@ -2405,10 +2473,12 @@ ppl_status ppl_stomp_solve(ppl_robot r, ppl_stomp_buffers bufs, const ppl_scene_
     };
     return solve_single(r, bufs, view, h_start, h_goal, num_timesteps, num_iterations, time_limit_ms, cp,
                         ss->control_cost_weight, ss->dt, opt, h_path, res, (hipStream_t)stream);
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_esdf_from_obbs(const ppl_scene_view* view, const float* h_params, const float* h_pose, float* d_features,
-                              ppl_stream stream) {
+                              ppl_stream stream) try {
     CHECK_ARG(view && h_params && h_pose && d_features, "NULL argument");
     DScene S{};
     ppl_status st = scene_from_view(view, S);
@@ -2417,6 +2487,8 @@ ppl_status ppl_esdf_from_obbs(const ppl_scene_view* view, const float* h_params,
     CHECK_ARG(h_params[3] > 0.0f, "voxel size must be positive");
     HIP_TRY(launch_esdf_build(S.obb_dims, S.obb_pose, S.n_obbs, h_params, h_pose, d_features, (hipStream_t)stream));
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 // ---------------------------------------------------------------------------
@@ -2463,36 +2535,44 @@ extern "C" {
 
 ppl_status ppl_graph_capture_stomp(ppl_plan P, const ppl_scene_view* view, const float* d_roll, const ppl_cost_params* prm,
                                    float w_u, float dt, float* d_state, float* d_total, ppl_stream stream,
-                                   ppl_graph* out) {
+                                   ppl_graph* out) try {
     CHECK_ARG(P && prm, "NULL argument");
     // Allocate every scratch buffer before capture (no allocation inside a graph).
     ppl_status st = stomp_impl(P, view, d_roll, prm, w_u, dt, d_state, d_total, (hipStream_t)stream);
     if (st != PPL_OK) return st;
     HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
     return capture([&](hipStream_t cs) { return stomp_impl(P, view, d_roll, prm, w_u, dt, d_state, d_total, cs); }, out);
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 ppl_status ppl_graph_capture_hinge(ppl_plan P, const ppl_scene_view* view, const float* d_x, const ppl_cost_params* prm,
-                                   float* d_state, float* d_total, float* d_grad, ppl_stream stream, ppl_graph* out) {
+                                   float* d_state, float* d_total, float* d_grad, ppl_stream stream, ppl_graph* out) try {
     CHECK_ARG(P && prm, "NULL argument");
     ppl_status st = hinge_impl(P, view, d_x, prm, d_state, d_total, d_grad, (hipStream_t)stream);
     if (st != PPL_OK) return st;
     HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
     return capture([&](hipStream_t cs) { return hinge_impl(P, view, d_x, prm, d_state, d_total, d_grad, cs); }, out);
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
-ppl_status ppl_graph_launch(ppl_graph g, ppl_stream stream) {
+ppl_status ppl_graph_launch(ppl_graph g, ppl_stream stream) try {
     CHECK_ARG(g && g->exec, "NULL graph");
     HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
-ppl_status ppl_graph_destroy(ppl_graph g) {
+ppl_status ppl_graph_destroy(ppl_graph g) try {
     if (!g) return PPL_OK;
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     delete g;
     return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
 }  // extern "C"
