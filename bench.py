@@ -18,8 +18,9 @@ timed on this host's cores; parity of the timed outputs vs the FP64 oracle.
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-extras] [--legs a,b,...] [--no-cpu]
 
 --legs picks the extra measurements (skip, any, edges_plain, edges_free, chomp,
-stomp, chomp_opt, stomp_opt, mitstar, rrtc, esdf, copy); the headline always runs. --no-cpu drops the CPU baseline leg
-(for profiler runs, whose JSON line then carries cpu_baseline null).
+stomp, chomp_opt, stomp_opt, mitstar, rrtc, esdf, copy, cpu_table); the headline always runs. --no-cpu drops the
+CPU baseline legs (for profiler runs, whose JSON line then carries cpu_baseline null). cpu_table (not run by
+default) times the FP32 CPU restatement of the other workloads and checks their GPU outputs against FP64.
 """
 from __future__ import annotations
 
@@ -163,8 +164,9 @@ def main():
                 ("mitstar", "mitstar", lambda: bench_mitstar(panda, S, W, torch, dev, args)),
                 ("rrtc", "prrtc", lambda: bench_rrtc(panda, S, W, torch, dev, args)),
                 ("esdf", "esdf", lambda: bench_esdf(S, W, torch, dev, args)),
-                ("copy", "copy", lambda: bench_copy(torch, dev, args))):
-            if key not in legs:
+                ("copy", "copy", lambda: bench_copy(torch, dev, args)),
+                ("cpu_table", "cpu_table", lambda: cpu_table_leg(S, W, torch, dev, args))):
+            if key not in legs or (key == "cpu_table" and args.no_cpu):
                 continue
             try:
                 extras[name] = fn()
@@ -230,6 +232,131 @@ def cpu_baseline_leg(e, fh_gpu, args):
            "sample": f"first {n} states of the edges workload, FP32 two-phase check, median of {reps}",
            "one_core_value": cpu_1, "cpu_model": cpu_model()}
     return cpu, parity, f_state
+
+
+def cpu_table_leg(S, W, torch, dev, args):
+    """The CPU-baseline leg widened to the other workloads: each one's FP32 CPU
+    restatement timed on this host (all cores, then one core on a bounded
+    sample), and the GPU output's worst error against the FP64 oracle with the
+    count of results inside the ambiguous band (clearance within 1e-5 of zero,
+    gradient kinks). GPU rates come from the other legs."""
+    import numpy as np
+    import pyoracle as PO
+    from prismplan_amd.planning_scene import HostScene
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    threads = max(1, min(threads, len(os.sched_getaffinity(0))))
+
+    def rate(fn, units, reps=3):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return units / float(np.median(ts))
+
+    def rows(theta, off, D):   # ragged [B] x [D][T_b] -> [waypoints, D]
+        return np.concatenate([theta[D * off[b]:D * off[b + 1]].reshape(D, -1).T for b in range(len(off) - 1)])
+
+    out = {"threads": threads}
+    # any-collision: 64 requests x 4,096 configs, 20 OBBs each
+    panda = S.Robot("panda")
+    reqs = W.any_collision(n_requests=64)
+    o, o1 = PO.Oracle(panda.arrays, reqs[0].scene, nthreads=threads), PO.Oracle(panda.arrays, reqs[0].scene, nthreads=1)
+    mism = amb = 0
+    for r in reqs:
+        o.set_scene(r.scene)
+        got = S.check_collision_scene(panda, S.DeviceScene.from_host(r.scene), r.configs)
+        ref, clr = o.check_collision(r.configs)
+        near = np.abs(clr) < 1e-5
+        mism += int(np.sum((got != ref) & ~near))
+        amb += int(near.sum())
+    o.set_scene(reqs[0].scene)
+    o1.set_scene(reqs[0].scene)
+    q0 = np.concatenate([r.configs for r in reqs[:16]])
+    out["any_collision"] = {"cpu_configs_per_s": rate(lambda: o.check_collision_f32_timed(q0), len(q0)),
+                            "cpu_1core_configs_per_s": rate(lambda: o1.check_collision_f32_timed(q0[:16384]), 16384, 1),
+                            "mismatches_outside_band": mism, "ambiguous": amb, "configs": 64 * 4096}
+    # CHOMP: 256 trajectories x 64 waypoints, 40 OBBs, margin 0.05
+    w = W.chomp()
+    D, off = panda.n_dof, w.offsets
+    ds = S.DeviceScene.from_host(w.scene)
+    plan = S.Plan.trajectories(panda, w.lengths())
+    th = torch.from_numpy(w.theta).to(dev)
+    prm = S.cost_params(world_margin=0.05, weight=1.0, use_self=False)
+    prm_o = PO.cost_params(world_margin=0.05, weight=1.0, use_obb=True, use_esdf=True, use_self=False)
+    _, tot, g = S.hinge_cost(plan, ds, th, prm, grad=True)
+    o = PO.Oracle(panda.arrays, w.scene, nthreads=threads)
+    o1 = PO.Oracle(panda.arrays, w.scene, nthreads=1)
+    tot_ref, g_ref, kink = o.traj_cost_grad(w.theta, off, prm_o)
+    gr, gg = rows(g_ref, off, D), rows(g.cpu().numpy(), off, D)
+    den = np.maximum(np.abs(gr).max(axis=1), 1e-3)
+    gerr = (np.abs(gg - gr).max(axis=1) / den)[~kink]
+    fd = S.chomp_fd_grad(plan, ds, th, prm, eps=1e-3).cpu().numpy()
+    fd_ref = o.chomp_fd_grad(w.theta, off, prm_o, eps=1e-3)
+    fr = rows(fd_ref, off, D)
+    fd_err = np.abs(rows(fd, off, D) - fr).max(axis=1) / np.maximum(np.abs(fr).max(axis=1), 1e-3)
+    qw = rows(w.theta, off, D)
+    B = w.n_traj
+    out["chomp"] = {"cpu_adjoint_traj_per_s": rate(lambda: o.hinge_grad_f32_timed(qw, prm_o), B),
+                    "cpu_1core_adjoint_traj_per_s": rate(lambda: o1.hinge_grad_f32_timed(qw[:64 * 32], prm_o), 32, 1),
+                    "cpu_fd_traj_per_s": rate(lambda: o.chomp_fd_grad(w.theta, off, prm_o, 1e-3, "f32"), B, 1),
+                    "cpu_1core_fd_traj_per_s": rate(lambda: o1.chomp_fd_grad(w.theta[:D * off[8]], off[:9], prm_o,
+                                                                              1e-3, "f32"), 8, 1),
+                    "adjoint_grad_max_rel_err": float(gerr.max()), "kinks": int(kink.sum()),
+                    "total_max_rel_err": float(np.max(np.abs(tot.cpu().numpy() - tot_ref) /
+                                                      np.maximum(np.abs(tot_ref), 0.05))),
+                    "fd_grad_max_rel_err": float(fd_err[~kink].max())}
+    # STOMP: Fetch, 128 problems x 32 rollouts, self pairs on
+    wk = W.stomp()
+    fetch = S.Robot("fetch")
+    R = wk.n_rollouts
+    plan = S.Plan.trajectories(fetch, np.repeat(wk.lengths, R), problem_units=np.arange(wk.n_problems + 1) * R,
+                               obb_offsets=wk.obb_offsets, waypoint_major=True)
+    ro = torch.from_numpy(wk.rollouts).to(dev)
+    prm = S.cost_params(world_margin=0.02, self_margin=0.01, weight=20.0)
+    prm_o = PO.cost_params(world_margin=0.02, self_margin=0.01, weight=20.0)
+    st, _ = S.stomp_rollout_costs(plan, S.DeviceScene.from_host(wk.scene), ro, prm, 0.1, 0.1)
+    o = PO.Oracle(wk.robot, nthreads=threads)
+    o1 = PO.Oracle(wk.robot, nthreads=1)
+    s_ref, _ = o.stomp_costs(wk.scene, wk.obb_offsets, wk.rollouts, wk.lengths, wk.roll_offsets, R, prm_o, 0.1, 0.1)
+    rel = np.abs(st.cpu().numpy() - s_ref) / np.maximum(np.abs(s_ref), 20.0 * 0.02)
+    P1 = 4
+    out["stomp"] = {"cpu_rollout_states_per_s": rate(lambda: o.stomp_costs(
+                        wk.scene, wk.obb_offsets, wk.rollouts, wk.lengths, wk.roll_offsets, R, prm_o, 0.1, 0.1, "f32"),
+                        wk.n_states, 1),
+                    "cpu_1core_rollout_states_per_s": rate(lambda: o1.stomp_costs(
+                        wk.scene, wk.obb_offsets[:P1 + 1], wk.rollouts[:int(wk.roll_offsets[P1]) * fetch.n_dof],
+                        wk.lengths[:P1], wk.roll_offsets[:P1 + 1], R, prm_o, 0.1, 0.1, "f32"),
+                        int(wk.roll_offsets[P1]), 1),
+                    "state_cost_max_rel_err": float(rel.max()), "states": int(wk.n_states)}
+    # Baxter trilinear ESDF adjoint: 1,024 trajectories, 128^3 grid from 60 OBBs
+    baxter = S.Robot("baxter")
+    wk = W.esdf()
+    params, pose = wk.scene.voxel_params, wk.scene.voxel_pose
+    feats = S.esdf_from_obbs(S.DeviceScene.from_host(wk.scene.source_obbs), params[0], pose[0])
+    wk.scene.voxel_features = feats.cpu().numpy().reshape(1, -1)
+    plan = S.Plan.trajectories(baxter, wk.lengths())
+    th = torch.from_numpy(wk.theta).to(dev)
+    prm = S.cost_params(world_margin=0.02, weight=1.0, use_obb=False, use_self=False)
+    prm_o = PO.cost_params(world_margin=0.02, weight=1.0, use_obb=False, use_self=False)
+    _, tot, g = S.hinge_cost(plan, S.DeviceScene.from_host(wk.scene), th, prm, grad=True)
+    o = PO.Oracle(wk.robot, wk.scene, nthreads=threads)
+    o1 = PO.Oracle(wk.robot, wk.scene, nthreads=1)
+    tot_ref, g_ref, kink = o.traj_cost_grad(wk.theta, wk.offsets, prm_o)
+    Db = baxter.n_dof
+    gr, gg = rows(g_ref, wk.offsets, Db), rows(g.cpu().numpy(), wk.offsets, Db)
+    interior = np.ones(len(gr), bool)
+    interior[wk.offsets[:-1]] = False
+    interior[wk.offsets[1:] - 1] = False
+    gerr = (np.abs(gg - gr).max(axis=1) / np.maximum(np.abs(gr).max(axis=1), 1e-3))[interior & ~kink]
+    qw = rows(wk.theta, wk.offsets, Db)
+    out["esdf"] = {"cpu_waypoints_per_s": rate(lambda: o.hinge_grad_f32_timed(qw, prm_o), len(qw)),
+                   "cpu_1core_waypoints_per_s": rate(lambda: o1.hinge_grad_f32_timed(qw[:8192], prm_o), 8192, 1),
+                   "grad_max_rel_err": float(gerr.max()), "kinks": int(kink.sum()),
+                   "total_max_rel_err": float(np.max(np.abs(tot.cpu().numpy() - tot_ref) /
+                                                     np.maximum(np.abs(tot_ref), 0.02)))}
+    return out
 
 
 def bench_copy(torch, dev, args):

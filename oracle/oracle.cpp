@@ -263,6 +263,12 @@ bool esdf_layer(const orc_scene& S, int layer, T x, T y, T z, int sphere_idx, T*
     }
     for (int k = 0; k < 3; ++k)
         if (i0[k] >= n[k] - kOff || i0[k] <= kOff) return false;
+    // The interpolant is continuous across a cell face but its gradient is
+    // not: the normal component is the secant slope of the cell on either
+    // side. A gradient taken within a hair of a face is ambiguous.
+    if (grad)
+        for (int k = 0; k < 3; ++k)
+            if (f[k] < (T)1e-4 || f[k] > (T)(1 - 1e-4)) *amb = true;
     T c[8];
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
@@ -785,6 +791,18 @@ double orc_sphere_obb_clearance_f64(const float* dims, const float* pose, double
 double orc_sphere_esdf_clearance_f64(const orc_scene* s, double x, double y, double z, double r, int sphere_idx) {
     bool amb = false;
     return esdf_clearance<double>(*s, x, y, z, r, sphere_idx, nullptr, &amb);
+}
+
+double orc_sphere_esdf_grad_f64(const orc_scene* s, double x, double y, double z, double r, int sphere_idx,
+                                double* grad3, int* amb) {
+    bool a = false;
+    V3<double> g{0, 0, 0};
+    const double c = esdf_clearance<double>(*s, x, y, z, r, sphere_idx, &g, &a);
+    grad3[0] = g.x;
+    grad3[1] = g.y;
+    grad3[2] = g.z;
+    *amb = a ? 1 : 0;
+    return c;
 }
 
 void orc_check_collision_f64(const orc_robot* r, const orc_scene* s, const float* q, int n, float margin,

@@ -213,6 +213,16 @@ class Oracle:
                                  self.nthreads)
         return cost, grad, kink.astype(bool)
 
+    def hinge_grad_f32_timed(self, q, params: OrcCostParams):
+        """FP32 hinge cost and gradient per configuration (the timed CPU path)."""
+        q = _c(q, np.float32).reshape(-1, self.robot.n_dof)
+        n = q.shape[0]
+        cost = np.zeros(n, np.float32)
+        grad = np.zeros((n, self.robot.n_dof), np.float32)
+        lib().orc_hinge_grad_f32(C.byref(self.r), C.byref(self.s), _ptr(q, _f32p), n, C.byref(params),
+                                 _ptr(cost, _f32p), _ptr(grad, _f32p), self.nthreads)
+        return cost, grad
+
     # ---- trajectories (ragged [B] x [D][T_b]) ----
     def traj_cost_grad(self, theta_flat, off, params: OrcCostParams, tol=1e-5):
         th = _c(theta_flat, np.float32)
@@ -348,6 +358,9 @@ def _setup_restypes():
     L.orc_sphere_esdf_clearance_f64.argtypes = [C.POINTER(OrcScene), C.c_double, C.c_double, C.c_double,
                                                 C.c_double, C.c_int]
     L.orc_edge_granularity.restype = C.c_int
+    L.orc_sphere_esdf_grad_f64.restype = C.c_double
+    L.orc_sphere_esdf_grad_f64.argtypes = [C.POINTER(OrcScene), C.c_double, C.c_double, C.c_double, C.c_double,
+                                           C.c_int, _f64p, C.POINTER(C.c_int)]
 
 
 def sphere_obb_clearance(dims, pose, p, r) -> float:
@@ -367,3 +380,13 @@ def sphere_obb_collide(dims, pose, p, r) -> bool:
 def sphere_esdf_clearance(orc: Oracle, p, r, sphere_idx=-1) -> float:
     _setup_restypes()
     return lib().orc_sphere_esdf_clearance_f64(C.byref(orc.s), *map(float, p), float(r), int(sphere_idx))
+
+
+def sphere_esdf_grad(orc: Oracle, p, r, sphere_idx=-1):
+    """(clearance, d clearance / d centre [3], ambiguous) of one sphere."""
+    _setup_restypes()
+    g = np.zeros(3, np.float64)
+    amb = C.c_int(0)
+    c = lib().orc_sphere_esdf_grad_f64(C.byref(orc.s), *map(float, p), float(r), int(sphere_idx), _ptr(g, _f64p),
+                                       C.byref(amb))
+    return c, g, bool(amb.value)

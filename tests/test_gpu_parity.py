@@ -514,6 +514,37 @@ def test_esdf_build_and_baxter_trilinear_adjoint(robots):
     assert worst < 1e-4
 
 
+def test_esdf_adjoint_at_full_size(robots):
+    """The benchmark's ESDF workload at its full size (1,024 Baxter
+    trajectories, 81,740 waypoints, 128^3 grid at 1 cm built on the GPU):
+    totals within 1e-5 and gradients within 1e-4 of FP64, away from kinks
+    (hinge boundaries and voxel faces, where the trilinear gradient jumps)."""
+    wk = W.esdf()
+    banner(f"ESDF adjoint, full size: {wk.n_traj} trajectories, {wk.n_waypoints} waypoints")
+    params, pose = wk.scene.voxel_params, wk.scene.voxel_pose
+    feats = S.esdf_from_obbs(S.DeviceScene.from_host(wk.scene.source_obbs), params[0], pose[0])
+    wk.scene.voxel_features = feats.cpu().numpy().reshape(1, -1)
+    plan = S.Plan.trajectories(robots["baxter"], wk.lengths())
+    th = torch.from_numpy(wk.theta).to(DEV)
+    prm = S.cost_params(world_margin=0.02, weight=1.0, use_obb=False, use_self=False)
+    _, total, g = S.hinge_cost(plan, S.DeviceScene.from_host(wk.scene), th, prm, grad=True)
+    o = PO.Oracle(wk.robot, wk.scene, nthreads=16)
+    tot_ref, g_ref, kink = o.traj_cost_grad(wk.theta, wk.offsets,
+                                            PO.cost_params(world_margin=0.02, weight=1.0, use_obb=False,
+                                                           use_self=False))
+    rel = np.abs(total.cpu().numpy() - tot_ref) / np.maximum(np.abs(tot_ref), 0.02)
+    g = traj_rows(g.cpu().numpy(), wk.offsets, 14)
+    gr = traj_rows(g_ref, wk.offsets, 14)
+    interior = np.ones(len(gr), bool)
+    interior[wk.offsets[:-1]] = False
+    interior[wk.offsets[1:] - 1] = False
+    worst = grad_rel_err(g, gr)[interior & ~kink].max()
+    banner(f"  total max rel err {rel.max():.2e}, grad max rel err {worst:.2e}, kinks {int(kink.sum())}")
+    assert rel.max() < 1e-5
+    assert worst < 1e-4
+    assert kink.mean() < 0.05
+
+
 def test_prrtc_compatible_surface():
     from prismplan_amd import prrtc
     from prismplan_amd.planning_scene import PlanningScene

@@ -248,6 +248,39 @@ def test_trilinear_reproduces_a_plane_field():
         assert c == pytest.approx(-(p @ a - 0.2) - 0.02, abs=1e-5)
 
 
+def test_trilinear_gradient_is_the_cell_secant_and_jumps_at_faces():
+    """Field d = k x^2 (world x): inside the cell [x_i, x_i+1] the trilinear
+    gradient's x component is the secant slope (F_i+1 - F_i) / vs, about
+    k (x_i + x_i+1), negated for clearance, with no y or z part. Across the
+    face at x_i+1 it jumps by about 2 k vs, so a lookup within a hair of a
+    face is flagged ambiguous, and one mid-cell is not."""
+    from prismplan_amd.workloads import esdf_grid
+    vs, n, kq = 0.05, 31, 3.0
+    params, pose = esdf_grid(center=(0.3, -0.1, 0.5), n=n, voxel=vs)
+    ii = np.arange(n)
+    node_x = ii * vs - params[0, 0] / 2 - pose[0, 0]
+    vals = np.broadcast_to((kq * node_x ** 2)[:, None, None], (n, n, n)).astype(np.float32)
+    s = HostScene(voxel_params=params, voxel_pose=pose, voxel_enable=np.ones(1, np.uint8),
+                  voxel_features=vals.reshape(1, -1).copy(), esdf_interp=1)
+    o = PO.Oracle(robot_arrays("panda"), s)
+    y, z = -0.1 + 0.5 * vs, 0.5 + 0.5 * vs   # mid-cell in y and z (grid centre (0.3, -0.1, 0.5))
+    i = 12
+    secant = lambda j: -(float(vals[j + 1, 0, 0]) - float(vals[j, 0, 0])) / vs
+    c, g, amb = PO.sphere_esdf_grad(o, (node_x[i] + 0.5 * vs, y, z), 0.01)
+    assert not amb
+    assert g[0] == pytest.approx(secant(i), rel=1e-5)
+    assert g[0] == pytest.approx(-kq * (node_x[i] + node_x[i + 1]), rel=1e-4)
+    assert abs(g[1]) < 1e-9 and abs(g[2]) < 1e-9
+    assert c == pytest.approx(-kq * (node_x[i] + 0.5 * vs) ** 2 - 0.01, abs=kq * vs * vs)
+    below = PO.sphere_esdf_grad(o, (node_x[i + 1] - 1e-6, y, z), 0.01)
+    above = PO.sphere_esdf_grad(o, (node_x[i + 1] + 1e-6, y, z), 0.01)
+    assert below[2] and above[2]
+    assert below[1][0] == pytest.approx(secant(i), rel=1e-4)
+    assert above[1][0] == pytest.approx(secant(i + 1), rel=1e-4)
+    assert above[1][0] - below[1][0] == pytest.approx(-2 * kq * vs, rel=1e-3)
+    assert below[0] == pytest.approx(above[0], abs=1e-5)   # the value itself is continuous
+
+
 def test_nearest_lookup_uses_truncated_index():
     s, a = plane_scene(interp=0)
     o = PO.Oracle(robot_arrays("panda"), s)
