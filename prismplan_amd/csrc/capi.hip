@@ -10,6 +10,7 @@
 #include <array>
 #include <cfloat>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -323,8 +324,9 @@ int edge_g(const float* a, const float* b, int dim, const ppl_edge_params& p, fl
     const float len = std::sqrt(len2);
     *len_out = len;
     if (!(p.resolution > 0.0f)) return p.granularity;
-    int g = (int)std::ceil(len / p.resolution);
-    return std::min(std::max(g, 2), 1024);
+    const float gf = std::ceil(len / p.resolution);   // clamped before the cast: huge or NaN -> 1024
+    const int g = gf < 1024.0f ? (int)gf : 1024;
+    return std::max(g, 2);
 }
 
 // Self pass. mode 0: any-collision; mode 1: min clearance, exact below `band`
@@ -887,7 +889,9 @@ ppl_status ppl_edge_eval_evaluate(ppl_edge_eval ev, const float* h_from, const f
     off[0] = 0;
     for (int e = 0; e < n; ++e) {
         const int G = edge_g(h_from + (size_t)e * D, h_to + (size_t)e * D, D, *p, &len[e]);
-        off[e + 1] = off[e] + std::max(G, 0);
+        const long next = (long)off[e] + std::max(G, 0);
+        if (next > INT_MAX) return fail(PPL_ERR_CAPACITY, "more than 2^31 - 1 edge states in one call");
+        off[e + 1] = (int)next;
     }
     const long total = off[n];
     const size_t cfg = (size_t)n * D * sizeof(float);

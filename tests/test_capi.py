@@ -169,3 +169,30 @@ def test_chomp_solve_rejects_bad_arguments():
     res = _ppl.ChompResult()
     assert L.ppl_chomp_solve(None, None, None, None, None, 51, 10, C.c_float(0), None, None, C.byref(res),
                              None) == _ppl.PPL_ERR_INVALID
+
+
+def test_edge_granularities_follow_the_rule_at_the_extremes():
+    """The host-side granularity rule of the edge evaluator (no device work)
+    against the oracle's restatement, including lengths whose ratio to the
+    resolution overflows an int and a NaN endpoint (both clamp to 1024)."""
+    import sys
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+    import pyoracle as PO
+    from prismplan_amd import substrate as S
+    rng = np.random.default_rng(3)
+    a = rng.uniform(-2, 2, (200, 7)).astype(np.float32)
+    b = (a + rng.normal(0, 0.3, (200, 7))).astype(np.float32)
+    b[0] = a[0]                                          # zero length
+    G, ln = S.edge_granularities(a, b, 32, 0.025)
+    for k in range(200):
+        g_ref, l_ref = PO.edge_granularity(a[k], b[k], 32, 0.025)
+        assert G[k] == g_ref and ln[k] == pytest.approx(l_ref, rel=1e-6)
+    assert G[0] == 2
+    G, _ = S.edge_granularities(a[:3], b[:3], 32, 1e-30)   # len / res far above 2^31
+    assert list(G) == [2, 1024, 1024]                      # the zero-length edge stays at 2
+    nan = b[:2].copy()
+    nan[1, 0] = np.nan
+    G, _ = S.edge_granularities(a[:2], nan, 32, 0.025)
+    assert G[1] == 1024
+    G, _ = S.edge_granularities(a[:2], b[:2], 17, 0.0)     # fixed granularity
+    assert list(G) == [17, 17]
