@@ -1,7 +1,8 @@
 """ctypes front end of oracle/_build/liboracle.so -- TEST INFRASTRUCTURE ONLY.
 
-Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
-parity checker and the timed CPU path; never by the product package.
+Used by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline legs
+(cpu_baseline, cpu_table) as the parity checker and the timed CPU path; never
+by the product package.
 """
 from __future__ import annotations
 
