@@ -3,10 +3,11 @@
 Follows planning/pRRTC.cu (Halton sampling :955-995, edge states :1668-1677
 and :1852-1866, the tree walk :1962-1988, the goal -> start assembly
 :2167-2186) and planning/shortcut.cuh:16-274 (skip-edge states and the
-greedy pass). The planner itself is concurrent and not reproducible state
-for state; its outputs are checked against what these functions and the
-FP64 collision oracle fix: bases, samples, edge states, path validity,
-costs and the shortcut's choices.
+greedy pass). With many workgroups the planner is concurrent and not
+reproducible state for state; its outputs are checked against what these
+functions and the FP64 collision oracle fix: bases, samples, edge states,
+path validity, costs and the shortcut's choices. With one workgroup it is
+sequential, and rrtc_single_workgroup restates the whole run.
 
 One documented difference of the product, shared here: the reference
 shuffles each workgroup's Halton bases with cuRAND (pRRTC.cu:59-66, 961);
@@ -141,3 +142,159 @@ def shortcut_greedy(n: int, free) -> list:
         keep.append(far)
         i = far
     return keep
+
+
+class Ambiguous(Exception):
+    """An edge state lies within the ambiguous clearance band: the FP32 check
+    may decide either way, so the sequential run cannot be reproduced."""
+
+
+def _fma32(a, b, c):
+    """FP32 fma through FP64: the product of two FP32 values is exact in FP64
+    and the FP64 sum is rounded once more to FP32 (exact unless the FP64 sum
+    lands on an FP32 halfway point)."""
+    return np.float32(np.float64(a) * np.float64(b) + np.float64(c))
+
+
+def _nearest(nodes, q):
+    """Index and squared distance of the nearest row: per row, s = fma(t, t, s)
+    over the dims in order, t = node - q in FP32; ties go to the smaller index
+    (pRRTC.cu:1637-1666)."""
+    t = (nodes - q[None, :]).astype(np.float32)
+    s = np.zeros(len(nodes), np.float32)
+    for k in range(nodes.shape[1]):
+        s = (t[:, k].astype(np.float64) ** 2 + s.astype(np.float64)).astype(np.float32)
+    i = int(np.argmin(s))
+    return i, s[i]
+
+
+def rrtc_single_workgroup(start, goals, lower, upper, collides, granularity=16, range_=0.5, balance=1,
+                          tree_ratio=0.5, dynamic_domain=True, dd_alpha=1e-4, dd_radius=4.0, dd_min_radius=1.0,
+                          max_samples=100000, max_iters=100000):
+    """pRRTC with one workgroup (num_new_configs = 1), pRRTC.cu:1556-2010.
+    Without concurrency the run is sequential, so the trees, the iteration
+    that connects them and the path are fixed. Workgroup 0 samples the
+    Halton sequence of the primes in order (pRRTC.cu:955-995). collides(states
+    [k, D] FP32) -> bool says whether any state of one edge check collides.
+    Returns dict(solved, iters, path start -> goal, reached_goal,
+    start_tree_size, goal_tree_size)."""
+    f32 = np.float32
+    FLT_MAX = np.finfo(np.float32).max
+    D = len(start)
+    trees = [[np.asarray(start, f32)], [np.asarray(g, f32) for g in goals]]
+    parents = [[0], list(range(len(goals)))]
+    radii = [[FLT_MAX], [FLT_MAX] * len(goals)]
+    b = np.float32(PRIMES[:D])
+    hn, hd = np.zeros(D, f32), np.ones(D, f32)
+    lo, hi = np.asarray(lower, f32), np.asarray(upper, f32)
+    rng = (hi - lo).astype(f32)
+    G = f32(granularity)
+    one = f32(1.0)
+
+    def halton():
+        out = np.empty(D, f32)
+        for i in range(D):
+            x = f32(hd[i] - hn[i])
+            if x == one:
+                hd[i] = np.floor(f32(hd[i] * b[i]))
+                hn[i] = one
+            else:
+                y = np.floor(f32(hd[i] / b[i]))
+                while x <= y:
+                    y = np.floor(f32(y / b[i]))
+                hn[i] = f32(np.floor(f32((b[i] + one) * y)) - x)
+            out[i] = f32(hn[i] / hd[i])
+        return out
+
+    def edge_free(base, step):
+        states = np.stack([np.float32(np.float64(k + 1) * np.float64(step) + np.float64(base))
+                           for k in range(granularity)])
+        return not collides(states)
+
+    def shrink(t, i):
+        old = radii[t][i]
+        radii[t][i] = f32(dd_radius) if old == FLT_MAX else max(f32(old * f32(one - f32(dd_alpha))), f32(dd_min_radius))
+
+    def grow(t, i):
+        if radii[t][i] != FLT_MAX:
+            radii[t][i] = f32(radii[t][i] * f32(one + f32(dd_alpha)))
+
+    def add(t, q, parent):
+        if len(trees[t]) >= max_samples:
+            return -1
+        trees[t].append(q)
+        parents[t].append(parent)
+        radii[t].append(FLT_MAX)
+        return len(trees[t]) - 1
+
+    def result(solved, it, win=None):
+        out = {"solved": solved, "iters": it, "start_tree_size": len(trees[0]), "goal_tree_size": len(trees[1])}
+        if solved:
+            wt, wlast, wo = win
+            ends = {wt: wlast, 1 - wt: wo}
+            walks = []
+            for tree in (0, 1):
+                cur, walk = ends[tree], []
+                while parents[tree][cur] != cur:
+                    walk.append(trees[tree][cur])
+                    cur = parents[tree][cur]
+                walks.append((walk, cur))
+            rows = [trees[0][0]] + walks[0][0][::-1] + walks[1][0] + [trees[1][walks[1][1]]]
+            out.update(path=np.stack(rows), reached_goal=walks[1][1])
+        return out
+
+    t = 0
+    it = 0
+    while True:
+        it += 1
+        if it > max_iters:
+            return result(False, it - 1)
+        f0, f1 = len(trees[0]), len(trees[1])
+        if balance == 0 or it == 1:
+            t = 1                                   # workgroup 0 of 1: 0 < 1 / 2 is false
+        elif balance == 1:
+            ratio = f32(f32(f0) / f32(f0 + f1))
+            if f32(abs(f0 - f1)) < f32(1.5):
+                t = 0 if f32(0.0) < f32(one - ratio) else 1
+            else:
+                t = 0 if ratio < f32(tree_ratio) else 1
+        else:
+            ft, fo = (f0, f1) if t == 0 else (f1, f0)
+            t = 1 - t if f32(f32(abs(ft - fo)) / f32(ft)) < f32(tree_ratio) else t
+        cfg = np.array([_fma32(h, r, l) for h, r, l in zip(halton(), rng, lo)], f32)
+        o = 1 - t
+        near, d2 = _nearest(np.stack(trees[t]), cfg)
+        dist = np.sqrt(f32(d2))
+        if dynamic_domain and radii[t][near] < dist:
+            continue
+        nq = trees[t][near]
+        scale = min(one, f32(f32(range_) / dist)) if dist > 0 else one
+        c = np.array([_fma32(f32(cfg[k] - nq[k]), scale, nq[k]) for k in range(D)], f32)
+        step = ((c - nq).astype(f32) / G).astype(f32)
+        if not edge_free(nq, step):
+            if dynamic_domain:
+                shrink(t, near)
+            continue
+        last = add(t, c, near)
+        if last < 0:
+            return result(False, it)
+        if dynamic_domain:
+            grow(t, near)
+        onear, d2o = _nearest(np.stack(trees[o]), c)
+        n_ext = int(np.ceil(f32(np.sqrt(f32(d2o)) / f32(range_))))
+        cur = c
+        if n_ext > 0:
+            vec = ((trees[o][onear] - cur).astype(f32) / f32(n_ext)).astype(f32)
+            stepc = (vec / G).astype(f32)
+        e = 0
+        while e < n_ext:
+            if not edge_free(cur, stepc):
+                break
+            nxt = (cur + vec).astype(f32)
+            idx = add(t, nxt, last)
+            if idx < 0:
+                return result(False, it)
+            cur, last = nxt, idx
+            e += 1
+        if e == n_ext:
+            return result(True, it, (t, last, onear))

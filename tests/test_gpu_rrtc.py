@@ -113,6 +113,45 @@ def test_rrtc_solution_is_valid(robots, name, seed):
     check_solution(o, res, start, goals, st, f"{name}/{seed}")
 
 
+@pytest.mark.parametrize("name,seed,balance,dd", [("panda", 1, 1, True), ("panda", 2, 2, True),
+                                                   ("panda", 3, 0, False), ("fetch", 4, 1, True)])
+def test_single_workgroup_run_matches_restatement(robots, name, seed, balance, dd):
+    """With one workgroup the planner has no concurrency: the device run must
+    reproduce the sequential restatement (oracle/rrtc_oracle.py) row for row:
+    the path, the iteration that connects, both tree sizes and the goal
+    reached. A run whose edge checks touch the ambiguous clearance band is
+    reported and not compared."""
+    rob = robots[name]
+    scene, o, start, goals = problem(rob, seed, n_goals=2)
+    ds = S.DeviceScene.from_host(scene)
+    st = S.rrtc_settings(num_new_configs=1, max_iters=20000, max_samples=50000, time_limit_ms=0.0,
+                         balance=balance, dynamic_domain=int(dd))
+    res = S.rrtc_solve(rob, ds, start, goals, st)
+
+    def collides(states):
+        hit, clr = o.check_collision(states, margin=st.collision_margin, check_self=True)
+        if np.any(np.abs(clr) < BAND):
+            raise RO.Ambiguous()
+        return bool(hit.any())
+
+    try:
+        ref = RO.rrtc_single_workgroup(start, goals, rob.arrays.joint_lower, rob.arrays.joint_upper, collides,
+                                       granularity=st.granularity, range_=st.range, balance=balance,
+                                       tree_ratio=st.tree_ratio, dynamic_domain=dd, dd_alpha=st.dd_alpha,
+                                       dd_radius=st.dd_radius, dd_min_radius=st.dd_min_radius,
+                                       max_samples=st.max_samples, max_iters=st.max_iters)
+    except RO.Ambiguous:
+        banner(f"single workgroup {name}/{seed}: an edge state is ambiguous, run not compared")
+        return
+    banner(f"single workgroup {name}/{seed} balance {balance} dd {dd}: device iters {res['iters']} trees "
+           f"{res['start_tree_size']}+{res['goal_tree_size']}, restatement iters {ref['iters']} trees "
+           f"{ref['start_tree_size']}+{ref['goal_tree_size']}")
+    assert res["solved"] and ref["solved"]
+    assert (res["iters"], res["start_tree_size"], res["goal_tree_size"], res["reached_goal"]) == \
+        (ref["iters"], ref["start_tree_size"], ref["goal_tree_size"], ref["reached_goal"])
+    np.testing.assert_array_equal(res["path"], ref["path"])
+
+
 @pytest.mark.parametrize("balance,dd,gran,margin", [(0, 0, 16, 0.0), (2, 1, 8, 0.01), (1, 1, 32, 0.005)])
 def test_rrtc_modes_and_goals(robots, balance, dd, gran, margin):
     rob = robots["panda"]

@@ -100,3 +100,49 @@ def test_rrtc_settings_defaults_and_arguments():
     assert L.ppl_shortcut_path(None, None, None, 0, C.c_float(0.025), C.c_float(0), None, None, None) == \
         _ppl.PPL_ERR_INVALID
     assert L.ppl_rrtc_halton_bases(C.c_ulonglong(1), 2, 17, None) == _ppl.PPL_ERR_INVALID
+
+
+def test_single_workgroup_run_in_free_space():
+    """In free space the sequential run connects at iteration 1: workgroup 0
+    of 1 grows the goal tree toward the first Halton sample (steered to at most
+    `range`), then extends from that node to the start in ceil(d / range)
+    equal steps (pRRTC.cu:1602-1627, 1668-1677, 1814-1960); the path is start,
+    the extension nodes back to the steered node, goal (pRRTC.cu:2167-2186)."""
+    lo = np.float32([-2.0, -1.0, -3.0])
+    hi = np.float32([2.0, 1.0, 3.0])
+    start = np.float32([0.5, 0.2, -1.0])
+    goal = np.float32([-0.4, 0.1, 1.5])
+    res = RO.rrtc_single_workgroup(start, goal[None], lo, hi, lambda s: False, range_=0.5)
+    assert res["solved"] and res["iters"] == 1 and res["reached_goal"] == 0
+    h = np.float32([1 / 3, 1 / 5, 1 / 7])                       # first samples of bases 3, 5, 7
+    sample = lo + h * (hi - lo)
+    d = np.linalg.norm(sample - goal)
+    c = goal + min(1.0, 0.5 / d) * (sample - goal)
+    n_ext = int(np.ceil(np.linalg.norm(start - c) / 0.5))
+    path = res["path"]
+    assert len(path) == n_ext + 3
+    np.testing.assert_array_equal(path[0], start)
+    np.testing.assert_array_equal(path[-1], goal)
+    np.testing.assert_allclose(path[-2], c, atol=1e-6)
+    np.testing.assert_allclose(path[1], start, atol=1e-5)          # the extension ends on the start
+    seg = np.linalg.norm(np.diff(path.astype(np.float64), axis=0), axis=1)
+    assert seg[1:].max() <= 0.5 + 1e-6
+    assert res["start_tree_size"] == 1 and res["goal_tree_size"] == n_ext + 2
+
+
+def test_single_workgroup_run_grows_around_a_wall():
+    """A wall between start and goal (a 1-D slab the edge states may not
+    enter): the run needs several iterations, and its path stays within
+    `range` per segment and off the wall at every checked state."""
+    lo, hi = np.float32([-1.0, -1.0]), np.float32([1.0, 1.0])
+    start, goal = np.float32([-0.6, 0.0]), np.float32([0.6, 0.0])
+
+    def wall(states):   # x in [-0.05, 0.05] and y < 0.5
+        return bool(np.any((np.abs(states[:, 0]) <= 0.05) & (states[:, 1] < 0.5)))
+
+    res = RO.rrtc_single_workgroup(start, goal[None], lo, hi, wall, range_=0.3, granularity=8, max_iters=5000)
+    assert res["solved"] and res["iters"] > 1
+    p = res["path"]
+    states = np.concatenate([RO.segment_states(p[i], p[i + 1], 8) for i in range(len(p) - 1)])
+    assert not wall(states)
+    assert np.linalg.norm(np.diff(p.astype(np.float64), axis=0), axis=1).max() <= 0.3 + 1e-6
