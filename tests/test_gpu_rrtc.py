@@ -113,19 +113,20 @@ def test_rrtc_solution_is_valid(robots, name, seed):
     check_solution(o, res, start, goals, st, f"{name}/{seed}")
 
 
-@pytest.mark.parametrize("name,seed,balance,dd", [("panda", 1, 1, True), ("panda", 2, 2, True),
-                                                   ("panda", 3, 0, False), ("fetch", 4, 1, True)])
-def test_single_workgroup_run_matches_restatement(robots, name, seed, balance, dd):
+@pytest.mark.parametrize("name,seed,balance,dd,gran,margin", [
+    ("panda", 1, 1, True, 16, 0.0), ("panda", 2, 2, True, 16, 0.0), ("panda", 3, 0, False, 16, 0.0),
+    ("fetch", 4, 1, True, 16, 0.0), ("panda", 5, 1, True, 32, 0.01), ("baxter", 6, 1, True, 8, 0.0)])
+def test_single_workgroup_run_matches_restatement(robots, name, seed, balance, dd, gran, margin):
     """With one workgroup the planner has no concurrency: the device run must
     reproduce the sequential restatement (oracle/rrtc_oracle.py) row for row:
     the path, the iteration that connects, both tree sizes and the goal
     reached. A run whose edge checks touch the ambiguous clearance band is
     reported and not compared."""
     rob = robots[name]
-    scene, o, start, goals = problem(rob, seed, n_goals=2)
+    scene, o, start, goals = problem(rob, seed, margin=margin, n_goals=2)
     ds = S.DeviceScene.from_host(scene)
     st = S.rrtc_settings(num_new_configs=1, max_iters=20000, max_samples=50000, time_limit_ms=0.0,
-                         balance=balance, dynamic_domain=int(dd))
+                         balance=balance, dynamic_domain=int(dd), granularity=gran, collision_margin=margin)
     res = S.rrtc_solve(rob, ds, start, goals, st)
 
     def collides(states):
@@ -143,7 +144,7 @@ def test_single_workgroup_run_matches_restatement(robots, name, seed, balance, d
     except RO.Ambiguous:
         banner(f"single workgroup {name}/{seed}: an edge state is ambiguous, run not compared")
         return
-    banner(f"single workgroup {name}/{seed} balance {balance} dd {dd}: device iters {res['iters']} trees "
+    banner(f"single workgroup {name}/{seed} balance {balance} dd {dd} granularity {gran} margin {margin}: device iters {res['iters']} trees "
            f"{res['start_tree_size']}+{res['goal_tree_size']}, restatement iters {ref['iters']} trees "
            f"{ref['start_tree_size']}+{ref['goal_tree_size']}")
     assert res["solved"] and ref["solved"]
