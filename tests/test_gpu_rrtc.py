@@ -113,6 +113,35 @@ def test_rrtc_solution_is_valid(robots, name, seed):
     check_solution(o, res, start, goals, st, f"{name}/{seed}")
 
 
+def single_workgroup_check(rob, ds, o, start, goals, st, tag):
+    """Solve with the device (st.num_new_configs = 1) and with the sequential
+    restatement; compare path rows, connecting iteration, tree sizes and goal."""
+    res = S.rrtc_solve(rob, ds, start, goals, st)
+
+    def collides(states):
+        hit, clr = o.check_collision(states, margin=st.collision_margin, check_self=True)
+        if np.any(np.abs(clr) < BAND):
+            raise RO.Ambiguous()
+        return bool(hit.any())
+
+    try:
+        ref = RO.rrtc_single_workgroup(start, goals, rob.arrays.joint_lower, rob.arrays.joint_upper, collides,
+                                       granularity=st.granularity, range_=st.range, balance=st.balance,
+                                       tree_ratio=st.tree_ratio, dynamic_domain=bool(st.dynamic_domain),
+                                       dd_alpha=st.dd_alpha, dd_radius=st.dd_radius, dd_min_radius=st.dd_min_radius,
+                                       max_samples=st.max_samples, max_iters=st.max_iters)
+    except RO.Ambiguous:
+        banner(f"single workgroup {tag}: an edge state is ambiguous, run not compared")
+        return
+    banner(f"single workgroup {tag}: device iters {res['iters']} trees "
+           f"{res['start_tree_size']}+{res['goal_tree_size']}, restatement iters {ref['iters']} trees "
+           f"{ref['start_tree_size']}+{ref['goal_tree_size']}")
+    assert res["solved"] and ref["solved"]
+    assert (res["iters"], res["start_tree_size"], res["goal_tree_size"], res["reached_goal"]) == \
+        (ref["iters"], ref["start_tree_size"], ref["goal_tree_size"], ref["reached_goal"])
+    np.testing.assert_array_equal(res["path"], ref["path"])
+
+
 @pytest.mark.parametrize("name,seed,balance,dd,gran,margin", [
     ("panda", 1, 1, True, 16, 0.0), ("panda", 2, 2, True, 16, 0.0), ("panda", 3, 0, False, 16, 0.0),
     ("fetch", 4, 1, True, 16, 0.0), ("panda", 5, 1, True, 32, 0.01), ("baxter", 6, 1, True, 8, 0.0)])
@@ -127,30 +156,8 @@ def test_single_workgroup_run_matches_restatement(robots, name, seed, balance, d
     ds = S.DeviceScene.from_host(scene)
     st = S.rrtc_settings(num_new_configs=1, max_iters=20000, max_samples=50000, time_limit_ms=0.0,
                          balance=balance, dynamic_domain=int(dd), granularity=gran, collision_margin=margin)
-    res = S.rrtc_solve(rob, ds, start, goals, st)
-
-    def collides(states):
-        hit, clr = o.check_collision(states, margin=st.collision_margin, check_self=True)
-        if np.any(np.abs(clr) < BAND):
-            raise RO.Ambiguous()
-        return bool(hit.any())
-
-    try:
-        ref = RO.rrtc_single_workgroup(start, goals, rob.arrays.joint_lower, rob.arrays.joint_upper, collides,
-                                       granularity=st.granularity, range_=st.range, balance=balance,
-                                       tree_ratio=st.tree_ratio, dynamic_domain=dd, dd_alpha=st.dd_alpha,
-                                       dd_radius=st.dd_radius, dd_min_radius=st.dd_min_radius,
-                                       max_samples=st.max_samples, max_iters=st.max_iters)
-    except RO.Ambiguous:
-        banner(f"single workgroup {name}/{seed}: an edge state is ambiguous, run not compared")
-        return
-    banner(f"single workgroup {name}/{seed} balance {balance} dd {dd} granularity {gran} margin {margin}: device iters {res['iters']} trees "
-           f"{res['start_tree_size']}+{res['goal_tree_size']}, restatement iters {ref['iters']} trees "
-           f"{ref['start_tree_size']}+{ref['goal_tree_size']}")
-    assert res["solved"] and ref["solved"]
-    assert (res["iters"], res["start_tree_size"], res["goal_tree_size"], res["reached_goal"]) == \
-        (ref["iters"], ref["start_tree_size"], ref["goal_tree_size"], ref["reached_goal"])
-    np.testing.assert_array_equal(res["path"], ref["path"])
+    single_workgroup_check(rob, ds, o, start, goals, st,
+                           f"{name}/{seed} balance {balance} dd {dd} granularity {gran} margin {margin}")
 
 
 @pytest.mark.parametrize("balance,dd,gran,margin", [(0, 0, 16, 0.0), (2, 1, 8, 0.01), (1, 1, 32, 0.005)])
@@ -227,6 +234,14 @@ def esdf_problem(robot, seed):
     goals = [g for g in free[1:] if line_blocked(o, start, g, 0.0)][:1]
     assert goals, "no blocked start-goal pair in this world"
     return scene, o, start, np.stack(goals), float(hit.mean())
+
+
+def test_single_workgroup_run_in_an_esdf_world(robots):
+    """The sequential run, row for row, in a nearest-voxel ESDF + box world."""
+    rob = robots["panda"]
+    scene, o, start, goals, _ = esdf_problem(rob, 602)
+    st = S.rrtc_settings(num_new_configs=1, max_iters=20000, max_samples=50000, time_limit_ms=0.0)
+    single_workgroup_check(rob, S.DeviceScene.from_host(scene), o, start, goals, st, "esdf/602")
 
 
 @pytest.mark.parametrize("seed", [600, 601])
