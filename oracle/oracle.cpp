@@ -10,6 +10,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 #ifdef _OPENMP
@@ -199,9 +200,14 @@ inline int robust_floor(float x, float threshold = 1e-4f) {
 // success sets the signed distance (positive inside) and, for trilinear
 // mode, its gradient in world coordinates. `amb` is set when the lookup sits
 // within a hair of a voxel or validity boundary (decision ambiguous).
+// `vamb` (nullable) is set when the looked-up VALUE is discontinuous within
+// that hair: a nearest lookup next to a voxel face whose neighbour holds
+// another value or is skipped, or either mode at the guarded grid boundary.
+// *dalt then receives the largest distance the other side can give (-inf
+// when that side is skipped, +inf when it is not known).
 template <typename T>
 bool esdf_layer(const orc_scene& S, int layer, T x, T y, T z, int sphere_idx, T* dist, V3<T>* grad,
-                bool* amb, T amb_r = 0) {
+                bool* amb, T amb_r = 0, bool* vamb = nullptr, T* dalt = nullptr) {
     if (S.voxel_enable[layer] == 0) return false;
     if (sphere_idx >= 0 && S.sphere_acm && S.sphere_acm[sphere_idx * S.n_voxel_layers + layer] == 0)
         return false;
@@ -243,6 +249,10 @@ bool esdf_layer(const orc_scene& S, int layer, T x, T y, T z, int sphere_idx, T*
             float d1 = 0.0f;
             const bool ok1 = lookup(alt, &d1);
             if ((ok0 && d0 > -amb_r) != (ok1 && d1 > -amb_r)) *amb = true;
+            if (vamb && (ok0 != ok1 || (ok0 && d0 != d1))) {
+                *vamb = true;
+                *dalt = std::max(*dalt, ok1 ? (T)d1 : -std::numeric_limits<T>::infinity());
+            }
         }
         if (!ok0) return false;
         *dist = (T)d0;
@@ -258,8 +268,13 @@ bool esdf_layer(const orc_scene& S, int layer, T x, T y, T z, int sphere_idx, T*
         i0[k] = (int)fl;
         f[k] = u[k] - fl;
         if ((i0[k] == kOff + 1 || i0[k] == kOff || i0[k] == n[k] - kOff - 1 || i0[k] == n[k] - kOff) &&
-            (f[k] < (T)1e-4 || f[k] > (T)(1 - 1e-4)))
+            (f[k] < (T)1e-4 || f[k] > (T)(1 - 1e-4))) {
             *amb = true;   // the layer switches between valid and skipped here
+            if (vamb) {
+                *vamb = true;
+                *dalt = std::numeric_limits<T>::infinity();
+            }
+        }
     }
     for (int k = 0; k < 3; ++k)
         if (i0[k] >= n[k] - kOff || i0[k] <= kOff) return false;
@@ -298,12 +313,26 @@ bool esdf_layer(const orc_scene& S, int layer, T x, T y, T z, int sphere_idx, T*
 
 // scene_collision.cuh:202-266 / 276-331 over every layer. Returns the minimum
 // clearance (-dist - r) and its layer, 1000 when no layer applies.
+// With `calt` (and `vamb`): *calt = the lowest clearance the other side of
+// a value jump can give (1000 when it gives none), for deciding whether the
+// jump can change a minimum or a hinge.
 template <typename T>
-T esdf_clearance(const orc_scene& S, T x, T y, T z, T r, int sphere_idx, V3<T>* grad, bool* amb) {
+T esdf_clearance(const orc_scene& S, T x, T y, T z, T r, int sphere_idx, V3<T>* grad, bool* amb,
+                 bool* vamb = nullptr, T* calt = nullptr) {
     T best = (T)1000;
+    if (calt) *calt = (T)1000;
     for (int layer = 0; layer < S.n_voxel_layers; ++layer) {
         T d; V3<T> g;
-        if (!esdf_layer<T>(S, layer, x, y, z, sphere_idx, &d, grad ? &g : nullptr, amb, r)) continue;
+        bool va = false;
+        T da = -std::numeric_limits<T>::infinity();
+        const bool ok = esdf_layer<T>(S, layer, x, y, z, sphere_idx, &d, grad ? &g : nullptr, amb, r,
+                                      vamb ? &va : nullptr, &da);
+        if (va) {
+            *vamb = true;
+            if (calt) *calt = std::min(*calt, std::isinf(da) && da > 0 ? -std::numeric_limits<T>::infinity()
+                                                                       : (std::isinf(da) ? (T)1000 : -da - r));
+        }
+        if (!ok) continue;
         const T c = -d - r;
         if (c < best) { best = c; if (grad) *grad = {-g.x, -g.y, -g.z}; }
     }
@@ -448,7 +477,7 @@ bool check_one(const orc_robot& R, const orc_scene& S, const float* q, T margin,
 // `grad` is given, also the analytic gradient through the argmin pairs.
 template <typename T>
 T hinge_one(const orc_robot& R, const orc_scene& S, const float* q, const orc_cost_params& p, Work<T>& w,
-            T* grad, bool* kink, T tol) {
+            T* grad, bool* kink, T tol, bool* vamb = nullptr) {
     fk_frames<T>(R, q, w.frames.data(), false);
     sphere_positions<T>(R, w.frames.data(), false, w.P.data());
     const T m = (T)p.world_margin;
@@ -482,8 +511,11 @@ T hinge_one(const orc_robot& R, const orc_scene& S, const float* q, const orc_co
         if (p.use_esdf && S.n_voxel_layers > 0) {
             V3<T> gc{0, 0, 0};
             bool amb = false;
-            const T c = esdf_clearance<T>(S, P.x, P.y, P.z, r, s, grad ? &gc : nullptr, &amb);
+            bool va = false;
+            T ca = (T)1000;
+            const T c = esdf_clearance<T>(S, P.x, P.y, P.z, r, s, grad ? &gc : nullptr, &amb, &va, &ca);
             const T h = m - c;
+            if (vamb && va && (c < m || ca < m)) *vamb = true;   // either side inside the band
             if (h > 0) {
                 world += h;
                 if (grad) { g[s].x -= gc.x; g[s].y -= gc.y; g[s].z -= gc.z; }
@@ -546,14 +578,21 @@ T hinge_one(const orc_robot& R, const orc_scene& S, const float* q, const orc_co
 // Clearance used by scene_min_clearance_runtime (scene_collision.cuh:339-384):
 // ESDF only, radius + margin. include_obb adds the OBB clearance.
 template <typename T>
-T min_clearance_one(const orc_robot& R, const orc_scene& S, const float* q, T margin, bool include_obb, Work<T>& w) {
+T min_clearance_one(const orc_robot& R, const orc_scene& S, const float* q, T margin, bool include_obb, Work<T>& w,
+                    bool* vamb = nullptr, T* lo = nullptr) {
     fk_frames<T>(R, q, w.frames.data(), false);
     sphere_positions<T>(R, w.frames.data(), false, w.P.data());
     T best = (T)1000;
     bool amb = false;
+    // spheres whose lookup sits at a value jump: (own clearance, other side's lowest)
+    std::vector<std::pair<T, T>> jumps;
     for (int s = 0; s < R.n_spheres; ++s) {
         const T r = (T)R.spheres[4 * s + 3] + margin;
-        best = std::min(best, esdf_clearance<T>(S, w.P[s].x, w.P[s].y, w.P[s].z, r, s, nullptr, &amb));
+        bool va = false;
+        T ca = (T)1000;
+        const T c = esdf_clearance<T>(S, w.P[s].x, w.P[s].y, w.P[s].z, r, s, nullptr, &amb, vamb ? &va : nullptr, &ca);
+        if (va) jumps.push_back({c, ca});
+        best = std::min(best, c);
         if (include_obb) {
             for (int o = 0; o < S.n_obbs; ++o) {
                 if (!obb_allowed(S, o, s, S.sphere_obb_acm)) continue;
@@ -561,6 +600,14 @@ T min_clearance_one(const orc_robot& R, const orc_scene& S, const float* q, T ma
             }
         }
     }
+    // the jump matters when that sphere holds the minimum or its other side undercuts it
+    if (lo) *lo = best;
+    if (vamb)
+        for (const auto& j : jumps)
+            if (j.first <= best || j.second < best) {
+                *vamb = true;
+                if (lo) *lo = std::min(*lo, j.second);
+            }
     return best;
 }
 
@@ -817,13 +864,38 @@ void orc_check_collision_f32(const orc_robot* r, const orc_scene* s, const float
 
 void orc_min_clearance_f64(const orc_robot* r, const orc_scene* s, const float* q, int n, float margin,
                            int include_obb, double* out, int nthreads) {
+    orc_min_clearance_amb_f64(r, s, q, n, margin, include_obb, out, nullptr, nullptr, nthreads);
+}
+
+void orc_min_clearance_amb_f64(const orc_robot* r, const orc_scene* s, const float* q, int n, float margin,
+                               int include_obb, double* out, uint8_t* vamb, double* lo, int nthreads) {
     set_threads(nthreads);
 #pragma omp parallel
     {
         Work<double> w(*r);
 #pragma omp for schedule(static)
-        for (int i = 0; i < n; ++i)
-            out[i] = min_clearance_one<double>(*r, *s, q + (size_t)i * r->n_dof, margin, include_obb != 0, w);
+        for (int i = 0; i < n; ++i) {
+            bool va = false;
+            double l = 0.0;
+            out[i] = min_clearance_one<double>(*r, *s, q + (size_t)i * r->n_dof, margin, include_obb != 0, w, &va, &l);
+            if (vamb) vamb[i] = va ? 1 : 0;
+            if (lo) lo[i] = l;
+        }
+    }
+}
+
+void orc_hinge_cost_amb_f64(const orc_robot* r, const orc_scene* s, const float* q, int n, const orc_cost_params* p,
+                            double* cost, uint8_t* vamb, int nthreads) {
+    set_threads(nthreads);
+#pragma omp parallel
+    {
+        Work<double> w(*r);
+#pragma omp for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            bool va = false;
+            cost[i] = hinge_one<double>(*r, *s, q + (size_t)i * r->n_dof, *p, w, nullptr, nullptr, 0.0, &va);
+            vamb[i] = va ? 1 : 0;
+        }
     }
 }
 

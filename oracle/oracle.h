@@ -93,6 +93,15 @@ void orc_check_collision_f32(const orc_robot* r, const orc_scene* s, const float
 /* scene_min_clearance_runtime (ESDF only, include_obb = 0) or OBB+ESDF (include_obb = 1) */
 void orc_min_clearance_f64(const orc_robot* r, const orc_scene* s, const float* q, int n,
                            float margin, int include_obb, double* out, int nthreads);
+/* The same, with vamb[i] = 1 where some ESDF lookup of configuration i sits
+ * within 1e-4 voxel of a face across which the looked-up value jumps (a
+ * nearest lookup whose neighbour voxel holds another value or is skipped,
+ * or the guarded grid boundary) and the jump can move the minimum: an FP32
+ * evaluation may take either side. lo[i] (nullable) = the lowest value the
+ * minimum can take with the other sides (= out[i] when none undercuts it). */
+void orc_min_clearance_amb_f64(const orc_robot* r, const orc_scene* s, const float* q, int n,
+                               float margin, int include_obb, double* out, uint8_t* vamb, double* lo,
+                               int nthreads);
 
 /* ---- per-configuration summed-hinge state cost and its gradient ----
  * cost[i] = weight * (obb hinge + esdf hinge + self min-hinge), chomp.cu:151-188.
@@ -102,6 +111,10 @@ void orc_hinge_cost_f64(const orc_robot* r, const orc_scene* s, const float* q, 
                         const orc_cost_params* p, double* cost, int nthreads);
 void orc_hinge_cost_f32(const orc_robot* r, const orc_scene* s, const float* q, int n,
                         const orc_cost_params* p, float* cost, int nthreads);
+/* FP64 state cost with the value-ambiguity flag of orc_min_clearance_amb_f64
+ * for the ESDF hinge terms (a sphere whose lookup sits at such a face). */
+void orc_hinge_cost_amb_f64(const orc_robot* r, const orc_scene* s, const float* q, int n,
+                            const orc_cost_params* p, double* cost, uint8_t* vamb, int nthreads);
 void orc_hinge_grad_f64(const orc_robot* r, const orc_scene* s, const float* q, int n,
                         const orc_cost_params* p, double* cost, double* grad, int* kink,
                         double tol, int nthreads);

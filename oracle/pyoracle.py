@@ -183,12 +183,30 @@ class Oracle:
                                   _ptr(out, _f64p), self.nthreads)
         return out
 
-    def min_clearance(self, q, margin=0.0, include_obb=False):
+    def min_clearance(self, q, margin=0.0, include_obb=False, with_amb=False, with_low=False):
+        """Per-configuration min clearance; with_amb also returns the value-
+        ambiguity flags (an ESDF lookup at a face where the value jumps and
+        the jump can move the minimum), with_low the lowest minimum the other
+        sides of those jumps can give."""
         q = _c(q, np.float32).reshape(-1, self.robot.n_dof)
         out = np.zeros(q.shape[0], np.float64)
-        lib().orc_min_clearance_f64(C.byref(self.r), C.byref(self.s), _ptr(q, _f32p), q.shape[0],
-                                    C.c_float(margin), int(include_obb), _ptr(out, _f64p), self.nthreads)
-        return out
+        amb = np.zeros(q.shape[0], np.uint8)
+        low = np.zeros(q.shape[0], np.float64)
+        lib().orc_min_clearance_amb_f64(C.byref(self.r), C.byref(self.s), _ptr(q, _f32p), q.shape[0],
+                                        C.c_float(margin), int(include_obb), _ptr(out, _f64p), _ptr(amb, _u8p),
+                                        _ptr(low, _f64p), self.nthreads)
+        if with_low:
+            return out, amb.astype(bool), low
+        return (out, amb.astype(bool)) if with_amb else out
+
+    def hinge_cost_amb(self, q, params: OrcCostParams):
+        """FP64 state cost and its value-ambiguity flags."""
+        q = _c(q, np.float32).reshape(-1, self.robot.n_dof)
+        out = np.zeros(q.shape[0], np.float64)
+        amb = np.zeros(q.shape[0], np.uint8)
+        lib().orc_hinge_cost_amb_f64(C.byref(self.r), C.byref(self.s), _ptr(q, _f32p), q.shape[0], C.byref(params),
+                                     _ptr(out, _f64p), _ptr(amb, _u8p), self.nthreads)
+        return out, amb.astype(bool)
 
     def hinge_cost(self, q, params: OrcCostParams, precision="f64"):
         q = _c(q, np.float32).reshape(-1, self.robot.n_dof)

@@ -118,7 +118,9 @@ class PlanningScene:
         self._robot_id = str(robot_id)
         self._device = device
         self._cuboids: Dict[str, Tuple[np.ndarray, np.ndarray, np.ndarray]] = {}
-        self._lock = threading.RLock()
+        # the reference's name (planning_scene.py:79): its planner base holds it
+        # while a SceneCollisionDataBuilder's device pointers are in use
+        self._voxel_plan_lock = threading.RLock()
         self._dirty = True
         self._tensors = None
         self._n_obbs = 0
@@ -143,7 +145,7 @@ class PlanningScene:
 
     @property
     def lock(self):
-        return self._lock
+        return self._voxel_plan_lock
 
     @property
     def current_obstacles(self) -> List[Dict[str, Any]]:
@@ -156,25 +158,25 @@ class PlanningScene:
         nrm = np.linalg.norm(q)
         if nrm > 0:
             q = q / nrm
-        with self._lock:
+        with self._voxel_plan_lock:
             self._cuboids[name] = (np.asarray(dims, np.float64).reshape(3),
                                    np.asarray(position, np.float64).reshape(3), q)
             self._dirty = True
 
     def remove_cuboid(self, name: str) -> bool:
-        with self._lock:
+        with self._voxel_plan_lock:
             existed = self._cuboids.pop(name, None) is not None
             self._dirty |= existed
             return existed
 
     def clear(self) -> None:
-        with self._lock:
+        with self._voxel_plan_lock:
             if self._cuboids:
                 self._cuboids.clear()
                 self._dirty = True
 
     def update_world(self, obstacles: List[Dict[str, Any]]) -> None:
-        with self._lock:
+        with self._voxel_plan_lock:
             self._cuboids.clear()
             for i, obs in enumerate(obstacles):
                 if "pose" in obs:
@@ -185,7 +187,7 @@ class PlanningScene:
             self._dirty = True
 
     def host_scene(self) -> HostScene:
-        with self._lock:
+        with self._voxel_plan_lock:
             if not self._cuboids:
                 return HostScene()
             d = np.stack([v[0] for v in self._cuboids.values()])
@@ -196,7 +198,7 @@ class PlanningScene:
     def device_scene(self):
         """DeviceScene (torch tensors on the scene device), rebuilt when dirty."""
         from .substrate import DeviceScene
-        with self._lock:
+        with self._voxel_plan_lock:
             if self._dirty or self._tensors is None:
                 hs = self.host_scene()
                 self._tensors = DeviceScene.from_host(hs, device=self._device)

@@ -45,6 +45,10 @@ PlannerResult = _m.PlannerResult
 SolverBuffers = _m.SolverBuffers
 solve_scene = _m.solve_scene
 
+__all__ = ["RobotModelBuilder", "SceneCollisionDataBuilder", "check_collision_scene", "CHOMPSettings", "CHOMPResult",
+           "CHOMPBuffers", "chomp_solve_scene", "STOMPSettings", "STOMPResult", "STOMPBuffers", "stomp_solve_scene",
+           "Settings", "PlannerResult", "SolverBuffers", "solve_scene", "scene_builder"]
+
 
 def scene_builder(ds) -> SceneCollisionDataBuilder:
     """SceneCollisionDataBuilder over a substrate DeviceScene's tensors; the

@@ -193,3 +193,23 @@ class RobotArrays:
 
 def robot_arrays(name_or_path=None) -> RobotArrays:
     return RobotArrays(load_robot_asset(name_or_path))
+
+
+def build_robot_model(path=None):
+    """The asset as a built ``prrtc.RobotModelBuilder``, the object the solve
+    entry points take (robot_model_loader.py:193-243): every builder field is
+    filled from load_robot_asset, the device tables are created at once (so
+    an allocation or table error surfaces here, not at the first solve), and
+    the builder is returned. `path` is a robot name or YAML path (default:
+    the bundled Panda)."""
+    from . import prrtc
+
+    asset = load_robot_asset(path)
+    b = prrtc.RobotModelBuilder()
+    for name in ("n_dof", "n_joints", "fixed_transforms", "joint_types", "spheres_flat", "sphere_to_joint",
+                 "self_cc_ranges_flat", "approx_spheres_flat", "approx_sphere_to_joint",
+                 "approx_self_cc_ranges_flat", "joint_lower", "joint_upper", "joint_parents", "joint_id_to_dof",
+                 "t_memory_idx", "dfs_order"):
+        setattr(b, name, asset[name])
+    b.build()
+    return b

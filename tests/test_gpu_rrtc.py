@@ -131,8 +131,10 @@ def single_workgroup_check(rob, ds, o, start, goals, st, tag):
                                        dd_alpha=st.dd_alpha, dd_radius=st.dd_radius, dd_min_radius=st.dd_min_radius,
                                        max_samples=st.max_samples, max_iters=st.max_iters)
     except RO.Ambiguous:
-        banner(f"single workgroup {tag}: an edge state is ambiguous, run not compared")
-        return
+        # the restatement runs on the CPU and is deterministic: the fixed seeds were chosen so that none of
+        # its edge checks lands in the band, and a change that makes one do so must pick another seed
+        pytest.fail(f"single workgroup {tag}: an edge state of the restatement is within {BAND} m of contact; "
+                    f"this case's seed no longer gives a comparable run")
     banner(f"single workgroup {tag}: device iters {res['iters']} trees "
            f"{res['start_tree_size']}+{res['goal_tree_size']}, restatement iters {ref['iters']} trees "
            f"{ref['start_tree_size']}+{ref['goal_tree_size']}")
@@ -149,8 +151,9 @@ def test_single_workgroup_run_matches_restatement(robots, name, seed, balance, d
     """With one workgroup the planner has no concurrency: the device run must
     reproduce the sequential restatement (oracle/rrtc_oracle.py) row for row:
     the path, the iteration that connects, both tree sizes and the goal
-    reached. A run whose edge checks touch the ambiguous clearance band is
-    reported and not compared."""
+    reached. The seeds are fixed so that no edge check of the run lies in the
+    ambiguous clearance band; the test fails rather than skip the comparison
+    if one does."""
     rob = robots[name]
     scene, o, start, goals = problem(rob, seed, margin=margin, n_goals=2)
     ds = S.DeviceScene.from_host(scene)

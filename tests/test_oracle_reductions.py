@@ -1,0 +1,488 @@
+"""Independent pins of the oracle's robot-level reductions (CPU).
+
+Each reduction of oracle/oracle.cpp is rebuilt here in numpy from primitives
+that other tests already pin (FK by closed forms and a matrix-chain
+restatement, the single sphere-OBB clearance by closed forms and scipy, the
+single-sphere ESDF lookup by the plane-field and truncation tests, the
+smoothness metric by its closed form), and compared with the oracle's own
+composition:
+
+  * summed OBB hinge   sum_s relu(m - min_o clr(s, o)), raw radius
+                       (scene_collision.cuh:449-495);
+  * summed ESDF hinge  sum_s relu(m - esdf_clr(s)), raw radius (:394-438);
+  * self min-hinge     relu(m_self - min_pairs(|pi - pj| - ri - rj))
+                       (runtime_kinematics.cuh:294-329);
+  * state cost         w * (OBB + ESDF + self) (chomp.cu:151-188);
+  * min clearance      min_s esdf_clr(s, r + margin) (scene_collision.cuh:339-384);
+  * STOMP totals       sum_t state + w_u sum_d theta_d^T R theta_d
+                       (stomp.cu:206-308);
+  * exhaustive any-collision clearance and the FD gradient built on the cost.
+
+Every comparison also asserts that the inputs discriminate: the plausible
+misreadings (a sum over OBBs instead of the minimum, the hinge margin or the
+collision margin added to the radius, the weight applied twice, a hinge per
+self pair instead of one of the minimum, ...) give values that differ from the
+oracle's by far more than the comparison tolerance, so an oracle carrying any
+of them would fail here. Closed-form cases on a one-sphere robot and a
+hand-counted gated-flop case finish the set.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as PO
+from prismplan_amd import workloads as W
+from prismplan_amd.planning_scene import HostScene
+from prismplan_amd.robot_model_loader import RobotArrays, robot_arrays
+
+REL = 1e-12
+
+
+# ---------------------------------------------------------------------------
+# numpy primitives
+# ---------------------------------------------------------------------------
+def np_box_clearance(scene, P, r):
+    """Box signed distance minus radius of points P [..., 3] against every OBB
+    of `scene`: [..., O]. Inverse pose (t, q) maps p to q p q* + t; the box
+    SDF is |max(d, 0)| + min(max_k d_k, 0) with d = |l| - h."""
+    dims = scene.obb_dims[:, :3].astype(np.float64)
+    pose = scene.obb_pose.astype(np.float64)
+    t, w, u = pose[:, :3], pose[:, 3], pose[:, 4:7]
+    p = np.asarray(P, np.float64)[..., None, :]                    # [..., 1, 3]
+    tt = 2.0 * np.cross(u, p)                                      # [..., O, 3]
+    loc = p + w[:, None] * tt + np.cross(u, tt) + t
+    d = np.abs(loc) - 0.5 * dims
+    out = np.linalg.norm(np.maximum(d, 0.0), axis=-1) + np.minimum(d.max(axis=-1), 0.0)
+    return out - np.asarray(r, np.float64)[..., None]
+
+
+def self_pairs(r: RobotArrays):
+    pr = [(i, j) for i, a, b in r.self_ranges for j in range(a, b + 1)]
+    return np.array(pr, np.int64).reshape(-1, 2)
+
+
+def np_self_min(r: RobotArrays, P):
+    """Minimum self-pair clearance per configuration, P [n, S, 3]; 1000 with no pairs."""
+    pr = self_pairs(r)
+    if len(pr) == 0:
+        return np.full(P.shape[0], 1000.0)
+    rad = r.spheres[:, 3].astype(np.float64)
+    d = np.linalg.norm(P[:, pr[:, 0]] - P[:, pr[:, 1]], axis=-1) - rad[pr[:, 0]] - rad[pr[:, 1]]
+    return d.min(axis=1)
+
+
+def esdf_clr(o, P, r):
+    """Per-sphere ESDF clearance through the single-sphere primitive, [n, S]."""
+    n, S = P.shape[:2]
+    out = np.empty((n, S))
+    for i in range(n):
+        for s in range(S):
+            out[i, s] = PO.sphere_esdf_clearance(o, P[i, s], float(r[s]), s)
+    return out
+
+
+def relu(x):
+    return np.maximum(x, 0.0)
+
+
+def f32(x):
+    """The FP64 value of the FP32 parameter the oracle receives."""
+    return float(np.float32(x))
+
+
+def assert_pinned(got, want, mutants, tol=REL, gap=1e3):
+    """got == want to `tol` (relative, floor 1e-9), and every mutant differs
+    from `got` somewhere by more than gap * tol."""
+    den = np.maximum(np.abs(want), 1e-9)
+    err = np.abs(got - want) / den
+    assert err.max() < tol, f"max rel err {err.max():.3e}"
+    for name, m in mutants.items():
+        sep = (np.abs(np.asarray(m) - got) / den).max()
+        assert sep > gap * tol, f"inputs do not tell the oracle from the misreading '{name}' ({sep:.2e})"
+
+
+def world(seed, n_obbs, lo, hi, half_hi=0.3):
+    return W.random_obbs(np.random.default_rng(seed), n_obbs, lo=lo, hi=hi, half_hi=half_hi)
+
+
+ROBOT_WORLDS = {
+    "panda": ((-0.2, -0.9, 0.0), (1.0, 0.9, 1.4)),
+    "fetch": ((0.1, -0.8, 0.0), (1.1, 0.8, 1.6)),
+    "baxter": ((0.25, -0.55, -0.25), (1.25, 0.55, 0.75)),
+}
+
+
+# ---------------------------------------------------------------------------
+# the numpy box clearance is the pinned primitive
+# ---------------------------------------------------------------------------
+def test_numpy_box_clearance_equals_the_pinned_primitive():
+    rng = np.random.default_rng(7)
+    scene = world(7, 12, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+    P = rng.uniform(-0.9, 0.9, (200, 3))
+    r = rng.uniform(0.0, 0.1, 200)
+    got = np_box_clearance(scene, P, r)
+    for i in range(0, 200, 7):
+        for o in range(12):
+            ref = PO.sphere_obb_clearance(scene.obb_dims[o], scene.obb_pose[o], P[i], r[i])
+            assert got[i, o] == pytest.approx(ref, rel=1e-12, abs=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# summed hinge terms
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["panda", "fetch", "baxter"])
+def test_obb_hinge_is_sum_over_spheres_of_hinge_of_min_over_boxes(name):
+    r = robot_arrays(name)
+    scene = world(100, 14, *ROBOT_WORLDS[name], half_hi=0.15)
+    scene.obb_enable[::7] = 0                      # disabled boxes take no part
+    o = PO.Oracle(r, scene, nthreads=4)
+    q = W.uniform_configs(np.random.default_rng(101), r, 150)
+    m, w = f32(0.05), 2.5
+    got = o.hinge_cost(q, PO.cost_params(world_margin=m, weight=w, use_obb=True, use_esdf=False, use_self=False))
+    P = o.fk_spheres(q)
+    rad = r.spheres[:, 3].astype(np.float64)
+    clr = np_box_clearance(scene, P, np.broadcast_to(rad, P.shape[:2]))      # [n, S, O]
+    en = scene.obb_enable.astype(bool)
+    cmin = clr[..., en].min(axis=-1)
+    want = w * relu(m - cmin).sum(axis=1)
+    assert (want > 0).mean() > 0.5 and (cmin > m).mean() > 0.5   # active and inactive spheres
+    assert_pinned(got, want, {
+        "sum over boxes": w * relu(m - clr[..., en]).sum(axis=(1, 2)),
+        "disabled boxes included": w * relu(m - clr.min(axis=-1)).sum(axis=1),
+        "margin added to the radius": w * relu(m - (cmin - m)).sum(axis=1),
+        "min over spheres": w * relu(m - cmin).max(axis=1),
+        "weight applied twice": w * want,
+        "no hinge (signed)": w * (m - cmin).sum(axis=1),
+    })
+
+
+def test_acm_masked_pairs_leave_the_box_minimum():
+    r = robot_arrays("panda")
+    scene = world(102, 14, *ROBOT_WORLDS["panda"], half_hi=0.15)
+    rng = np.random.default_rng(103)
+    acm = (rng.random((r.n_spheres, scene.n_obbs)) > 0.3).astype(np.uint8)
+    s2 = HostScene(obb_dims=scene.obb_dims, obb_pose=scene.obb_pose, obb_enable=scene.obb_enable,
+                   sphere_obb_acm=acm)
+    o = PO.Oracle(r, s2, nthreads=4)
+    q = W.uniform_configs(rng, r, 120)
+    m = f32(0.05)
+    got = o.hinge_cost(q, PO.cost_params(world_margin=m, weight=1.0, use_esdf=False, use_self=False))
+    P = o.fk_spheres(q)
+    clr = np_box_clearance(scene, P, np.broadcast_to(r.spheres[:, 3].astype(np.float64), P.shape[:2]))
+    masked = np.where(acm[None].astype(bool), clr, 1000.0)
+    want = relu(m - masked.min(axis=-1)).sum(axis=1)
+    assert_pinned(got, want, {"mask ignored": relu(m - clr.min(axis=-1)).sum(axis=1),
+                              "mask transposed rule (masked = allowed)":
+                                  relu(m - np.where(acm[None].astype(bool), 1000.0, clr).min(axis=-1)).sum(axis=1)})
+
+
+def esdf_world(interp, seed=104):
+    rng = np.random.default_rng(seed)
+    obbs = W.random_obbs(rng, 10, lo=(0.2, -0.4, 0.1), hi=(0.7, 0.4, 0.8), half_hi=0.12)
+    params, pose = W.esdf_grid(center=(0.45, 0.0, 0.45), n=40, voxel=0.025)
+    vals = PO.esdf_from_obbs(obbs.obb_dims, obbs.obb_pose, params, pose).astype(np.float32)
+    return HostScene(voxel_params=params, voxel_pose=pose, voxel_enable=np.ones(1, np.uint8),
+                     voxel_features=vals.reshape(1, -1), esdf_interp=interp)
+
+
+@pytest.mark.parametrize("interp", [0, 1])
+def test_esdf_hinge_and_min_clearance(interp):
+    r = robot_arrays("panda")
+    s = esdf_world(interp)
+    o = PO.Oracle(r, s, nthreads=4)
+    q = W.uniform_configs(np.random.default_rng(105), r, 40)
+    P = o.fk_spheres(q)
+    rad = r.spheres[:, 3].astype(np.float64)
+    m, w = f32(0.05), 3.0
+    c_raw = esdf_clr(o, P, rad)
+    got = o.hinge_cost(q, PO.cost_params(world_margin=m, weight=w, use_obb=False, use_esdf=True, use_self=False))
+    want = w * relu(m - c_raw).sum(axis=1)
+    assert (want > 0).mean() > 0.3
+    c_marg = esdf_clr(o, P, rad + m)
+    assert_pinned(got, want, {
+        "margin added to the radius": w * relu(m - c_marg).sum(axis=1),
+        "hinge of the minimum only": w * relu(m - c_raw.min(axis=1)),
+        "weight applied twice": w * want,
+    })
+    # scene_min_clearance_runtime: min over spheres with radius + margin
+    cm = f32(0.01)
+    got_mc = o.min_clearance(q, margin=cm)
+    c_cm = esdf_clr(o, P, rad + cm)
+    want_mc = c_cm.min(axis=1)
+    assert_pinned(got_mc, want_mc, {"margin left out": c_raw.min(axis=1),
+                                    "mean over spheres": c_cm.mean(axis=1)})
+
+
+@pytest.mark.parametrize("name", ["panda", "fetch", "baxter"])
+def test_self_min_hinge_is_a_hinge_of_the_minimum(name):
+    r = robot_arrays(name)
+    o = PO.Oracle(r, HostScene(), nthreads=4)
+    q = W.uniform_configs(np.random.default_rng(106), r, 300)
+    ms, w = f32(0.03), 2.0
+    got = o.hinge_cost(q, PO.cost_params(self_margin=ms, weight=w, use_obb=False, use_esdf=False, use_self=True))
+    P = o.fk_spheres(q)
+    smin = np_self_min(r, P)
+    want = w * relu(ms - smin)
+    assert (want > 0).mean() > 0.2
+    pr = self_pairs(r)
+    rad = r.spheres[:, 3].astype(np.float64)
+    dpair = np.linalg.norm(P[:, pr[:, 0]] - P[:, pr[:, 1]], axis=-1) - rad[pr[:, 0]] - rad[pr[:, 1]]
+    assert_pinned(got, want, {
+        "hinge per pair, summed": w * relu(ms - dpair).sum(axis=1),
+        "squared distances": w * relu(ms - (np.linalg.norm(P[:, pr[:, 0]] - P[:, pr[:, 1]], axis=-1) ** 2
+                                            - (rad[pr[:, 0]] + rad[pr[:, 1]]) ** 2).min(axis=1)),
+        "weight applied twice": w * want,
+    })
+
+
+@pytest.mark.parametrize("name", ["panda", "fetch"])
+def test_state_cost_is_weight_times_world_plus_self(name):
+    r = robot_arrays(name)
+    scene = world(107, 14, *ROBOT_WORLDS[name], half_hi=0.15)
+    o = PO.Oracle(r, scene, nthreads=4)
+    q = W.uniform_configs(np.random.default_rng(108), r, 150)
+    m, ms, w = f32(0.04), f32(0.02), 20.0
+    got = o.hinge_cost(q, PO.cost_params(world_margin=m, self_margin=ms, weight=w))
+    P = o.fk_spheres(q)
+    rad = r.spheres[:, 3].astype(np.float64)
+    obb = relu(m - np_box_clearance(scene, P, np.broadcast_to(rad, P.shape[:2])).min(axis=-1)).sum(axis=1)
+    slf = relu(ms - np_self_min(r, P))
+    want = w * (obb + slf)
+    assert (slf > 0).any() and (obb > 0).any()
+    assert_pinned(got, want, {"self unweighted": w * obb + slf,
+                              "self margin = world margin": w * (obb + relu(m - np_self_min(r, P)))})
+
+
+def test_exhaustive_check_clearance_and_verdict():
+    """check_collision mode 1 (the exhaustive exact check) returns, per
+    configuration, the minimum over spheres and enabled boxes of the box
+    clearance with radius + collision margin, and collides iff some pair has
+    squared distance below (r + margin)^2 (strict)."""
+    r = robot_arrays("panda")
+    scene = world(109, 20, (0.2, -0.6, 0.0), (0.9, 0.6, 1.0))
+    o = PO.Oracle(r, scene, nthreads=4)
+    q = W.uniform_configs(np.random.default_rng(110), r, 400)
+    for margin in (0.0, 0.02):
+        hit, clr = o.check_collision(q, margin=margin, mode=1)
+        P = o.fk_spheres(q)
+        rm = r.spheres[:, 3].astype(np.float64) + np.float32(margin)
+        c = np_box_clearance(scene, P, np.broadcast_to(rm, P.shape[:2]))[..., scene.obb_enable.astype(bool)]
+        want = c.min(axis=(1, 2))
+        np.testing.assert_allclose(clr, want, rtol=1e-12, atol=1e-12)
+        ok = np.abs(want) >= 1e-9
+        np.testing.assert_array_equal(hit[ok], want[ok] < 0)
+        assert 0.1 < hit.mean() < 0.95
+
+
+# ---------------------------------------------------------------------------
+# trajectories: totals, FD gradient, STOMP rollouts
+# ---------------------------------------------------------------------------
+def test_traj_totals_and_fd_gradient_built_on_the_state_cost():
+    c = W.chomp(n_traj=3, n_wp=12, seed=111)
+    o = PO.Oracle(c.robot, c.scene, nthreads=4)
+    D = c.robot.n_dof
+    w = 4.0
+    prm = PO.cost_params(world_margin=0.05, weight=w, use_self=False)
+    prm1 = PO.cost_params(world_margin=0.05, weight=1.0, use_self=False)
+    total, _, _ = o.traj_cost_grad(c.theta, c.offsets, prm)
+    eps = np.float32(1e-3)
+    fd = o.chomp_fd_grad(c.theta, c.offsets, prm, eps=float(eps))
+    for b in range(c.n_traj):
+        t0, t1 = c.offsets[b], c.offsets[b + 1]
+        T = t1 - t0
+        th = c.theta[t0 * D:t1 * D].reshape(D, T)
+        per = o.hinge_cost(th.T.copy(), prm1)
+        assert total[b] == pytest.approx(w * per.sum(), rel=1e-12)
+        g = fd[t0 * D:t1 * D].reshape(D, T)
+        assert np.all(g[:, 0] == 0) and np.all(g[:, -1] == 0)
+        for t in range(1, T - 1):
+            qp = np.repeat(th[:, t][None], D, axis=0)
+            qm = qp.copy()
+            for d in range(D):
+                qp[d, d] = np.float32(qp[d, d] + eps)                     # q[d] += eps
+                qm[d, d] = np.float32(np.float32(qm[d, d] + eps) - np.float32(2 * eps))   # then -= 2 eps
+            cp, cm = o.hinge_cost(qp, prm1), o.hinge_cost(qm, prm1)
+            want = w * (cp - cm) / (2.0 * float(eps))
+            np.testing.assert_allclose(g[:, t], want, rtol=1e-12, atol=1e-12)
+
+
+def test_stomp_totals_are_state_sums_plus_control():
+    wk = W.stomp(n_problems=3, n_rollouts=4, t_lo=10, t_hi=16, o_lo=8, o_hi=14, seed=112)
+    R = wk.n_rollouts
+    D = wk.robot.n_dof
+    m, ms, wc, wu, dt = f32(0.02), f32(0.01), 20.0, 0.1, 0.1
+    prm = PO.cost_params(world_margin=m, self_margin=ms, weight=wc)
+    o = PO.Oracle(wk.robot)
+    state, total = o.stomp_costs(wk.scene, wk.obb_offsets, wk.rollouts, wk.lengths, wk.roll_offsets, R, prm, wu, dt)
+    want_state, want_total, ctrl = [], [], []
+    for p in range(wk.n_problems):
+        Tn = int(wk.lengths[p])
+        a, b = wk.obb_offsets[p], wk.obb_offsets[p + 1]
+        sc = HostScene(obb_dims=wk.scene.obb_dims[a:b], obb_pose=wk.scene.obb_pose[a:b],
+                       obb_enable=wk.scene.obb_enable[a:b])
+        op = PO.Oracle(wk.robot, sc)
+        Rm = W.control_R_closed_form(Tn, dt)
+        for k in range(R):
+            base = int(wk.roll_offsets[p]) + k * Tn
+            th = wk.rollouts[base * D:(base + Tn) * D].reshape(D, Tn).astype(np.float64)
+            st = op.hinge_cost(th.T.astype(np.float32).copy(), prm)
+            cc = wu * sum(float(th[d] @ Rm @ th[d]) for d in range(D))
+            want_state.append(st)
+            want_total.append(st.sum() + cc)
+            ctrl.append(cc)
+    want_state = np.concatenate(want_state)
+    want_total = np.array(want_total)
+    ctrl = np.array(ctrl)
+    np.testing.assert_allclose(state, want_state, rtol=1e-12, atol=1e-12)
+    assert_pinned(total, want_total, {"control weight left out": want_total - ctrl + ctrl / wu,
+                                      "control cost per waypoint": want_total + ctrl * (wk.lengths[0] - 1),
+                                      "states unweighted": (want_total - ctrl) / wc + ctrl})
+
+
+# ---------------------------------------------------------------------------
+# closed forms on a one-joint robot
+# ---------------------------------------------------------------------------
+def tiny_robot(spheres, joints_of, n_dof=1, second_joint_offset=None, self_ranges=()):
+    """Root (fixed) + revolute z joint(s) with identity fixed transforms; the
+    optional second revolute joint sits at x = second_joint_offset."""
+    eye = np.eye(4).reshape(-1).tolist()
+    n_joints = 2 if second_joint_offset is None else 3
+    ft = eye + eye
+    types = [-1, 5]
+    parents, dofs, tmem, dfs = [0, 0], [-1, 0], [0, 1], [0, 1]
+    if second_joint_offset is not None:
+        t = np.eye(4)
+        t[0, 3] = second_joint_offset
+        ft = ft + t.reshape(-1).tolist()
+        types.append(5)
+        parents.append(1)
+        dofs.append(1)
+        tmem.append(1)
+        dfs.append(2)
+    sp = np.asarray(spheres, np.float64).reshape(-1, 4)
+    apx, apx_j = [], []
+    for j in range(n_joints):
+        sel = sp[np.asarray(joints_of) == j]
+        if len(sel) == 0:
+            continue
+        c = sel[:, :3].mean(axis=0)
+        rad = max(np.linalg.norm(s[:3] - c) + s[3] for s in sel)
+        apx += [*c, rad]
+        apx_j.append(j)
+    return RobotArrays({
+        "name": "tiny", "n_dof": n_dof, "n_joints": n_joints, "fixed_transforms": ft, "joint_types": types,
+        "spheres_flat": sp.reshape(-1).tolist(), "sphere_to_joint": list(joints_of),
+        "self_cc_ranges_flat": [v for rg in self_ranges for v in rg],
+        "approx_spheres_flat": apx, "approx_sphere_to_joint": apx_j, "approx_self_cc_ranges_flat": [],
+        "joint_lower": [-3.0] * n_dof, "joint_upper": [3.0] * n_dof,
+        "joint_parents": parents, "joint_id_to_dof": dofs, "t_memory_idx": tmem, "dfs_order": dfs})
+
+
+def slab(x_face, h=0.2):
+    """Axis-aligned box occupying x in [x_face, x_face + 2h]."""
+    return (x_face + h, 0.0, 0.0), (2 * h, 2 * h, 2 * h)
+
+
+def test_one_sphere_face_distance_closed_form():
+    a, rad = 0.3, 0.05
+    r = tiny_robot([[a, 0.0, 0.0, rad]], [1])
+    m, w = 0.08, 1.7
+    prm = PO.cost_params(world_margin=m, weight=w, use_esdf=False, use_self=False)
+    a32, rad32, m32, w32 = f32(a), f32(rad), f32(m), f32(w)       # the FP32 values the oracle receives
+    for d in (0.34, 0.38, 0.42, 0.5):
+        c, dims = slab(d)
+        o = PO.Oracle(r, HostScene.from_boxes([dims], [c], [[1, 0, 0, 0]]))
+        face = f32(c[0]) - 0.5 * f32(dims[0])
+        for qz in (0.0, 0.2, -0.35):
+            q32 = f32(qz)
+            clr = face - a32 * np.cos(q32) - rad32           # sphere centre at (a cos q, a sin q, 0)
+            want = w32 * max(0.0, m32 - clr)
+            got, g, kink = o.hinge_grad(np.array([[qz]], np.float32), prm)
+            assert got[0] == pytest.approx(want, rel=1e-12, abs=1e-15)
+            # d/dq of w (m - clr) = -w a sin q (clr grows as the sphere turns away)
+            want_g = -w32 * a32 * np.sin(q32) if want > 0 else 0.0
+            assert g[0, 0] == pytest.approx(want_g, rel=1e-12, abs=1e-15)
+
+
+def test_two_boxes_give_the_minimum_and_margins_stay_out_of_the_hinge():
+    # dyadic sizes: every input and result is exact in FP32 and FP64
+    rad = 0.0625
+    r = tiny_robot([[0.0, 0.0, 0.0, rad]], [1])
+    (c1, d1), (c2, d2) = slab(0.09375, h=0.25), slab(-0.15625 - 0.5, h=0.25)   # faces at x = 3/32 and x = -5/32
+    sc = HostScene.from_boxes([d1, d2], [c1, c2], [[1, 0, 0, 0]] * 2)
+    o = PO.Oracle(r, sc)
+    m, w = 0.125, 1.0
+    q0 = np.zeros((1, 1), np.float32)
+    got = o.hinge_cost(q0, PO.cost_params(world_margin=m, weight=w, use_esdf=False, use_self=False))[0]
+    # nearest box: clearance 3/32 - 1/16 = 1/32 -> hinge 3/32 (a sum over boxes would add 1/32 more)
+    assert got == 0.09375
+    # the collision check's margin inflates the radius; the hinge has no such margin
+    hit0, clr0 = o.check_collision(q0, margin=0.0, mode=1)
+    hit1, clr1 = o.check_collision(q0, margin=0.0625, mode=1)
+    assert not hit0[0] and hit1[0]
+    assert clr0[0] == 0.03125 and clr1[0] == -0.03125
+
+
+def test_two_sphere_self_hinge_closed_form():
+    L, r1, r2 = 0.25, 0.0625, 0.078125
+    r = tiny_robot([[0.0, 0.0, 0.0, r1], [0.0, 0.0, 0.0, r2]], [1, 2], n_dof=2, second_joint_offset=L,
+                   self_ranges=[(0, 1, 1)])
+    o = PO.Oracle(r, HostScene())
+    for ms in (0.0625, 0.125, 0.25):
+        got = o.hinge_cost(np.array([[0.4, -1.1]], np.float32),
+                           PO.cost_params(self_margin=ms, weight=2.0, use_obb=False, use_esdf=False))[0]
+        # |p1 - p2| = L for any q: the second joint sits at x = L on the first, its sphere on its axis
+        assert got == pytest.approx(2.0 * max(0.0, ms - (L - r1 - r2)), rel=1e-12, abs=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# gated flops: hand counts
+# ---------------------------------------------------------------------------
+def test_gated_flops_hand_counts():
+    """Canonical traversal: 83 per revolute joint past the root, then per
+    joint with spheres 18 (approximate sphere) + 30 per enabled box; a hit
+    approximate sphere adds 18 + 30 per exact sphere of the joint, and the
+    configuration stops after the box that holds a hit."""
+    r = robot_arrays("panda")
+    n_apx_joints = len(set(r.approx_sphere_to_joint.tolist()))
+    assert r.n_joints == 8 and n_apx_joints == 8
+    q = np.zeros((1, 7), np.float32)
+    far = HostScene.from_boxes([[0.1, 0.1, 0.1]] * 3, [[5, 5, 5], [6, 5, 5], [7, 5, 5]], [[1, 0, 0, 0]] * 3)
+    far.obb_enable[1] = 0
+    f = PO.Oracle(r, far).gated_flops(q)[0]
+    assert f == 7 * 83 + 8 * (18 + 2 * 30)                    # 1,205: two enabled boxes, no hit
+    # a box around the base: the root's approximate sphere hits the first box,
+    # its exact spheres are placed and tested, one hits, the walk stops there
+    base = HostScene.from_boxes([[0.4, 0.4, 0.4], [0.1, 0.1, 0.1]], [[0, 0, 0.1], [5, 5, 5]], [[1, 0, 0, 0]] * 2)
+    n0 = int(np.sum(r.sphere_to_joint == r.dfs_order[0]))
+    f = PO.Oracle(r, base).gated_flops(q)[0]
+    assert f == 18 + 30 + n0 * (18 + 30)
+
+
+# ---------------------------------------------------------------------------
+# value-ambiguity flags of ESDF lookups
+# ---------------------------------------------------------------------------
+def test_value_ambiguity_flags_only_jumps_that_matter():
+    """A one-sphere robot over a nearest-voxel grid whose values grow along x:
+    the sphere centre a hair from a voxel face (in grid-local x) is flagged,
+    because the FP32 lookup may take the neighbour's value; mid-voxel it is
+    not; and for the hinge only a jump with a side inside the band counts."""
+    from prismplan_amd.planning_scene import grid_dims
+    vs, n = 0.0625, 24
+    params, pose = W.esdf_grid(center=(0.0, 0.0, 0.0), n=n, voxel=vs)
+    nx, ny, nz = grid_dims(params[0])
+    node_x = np.arange(nx) * vs - params[0, 0] / 2
+    vals = np.broadcast_to((-1.0 + 0.1 * node_x)[:, None, None], (nx, ny, nz)).astype(np.float32)   # free, far
+    s = HostScene(voxel_params=params, voxel_pose=pose, voxel_enable=np.ones(1, np.uint8),
+                  voxel_features=vals.reshape(1, -1).copy(), esdf_interp=0)
+    rad = 0.03125
+    for x, flagged in ((node_x[10] + 0.5 * vs, False), (node_x[10] + 1e-7, True), (node_x[11] - 1e-7, True)):
+        r = tiny_robot([[float(x), 0.5 * vs, 0.5 * vs, rad]], [1])
+        o = PO.Oracle(r, s)
+        q = np.zeros((1, 1), np.float32)
+        _, amb = o.min_clearance(q, with_amb=True)
+        assert bool(amb[0]) is flagged, x
+        # clearances near 1 m: far outside any hinge band, so the jump cannot change the hinge
+        _, hamb = o.hinge_cost_amb(q, PO.cost_params(world_margin=0.05, use_obb=False, use_self=False))
+        assert not hamb[0]
+        _, hamb = o.hinge_cost_amb(q, PO.cost_params(world_margin=2.0, use_obb=False, use_self=False))
+        assert bool(hamb[0]) is flagged
