@@ -477,7 +477,8 @@ bool check_one(const orc_robot& R, const orc_scene& S, const float* q, T margin,
 // `grad` is given, also the analytic gradient through the argmin pairs.
 template <typename T>
 T hinge_one(const orc_robot& R, const orc_scene& S, const float* q, const orc_cost_params& p, Work<T>& w,
-            T* grad, bool* kink, T tol, bool* vamb = nullptr) {
+            T* grad, bool* kink, T tol, bool* vamb = nullptr, int* nterm = nullptr) {
+    int terms = 0;
     fk_frames<T>(R, q, w.frames.data(), false);
     sphere_positions<T>(R, w.frames.data(), false, w.P.data());
     const T m = (T)p.world_margin;
@@ -499,6 +500,7 @@ T hinge_one(const orc_robot& R, const orc_scene& S, const float* q, const orc_co
             const T h = m - best;
             if (h > 0) {
                 world += h;
+                ++terms;
                 if (grad && arg >= 0) {
                     bool kk = false;
                     V3<T> gc = obb_clearance_grad<T>(S.obb_dims + 4 * arg, S.obb_pose + 8 * arg, P.x, P.y, P.z, tol, &kk);
@@ -518,6 +520,7 @@ T hinge_one(const orc_robot& R, const orc_scene& S, const float* q, const orc_co
             if (vamb && va && (c < m || ca < m)) *vamb = true;   // either side inside the band
             if (h > 0) {
                 world += h;
+                ++terms;
                 if (grad) { g[s].x -= gc.x; g[s].y -= gc.y; g[s].z -= gc.z; }
             }
             if (kink && (amb || std::fabs(h) < tol)) *kink = true;
@@ -531,6 +534,7 @@ T hinge_one(const orc_robot& R, const orc_scene& S, const float* q, const orc_co
         const T h = (T)p.self_margin - c;
         if (h > 0) {
             self = h;
+            ++terms;
             if (grad) {
                 V3<T> d{w.P[bi].x - w.P[bj].x, w.P[bi].y - w.P[bj].y, w.P[bi].z - w.P[bj].z};
                 const T n = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
@@ -541,6 +545,7 @@ T hinge_one(const orc_robot& R, const orc_scene& S, const float* q, const orc_co
         }
         if (kink && std::fabs(h) < tol) *kink = true;
     }
+    if (nterm) *nterm = terms;
     const T wgt = (T)p.weight;
     if (grad) {
         // dp_s/dq for every ancestor joint of sphere s's link: revolute
@@ -758,7 +763,7 @@ orc_scene sub_scene(const orc_scene& all, int o0, int o1) {
 template <typename T>
 void stomp_costs_t(const orc_robot* r, const orc_scene* all, const int* obb_off, const float* rollouts,
                    const int* T_p, const int* roff, int P, int R, const orc_cost_params* p, double w_u, double dt,
-                   T* state, T* total, int nthreads) {
+                   T* state, T* total, int nthreads, int* nterm = nullptr) {
     set_threads(nthreads);
     const int D = r->n_dof;
     // Waypoint-level state costs.
@@ -778,7 +783,8 @@ void stomp_costs_t(const orc_robot* r, const orc_scene* all, const int* obb_off,
             const orc_scene sc = sub_scene(*all, obb_off[I.prob], obb_off[I.prob + 1]);
             const float* th = rollouts + ((size_t)roff[I.prob] + (size_t)I.roll * Tn) * D;
             for (int d = 0; d < D; ++d) q[d] = th[(size_t)d * Tn + I.t];
-            state[(size_t)roff[I.prob] + (size_t)I.roll * Tn + I.t] = hinge_one<T>(*r, sc, q, *p, w, nullptr, nullptr, (T)0);
+            const size_t at = (size_t)roff[I.prob] + (size_t)I.roll * Tn + I.t;
+            state[at] = hinge_one<T>(*r, sc, q, *p, w, nullptr, nullptr, (T)0, nullptr, nterm ? nterm + at : nullptr);
         }
     }
     for (int pi = 0; pi < P; ++pi) {
@@ -885,7 +891,7 @@ void orc_min_clearance_amb_f64(const orc_robot* r, const orc_scene* s, const flo
 }
 
 void orc_hinge_cost_amb_f64(const orc_robot* r, const orc_scene* s, const float* q, int n, const orc_cost_params* p,
-                            double* cost, uint8_t* vamb, int nthreads) {
+                            double* cost, uint8_t* vamb, int* nterm, int nthreads) {
     set_threads(nthreads);
 #pragma omp parallel
     {
@@ -893,8 +899,9 @@ void orc_hinge_cost_amb_f64(const orc_robot* r, const orc_scene* s, const float*
 #pragma omp for schedule(static)
         for (int i = 0; i < n; ++i) {
             bool va = false;
-            cost[i] = hinge_one<double>(*r, *s, q + (size_t)i * r->n_dof, *p, w, nullptr, nullptr, 0.0, &va);
-            vamb[i] = va ? 1 : 0;
+            cost[i] = hinge_one<double>(*r, *s, q + (size_t)i * r->n_dof, *p, w, nullptr, nullptr, 0.0, &va,
+                                        nterm ? nterm + i : nullptr);
+            if (vamb) vamb[i] = va ? 1 : 0;
         }
     }
 }
@@ -1063,8 +1070,8 @@ void orc_control_cost_f64(const float* theta, const int* off, int B, int D, doub
 
 void orc_stomp_costs_f64(const orc_robot* r, const orc_scene* all, const int* obb_off, const float* rollouts,
                          const int* T_p, const int* roff, int P, int R, const orc_cost_params* p, double w_u,
-                         double dt, double* state, double* total, int nthreads) {
-    stomp_costs_t<double>(r, all, obb_off, rollouts, T_p, roff, P, R, p, w_u, dt, state, total, nthreads);
+                         double dt, double* state, double* total, int nthreads, int* nterm) {
+    stomp_costs_t<double>(r, all, obb_off, rollouts, T_p, roff, P, R, p, w_u, dt, state, total, nthreads, nterm);
 }
 
 void orc_stomp_costs_f32(const orc_robot* r, const orc_scene* all, const int* obb_off, const float* rollouts,

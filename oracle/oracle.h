@@ -112,9 +112,13 @@ void orc_hinge_cost_f64(const orc_robot* r, const orc_scene* s, const float* q, 
 void orc_hinge_cost_f32(const orc_robot* r, const orc_scene* s, const float* q, int n,
                         const orc_cost_params* p, float* cost, int nthreads);
 /* FP64 state cost with the value-ambiguity flag of orc_min_clearance_amb_f64
- * for the ESDF hinge terms (a sphere whose lookup sits at such a face). */
+ * for the ESDF hinge terms (a sphere whose lookup sits at such a face), and
+ * the number of non-zero hinge terms (OBB, ESDF, self) in each cost: the
+ * parity tests' tolerance of a sum is the sum of its terms' tolerances.
+ * vamb and nterm may be null. */
 void orc_hinge_cost_amb_f64(const orc_robot* r, const orc_scene* s, const float* q, int n,
-                            const orc_cost_params* p, double* cost, uint8_t* vamb, int nthreads);
+                            const orc_cost_params* p, double* cost, uint8_t* vamb, int* nterm,
+                            int nthreads);
 void orc_hinge_grad_f64(const orc_robot* r, const orc_scene* s, const float* q, int n,
                         const orc_cost_params* p, double* cost, double* grad, int* kink,
                         double tol, int nthreads);
@@ -165,11 +169,12 @@ void orc_control_cost_f64(const float* theta, const int* off, int B, int D, doub
  * Problem p: rollouts [R][D][T_p] at waypoint offset roff[p] (R*T_p waypoints),
  * OBBs [obb_off[p], obb_off[p+1]) of the concatenated scene arrays.
  * state[...] = weight*(world hinge + self min-hinge) per rollout waypoint,
- * total[p*R + r] = sum_t state + w_u * sum_d theta^T R theta.                     */
+ * total[p*R + r] = sum_t state + w_u * sum_d theta^T R theta.
+ * nterm (may be null): non-zero hinge terms of each state cost.                   */
 void orc_stomp_costs_f64(const orc_robot* r, const orc_scene* all, const int* obb_off,
                          const float* rollouts, const int* T_p, const int* roff, int P, int R,
                          const orc_cost_params* p, double w_u, double dt, double* state,
-                         double* total, int nthreads);
+                         double* total, int nthreads, int* nterm);
 void orc_stomp_costs_f32(const orc_robot* r, const orc_scene* all, const int* obb_off,
                          const float* rollouts, const int* T_p, const int* roff, int P, int R,
                          const orc_cost_params* p, double w_u, double dt, float* state,

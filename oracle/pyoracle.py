@@ -138,8 +138,12 @@ class Oracle:
         self._scene_keep = keep
 
     # ---- kinematics ----
-    def fk_spheres(self, q) -> np.ndarray:
+    def fk_spheres(self, q, precision="f64") -> np.ndarray:
         q = _c(q, np.float32).reshape(-1, self.robot.n_dof)
+        if precision == "f32":
+            out = np.zeros((q.shape[0], self.robot.n_spheres, 3), np.float32)
+            lib().orc_fk_spheres_f32(C.byref(self.r), _ptr(q, _f32p), q.shape[0], _ptr(out, _f32p))
+            return out
         out = np.zeros((q.shape[0], self.robot.n_spheres, 3), np.float64)
         lib().orc_fk_spheres_f64(C.byref(self.r), _ptr(q, _f32p), q.shape[0], _ptr(out, _f64p))
         return out
@@ -199,14 +203,20 @@ class Oracle:
             return out, amb.astype(bool), low
         return (out, amb.astype(bool)) if with_amb else out
 
-    def hinge_cost_amb(self, q, params: OrcCostParams):
-        """FP64 state cost and its value-ambiguity flags."""
+    def hinge_cost_amb(self, q, params: OrcCostParams, with_terms=False):
+        """FP64 state cost and its value-ambiguity flags (and, with_terms,
+        the number of non-zero hinge terms in each cost)."""
         q = _c(q, np.float32).reshape(-1, self.robot.n_dof)
         out = np.zeros(q.shape[0], np.float64)
         amb = np.zeros(q.shape[0], np.uint8)
+        nterm = np.zeros(q.shape[0], np.int32)
         lib().orc_hinge_cost_amb_f64(C.byref(self.r), C.byref(self.s), _ptr(q, _f32p), q.shape[0], C.byref(params),
-                                     _ptr(out, _f64p), _ptr(amb, _u8p), self.nthreads)
-        return out, amb.astype(bool)
+                                     _ptr(out, _f64p), _ptr(amb, _u8p), _ptr(nterm, _i32p), self.nthreads)
+        return (out, amb.astype(bool), nterm) if with_terms else (out, amb.astype(bool))
+
+    def hinge_terms(self, q, params: OrcCostParams):
+        """Number of non-zero hinge terms (OBB, ESDF, self) in each FP64 state cost."""
+        return self.hinge_cost_amb(q, params, with_terms=True)[2]
 
     def hinge_cost(self, q, params: OrcCostParams, precision="f64"):
         q = _c(q, np.float32).reshape(-1, self.robot.n_dof)
@@ -295,8 +305,10 @@ class Oracle:
            int(skip_first), C.c_float(margin), int(check_self), _ptr(fh, _i32p), self.nthreads)
         return fh
 
-    def stomp_costs(self, all_scene, obb_off, rollouts, T_p, roff, R, params, w_u, dt, precision="f64"):
-        """all_scene: HostScene with every problem's OBBs concatenated."""
+    def stomp_costs(self, all_scene, obb_off, rollouts, T_p, roff, R, params, w_u, dt, precision="f64",
+                    with_terms=False):
+        """all_scene: HostScene with every problem's OBBs concatenated.
+        with_terms (FP64 only): also the non-zero hinge terms per state."""
         self.set_scene(all_scene)
         ro = _c(rollouts, np.float32)
         obb_off = _c(obb_off, np.int32)
@@ -307,11 +319,15 @@ class Oracle:
         dt_ = np.float64 if precision == "f64" else np.float32
         state = np.zeros(n_wp, dt_)
         total = np.zeros(P * R, dt_)
-        fn = lib().orc_stomp_costs_f64 if precision == "f64" else lib().orc_stomp_costs_f32
         tp = _f64p if precision == "f64" else _f32p
-        fn(C.byref(self.r), C.byref(self.s), _ptr(obb_off, _i32p), _ptr(ro, _f32p), _ptr(T_p, _i32p),
-           _ptr(roff, _i32p), P, int(R), C.byref(params), C.c_double(w_u), C.c_double(dt),
-           _ptr(state, tp), _ptr(total, tp), self.nthreads)
+        args = (C.byref(self.r), C.byref(self.s), _ptr(obb_off, _i32p), _ptr(ro, _f32p), _ptr(T_p, _i32p),
+                _ptr(roff, _i32p), P, int(R), C.byref(params), C.c_double(w_u), C.c_double(dt),
+                _ptr(state, tp), _ptr(total, tp), self.nthreads)
+        if precision == "f64":
+            nterm = np.zeros(n_wp, np.int32)
+            lib().orc_stomp_costs_f64(*args, _ptr(nterm, _i32p))
+            return (state, total, nterm) if with_terms else (state, total)
+        lib().orc_stomp_costs_f32(*args)
         return state, total
 
 

@@ -8,20 +8,18 @@
     boxes per problem, self pairs on: per-waypoint state cost and
     per-rollout totals, unit-major and waypoint-major plans.
 
-Bars (north_star): verdicts exact outside |clearance| < 1e-5 m (counted);
-costs 1e-5 relative (denominator max(|ref|, w m)); gradients 1e-4
-relative, norm-wise per waypoint, away from oracle-flagged kinks (counted).
-The FD kernel's bar follows from the cost bar: each FD entry is
-w (C+ - C-) / (2 eps) of two FP32 cost evaluations, each within
-1e-5 max(|C|, m) of FP64, so it may differ from the FP64 FD by at most
-w 1e-5 (max(|C+|, m) + max(|C-|, m)) / (2 eps).
+Bars: verdicts exact outside |clearance| < 1e-5 m (counted); costs 1e-5
+relative with the denominator cost_den of parity_common (its FP32 error
+model: 0.05 m per non-zero hinge term); gradients 1e-4 relative, norm-wise
+per waypoint, away from oracle-flagged kinks (counted). The FD kernel's bar
+follows from the cost bar of its two evaluations (parity_common.fd_bar).
 """
 import numpy as np
 import pytest
 import torch
 
 import pyoracle as PO
-from parity_common import banner
+from parity_common import banner, cost_den, fd_bar, traj_rows
 from prismplan_amd import substrate as S
 from prismplan_amd import workloads as W
 
@@ -34,14 +32,6 @@ THREADS = 16
 @pytest.fixture(scope="module")
 def robots():
     return {n: S.Robot(n) for n in ("panda", "fetch")}
-
-
-def traj_rows(flat, off, D):
-    rows = []
-    for b in range(len(off) - 1):
-        T = off[b + 1] - off[b]
-        rows.append(flat[off[b] * D:off[b + 1] * D].reshape(D, T).T)
-    return np.concatenate(rows)
 
 
 def test_edges_at_full_size(robots):
@@ -95,9 +85,9 @@ def test_chomp_at_full_size(robots):
     o = PO.Oracle(w.robot, w.scene, nthreads=THREADS)
     tot_ref, g_ref, kink = o.traj_cost_grad(w.theta, w.offsets, prm_o)
     rows = traj_rows(w.theta, w.offsets, D)
-    st_ref = o.hinge_cost(rows, prm_o)
-    st_err = np.abs(st.cpu().numpy() - st_ref) / np.maximum(np.abs(st_ref), wt * m)
-    rel = np.abs(total.cpu().numpy() - tot_ref) / np.maximum(np.abs(tot_ref), wt * m)
+    st_ref, _, nt = o.hinge_cost_amb(rows, prm_o, with_terms=True)
+    st_err = np.abs(st.cpu().numpy() - st_ref) / cost_den(st_ref, wt, nt)
+    rel = np.abs(total.cpu().numpy() - tot_ref) / cost_den(tot_ref, wt, np.add.reduceat(nt, w.offsets[:-1]))
     banner(f"  state cost max rel err {st_err.max():.2e}, trajectory total {rel.max():.2e} "
            f"(active waypoints {np.mean(st_ref > 0):.2f})")
     assert st_err.max() < 1e-5 and rel.max() < 1e-5
@@ -117,16 +107,7 @@ def test_chomp_at_full_size(robots):
     eps = np.float32(1e-3)
     gfd = traj_rows(S.chomp_fd_grad(plan, ds, th, prm, eps=float(eps)).cpu().numpy(), w.offsets, D)
     gfd_ref = traj_rows(o.chomp_fd_grad(w.theta, w.offsets, prm_o, eps=float(eps)), w.offsets, D)
-    qp = np.repeat(rows[:, None, :], D, axis=1)            # [wp, d, D]
-    qm = qp.copy()
-    idx = np.arange(D)
-    plus = (rows + eps).astype(np.float32)                 # q[d] += eps, then -= 2 eps (FP32)
-    qp[:, idx, idx] = plus
-    qm[:, idx, idx] = (plus - np.float32(2 * eps)).astype(np.float32)
-    prm1 = PO.cost_params(world_margin=m, weight=1.0, use_self=False)
-    cp = o.hinge_cost(qp.reshape(-1, D), prm1).reshape(-1, D)
-    cm = o.hinge_cost(qm.reshape(-1, D), prm1).reshape(-1, D)
-    tol = wt * 1e-5 * (np.maximum(np.abs(cp), m) + np.maximum(np.abs(cm), m)) / (2 * float(eps))
+    tol = fd_bar(o, rows, PO.cost_params(world_margin=m, weight=1.0, use_self=False), wt, eps)
     diff = np.abs(gfd - gfd_ref)
     bad = (diff > tol) & interior[:, None]
     banner(f"  FD kernel: max |diff| {diff.max():.2e}, max diff / bar {np.max(diff / tol):.2f}, over bar "
@@ -149,9 +130,10 @@ def test_stomp_at_full_size(robots, wmajor):
     prm = S.cost_params(world_margin=m, self_margin=0.01, weight=wc)
     state, total = S.stomp_rollout_costs(plan, ds, ro, prm, control_weight=0.1, dt=0.1)
     o = PO.Oracle(wk.robot, nthreads=THREADS)
-    s_ref, t_ref = o.stomp_costs(wk.scene, wk.obb_offsets, wk.rollouts, wk.lengths, wk.roll_offsets, R,
-                                 PO.cost_params(world_margin=m, self_margin=0.01, weight=wc), 0.1, 0.1)
-    srel = np.abs(state.cpu().numpy() - s_ref) / np.maximum(np.abs(s_ref), wc * m)
+    s_ref, t_ref, nt = o.stomp_costs(wk.scene, wk.obb_offsets, wk.rollouts, wk.lengths, wk.roll_offsets, R,
+                                     PO.cost_params(world_margin=m, self_margin=0.01, weight=wc), 0.1, 0.1,
+                                     with_terms=True)
+    srel = np.abs(state.cpu().numpy() - s_ref) / cost_den(s_ref, wc, nt)
     trel = np.abs(total.cpu().numpy() - t_ref) / np.abs(t_ref)
     banner(f"  state cost max rel err {srel.max():.2e} (active {np.mean(s_ref > 0):.2f}), rollout total "
            f"{trel.max():.2e}")

@@ -5,8 +5,9 @@ sparse_cc / full_cc's collision part follows MITStar.cu:832-1375: breadth-
 first midpoints t = (2 (k + 1 - 2^L) + 1) / 2^(L + 1), q = q_src (1 - t) +
 q_dst t, checked with the two-phase test and the self check. Verdicts must
 match exactly except for states within 1e-5 m of contact (counted);
-clearances agree to 1e-5 relative (denominator max(|ref|, margin)) on 99.5%
-of samples, the nearest-voxel bar of the ESDF tests.
+clearances agree to 1e-5 relative (denominator max(|ref|, 0.05 m), the FP32
+error model of parity_common) at every result the oracle does not flag as
+value-ambiguous (a nearest-voxel lookup on a face; counted).
 """
 import numpy as np
 import pytest
@@ -16,7 +17,7 @@ import pyoracle as PO
 from prismplan_amd import substrate as S
 from prismplan_amd import workloads as W
 from prismplan_amd.planning_scene import HostScene
-from parity_common import assert_values
+from parity_common import assert_values, clearance_den
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
@@ -68,7 +69,7 @@ def test_batch_cc_and_node_clearance():
     assert 0.2 < ref.mean() < 0.8
     assert np.all(out[cc == 1] == -7.0)                 # colliding samples keep their entry
     free = cc == 0
-    assert_values(out[free], mref[free], vamb[free], np.maximum(np.abs(mref[free]), 0.01),
+    assert_values(out[free], mref[free], vamb[free], clearance_den(mref[free]),
                   "  free-sample clearance")
     out2 = torch.full((len(q),), -7.0, device=DEV)
     cc2 = S.mitstar_batch_cc(panda, ds, x, 0.01, out2, clearance_weight=0.0).cpu().numpy()
@@ -83,12 +84,12 @@ def test_batch_cc_and_node_clearance():
     rest = np.ones(len(q), bool)
     rest[idx] = False
     assert np.all(dst[rest] == -3.0)
-    assert_values(dst[idx], mref[idx], vamb[idx], np.maximum(np.abs(mref[idx]), 0.01), "  gathered node clearance")
+    assert_values(dst[idx], mref[idx], vamb[idx], clearance_den(mref[idx]), "  gathered node clearance")
     seq = torch.full((len(q),), -3.0, device=DEV)
     S.mitstar_node_clearance(panda, ds, x, seq, node_start=100, n_nodes=50, collision_margin=0.01)
     seq = seq.cpu().numpy()
     assert np.all(seq[:100] == -3.0) and np.all(seq[150:] == -3.0)
-    assert_values(seq[100:150], mref[100:150], vamb[100:150], np.maximum(np.abs(mref[100:150]), 0.01),
+    assert_values(seq[100:150], mref[100:150], vamb[100:150], clearance_den(mref[100:150]),
                   "  node clearance range")
 
 

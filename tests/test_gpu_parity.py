@@ -3,8 +3,10 @@
 Tolerances (stated per test):
   * Boolean verdicts match exactly except where the FP64 clearance of the
     configuration satisfies |clr| < 1e-5 m (ambiguous band, counted);
-  * costs and clearances within 1e-5 relative, denominator max(|ref|, w*m)
-    (w the cost weight, m the hinge margin: one sphere at full band depth);
+  * costs and clearances within 1e-5 relative, with the denominators of
+    parity_common's FP32 error model: max(|ref|, 0.05 m) for a clearance,
+    max(|ref|, w * 0.05 m * n) for a weighted sum of n non-zero hinge terms
+    (n from the FP64 oracle);
   * gradients within 1e-4 relative of the FP64 analytic adjoint, norm-wise
     per waypoint: max_d |g - ref| / max(max_d |ref|, 1e-3 * w), excluding
     waypoints within 1e-5 m of a hinge kink, an argmin tie or a box-SDF kink
@@ -24,7 +26,7 @@ from prismplan_amd import _ppl
 from prismplan_amd import substrate as S
 from prismplan_amd import workloads as W
 from prismplan_amd.planning_scene import HostScene
-from parity_common import assert_values
+from parity_common import assert_values, clearance_den, cost_den, fd_bar, traj_rows, traj_terms
 
 pytestmark = pytest.mark.gpu
 BAND = 1e-5
@@ -36,15 +38,6 @@ def grad_rel_err(g, ref, w=1.0):
     num = np.abs(g - ref).max(axis=1)
     den = np.maximum(np.abs(ref).max(axis=1), 1e-3 * w)
     return num / den
-
-
-def traj_rows(flat, off, D):
-    """Ragged [B] x [D][T_b] -> [waypoints, D]."""
-    rows = []
-    for b in range(len(off) - 1):
-        T = off[b + 1] - off[b]
-        rows.append(flat[off[b] * D:off[b + 1] * D].reshape(D, T).T)
-    return np.concatenate(rows)
 
 
 def banner(msg):
@@ -160,7 +153,7 @@ def test_esdf_any_collision_and_min_clearance(robots, interp):
     mc = S.min_clearance_batched(plan, ds, torch.from_numpy(q).to(DEV), collision_margin=0.01).cpu().numpy()
     mref, vamb = o.min_clearance(q, margin=0.01, with_amb=True)
     # a lookup at a voxel face may take either voxel in FP32: the oracle flags exactly those
-    assert_values(mc, mref, vamb, np.maximum(np.abs(mref), 0.01), f"min clearance interp={interp}")
+    assert_values(mc, mref, vamb, clearance_den(mref), f"min clearance interp={interp}")
 
 
 def test_plain_clearance_and_hinge_forms(robots):
@@ -176,17 +169,17 @@ def test_plain_clearance_and_hinge_forms(robots):
         o = PO.Oracle(r, s, nthreads=8)
         mc = S.min_clearance_scene(robots[name], ds, q, collision_margin=0.01)
         mref, vamb = o.min_clearance(q, margin=0.01, with_amb=True)
-        assert_values(mc, mref, vamb, np.maximum(np.abs(mref), 0.01), f"  plain clearance {name}")
+        assert_values(mc, mref, vamb, clearance_den(mref), f"  plain clearance {name}")
         plan = S.Plan.configs(robots[name], len(q))
         x = torch.from_numpy(q).to(DEV)
         np.testing.assert_array_equal(mc, S.min_clearance_batched(plan, ds, x, collision_margin=0.01).cpu().numpy())
         for use_self in (False, True):
             prm = S.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0, use_self=use_self)
             c = S.hinge_cost_scene(robots[name], ds, q, prm)
-            c_ref, vamb = o.hinge_cost_amb(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0,
-                                                             use_self=use_self))
+            c_ref, vamb, nt = o.hinge_cost_amb(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0,
+                                                                 use_self=use_self), with_terms=True)
             banner(f"  hinge use_self={use_self}: active {np.mean(c_ref > 0):.2f}")
-            assert_values(c, c_ref, vamb, np.maximum(np.abs(c_ref), 2.0 * 0.05), f"  plain hinge {name}")
+            assert_values(c, c_ref, vamb, cost_den(c_ref, 2.0, nt), f"  plain hinge {name}")
             # the batched form may split a state's spheres over lanes: same terms, other summation order
             np.testing.assert_allclose(c, S.hinge_cost(plan, ds, x, prm)[0].cpu().numpy(), rtol=2e-6, atol=1e-7)
     assert S.min_clearance_scene(robots["panda"], ds, np.zeros((0, 7), np.float32)).shape == (0,)
@@ -300,7 +293,7 @@ def test_plain_edge_clearance_with_esdf(robots):
         assert m == pytest.approx(rclr[k], abs=1e-12)
         # a flagged state moves the edge's minimum when it holds it or its other side undercuts it
         amb[k] = bool(np.any(a_st & ((c_st <= m) | (lo_st < m))))
-    assert_values(clr, rclr, amb, np.maximum(np.abs(rclr), 0.01), "edge clearance")
+    assert_values(clr, rclr, amb, clearance_den(rclr), "edge clearance")
 
 
 @pytest.mark.parametrize("n_obbs,seed", [(64, 1), (10, 42)])
@@ -348,9 +341,9 @@ def test_self_culling_without_enclosing_approx_spheres():
     plan = S.Plan.configs(rob, len(q))
     ds = S.DeviceScene.from_host(scene)
     c = S.hinge_cost(plan, ds, torch.from_numpy(q).to(DEV), prm)[0].cpu().numpy()
-    c_ref = PO.Oracle(r, scene, nthreads=8).hinge_cost(q, PO.cost_params(world_margin=0.05, self_margin=0.05,
-                                                                         weight=1.0))
-    rel = np.abs(c - c_ref) / np.maximum(np.abs(c_ref), 0.05)
+    c_ref, _, nt = PO.Oracle(r, scene, nthreads=8).hinge_cost_amb(
+        q, PO.cost_params(world_margin=0.05, self_margin=0.05, weight=1.0), with_terms=True)
+    rel = np.abs(c - c_ref) / cost_den(c_ref, 1.0, nt)
     banner(f"self hinge with shrunk approx spheres: max rel err {rel.max():.2e}, active {np.mean(c_ref > 0):.2f}")
     assert np.mean(c_ref > 0) > 0.05
     assert rel.max() < 1e-5
@@ -373,7 +366,7 @@ def test_hinge_cost_configs(robots):
         o = PO.Oracle(r, scene, nthreads=8)
         c_ref, g_ref, kink = o.hinge_grad(q, prm_o)
         c = st.cpu().numpy()
-        rel = np.abs(c - c_ref) / np.maximum(np.abs(c_ref), 2.0 * 0.05)
+        rel = np.abs(c - c_ref) / cost_den(c_ref, 2.0, o.hinge_terms(q, prm_o))
         banner(f"  cost max rel err {rel.max():.2e} (active {np.mean(c_ref > 0):.2f})")
         assert rel.max() < 1e-5
         g = g.cpu().numpy()
@@ -394,7 +387,7 @@ def test_chomp_adjoint_vs_oracle_and_fd(robots):
     o = PO.Oracle(w.robot, w.scene, nthreads=8)
     tot_ref, g_ref, kink = o.traj_cost_grad(w.theta, w.offsets, prm_o)
     total = total.cpu().numpy()
-    rel = np.abs(total - tot_ref) / np.maximum(np.abs(tot_ref), 0.05)
+    rel = np.abs(total - tot_ref) / cost_den(tot_ref, 1.0, traj_terms(o, w.theta, w.offsets, 7, prm_o))
     banner(f"  total max rel err {rel.max():.2e}")
     assert rel.max() < 1e-5
     g = traj_rows(g.cpu().numpy(), w.offsets, 7)
@@ -408,13 +401,15 @@ def test_chomp_adjoint_vs_oracle_and_fd(robots):
     assert e.max() < 1e-4
     assert np.all(g[~interior] == 0)
     # the finite-difference kernel (chomp_obs_grad_kernel semantics) vs the FP64 oracle FD
-    gfd = S.chomp_fd_grad(plan, ds, th, prm, eps=1e-3).cpu().numpy()
-    gfd_ref = o.chomp_fd_grad(w.theta, w.offsets, prm_o, eps=1e-3)
-    # FP32 FD carries ~ulp(C)/(2 eps) of cancellation noise on top of FP64
-    tol = 1e-4 * np.abs(gfd_ref) + 2e-3
-    bad = np.abs(gfd - gfd_ref) > tol
-    banner(f"  FD kernel: max abs diff {np.abs(gfd - gfd_ref).max():.2e}, entries over tol {int(bad.sum())}")
-    assert bad.mean() < 1e-3
+    gfd = traj_rows(S.chomp_fd_grad(plan, ds, th, prm, eps=1e-3).cpu().numpy(), w.offsets, 7)
+    gfd_ref = traj_rows(o.chomp_fd_grad(w.theta, w.offsets, prm_o, eps=1e-3), w.offsets, 7)
+    tol = fd_bar(o, traj_rows(w.theta, w.offsets, 7), PO.cost_params(world_margin=0.05, weight=1.0, use_self=False),
+                 1.0, 1e-3)
+    diff = np.abs(gfd - gfd_ref)
+    bad = (diff > tol) & interior[:, None]
+    banner(f"  FD kernel: max |diff| {diff.max():.2e}, max diff / bar {np.max(diff / tol):.2f}, over bar "
+           f"{int(bad.sum())}")
+    assert not bad.any()
 
 
 def test_autograd_function(robots):
@@ -446,10 +441,11 @@ def test_stomp_rollout_costs_and_graph(robots, wmajor):
     prm = S.cost_params(world_margin=0.02, self_margin=0.01, weight=20.0)
     state, total = S.stomp_rollout_costs(plan, ds, ro, prm, control_weight=0.1, dt=0.1)
     o = PO.Oracle(wk.robot)
-    s_ref, t_ref = o.stomp_costs(wk.scene, wk.obb_offsets, wk.rollouts, wk.lengths, wk.roll_offsets, R,
-                                 PO.cost_params(world_margin=0.02, self_margin=0.01, weight=20.0), 0.1, 0.1)
+    s_ref, t_ref, nt = o.stomp_costs(wk.scene, wk.obb_offsets, wk.rollouts, wk.lengths, wk.roll_offsets, R,
+                                     PO.cost_params(world_margin=0.02, self_margin=0.01, weight=20.0), 0.1, 0.1,
+                                     with_terms=True)
     st = state.cpu().numpy()
-    rel = np.abs(st - s_ref) / np.maximum(np.abs(s_ref), 20.0 * 0.02)
+    rel = np.abs(st - s_ref) / cost_den(s_ref, 20.0, nt)
     banner(f"  state cost max rel err {rel.max():.2e}")
     assert rel.max() < 1e-5
     tt = total.cpu().numpy()
@@ -515,7 +511,7 @@ def test_esdf_build_and_baxter_trilinear_adjoint(robots):
     o = PO.Oracle(wk.robot, wk.scene, nthreads=8)
     prm_o = PO.cost_params(world_margin=0.05, self_margin=0.01, weight=1.0, use_obb=False, use_self=False)
     tot_ref, g_ref, kink = o.traj_cost_grad(wk.theta, wk.offsets, prm_o)
-    rel = np.abs(total.cpu().numpy() - tot_ref) / np.maximum(np.abs(tot_ref), 0.05)
+    rel = np.abs(total.cpu().numpy() - tot_ref) / cost_den(tot_ref, 1.0, traj_terms(o, wk.theta, wk.offsets, 14, prm_o))
     banner(f"  total max rel err {rel.max():.2e} active={np.mean(tot_ref > 0):.2f}")
     assert rel.max() < 1e-5
     g = traj_rows(g.cpu().numpy(), wk.offsets, 14)
@@ -543,10 +539,9 @@ def test_esdf_adjoint_at_full_size(robots):
     prm = S.cost_params(world_margin=0.02, weight=1.0, use_obb=False, use_self=False)
     _, total, g = S.hinge_cost(plan, S.DeviceScene.from_host(wk.scene), th, prm, grad=True)
     o = PO.Oracle(wk.robot, wk.scene, nthreads=16)
-    tot_ref, g_ref, kink = o.traj_cost_grad(wk.theta, wk.offsets,
-                                            PO.cost_params(world_margin=0.02, weight=1.0, use_obb=False,
-                                                           use_self=False))
-    rel = np.abs(total.cpu().numpy() - tot_ref) / np.maximum(np.abs(tot_ref), 0.02)
+    prm_o = PO.cost_params(world_margin=0.02, weight=1.0, use_obb=False, use_self=False)
+    tot_ref, g_ref, kink = o.traj_cost_grad(wk.theta, wk.offsets, prm_o)
+    rel = np.abs(total.cpu().numpy() - tot_ref) / cost_den(tot_ref, 1.0, traj_terms(o, wk.theta, wk.offsets, 14, prm_o))
     g = traj_rows(g.cpu().numpy(), wk.offsets, 14)
     gr = traj_rows(g_ref, wk.offsets, 14)
     interior = np.ones(len(gr), bool)
@@ -648,19 +643,19 @@ def test_device_inline_api(robots, name, G, frames):
     assert 0.05 < ref.mean() < 0.95, "the world should make some configurations collide and leave others free"
     assert_verdicts(hit.cpu().numpy(), ref, clr, f"device two_phase_cc {name}")
     ref_mc, vamb = o.min_clearance(q, with_amb=True)
-    assert_values(mc, ref_mc, vamb, np.maximum(np.abs(ref_mc), 0.01), f"  device min clearance {name}")
-    # cost tolerance 1e-5 relative, denominator max(|ref|, weight * margin) as in the other hinge tests
-    for what, prm, got, scale in (("obb hinge", PO.cost_params(world_margin=0.03, use_obb=True, use_esdf=False,
-                                                               use_self=False), ho, 0.03),
-                                  ("esdf hinge", PO.cost_params(world_margin=0.03, use_obb=False, use_esdf=True,
-                                                                use_self=False), he, 0.03),
-                                  ("self hinge", PO.cost_params(self_margin=0.01, use_obb=False, use_esdf=False,
-                                                                use_self=True), sh, 0.01)):
-        exp, hamb = o.hinge_cost_amb(q, prm)
-        err32 = np.abs(o.hinge_cost(q, prm, precision="f32") - exp) / np.maximum(np.abs(exp), scale)
-        banner(f"  {what}: FP32 restatement max rel err {err32.max():.2e} (denominator max(|ref|, {scale}), "
-               f"active {np.mean(exp > 0):.2f})")
-        assert_values(got, exp, hamb, np.maximum(np.abs(exp), scale), f"  device {what} {name}")
+    assert_values(mc, ref_mc, vamb, clearance_den(ref_mc), f"  device min clearance {name}")
+    # cost tolerance 1e-5 relative, denominator cost_den (the FP32 error model of parity_common)
+    for what, prm, got in (("obb hinge", PO.cost_params(world_margin=0.03, use_obb=True, use_esdf=False,
+                                                        use_self=False), ho),
+                           ("esdf hinge", PO.cost_params(world_margin=0.03, use_obb=False, use_esdf=True,
+                                                         use_self=False), he),
+                           ("self hinge", PO.cost_params(self_margin=0.01, use_obb=False, use_esdf=False,
+                                                         use_self=True), sh)):
+        exp, hamb, nt = o.hinge_cost_amb(q, prm, with_terms=True)
+        den = cost_den(exp, 1.0, nt)
+        err32 = np.abs(o.hinge_cost(q, prm, precision="f32") - exp) / den
+        banner(f"  {what}: FP32 restatement max rel err {err32.max():.2e}, active {np.mean(exp > 0):.2f}")
+        assert_values(got, exp, hamb, den, f"  device {what} {name}")
 
 
 def test_allowed_collision_masks(robots):
@@ -696,14 +691,14 @@ def test_allowed_collision_masks(robots):
     assert_verdicts(got, ref_s, clr_s, "batched any-collision + self with ACM")
     mc = S.min_clearance_batched(plan, ds, x, collision_margin=0.01).cpu().numpy()
     mref, vamb = o.min_clearance(q, margin=0.01, with_amb=True)
-    assert_values(mc, mref, vamb, np.maximum(np.abs(mref), 0.01), "  min clearance with ACM")
+    assert_values(mc, mref, vamb, clearance_den(mref), "  min clearance with ACM")
     prm = S.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0, use_self=False)
     c = S.hinge_cost(plan, ds, x, prm)[0].cpu().numpy()
-    c_ref, hamb = o.hinge_cost_amb(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0,
-                                                     use_self=False))
+    c_ref, hamb, nt = o.hinge_cost_amb(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0,
+                                                         use_self=False), with_terms=True)
     c_open = o_open.hinge_cost(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0, use_self=False))
     banner(f"  hinge with ACM: cost lowered on {np.mean(c_ref < c_open - 1e-6):.2f}")
-    assert_values(c, c_ref, hamb, np.maximum(np.abs(c_ref), 2.0 * 0.05), "  hinge with ACM")
+    assert_values(c, c_ref, hamb, cost_den(c_ref, 2.0, nt), "  hinge with ACM")
     # the fused adjoint's gradient follows the masked argmin
     _, _, g = S.hinge_cost(plan, ds, x, prm, grad=True)
     c_g, g_ref, kink = o.hinge_grad(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0,
@@ -735,10 +730,11 @@ def test_allowed_collision_masks(robots):
                                                                 use_self=False)),
                                ("esdf", outs[2], PO.cost_params(world_margin=0.05, use_obb=False, use_esdf=True,
                                                                 use_self=False))):
-            exp, hamb = o.hinge_cost_amb(q, prm)
-            err32 = np.abs(o.hinge_cost(q, prm, precision="f32") - exp) / np.maximum(np.abs(exp), 0.05)
+            exp, hamb, nt = o.hinge_cost_amb(q, prm, with_terms=True)
+            den = cost_den(exp, 1.0, nt)
+            err32 = np.abs(o.hinge_cost(q, prm, precision="f32") - exp) / den
             banner(f"  device {what} hinge with ACM: FP32 restatement max rel err {err32.max():.2e}")
-            assert_values(got.cpu().numpy(), exp, hamb, np.maximum(np.abs(exp), 0.05), f"  device {what} hinge with ACM")
+            assert_values(got.cpu().numpy(), exp, hamb, den, f"  device {what} hinge with ACM")
 
 
 def varied_scene(interp, layer1_enabled, seed=90):
@@ -792,12 +788,13 @@ def test_scene_variants(robots, interp, layer1):
                     "batched any-collision + self, varied scene")
     mc = S.min_clearance_batched(plan, ds, x, collision_margin=0.01).cpu().numpy()
     mref, vamb = o.min_clearance(q, margin=0.01, with_amb=True)
-    assert_values(mc, mref, vamb, np.maximum(np.abs(mref), 0.01), "  min clearance, varied scene")
+    assert_values(mc, mref, vamb, clearance_den(mref), "  min clearance, varied scene")
     prm = S.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0, use_self=True)
     c = S.hinge_cost(plan, ds, x, prm)[0].cpu().numpy()
-    c_ref, hamb = o.hinge_cost_amb(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0, use_self=True))
+    c_ref, hamb, nt = o.hinge_cost_amb(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0,
+                                                         use_self=True), with_terms=True)
     banner(f"  hinge: active {np.mean(c_ref > 0):.2f}")
-    assert_values(c, c_ref, hamb, np.maximum(np.abs(c_ref), 2.0 * 0.05), "  hinge, varied scene")
+    assert_values(c, c_ref, hamb, cost_den(c_ref, 2.0, nt), "  hinge, varied scene")
 
 
 def test_algorithmic_flop_counts(robots):
@@ -871,8 +868,9 @@ def test_plain_chomp_forms(robots):
         th = th_all[b * D * T:(b + 1) * D * T]
         sc = S.chomp_state_cost(panda, ds, th, T, prm).cpu().numpy()
         q = th.cpu().numpy().reshape(D, T).T
-        ref = o.hinge_cost(q, PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0))
-        rel = np.abs(sc - ref[None, :]) / np.maximum(np.abs(ref[None, :]), 2.0 * 0.05)
+        prm_o = PO.cost_params(world_margin=0.05, self_margin=0.01, weight=2.0)
+        ref, _, nt = o.hinge_cost_amb(q, prm_o, with_terms=True)
+        rel = np.abs(sc - ref[None, :]) / cost_den(ref, 2.0, nt)[None, :]
         assert rel.max() < 1e-5
         fd = S.chomp_fd_grad_plain(panda, ds, th, T, prm, eps=1e-3).cpu().numpy()
         np.testing.assert_array_equal(fd.reshape(-1), fd_b[b * D * T:(b + 1) * D * T])
