@@ -478,6 +478,19 @@ ppl_status ppl_mitstar_batch_cc(ppl_robot robot, const ppl_scene_view* scene, co
                                 float collision_margin, uint8_t* d_cc, float* d_clearance, float clearance_weight,
                                 ppl_stream stream);
 
+/* Allocation-free form of ppl_mitstar_batch_cc. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as ppl_mitstar_batch_cc; its scratch (self
+ * verdicts and clearances) is a fixed device array of the library, one per
+ * GPU, held until the process ends, so n above PPL_BATCH_CC_STATIC_MAX
+ * returns PPL_ERR_CAPACITY. Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (different streams or threads) race. */
+#define PPL_BATCH_CC_STATIC_MAX 262144
+ppl_status ppl_mitstar_batch_cc_static(ppl_robot robot, const ppl_scene_view* scene, const float* d_samples, int n,
+                                       float collision_margin, uint8_t* d_cc, float* d_clearance,
+                                       float clearance_weight, ppl_stream stream);
+
 /* compute_node_clearance_kernel (MITStar.cu:283-329): for i < n_nodes, node
  * = d_node_indices ? d_node_indices[i] : node_start + i, and
  * d_clearance[node] = min ESDF clearance of d_nodes[node] (+ margin). */

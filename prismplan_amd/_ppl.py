@@ -172,6 +172,8 @@ SIGNATURES = [
     ("ppl_hinge_cost_scene", C.c_int, [_vp, C.POINTER(SceneView), _f32p, C.c_int, C.POINTER(CostParams), _f32p]),
     ("ppl_mitstar_batch_cc", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.c_int, C.c_float, _vp, _vp, C.c_float,
                                        _vp]),
+    ("ppl_mitstar_batch_cc_static", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.c_int, C.c_float, _vp, _vp, C.c_float,
+                                       _vp]),
     ("ppl_mitstar_node_clearance", C.c_int, [_vp, C.POINTER(SceneView), _vp, _vp, C.c_int, C.c_int, C.c_float, _vp,
                                              _vp]),
     ("ppl_mitstar_edge_cc", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_float,

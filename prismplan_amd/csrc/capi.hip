@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -2006,20 +2007,24 @@ ppl_status ppl_hinge_cost_scene(ppl_robot r, const ppl_scene_view* view, const f
     return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
-ppl_status ppl_mitstar_batch_cc(ppl_robot r, const ppl_scene_view* view, const float* d_samples, int n, float margin,
-                                uint8_t* d_cc, float* d_clearance, float clearance_weight, ppl_stream stream) try {
-    CHECK_ARG(r && d_samples && d_cc, "NULL argument");
-    CHECK_ARG(n >= 0, "negative sample count");
-    if (n == 0) return PPL_OK;
-    DScene S{};
-    ppl_status st = scene_from_view(view, S);
-    if (st != PPL_OK) return st;
-    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
-    hipStream_t s = (hipStream_t)stream;
-    // scratch: self verdicts [n] bytes, clearances [n] floats
-    HIP_TRY(r->d_mit.ensure(align256((size_t)n) + (size_t)n * sizeof(float)));
-    uint8_t* self = r->d_mit.as<uint8_t>();
-    float* clr = reinterpret_cast<float*>(self + align256((size_t)n));
+}  // extern "C"
+
+// The allocation-free form's scratch: self verdicts then clearances for up to
+// PPL_BATCH_CC_STATIC_MAX samples, one array per GPU, held until the process
+// ends (1.25 MiB). Host code never passes `scratch` itself to a kernel (on the
+// host the name is a stand-in, not device memory); it uses the device address
+// hipGetSymbolAddress returns on each device, kept in g_scratch_dev below.
+constexpr size_t kStaticSelfBytes = ((size_t)PPL_BATCH_CC_STATIC_MAX + 255) & ~(size_t)255;
+constexpr size_t kStaticScratchBytes = kStaticSelfBytes + (size_t)PPL_BATCH_CC_STATIC_MAX * sizeof(float);
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+namespace {
+// batch_cc over caller-provided scratch: self verdicts [n] bytes at `self`,
+// clearances [n] floats at `clr`. Every entry below n is written before it is
+// read, so the scratch needs no clearing. Allocates nothing on the device.
+ppl_status batch_cc_impl(ppl_robot r, const DScene& S, const float* d_samples, int n, float margin, uint8_t* d_cc,
+                         float* d_clearance, float clearance_weight, uint8_t* self, float* clr, hipStream_t s) {
+    ppl_status st = PPL_OK;
     CollideArgs a{};
     a.R = r->R;
     a.S = S;
@@ -2042,6 +2047,67 @@ ppl_status ppl_mitstar_batch_cc(ppl_robot r, const ppl_scene_view* view, const f
         HIP_TRY(launch_sample_clearance(d_cc, clr, n, clearance_weight, d_clearance, s));
     }
     return PPL_OK;
+}
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxScratchDevices] = {};
+
+ppl_status static_scratch(uint8_t** out) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) return fail(PPL_ERR_UNSUPPORTED, "device index %d >= %d", dev, kMaxScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(PPL_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *out = static_cast<uint8_t*>(g_scratch_dev[dev]);
+    return PPL_OK;
+}
+}  // namespace
+
+extern "C" {
+
+ppl_status ppl_mitstar_batch_cc(ppl_robot r, const ppl_scene_view* view, const float* d_samples, int n, float margin,
+                                uint8_t* d_cc, float* d_clearance, float clearance_weight, ppl_stream stream) try {
+    CHECK_ARG(r && d_samples && d_cc, "NULL argument");
+    CHECK_ARG(n >= 0, "negative sample count");
+    if (n == 0) return PPL_OK;
+    DScene S{};
+    ppl_status st = scene_from_view(view, S);
+    if (st != PPL_OK) return st;
+    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
+    // scratch: self verdicts [n] bytes, clearances [n] floats
+    HIP_TRY(r->d_mit.ensure(align256((size_t)n) + (size_t)n * sizeof(float)));
+    uint8_t* self = r->d_mit.as<uint8_t>();
+    float* clr = reinterpret_cast<float*>(self + align256((size_t)n));
+    return batch_cc_impl(r, S, d_samples, n, margin, d_cc, d_clearance, clearance_weight, self, clr,
+                         (hipStream_t)stream);
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
+}
+
+ppl_status ppl_mitstar_batch_cc_static(ppl_robot r, const ppl_scene_view* view, const float* d_samples, int n,
+                                       float margin, uint8_t* d_cc, float* d_clearance, float clearance_weight,
+                                       ppl_stream stream) try {
+    CHECK_ARG(r && d_samples && d_cc, "NULL argument");
+    CHECK_ARG(n >= 0, "negative sample count");
+    if (n > PPL_BATCH_CC_STATIC_MAX)
+        return fail(PPL_ERR_CAPACITY, "batch_cc_static: %d samples exceed the fixed scratch of %d", n,
+                    PPL_BATCH_CC_STATIC_MAX);
+    if (n == 0) return PPL_OK;
+    DScene S{};
+    ppl_status st = scene_from_view(view, S);
+    if (st != PPL_OK) return st;
+    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
+    uint8_t* base = nullptr;
+    if ((st = static_scratch(&base)) != PPL_OK) return st;
+    return batch_cc_impl(r, S, d_samples, n, margin, d_cc, d_clearance, clearance_weight, base,
+                         reinterpret_cast<float*>(base + kStaticSelfBytes), (hipStream_t)stream);
 } catch (const std::bad_alloc&) {
     return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
