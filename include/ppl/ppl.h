@@ -270,17 +270,21 @@ ppl_status ppl_hinge_cost_batched(ppl_plan plan, const ppl_scene_view* scene, co
 /* Autotuner of the cost kernel's tile shape: times the plan's summed-hinge
  * evaluation (with the fused adjoint if grad) on the caller's own inputs for
  * every lane-group size G = 1, 2, 4, ..., 16 lanes per state (64 / G states
- * per wavefront, a state's spheres split over G lanes) that fits in LDS, and
- * keeps the fastest for this plan's later hinge / STOMP calls. reps timed
- * runs per candidate on `stream`; outputs go to plan scratch. No counterpart
- * in the reference. */
+ * per wavefront, a state's spheres split over G lanes) that fits in LDS and
+ * for the grouped layout (*best_group = PPL_COST_GROUPED), and keeps the
+ * fastest for this plan's later hinge / STOMP calls. reps timed runs per
+ * candidate on `stream`; outputs go to plan scratch. No counterpart in the
+ * reference. */
 ppl_status ppl_plan_autotune_cost(ppl_plan plan, const ppl_scene_view* scene, const float* d_x,
                                   const ppl_cost_params* params, int grad, int reps, int* best_group,
                                   float* best_ms, ppl_stream stream);
 
-/* Pin the cost kernel's lanes per state for this plan (grad: the adjoint
- * variant); 0 returns to the built-in choice. Power of two <= 16 that fits
- * in LDS. */
+/* Pin the cost kernel's layout for this plan (grad: the adjoint variant):
+ * lanes per state of the lane-group kernel (a power of two <= 16 that fits in
+ * LDS), PPL_COST_GROUPED for the grouped kernel (16 lanes per state, the
+ * state's spheres spread over them across joints, joint frames and wrenches in
+ * LDS), or 0 for the built-in choice (grouped for the adjoint). */
+#define PPL_COST_GROUPED (-1)
 ppl_status ppl_plan_set_cost_group(ppl_plan plan, int grad, int lanes_per_state);
 
 /* Central-FD obstacle gradient over a trajectory plan (chomp_obs_grad_kernel

@@ -15,6 +15,7 @@ import torch  # noqa: F401  (must precede the library load, see above)
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libppl.so"
 
 PPL_OK, PPL_ERR_INVALID, PPL_ERR_HIP, PPL_ERR_ALLOC, PPL_ERR_UNSUPPORTED, PPL_ERR_CAPACITY = range(6)
+PPL_COST_GROUPED = -1   # ppl_plan_set_cost_group: the grouped cost kernel
 PLAN_CONFIGS, PLAN_EDGES, PLAN_TRAJ = 0, 1, 2
 
 _f32p = C.POINTER(C.c_float)

@@ -218,11 +218,12 @@ ppl_status check_obb_capacity(int n) {
     return PPL_OK;
 }
 
-enum KernelKind { kKCollide, kKCost, kKCostGrad };
+enum KernelKind { kKCollide, kKCost, kKCostGrad, kKCostGrp, kKCostGrpGrad };
 
 // Persistent grid for a kernel family at the LDS footprint of this launch.
 int grid_for(KernelKind k, const DRobot& R, int obb_cap, int G) {
     if (k == kKCollide) return grid_collide(lds_bytes(obb_cap, R.n_dof));
+    if (k == kKCostGrp || k == kKCostGrpGrad) return INT_MAX;   // launch_cost_grp caps it per kernel variant
     const bool grad = k == kKCostGrad;
     return grid_cost(R.n_joints, grad, lds_bytes_cost(obb_cap, R.n_dof, R.n_joints, G, grad));
 }
@@ -598,6 +599,14 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     // Parent of each DFS position, as a DFS position (root: itself).
     std::vector<int> close(J, 0);
     for (int k = 0; k < J; ++k) close[k] = k == 0 ? 0 : pos[parents[dfs[k]]];
+    // Subtree ends in DFS order (a subtree is a contiguous run of DFS
+    // positions) and the DFS joint of every emitted sphere.
+    std::vector<int> jend(J);
+    for (int k = 0; k < J; ++k) jend[k] = k + 1;
+    for (int k = J - 1; k >= 1; --k) jend[close[k]] = std::max(jend[close[k]], jend[k]);
+    std::vector<int> sph_jnt(S);
+    for (int k = 0; k < J; ++k)
+        for (int t = sph_off[k]; t < sph_off[k + 1]; ++t) sph_jnt[t] = k;
     // One device blob for every table.
     size_t off = 0;
     auto place = [&](size_t bytes) {
@@ -612,7 +621,7 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
                  o_apairs = place(apairs.size() * 8), o_aorig = place(apx_orig.size() * 16),
                  o_sblk = place(sblk.size() * 16), o_rmask = place(rowmask.size() * 8), o_jorig = place(J * 4),
                  o_lim = place((size_t)D * 8),
-                 o_jenc = place((size_t)J * 16);
+                 o_jenc = place((size_t)J * 16), o_sjnt = place((size_t)S * 4), o_jend = place((size_t)J * 4);
     std::vector<char> host(off + 16, 0);
     auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) memcpy(host.data() + o, src, bytes); };
     put(o_ft, ft.data(), ft.size() * 4);
@@ -635,6 +644,8 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     for (int k = 0; k < D; ++k) lim[k] = make_float2(d->joint_lower[k], d->joint_upper[k]);
     put(o_lim, lim.data(), (size_t)D * 8);
     put(o_jenc, jenc.data(), (size_t)J * 16);
+    put(o_sjnt, sph_jnt.data(), (size_t)S * 4);
+    put(o_jend, jend.data(), (size_t)J * 4);
 
     ppl_robot_s* r = new (std::nothrow) ppl_robot_s();
     if (!r) return fail(PPL_ERR_ALLOC, "out of host memory");
@@ -670,6 +681,8 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     R.jorig = reinterpret_cast<const int*>(b + o_jorig);
     R.lim = reinterpret_cast<const float2*>(b + o_lim);
     R.jenc = reinterpret_cast<const float4*>(b + o_jenc);
+    R.sph_jnt = reinterpret_cast<const int*>(b + o_sjnt);
+    R.jend = reinterpret_cast<const int*>(b + o_jend);
     R.n_sblk = (int)sblk.size() / 2;
     r->n_slots = n_slots;
     r->n_pairs = (int)pairs.size();
@@ -1157,6 +1170,7 @@ ppl_status cost_self(ppl_plan_s* P, const DStates& X, const ppl_cost_params* prm
 
 bool group_fits(const ppl_plan_s* P, int G, bool grad, int obb_cap) {
     const DRobot& R = P->robot->R;
+    if (G == PPL_COST_GROUPED) return lds_bytes_grp(obb_cap, R.n_dof, R.n_joints, grad) <= 65536;
     return lds_bytes_cost(obb_cap, R.n_dof, R.n_joints, G, grad) <= 65536;
 }
 
@@ -1183,16 +1197,22 @@ ppl_status hinge_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_
     a.X = X;
     int grid = 0;
     const int tuned = P->tuned_group[d_grad ? 1 : 0];
-    const int G = force_G > 0 ? force_G : tuned > 0 ? tuned : pick_group(P->n_states, P->robot->R, d_grad != nullptr);
-    HIP_TRY(ensure_table(P, G));
-    a.C = plan_chunks(P, d_grad ? kKCostGrad : kKCost, plan_obb_cap(P, S), G, &grid);
-    a.group = G;
+    // default: the grouped layout for the adjoint, the lane-group heuristic for the cost alone
+    const int G = force_G != 0 ? force_G
+                : tuned != 0   ? tuned
+                : d_grad       ? PPL_COST_GROUPED
+                               : pick_group(P->n_states, P->robot->R, false);
+    const bool grouped = G == PPL_COST_GROUPED;
+    HIP_TRY(ensure_table(P, grouped ? kGrp : G));
+    a.C = plan_chunks(P, grouped ? (d_grad ? kKCostGrpGrad : kKCostGrp) : (d_grad ? kKCostGrad : kKCost),
+                      plan_obb_cap(P, S), grouped ? kGrp : G, &grid);
+    a.group = grouped ? kGrp : G;
     a.P = {prm->world_margin, prm->self_margin, prm->weight, prm->use_obb, prm->use_esdf, prm->use_self};
     a.grad = d_grad != nullptr;
     a.self_in = self_in;
     a.cost_out = state;
     a.grad_out = d_grad;
-    HIP_TRY(launch_cost(a, grid, s));
+    HIP_TRY(grouped ? launch_cost_grp(a, grid, s) : launch_cost(a, grid, s));
     if (d_total && P->kind == PPL_PLAN_TRAJ) HIP_TRY(launch_segment_sum(state, P->d_off, P->n_units, 0.0f, d_total, s));
     return PPL_OK;
 }
@@ -2228,8 +2248,13 @@ ppl_status ppl_plan_autotune_cost(ppl_plan P, const ppl_scene_view* view, const 
     int best = 0;
     float best_t = 0.0f;
     const int cap = plan_obb_cap(P, S);
-    for (int G = 1; G <= kMaxGroup && st == PPL_OK; G <<= 1) {
-        if (!group_fits(P, G, grad != 0, cap)) continue;
+    // candidates: lane groups G = 1 .. 16 of k_cost, then the grouped layout
+    for (int G = 1; G <= 2 * kMaxGroup && st == PPL_OK; G <<= 1) {
+        if (G > kMaxGroup) G = PPL_COST_GROUPED;
+        if (!group_fits(P, G, grad != 0, cap)) {
+            if (G == PPL_COST_GROUPED) break;
+            continue;
+        }
         st = hinge_impl(P, view, d_x, prm, state, total, g, s, G);   // warm-up (builds the chunk table)
         he = hipEventRecord(e0, s);
         for (int k = 0; k < reps && st == PPL_OK && he == hipSuccess; ++k)
@@ -2240,6 +2265,7 @@ ppl_status ppl_plan_autotune_cost(ppl_plan P, const ppl_scene_view* view, const 
         if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
         if (he != hipSuccess) st = fail(PPL_ERR_HIP, "autotune timing: %s", hipGetErrorString(he));
         if (st == PPL_OK && (best == 0 || ms < best_t)) { best = G; best_t = ms; }
+        if (G == PPL_COST_GROUPED) break;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -2254,10 +2280,10 @@ ppl_status ppl_plan_autotune_cost(ppl_plan P, const ppl_scene_view* view, const 
 
 ppl_status ppl_plan_set_cost_group(ppl_plan P, int grad, int lanes_per_state) try {
     CHECK_ARG(P, "NULL argument");
-    CHECK_ARG(lanes_per_state == 0 || (lanes_per_state >= 1 && lanes_per_state <= kMaxGroup &&
-                                       (lanes_per_state & (lanes_per_state - 1)) == 0),
-              "lanes_per_state must be 0 or a power of two in [1, %d]", kMaxGroup);
-    if (lanes_per_state > 0) {
+    CHECK_ARG(lanes_per_state == 0 || lanes_per_state == PPL_COST_GROUPED ||
+                  (lanes_per_state >= 1 && lanes_per_state <= kMaxGroup && (lanes_per_state & (lanes_per_state - 1)) == 0),
+              "lanes_per_state must be 0, PPL_COST_GROUPED or a power of two in [1, %d]", kMaxGroup);
+    if (lanes_per_state != 0) {
         const int cap = P->h_obb_off.empty() ? kMaxObbLds : 0;
         int need = cap;
         for (size_t p = 0; p + 1 < P->h_obb_off.size(); ++p) need = std::max(need, P->h_obb_off[p + 1] - P->h_obb_off[p]);

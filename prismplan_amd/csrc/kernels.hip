@@ -971,6 +971,296 @@ __global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
 }
 
 // ===========================================================================
+// Grouped summed-hinge cost (+ adjoint): one state per aligned group of
+// kGrp = 16 lanes, four states per wave, with the state's spheres spread over
+// the group's lanes across joints instead of joint by joint.
+//  1. Forward kinematics in parallel: lane jj of the group computes joint
+//     jj's step (one sin/cos per lane) and the frames are composed by
+//     pointer jumping over the DFS tree in ceil(log2(depth)) rounds through
+//     LDS. Then every joint's approximate sphere is placed at once and one
+//     wave pass per joint over the OBBs gives its candidate mask (the broad
+//     phase of near_obbs over the wave's four states).
+//  2. Sphere pass: emission slot k (spheres are emitted joint by joint) goes
+//     to lane k mod 16, which reads its joint's frame and mask from LDS,
+//     finds the argmin OBB and the hinge terms. Panda's 59 spheres take four
+//     rounds instead of one round per joint with most lanes idle.
+//  3. Adjoint: every sphere with a non-zero point gradient g adds its force g
+//     and moment (p - o_j) x g about its joint's origin o_j into the joint's
+//     wrench in LDS. Lane jj then folds DFS positions [jj, jend[jj]) -- the
+//     joint's subtree -- into a total wrench about o_jj, with each lever arm
+//     o_i - o_jj taken relative to jj, and reads d cost / d q = z . tau
+//     (revolute) or z . f (prismatic).
+// Results equal k_cost's up to FP32 summation order.
+// ===========================================================================
+static_assert(kGrp >= kMaxJoints, "the grouped kernel gives every joint its own lane of the group");
+
+size_t lds_floats_grp(int obb_cap, int n_dof, int n_joints, bool grad) {
+    size_t f = (size_t)obb_cap * 16 + (size_t)n_joints * kMaskWords * 2 + (size_t)n_dof * kWave + 2 * kWave +
+               (size_t)kGrpStates * n_joints * 13;
+    if (grad) f += (size_t)kGrpStates * ((size_t)n_joints * 6 + kMaxDof + 1);
+    return f;
+}
+
+// WORLD: bit 0 the scene's boxes may enter (OBB hinge / argmin), bit 1 its
+// ESDF layers may; the launch picks the smallest variant the call needs, which
+// keeps the other path's registers out of the sphere pass.
+template <bool GRAD, int WORLD>
+__global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
+    extern __shared__ float4 s_lds[];
+    const DRobot& R = a.R;
+    const int J = R.n_joints, D = R.n_dof, S = R.n_sph;
+    float4* s_obb = s_lds;
+    unsigned long long* s_mask = reinterpret_cast<unsigned long long*>(s_lds + 4 * a.C.obb_cap);
+    float* s_q = reinterpret_cast<float*>(s_mask + (size_t)J * kMaskWords);
+    int* s_end = reinterpret_cast<int*>(s_q + D * kWave);
+    float* s_T = reinterpret_cast<float*>(s_end + 2 * kWave);   // [slot][J][12]
+    int* s_par = reinterpret_cast<int*>(s_T + kGrpStates * J * 12);   // [slot][J] pointer-jumping links
+    float* s_w = reinterpret_cast<float*>(s_par + kGrpStates * J);    // GRAD: [slot][J][6] wrenches
+    float* s_dq = s_w + kGrpStates * J * 6;                       // GRAD: [slot][kMaxDof + 1]
+    const int lane = lane_id();
+    const int sub = lane & (kGrp - 1);
+    const int slot = lane / kGrp;
+    float* Ts = s_T + slot * J * 12;
+    const bool use_obb = (WORLD & 1) && a.P.use_obb != 0 && !a.clearance_only;
+    const bool use_esdf = (WORLD & 2) && (a.P.use_esdf != 0 || a.clearance_only) && a.S.n_voxel_layers > 0;
+    const bool want_grad = GRAD && !a.clearance_only;
+    const float wt = a.P.weight;
+    int cur_prob = -1, n_obb = 0;
+    int4 ch;
+    ChunkCursor cur = chunk_cursor();
+    const long n_states = state_count(a.X);
+    const int n_chunks = chunk_count(a.C, n_states);
+    while (next_chunk(a.C, n_states, n_chunks, cur, ch)) {
+        if ((WORLD & 1) && ch.x != cur_prob) {
+            int o0, o1;
+            problem_obbs(a.C, a.S, ch.x, o0, o1);
+            __syncthreads();
+            n_obb = uniform(stage_obbs(a.S, o0, o1, s_obb, a.C.obb_cap));
+            cur_prob = ch.x;
+        }
+        const long g = (long)ch.y + slot;
+        const bool active = g < ch.z;
+        stage_unit_ends(a.X, ch.w, s_end);
+        if (want_grad)
+            for (int i = lane; i < kGrpStates * (J * 6 + kMaxDof + 1); i += kWave) s_w[i] = 0.0f;
+        __syncthreads();
+        StateRef ref = load_state(a.X, D, g, active, s_q, ch.w, s_end, ch.x);
+        __syncthreads();
+        // 1a. joint frames by pointer jumping over the DFS tree: lane jj of
+        // the group starts from its joint step A = S_jj (the root's is the
+        // identity) and its parent link P; each round composes the frame of
+        // link P onto A and jumps P to P's link, until P is the root. Then A
+        // is the product of the steps from the root down to jj: the frame.
+        // Frames live in LDS between rounds (registers hold one product).
+        {
+            const bool own = sub < J;
+            int P = 0;
+            if (own) {
+                float A[12];
+                if (sub == 0) {
+                    aff_identity(A);
+                } else {
+                    const int4 meta = R.jmeta[sub];
+                    const float q = meta.y >= 0 ? s_q[meta.y * kWave + lane] : 0.0f;
+                    joint_step(R.ft + 12 * sub, meta.x, q, A);
+                    P = R.parent[sub];
+                }
+                float4* d = reinterpret_cast<float4*>(Ts + 12 * sub);
+                d[0] = make_float4(A[0], A[1], A[2], A[3]);
+                d[1] = make_float4(A[4], A[5], A[6], A[7]);
+                d[2] = make_float4(A[8], A[9], A[10], A[11]);
+                s_par[slot * J + sub] = P;
+            }
+            __syncthreads();
+            while (wave_any(own && P != 0)) {
+                const bool jump = own && P != 0;
+                float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0, c2 = c0;
+                int PP = 0;
+                if (jump) {
+                    const float4* b = reinterpret_cast<const float4*>(Ts + 12 * P);
+                    const float4* m = reinterpret_cast<const float4*>(Ts + 12 * sub);
+                    const float4 m0 = m[0], m1 = m[1], m2 = m[2];
+                    auto row = [&](const float4 br) {
+                        return make_float4(fmaf(br.z, m2.x, fmaf(br.y, m1.x, br.x * m0.x)),
+                                           fmaf(br.z, m2.y, fmaf(br.y, m1.y, br.x * m0.y)),
+                                           fmaf(br.z, m2.z, fmaf(br.y, m1.z, br.x * m0.z)),
+                                           fmaf(br.z, m2.w, fmaf(br.y, m1.w, br.x * m0.w)) + br.w);
+                    };
+                    c0 = row(b[0]);
+                    c1 = row(b[1]);
+                    c2 = row(b[2]);
+                    PP = s_par[slot * J + P];
+                }
+                __syncthreads();
+                if (jump) {
+                    float4* d = reinterpret_cast<float4*>(Ts + 12 * sub);
+                    d[0] = c0;
+                    d[1] = c1;
+                    d[2] = c2;
+                    P = PP;
+                    s_par[slot * J + sub] = P;
+                }
+                __syncthreads();
+            }
+        }
+        // 1b. candidate masks per joint (the broad phase of near_obbs, all
+        // joints' approximate centres placed at once): lane jj of each group
+        // places joint jj's approximate sphere; the four states' spread
+        // about state slot 0's centre sets the joint's reach; one pass of 64
+        // lanes over the OBBs per joint gives its mask.
+        if (use_obb) {
+            float ax = 0.0f, ay = 0.0f, az = 0.0f;
+            if (sub < J && active) {
+                const int a0 = R.apx_off[sub];
+                if (R.apx_off[sub + 1] - a0 == 1) {
+                    const float4 as = R.apx[a0];
+                    const float4* t = reinterpret_cast<const float4*>(Ts + 12 * sub);
+                    const float4 t0 = t[0], t1 = t[1], t2 = t[2];
+                    ax = fmaf(t0.x, as.x, fmaf(t0.y, as.y, fmaf(t0.z, as.z, t0.w)));
+                    ay = fmaf(t1.x, as.x, fmaf(t1.y, as.y, fmaf(t1.z, as.z, t1.w)));
+                    az = fmaf(t2.x, as.x, fmaf(t2.y, as.y, fmaf(t2.z, as.z, t2.w)));
+                }
+            }
+            const float cx = __shfl(ax, sub), cy = __shfl(ay, sub), cz = __shfl(az, sub);   // slot 0's centre
+            const float dx = ax - cx, dy = ay - cy, dz = az - cz;
+            float d2 = (sub < J && active) ? fmaf(dz, dz, fmaf(dy, dy, dx * dx)) : 0.0f;
+            d2 = fmaxf(d2, __shfl_xor(d2, 16));
+            d2 = fmaxf(d2, __shfl_xor(d2, 32));
+            for (int jj = 0; jj < J; ++jj) {
+                if (R.sph_off[jj + 1] == R.sph_off[jj]) continue;
+                const int a0 = R.apx_off[jj];
+                ObbMask mk;
+                if (R.apx_off[jj + 1] - a0 != 1) {
+                    mask_all(n_obb, mk);   // no single enclosing sphere: every OBB is a candidate
+                } else {
+                    const float c0x = read_lane(cx, jj), c0y = read_lane(cy, jj), c0z = read_lane(cz, jj);
+                    const float reach = a.P.world_margin + R.apx[a0].w + sqrtf(read_lane(d2, jj)) * 1.0001f + 1e-5f;
+                    const float thr = uniform(reach * fabsf(reach));
+#pragma unroll
+                    for (int k = 0; k < kMaskWords; ++k) {
+                        bool c = false;
+                        if (k * 64 < n_obb) {
+                            const int o = k * 64 + lane;
+                            if (o < n_obb) {
+                                float lx, ly, lz;
+                                to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], c0x, c0y, c0z, lx, ly, lz);
+                                c = box_key(lx, ly, lz, s_obb[4 * o + 3]) < thr;
+                            }
+                        }
+                        mk.w[k] = __ballot(c);
+                    }
+                }
+                if (lane < kMaskWords) {
+                    unsigned long long v = 0ull;
+#pragma unroll
+                    for (int k = 0; k < kMaskWords; ++k) if (k == lane) v = mk.w[k];
+                    s_mask[jj * kMaskWords + lane] = v;
+                }
+            }
+        }
+        float self_h = 0.0f, snx = 0.0f, sny = 0.0f, snz = 0.0f;
+        int si = -1, sj = -1;
+        if (a.self_in && active && !a.clearance_only) {
+            const float* so = a.self_in + 8 * (size_t)ref.gs;
+            self_h = fmaxf(a.P.self_margin - so[0], 0.0f);
+            if (self_h > 0.0f) { si = __float_as_int(so[1]); sj = __float_as_int(so[2]); snx = so[3]; sny = so[4]; snz = so[5]; }
+        }
+        // 2. sphere pass
+        float world = 0.0f, minclr = 1000.0f;
+        if (active) {
+            for (int s = sub; s < S; s += kGrp) {
+                const int jj = R.sph_jnt[s];
+                const float4* Tq = reinterpret_cast<const float4*>(Ts + 12 * jj);
+                const float4 t0 = Tq[0], t1 = Tq[1], t2 = Tq[2];
+                const float T[12] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w, t2.x, t2.y, t2.z, t2.w};
+                const float4 sp = R.sph[s];
+                const int sid = R.sph_id[s];
+                float px, py, pz;
+                aff_point(T, sp.x, sp.y, sp.z, px, py, pz);
+                float best = FLT_MAX;
+                int arg = -1;
+                if (use_obb) {
+                    ObbMask mk;
+#pragma unroll
+                    for (int k = 0; k < kMaskWords; ++k) mk.w[k] = s_mask[jj * kMaskWords + k];
+                    const uint8_t* acm_row = a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)sid * a.S.n_obbs : nullptr;
+                    obb_argmin(s_obb, mk, px, py, pz, acm_row, best, arg);
+                }
+                float gx, gy, gz;
+                const float hs = sphere_hinge_at<GRAD>(a, s_obb, best, arg, px, py, pz, sp.w, sid, use_obb, use_esdf,
+                                                       gx, gy, gz);
+                if (a.clearance_only) minclr = fminf(minclr, hs);
+                else world += hs;
+                if (want_grad) {
+                    if (si >= 0) {
+                        if (sid == si) { gx -= snx; gy -= sny; gz -= snz; }
+                        if (sid == sj) { gx += snx; gy += sny; gz += snz; }
+                    }
+                    if (gx != 0.0f || gy != 0.0f || gz != 0.0f) {
+                        const float rx = px - T[3], ry = py - T[7], rz = pz - T[11];
+                        float* w = s_w + (slot * J + jj) * 6;
+                        atomicAdd(w + 0, gx);
+                        atomicAdd(w + 1, gy);
+                        atomicAdd(w + 2, gz);
+                        atomicAdd(w + 3, ry * gz - rz * gy);
+                        atomicAdd(w + 4, rz * gx - rx * gz);
+                        atomicAdd(w + 5, rx * gy - ry * gx);
+                    }
+                }
+            }
+        }
+        // 3. adjoint: subtree wrench of joint `sub` -> d cost / d q
+        if (want_grad) {
+            __syncthreads();
+            float* dq = s_dq + slot * (kMaxDof + 1);
+            if (active && sub >= 1 && sub < J) {
+                const int jj = sub;
+                const int4 meta = R.jmeta[jj];
+                if (meta.y >= 0 && meta.x >= 0) {
+                    const float* Tj = Ts + 12 * jj;
+                    const float ox = Tj[3], oy = Tj[7], oz = Tj[11];
+                    float fx = 0.0f, fy = 0.0f, fz = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
+                    const int e = R.jend[jj];
+                    for (int i = jj; i < e; ++i) {
+                        const float* w = s_w + (slot * J + i) * 6;
+                        const float* Ti = Ts + 12 * i;
+                        const float lx = Ti[3] - ox, ly = Ti[7] - oy, lz = Ti[11] - oz;
+                        fx += w[0]; fy += w[1]; fz += w[2];
+                        tx += w[3] + (ly * w[2] - lz * w[1]);
+                        ty += w[4] + (lz * w[0] - lx * w[2]);
+                        tz += w[5] + (lx * w[1] - ly * w[0]);
+                    }
+                    const int type = meta.x;
+                    const int ax = type >= kXRot ? type - kXRot : type;
+                    const float zx = Tj[ax], zy = Tj[4 + ax], zz = Tj[8 + ax];
+                    const float v = type >= kXRot ? fmaf(zx, tx, fmaf(zy, ty, zz * tz)) : fmaf(zx, fx, fmaf(zy, fy, zz * fz));
+                    atomicAdd(dq + meta.y, v);
+                }
+            }
+            __syncthreads();
+            if (active && sub < D) {
+                bool endpoint = false;
+                size_t base = (size_t)ref.gs * D, stride = 1;
+                if (a.X.source == kSrcTraj) {
+                    endpoint = (ref.k == 0 || ref.k == ref.n - 1);
+                    base = (size_t)a.X.traj_off[ref.unit] * D + ref.k;
+                    stride = ref.n;
+                }
+                a.grad_out[base + (size_t)sub * stride] = endpoint ? 0.0f : wt * dq[sub];
+            }
+        }
+        if (a.clearance_only) {
+            minclr = group_min(minclr, kGrp);
+            if (active && sub == 0) a.cost_out[ref.gs] = minclr;
+        } else {
+            world = group_sum(world, kGrp);
+            if (active && sub == 0 && a.cost_out) a.cost_out[ref.gs] = wt * (world + self_h);
+        }
+        __syncthreads();   // frames, masks and wrenches of this chunk are dead before the next one
+    }
+}
+
+// ===========================================================================
 // Self collision: 16 lanes per state, sphere positions in LDS.
 // mode 0: any pair with |d|^2 - (ri+rj)^2 < 0 (utils.cuh:239-267,
 //         runtime_kinematics.cuh:248-283);
@@ -1843,6 +2133,49 @@ hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s) {
     const size_t lds = lds_bytes_cost(a.C.obb_cap, a.R.n_dof, a.R.n_joints, a.group, a.grad != 0);
     if (a.grad) hipLaunchKernelGGL(k_cost<true>, dim3(grid), dim3(64), lds, s, a);
     else hipLaunchKernelGGL(k_cost<false>, dim3(grid), dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+
+size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, bool grad) {
+    return lds_floats_grp(obb_cap, n_dof, n_joints, grad) * sizeof(float);
+}
+
+// Variant table: [grad][world - 1].
+const void* grp_kernel(bool grad, int world) {
+    static const void* const tab[2][3] = {
+        {(const void*)k_cost_grp<false, 1>, (const void*)k_cost_grp<false, 2>, (const void*)k_cost_grp<false, 3>},
+        {(const void*)k_cost_grp<true, 1>, (const void*)k_cost_grp<true, 2>, (const void*)k_cost_grp<true, 3>}};
+    return tab[grad ? 1 : 0][world - 1];
+}
+
+int grp_world(const CostArgs& a) {
+    const bool obb = a.P.use_obb != 0 && !a.clearance_only && a.S.n_obbs > 0;
+    const bool esdf = (a.P.use_esdf != 0 || a.clearance_only) && a.S.n_voxel_layers > 0;
+    return obb && esdf ? 3 : esdf ? 2 : 1;
+}
+
+int grid_cost_grp(bool grad, int world, size_t lds) {
+    static size_t last[2][3] = {{(size_t)-1, (size_t)-1, (size_t)-1}, {(size_t)-1, (size_t)-1, (size_t)-1}};
+    static int g[2][3] = {};
+    const int i = grad ? 1 : 0, w = world - 1;
+    if (lds != last[i][w]) {
+        g[i][w] = occupancy_grid(grp_kernel(grad, world), lds);
+        last[i][w] = lds;
+    }
+    return g[i][w];
+}
+
+hipError_t launch_cost_grp(const CostArgs& a, int grid, hipStream_t s) {
+    const size_t lds = lds_bytes_grp(a.C.obb_cap, a.R.n_dof, a.R.n_joints, a.grad != 0);
+    const int world = grp_world(a);
+    grid = std::max(1, std::min(grid, grid_cost_grp(a.grad != 0, world, lds)));   // persistent: occupancy x CUs
+#define PPL_GRP_LAUNCH(G_, W_) hipLaunchKernelGGL((k_cost_grp<G_, W_>), dim3(grid), dim3(64), lds, s, a)
+    if (a.grad) {
+        if (world == 1) PPL_GRP_LAUNCH(true, 1); else if (world == 2) PPL_GRP_LAUNCH(true, 2); else PPL_GRP_LAUNCH(true, 3);
+    } else {
+        if (world == 1) PPL_GRP_LAUNCH(false, 1); else if (world == 2) PPL_GRP_LAUNCH(false, 2); else PPL_GRP_LAUNCH(false, 3);
+    }
+#undef PPL_GRP_LAUNCH
     return hipGetLastError();
 }
 

@@ -29,6 +29,8 @@ constexpr int kWave = 64;
 constexpr int kMaxObbLds = 256;     // OBBs staged per problem
 constexpr int kSelfGroup = 16;      // lanes per state in the self-collision pass
 constexpr int kMaxGroup = 16;       // lanes per state of the cost kernel (power of two)
+constexpr int kGrp = 16;            // lanes per state of the grouped cost kernel (>= kMaxJoints)
+constexpr int kGrpStates = kWave / kGrp;
 constexpr int kGroupKinds = 6;      // chunk tables for 64 / G states, G = 1 .. 32
 
 enum JointType { kFixed = -1, kXPrism = 0, kYPrism = 1, kZPrism = 2, kXRot = 3, kYRot = 4, kZRot = 5 };
@@ -56,6 +58,8 @@ struct DRobot {
     const int2* apx_pairs;      // [n_apx_pairs] approx self pairs (approx original indices)
     const float4* apx_orig;     // [A] approx spheres by original index
     const float4* jenc;         // [J] link-frame sphere containing every sphere of DFS joint k (w < 0: none)
+    const int* sph_jnt;         // [S] DFS joint of emission slot k
+    const int* jend;            // [J] end of DFS position k's subtree: its DFS positions are [k, jend[k])
 };
 
 // Non-owning scene pointers, SceneCollisionData semantics.
@@ -161,6 +165,7 @@ struct SelfArgs {
 
 hipError_t launch_collide(const CollideArgs& a, int grid, hipStream_t s);
 hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s);
+hipError_t launch_cost_grp(const CostArgs& a, int grid, hipStream_t s);
 hipError_t launch_self(const SelfArgs& a, hipStream_t s);
 hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t s);
 // CHOMP optimiser pieces (trajectory plans, dim-major [D][T] blocks).
@@ -235,5 +240,8 @@ __host__ __device__ inline int dq_stride(int n_dof) { return n_dof | 1; }
 int device_cus();
 int grid_collide(size_t lds);
 int grid_cost(int n_joints, bool grad, size_t lds);
+size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, bool grad);
+int grid_cost_grp(bool grad, int world, size_t lds);
+int grp_world(const CostArgs& a);
 
 }  // namespace ppl

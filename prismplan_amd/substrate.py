@@ -263,8 +263,9 @@ class Plan:
 
     def autotune_cost(self, scene: "DeviceScene", x: torch.Tensor, params: _ppl.CostParams, grad=False, reps=5,
                       stream=None):
-        """Time the cost kernel at every lane-group size on these inputs and
-        keep the fastest for this plan; returns (lanes per state, ms per call)."""
+        """Time the cost kernel at every lane-group size and in the grouped
+        layout on these inputs and keep the fastest for this plan; returns
+        (lanes per state or _ppl.PPL_COST_GROUPED, ms per call)."""
         _dev(x)
         g = C.c_int(0)
         ms = C.c_float(0.0)
@@ -274,7 +275,9 @@ class Plan:
         return g.value, ms.value
 
     def set_cost_group(self, lanes_per_state: int, grad=False):
-        """Pin the cost kernel's lanes per state (0: built-in choice)."""
+        """Pin the cost kernel's lanes per state (a power of two <= 16),
+        _ppl.PPL_COST_GROUPED for the grouped kernel, or 0 for the built-in
+        choice (grouped for the adjoint)."""
         call("ppl_plan_set_cost_group", self.handle, int(bool(grad)), int(lanes_per_state))
 
     def close(self):
