@@ -1170,7 +1170,7 @@ ppl_status cost_self(ppl_plan_s* P, const DStates& X, const ppl_cost_params* prm
 
 bool group_fits(const ppl_plan_s* P, int G, bool grad, int obb_cap) {
     const DRobot& R = P->robot->R;
-    if (G == PPL_COST_GROUPED) return lds_bytes_grp(obb_cap, R.n_dof, R.n_joints, grad) <= 65536;
+    if (G == PPL_COST_GROUPED) return lds_bytes_grp(obb_cap, R.n_dof, R.n_joints, R.n_sph, grad) <= 65536;
     return lds_bytes_cost(obb_cap, R.n_dof, R.n_joints, G, grad) <= 65536;
 }
 

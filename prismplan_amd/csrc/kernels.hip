@@ -365,10 +365,13 @@ __device__ __forceinline__ int find_unit(const int* off, int n, long g, int u0, 
 
 // Writes q_d into s_q[d * 64 + lane] for d < D (inactive lanes write nothing).
 // g is the chunk-order index; prob the chunk's problem (waypoint-major plans).
+// qcol / qstride: where q_d goes, s_q[d * qstride + qcol] (default: the
+// lane's own column of a [D][64] array).
 __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, bool active, float* s_q,
-                                               int u0 = -1, const int* s_end = nullptr, int prob = 0) {
+                                               int u0 = -1, const int* s_end = nullptr, int prob = 0,
+                                               int qcol = -1, int qstride = kWave) {
     StateRef ref{0, 0, 0, 0, g};
-    const int lane = lane_id();
+    const int lane = qcol >= 0 ? qcol : lane_id();
     if (!active) return ref;
     if (X.source == kSrcDense) {
         int d0, d1;
@@ -382,13 +385,13 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         ref.u0 = d0;
         const float* th = X.theta + (size_t)t0 * D;
         if (f >= (T - 1) * ns) {
-            for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = th[(size_t)d * T + T - 1];
+            for (int d = 0; d < D; ++d) s_q[d * qstride + lane] = th[(size_t)d * T + T - 1];
         } else {
             const int seg = f / ns, sub = f - seg * ns;
             const float frac = __fdiv_rn((float)sub, (float)ns);
             for (int d = 0; d < D; ++d) {
                 const float x0 = th[(size_t)d * T + seg], x1 = th[(size_t)d * T + seg + 1];
-                s_q[d * kWave + lane] = __fadd_rn(x0, __fmul_rn(__fsub_rn(x1, x0), frac));
+                s_q[d * qstride + lane] = __fadd_rn(x0, __fmul_rn(__fsub_rn(x1, x0), frac));
             }
         }
         return ref;
@@ -411,7 +414,7 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         const float* a = X.cfg + (size_t)X.node_src[e] * X.cfg_stride;
         const float* b = X.cfg + (size_t)X.node_dst[e] * X.cfg_stride;
         for (int d = 0; d < D; ++d)
-            s_q[d * kWave + lane] = __fadd_rn(__fmul_rn(a[d], 1.0f - t), __fmul_rn(b[d], t));
+            s_q[d * qstride + lane] = __fadd_rn(__fmul_rn(a[d], 1.0f - t), __fmul_rn(b[d], t));
         return ref;
     }
     if (X.source == kSrcTraj && X.wmajor) {
@@ -424,14 +427,14 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         ref.u0 = (long)pr.w + (long)r * pr.z;
         ref.gs = ref.u0 + t;
         const float* th = X.theta + (size_t)ref.u0 * D;
-        for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = th[(size_t)d * pr.z + t];
+        for (int d = 0; d < D; ++d) s_q[d * qstride + lane] = th[(size_t)d * pr.z + t];
         return ref;
     }
     if (X.source == kSrcConfig) {
         ref.unit = (int)g;
         ref.u0 = g;
         const float* q = X.cfg + (size_t)g * X.cfg_stride;
-        for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = q[d];
+        for (int d = 0; d < D; ++d) s_q[d * qstride + lane] = q[d];
     } else if (X.source == kSrcEdge) {
         int b0, b1;
         const int e = find_unit(X.edge_off, X.n_edges, g, u0, s_end, &b0, &b1);
@@ -443,7 +446,7 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         const float t = ref.n > 1 ? (float)ref.k / (float)(ref.n - 1) : 0.0f;
         const float* a = X.from + (size_t)e * D;
         const float* b = X.to + (size_t)e * D;
-        for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = fmaf(t, b[d] - a[d], a[d]);
+        for (int d = 0; d < D; ++d) s_q[d * qstride + lane] = fmaf(t, b[d] - a[d], a[d]);
     } else {
         int t0, t1;
         const int b = find_unit(X.traj_off, X.n_traj, g, u0, s_end, &t0, &t1);
@@ -453,7 +456,7 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         ref.k = (int)(g - t0);
         ref.n = Tn;
         const float* th = X.theta + (size_t)t0 * D;
-        for (int d = 0; d < D; ++d) s_q[d * kWave + lane] = th[(size_t)d * Tn + ref.k];
+        for (int d = 0; d < D; ++d) s_q[d * qstride + lane] = th[(size_t)d * Tn + ref.k];
     }
     return ref;
 }
@@ -994,9 +997,10 @@ __global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
 // ===========================================================================
 static_assert(kGrp >= kMaxJoints, "the grouped kernel gives every joint its own lane of the group");
 
-size_t lds_floats_grp(int obb_cap, int n_dof, int n_joints, bool grad) {
-    size_t f = (size_t)obb_cap * 16 + (size_t)n_joints * kMaskWords * 2 + (size_t)n_dof * kWave + 2 * kWave +
-               (size_t)kGrpStates * n_joints * 13;
+size_t lds_floats_grp(int obb_cap, int n_dof, int n_joints, int n_sph, bool grad) {
+    (void)n_sph;
+    size_t f = (size_t)obb_cap * 16 + (size_t)n_joints * kMaskWords * 2 +
+               (((size_t)n_dof * kGrpStates + 3) & ~(size_t)3) + 2 * kWave + (size_t)kGrpStates * (n_joints * 13 + 4);
     if (grad) f += (size_t)kGrpStates * ((size_t)n_joints * 6 + kMaxDof + 1);
     return f;
 }
@@ -1012,15 +1016,18 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
     float4* s_obb = s_lds;
     unsigned long long* s_mask = reinterpret_cast<unsigned long long*>(s_lds + 4 * a.C.obb_cap);
     float* s_q = reinterpret_cast<float*>(s_mask + (size_t)J * kMaskWords);
-    int* s_end = reinterpret_cast<int*>(s_q + D * kWave);
-    float* s_T = reinterpret_cast<float*>(s_end + 2 * kWave);   // [slot][J][12]
-    int* s_par = reinterpret_cast<int*>(s_T + kGrpStates * J * 12);   // [slot][J] pointer-jumping links
+    int* s_end = reinterpret_cast<int*>(s_q + D * kGrpStates);   // s_q: [D][slot]
+    // frames [slot][J][12], slot stride padded by 4 floats so the four
+    // slots' frames of one joint start in different LDS banks
+    const int fstride = J * 12 + 4;
+    float* s_T = reinterpret_cast<float*>(s_end + 2 * kWave);
+    int* s_par = reinterpret_cast<int*>(s_T + kGrpStates * fstride);   // [slot][J] pointer-jumping links
     float* s_w = reinterpret_cast<float*>(s_par + kGrpStates * J);    // GRAD: [slot][J][6] wrenches
     float* s_dq = s_w + kGrpStates * J * 6;                       // GRAD: [slot][kMaxDof + 1]
     const int lane = lane_id();
     const int sub = lane & (kGrp - 1);
     const int slot = lane / kGrp;
-    float* Ts = s_T + slot * J * 12;
+    float* Ts = s_T + slot * fstride;
     const bool use_obb = (WORLD & 1) && a.P.use_obb != 0 && !a.clearance_only;
     const bool use_esdf = (WORLD & 2) && (a.P.use_esdf != 0 || a.clearance_only) && a.S.n_voxel_layers > 0;
     const bool want_grad = GRAD && !a.clearance_only;
@@ -1044,7 +1051,7 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
         if (want_grad)
             for (int i = lane; i < kGrpStates * (J * 6 + kMaxDof + 1); i += kWave) s_w[i] = 0.0f;
         __syncthreads();
-        StateRef ref = load_state(a.X, D, g, active, s_q, ch.w, s_end, ch.x);
+        StateRef ref = load_state(a.X, D, g, active, s_q, ch.w, s_end, ch.x, slot, kGrpStates);
         __syncthreads();
         // 1a. joint frames by pointer jumping over the DFS tree: lane jj of
         // the group starts from its joint step A = S_jj (the root's is the
@@ -1061,7 +1068,7 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
                     aff_identity(A);
                 } else {
                     const int4 meta = R.jmeta[sub];
-                    const float q = meta.y >= 0 ? s_q[meta.y * kWave + lane] : 0.0f;
+                    const float q = meta.y >= 0 ? s_q[meta.y * kGrpStates + slot] : 0.0f;
                     joint_step(R.ft + 12 * sub, meta.x, q, A);
                     P = R.parent[sub];
                 }
@@ -1112,8 +1119,8 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
             float ax = 0.0f, ay = 0.0f, az = 0.0f;
             if (sub < J && active) {
                 const int a0 = R.apx_off[sub];
-                if (R.apx_off[sub + 1] - a0 == 1) {
-                    const float4 as = R.apx[a0];
+                const float4 as = R.apx_off[sub + 1] - a0 == 1 ? R.apx[a0] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+                if (as.w >= 0.0f) {
                     const float4* t = reinterpret_cast<const float4*>(Ts + 12 * sub);
                     const float4 t0 = t[0], t1 = t[1], t2 = t[2];
                     ax = fmaf(t0.x, as.x, fmaf(t0.y, as.y, fmaf(t0.z, as.z, t0.w)));
@@ -1127,10 +1134,12 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
             d2 = fmaxf(d2, __shfl_xor(d2, 16));
             d2 = fmaxf(d2, __shfl_xor(d2, 32));
             for (int jj = 0; jj < J; ++jj) {
-                if (R.sph_off[jj + 1] == R.sph_off[jj]) continue;
                 const int a0 = R.apx_off[jj];
                 ObbMask mk;
-                if (R.apx_off[jj + 1] - a0 != 1) {
+                if (R.sph_off[jj + 1] == R.sph_off[jj]) {
+#pragma unroll
+                    for (int k = 0; k < kMaskWords; ++k) mk.w[k] = 0ull;   // no spheres: no candidates
+                } else if (R.apx_off[jj + 1] - a0 != 1) {
                     mask_all(n_obb, mk);   // no single enclosing sphere: every OBB is a candidate
                 } else {
                     const float c0x = read_lane(cx, jj), c0y = read_lane(cy, jj), c0z = read_lane(cz, jj);
@@ -1165,47 +1174,63 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
             self_h = fmaxf(a.P.self_margin - so[0], 0.0f);
             if (self_h > 0.0f) { si = __float_as_int(so[1]); sj = __float_as_int(so[2]); snx = so[3]; sny = so[4]; snz = so[5]; }
         }
-        // 2. sphere pass
+        // 2. sphere pass, in rounds of 16 emission slots per group: each lane
+        // places its sphere with its joint's frame and walks its joint's
+        // candidate mask for the argmin box.
         float world = 0.0f, minclr = 1000.0f;
-        if (active) {
-            for (int s = sub; s < S; s += kGrp) {
-                const int jj = R.sph_jnt[s];
-                const float4* Tq = reinterpret_cast<const float4*>(Ts + 12 * jj);
-                const float4 t0 = Tq[0], t1 = Tq[1], t2 = Tq[2];
-                const float T[12] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w, t2.x, t2.y, t2.z, t2.w};
-                const float4 sp = R.sph[s];
-                const int sid = R.sph_id[s];
-                float px, py, pz;
-                aff_point(T, sp.x, sp.y, sp.z, px, py, pz);
-                float best = FLT_MAX;
-                int arg = -1;
-                if (use_obb) {
-                    ObbMask mk;
+        for (int r0 = 0; r0 < S; r0 += kGrp) {
+            const int s = r0 + sub;
+            const bool valid = active && s < S;
+            const int sc = valid ? s : r0;
+            const int jj = R.sph_jnt[sc];
+            const float4* Tq = reinterpret_cast<const float4*>(Ts + 12 * jj);
+            const float4 t0 = Tq[0], t1 = Tq[1], t2 = Tq[2];
+            const float4 sp = R.sph[sc];
+            const int sid = R.sph_id[sc];
+            const float px = fmaf(t0.x, sp.x, fmaf(t0.y, sp.y, fmaf(t0.z, sp.z, t0.w)));
+            const float py = fmaf(t1.x, sp.x, fmaf(t1.y, sp.y, fmaf(t1.z, sp.z, t1.w)));
+            const float pz = fmaf(t2.x, sp.x, fmaf(t2.y, sp.y, fmaf(t2.z, sp.z, t2.w)));
+            float best = FLT_MAX;
+            int arg = -1;
+            if (use_obb) {
+                ObbMask own;
 #pragma unroll
-                    for (int k = 0; k < kMaskWords; ++k) mk.w[k] = s_mask[jj * kMaskWords + k];
-                    const uint8_t* acm_row = a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)sid * a.S.n_obbs : nullptr;
-                    obb_argmin(s_obb, mk, px, py, pz, acm_row, best, arg);
+                for (int k = 0; k < kMaskWords; ++k) own.w[k] = s_mask[jj * kMaskWords + k];
+                const uint8_t* acm_row = a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)sid * a.S.n_obbs : nullptr;
+#pragma unroll
+                for (int k = 0; k < kMaskWords; ++k) {
+                    unsigned long long bits = valid ? own.w[k] : 0ull;
+                    while (bits) {
+                        const int o = k * 64 + (__ffsll(bits) - 1);
+                        bits &= bits - 1ull;
+                        if (acm_row && acm_row[__float_as_int(s_obb[4 * o + 3].w)] == 0) continue;
+                        float lx, ly, lz;
+                        to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], px, py, pz, lx, ly, lz);
+                        const float key = box_key(lx, ly, lz, s_obb[4 * o + 3]);
+                        if (key < best) { best = key; arg = o; }
+                    }
                 }
-                float gx, gy, gz;
-                const float hs = sphere_hinge_at<GRAD>(a, s_obb, best, arg, px, py, pz, sp.w, sid, use_obb, use_esdf,
-                                                       gx, gy, gz);
-                if (a.clearance_only) minclr = fminf(minclr, hs);
-                else world += hs;
-                if (want_grad) {
-                    if (si >= 0) {
-                        if (sid == si) { gx -= snx; gy -= sny; gz -= snz; }
-                        if (sid == sj) { gx += snx; gy += sny; gz += snz; }
-                    }
-                    if (gx != 0.0f || gy != 0.0f || gz != 0.0f) {
-                        const float rx = px - T[3], ry = py - T[7], rz = pz - T[11];
-                        float* w = s_w + (slot * J + jj) * 6;
-                        atomicAdd(w + 0, gx);
-                        atomicAdd(w + 1, gy);
-                        atomicAdd(w + 2, gz);
-                        atomicAdd(w + 3, ry * gz - rz * gy);
-                        atomicAdd(w + 4, rz * gx - rx * gz);
-                        atomicAdd(w + 5, rx * gy - ry * gx);
-                    }
+            }
+            if (!valid) continue;
+            float gx, gy, gz;
+            const float hs = sphere_hinge_at<GRAD>(a, s_obb, best, arg, px, py, pz, sp.w, sid, use_obb, use_esdf,
+                                                   gx, gy, gz);
+            if (a.clearance_only) minclr = fminf(minclr, hs);
+            else world += hs;
+            if (want_grad) {
+                if (si >= 0) {
+                    if (sid == si) { gx -= snx; gy -= sny; gz -= snz; }
+                    if (sid == sj) { gx += snx; gy += sny; gz += snz; }
+                }
+                if (gx != 0.0f || gy != 0.0f || gz != 0.0f) {
+                    const float rx = px - t0.w, ry = py - t1.w, rz = pz - t2.w;
+                    float* w = s_w + (slot * J + jj) * 6;
+                    atomicAdd(w + 0, gx);
+                    atomicAdd(w + 1, gy);
+                    atomicAdd(w + 2, gz);
+                    atomicAdd(w + 3, ry * gz - rz * gy);
+                    atomicAdd(w + 4, rz * gx - rx * gz);
+                    atomicAdd(w + 5, rx * gy - ry * gx);
                 }
             }
         }
@@ -2136,8 +2161,8 @@ hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s) {
     return hipGetLastError();
 }
 
-size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, bool grad) {
-    return lds_floats_grp(obb_cap, n_dof, n_joints, grad) * sizeof(float);
+size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, int n_sph, bool grad) {
+    return lds_floats_grp(obb_cap, n_dof, n_joints, n_sph, grad) * sizeof(float);
 }
 
 // Variant table: [grad][world - 1].
@@ -2166,7 +2191,7 @@ int grid_cost_grp(bool grad, int world, size_t lds) {
 }
 
 hipError_t launch_cost_grp(const CostArgs& a, int grid, hipStream_t s) {
-    const size_t lds = lds_bytes_grp(a.C.obb_cap, a.R.n_dof, a.R.n_joints, a.grad != 0);
+    const size_t lds = lds_bytes_grp(a.C.obb_cap, a.R.n_dof, a.R.n_joints, a.R.n_sph, a.grad != 0);
     const int world = grp_world(a);
     grid = std::max(1, std::min(grid, grid_cost_grp(a.grad != 0, world, lds)));   // persistent: occupancy x CUs
 #define PPL_GRP_LAUNCH(G_, W_) hipLaunchKernelGGL((k_cost_grp<G_, W_>), dim3(grid), dim3(64), lds, s, a)
