@@ -989,9 +989,9 @@ __global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
 //     rounds instead of one round per joint with most lanes idle.
 //  3. Adjoint: every sphere with a non-zero point gradient g adds its force g
 //     and moment (p - o_j) x g about its joint's origin o_j into the joint's
-//     wrench in LDS. Lane jj then folds DFS positions [jj, jend[jj]) -- the
-//     joint's subtree -- into a total wrench about o_jj, with each lever arm
-//     o_i - o_jj taken relative to jj, and reads d cost / d q = z . tau
+//     wrench in LDS. A prefix scan over the group's lanes (one joint per lane)
+//     sums each joint's DFS subtree -- a contiguous run of positions -- into a
+//     total wrench about o_jj, and lane jj reads d cost / d q = z . tau
 //     (revolute) or z . f (prismatic).
 // Results equal k_cost's up to FP32 summation order.
 // ===========================================================================
@@ -1197,17 +1197,30 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
 #pragma unroll
                 for (int k = 0; k < kMaskWords; ++k) own.w[k] = s_mask[jj * kMaskWords + k];
                 const uint8_t* acm_row = a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)sid * a.S.n_obbs : nullptr;
+                // two candidates per step: both records are read before
+                // either test, so the LDS latency of one hides the other's
 #pragma unroll
                 for (int k = 0; k < kMaskWords; ++k) {
                     unsigned long long bits = valid ? own.w[k] : 0ull;
                     while (bits) {
-                        const int o = k * 64 + (__ffsll(bits) - 1);
+                        const int oa = k * 64 + (__ffsll(bits) - 1);
                         bits &= bits - 1ull;
-                        if (acm_row && acm_row[__float_as_int(s_obb[4 * o + 3].w)] == 0) continue;
+                        const bool two = bits != 0ull;
+                        const int ob = two ? k * 64 + (__ffsll(bits) - 1) : oa;
+                        if (two) bits &= bits - 1ull;
+                        const float4 a0 = s_obb[4 * oa], a1 = s_obb[4 * oa + 1], a2 = s_obb[4 * oa + 2], ah = s_obb[4 * oa + 3];
+                        const float4 b0 = s_obb[4 * ob], b1 = s_obb[4 * ob + 1], b2 = s_obb[4 * ob + 2], bh = s_obb[4 * ob + 3];
                         float lx, ly, lz;
-                        to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], px, py, pz, lx, ly, lz);
-                        const float key = box_key(lx, ly, lz, s_obb[4 * o + 3]);
-                        if (key < best) { best = key; arg = o; }
+                        to_box(a0, a1, a2, px, py, pz, lx, ly, lz);
+                        float ka = box_key(lx, ly, lz, ah);
+                        to_box(b0, b1, b2, px, py, pz, lx, ly, lz);
+                        float kb = box_key(lx, ly, lz, bh);
+                        if (acm_row) {
+                            if (acm_row[__float_as_int(ah.w)] == 0) ka = FLT_MAX;
+                            if (acm_row[__float_as_int(bh.w)] == 0) kb = FLT_MAX;
+                        }
+                        if (ka < best) { best = ka; arg = oa; }        // oa < ob: ties keep the lower index
+                        if (two && kb < best) { best = kb; arg = ob; }
                     }
                 }
             }
@@ -1238,27 +1251,45 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
         if (want_grad) {
             __syncthreads();
             float* dq = s_dq + slot * (kMaxDof + 1);
+            // Subtree sums by one prefix scan over the group's lanes: lane
+            // i holds joint i's force f_i and its moment about the world
+            // origin m_i = tau_i + o_i x f_i; DFS subtrees are contiguous, so
+            // joint j's subtree force and moment are P(jend[j] - 1) -
+            // P(j - 1), and its moment about o_j is M - o_j x F.
+            float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
+            float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+            if (sub < J) {
+                const float* w = s_w + (slot * J + sub) * 6;
+                const float* Tj = Ts + 12 * sub;
+                ox = Tj[3]; oy = Tj[7]; oz = Tj[11];
+                f0 = w[0]; f1 = w[1]; f2 = w[2];
+                m0 = w[3] + (oy * f2 - oz * f1);
+                m1 = w[4] + (oz * f0 - ox * f2);
+                m2 = w[5] + (ox * f1 - oy * f0);
+            }
+#pragma unroll
+            for (int off = 1; off < kGrp; off <<= 1) {
+                const float u0 = __shfl_up(f0, off, kGrp), u1 = __shfl_up(f1, off, kGrp), u2 = __shfl_up(f2, off, kGrp);
+                const float u3 = __shfl_up(m0, off, kGrp), u4 = __shfl_up(m1, off, kGrp), u5 = __shfl_up(m2, off, kGrp);
+                if (sub >= off) { f0 += u0; f1 += u1; f2 += u2; m0 += u3; m1 += u4; m2 += u5; }
+            }
+            const int gbase = lane - sub;
+            const int e = sub < J ? R.jend[sub] : sub + 1;
+            const int hi = gbase + e - 1, lo = gbase + (sub > 0 ? sub - 1 : 0);
+            float F0 = __shfl(f0, hi), F1 = __shfl(f1, hi), F2 = __shfl(f2, hi);
+            float M0 = __shfl(m0, hi), M1 = __shfl(m1, hi), M2 = __shfl(m2, hi);
+            const float l0 = __shfl(f0, lo), l1 = __shfl(f1, lo), l2 = __shfl(f2, lo);
+            const float l3 = __shfl(m0, lo), l4 = __shfl(m1, lo), l5 = __shfl(m2, lo);
+            if (sub > 0) { F0 -= l0; F1 -= l1; F2 -= l2; M0 -= l3; M1 -= l4; M2 -= l5; }
             if (active && sub >= 1 && sub < J) {
-                const int jj = sub;
-                const int4 meta = R.jmeta[jj];
+                const int4 meta = R.jmeta[sub];
                 if (meta.y >= 0 && meta.x >= 0) {
-                    const float* Tj = Ts + 12 * jj;
-                    const float ox = Tj[3], oy = Tj[7], oz = Tj[11];
-                    float fx = 0.0f, fy = 0.0f, fz = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
-                    const int e = R.jend[jj];
-                    for (int i = jj; i < e; ++i) {
-                        const float* w = s_w + (slot * J + i) * 6;
-                        const float* Ti = Ts + 12 * i;
-                        const float lx = Ti[3] - ox, ly = Ti[7] - oy, lz = Ti[11] - oz;
-                        fx += w[0]; fy += w[1]; fz += w[2];
-                        tx += w[3] + (ly * w[2] - lz * w[1]);
-                        ty += w[4] + (lz * w[0] - lx * w[2]);
-                        tz += w[5] + (lx * w[1] - ly * w[0]);
-                    }
+                    const float* Tj = Ts + 12 * sub;
+                    const float tx = M0 - (oy * F2 - oz * F1), ty = M1 - (oz * F0 - ox * F2), tz = M2 - (ox * F1 - oy * F0);
                     const int type = meta.x;
                     const int ax = type >= kXRot ? type - kXRot : type;
                     const float zx = Tj[ax], zy = Tj[4 + ax], zz = Tj[8 + ax];
-                    const float v = type >= kXRot ? fmaf(zx, tx, fmaf(zy, ty, zz * tz)) : fmaf(zx, fx, fmaf(zy, fy, zz * fz));
+                    const float v = type >= kXRot ? fmaf(zx, tx, fmaf(zy, ty, zz * tz)) : fmaf(zx, F0, fmaf(zy, F1, zz * F2));
                     atomicAdd(dq + meta.y, v);
                 }
             }
