@@ -367,11 +367,14 @@ __device__ __forceinline__ int find_unit(const int* off, int n, long g, int u0, 
 // g is the chunk-order index; prob the chunk's problem (waypoint-major plans).
 // qcol / qstride: where q_d goes, s_q[d * qstride + qcol] (default: the
 // lane's own column of a [D][64] array).
+// store = false resolves the state without writing q (lanes that share a
+// column with a storing lane).
 __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, bool active, float* s_q,
                                                int u0 = -1, const int* s_end = nullptr, int prob = 0,
-                                               int qcol = -1, int qstride = kWave) {
+                                               int qcol = -1, int qstride = kWave, bool store = true) {
     StateRef ref{0, 0, 0, 0, g};
     const int lane = qcol >= 0 ? qcol : lane_id();
+    const int nd = store ? D : 0;   // dimensions to write
     if (!active) return ref;
     if (X.source == kSrcDense) {
         int d0, d1;
@@ -385,11 +388,11 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         ref.u0 = d0;
         const float* th = X.theta + (size_t)t0 * D;
         if (f >= (T - 1) * ns) {
-            for (int d = 0; d < D; ++d) s_q[d * qstride + lane] = th[(size_t)d * T + T - 1];
+            for (int d = 0; d < nd; ++d) s_q[d * qstride + lane] = th[(size_t)d * T + T - 1];
         } else {
             const int seg = f / ns, sub = f - seg * ns;
             const float frac = __fdiv_rn((float)sub, (float)ns);
-            for (int d = 0; d < D; ++d) {
+            for (int d = 0; d < nd; ++d) {
                 const float x0 = th[(size_t)d * T + seg], x1 = th[(size_t)d * T + seg + 1];
                 s_q[d * qstride + lane] = __fadd_rn(x0, __fmul_rn(__fsub_rn(x1, x0), frac));
             }
@@ -413,7 +416,7 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         const float t = __fdiv_rn((float)(2 * (kk - (1 << lvl)) + 1), (float)(1 << (lvl + 1)));
         const float* a = X.cfg + (size_t)X.node_src[e] * X.cfg_stride;
         const float* b = X.cfg + (size_t)X.node_dst[e] * X.cfg_stride;
-        for (int d = 0; d < D; ++d)
+        for (int d = 0; d < nd; ++d)
             s_q[d * qstride + lane] = __fadd_rn(__fmul_rn(a[d], 1.0f - t), __fmul_rn(b[d], t));
         return ref;
     }
@@ -427,14 +430,14 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         ref.u0 = (long)pr.w + (long)r * pr.z;
         ref.gs = ref.u0 + t;
         const float* th = X.theta + (size_t)ref.u0 * D;
-        for (int d = 0; d < D; ++d) s_q[d * qstride + lane] = th[(size_t)d * pr.z + t];
+        for (int d = 0; d < nd; ++d) s_q[d * qstride + lane] = th[(size_t)d * pr.z + t];
         return ref;
     }
     if (X.source == kSrcConfig) {
         ref.unit = (int)g;
         ref.u0 = g;
         const float* q = X.cfg + (size_t)g * X.cfg_stride;
-        for (int d = 0; d < D; ++d) s_q[d * qstride + lane] = q[d];
+        for (int d = 0; d < nd; ++d) s_q[d * qstride + lane] = q[d];
     } else if (X.source == kSrcEdge) {
         int b0, b1;
         const int e = find_unit(X.edge_off, X.n_edges, g, u0, s_end, &b0, &b1);
@@ -446,7 +449,7 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         const float t = ref.n > 1 ? (float)ref.k / (float)(ref.n - 1) : 0.0f;
         const float* a = X.from + (size_t)e * D;
         const float* b = X.to + (size_t)e * D;
-        for (int d = 0; d < D; ++d) s_q[d * qstride + lane] = fmaf(t, b[d] - a[d], a[d]);
+        for (int d = 0; d < nd; ++d) s_q[d * qstride + lane] = fmaf(t, b[d] - a[d], a[d]);
     } else {
         int t0, t1;
         const int b = find_unit(X.traj_off, X.n_traj, g, u0, s_end, &t0, &t1);
@@ -456,7 +459,7 @@ __device__ __forceinline__ StateRef load_state(const DStates& X, int D, long g, 
         ref.k = (int)(g - t0);
         ref.n = Tn;
         const float* th = X.theta + (size_t)t0 * D;
-        for (int d = 0; d < D; ++d) s_q[d * qstride + lane] = th[(size_t)d * Tn + ref.k];
+        for (int d = 0; d < nd; ++d) s_q[d * qstride + lane] = th[(size_t)d * Tn + ref.k];
     }
     return ref;
 }
@@ -1051,7 +1054,7 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
         if (want_grad)
             for (int i = lane; i < kGrpStates * (J * 6 + kMaxDof + 1); i += kWave) s_w[i] = 0.0f;
         __syncthreads();
-        StateRef ref = load_state(a.X, D, g, active, s_q, ch.w, s_end, ch.x, slot, kGrpStates);
+        StateRef ref = load_state(a.X, D, g, active, s_q, ch.w, s_end, ch.x, slot, kGrpStates, sub == 0);
         __syncthreads();
         // 1a. joint frames by pointer jumping over the DFS tree: lane jj of
         // the group starts from its joint step A = S_jj (the root's is the
