@@ -434,6 +434,19 @@ ppl_status ppl_stomp_optimize(ppl_plan plan, const ppl_scene_view* scene, const 
                               float* d_theta, float* d_best, float* d_record, int iterations, int first_call,
                               ppl_stream stream);
 
+/* Whole optimiser iterations as hipGraphs (default on). With it, a
+ * ppl_chomp_optimize call of >= 2 iterations captures one iteration (update,
+ * cost and adjoint, control cost, densified validity, bookkeeping) once and
+ * replays it; ppl_stomp_optimize captures two iterations (reuse, sampling,
+ * rollout costs, update, optimum cost, validity, bookkeeping; one per
+ * keep-buffer parity) once its population has its steady shape, with the
+ * sampling counter's iteration word on the device. A graph is re-captured when
+ * any pointer, scene view field or setting it was captured with changes.
+ * Results equal the eager launches' bit for bit. enable = 0 launches every
+ * kernel eagerly. No counterpart in the reference (chomp.cu:508-555 and
+ * stomp.cu:786-889 read the cost back on the host every iteration). */
+ppl_status ppl_plan_set_optimizer_graphs(ppl_plan plan, int enable);
+
 /* Single-goal STOMP solve, the reference's STOMP::solve_runtime_scene
  * (planning/stomp.cu:565-930) and pybind stomp_solve_scene
  * (python/bindings.cpp:801-831), with the same contract as ppl_chomp_solve. */

@@ -175,6 +175,12 @@ struct ppl_plan_s {
     float ainv_dt = 0.0f;
     DBuf d_opt;
     StompState* stomp = nullptr;
+    // Optimiser iterations captured as hipGraphs (one CHOMP iteration; two
+    // STOMP iterations, one per keep-buffer parity), keyed by their inputs.
+    int opt_graphs = 1;
+    ppl_graph chomp_graph = nullptr, stomp_graph = nullptr;
+    unsigned long long chomp_key = 0, stomp_key = 0;
+    unsigned* d_iter = nullptr;      // STOMP graphs: the iteration word of the sampling counter
 };
 
 struct ppl_graph_s {
@@ -1101,6 +1107,9 @@ ppl_status ppl_plan_destroy(ppl_plan P) try {
     if (P->d_prob) (void)hipFree(P->d_prob);
     for (auto& e : P->ainv) (void)hipFree(e.second);
     if (P->d_ainv_ptr) (void)hipFree(P->d_ainv_ptr);
+    ppl_graph_destroy(P->chomp_graph);
+    ppl_graph_destroy(P->stomp_graph);
+    if (P->d_iter) (void)hipFree(P->d_iter);
     if (StompState* z = P->stomp) {
         ppl_plan_destroy(z->first_plan);
         ppl_plan_destroy(z->fresh_plan);
@@ -1626,6 +1635,57 @@ ppl_status ensure_stomp(ppl_plan_s* P, const ppl_stomp_settings* ss) {
     return PPL_OK;
 }
 
+// Capture the launches `body` issues on a private stream as one graph.
+template <typename F>
+ppl_status capture(F&& body, ppl_graph* out) {
+    CHECK_ARG(out, "NULL argument");
+    *out = nullptr;
+    hipStream_t cs = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    ppl_graph_s* g = new (std::nothrow) ppl_graph_s();
+    if (!g) {
+        (void)hipStreamDestroy(cs);
+        return fail(PPL_ERR_ALLOC, "out of host memory");
+    }
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    ppl_status st = PPL_OK;
+    if (e == hipSuccess) {
+        st = body(cs);
+        hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
+        if (st == PPL_OK && e2 != hipSuccess) st = fail(PPL_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+    } else {
+        st = fail(PPL_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    }
+    if (st == PPL_OK) {
+        e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) st = fail(PPL_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    (void)hipStreamDestroy(cs);
+    if (st != PPL_OK) {
+        ppl_graph_destroy(g);
+        return st;
+    }
+    *out = g;
+    return PPL_OK;
+}
+
+// Key of a captured optimiser iteration: every pointer and value its
+// launches take (FNV-1a over the bytes).
+struct KeyHash {
+    unsigned long long h = 1469598103934665603ull;
+    template <typename T> KeyHash& add(const T& v) {
+        const unsigned char* b = reinterpret_cast<const unsigned char*>(&v);
+        for (size_t i = 0; i < sizeof(T); ++i) h = (h ^ b[i]) * 1099511628211ull;
+        return *this;
+    }
+};
+
+unsigned long long plan_buffers_key(const ppl_plan_s* P, KeyHash k) {
+    k.add(P->d_opt.p).add(P->d_self.p).add(P->d_cost.p).add(P->d_fdcfg.p).add(P->d_fdcost.p).add(P->tuned_group);
+    for (int i = 0; i < kGroupKinds; ++i) k.add(P->d_tab[i]);
+    return k.h;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1677,24 +1737,43 @@ ppl_status ppl_chomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_
                                 first_hit + B, first_hit + 2 * B, first_hit + 3 * B + 1, a, dc)) != PPL_OK)
         return st;
     // cost (+ gradient) of the current theta, its totals and its validity
-    auto evaluate = [&]() -> ppl_status {
-        ppl_status e = cs->fd_gradient ? fd_impl(P, view, d_theta, &cp, cs->fd_eps, grad, s) : PPL_OK;
+    auto evaluate = [&](hipStream_t q) -> ppl_status {
+        ppl_status e = cs->fd_gradient ? fd_impl(P, view, d_theta, &cp, cs->fd_eps, grad, q) : PPL_OK;
         if (e != PPL_OK) return e;
-        e = hinge_impl(P, view, d_theta, &cp, state, total, cs->fd_gradient ? nullptr : grad, s);
+        e = hinge_impl(P, view, d_theta, &cp, state, total, cs->fd_gradient ? nullptr : grad, q);
         if (e != PPL_OK) return e;
-        HIP_TRY(launch_control_cost(d_theta, P->d_off, P->n_units, D, cs->dt, cs->smoothness_weight, total, 1, s));
-        return dense_check_run(r, a, dc, s);
+        HIP_TRY(launch_control_cost(d_theta, P->d_off, P->n_units, D, cs->dt, cs->smoothness_weight, total, 1, q));
+        return dense_check_run(r, a, dc, q);
+    };
+    auto iteration = [&](hipStream_t q) -> ppl_status {
+        HIP_TRY(launch_chomp_update(a, max_t, q));
+        ppl_status e = evaluate(q);
+        if (e != PPL_OK) return e;
+        HIP_TRY(launch_chomp_book(a, 0, q));
+        return PPL_OK;
     };
     // the gradient at the current theta feeds the first step of this call
+    // (this evaluation also sizes every buffer an iteration touches)
     if (first_call || iterations > 0) {
-        if ((st = evaluate()) != PPL_OK) return st;
+        if ((st = evaluate(s)) != PPL_OK) return st;
     }
     if (first_call) HIP_TRY(launch_chomp_book(a, 1, s));
-    for (int it = 0; it < iterations; ++it) {
-        HIP_TRY(launch_chomp_update(a, max_t, s));
-        if ((st = evaluate()) != PPL_OK) return st;
-        HIP_TRY(launch_chomp_book(a, 0, s));
+    if (P->opt_graphs && iterations >= 2) {
+        // one iteration captured once, replayed: no host work between iterations
+        KeyHash k;
+        k.add(d_theta).add(d_best).add(d_record).add(*view).add(*cs).add(max_t);
+        const unsigned long long key = plan_buffers_key(P, k);
+        if (!P->chomp_graph || P->chomp_key != key) {
+            ppl_graph_destroy(P->chomp_graph);
+            P->chomp_graph = nullptr;
+            if ((st = capture(iteration, &P->chomp_graph)) != PPL_OK) return st;
+            P->chomp_key = key;
+        }
+        for (int it = 0; it < iterations; ++it) HIP_TRY(hipGraphLaunch(P->chomp_graph->exec, s));
+        return PPL_OK;
     }
+    for (int it = 0; it < iterations; ++it)
+        if ((st = iteration(s)) != PPL_OK) return st;
     return PPL_OK;
 } catch (const std::bad_alloc&) {
     return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
@@ -1750,18 +1829,18 @@ ppl_status ppl_stomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_
     if ((st = dense_check_setup(P, S, ss->collision_check_step, ss->collision_margin, nsub_max, z->first_hit, z->nsub,
                                 z->dense_off, z->hint.as<int>(), a, dc)) != PPL_OK)
         return st;
-    auto evaluate = [&]() -> ppl_status {
-        ppl_status e = hinge_impl(P, view, d_theta, &cp, z->theta_cost, z->totals, nullptr, s);
+    auto evaluate = [&](hipStream_t q) -> ppl_status {
+        ppl_status e = hinge_impl(P, view, d_theta, &cp, z->theta_cost, z->totals, nullptr, q);
         if (e != PPL_OK) return e;
-        HIP_TRY(launch_control_cost(d_theta, P->d_off, B, D, ss->dt, ss->control_cost_weight, z->totals, 1, s));
-        return dense_check_run(r, a, dc, s);
+        HIP_TRY(launch_control_cost(d_theta, P->d_off, B, D, ss->dt, ss->control_cost_weight, z->totals, 1, q));
+        return dense_check_run(r, a, dc, q);
     };
     if (first_call) {
         z->iter = 0;
         z->parity = 0;
         z->nf_prev = 0;
         z->started = 1;
-        if ((st = evaluate()) != PPL_OK) return st;
+        if ((st = evaluate(s)) != PPL_OK) return st;
         HIP_TRY(launch_chomp_book(a, 1, s));
     }
     StompArgs g{};
@@ -1786,27 +1865,74 @@ ppl_status ppl_stomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_
     g.slot_cost = z->slot_cost;
     g.ctrl_sum = z->ctrl_sum;
     g.sel = z->sel;
-    for (int it = 0; it < iterations; ++it) {
-        const bool first = z->iter == 0;
-        g.iter = z->iter;
-        g.keep = z->keep[z->parity];
-        g.keep_cost = z->keep_cost[z->parity];
+    // One iteration on stream q. iter_dev: the sampling counter's iteration
+    // word comes from device memory plus g.iter (graph replays), else g.iter.
+    auto iteration = [&](hipStream_t q, unsigned iter_word, int parity, int nf_prev, const unsigned* iter_dev) -> ppl_status {
+        const bool first = iter_dev == nullptr && z->iter == 0;
+        g.iter = iter_word;
+        g.iter_dev = iter_dev;
+        g.keep = z->keep[parity];
+        g.keep_cost = z->keep_cost[parity];
         g.cur = StompPop{z->fresh, z->fresh_cost, first ? g.n_new + g.n_old : g.n_new, g.keep, g.keep_cost,
                          first ? 0 : g.n_old, z->slot, z->slot_cost};
-        g.prev = StompPop{z->fresh, z->fresh_cost, z->nf_prev, z->keep[z->parity ^ 1], z->keep_cost[z->parity ^ 1],
-                          z->nf_prev == g.n_new + g.n_old ? 0 : g.n_old, z->slot, z->slot_cost};
-        if (!first) HIP_TRY(launch_stomp_reuse(g, max_t, s));
-        HIP_TRY(launch_stomp_sample(g, max_t, s));
-        if ((st = hinge_impl(first ? z->first_plan : z->fresh_plan, view, z->fresh, &cp, z->fresh_cost, nullptr,
-                             nullptr, s)) != PPL_OK)
-            return st;
-        HIP_TRY(launch_stomp_update(g, max_t, s));
-        if ((st = evaluate()) != PPL_OK) return st;
-        HIP_TRY(launch_chomp_book(a, 0, s));
+        g.prev = StompPop{z->fresh, z->fresh_cost, nf_prev, z->keep[parity ^ 1], z->keep_cost[parity ^ 1],
+                          nf_prev == g.n_new + g.n_old ? 0 : g.n_old, z->slot, z->slot_cost};
+        if (!first) HIP_TRY(launch_stomp_reuse(g, max_t, q));
+        HIP_TRY(launch_stomp_sample(g, max_t, q));
+        ppl_status e = hinge_impl(first ? z->first_plan : z->fresh_plan, view, z->fresh, &cp, z->fresh_cost, nullptr,
+                                  nullptr, q);
+        if (e != PPL_OK) return e;
+        HIP_TRY(launch_stomp_update(g, max_t, q));
+        if ((e = evaluate(q)) != PPL_OK) return e;
+        HIP_TRY(launch_chomp_book(a, 0, q));
+        return PPL_OK;
+    };
+    auto eager = [&]() -> ppl_status {
+        ppl_status e = iteration(s, (unsigned)z->iter, z->parity, z->nf_prev, nullptr);
+        if (e != PPL_OK) return e;
         z->nf_prev = g.cur.nf;
         z->parity ^= 1;
         ++z->iter;
+        return PPL_OK;
+    };
+    int it = 0;
+    // eagerly until the population is in its steady shape: the first
+    // iteration samples new + old rows, the second reuses none of them
+    while (it < iterations && (z->iter == 0 || z->nf_prev != g.n_new)) {
+        if ((st = eager()) != PPL_OK) return st;
+        ++it;
     }
+    if (P->opt_graphs && iterations - it >= 4) {
+        // two iterations (one per keep-buffer parity) captured once and
+        // replayed; the sampling counter's iteration word lives on the device
+        if (!P->d_iter) HIP_TRY(hipMalloc((void**)&P->d_iter, sizeof(unsigned)));
+        KeyHash k;
+        k.add(d_theta).add(d_best).add(d_record).add(*view).add(*ss).add(max_t).add(z->parity).add(z->nf_prev)
+            .add(z->buf.p).add(z->hint.p).add(z->first_plan).add(z->fresh_plan).add(z->d_lt);
+        const unsigned long long key = plan_buffers_key(z->fresh_plan, KeyHash{}.add(plan_buffers_key(P, k)));
+        if (!P->stomp_graph || P->stomp_key != key) {
+            ppl_graph_destroy(P->stomp_graph);
+            P->stomp_graph = nullptr;
+            const int p0 = z->parity, nf = z->nf_prev;
+            st = capture([&](hipStream_t q) -> ppl_status {
+                ppl_status e = iteration(q, 0u, p0, nf, P->d_iter);
+                if (e != PPL_OK) return e;
+                if ((e = iteration(q, 1u, p0 ^ 1, g.n_new, P->d_iter)) != PPL_OK) return e;
+                HIP_TRY(launch_add_u32(P->d_iter, 2u, q));
+                return PPL_OK;
+            }, &P->stomp_graph);
+            if (st != PPL_OK) return st;
+            P->stomp_key = key;
+        }
+        HIP_TRY(launch_fill_i32(reinterpret_cast<int*>(P->d_iter), z->iter, 1, s));
+        for (; it + 2 <= iterations; it += 2) {
+            HIP_TRY(hipGraphLaunch(P->stomp_graph->exec, s));
+            z->nf_prev = g.n_new;
+            z->iter += 2;
+        }
+    }
+    for (; it < iterations; ++it)
+        if ((st = eager()) != PPL_OK) return st;
     return PPL_OK;
 } catch (const std::bad_alloc&) {
     return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
@@ -2278,6 +2404,14 @@ ppl_status ppl_plan_autotune_cost(ppl_plan P, const ppl_scene_view* view, const 
     return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
 }
 
+ppl_status ppl_plan_set_optimizer_graphs(ppl_plan P, int enable) try {
+    CHECK_ARG(P, "NULL argument");
+    P->opt_graphs = enable ? 1 : 0;
+    return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
+}
+
 ppl_status ppl_plan_set_cost_group(ppl_plan P, int grad, int lanes_per_state) try {
     CHECK_ARG(P, "NULL argument");
     CHECK_ARG(lanes_per_state == 0 || lanes_per_state == PPL_COST_GROUPED ||
@@ -2592,40 +2726,6 @@ ppl_status ppl_esdf_from_obbs(const ppl_scene_view* view, const float* h_params,
 // ---------------------------------------------------------------------------
 }  // extern "C"
 
-namespace {
-template <typename F>
-ppl_status capture(F&& body, ppl_graph* out) {
-    CHECK_ARG(out, "NULL argument");
-    *out = nullptr;
-    hipStream_t cs = nullptr;
-    HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-    ppl_graph_s* g = new (std::nothrow) ppl_graph_s();
-    if (!g) {
-        (void)hipStreamDestroy(cs);
-        return fail(PPL_ERR_ALLOC, "out of host memory");
-    }
-    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-    ppl_status st = PPL_OK;
-    if (e == hipSuccess) {
-        st = body(cs);
-        hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
-        if (st == PPL_OK && e2 != hipSuccess) st = fail(PPL_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
-    } else {
-        st = fail(PPL_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
-    }
-    if (st == PPL_OK) {
-        e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
-        if (e != hipSuccess) st = fail(PPL_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-    }
-    (void)hipStreamDestroy(cs);
-    if (st != PPL_OK) {
-        ppl_graph_destroy(g);
-        return st;
-    }
-    *out = g;
-    return PPL_OK;
-}
-}  // namespace
 
 extern "C" {
 

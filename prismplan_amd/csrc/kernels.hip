@@ -1752,7 +1752,8 @@ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, unsigned k0, unsigned k1
 // z ~ N(0, 1) for waypoint t of row (b, r, d): Box-Muller on the Philox words
 // of counter (t >> 2, r | d << 16, b, iter); t % 4 picks the pair and cos / sin.
 __device__ __forceinline__ float stomp_normal(const StompArgs& a, int b, int r, int d, int t) {
-    const uint4 w = philox4x32_10(make_uint4((unsigned)t >> 2, (unsigned)r | ((unsigned)d << 16), (unsigned)b, a.iter),
+    const unsigned it = a.iter_dev ? *a.iter_dev + a.iter : a.iter;
+    const uint4 w = philox4x32_10(make_uint4((unsigned)t >> 2, (unsigned)r | ((unsigned)d << 16), (unsigned)b, it),
                                   a.key0, a.key1);
     const int j = t & 3;
     const unsigned w1 = j < 2 ? w.x : w.z, w2 = j < 2 ? w.y : w.w;
@@ -2038,6 +2039,10 @@ __global__ void k_scatter(const float* v, const int* idx, int base, int n, float
 // ===========================================================================
 // Small helpers
 // ===========================================================================
+__global__ void k_add_u32(unsigned* p, unsigned v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
+}
+
 __global__ void k_fill_i32(int* p, int v, long n) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -2336,6 +2341,11 @@ hipError_t launch_gather_rows(const float* src, int dim, const int* idx, int bas
 hipError_t launch_scatter(const float* v, const int* idx, int base, int n, float* out, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, s, v, idx, base, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_u32(unsigned* p, unsigned v, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, s, p, v);
     return hipGetLastError();
 }
 

@@ -168,6 +168,7 @@ hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s);
 hipError_t launch_cost_grp(const CostArgs& a, int grid, hipStream_t s);
 hipError_t launch_self(const SelfArgs& a, hipStream_t s);
 hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t s);
+hipError_t launch_add_u32(unsigned* p, unsigned v, hipStream_t s);   // *p += v (one thread)
 // CHOMP optimiser pieces (trajectory plans, dim-major [D][T] blocks).
 struct ChompArgs {
     DRobot R; DStates X;        // X: trajectory source over theta
@@ -197,6 +198,7 @@ struct StompArgs {
     int n_new, n_old, n_roll;         // n_roll = n_new + n_old + 1
     float w_u, h, dt;                 // control weight, cost sensitivity, metric step
     unsigned key0, key1, iter;        // Philox key and the iteration word of the counter
+    const unsigned* iter_dev;         // non-null: the iteration word is *iter_dev + iter (graph replays)
     const float* const* Lt;           // [B] sampling factor L^T (T x T, entry k * T + t = L[t][k])
     const float* const* Mt;           // [B] projection M^T (entry u * T + t = M[t][u])
     float* theta; const float* theta_cost; const float* record;

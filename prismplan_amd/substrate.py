@@ -274,6 +274,10 @@ class Plan:
              C.byref(g), C.byref(ms), _stream(stream))
         return g.value, ms.value
 
+    def set_optimizer_graphs(self, enable: bool):
+        """Run optimiser iterations as captured hipGraphs (default) or eagerly."""
+        call("ppl_plan_set_optimizer_graphs", self.handle, int(bool(enable)))
+
     def set_cost_group(self, lanes_per_state: int, grad=False):
         """Pin the cost kernel's lanes per state (a power of two <= 16),
         _ppl.PPL_COST_GROUPED for the grouped kernel, or 0 for the built-in
