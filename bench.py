@@ -19,8 +19,8 @@ timed on this host's cores; parity of the timed outputs vs the FP64 oracle.
 
 --legs picks the extra measurements (skip, any, edges_plain, edges_free, chomp,
 stomp, chomp_opt, stomp_opt, mitstar, rrtc, esdf, copy, cpu_table); the headline always runs. --no-cpu drops the
-CPU baseline legs (for profiler runs, whose JSON line then carries cpu_baseline null). cpu_table (not run by
-default) times the FP32 CPU restatement of the other workloads and checks their GPU outputs against FP64.
+CPU baseline legs (for profiler runs, whose JSON line then carries cpu_baseline null). cpu_table times the FP32
+CPU restatement of the other workloads and checks their GPU outputs against FP64.
 """
 from __future__ import annotations
 
@@ -43,6 +43,16 @@ PEAK_FP32_UNPACKED = 78.6e12  # without packed FP32 (this build disables packed 
 # Mean gated early-exit flops per edges-workload state, as the CPU leg counts
 # them on its 20,000-state sample; used when that leg is switched off.
 GATED_FLOPS_EDGES = 3191.0
+FLOPS_RULE = ("achieved = gated early-exit flops per state x states / kernel time; the gated count is the FP64 "
+              "oracle's mean over a 20,000-state sample of the work its two-phase check does (FK, approximate "
+              "spheres, exact spheres of flagged joints against every OBB; include/ppl/roofline.h item costs)")
+# The headline kernel's instruction counters (rocprofv3 --pmc, one launch;
+# profiles/edges_collide_instruction_mix.txt): VALU instructions and
+# SQ_INSTS_VALU_FLOPS_FP32 per dispatch, with the kernel time of that run.
+COUNTER_VALU_INSTS_EDGES = 15_144_020
+COUNTER_VALU_FLOPS_FP32_EDGES = 7_922_712
+COUNTER_KERNEL_US_EDGES = 44.98
+CUS, SIMDS_PER_CU, CLOCK_HZ = 256, 4, 2.4e9
 # HBM bytes one headline launch moves (k_collide), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE scaled by a 256 MiB copy (profiles/edges_collide_hbm_traffic.txt);
 # the algorithmic bytes are 16,384 edges x 69 B.
@@ -90,7 +100,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
-    ap.add_argument("--legs", default="skip,any,edges_plain,edges_free,chomp,stomp,chomp_opt,stomp_opt,mitstar,rrtc,esdf")
+    ap.add_argument("--legs", default="skip,any,edges_plain,edges_free,chomp,stomp,chomp_opt,stomp_opt,mitstar,rrtc,esdf,"
+                                      "cpu_table")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     legs = set(x for x in args.legs.split(",") if x)
@@ -146,9 +157,15 @@ def main():
                                 f"{ALGORITHMIC_BYTES_EDGES}; achieved HBM GB/s "
                                 f"{TRAFFIC_BYTES_EDGES / ev / 1e9:.1f} of 8000",
                 "frac_of_unpacked_peak": achieved / PEAK_FP32_UNPACKED,
-                "flops_per_state": f_state, "flops_rule": "gated early-exit count (oracle, 20k-state sample)",
+                "flops_per_state": f_state, "flops_rule": FLOPS_RULE,
                 "exhaustive_flops_per_state": f_exh,
-                "exhaustive_frac": f_exh * total_states / ev / PEAK_FP32, "kernel_ms": ev * 1e3}
+                "exhaustive_frac": f_exh * total_states / ev / PEAK_FP32, "kernel_ms": ev * 1e3,
+                # counter-derived, from the committed profile of this kernel: lane flops if every lane of
+                # each FP32 VALU instruction were active (an upper bound), and VALU issue occupancy
+                # (wave64 VALU instructions x 4 cycles over every SIMD's cycles at 2.4 GHz)
+                "counter_fp32_tflops_upper": 64 * COUNTER_VALU_FLOPS_FP32_EDGES / (COUNTER_KERNEL_US_EDGES * 1e-6) / 1e12,
+                "counter_valu_busy": COUNTER_VALU_INSTS_EDGES * 4 / (CUS * SIMDS_PER_CU * COUNTER_KERNEL_US_EDGES * 1e-6
+                                                                     * CLOCK_HZ)}
 
     extras = {}
     if not args.no_extras:
@@ -190,22 +207,26 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline_leg(e, fh_gpu, args):
-    """The only place bench.py touches the oracle: time the FP32 CPU
-    restatement on this host's cores, check the timed GPU output against the
-    FP64 oracle, and count the gated algorithmic work per state."""
+def host_threads():
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    return max(1, min(threads, len(os.sched_getaffinity(0))))
+
+
+def cpu_edges_leg(e, args, n_max=None):
+    """FP32 CPU restatement of the edge-state check timed on this host (all
+    cores, warmed median of 7 on a sample sized to ~cpu_seconds / 4 per run;
+    one core, warmed median of 3 on a smaller sample), and the oracle's gated
+    flops per state on a 20,000-state sample."""
     import numpy as np
     import pyoracle as PO
-    o = PO.Oracle(e.robot, e.scene, nthreads=min(16, os.cpu_count() or 1))
-    n_chk = 2000
-    fh_ref = o.edge_batch(e.frm[:n_chk], e.to[:n_chk], e.n_states[:n_chk])
-    parity = {"edges_checked": n_chk, "first_hit_mismatches": int(np.sum(fh_gpu[:n_chk] != fh_ref))}
-    rng = np.random.default_rng(0)
     states = PO.edge_states(e.frm, e.to, e.n_states)
+    if n_max:
+        states = states[:n_max]
+    rng = np.random.default_rng(0)
+    o = PO.Oracle(e.robot, e.scene, nthreads=host_threads())
     samp = states[rng.choice(len(states), size=min(20000, len(states)), replace=False)]
     f_state = float(np.mean(o.gated_flops(samp)))
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    threads = max(1, min(threads, len(os.sched_getaffinity(0))))
+    threads = host_threads()
     ocpu = PO.Oracle(e.robot, e.scene, nthreads=threads)
     n = 20000
     while True:
@@ -215,22 +236,38 @@ def cpu_baseline_leg(e, fh_gpu, args):
         if dt > args.cpu_seconds / 4 or n >= len(states):
             break
         n = min(len(states), n * 4)
-    reps = min(5, max(1, int(args.cpu_seconds / max(dt, 1e-3))))
-    times = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        ocpu.check_collision_f32_timed(states[:n])
-        times.append(time.perf_counter() - t0)
-    cpu_value = n / float(np.median(times))
-    o1 = PO.Oracle(e.robot, e.scene, nthreads=1)
-    n1 = min(n, 50000)
-    t0 = time.perf_counter()
-    o1.check_collision_f32_timed(states[:n1])
-    cpu_1 = n1 / (time.perf_counter() - t0)
+
+    def median_rate(oo, m, reps):
+        oo.check_collision_f32_timed(states[:m])   # warm
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            oo.check_collision_f32_timed(states[:m])
+            ts.append(time.perf_counter() - t0)
+        return m / float(np.median(ts))
+    cpu_value = median_rate(ocpu, n, 7)
+    n1 = min(n, 20000)
+    cpu_1 = median_rate(PO.Oracle(e.robot, e.scene, nthreads=1), n1, 3)
     log(f"cpu baseline: {cpu_value:.3e} states/s on {threads} threads, {cpu_1:.3e} on 1")
     cpu = {"value": cpu_value, "unit": "configs/s", "cores": threads, "kind": "port",
-           "sample": f"first {n} states of the edges workload, FP32 two-phase check, median of {reps}",
+           "sample": f"first {n} states of the workload, FP32 two-phase check, warmed median of 7 "
+                     f"(one core: first {n1} states, warmed median of 3)",
            "one_core_value": cpu_1, "cpu_model": cpu_model()}
+    return f_state, cpu
+
+
+def cpu_baseline_leg(e, fh_gpu, args):
+    """The only place bench.py touches the oracle (with cpu_edges_leg and
+    cpu_table_leg): time the FP32 CPU restatement on this host's cores, check
+    the timed GPU output against the FP64 oracle, and count the gated
+    algorithmic work per state."""
+    import numpy as np
+    import pyoracle as PO
+    o = PO.Oracle(e.robot, e.scene, nthreads=host_threads())
+    n_chk = 2000
+    fh_ref = o.edge_batch(e.frm[:n_chk], e.to[:n_chk], e.n_states[:n_chk])
+    parity = {"edges_checked": n_chk, "first_hit_mismatches": int(np.sum(fh_gpu[:n_chk] != fh_ref))}
+    f_state, cpu = cpu_edges_leg(e, args)
     return cpu, parity, f_state
 
 
@@ -431,9 +468,22 @@ def bench_edges_free(panda, S, W, torch, dev, args):
                           args.steps, args.warmup, torch)
     _, hits = S.edge_check_batched(plan, ds, frm, to, skip_resolved=False, state_hits=True)
     fh_h = fh.cpu().numpy()
-    return {"states_per_s": e.total_states / wall, "kernel_ms": ev * 1e3, "states": int(e.total_states),
-            "obbs": int(e.scene.n_obbs), "colliding_state_frac": float(hits.float().mean().item()),
-            "colliding_edge_frac": float(np.mean(fh_h >= 0))}
+    out = {"states_per_s": e.total_states / wall, "kernel_ms": ev * 1e3, "states": int(e.total_states),
+           "obbs": int(e.scene.n_obbs), "colliding_state_frac": float(hits.float().mean().item()),
+           "colliding_edge_frac": float(np.mean(fh_h >= 0))}
+    if not args.no_cpu:
+        # the same roofline rule and CPU baseline as the headline, on this world
+        f_state, cpu = cpu_edges_leg(e, args, n_max=200000)
+        from prismplan_amd import _ppl
+        f_exh = panda.flops(_ppl.WORK_ANY_COLLISION, n_obbs=e.scene.n_obbs)
+        achieved = f_state * e.total_states / ev
+        out["roofline"] = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                           "frac": achieved / PEAK_FP32, "frac_of_unpacked_peak": achieved / PEAK_FP32_UNPACKED,
+                           "flops_per_state": f_state, "flops_rule": FLOPS_RULE,
+                           "exhaustive_flops_per_state": f_exh,
+                           "exhaustive_frac": f_exh * e.total_states / ev / PEAK_FP32}
+        out["cpu_baseline"] = cpu
+    return out
 
 
 def bench_edges_plain(panda, e, ds, S, torch, dev, args):
@@ -521,18 +571,26 @@ def bench_chomp_opt(panda, S, W, torch, dev, args):
     st = S.chomp_settings(world_margin=0.05)
     th = torch.from_numpy(w.theta.copy()).to(dev)
     opt = S.ChompOptimizer(plan, ds, th, st).step(0)
-    opt.step(2)
-    torch.cuda.synchronize()
+    opt.step(4)   # captures the iteration graph
     iters = 20
-    t0 = time.perf_counter()
-    opt.step(iters)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    dt = timed_steps(opt, iters, torch)
+    plan.set_optimizer_graphs(False)
+    dt_eager = timed_steps(opt, iters, torch)
+    plan.set_optimizer_graphs(True)
     rec = opt.record.cpu()
-    return {"iterations": iters, "ms_per_iteration": dt / iters * 1e3,
+    return {"iterations": iters, "ms_per_iteration": dt / iters * 1e3, "eager_ms_per_iteration": dt_eager / iters * 1e3,
             "trajectory_iterations_per_s": w.n_traj * iters / dt,
             "solved": int(rec[:, _ppl.CHOMP_SOLVED].sum()), "trajectories": w.n_traj,
             "done": int(rec[:, _ppl.CHOMP_DONE].sum())}
+
+
+def timed_steps(opt, iters, torch):
+    """Wall time of one optimiser call of `iters` iterations, synchronised."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    opt.step(iters)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
 
 
 def bench_stomp_opt(panda, S, W, torch, dev, args):
@@ -549,13 +607,12 @@ def bench_stomp_opt(panda, S, W, torch, dev, args):
     st = S.stomp_settings(world_margin=0.05, iterations_after_valid=1000)
     th = torch.from_numpy(w.theta.copy()).to(dev)
     opt = S.StompOptimizer(plan, ds, th, st).step(0)
-    opt.step(2)
-    torch.cuda.synchronize()
+    opt.step(6)   # steady population; captures the two-iteration graph
     iters = 20
-    t0 = time.perf_counter()
-    opt.step(iters)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    dt = timed_steps(opt, iters, torch)
+    plan.set_optimizer_graphs(False)
+    dt_eager = timed_steps(opt, iters, torch)
+    plan.set_optimizer_graphs(True)
     rec = opt.record.cpu()
     n_fresh = w.n_traj * st.num_rollouts_new * 51
     # one request at a time through the single-goal solve (one trajectory, same settings)
@@ -563,7 +620,8 @@ def bench_stomp_opt(panda, S, W, torch, dev, args):
     start, goal = np.ascontiguousarray(x0[:, 0]), np.ascontiguousarray(x0[:, -1])
     single = S.stomp_solve(panda, ds, start, goal, 51, 10, st)
     single = S.stomp_solve(panda, ds, start, goal, 51, 10, st)
-    return {"iterations": iters, "ms_per_iteration": dt / iters * 1e3, "trajectories": w.n_traj,
+    return {"iterations": iters, "ms_per_iteration": dt / iters * 1e3, "eager_ms_per_iteration": dt_eager / iters * 1e3,
+            "trajectories": w.n_traj,
             "trajectory_iterations_per_s": w.n_traj * iters / dt, "fresh_rollout_states_per_s": n_fresh * iters / dt,
             "solved": int(rec[:, _ppl.CHOMP_SOLVED].sum()), "done": int(rec[:, _ppl.CHOMP_DONE].sum()),
             "single_goal_ms_per_iteration": single["wall_time_ms"] / max(single["iterations_run"], 1),
