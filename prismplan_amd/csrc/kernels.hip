@@ -998,6 +998,69 @@ __global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
 //     (revolute) or z . f (prismatic).
 // Results equal k_cost's up to FP32 summation order.
 // ===========================================================================
+// Joint frames of the state of a 16-lane group (one state per group, a
+// 1-wave workgroup), DFS positions 0..J-1, into F (12 floats each, row-major
+// 3x4), by pointer jumping over the DFS tree: lane jj of the group starts
+// from its joint step A = S_jj (the root's is the identity) and its parent
+// link P; each round composes the frame of link P onto A and jumps P to P's
+// link, until P is the root. Then A is the product of the steps from the root
+// down to jj: the frame. ceil(log2(depth)) rounds through LDS. qs: the
+// state's q (q_d = qs[d * qstride]); par: J ints of LDS for the links.
+// Contains block barriers: every lane of the workgroup must call it.
+__device__ __forceinline__ void group_frames(const DRobot& R, const float* qs, int qstride, float* F, int* par,
+                                             int sub) {
+    const int J = R.n_joints;
+    const bool own = sub < J;
+    int P = 0;
+    if (own) {
+        float A[12];
+        if (sub == 0) {
+            aff_identity(A);
+        } else {
+            const int4 meta = R.jmeta[sub];
+            const float q = meta.y >= 0 ? qs[meta.y * qstride] : 0.0f;
+            joint_step(R.ft + 12 * sub, meta.x, q, A);
+            P = R.parent[sub];
+        }
+        float4* d = reinterpret_cast<float4*>(F + 12 * sub);
+        d[0] = make_float4(A[0], A[1], A[2], A[3]);
+        d[1] = make_float4(A[4], A[5], A[6], A[7]);
+        d[2] = make_float4(A[8], A[9], A[10], A[11]);
+        par[sub] = P;
+    }
+    __syncthreads();
+    while (wave_any(own && P != 0)) {
+        const bool jump = own && P != 0;
+        float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0, c2 = c0;
+        int PP = 0;
+        if (jump) {
+            const float4* b = reinterpret_cast<const float4*>(F + 12 * P);
+            const float4* m = reinterpret_cast<const float4*>(F + 12 * sub);
+            const float4 m0 = m[0], m1 = m[1], m2 = m[2];
+            auto row = [&](const float4 br) {
+                return make_float4(fmaf(br.z, m2.x, fmaf(br.y, m1.x, br.x * m0.x)),
+                                   fmaf(br.z, m2.y, fmaf(br.y, m1.y, br.x * m0.y)),
+                                   fmaf(br.z, m2.z, fmaf(br.y, m1.z, br.x * m0.z)),
+                                   fmaf(br.z, m2.w, fmaf(br.y, m1.w, br.x * m0.w)) + br.w);
+            };
+            c0 = row(b[0]);
+            c1 = row(b[1]);
+            c2 = row(b[2]);
+            PP = par[P];
+        }
+        __syncthreads();
+        if (jump) {
+            float4* d = reinterpret_cast<float4*>(F + 12 * sub);
+            d[0] = c0;
+            d[1] = c1;
+            d[2] = c2;
+            P = PP;
+            par[sub] = P;
+        }
+        __syncthreads();
+    }
+}
+
 static_assert(kGrp >= kMaxJoints, "the grouped kernel gives every joint its own lane of the group");
 
 size_t lds_floats_grp(int obb_cap, int n_dof, int n_joints, int n_sph, bool grad) {
@@ -1056,63 +1119,8 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
         __syncthreads();
         StateRef ref = load_state(a.X, D, g, active, s_q, ch.w, s_end, ch.x, slot, kGrpStates, sub == 0);
         __syncthreads();
-        // 1a. joint frames by pointer jumping over the DFS tree: lane jj of
-        // the group starts from its joint step A = S_jj (the root's is the
-        // identity) and its parent link P; each round composes the frame of
-        // link P onto A and jumps P to P's link, until P is the root. Then A
-        // is the product of the steps from the root down to jj: the frame.
-        // Frames live in LDS between rounds (registers hold one product).
-        {
-            const bool own = sub < J;
-            int P = 0;
-            if (own) {
-                float A[12];
-                if (sub == 0) {
-                    aff_identity(A);
-                } else {
-                    const int4 meta = R.jmeta[sub];
-                    const float q = meta.y >= 0 ? s_q[meta.y * kGrpStates + slot] : 0.0f;
-                    joint_step(R.ft + 12 * sub, meta.x, q, A);
-                    P = R.parent[sub];
-                }
-                float4* d = reinterpret_cast<float4*>(Ts + 12 * sub);
-                d[0] = make_float4(A[0], A[1], A[2], A[3]);
-                d[1] = make_float4(A[4], A[5], A[6], A[7]);
-                d[2] = make_float4(A[8], A[9], A[10], A[11]);
-                s_par[slot * J + sub] = P;
-            }
-            __syncthreads();
-            while (wave_any(own && P != 0)) {
-                const bool jump = own && P != 0;
-                float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0, c2 = c0;
-                int PP = 0;
-                if (jump) {
-                    const float4* b = reinterpret_cast<const float4*>(Ts + 12 * P);
-                    const float4* m = reinterpret_cast<const float4*>(Ts + 12 * sub);
-                    const float4 m0 = m[0], m1 = m[1], m2 = m[2];
-                    auto row = [&](const float4 br) {
-                        return make_float4(fmaf(br.z, m2.x, fmaf(br.y, m1.x, br.x * m0.x)),
-                                           fmaf(br.z, m2.y, fmaf(br.y, m1.y, br.x * m0.y)),
-                                           fmaf(br.z, m2.z, fmaf(br.y, m1.z, br.x * m0.z)),
-                                           fmaf(br.z, m2.w, fmaf(br.y, m1.w, br.x * m0.w)) + br.w);
-                    };
-                    c0 = row(b[0]);
-                    c1 = row(b[1]);
-                    c2 = row(b[2]);
-                    PP = s_par[slot * J + P];
-                }
-                __syncthreads();
-                if (jump) {
-                    float4* d = reinterpret_cast<float4*>(Ts + 12 * sub);
-                    d[0] = c0;
-                    d[1] = c1;
-                    d[2] = c2;
-                    P = PP;
-                    s_par[slot * J + sub] = P;
-                }
-                __syncthreads();
-            }
-        }
+        // 1a. joint frames (pointer jumping over the DFS tree)
+        group_frames(R, s_q + slot, kGrpStates, Ts, s_par + slot * J, sub);
         // 1b. candidate masks per joint (the broad phase of near_obbs, all
         // joints' approximate centres placed at once): lane jj of each group
         // places joint jj's approximate sphere; the four states' spread
@@ -1327,6 +1335,7 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
 //         (runtime_kinematics.cuh:294-329).
 // ===========================================================================
 constexpr int kSelfStates = kWave / kSelfGroup;
+static_assert(kSelfGroup >= kMaxJoints, "the self pass composes frames with one joint per lane of a state group");
 
 __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
     extern __shared__ float4 s_dyn4[];
@@ -1337,8 +1346,11 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
     const int S = R.n_sph, D = R.n_dof, J = R.n_joints;
     float4* s_pos = s_dyn4;                                          // [kSelfStates][S] (x, y, z, r)
     float4* s_jnt = s_dyn4 + kSelfStates * S;                         // [kSelfStates][J] enclosing sphere, r < 0: none
-    float* s_q = reinterpret_cast<float*>(s_jnt + kSelfStates * J);   // [kMaxDof][64]
-    int* s_end = reinterpret_cast<int*>(s_q + kMaxDof * kWave);       // [2][64] unit ends / starts of a chunk
+    const int fstride = J * 12 + 4;                                   // frame block per state, padded
+    float* s_T = reinterpret_cast<float*>(s_jnt + kSelfStates * J);   // [kSelfStates][J][12] joint frames
+    float* s_q = s_T + kSelfStates * fstride;                         // [kMaxDof][kSelfStates]
+    int* s_par = reinterpret_cast<int*>(s_q + kMaxDof * kSelfStates); // [kSelfStates][J] pointer-jumping links
+    int* s_end = s_par + kSelfStates * J;                             // [2][64] unit ends / starts of a chunk
     const long n_states = state_count(a.X);
     // With the plan's chunk table a chunk names its first unit (staged unit
     // ends make the state -> unit lookup an LDS bisection) or, waypoint-major,
@@ -1377,33 +1389,34 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
         if (active && a.unit_first_hit[u] >= 0) active = false;
         if (!wave_any(active)) continue;
     }
-    // Every lane of a state group loads the same q (lane-indexed LDS column).
-    const StateRef ref = load_state(a.X, D, active ? g : base, true, s_q, u0, s_end, prob);
+    // Every lane of a state group resolves the state; lane 0 stores its q.
+    const StateRef ref = load_state(a.X, D, active ? g : base, true, s_q, u0, s_end, prob, st, kSelfStates, sub == 0);
     if (a.unit_first_hit && !by_unit) {
         if (active && a.unit_first_hit[ref.unit] >= 0) active = false;
         if (!wave_any(active)) continue;
     }
     __syncthreads();
-    Slots sl;
     float4* pos = s_pos + st * S;
     float4* jnt = s_jnt + st * J;
-    for (int jj = 0; jj < J; ++jj) {
-        float T[12];
-        int4 meta;
-        fk_joint(R, jj, s_q + lane, sl, T, meta);
-        const int s0 = R.sph_off[jj], s1 = R.sph_off[jj + 1];
-        for (int s = s0 + sub; s < s1; s += kSelfGroup) {
-            const float4 sp = R.sph[s];
-            float x, y, z;
-            aff_point(T, sp.x, sp.y, sp.z, x, y, z);
-            pos[s] = make_float4(x, y, z, sp.w);   // emission slot: a joint's spheres are contiguous
-        }
-        if (sub == 0) {
-            const float4 je = R.jenc[jj];
-            float4 e = make_float4(0.0f, 0.0f, 0.0f, je.w);
-            aff_point(T, je.x, je.y, je.z, e.x, e.y, e.z);
-            jnt[jj] = e;
-        }
+    float* Ts = s_T + st * fstride;
+    group_frames(R, s_q + st, kSelfStates, Ts, s_par + st * J, sub);
+    // sphere centres (emission slot k: a joint's spheres are contiguous) and
+    // each joint's enclosing sphere, from the frames
+    for (int k = sub; k < S; k += kSelfGroup) {
+        const float4 sp = R.sph[k];
+        const float4* Tq = reinterpret_cast<const float4*>(Ts + 12 * R.sph_jnt[k]);
+        const float4 t0 = Tq[0], t1 = Tq[1], t2 = Tq[2];
+        pos[k] = make_float4(fmaf(t0.x, sp.x, fmaf(t0.y, sp.y, fmaf(t0.z, sp.z, t0.w))),
+                             fmaf(t1.x, sp.x, fmaf(t1.y, sp.y, fmaf(t1.z, sp.z, t1.w))),
+                             fmaf(t2.x, sp.x, fmaf(t2.y, sp.y, fmaf(t2.z, sp.z, t2.w))), sp.w);
+    }
+    if (sub < J) {
+        const float4 je = R.jenc[sub];
+        const float4* Tq = reinterpret_cast<const float4*>(Ts + 12 * sub);
+        const float4 t0 = Tq[0], t1 = Tq[1], t2 = Tq[2];
+        jnt[sub] = make_float4(fmaf(t0.x, je.x, fmaf(t0.y, je.y, fmaf(t0.z, je.z, t0.w))),
+                               fmaf(t1.x, je.x, fmaf(t1.y, je.y, fmaf(t1.z, je.z, t1.w))),
+                               fmaf(t2.x, je.x, fmaf(t2.y, je.y, fmaf(t2.z, je.z, t2.w))), je.w);
     }
     __syncthreads();
     // Pairs come in blocks: rows = up to 64 spheres of joint A, columns = up
@@ -2250,7 +2263,9 @@ hipError_t launch_self(const SelfArgs& a, hipStream_t s) {
     if (a.X.n_dyn && blocks > 16384) blocks = 16384;
     if (blocks <= 0) return hipSuccess;
     const size_t smem = sizeof(float4) * (size_t)kSelfStates * (a.R.n_sph + a.R.n_joints) +
-                        sizeof(float) * (size_t)kMaxDof * kWave + sizeof(int) * 2 * kWave;
+                        sizeof(float) * (size_t)kSelfStates * (a.R.n_joints * 12 + 4) +
+                        sizeof(float) * (size_t)kMaxDof * kSelfStates + sizeof(int) * (size_t)kSelfStates * a.R.n_joints +
+                        sizeof(int) * 2 * kWave;
     hipLaunchKernelGGL(k_self, dim3((unsigned)blocks), dim3(64), smem, s, a);
     return hipGetLastError();
 }
