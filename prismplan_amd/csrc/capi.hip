@@ -218,13 +218,11 @@ ppl_status scene_from_view(const ppl_scene_view* v, DScene& S) {
     return PPL_OK;
 }
 
-ppl_status check_obb_capacity(int n) {
-    if (n > kMaxObbLds)
-        return fail(PPL_ERR_CAPACITY, "%d OBBs in one problem exceeds the staged limit of %d", n, kMaxObbLds);
-    return PPL_OK;
-}
-
 enum KernelKind { kKCollide, kKCost, kKCostGrad, kKCostGrp, kKCostGrpGrad };
+
+// LDS OBB records of a launch whose largest problem has max_obbs OBBs: all of
+// them, or one tile of kMaxObbLds when they do not fit (tiled launch).
+int obb_tile(int max_obbs) { return std::max(1, std::min(max_obbs, kMaxObbLds)); }
 
 // Persistent grid for a kernel family at the LDS footprint of this launch.
 int grid_for(KernelKind k, const DRobot& R, int obb_cap, int G) {
@@ -234,16 +232,30 @@ int grid_for(KernelKind k, const DRobot& R, int obb_cap, int G) {
     return grid_cost(R.n_joints, grad, lds_bytes_cost(obb_cap, R.n_dof, R.n_joints, G, grad));
 }
 
-DChunks implicit_chunks(long n_states, KernelKind k, const DRobot& R, int obb_cap, int* grid_out, int G = 1) {
-    const int grid_cap = grid_for(k, R, obb_cap, G);
+// max_obbs: OBBs of the largest problem of the launch.
+DChunks implicit_chunks(long n_states, KernelKind k, const DRobot& R, int max_obbs, int* grid_out, int G = 1) {
+    const int grid_cap = grid_for(k, R, obb_tile(max_obbs), G);
     DChunks C{};
-    C.obb_cap = obb_cap;
+    C.obb_cap = obb_tile(max_obbs);
+    C.tiled = max_obbs > kMaxObbLds;
     C.tab = nullptr;
     C.chunk = kWave / G;
     C.n_chunks = (int)((n_states + C.chunk - 1) / C.chunk);
     C.obb_off = nullptr;
     *grid_out = std::max(1, std::min(grid_cap, C.n_chunks));
     return C;
+}
+
+// The cost kernel over n_states implicit chunks: k_cost with one lane per
+// state when the scene's OBBs fit one LDS staging, else the grouped kernel,
+// which streams them through LDS in tiles.
+hipError_t launch_cost_flat(CostArgs& a, long n_states, hipStream_t s) {
+    int grid = 0;
+    const bool tiled = a.S.n_obbs > kMaxObbLds;
+    const KernelKind k = tiled ? (a.grad ? kKCostGrpGrad : kKCostGrp) : (a.grad ? kKCostGrad : kKCost);
+    a.C = implicit_chunks(n_states, k, a.R, a.S.n_obbs, &grid, tiled ? kGrp : 1);
+    a.group = tiled ? kGrp : 1;
+    return tiled ? launch_cost_grp(a, grid, s) : launch_cost(a, grid, s);
 }
 
 int group_index(int G) { return G >= 32 ? 5 : G >= 16 ? 4 : G >= 8 ? 3 : G >= 4 ? 2 : G >= 2 ? 1 : 0; }
@@ -271,11 +283,12 @@ hipError_t ensure_table(ppl_plan_s* P, int G) {
     return e;
 }
 
-DChunks plan_chunks(const ppl_plan_s* P, KernelKind k, int obb_cap, int G, int* grid_out) {
+DChunks plan_chunks(const ppl_plan_s* P, KernelKind k, int max_obbs, int G, int* grid_out) {
     const int gi = group_index(G);
-    const int grid_cap = grid_for(k, P->robot->R, obb_cap, G);
+    const int grid_cap = grid_for(k, P->robot->R, obb_tile(max_obbs), G);
     DChunks C{};
-    C.obb_cap = obb_cap;
+    C.obb_cap = obb_tile(max_obbs);
+    C.tiled = max_obbs > kMaxObbLds;
     C.tab = P->d_tab[gi];
     C.n_chunks = P->n_chunks[gi];
     C.chunk = kWave / G;
@@ -378,7 +391,7 @@ ppl_status fail_msg(ppl_status s, const char* msg) {
 ppl_status scene_view_check(const ppl_scene_view* v) {
     DScene S{};
     ppl_status st = scene_from_view(v, S);
-    return st != PPL_OK ? st : check_obb_capacity(S.n_obbs);
+    return st;
 }
 bool robot_self_gate_implied(ppl_robot r) { return r && r->self_gate_implied; }
 }  // namespace ppl
@@ -787,7 +800,6 @@ ppl_status ppl_check_collision_scene(ppl_robot r, const ppl_scene_view* view, co
     DScene S{};
     ppl_status st = scene_from_view(view, S);
     if (st != PPL_OK) return st;
-    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
     const int D = r->R.n_dof;
     const size_t cfg_bytes = (size_t)n * D * sizeof(float);
     HIP_TRY(r->d_cfg.ensure(cfg_bytes));
@@ -827,7 +839,6 @@ ppl_status ppl_edge_eval_create(ppl_robot r, const ppl_scene_view* view, int dim
     DScene S{};
     ppl_status st = scene_from_view(view, S);
     if (st != PPL_OK) return st;
-    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
     ppl_edge_eval_s* ev = new (std::nothrow) ppl_edge_eval_s();
     if (!ev) return fail(PPL_ERR_ALLOC, "out of host memory");
     ev->robot = r;
@@ -960,14 +971,11 @@ ppl_status ppl_edge_eval_evaluate(ppl_edge_eval ev, const float* h_from, const f
         c.R = r->R;
         c.S = ev->S;
         c.X = X;
-        int grid = 0;
-        c.C = implicit_chunks(total, kKCost, r->R, ev->S.n_obbs, &grid);
-        c.group = 1;
         c.P.weight = 1.0f;
         c.clearance_only = 1;
         c.margin = p->collision_margin;
         c.cost_out = ev->d_clr_state.as<float>();
-        HIP_TRY(launch_cost(c, grid, ev->stream));
+        HIP_TRY(launch_cost_flat(c, total, ev->stream));
         hipLaunchKernelGGL(k_segment_min, dim3(n), dim3(64), 0, ev->stream, ev->d_clr_state.as<float>(), d_off, n,
                            ev->d_clr_edge.as<float>());
         HIP_TRY(hipGetLastError());
@@ -1045,9 +1053,9 @@ ppl_status ppl_plan_create(ppl_robot r, const ppl_plan_desc* d, ppl_plan* out) t
         P->h_obb_off.assign(d->obb_offsets, d->obb_offsets + d->n_problems + 1);
         for (int p = 0; p < d->n_problems; ++p) {
             const int n = P->h_obb_off[p + 1] - P->h_obb_off[p];
-            if (n < 0 || n > kMaxObbLds) {
+            if (n < 0) {
                 delete P;
-                return fail(n < 0 ? PPL_ERR_INVALID : PPL_ERR_CAPACITY, "problem %d has %d OBBs (limit %d)", p, n, kMaxObbLds);
+                return fail(PPL_ERR_INVALID, "problem %d has a negative OBB range", p);
             }
         }
     }
@@ -1160,7 +1168,7 @@ int plan_obb_cap(const ppl_plan_s* P, const DScene& S) {
 ppl_status plan_scene(const ppl_plan_s* P, const ppl_scene_view* view, DScene& S) {
     ppl_status st = scene_from_view(view, S);
     if (st != PPL_OK) return st;
-    if (P->h_obb_off.empty()) return check_obb_capacity(S.n_obbs);
+    if (P->h_obb_off.empty()) return PPL_OK;
     if (P->h_obb_off.back() > S.n_obbs)
         return fail(PPL_ERR_INVALID, "plan OBB offsets reach %d but the scene has %d OBBs", P->h_obb_off.back(), S.n_obbs);
     return PPL_OK;
@@ -1177,10 +1185,14 @@ ppl_status cost_self(ppl_plan_s* P, const DStates& X, const ppl_cost_params* prm
     return PPL_OK;
 }
 
-bool group_fits(const ppl_plan_s* P, int G, bool grad, int obb_cap) {
+// Whether lane group G (or the grouped layout) fits LDS for problems of up to
+// max_obbs OBBs; only the grouped layout takes problems beyond one staging.
+bool group_fits(const ppl_plan_s* P, int G, bool grad, int max_obbs) {
     const DRobot& R = P->robot->R;
-    if (G == PPL_COST_GROUPED) return lds_bytes_grp(obb_cap, R.n_dof, R.n_joints, R.n_sph, grad) <= 65536;
-    return lds_bytes_cost(obb_cap, R.n_dof, R.n_joints, G, grad) <= 65536;
+    const bool tiled = max_obbs > kMaxObbLds;
+    if (G == PPL_COST_GROUPED)
+        return lds_bytes_grp(obb_tile(max_obbs), R.n_dof, R.n_joints, R.n_sph, grad, tiled) <= 65536;
+    return !tiled && lds_bytes_cost(max_obbs, R.n_dof, R.n_joints, G, grad) <= 65536;
 }
 
 // force_G > 0 overrides the plan's lane-group choice (autotuning).
@@ -1207,14 +1219,17 @@ ppl_status hinge_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_
     int grid = 0;
     const int tuned = P->tuned_group[d_grad ? 1 : 0];
     // default: the grouped layout for the adjoint, the lane-group heuristic for the cost alone
-    const int G = force_G != 0 ? force_G
-                : tuned != 0   ? tuned
-                : d_grad       ? PPL_COST_GROUPED
-                               : pick_group(P->n_states, P->robot->R, false);
+    // problems with more OBBs than one LDS staging take the grouped kernel (OBB tiles)
+    const int max_obbs = plan_obb_cap(P, S);
+    const int G = max_obbs > kMaxObbLds ? PPL_COST_GROUPED
+                : force_G != 0          ? force_G
+                : tuned != 0            ? tuned
+                : d_grad                ? PPL_COST_GROUPED
+                                        : pick_group(P->n_states, P->robot->R, false);
     const bool grouped = G == PPL_COST_GROUPED;
     HIP_TRY(ensure_table(P, grouped ? kGrp : G));
     a.C = plan_chunks(P, grouped ? (d_grad ? kKCostGrpGrad : kKCostGrp) : (d_grad ? kKCostGrad : kKCost),
-                      plan_obb_cap(P, S), grouped ? kGrp : G, &grid);
+                      max_obbs, grouped ? kGrp : G, &grid);
     a.group = grouped ? kGrp : G;
     a.P = {prm->world_margin, prm->self_margin, prm->weight, prm->use_obb, prm->use_esdf, prm->use_self};
     a.grad = d_grad != nullptr;
@@ -1299,13 +1314,10 @@ ppl_status fd_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_the
     a.R = P->robot->R;
     a.S = S;
     a.X = X;
-    int grid = 0;
-    a.C = implicit_chunks(n_cfg, kKCost, a.R, S.n_obbs, &grid);
-    a.group = 1;
     a.P = {prm->world_margin, prm->self_margin, 1.0f, prm->use_obb, prm->use_esdf, prm->use_self};
     a.self_in = self_in;
     a.cost_out = P->d_fdcost.as<float>();
-    HIP_TRY(launch_cost(a, grid, s));
+    HIP_TRY(launch_cost_flat(a, n_cfg, s));
     {
         const long th = n_wp * D;
         hipLaunchKernelGGL(k_fd_combine, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, s, P->d_fdcost.as<float>(),
@@ -2034,13 +2046,16 @@ ppl_status ppl_min_clearance_batched(ppl_plan P, const ppl_scene_view* view, con
     a.S = S;
     a.X = plan_states(P, d_cfg);
     int grid = 0;
-    a.C = plan_chunks(P, kKCost, plan_obb_cap(P, S), 1, &grid);
-    a.group = 1;
+    const int max_obbs = plan_obb_cap(P, S);
+    const bool tiled = max_obbs > kMaxObbLds;   // the grouped kernel streams the OBBs in tiles
+    HIP_TRY(ensure_table(P, tiled ? kGrp : 1));
+    a.C = plan_chunks(P, tiled ? kKCostGrp : kKCost, max_obbs, tiled ? kGrp : 1, &grid);
+    a.group = tiled ? kGrp : 1;
     a.P.weight = 1.0f;
     a.clearance_only = 1;
     a.margin = margin;
     a.cost_out = d_out;
-    HIP_TRY(launch_cost(a, grid, (hipStream_t)stream));
+    HIP_TRY(tiled ? launch_cost_grp(a, grid, (hipStream_t)stream) : launch_cost(a, grid, (hipStream_t)stream));
     return PPL_OK;
 } catch (const std::bad_alloc&) {
     return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
@@ -2061,14 +2076,11 @@ ppl_status configs_clearance(const ppl_robot_s* r, const DScene& S, const float*
     c.X.n_states = n;
     c.X.cfg = d_cfg;
     c.X.cfg_stride = r->R.n_dof;
-    int grid = 0;
-    c.C = implicit_chunks(n, kKCost, r->R, S.n_obbs, &grid, 1);
-    c.group = 1;
     c.P.weight = 1.0f;
     c.clearance_only = 1;
     c.margin = margin;
     c.cost_out = d_out;
-    HIP_TRY(launch_cost(c, grid, s));
+    HIP_TRY(launch_cost_flat(c, n, s));
     return PPL_OK;
 }
 
@@ -2107,7 +2119,6 @@ ppl_status ppl_min_clearance_scene(ppl_robot r, const ppl_scene_view* view, cons
     DScene S{};
     ppl_status st = scene_from_view(view, S);
     if (st != PPL_OK) return st;
-    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
     float* d_cfg = nullptr;
     if ((st = stage_configs(r, h_configs, n, &d_cfg)) != PPL_OK) return st;
     HIP_TRY(r->d_out.ensure((size_t)n * sizeof(float)));
@@ -2126,7 +2137,6 @@ ppl_status ppl_hinge_cost_scene(ppl_robot r, const ppl_scene_view* view, const f
     DScene S{};
     ppl_status st = scene_from_view(view, S);
     if (st != PPL_OK) return st;
-    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
     float* d_cfg = nullptr;
     if ((st = stage_configs(r, h_configs, n, &d_cfg)) != PPL_OK) return st;
     CostArgs c{};
@@ -2142,12 +2152,9 @@ ppl_status ppl_hinge_cost_scene(ppl_robot r, const ppl_scene_view* view, const f
         c.self_in = r->d_self.as<float>();
     }
     HIP_TRY(r->d_out.ensure((size_t)n * sizeof(float)));
-    int grid = 0;
-    c.C = implicit_chunks(n, kKCost, r->R, S.n_obbs, &grid, 1);
-    c.group = 1;
     c.P = {prm->world_margin, prm->self_margin, prm->weight, prm->use_obb, prm->use_esdf, prm->use_self};
     c.cost_out = r->d_out.as<float>();
-    HIP_TRY(launch_cost(c, grid, r->stream));
+    HIP_TRY(launch_cost_flat(c, n, r->stream));
     return fetch_floats(r, r->d_out.as<float>(), n, h_cost);
 } catch (const std::bad_alloc&) {
     return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
@@ -2226,7 +2233,6 @@ ppl_status ppl_mitstar_batch_cc(ppl_robot r, const ppl_scene_view* view, const f
     DScene S{};
     ppl_status st = scene_from_view(view, S);
     if (st != PPL_OK) return st;
-    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
     // scratch: self verdicts [n] bytes, clearances [n] floats
     HIP_TRY(r->d_mit.ensure(align256((size_t)n) + (size_t)n * sizeof(float)));
     uint8_t* self = r->d_mit.as<uint8_t>();
@@ -2249,7 +2255,6 @@ ppl_status ppl_mitstar_batch_cc_static(ppl_robot r, const ppl_scene_view* view, 
     DScene S{};
     ppl_status st = scene_from_view(view, S);
     if (st != PPL_OK) return st;
-    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
     uint8_t* base = nullptr;
     if ((st = static_scratch(&base)) != PPL_OK) return st;
     return batch_cc_impl(r, S, d_samples, n, margin, d_cc, d_clearance, clearance_weight, base,
@@ -2267,7 +2272,6 @@ ppl_status ppl_mitstar_node_clearance(ppl_robot r, const ppl_scene_view* view, c
     DScene S{};
     ppl_status st = scene_from_view(view, S);
     if (st != PPL_OK) return st;
-    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
     hipStream_t s = (hipStream_t)stream;
     const int D = r->R.n_dof;
     const size_t cfg_bytes = align256((size_t)n_nodes * D * sizeof(float));
@@ -2294,7 +2298,6 @@ ppl_status ppl_mitstar_edge_cc(ppl_robot r, const ppl_scene_view* view, const fl
     DScene S{};
     ppl_status st = scene_from_view(view, S);
     if (st != PPL_OK) return st;
-    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
     hipStream_t s = (hipStream_t)stream;
     const size_t E = (size_t)n_edges;
     // scratch: first midpoint [E], count [E], offsets [E + 1], first hit [E], 64-bit total
@@ -2421,7 +2424,8 @@ ppl_status ppl_plan_set_cost_group(ppl_plan P, int grad, int lanes_per_state) tr
         const int cap = P->h_obb_off.empty() ? kMaxObbLds : 0;
         int need = cap;
         for (size_t p = 0; p + 1 < P->h_obb_off.size(); ++p) need = std::max(need, P->h_obb_off[p + 1] - P->h_obb_off[p]);
-        CHECK_ARG(group_fits(P, lanes_per_state, grad != 0, need), "tile does not fit in LDS");
+        // problems beyond one OBB staging take the grouped layout whatever is set here
+        CHECK_ARG(group_fits(P, lanes_per_state, grad != 0, std::min(need, kMaxObbLds)), "tile does not fit in LDS");
     }
     P->tuned_group[grad ? 1 : 0] = lanes_per_state;
     return PPL_OK;
@@ -2713,7 +2717,6 @@ ppl_status ppl_esdf_from_obbs(const ppl_scene_view* view, const float* h_params,
     DScene S{};
     ppl_status st = scene_from_view(view, S);
     if (st != PPL_OK) return st;
-    if ((st = check_obb_capacity(S.n_obbs)) != PPL_OK) return st;
     CHECK_ARG(h_params[3] > 0.0f, "voxel size must be positive");
     HIP_TRY(launch_esdf_build(S.obb_dims, S.obb_pose, S.n_obbs, h_params, h_pose, d_features, (hipStream_t)stream));
     return PPL_OK;

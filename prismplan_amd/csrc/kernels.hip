@@ -126,6 +126,27 @@ __device__ __forceinline__ void fk_joint(const DRobot& R, int jj, const float* s
 // ---------------------------------------------------------------------------
 // OBB staging: enabled OBBs of [o0, o1) -> LDS records (see ppl_internal.hpp)
 // ---------------------------------------------------------------------------
+// Record of OBB o from its stored inverse pose and full extents.
+__device__ __forceinline__ void obb_record(const float4& d4, const float4& p0, const float4& p1, int o, float4* rec) {
+    const float w = p0.w, x = p1.x, y = p1.y, z = p1.z;
+    // Matrix of v -> v + w t + u x t, t = 2 u x v (scene_collision.cuh:36-45).
+    const float r00 = 1.0f - 2.0f * (y * y + z * z), r01 = 2.0f * (x * y - w * z), r02 = 2.0f * (x * z + w * y);
+    const float r10 = 2.0f * (x * y + w * z), r11 = 1.0f - 2.0f * (x * x + z * z), r12 = 2.0f * (y * z - w * x);
+    const float r20 = 2.0f * (x * z - w * y), r21 = 2.0f * (y * z + w * x), r22 = 1.0f - 2.0f * (x * x + y * y);
+    rec[0] = make_float4(r00, r01, r02, p0.x);
+    rec[1] = make_float4(r10, r11, r12, p0.y);
+    rec[2] = make_float4(r20, r21, r22, p0.z);
+    rec[3] = make_float4(0.5f * d4.x, 0.5f * d4.y, 0.5f * d4.z, __int_as_float(o));
+}
+
+__device__ __forceinline__ void obb_record(const DScene& S, int o, float4* rec) {
+    obb_record(*reinterpret_cast<const float4*>(S.obb_dims + 4 * (size_t)o),
+               *reinterpret_cast<const float4*>(S.obb_pose + 8 * (size_t)o),
+               *reinterpret_cast<const float4*>(S.obb_pose + 8 * (size_t)o + 4), o, rec);
+}
+
+// Stages the enabled OBBs of [o0, o1) (at most cap of them: callers pass
+// ranges of at most cap OBBs) and returns their count.
 __device__ int stage_obbs(const DScene& S, int o0, int o1, float4* s_obb, int cap) {
     const int lane = lane_id();
     int n = 0;
@@ -143,24 +164,18 @@ __device__ int stage_obbs(const DScene& S, int o0, int o1, float4* s_obb, int ca
         const unsigned long long m = __ballot(ok);
         if (ok) {
             const int pos = n + __popcll(m & ((1ull << lane) - 1ull));
-            if (pos < cap) {
-                const float d[3] = {d4.x, d4.y, d4.z};
-                const float p[3] = {p0.x, p0.y, p0.z};
-                const float w = p0.w, x = p1.x, y = p1.y, z = p1.z;
-                // Matrix of v -> v + w t + u x t, t = 2 u x v (scene_collision.cuh:36-45).
-                const float r00 = 1.0f - 2.0f * (y * y + z * z), r01 = 2.0f * (x * y - w * z), r02 = 2.0f * (x * z + w * y);
-                const float r10 = 2.0f * (x * y + w * z), r11 = 1.0f - 2.0f * (x * x + z * z), r12 = 2.0f * (y * z - w * x);
-                const float r20 = 2.0f * (x * z - w * y), r21 = 2.0f * (y * z + w * x), r22 = 1.0f - 2.0f * (x * x + y * y);
-                s_obb[4 * pos + 0] = make_float4(r00, r01, r02, p[0]);
-                s_obb[4 * pos + 1] = make_float4(r10, r11, r12, p[1]);
-                s_obb[4 * pos + 2] = make_float4(r20, r21, r22, p[2]);
-                s_obb[4 * pos + 3] = make_float4(0.5f * d[0], 0.5f * d[1], 0.5f * d[2], __int_as_float(o));
-            }
+            if (pos < cap) obb_record(d4, p0, p1, o, s_obb + 4 * pos);
         }
         n += __popcll(m);
     }
     __syncthreads();
     return n < cap ? n : cap;
+}
+
+// OBB tiles of a problem's range [o0, o1): ranges of at most `cap` OBBs
+// (cap >= 1), one LDS staging each.
+__device__ __forceinline__ int obb_tiles(int o0, int o1, int cap) {
+    return o1 - o0 <= cap ? 1 : (o1 - o0 + cap - 1) / cap;
 }
 
 __device__ __forceinline__ void to_box(const float4& r0, const float4& r1, const float4& r2, float px, float py, float pz,
@@ -555,17 +570,20 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
     int* s_end = reinterpret_cast<int*>(s_q + D * kWave);
     const bool has_esdf = a.S.n_voxel_layers > 0;
     const uint8_t* acm = a.S.sphere_obb_acm;
-    int cur_prob = -1, n_obb = 0;
+    const int cap = a.C.obb_cap;
+    int cur_prob = -1, n_obb = 0, o0 = 0, o1 = 0, n_tiles = 1;
     int4 ch;
     ChunkCursor cur = chunk_cursor();
     const long n_states = state_count(a.X);
     const int n_chunks = chunk_count(a.C, n_states);
     while (next_chunk(a.C, n_states, n_chunks, cur, ch)) {
         if (ch.x != cur_prob) {
-            int o0, o1;
             problem_obbs(a.C, a.S, ch.x, o0, o1);
-            __syncthreads();
-            n_obb = uniform(stage_obbs(a.S, o0, o1, s_obb, a.C.obb_cap));
+            n_tiles = uniform(obb_tiles(o0, o1, cap));
+            if (n_tiles == 1) {
+                __syncthreads();
+                n_obb = uniform(stage_obbs(a.S, o0, o1, s_obb, cap));
+            }
             cur_prob = ch.x;
         }
         const long g = (long)ch.y + lane;
@@ -580,114 +598,132 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
             if (fh < (unsigned)ref.k) active = false;
         }
         __syncthreads();
-        Slots sl;
-        for (int jj = 0; jj < J && wave_any(active); ++jj) {
-            float T[12];
-            int4 meta;
-            fk_joint(R, jj, s_q + lane, sl, T, meta);
-            const int s0 = R.sph_off[jj], s1 = R.sph_off[jj + 1];
-            const int a0 = R.apx_off[jj], a1 = R.apx_off[jj + 1];
-            if (s1 == s0 || a1 == a0) continue;
-            // Approximate spheres of this joint in world coordinates (usually one).
-            float apx[3], apr2;
-            const float4 as0 = R.apx[a0];
-            aff_point(T, as0.x, as0.y, as0.z, apx[0], apx[1], apx[2]);
-            {
-                const float rr = as0.w + a.margin;
-                apr2 = rr * rr;
+        // A problem with more OBBs than one staging holds is checked tile by
+        // tile (FK redone per tile); bit jj of `flagged`: joint jj's
+        // approximate spheres hit an OBB of an earlier tile, which the ESDF
+        // pass of the last tile counts as the joint's flag.
+        unsigned flagged = 0u;
+        for (int t = 0; t < n_tiles && wave_any(active); ++t) {
+            const bool last = t == n_tiles - 1;
+            if (n_tiles > 1) {
+                const int t0 = o0 + t * cap;
+                __syncthreads();
+                n_obb = uniform(stage_obbs(a.S, t0, min(o1, t0 + cap), s_obb, cap));
             }
-            // Broad phase: the OBBs some active lane's approximate sphere can
-            // reach. The active lanes' centres lie within rw of the first
-            // active lane's centre c0, so every OBB an approximate sphere
-            // overlaps lies within rw + r of c0 (slack covers rounding).
-            const bool broad = a1 - a0 == 1;
-            float cx = 0.0f, cy = 0.0f, cz = 0.0f, reach2 = 0.0f;
-            if (broad) {
-                const int l0 = __ffsll(__ballot(active)) - 1;
-                cx = read_lane(apx[0], l0); cy = read_lane(apx[1], l0); cz = read_lane(apx[2], l0);
-                const float dx = apx[0] - cx, dy = apx[1] - cy, dz = apx[2] - cz;
-                const float d2 = wave_max(active ? fmaf(dz, dz, fmaf(dy, dy, dx * dx)) : 0.0f);
-                const float reach = sqrtf(d2) * 1.0001f + fabsf(as0.w + a.margin) + 1e-5f;
-                reach2 = uniform(reach * reach);
-            }
-            bool flag = false;
-            bool done = false;
-            for (int base = 0; base < n_obb && !done; base += kWave) {
-                // candidates of this block of 64 OBBs: lane i tests OBB base + i
-                unsigned long long bits;
+            Slots sl;
+            for (int jj = 0; jj < J && wave_any(active); ++jj) {
+                float T[12];
+                int4 meta;
+                fk_joint(R, jj, s_q + lane, sl, T, meta);
+                const int s0 = R.sph_off[jj], s1 = R.sph_off[jj + 1];
+                const int a0 = R.apx_off[jj], a1 = R.apx_off[jj + 1];
+                if (s1 == s0 || a1 == a0) continue;
+                // Approximate spheres of this joint in world coordinates (usually one).
+                float apx[3], apr2;
+                const float4 as0 = R.apx[a0];
+                aff_point(T, as0.x, as0.y, as0.z, apx[0], apx[1], apx[2]);
                 {
-                    const int o = base + lane;
-                    bool c = o < n_obb;
-                    if (broad && c) {
-                        float lx, ly, lz;
-                        to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], cx, cy, cz, lx, ly, lz);
-                        c = box_hit(lx, ly, lz, s_obb[4 * o + 3], reach2);
-                    }
-                    bits = __ballot(c);
+                    const float rr = as0.w + a.margin;
+                    apr2 = rr * rr;
                 }
-                while (bits && !done) {
-                    const int o = base + (__ffsll(bits) - 1);
-                    bits &= bits - 1ull;
-                    const float4 r0 = s_obb[4 * o], r1 = s_obb[4 * o + 1], r2 = s_obb[4 * o + 2], h = s_obb[4 * o + 3];
-                    float lx, ly, lz;
-                    to_box(r0, r1, r2, apx[0], apx[1], apx[2], lx, ly, lz);
-                    bool ah = box_hit(lx, ly, lz, h, apr2);
-                    for (int ai = a0 + 1; ai < a1; ++ai) {
+                // Broad phase: the OBBs some active lane's approximate sphere can
+                // reach. The active lanes' centres lie within rw of the first
+                // active lane's centre c0, so every OBB an approximate sphere
+                // overlaps lies within rw + r of c0 (slack covers rounding).
+                const bool broad = a1 - a0 == 1;
+                float cx = 0.0f, cy = 0.0f, cz = 0.0f, reach2 = 0.0f;
+                if (broad) {
+                    const int l0 = __ffsll(__ballot(active)) - 1;
+                    cx = read_lane(apx[0], l0); cy = read_lane(apx[1], l0); cz = read_lane(apx[2], l0);
+                    const float dx = apx[0] - cx, dy = apx[1] - cy, dz = apx[2] - cz;
+                    const float d2 = wave_max(active ? fmaf(dz, dz, fmaf(dy, dy, dx * dx)) : 0.0f);
+                    const float reach = sqrtf(d2) * 1.0001f + fabsf(as0.w + a.margin) + 1e-5f;
+                    reach2 = uniform(reach * reach);
+                }
+                bool flag = false;
+                bool done = false;
+                for (int base = 0; base < n_obb && !done; base += kWave) {
+                    // candidates of this block of 64 OBBs: lane i tests OBB base + i
+                    unsigned long long bits;
+                    {
+                        const int o = base + lane;
+                        bool c = o < n_obb;
+                        if (broad && c) {
+                            float lx, ly, lz;
+                            to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], cx, cy, cz, lx, ly, lz);
+                            c = box_hit(lx, ly, lz, s_obb[4 * o + 3], reach2);
+                        }
+                        bits = __ballot(c);
+                    }
+                    while (bits && !done) {
+                        const int o = base + (__ffsll(bits) - 1);
+                        bits &= bits - 1ull;
+                        const float4 r0 = s_obb[4 * o], r1 = s_obb[4 * o + 1], r2 = s_obb[4 * o + 2], h = s_obb[4 * o + 3];
+                        float lx, ly, lz;
+                        to_box(r0, r1, r2, apx[0], apx[1], apx[2], lx, ly, lz);
+                        bool ah = box_hit(lx, ly, lz, h, apr2);
+                        for (int ai = a0 + 1; ai < a1; ++ai) {
+                            const float4 as = R.apx[ai];
+                            float x, y, z;
+                            aff_point(T, as.x, as.y, as.z, x, y, z);
+                            to_box(r0, r1, r2, x, y, z, lx, ly, lz);
+                            const float rr = as.w + a.margin;
+                            ah = ah || box_hit(lx, ly, lz, h, rr * rr);
+                        }
+                        flag = flag || ah;
+                        if (!wave_any(ah && active)) continue;
+                        // M = R_o * T_j: sphere-local -> box-local in one affine map.
+                        float M[12];
+    #pragma unroll
+                        for (int rr = 0; rr < 3; ++rr) {
+                            const float4 row = rr == 0 ? r0 : (rr == 1 ? r1 : r2);
+    #pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                float v = fmaf(row.z, T[8 + c], fmaf(row.y, T[4 + c], row.x * T[c]));
+                                if (c == 3) v += row.w;
+                                M[rr * 4 + c] = v;
+                            }
+                        }
+                        const int o_orig = __float_as_int(h.w);
+                        bool h_any = false;
+                        for (int s = s0; s < s1; ++s) {
+                            const float4 sp = R.sph[s];
+                            float x, y, z;
+                            aff_point(M, sp.x, sp.y, sp.z, x, y, z);
+                            const float rr = sp.w + a.margin;
+                            bool hs = box_hit(x, y, z, h, rr * rr);
+                            if (acm) hs = hs && acm[(size_t)R.sph_id[s] * a.S.n_obbs + o_orig] != 0;
+                            h_any = h_any || hs;
+                        }
+                        hit = hit || (active && ah && h_any);
+                        active = active && !hit;
+                        done = !wave_any(active);
+                    }
+                }
+                if (!last) {
+                    if (flag) flagged |= 1u << jj;
+                    continue;
+                }
+                flag = flag || ((flagged >> jj) & 1u) != 0u;
+                if (has_esdf) {
+                    // Approx ESDF pass (ACM disabled, scene_collision.cuh:574-582),
+                    // then the joint's exact spheres when it is flagged.
+                    for (int ai = a0; ai < a1; ++ai) {
                         const float4 as = R.apx[ai];
                         float x, y, z;
                         aff_point(T, as.x, as.y, as.z, x, y, z);
-                        to_box(r0, r1, r2, x, y, z, lx, ly, lz);
-                        const float rr = as.w + a.margin;
-                        ah = ah || box_hit(lx, ly, lz, h, rr * rr);
+                        flag = flag || (active && esdf_hit(a.S, x, y, z, as.w + a.margin, -1));
                     }
-                    flag = flag || ah;
-                    if (!wave_any(ah && active)) continue;
-                    // M = R_o * T_j: sphere-local -> box-local in one affine map.
-                    float M[12];
-#pragma unroll
-                    for (int rr = 0; rr < 3; ++rr) {
-                        const float4 row = rr == 0 ? r0 : (rr == 1 ? r1 : r2);
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            float v = fmaf(row.z, T[8 + c], fmaf(row.y, T[4 + c], row.x * T[c]));
-                            if (c == 3) v += row.w;
-                            M[rr * 4 + c] = v;
+                    if (wave_any(flag && active)) {
+                        for (int s = s0; s < s1; ++s) {
+                            const float4 sp = R.sph[s];
+                            float x, y, z;
+                            aff_point(T, sp.x, sp.y, sp.z, x, y, z);
+                            const bool hs = flag && active && esdf_hit(a.S, x, y, z, sp.w + a.margin, R.sph_id[s]);
+                            hit = hit || hs;
                         }
+                        active = active && !hit;
                     }
-                    const int o_orig = __float_as_int(h.w);
-                    bool h_any = false;
-                    for (int s = s0; s < s1; ++s) {
-                        const float4 sp = R.sph[s];
-                        float x, y, z;
-                        aff_point(M, sp.x, sp.y, sp.z, x, y, z);
-                        const float rr = sp.w + a.margin;
-                        bool hs = box_hit(x, y, z, h, rr * rr);
-                        if (acm) hs = hs && acm[(size_t)R.sph_id[s] * a.S.n_obbs + o_orig] != 0;
-                        h_any = h_any || hs;
-                    }
-                    hit = hit || (active && ah && h_any);
-                    active = active && !hit;
-                    done = !wave_any(active);
-                }
-            }
-            if (has_esdf) {
-                // Approx ESDF pass (ACM disabled, scene_collision.cuh:574-582),
-                // then the joint's exact spheres when it is flagged.
-                for (int ai = a0; ai < a1; ++ai) {
-                    const float4 as = R.apx[ai];
-                    float x, y, z;
-                    aff_point(T, as.x, as.y, as.z, x, y, z);
-                    flag = flag || (active && esdf_hit(a.S, x, y, z, as.w + a.margin, -1));
-                }
-                if (wave_any(flag && active)) {
-                    for (int s = s0; s < s1; ++s) {
-                        const float4 sp = R.sph[s];
-                        float x, y, z;
-                        aff_point(T, sp.x, sp.y, sp.z, x, y, z);
-                        const bool hs = flag && active && esdf_hit(a.S, x, y, z, sp.w + a.margin, R.sph_id[s]);
-                        hit = hit || hs;
-                    }
-                    active = active && !hit;
                 }
             }
         }
@@ -779,23 +815,26 @@ __device__ __forceinline__ void obb_argmin(const float4* __restrict__ s_obb, con
 }
 
 // Hinge terms of one sphere at world position p given its OBB argmin (best
-// key, arg): the summed hinge (OBB + ESDF) and its point gradient d hinge / d p
-// in g*; in clearance-only mode the ESDF clearance with radius + margin.
+// key; rec: the argmin OBB's record, NULL when there is none): the summed
+// hinge (OBB + ESDF) and its point gradient d hinge / d p in g*; in
+// clearance-only mode the ESDF clearance with radius + margin.
 template <bool GRAD>
-__device__ __forceinline__ float sphere_hinge_at(const CostArgs& a, const float4* __restrict__ s_obb, float best,
-                                                 int arg, float px, float py, float pz, float r, int sph_id,
-                                                 bool use_obb, bool use_esdf, float& gx, float& gy, float& gz) {
+__device__ __forceinline__ float sphere_hinge_at(const CostArgs& a, const float4* rec, float best, float px, float py,
+                                                 float pz, float r, int sph_id, bool use_obb, bool use_esdf, float& gx,
+                                                 float& gy, float& gz) {
     const float m = a.P.world_margin;
     float world = 0.0f;
     gx = gy = gz = 0.0f;
+    // clearance mode is ESDF-only; no layer gives sphere_esdf_clearance's 1000
+    if (a.clearance_only && !use_esdf) return 1000.0f;
     if (use_obb) {
-        const float clr = arg >= 0 ? key_to_sdf(best) - r : 1000.0f;
+        const float clr = rec ? key_to_sdf(best) - r : 1000.0f;
         const float hh = m - clr;
         if (hh > 0.0f) {
             world += hh;
             if (GRAD) {
                 float dx, dy, dz;
-                box_sdf_grad(s_obb[4 * arg], s_obb[4 * arg + 1], s_obb[4 * arg + 2], s_obb[4 * arg + 3], px, py, pz, dx, dy, dz);
+                box_sdf_grad(rec[0], rec[1], rec[2], rec[3], px, py, pz, dx, dy, dz);
                 gx -= dx; gy -= dy; gz -= dz;
             }
         }
@@ -898,7 +937,8 @@ __global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
                         a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)R.sph_id[s] * a.S.n_obbs : nullptr;
                     if (use_obb) obb_argmin(s_obb, mk, px, py, pz, acm_row, best, arg);
                     float gx, gy, gz;
-                    const float hs = sphere_hinge_at<GRAD>(a, s_obb, best, arg, px, py, pz, sp.w, R.sph_id[s], use_obb,
+                    const float hs = sphere_hinge_at<GRAD>(a, arg >= 0 ? s_obb + 4 * arg : nullptr, best, px, py, pz,
+                                                           sp.w, R.sph_id[s], use_obb,
                                                            use_esdf, gx, gy, gz);
                     if (a.clearance_only) minclr = fminf(minclr, hs);
                     else world += hs;
@@ -1063,12 +1103,104 @@ __device__ __forceinline__ void group_frames(const DRobot& R, const float* qs, i
 
 static_assert(kGrp >= kMaxJoints, "the grouped kernel gives every joint its own lane of the group");
 
-size_t lds_floats_grp(int obb_cap, int n_dof, int n_joints, int n_sph, bool grad) {
-    (void)n_sph;
+size_t lds_floats_grp(int obb_cap, int n_dof, int n_joints, int n_sph, bool grad, bool tiled) {
     size_t f = (size_t)obb_cap * 16 + (size_t)n_joints * kMaskWords * 2 +
                (((size_t)n_dof * kGrpStates + 3) & ~(size_t)3) + 2 * kWave + (size_t)kGrpStates * (n_joints * 13 + 4);
     if (grad) f += (size_t)kGrpStates * ((size_t)n_joints * 6 + kMaxDof + 1);
+    if (tiled) f += (size_t)kGrpStates * n_sph * 2;
     return f;
+}
+
+// Candidate masks of every joint of the wave's four states over the staged
+// OBBs (the broad phase of near_obbs, all joints' enclosing spheres placed at
+// once): lane jj of each group places joint jj's enclosing sphere; the four
+// states' spread about state slot 0's centre sets the joint's reach; one pass
+// of 64 lanes over the OBBs per joint gives its mask.
+__device__ __forceinline__ void grp_masks(const DRobot& R, const float4* __restrict__ s_obb, int n_obb, const float* Ts,
+                                          float band, bool active, int sub, unsigned long long* s_mask) {
+    const int J = R.n_joints, lane = lane_id();
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    if (sub < J && active) {
+        const float4 je = R.jenc[sub];
+        if (je.w >= 0.0f) {
+            const float4* t = reinterpret_cast<const float4*>(Ts + 12 * sub);
+            const float4 t0 = t[0], t1 = t[1], t2 = t[2];
+            ax = fmaf(t0.x, je.x, fmaf(t0.y, je.y, fmaf(t0.z, je.z, t0.w)));
+            ay = fmaf(t1.x, je.x, fmaf(t1.y, je.y, fmaf(t1.z, je.z, t1.w)));
+            az = fmaf(t2.x, je.x, fmaf(t2.y, je.y, fmaf(t2.z, je.z, t2.w)));
+        }
+    }
+    const float cx = __shfl(ax, sub), cy = __shfl(ay, sub), cz = __shfl(az, sub);   // slot 0's centre
+    const float dx = ax - cx, dy = ay - cy, dz = az - cz;
+    float d2 = (sub < J && active) ? fmaf(dz, dz, fmaf(dy, dy, dx * dx)) : 0.0f;
+    d2 = fmaxf(d2, __shfl_xor(d2, 16));
+    d2 = fmaxf(d2, __shfl_xor(d2, 32));
+    for (int jj = 0; jj < J; ++jj) {
+        const float4 je = R.jenc[jj];
+        ObbMask mk;
+        if (je.w < 0.0f) {
+#pragma unroll
+            for (int k = 0; k < kMaskWords; ++k) mk.w[k] = 0ull;   // no spheres: no candidates
+        } else {
+            const float c0x = read_lane(cx, jj), c0y = read_lane(cy, jj), c0z = read_lane(cz, jj);
+            const float reach = band + je.w + sqrtf(read_lane(d2, jj)) * 1.0001f + 1e-5f;
+            const float thr = uniform(reach * fabsf(reach));
+#pragma unroll
+            for (int k = 0; k < kMaskWords; ++k) {
+                bool c = false;
+                if (k * 64 < n_obb) {
+                    const int o = k * 64 + lane;
+                    if (o < n_obb) {
+                        float lx, ly, lz;
+                        to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], c0x, c0y, c0z, lx, ly, lz);
+                        c = box_key(lx, ly, lz, s_obb[4 * o + 3]) < thr;
+                    }
+                }
+                mk.w[k] = __ballot(c);
+            }
+        }
+        if (lane < kMaskWords) {
+            unsigned long long v = 0ull;
+#pragma unroll
+            for (int k = 0; k < kMaskWords; ++k) if (k == lane) v = mk.w[k];
+            s_mask[jj * kMaskWords + lane] = v;
+        }
+    }
+}
+
+// Smallest box key at p over the candidate OBBs of `own` (a joint's mask in
+// LDS) and its LDS record index (-1: none). Two candidates per step: both
+// records are read before either test, so the LDS latency of one hides the
+// other's. Ties keep the lower index.
+__device__ __forceinline__ void grp_argmin(const float4* __restrict__ s_obb, const unsigned long long* own, bool valid,
+                                           float px, float py, float pz, const uint8_t* __restrict__ acm_row,
+                                           float& best, int& arg) {
+    best = FLT_MAX;
+    arg = -1;
+#pragma unroll
+    for (int k = 0; k < kMaskWords; ++k) {
+        unsigned long long bits = valid ? own[k] : 0ull;
+        while (bits) {
+            const int oa = k * 64 + (__ffsll(bits) - 1);
+            bits &= bits - 1ull;
+            const bool two = bits != 0ull;
+            const int ob = two ? k * 64 + (__ffsll(bits) - 1) : oa;
+            if (two) bits &= bits - 1ull;
+            const float4 a0 = s_obb[4 * oa], a1 = s_obb[4 * oa + 1], a2 = s_obb[4 * oa + 2], ah = s_obb[4 * oa + 3];
+            const float4 b0 = s_obb[4 * ob], b1 = s_obb[4 * ob + 1], b2 = s_obb[4 * ob + 2], bh = s_obb[4 * ob + 3];
+            float lx, ly, lz;
+            to_box(a0, a1, a2, px, py, pz, lx, ly, lz);
+            float ka = box_key(lx, ly, lz, ah);
+            to_box(b0, b1, b2, px, py, pz, lx, ly, lz);
+            float kb = box_key(lx, ly, lz, bh);
+            if (acm_row) {
+                if (acm_row[__float_as_int(ah.w)] == 0) ka = FLT_MAX;
+                if (acm_row[__float_as_int(bh.w)] == 0) kb = FLT_MAX;
+            }
+            if (ka < best) { best = ka; arg = oa; }        // oa < ob: ties keep the lower index
+            if (two && kb < best) { best = kb; arg = ob; }
+        }
+    }
 }
 
 // WORLD: bit 0 the scene's boxes may enter (OBB hinge / argmin), bit 1 its
@@ -1090,6 +1222,10 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
     int* s_par = reinterpret_cast<int*>(s_T + kGrpStates * fstride);   // [slot][J] pointer-jumping links
     float* s_w = reinterpret_cast<float*>(s_par + kGrpStates * J);    // GRAD: [slot][J][6] wrenches
     float* s_dq = s_w + kGrpStates * J * 6;                       // GRAD: [slot][kMaxDof + 1]
+    // tiled problems: per state slot and emission slot, the smallest box key
+    // over the tiles so far and its OBB (original index, -1: none)
+    float* s_best = GRAD ? s_dq + kGrpStates * (kMaxDof + 1) : s_w;
+    int* s_barg = reinterpret_cast<int*>(s_best + kGrpStates * S);
     const int lane = lane_id();
     const int sub = lane & (kGrp - 1);
     const int slot = lane / kGrp;
@@ -1098,19 +1234,23 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
     const bool use_esdf = (WORLD & 2) && (a.P.use_esdf != 0 || a.clearance_only) && a.S.n_voxel_layers > 0;
     const bool want_grad = GRAD && !a.clearance_only;
     const float wt = a.P.weight;
-    int cur_prob = -1, n_obb = 0;
+    const int cap = a.C.obb_cap;
+    int cur_prob = -1, n_obb = 0, o0 = 0, o1 = 0, n_tiles = 1;
     int4 ch;
     ChunkCursor cur = chunk_cursor();
     const long n_states = state_count(a.X);
     const int n_chunks = chunk_count(a.C, n_states);
     while (next_chunk(a.C, n_states, n_chunks, cur, ch)) {
         if ((WORLD & 1) && ch.x != cur_prob) {
-            int o0, o1;
             problem_obbs(a.C, a.S, ch.x, o0, o1);
-            __syncthreads();
-            n_obb = uniform(stage_obbs(a.S, o0, o1, s_obb, a.C.obb_cap));
+            n_tiles = uniform(obb_tiles(o0, o1, cap));
+            if (n_tiles == 1) {
+                __syncthreads();
+                n_obb = uniform(stage_obbs(a.S, o0, o1, s_obb, cap));
+            }
             cur_prob = ch.x;
         }
+        const bool tiled = use_obb && n_tiles > 1;
         const long g = (long)ch.y + slot;
         const bool active = g < ch.z;
         stage_unit_ends(a.X, ch.w, s_end);
@@ -1121,60 +1261,39 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
         __syncthreads();
         // 1a. joint frames (pointer jumping over the DFS tree)
         group_frames(R, s_q + slot, kGrpStates, Ts, s_par + slot * J, sub);
-        // 1b. candidate masks per joint (the broad phase of near_obbs, all
-        // joints' approximate centres placed at once): lane jj of each group
-        // places joint jj's approximate sphere; the four states' spread
-        // about state slot 0's centre sets the joint's reach; one pass of 64
-        // lanes over the OBBs per joint gives its mask.
-        if (use_obb) {
-            float ax = 0.0f, ay = 0.0f, az = 0.0f;
-            if (sub < J && active) {
-                const int a0 = R.apx_off[sub];
-                const float4 as = R.apx_off[sub + 1] - a0 == 1 ? R.apx[a0] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-                if (as.w >= 0.0f) {
-                    const float4* t = reinterpret_cast<const float4*>(Ts + 12 * sub);
-                    const float4 t0 = t[0], t1 = t[1], t2 = t[2];
-                    ax = fmaf(t0.x, as.x, fmaf(t0.y, as.y, fmaf(t0.z, as.z, t0.w)));
-                    ay = fmaf(t1.x, as.x, fmaf(t1.y, as.y, fmaf(t1.z, as.z, t1.w)));
-                    az = fmaf(t2.x, as.x, fmaf(t2.y, as.y, fmaf(t2.z, as.z, t2.w)));
-                }
-            }
-            const float cx = __shfl(ax, sub), cy = __shfl(ay, sub), cz = __shfl(az, sub);   // slot 0's centre
-            const float dx = ax - cx, dy = ay - cy, dz = az - cz;
-            float d2 = (sub < J && active) ? fmaf(dz, dz, fmaf(dy, dy, dx * dx)) : 0.0f;
-            d2 = fmaxf(d2, __shfl_xor(d2, 16));
-            d2 = fmaxf(d2, __shfl_xor(d2, 32));
-            for (int jj = 0; jj < J; ++jj) {
-                const int a0 = R.apx_off[jj];
-                ObbMask mk;
-                if (R.sph_off[jj + 1] == R.sph_off[jj]) {
-#pragma unroll
-                    for (int k = 0; k < kMaskWords; ++k) mk.w[k] = 0ull;   // no spheres: no candidates
-                } else if (R.apx_off[jj + 1] - a0 != 1) {
-                    mask_all(n_obb, mk);   // no single enclosing sphere: every OBB is a candidate
-                } else {
-                    const float c0x = read_lane(cx, jj), c0y = read_lane(cy, jj), c0z = read_lane(cz, jj);
-                    const float reach = a.P.world_margin + R.apx[a0].w + sqrtf(read_lane(d2, jj)) * 1.0001f + 1e-5f;
-                    const float thr = uniform(reach * fabsf(reach));
-#pragma unroll
-                    for (int k = 0; k < kMaskWords; ++k) {
-                        bool c = false;
-                        if (k * 64 < n_obb) {
-                            const int o = k * 64 + lane;
-                            if (o < n_obb) {
-                                float lx, ly, lz;
-                                to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], c0x, c0y, c0z, lx, ly, lz);
-                                c = box_key(lx, ly, lz, s_obb[4 * o + 3]) < thr;
-                            }
-                        }
-                        mk.w[k] = __ballot(c);
+        // 1b. candidate masks per joint; a tiled problem's OBBs pass through
+        // LDS tile by tile, each tile's argmins folded into s_best / s_barg
+        // (tiles in index order and a strict < keep the lowest index on ties)
+        if (use_obb && !tiled) grp_masks(R, s_obb, n_obb, Ts, a.P.world_margin, active, sub, s_mask);
+        for (int t = 0; tiled && t < n_tiles; ++t) {
+            const int t0 = o0 + t * cap;
+            __syncthreads();
+            n_obb = uniform(stage_obbs(a.S, t0, min(o1, t0 + cap), s_obb, cap));
+            grp_masks(R, s_obb, n_obb, Ts, a.P.world_margin, active, sub, s_mask);
+            __syncthreads();
+            for (int r0 = 0; r0 < S; r0 += kGrp) {
+                const int s = r0 + sub;
+                const bool valid = active && s < S;
+                const int sc = valid ? s : r0;
+                const int jj = R.sph_jnt[sc];
+                const float4* Tq = reinterpret_cast<const float4*>(Ts + 12 * jj);
+                const float4 t0q = Tq[0], t1q = Tq[1], t2q = Tq[2];
+                const float4 sp = R.sph[sc];
+                const int sid = R.sph_id[sc];
+                const float px = fmaf(t0q.x, sp.x, fmaf(t0q.y, sp.y, fmaf(t0q.z, sp.z, t0q.w)));
+                const float py = fmaf(t1q.x, sp.x, fmaf(t1q.y, sp.y, fmaf(t1q.z, sp.z, t1q.w)));
+                const float pz = fmaf(t2q.x, sp.x, fmaf(t2q.y, sp.y, fmaf(t2q.z, sp.z, t2q.w)));
+                const uint8_t* acm_row = a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)sid * a.S.n_obbs : nullptr;
+                float best;
+                int arg;
+                grp_argmin(s_obb, s_mask + jj * kMaskWords, valid, px, py, pz, acm_row, best, arg);
+                if (valid) {
+                    float* bk = s_best + slot * S + s;
+                    int* ba = s_barg + slot * S + s;
+                    if (t == 0 || best < *bk) {
+                        *bk = best;
+                        *ba = arg >= 0 ? __float_as_int(s_obb[4 * arg + 3].w) : -1;
                     }
-                }
-                if (lane < kMaskWords) {
-                    unsigned long long v = 0ull;
-#pragma unroll
-                    for (int k = 0; k < kMaskWords; ++k) if (k == lane) v = mk.w[k];
-                    s_mask[jj * kMaskWords + lane] = v;
                 }
             }
         }
@@ -1187,7 +1306,8 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
         }
         // 2. sphere pass, in rounds of 16 emission slots per group: each lane
         // places its sphere with its joint's frame and walks its joint's
-        // candidate mask for the argmin box.
+        // candidate mask for the argmin box (tiled: reads the folded argmin
+        // and rebuilds its record from the scene when the gradient needs it).
         float world = 0.0f, minclr = 1000.0f;
         for (int r0 = 0; r0 < S; r0 += kGrp) {
             const int s = r0 + sub;
@@ -1202,43 +1322,26 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
             const float py = fmaf(t1.x, sp.x, fmaf(t1.y, sp.y, fmaf(t1.z, sp.z, t1.w)));
             const float pz = fmaf(t2.x, sp.x, fmaf(t2.y, sp.y, fmaf(t2.z, sp.z, t2.w)));
             float best = FLT_MAX;
-            int arg = -1;
-            if (use_obb) {
-                ObbMask own;
-#pragma unroll
-                for (int k = 0; k < kMaskWords; ++k) own.w[k] = s_mask[jj * kMaskWords + k];
-                const uint8_t* acm_row = a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)sid * a.S.n_obbs : nullptr;
-                // two candidates per step: both records are read before
-                // either test, so the LDS latency of one hides the other's
-#pragma unroll
-                for (int k = 0; k < kMaskWords; ++k) {
-                    unsigned long long bits = valid ? own.w[k] : 0ull;
-                    while (bits) {
-                        const int oa = k * 64 + (__ffsll(bits) - 1);
-                        bits &= bits - 1ull;
-                        const bool two = bits != 0ull;
-                        const int ob = two ? k * 64 + (__ffsll(bits) - 1) : oa;
-                        if (two) bits &= bits - 1ull;
-                        const float4 a0 = s_obb[4 * oa], a1 = s_obb[4 * oa + 1], a2 = s_obb[4 * oa + 2], ah = s_obb[4 * oa + 3];
-                        const float4 b0 = s_obb[4 * ob], b1 = s_obb[4 * ob + 1], b2 = s_obb[4 * ob + 2], bh = s_obb[4 * ob + 3];
-                        float lx, ly, lz;
-                        to_box(a0, a1, a2, px, py, pz, lx, ly, lz);
-                        float ka = box_key(lx, ly, lz, ah);
-                        to_box(b0, b1, b2, px, py, pz, lx, ly, lz);
-                        float kb = box_key(lx, ly, lz, bh);
-                        if (acm_row) {
-                            if (acm_row[__float_as_int(ah.w)] == 0) ka = FLT_MAX;
-                            if (acm_row[__float_as_int(bh.w)] == 0) kb = FLT_MAX;
-                        }
-                        if (ka < best) { best = ka; arg = oa; }        // oa < ob: ties keep the lower index
-                        if (two && kb < best) { best = kb; arg = ob; }
+            const float4* rec = nullptr;
+            float4 grec[4];
+            if (tiled) {
+                if (valid) {
+                    best = s_best[slot * S + s];
+                    const int orig = s_barg[slot * S + s];
+                    if (orig >= 0) {
+                        if (want_grad && a.P.world_margin - (key_to_sdf(best) - sp.w) > 0.0f) obb_record(a.S, orig, grec);
+                        rec = grec;
                     }
                 }
+            } else if (use_obb) {
+                const uint8_t* acm_row = a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)sid * a.S.n_obbs : nullptr;
+                int arg;
+                grp_argmin(s_obb, s_mask + jj * kMaskWords, valid, px, py, pz, acm_row, best, arg);
+                if (arg >= 0) rec = s_obb + 4 * arg;
             }
             if (!valid) continue;
             float gx, gy, gz;
-            const float hs = sphere_hinge_at<GRAD>(a, s_obb, best, arg, px, py, pz, sp.w, sid, use_obb, use_esdf,
-                                                   gx, gy, gz);
+            const float hs = sphere_hinge_at<GRAD>(a, rec, best, px, py, pz, sp.w, sid, use_obb, use_esdf, gx, gy, gz);
             if (a.clearance_only) minclr = fminf(minclr, hs);
             else world += hs;
             if (want_grad) {
@@ -2107,23 +2210,30 @@ __global__ __launch_bounds__(256) void k_esdf_build(const float* obb_dims, const
                                                     float4 prm, float4 gp0, float4 gp1, int nx, int ny, int nz, float* out) {
     const float gpose[8] = {gp0.x, gp0.y, gp0.z, gp0.w, gp1.x, gp1.y, gp1.z, gp1.w};
     __shared__ float4 s_obb[4 * kMaxObbLds];
-    // stage with the first wave (same record format as the collision kernels)
-    for (int base = 0; base < n_obbs && base < kMaxObbLds; base += blockDim.x) {
-        const int o = base + threadIdx.x;
-        if (o < n_obbs && o < kMaxObbLds) {
+    // OBBs [t0, t1) -> LDS records (the collision kernels' format)
+    auto stage = [&](int t0, int t1) {
+        for (int o = t0 + (int)threadIdx.x; o < t1; o += blockDim.x) {
             const float* d = obb_dims + 4 * o;
             const float* p = obb_pose + 8 * o;
             const float w = p[3], x = p[4], y = p[5], z = p[6];
-            s_obb[4 * o + 0] = make_float4(1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y - w * z), 2.0f * (x * z + w * y), p[0]);
-            s_obb[4 * o + 1] = make_float4(2.0f * (x * y + w * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z - w * x), p[1]);
-            s_obb[4 * o + 2] = make_float4(2.0f * (x * z - w * y), 2.0f * (y * z + w * x), 1.0f - 2.0f * (x * x + y * y), p[2]);
-            s_obb[4 * o + 3] = make_float4(0.5f * d[0], 0.5f * d[1], 0.5f * d[2], 0.0f);
+            float4* r = s_obb + 4 * (o - t0);
+            r[0] = make_float4(1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y - w * z), 2.0f * (x * z + w * y), p[0]);
+            r[1] = make_float4(2.0f * (x * y + w * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z - w * x), p[1]);
+            r[2] = make_float4(2.0f * (x * z - w * y), 2.0f * (y * z + w * x), 1.0f - 2.0f * (x * x + y * y), p[2]);
+            r[3] = make_float4(0.5f * d[0], 0.5f * d[1], 0.5f * d[2], 0.0f);
         }
+    };
+    // more OBBs than one staging holds: every block batch of voxels walks
+    // the OBBs tile by tile (block-uniform trip counts around the barriers)
+    const bool tiled = n_obbs > kMaxObbLds;
+    if (!tiled) {
+        stage(0, n_obbs);
+        __syncthreads();
     }
-    __syncthreads();
     const long n = (long)nx * ny * nz;
-    const int no = n_obbs < kMaxObbLds ? n_obbs : kMaxObbLds;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    for (long base = (long)blockIdx.x * blockDim.x; base < n; base += (long)gridDim.x * blockDim.x) {
+        const long i = base + threadIdx.x;
+        const bool in = i < n;
         const int ix = (int)(i / ((long)ny * nz));
         const int iy = (int)((i / nz) % ny);
         const int iz = (int)(i % nz);
@@ -2137,12 +2247,20 @@ __global__ __launch_bounds__(256) void k_esdf_build(const float* obb_dims, const
         const float py = ly + qw * ty + (qz * tx - qx * tz);
         const float pz = lz + qw * tz + (qx * ty - qy * tx);
         float best = FLT_MAX;
-        for (int o = 0; o < no; ++o) {
-            float bx, by, bz;
-            to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], px, py, pz, bx, by, bz);
-            best = fminf(best, box_key(bx, by, bz, s_obb[4 * o + 3]));
+        for (int t0 = 0; t0 < n_obbs; t0 += kMaxObbLds) {
+            const int t1 = min(n_obbs, t0 + kMaxObbLds);
+            if (tiled) {
+                __syncthreads();
+                stage(t0, t1);
+                __syncthreads();
+            }
+            for (int o = 0; o < t1 - t0; ++o) {
+                float bx, by, bz;
+                to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], px, py, pz, bx, by, bz);
+                best = fminf(best, box_key(bx, by, bz, s_obb[4 * o + 3]));
+            }
         }
-        out[i] = -key_to_sdf(best);
+        if (in) out[i] = -key_to_sdf(best);
     }
 }
 
@@ -2207,14 +2325,15 @@ hipError_t launch_collide(const CollideArgs& a, int grid, hipStream_t s) {
 }
 
 hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s) {
+    if (a.C.tiled) return hipErrorInvalidValue;   // k_cost stages one OBB tile: tiled problems take k_cost_grp
     const size_t lds = lds_bytes_cost(a.C.obb_cap, a.R.n_dof, a.R.n_joints, a.group, a.grad != 0);
     if (a.grad) hipLaunchKernelGGL(k_cost<true>, dim3(grid), dim3(64), lds, s, a);
     else hipLaunchKernelGGL(k_cost<false>, dim3(grid), dim3(64), lds, s, a);
     return hipGetLastError();
 }
 
-size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, int n_sph, bool grad) {
-    return lds_floats_grp(obb_cap, n_dof, n_joints, n_sph, grad) * sizeof(float);
+size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, int n_sph, bool grad, bool tiled) {
+    return lds_floats_grp(obb_cap, n_dof, n_joints, n_sph, grad, tiled) * sizeof(float);
 }
 
 // Variant table: [grad][world - 1].
@@ -2243,7 +2362,7 @@ int grid_cost_grp(bool grad, int world, size_t lds) {
 }
 
 hipError_t launch_cost_grp(const CostArgs& a, int grid, hipStream_t s) {
-    const size_t lds = lds_bytes_grp(a.C.obb_cap, a.R.n_dof, a.R.n_joints, a.R.n_sph, a.grad != 0);
+    const size_t lds = lds_bytes_grp(a.C.obb_cap, a.R.n_dof, a.R.n_joints, a.R.n_sph, a.grad != 0, a.C.tiled != 0);
     const int world = grp_world(a);
     grid = std::max(1, std::min(grid, grid_cost_grp(a.grad != 0, world, lds)));   // persistent: occupancy x CUs
 #define PPL_GRP_LAUNCH(G_, W_) hipLaunchKernelGGL((k_cost_grp<G_, W_>), dim3(grid), dim3(64), lds, s, a)

@@ -118,7 +118,9 @@ struct DChunks {
     int n_chunks;
     int chunk;                  // states per chunk (64 / lanes per state)
     const int* obb_off;         // [P+1] OBB ranges per problem; NULL = single problem
-    int obb_cap;                // OBB records the LDS staging area holds (>= OBBs of any problem)
+    int obb_cap;                // OBB records one LDS staging holds (a tile; 1 .. kMaxObbLds)
+    int tiled;                  // some problem has more OBBs than obb_cap: they pass through LDS in tiles
+                                // (k_collide, the grouped cost kernel; k_cost refuses such launches)
     const int* unit_hint;       // implicit chunks (tab NULL), nullable: unit holding state 64 k, for each k
 };
 
@@ -242,7 +244,7 @@ __host__ __device__ inline int dq_stride(int n_dof) { return n_dof | 1; }
 int device_cus();
 int grid_collide(size_t lds);
 int grid_cost(int n_joints, bool grad, size_t lds);
-size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, int n_sph, bool grad);
+size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, int n_sph, bool grad, bool tiled);
 int grid_cost_grp(bool grad, int world, size_t lds);
 int grp_world(const CostArgs& a);
 
