@@ -102,6 +102,11 @@ typedef struct {
     const int* pairs;         /* [n_pairs][2] self pairs expanded from self_cc_ranges (original indices) */
     const int* apx_pairs;     /* [n_approx_pairs][2] */
     const float* lim;         /* [n_dof][2] joint lower, upper */
+    /* The approximate spheres' own chain (RobotModel's approx_fixed_transforms /
+     * approx_joint_types, runtime_kinematics.cuh:176-243), DFS order; both NULL
+     * when the approximate spheres ride the full chain. */
+    const float* aft;         /* [n_joints][12] rows 0..2 of each approximate fixed transform */
+    const int* atype;         /* [n_joints] approximate joint types */
 } ppl_robot_device_view;
 
 ppl_status ppl_robot_get_device_view(ppl_robot robot, ppl_robot_device_view* out);

@@ -217,14 +217,18 @@ __device__ void fk_spheres(const ppl_robot_device_view& R, const float* q, float
     float slot[2][12];
     aff_identity(slot[0]);
     aff_identity(slot[1]);
+    // the approximate spheres ride their own chain when the model has one
+    const bool own = approx && R.aft != nullptr;
+    const float* ftab = own ? R.aft : R.ft;
     for (int jj = 0; jj < R.n_joints; ++jj) {
-        const int type = R.jmeta[4 * jj], dof = R.jmeta[4 * jj + 1], sl = R.jmeta[4 * jj + 2], ps = R.jmeta[4 * jj + 3];
+        const int dof = R.jmeta[4 * jj + 1], sl = R.jmeta[4 * jj + 2], ps = R.jmeta[4 * jj + 3];
+        const int type = own ? R.atype[jj] : R.jmeta[4 * jj];
         float T[12];
         if (jj == 0) {
             aff_identity(T);
         } else {
             float S[12];
-            joint_step(R.ft + 12 * jj, type, dof >= 0 ? q[dof] : 0.0f, S);
+            joint_step(ftab + 12 * jj, type, dof >= 0 ? q[dof] : 0.0f, S);
             aff_compose(ps == 0 ? slot[0] : slot[1], S, T);
         }
 #pragma unroll
@@ -436,11 +440,14 @@ __device__ TwoPhaseResult two_phase_cc(const ppl_robot_device_view& R, const ppl
 // the reference's FK scratch `T`) keeps the joint frames of the approximate
 // pass, and the exact spheres are placed from them. Lanes take the
 // approximate spheres, then the exact spheres of flagged joints, in turn
-// across all joints.
+// across all joints. A model whose approximate spheres have a chain of their
+// own (R.aft) takes the two-walk form above.
 template <int G>
 __device__ TwoPhaseResult two_phase_cc(const ppl_robot_device_view& R, const ppl_scene_view& S, const float* q,
                                        float* sphere_pos, float* approx_sphere_pos, float* T, int tid,
                                        float collision_margin, bool check_self) {
+    // an approximate chain of its own: the frames of one walk do not serve both sphere sets
+    if (R.aft) return two_phase_cc<G>(R, S, q, sphere_pos, approx_sphere_pos, tid, collision_margin, check_self);
     const int lane = tid & (G - 1);
     TwoPhaseResult out{false, false, false};
     fk_spheres<G>(R, q, approx_sphere_pos, tid, true, T);

@@ -77,8 +77,11 @@ inline void joint_step(const float* ft, int type, T q, T* S) {
 // onto its parent's frame.
 template <typename T>
 void fk_frames(const orc_robot& R, const float* q, T* frames, bool approx_chain) {
-    const float* fts = (approx_chain && R.approx_fixed_transforms) ? R.approx_fixed_transforms : R.fixed_transforms;
-    const int* types = (approx_chain && R.approx_joint_types) ? R.approx_joint_types : R.joint_types;
+    // A model without approximate transforms gets the full chain, types
+    // included (python/bindings.cpp:157-164 copies both).
+    const bool own = approx_chain && R.approx_fixed_transforms;
+    const float* fts = own ? R.approx_fixed_transforms : R.fixed_transforms;
+    const int* types = (own && R.approx_joint_types) ? R.approx_joint_types : R.joint_types;
     for (int jj = 0; jj < R.n_joints; ++jj) {
         const int i = R.dfs_order[jj];
         T* out = frames + 12 * i;

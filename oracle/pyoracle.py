@@ -103,8 +103,18 @@ class Oracle:
         r.n_spheres, r.n_approx = robot.n_spheres, robot.n_approx
         r.n_self_ranges = int(robot.self_ranges.shape[0])
         r.n_approx_self_ranges = int(robot.approx_self_ranges.shape[0])
+        aft = getattr(robot, "approx_fixed_transforms", None)
+        aty = getattr(robot, "approx_joint_types", None)
         r.approx_fixed_transforms = None
         r.approx_joint_types = None
+        if aft is not None:
+            a = _c(aft, np.float32)
+            k(a)
+            r.approx_fixed_transforms = _ptr(a, _f32p)
+        if aty is not None:
+            a = _c(aty, np.int32)
+            k(a)
+            r.approx_joint_types = _ptr(a, _i32p)
         self.set_scene(scene)
 
     def set_scene(self, scene):

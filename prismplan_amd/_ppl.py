@@ -53,7 +53,7 @@ class RobotDeviceView(C.Structure):
                 ("n_pairs", C.c_int), ("n_approx_pairs", C.c_int),
                 ("ft", _vp), ("jmeta", _vp), ("sph_off", _vp), ("sph", _vp), ("sph_id", _vp), ("apx_off", _vp),
                 ("apx", _vp), ("apx_id", _vp), ("sph_orig", _vp), ("apx_orig", _vp), ("pairs", _vp),
-                ("apx_pairs", _vp), ("lim", _vp)]
+                ("apx_pairs", _vp), ("lim", _vp), ("aft", _vp), ("atype", _vp)]
 
 
 class SceneView(C.Structure):

@@ -99,7 +99,6 @@ struct ppl_robot_s {
     int n_slots = 1;
     int n_pairs = 0;
     int n_revolute = 0, n_prismatic = 0;   // joints that move with a dof
-    bool self_gate_implied = true;          // exact self hits imply approximate ones
     std::vector<float> lower, upper;
     std::vector<std::pair<int, ppl_plan>> traj_plans;   // single-trajectory plans of the plain CHOMP forms, by T
     hipStream_t stream = nullptr;
@@ -355,10 +354,6 @@ int edge_g(const float* a, const float* b, int dim, const ppl_edge_params& p, fl
 ppl_status run_self(const ppl_robot_s* r, const DStates& X, int mode, uint8_t* hit_out, float* out, hipStream_t s,
                     float band = FLT_MAX, int* unit_first_hit = nullptr, ppl_plan_s* P = nullptr,
                     const int* unit_hint = nullptr) {
-    if (mode == 0 && !r->self_gate_implied)
-        return fail(PPL_ERR_UNSUPPORTED,
-                    "self any-collision: the approximate model does not cover every exact self pair (an exact "
-                    "pair's spheres must lie in approximate spheres that form an approximate pair)");
     SelfArgs a{};
     a.R = r->R;
     a.X = X;
@@ -393,7 +388,6 @@ ppl_status scene_view_check(const ppl_scene_view* v) {
     ppl_status st = scene_from_view(v, S);
     return st;
 }
-bool robot_self_gate_implied(ppl_robot r) { return r && r->self_gate_implied; }
 }  // namespace ppl
 
 extern "C" {
@@ -446,12 +440,24 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
                       std::fabs(ft[15] - 1.0f) < 1e-6f,
                   "fixed_transforms[%d] last row is not [0 0 0 1]", i);
         if (d->approx_fixed_transforms) {
-            for (int k = 0; k < 16; ++k)
-                if (d->approx_fixed_transforms[16 * i + k] != ft[k])
-                    return fail(PPL_ERR_UNSUPPORTED, "approx chain differing from the full chain is not supported");
+            const float* af = d->approx_fixed_transforms + 16 * i;
+            CHECK_ARG(std::fabs(af[12]) < 1e-6f && std::fabs(af[13]) < 1e-6f && std::fabs(af[14]) < 1e-6f &&
+                          std::fabs(af[15] - 1.0f) < 1e-6f,
+                      "approx_fixed_transforms[%d] last row is not [0 0 0 1]", i);
         }
-        if (d->approx_joint_types && d->approx_joint_types[i] != t)
-            return fail(PPL_ERR_UNSUPPORTED, "approx joint types differing from the full chain are not supported");
+        if (d->approx_joint_types)
+            CHECK_ARG(d->approx_joint_types[i] >= -1 && d->approx_joint_types[i] <= 5, "approx_joint_types[%d] = %d unknown",
+                      i, d->approx_joint_types[i]);
+    }
+    // The approximate spheres ride a chain of their own when the model gives
+    // one that differs from the full chain (bindings.cpp:161 copies the full
+    // chain when none is set; runtime_kinematics.cuh:176-243 walks it).
+    // Without approximate transforms the full chain serves, types included.
+    bool own_chain = false;
+    for (int i = 0; i < J && !own_chain && d->approx_fixed_transforms; ++i) {
+        if (d->approx_joint_types && d->approx_joint_types[i] != d->joint_types[i]) own_chain = true;
+        for (int k = 0; k < 16; ++k)
+            if (d->approx_fixed_transforms[16 * i + k] != d->fixed_transforms[16 * i + k]) own_chain = true;
     }
     if (n_slots > kMaxSlots) return fail(PPL_ERR_UNSUPPORTED, "%d FK slots requested, %d supported", n_slots, kMaxSlots);
     // The slot each joint reads must still hold its parent's frame.
@@ -482,9 +488,10 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     // after an approximate pair hits (two_phase_cc.cuh:114-131). When every
     // exact pair's spheres lie inside approximate spheres of their joints that
     // form an approximate pair (the loader's models), an exact hit implies an
-    // approximate one and the self pass checks exact pairs alone.
-    bool self_gate_implied = true;
-    {
+    // approximate one and the self pass checks exact pairs alone; otherwise
+    // the self pass runs the approximate pairs first.
+    bool self_gate_implied = !own_chain || pairs.empty();
+    if (!own_chain) {
         std::vector<char> apair((size_t)A * A, 0);
         for (const int2& p : apairs) apair[(size_t)p.x * A + p.y] = apair[(size_t)p.y * A + p.x] = 1;
         auto inside = [&](int s, int a) {
@@ -506,6 +513,23 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
             }
         }
     }
+    // The any-collision kernel may gate each (joint, OBB) pair on its own when
+    // every exact sphere lies inside an approximate sphere of its joint on the
+    // same chain: an exact hit of a box then implies the approximate hit of
+    // that box, so flagging joints first (scene_collision.cuh:519-628) adds
+    // nothing. Otherwise the kernel flags joints over the whole scene first.
+    bool scene_gate_fused = !own_chain;
+    for (int si = 0; si < S && scene_gate_fused; ++si) {
+        const float* f = d->spheres + 4 * si;
+        bool in = false;
+        for (int a = 0; a < A && !in; ++a) {
+            if (d->approx_sphere_to_joint[a] != d->sphere_to_joint[si]) continue;
+            const float* g = d->approx_spheres + 4 * a;
+            const double dx = (double)f[0] - g[0], dy = (double)f[1] - g[1], dz = (double)f[2] - g[2];
+            in = std::sqrt(dx * dx + dy * dy + dz * dz) + f[3] <= (double)g[3] + 1e-6;
+        }
+        scene_gate_fused = in;
+    }
     // DFS-ordered tables.
     std::vector<float> ft(12 * J);
     std::vector<int4> meta(J);
@@ -515,6 +539,21 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
             for (int c = 0; c < 4; ++c) ft[12 * k + 4 * r + c] = d->fixed_transforms[16 * i + 4 * r + c];
         meta[k] = make_int4(d->joint_types[i], id2dof[i], tmem[i], tmem[parents[i]]);
     }
+    std::vector<float> aft;
+    std::vector<int> atype;
+    if (own_chain) {
+        aft.resize(12 * J);
+        atype.resize(J);
+        const float* src = d->approx_fixed_transforms;
+        for (int k = 0; k < J; ++k) {
+            const int i = dfs[k];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 4; ++c) aft[12 * k + 4 * r + c] = src[16 * i + 4 * r + c];
+            atype[k] = d->approx_joint_types ? d->approx_joint_types[i] : d->joint_types[i];
+        }
+    }
+    std::vector<int> apx_jnt(A);
+    for (int a = 0; a < A; ++a) apx_jnt[a] = pos[d->approx_sphere_to_joint[a]];
     auto emit = [&](int n, const float* sp, const int* s2j, std::vector<int>& off, std::vector<float4>& v,
                     std::vector<int>& id) {
         off.assign(J + 1, 0);
@@ -640,7 +679,8 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
                  o_apairs = place(apairs.size() * 8), o_aorig = place(apx_orig.size() * 16),
                  o_sblk = place(sblk.size() * 16), o_rmask = place(rowmask.size() * 8), o_jorig = place(J * 4),
                  o_lim = place((size_t)D * 8),
-                 o_jenc = place((size_t)J * 16), o_sjnt = place((size_t)S * 4), o_jend = place((size_t)J * 4);
+                 o_jenc = place((size_t)J * 16), o_sjnt = place((size_t)S * 4), o_jend = place((size_t)J * 4),
+                 o_aft = place(aft.size() * 4), o_atype = place(atype.size() * 4), o_ajnt = place(apx_jnt.size() * 4);
     std::vector<char> host(off + 16, 0);
     auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) memcpy(host.data() + o, src, bytes); };
     put(o_ft, ft.data(), ft.size() * 4);
@@ -665,6 +705,9 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     put(o_jenc, jenc.data(), (size_t)J * 16);
     put(o_sjnt, sph_jnt.data(), (size_t)S * 4);
     put(o_jend, jend.data(), (size_t)J * 4);
+    put(o_aft, aft.data(), aft.size() * 4);
+    put(o_atype, atype.data(), atype.size() * 4);
+    put(o_ajnt, apx_jnt.data(), apx_jnt.size() * 4);
 
     ppl_robot_s* r = new (std::nothrow) ppl_robot_s();
     if (!r) return fail(PPL_ERR_ALLOC, "out of host memory");
@@ -702,10 +745,14 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     R.jenc = reinterpret_cast<const float4*>(b + o_jenc);
     R.sph_jnt = reinterpret_cast<const int*>(b + o_sjnt);
     R.jend = reinterpret_cast<const int*>(b + o_jend);
+    R.aft = own_chain ? reinterpret_cast<const float*>(b + o_aft) : nullptr;
+    R.atype = own_chain ? reinterpret_cast<const int*>(b + o_atype) : nullptr;
+    R.apx_jnt = reinterpret_cast<const int*>(b + o_ajnt);
+    R.scene_gate_fused = scene_gate_fused ? 1 : 0;
+    R.self_gate_implied = self_gate_implied ? 1 : 0;
     R.n_sblk = (int)sblk.size() / 2;
     r->n_slots = n_slots;
     r->n_pairs = (int)pairs.size();
-    r->self_gate_implied = self_gate_implied;
     for (int i = 0; i < J; ++i) {
         if (i == root || id2dof[i] < 0) continue;
         if (d->joint_types[i] >= kXRot) ++r->n_revolute;
@@ -741,6 +788,8 @@ ppl_status ppl_robot_get_device_view(ppl_robot r, ppl_robot_device_view* v) try 
     v->pairs = reinterpret_cast<const int*>(R.pairs);
     v->apx_pairs = reinterpret_cast<const int*>(R.apx_pairs);
     v->lim = reinterpret_cast<const float*>(R.lim);
+    v->aft = R.aft;
+    v->atype = R.atype;
     return PPL_OK;
 } catch (const std::bad_alloc&) {
     return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");

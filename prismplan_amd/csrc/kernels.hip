@@ -109,6 +109,10 @@ struct Slots {
 };
 
 // Transform of DFS position jj: identity at the root, parent * step otherwise.
+// APPROX walks the approximate spheres' own chain (R.aft / R.atype, set when
+// the model has one): same tree, dofs and slots, its own fixed transforms and
+// joint types.
+template <bool APPROX = false>
 __device__ __forceinline__ void fk_joint(const DRobot& R, int jj, const float* s_q, Slots& sl, float* T, int4& meta) {
     meta = R.jmeta[jj];
     if (jj == 0) {
@@ -117,7 +121,7 @@ __device__ __forceinline__ void fk_joint(const DRobot& R, int jj, const float* s
         float P[12], S[12];
         sl.load(meta.w, P);
         const float q = meta.y >= 0 ? s_q[meta.y * kWave] : 0.0f;
-        joint_step(R.ft + 12 * jj, meta.x, q, S);
+        joint_step(APPROX ? R.aft + 12 * jj : R.ft + 12 * jj, APPROX ? R.atype[jj] : meta.x, q, S);
         aff_compose(P, S, T);
     }
     sl.store(meta.z, T);
@@ -532,14 +536,6 @@ struct ObbMask {
     unsigned long long w[kMaskWords];
 };
 
-__device__ __forceinline__ void mask_all(int n_obb, ObbMask& mk) {
-#pragma unroll
-    for (int k = 0; k < kMaskWords; ++k) {
-        const int n = n_obb - k * 64;
-        mk.w[k] = n >= 64 ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
-    }
-}
-
 __device__ __forceinline__ float read_lane(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
@@ -743,6 +739,135 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
 }
 
 // ===========================================================================
+// Any-collision in the gated two-phase form (two_phase_cc.cuh:57-104,
+// scene_collision.cuh:519-628), for models whose exact (sphere, box) hits
+// need not imply the approximate hit of the same box: approximate spheres
+// that do not enclose the exact ones, or that ride a chain of their own.
+// The approximate pass flags joints over the whole scene (every enabled box,
+// no ACM, then the ESDF layers), then the exact spheres of flagged joints are
+// checked against the whole scene (box and layer ACMs). One lane per state,
+// OBBs in LDS tiles as in k_collide.
+// ===========================================================================
+__global__ __launch_bounds__(64) void k_collide_gated(CollideArgs a) {
+    extern __shared__ float4 s_lds[];
+    float4* s_obb = s_lds;
+    float* s_q = reinterpret_cast<float*>(s_lds + 4 * a.C.obb_cap);
+    const DRobot& R = a.R;
+    const int lane = lane_id();
+    const int J = R.n_joints, D = R.n_dof;
+    int* s_end = reinterpret_cast<int*>(s_q + D * kWave);
+    const bool has_esdf = a.S.n_voxel_layers > 0;
+    const bool own_chain = R.aft != nullptr;
+    const uint8_t* acm = a.S.sphere_obb_acm;
+    const int cap = a.C.obb_cap;
+    int cur_prob = -1, n_obb = 0, o0 = 0, o1 = 0, n_tiles = 1;
+    int4 ch;
+    ChunkCursor cur = chunk_cursor();
+    const long n_states = state_count(a.X);
+    const int n_chunks = chunk_count(a.C, n_states);
+    while (next_chunk(a.C, n_states, n_chunks, cur, ch)) {
+        if (ch.x != cur_prob) {
+            problem_obbs(a.C, a.S, ch.x, o0, o1);
+            n_tiles = uniform(obb_tiles(o0, o1, cap));
+            if (n_tiles == 1) {
+                __syncthreads();
+                n_obb = uniform(stage_obbs(a.S, o0, o1, s_obb, cap));
+            }
+            cur_prob = ch.x;
+        }
+        const long g = (long)ch.y + lane;
+        bool active = g < ch.z;
+        stage_unit_ends(a.X, ch.w, s_end);
+        __syncthreads();
+        StateRef ref = load_state(a.X, D, g, active, s_q, ch.w, s_end, ch.x);
+        bool hit = false;
+        if (active && a.self_hit && a.self_hit[ref.gs]) { hit = true; active = false; }
+        if (active && a.mode == 1 && a.skip_resolved) {
+            const unsigned fh = (unsigned)a.first_hit[ref.unit];
+            if (fh < (unsigned)ref.k) active = false;
+        }
+        __syncthreads();
+        // 1. approximate pass: bit jj of `flagged` once an approximate sphere
+        // of joint jj (its own chain's frame) hits an enabled box or a layer
+        unsigned flagged = 0u;
+        for (int t = 0; t < n_tiles && wave_any(active); ++t) {
+            if (n_tiles > 1) {
+                const int t0 = o0 + t * cap;
+                __syncthreads();
+                n_obb = uniform(stage_obbs(a.S, t0, min(o1, t0 + cap), s_obb, cap));
+            }
+            const bool last = t == n_tiles - 1;
+            Slots sl;
+            for (int jj = 0; jj < J; ++jj) {
+                float T[12];
+                int4 meta;
+                if (own_chain) fk_joint<true>(R, jj, s_q + lane, sl, T, meta);
+                else fk_joint(R, jj, s_q + lane, sl, T, meta);
+                for (int ai = R.apx_off[jj]; ai < R.apx_off[jj + 1]; ++ai) {
+                    const float4 as = R.apx[ai];
+                    float x, y, z;
+                    aff_point(T, as.x, as.y, as.z, x, y, z);
+                    const float rr = as.w + a.margin;
+                    bool f = !active || ((flagged >> jj) & 1u) != 0u;   // nothing left to find
+                    for (int o = 0; o < n_obb && wave_any(!f); ++o) {
+                        float lx, ly, lz;
+                        to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], x, y, z, lx, ly, lz);
+                        f = f || box_hit(lx, ly, lz, s_obb[4 * o + 3], rr * rr);
+                    }
+                    if (last && has_esdf && !f) f = esdf_hit(a.S, x, y, z, rr, -1);
+                    if (active && f) flagged |= 1u << jj;
+                }
+            }
+        }
+        // 2. exact pass over the flagged joints' spheres
+        for (int t = 0; t < n_tiles && wave_any(active && flagged != 0u); ++t) {
+            if (n_tiles > 1) {
+                const int t0 = o0 + t * cap;
+                __syncthreads();
+                n_obb = uniform(stage_obbs(a.S, t0, min(o1, t0 + cap), s_obb, cap));
+            }
+            const bool last = t == n_tiles - 1;
+            Slots sl;
+            for (int jj = 0; jj < J && wave_any(active); ++jj) {
+                float T[12];
+                int4 meta;
+                fk_joint(R, jj, s_q + lane, sl, T, meta);
+                const bool fj = active && ((flagged >> jj) & 1u) != 0u;
+                if (!wave_any(fj)) continue;
+                for (int s = R.sph_off[jj]; s < R.sph_off[jj + 1]; ++s) {
+                    const float4 sp = R.sph[s];
+                    float x, y, z;
+                    aff_point(T, sp.x, sp.y, sp.z, x, y, z);
+                    const float rr = sp.w + a.margin;
+                    const uint8_t* acm_row = acm ? acm + (size_t)R.sph_id[s] * a.S.n_obbs : nullptr;
+                    bool hs = false;
+                    for (int o = 0; o < n_obb && wave_any(fj && !hs); ++o) {
+                        const float4 h = s_obb[4 * o + 3];
+                        if (acm_row && acm_row[__float_as_int(h.w)] == 0) continue;
+                        float lx, ly, lz;
+                        to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], x, y, z, lx, ly, lz);
+                        hs = hs || (fj && box_hit(lx, ly, lz, h, rr * rr));
+                    }
+                    if (last && has_esdf && fj && !hs) hs = esdf_hit(a.S, x, y, z, rr, R.sph_id[s]);
+                    hit = hit || hs;
+                }
+                active = active && !hit;
+            }
+        }
+        if (g < ch.z && a.out) a.out[ref.gs] = hit ? 1 : 0;
+        if (a.mode == 1) {
+            const bool rep = hit && g < ch.z;
+            const unsigned long long hm = __ballot(rep);
+            if (rep) {
+                const int l_e0 = ref.u0 > ch.y ? (int)(ref.u0 - ch.y) : 0;
+                const unsigned long long same = ((1ull << lane) - 1ull) & ~((1ull << l_e0) - 1ull);
+                if ((hm & same) == 0ull) atomicMin((unsigned*)&a.first_hit[ref.unit], (unsigned)ref.k);
+            }
+        }
+    }
+}
+
+// ===========================================================================
 // Summed-hinge state cost (+ adjoint). chomp.cu:151-188 / stomp.cu:206-246
 // cost = w * (sum_s relu(m - min_o clr(s,o)) + sum_s relu(m - esdf clr(s))
 //             + relu(m_self - min_pairs clr)), raw radii.
@@ -754,8 +879,8 @@ __global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
 // ===========================================================================
 // Candidate OBBs of one joint for the hinge: a wave-uniform bitmask of the
 // OBBs some active lane's spheres of this joint can come within the hinge
-// band of. Every sphere s lies inside the joint's approximate sphere (centre
-// a, radius R), so clr(s, o) >= sdf_o(a) - R; the active lanes' centres lie
+// band of. Every sphere s lies inside the joint's enclosing sphere (centre
+// a, radius R; DRobot::jenc), so clr(s, o) >= sdf_o(a) - R; the active lanes' centres lie
 // within rw of the first active lane's centre c0, so sdf_o(a) >= sdf_o(c0) -
 // rw. An OBB with sdf_o(c0) >= band + R + rw therefore has clearance >= band
 // from every sphere: it cannot change a hinge, its argmin or its gradient.
@@ -922,11 +1047,7 @@ __global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
             const int s0 = R.sph_off[jj], s1 = R.sph_off[jj + 1];
             if (s1 > s0) {
                 ObbMask mk;
-                if (use_obb) {
-                    const int a0 = R.apx_off[jj], a1 = R.apx_off[jj + 1];
-                    if (a1 - a0 == 1) near_obbs(s_obb, n_obb, T, R.apx[a0], a.P.world_margin, active, mk);
-                    else mask_all(n_obb, mk);   // no single enclosing sphere: every OBB is a candidate
-                }
+                if (use_obb) near_obbs(s_obb, n_obb, T, R.jenc[jj], a.P.world_margin, active, mk);
                 for (int s = s0 + sub; s < s1; s += G) {
                     const float4 sp = R.sph[s];
                     float px, py, pz;
@@ -1046,7 +1167,9 @@ __global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
 // link, until P is the root. Then A is the product of the steps from the root
 // down to jj: the frame. ceil(log2(depth)) rounds through LDS. qs: the
 // state's q (q_d = qs[d * qstride]); par: J ints of LDS for the links.
+// APPROX: the approximate spheres' own chain (R.aft / R.atype).
 // Contains block barriers: every lane of the workgroup must call it.
+template <bool APPROX = false>
 __device__ __forceinline__ void group_frames(const DRobot& R, const float* qs, int qstride, float* F, int* par,
                                              int sub) {
     const int J = R.n_joints;
@@ -1059,7 +1182,7 @@ __device__ __forceinline__ void group_frames(const DRobot& R, const float* qs, i
         } else {
             const int4 meta = R.jmeta[sub];
             const float q = meta.y >= 0 ? qs[meta.y * qstride] : 0.0f;
-            joint_step(R.ft + 12 * sub, meta.x, q, A);
+            joint_step(APPROX ? R.aft + 12 * sub : R.ft + 12 * sub, APPROX ? R.atype[sub] : meta.x, q, A);
             P = R.parent[sub];
         }
         float4* d = reinterpret_cast<float4*>(F + 12 * sub);
@@ -1454,6 +1577,8 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
     float* s_q = s_T + kSelfStates * fstride;                         // [kMaxDof][kSelfStates]
     int* s_par = reinterpret_cast<int*>(s_q + kMaxDof * kSelfStates); // [kSelfStates][J] pointer-jumping links
     int* s_end = s_par + kSelfStates * J;                             // [2][64] unit ends / starts of a chunk
+    float4* s_apos = reinterpret_cast<float4*>(s_end + 2 * kWave);    // gated mode 0: [kSelfStates][A] approx spheres
+    const bool gated = a.mode == 0 && !R.self_gate_implied;
     const long n_states = state_count(a.X);
     // With the plan's chunk table a chunk names its first unit (staged unit
     // ends make the state -> unit lookup an LDS bisection) or, waypoint-major,
@@ -1522,6 +1647,35 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
                                fmaf(t2.x, je.x, fmaf(t2.y, je.y, fmaf(t2.z, je.z, t2.w))), je.w);
     }
     __syncthreads();
+    bool live = active;   // states whose pairs are checked (gated: an approximate pair collides)
+    if (gated) {
+        // Approximate pairs first (two_phase_cc.cuh:114-131): a state none of
+        // whose approximate pairs collide has no self collision. Their frames
+        // come from the approximate chain when the model has its own (the
+        // exact positions above no longer need the full chain's).
+        if (R.aft) {
+            group_frames<true>(R, s_q + st, kSelfStates, Ts, s_par + st * J, sub);
+        }
+        float4* apos = s_apos + st * R.n_apx;
+        for (int k = sub; k < R.n_apx; k += kSelfGroup) {
+            const float4 sp = R.apx_orig[k];
+            const float4* Tq = reinterpret_cast<const float4*>(Ts + 12 * R.apx_jnt[k]);
+            const float4 t0 = Tq[0], t1 = Tq[1], t2 = Tq[2];
+            apos[k] = make_float4(fmaf(t0.x, sp.x, fmaf(t0.y, sp.y, fmaf(t0.z, sp.z, t0.w))),
+                                  fmaf(t1.x, sp.x, fmaf(t1.y, sp.y, fmaf(t1.z, sp.z, t1.w))),
+                                  fmaf(t2.x, sp.x, fmaf(t2.y, sp.y, fmaf(t2.z, sp.z, t2.w))), sp.w);
+        }
+        __syncthreads();
+        bool ah = false;
+        for (int p = sub; p < R.n_apx_pairs && live; p += kSelfGroup) {
+            const int2 pr = R.apx_pairs[p];
+            const float4 u = apos[pr.x], v = apos[pr.y];
+            const float ux = u.x - v.x, uy = u.y - v.y, uz = u.z - v.z;
+            const float r2 = u.w + v.w;
+            ah = ah || fmaf(uz, uz, fmaf(uy, uy, ux * ux)) - r2 * r2 < 0.0f;
+        }
+        live = live && ((__ballot(ah) >> (st * kSelfGroup)) & 0xFFFFull) != 0ull;
+    }
     // Pairs come in blocks: rows = up to 64 spheres of joint A, columns = up
     // to 64 spheres of joint B, a 64-bit mask of partners per row. Each
     // joint's enclosing sphere contains all of its spheres, which culls at
@@ -1556,7 +1710,7 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
     for (int b = 0; b < R.n_sblk; ++b) {
         const int4 h = R.sblk[2 * b], c = R.sblk[2 * b + 1];   // (jA, jB, a0, na), (b0, nb, mask offset, -)
         const float4 ea = jnt[h.x], eb = jnt[h.y];
-        const bool gate = active && !hit && near(make_float4(ea.x, ea.y, ea.z, ea.w < 0.0f ? 0.0f : ea.w), eb);
+        const bool gate = live && !hit && near(make_float4(ea.x, ea.y, ea.z, ea.w < 0.0f ? 0.0f : ea.w), eb);
         if (!wave_any(gate)) continue;
         const unsigned long long ma = gate ? near_mask(h.z, h.w, eb) : 0ull;
         const unsigned long long mb = gate && ma ? near_mask(c.x, c.y, ea) : 0ull;
@@ -1591,7 +1745,7 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
         if (a.mode == 0) {
             // share the state's verdict so its lanes stop
             hit = ((__ballot(hit) >> (st * kSelfGroup)) & 0xFFFFull) != 0ull;
-            if (__ballot(active && !hit) == 0ull) break;
+            if (__ballot(live && !hit) == 0ull) break;
         }
     }
     if (a.mode == 0) {
@@ -2320,7 +2474,10 @@ int grid_cost(int n_joints, bool grad, size_t lds) {
 }
 
 hipError_t launch_collide(const CollideArgs& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_collide, dim3(grid), dim3(64), lds_bytes(a.C.obb_cap, a.R.n_dof), s, a);
+    if (a.R.scene_gate_fused)
+        hipLaunchKernelGGL(k_collide, dim3(grid), dim3(64), lds_bytes(a.C.obb_cap, a.R.n_dof), s, a);
+    else
+        hipLaunchKernelGGL(k_collide_gated, dim3(grid), dim3(64), lds_bytes(a.C.obb_cap, a.R.n_dof), s, a);
     return hipGetLastError();
 }
 
@@ -2381,10 +2538,11 @@ hipError_t launch_self(const SelfArgs& a, hipStream_t s) {
     long blocks = a.C.tab ? (long)a.C.n_chunks : (a.X.n_states + kSelfStates - 1) / kSelfStates;
     if (a.X.n_dyn && blocks > 16384) blocks = 16384;
     if (blocks <= 0) return hipSuccess;
+    const bool gate = a.mode == 0 && !a.R.self_gate_implied;   // approximate pair positions
     const size_t smem = sizeof(float4) * (size_t)kSelfStates * (a.R.n_sph + a.R.n_joints) +
                         sizeof(float) * (size_t)kSelfStates * (a.R.n_joints * 12 + 4) +
                         sizeof(float) * (size_t)kMaxDof * kSelfStates + sizeof(int) * (size_t)kSelfStates * a.R.n_joints +
-                        sizeof(int) * 2 * kWave;
+                        sizeof(int) * 2 * kWave + (gate ? sizeof(float4) * (size_t)kSelfStates * a.R.n_apx : 0);
     hipLaunchKernelGGL(k_self, dim3((unsigned)blocks), dim3(64), smem, s, a);
     return hipGetLastError();
 }

@@ -60,6 +60,20 @@ struct DRobot {
     const float4* jenc;         // [J] link-frame sphere containing every sphere of DFS joint k (w < 0: none)
     const int* sph_jnt;         // [S] DFS joint of emission slot k
     const int* jend;            // [J] end of DFS position k's subtree: its DFS positions are [k, jend[k])
+    // The approximate spheres' own chain (DFS order), NULL when they ride the
+    // full chain: rows 0..2 of each fixed transform and the joint types.
+    const float* aft;
+    const int* atype;
+    const int* apx_jnt;         // [A] DFS joint of approximate sphere a (original index)
+    // 1: every exact sphere lies inside an approximate sphere of its joint on
+    // the same chain, so an exact (sphere, OBB) hit implies the approximate
+    // hit of that OBB and the any-collision kernel gates each (joint, OBB)
+    // pair by itself; 0: the gated form (joint flags from the approximate
+    // pass over the whole scene, then the flagged joints' exact spheres).
+    int scene_gate_fused;
+    // 1: an exact self hit implies an approximate one (every exact pair lies in
+    // an approximate pair); 0: the self verdict needs the approximate pass too.
+    int self_gate_implied;
 };
 
 // Non-owning scene pointers, SceneCollisionData semantics.

@@ -40,8 +40,7 @@
 
 namespace ppl {   // capi.hip
 ppl_status fail_msg(ppl_status s, const char* msg);          // sets ppl_last_error
-ppl_status scene_view_check(const ppl_scene_view* v);        // pointers, alignment, OBB limit
-bool robot_self_gate_implied(ppl_robot r);                   // exact self pairs covered by approximate ones
+ppl_status scene_view_check(const ppl_scene_view* v);        // pointers and alignment
 }
 
 namespace {
@@ -910,9 +909,6 @@ ppl_status rrtc_solve(ppl_robot robot, const ppl_scene_view* scene, const float*
     if ((st = ppl::scene_view_check(scene)) != PPL_OK) return st;
     if (scene->esdf_interp != 0)
         return ppl::fail_msg(PPL_ERR_UNSUPPORTED, "pRRTC checks use the nearest-voxel ESDF lookup (esdf_interp = 0)");
-    if (R.n_pairs > 0 && !ppl::robot_self_gate_implied(robot))
-        return ppl::fail_msg(PPL_ERR_UNSUPPORTED,
-                             "the approximate self pairs do not cover the exact ones: the gated self check would miss hits");
     // lanes per edge state: the widest power of two that gives every state its own group
     int G = 64;
     while (G > 1 && G * cfg->granularity > kThreads) G >>= 1;

@@ -176,6 +176,11 @@ class RobotArrays:
         self.joint_id_to_dof = i32(asset["joint_id_to_dof"])
         self.t_memory_idx = i32(asset["t_memory_idx"])
         self.dfs_order = i32(asset["dfs_order"])
+        # the approximate spheres' own chain (RobotModel approx_fixed_transforms /
+        # approx_joint_types); None: they ride the full chain
+        aft, aty = asset.get("approx_fixed_transforms"), asset.get("approx_joint_types")
+        self.approx_fixed_transforms = f32(aft) if aft is not None and len(aft) else None
+        self.approx_joint_types = i32(aty) if aty is not None and len(aty) else None
 
     @property
     def n_spheres(self) -> int:

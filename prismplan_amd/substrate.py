@@ -78,6 +78,12 @@ class Robot:
         d.joint_id_to_dof = _p(arr.joint_id_to_dof, _i32p)
         d.t_memory_idx = _p(arr.t_memory_idx, _i32p)
         d.dfs_order = _p(arr.dfs_order, _i32p)
+        if arr.approx_fixed_transforms is not None:
+            keep.append(arr.approx_fixed_transforms)
+            d.approx_fixed_transforms = _p(arr.approx_fixed_transforms, _f32p)
+        if arr.approx_joint_types is not None:
+            keep.append(arr.approx_joint_types)
+            d.approx_joint_types = _p(arr.approx_joint_types, _i32p)
         h = C.c_void_p()
         call("ppl_robot_create", C.byref(d), C.byref(h))
         self.handle = h

@@ -56,6 +56,10 @@ def _desc(r, **over):
     d.dfs_order = i32(over.get("dfs", r.dfs_order))
     d.joint_id_to_dof = i32(r.joint_id_to_dof)
     d.t_memory_idx = i32(over.get("tmem", r.t_memory_idx))
+    if "aft" in over:
+        d.approx_fixed_transforms = f32(over["aft"])
+    if "atypes" in over:
+        d.approx_joint_types = i32(over["atypes"])
     if "n_dof" in over:
         d.n_dof = over["n_dof"]
     return d, keep
@@ -69,6 +73,10 @@ def _desc(r, **over):
     ({"parents": [0, 0, 1, 2, 3, 4, 7, 6]}, _ppl.PPL_ERR_INVALID),       # child before parent
     ({"types": [3, 5, 5, 9, 5, 5, 5, 5]}, _ppl.PPL_ERR_INVALID),         # unknown joint type
     ({"tmem": [0, 1, 2, 3, 1, 1, 1, 1]}, _ppl.PPL_ERR_UNSUPPORTED),      # more FK slots than supported
+    ({"aft": np.where(np.arange(128) == 28, 0.5, robot_arrays("panda").fixed_transforms)},
+     _ppl.PPL_ERR_INVALID),                                             # approximate chain not affine
+    ({"aft": robot_arrays("panda").fixed_transforms, "atypes": [3, 5, 5, 5, 9, 5, 5, 5]},
+     _ppl.PPL_ERR_INVALID),                                             # unknown approximate joint type
 ])
 def test_robot_descriptor_validation(over, status):
     r = robot_arrays("panda")

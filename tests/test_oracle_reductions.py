@@ -486,3 +486,51 @@ def test_value_ambiguity_flags_only_jumps_that_matter():
         assert not hamb[0]
         _, hamb = o.hinge_cost_amb(q, PO.cost_params(world_margin=2.0, use_obb=False, use_self=False))
         assert bool(hamb[0]) is flagged
+
+
+# ---------------------------------------------------------------------------
+# the two-phase gate on models where it decides verdicts
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("kind", ["shrunk", "own_chain", "sparse_self"])
+def test_two_phase_gate_recomposed(kind):
+    """check_collision's default mode is the reference's two-phase gate
+    (two_phase_cc.cuh:57-138, scene_collision.cuh:519-628): a joint is
+    flagged when one of its approximate spheres (on the approximate chain,
+    radius + margin, no ACM) overlaps an enabled box; the exact spheres of
+    flagged joints are checked against every enabled box; the exact self
+    pairs count only when an approximate self pair collides. Rebuilt in numpy
+    from the pinned FK (the model's, and its approximate spheres' as a robot
+    of their own) and the pinned box clearance, on models where the gate
+    changes verdicts: the ungated verdict (every exact sphere, every exact
+    self pair) differs from the gated one on some states, so an oracle that
+    skipped the gate, or walked the approximate spheres on the full chain,
+    would fail here."""
+    from parity_common import approx_model, box_clearances, model_variant
+    r = RobotArrays(model_variant("panda", kind))
+    ra = approx_model(r)
+    scene = world(131, 12, (0.1, -0.8, 0.0), (1.0, 0.8, 1.2), half_hi=0.12)
+    q = W.uniform_configs(np.random.default_rng(132), r, 3000)
+    o, oa = PO.Oracle(r, scene, nthreads=4), PO.Oracle(ra, scene, nthreads=4)
+    en = scene.obb_enable.astype(bool)
+    pairs = lambda rng_: np.array([(i, j) for i, a, b in rng_ for j in range(a, b + 1)], np.int64).reshape(-1, 2)
+    pe, pa = pairs(r.self_ranges), pairs(r.approx_self_ranges)
+    P, A = o.fk_spheres(q), oa.fk_spheres(q)
+    rs, ra_ = r.spheres[:, 3].astype(np.float64), r.approx_spheres[:, 3].astype(np.float64)
+    de = np.linalg.norm(P[:, pe[:, 0]] - P[:, pe[:, 1]], axis=-1) - rs[pe[:, 0]] - rs[pe[:, 1]]
+    da = np.linalg.norm(A[:, pa[:, 0]] - A[:, pa[:, 1]], axis=-1) - ra_[pa[:, 0]] - ra_[pa[:, 1]]
+    for margin in (0.0, 0.01):
+        hit, _ = o.check_collision(q, margin=margin, check_self=True)
+        m = np.float32(margin)
+        ce = box_clearances(scene, P, np.broadcast_to(rs + m, P.shape[:2]))[..., en]   # [n, S, O]
+        ca = box_clearances(scene, A, np.broadcast_to(ra_ + m, A.shape[:2]))[..., en]  # [n, A, O]
+        flag = np.zeros((len(q), r.n_joints), bool)
+        for a_ in range(r.n_approx):
+            flag[:, r.approx_sphere_to_joint[a_]] |= (ca[:, a_] < 0).any(axis=1)
+        scene_hit = ((ce < 0).any(axis=2) & flag[:, r.sphere_to_joint]).any(axis=1)
+        want = scene_hit | ((da < 0).any(axis=1) & (de < 0).any(axis=1))
+        tie = (np.abs(ce) < 1e-9).any(axis=(1, 2)) | (np.abs(ca) < 1e-9).any(axis=(1, 2)) | \
+              (np.abs(de) < 1e-9).any(axis=1) | (np.abs(da) < 1e-9).any(axis=1)
+        np.testing.assert_array_equal(hit[~tie], want[~tie])
+        ungated = (ce < 0).any(axis=(1, 2)) | (de < 0).any(axis=1)
+        assert (ungated != want).sum() > 0, f"{kind}: the gate changes no verdict at margin {margin}"
+        assert 0.05 < want.mean() < 0.95
