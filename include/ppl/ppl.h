@@ -278,8 +278,11 @@ ppl_status ppl_hinge_cost_batched(ppl_plan plan, const ppl_scene_view* scene, co
  * per wavefront, a state's spheres split over G lanes) that fits in LDS and
  * for the grouped layout (*best_group = PPL_COST_GROUPED), and keeps the
  * fastest for this plan's later hinge / STOMP calls. reps timed runs per
- * candidate on `stream`; outputs go to plan scratch. No counterpart in the
- * reference. */
+ * candidate on `stream`; outputs go to plan scratch. The choice is also
+ * recorded in the tuning cache (ppl_tune_cache_file) under the robot's
+ * tables, the largest problem's OBB count, the state count's power of two
+ * and the GPU architecture; untuned plans that match such an entry use it.
+ * No counterpart in the reference. */
 ppl_status ppl_plan_autotune_cost(ppl_plan plan, const ppl_scene_view* scene, const float* d_x,
                                   const ppl_cost_params* params, int grad, int reps, int* best_group,
                                   float* best_ms, ppl_stream stream);
@@ -291,6 +294,40 @@ ppl_status ppl_plan_autotune_cost(ppl_plan plan, const ppl_scene_view* scene, co
  * LDS), or 0 for the built-in choice (grouped for the adjoint). */
 #define PPL_COST_GROUPED (-1)
 ppl_status ppl_plan_set_cost_group(ppl_plan plan, int grad, int lanes_per_state);
+
+/* Tuning cache file: the autotuners' choices, one "key value..." line each,
+ * read at the first lookup and rewritten at each new choice, so later plans
+ * and processes skip the timing. Default: $PPL_TUNE_CACHE, else
+ * $XDG_CACHE_HOME/ppl/tune.txt, else $HOME/.cache/ppl/tune.txt; NULL or ""
+ * keeps choices in memory only. Setting it drops the entries read so far.
+ * No counterpart in the reference. */
+ppl_status ppl_tune_cache_file(const char* path);
+
+/* Autotuner of the any-collision kernel's launch shape for a configuration
+ * plan (d_x: configurations) or an edge plan (d_x, d_x2: the edges' start and
+ * end configurations; first-hit mode with skip_resolved): times reps runs of
+ * the plan's batched check on `stream` for each OBB staging tile (the largest
+ * problem's OBBs in one staging, then 128, 64 and 32 records below that; the
+ * stagings of a problem then pass through LDS in turn) and each grid form
+ * (0: persistent at occupancy, 1: twice that, 2: one workgroup per 64-state
+ * chunk), keeps the fastest for this plan's later any-collision calls and
+ * records it in the tuning cache under the robot's tables, the largest
+ * problem's OBB count and the GPU architecture. The fixed tile it replaces
+ * is the reference's BATCH_SIZE (collision/scene_collision.cuh:9-11). Results
+ * do not depend on the shape. No counterpart in the reference. */
+ppl_status ppl_plan_autotune_collide(ppl_plan plan, const ppl_scene_view* scene, const float* d_x, const float* d_x2,
+                                     int reps, int* best_tile, int* best_grid, float* best_ms, ppl_stream stream);
+
+/* The any-collision launch shape this plan uses for problems of up to
+ * max_obbs OBBs (*tile records per staging, *grid form as above) and where it
+ * comes from (*source: 0 built-in, 1 tuning cache, 2 this plan's tuning or
+ * ppl_plan_set_collide_shape). */
+ppl_status ppl_plan_collide_shape(ppl_plan plan, int max_obbs, int* tile, int* grid, int* source);
+
+/* Pin the any-collision launch shape (tile 0: every OBB of a problem in one
+ * staging, up to 256; grid 0, 1, 2 as above); (0, 0) returns to the tuning
+ * cache / built-in choice. */
+ppl_status ppl_plan_set_collide_shape(ppl_plan plan, int tile, int grid);
 
 /* Central-FD obstacle gradient over a trajectory plan (chomp_obs_grad_kernel
  * semantics), the baseline for the fused adjoint. */

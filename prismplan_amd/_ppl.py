@@ -153,6 +153,11 @@ SIGNATURES = [
     ("ppl_plan_autotune_cost", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), C.c_int, C.c_int,
                                          _i32p, _f32p, _vp]),
     ("ppl_plan_set_cost_group", C.c_int, [_vp, C.c_int, C.c_int]),
+    ("ppl_tune_cache_file", C.c_int, [C.c_char_p]),
+    ("ppl_plan_autotune_collide", C.c_int, [_vp, C.POINTER(SceneView), _vp, _vp, C.c_int, C.POINTER(C.c_int),
+                                            C.POINTER(C.c_int), C.POINTER(C.c_float), _vp]),
+    ("ppl_plan_collide_shape", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("ppl_plan_set_collide_shape", C.c_int, [_vp, C.c_int, C.c_int]),
     ("ppl_plan_set_optimizer_graphs", C.c_int, [_vp, C.c_int]),
     ("ppl_chomp_fd_grad_batched", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), C.c_float, _vp,
                                             _vp]),

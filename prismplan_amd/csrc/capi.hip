@@ -15,10 +15,14 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
+#include <unistd.h>
 
 using namespace ppl;
 
@@ -101,6 +105,7 @@ struct ppl_robot_s {
     int n_revolute = 0, n_prismatic = 0;   // joints that move with a dof
     std::vector<float> lower, upper;
     std::vector<std::pair<int, ppl_plan>> traj_plans;   // single-trajectory plans of the plain CHOMP forms, by T
+    unsigned long long digest = 0;   // FNV-1a of the device tables: the robot's key in the tuning cache
     hipStream_t stream = nullptr;
     DBuf d_cfg, d_out, d_off, d_aux, d_self, d_mit, d_hint;
     HBuf h_stage, h_mit;
@@ -165,6 +170,11 @@ struct ppl_plan_s {
     int* d_obb_off = nullptr;
     int wmajor = 0;                  // waypoint-major trajectory plan
     int tuned_group[2] = {0, 0};     // cost kernel lanes per state chosen by ppl_plan_autotune_cost (0: heuristic)
+    // any-collision launch shape (OBB records per LDS staging, grid form) for
+    // problems of up to shape_obbs OBBs; shape_src 0 built-in, 1 tuning cache,
+    // 2 this plan's tuning or setting (kept whatever the OBB count)
+    int ctile = 0, cgrid = 0, shape_obbs = -1, shape_src = 0;
+    int cost_obbs[2] = {-1, -1}, cost_cached[2] = {0, 0};   // cost layout from the tuning cache, per OBB count
     int4* d_prob = nullptr;          // wmajor: per problem (first trajectory, trajectories, length, first state)
     DBuf d_self, d_cost, d_fdcfg, d_fdcost, d_tune;
     // CHOMP optimiser: A^-1 (FP32, T x T) per trajectory length, the
@@ -257,6 +267,111 @@ hipError_t launch_cost_flat(CostArgs& a, long n_states, hipStream_t s) {
     return tiled ? launch_cost_grp(a, grid, s) : launch_cost(a, grid, s);
 }
 
+// ---------------------------------------------------------------------------
+// Tuning cache: the launch shapes the autotuners choose, kept in a text file
+// ("key value..." per line) so that later plans and processes skip the
+// timing. A key names the kernel, the robot tables' digest, the largest
+// problem's OBB count, (cost kernel) the state count's power of two, and the
+// GPU architecture. The file is $PPL_TUNE_CACHE, else
+// $XDG_CACHE_HOME/ppl/tune.txt, else $HOME/.cache/ppl/tune.txt; an empty
+// path (or ppl_tune_cache_file(NULL)) keeps the cache in memory only.
+// ---------------------------------------------------------------------------
+std::mutex g_tune_mu;
+bool g_tune_loaded = false, g_tune_path_set = false;
+std::string g_tune_path;
+std::map<std::string, std::string> g_tune;
+
+std::string default_tune_path() {
+    if (const char* e = getenv("PPL_TUNE_CACHE")) return e;
+    const char* xdg = getenv("XDG_CACHE_HOME");
+    if (xdg && *xdg) return std::string(xdg) + "/ppl/tune.txt";
+    const char* home = getenv("HOME");
+    return home && *home ? std::string(home) + "/.cache/ppl/tune.txt" : std::string();
+}
+
+void tune_load_locked() {
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    if (!g_tune_path_set) g_tune_path = default_tune_path();
+    if (g_tune_path.empty()) return;
+    FILE* f = fopen(g_tune_path.c_str(), "r");
+    if (!f) return;
+    char line[512];
+    while (fgets(line, sizeof line, f)) {
+        std::string l(line);
+        while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
+        const size_t sp = l.find(' ');
+        if (l.empty() || l[0] == '#' || sp == std::string::npos || sp == 0) continue;
+        g_tune[l.substr(0, sp)] = l.substr(sp + 1);
+    }
+    fclose(f);
+}
+
+bool tune_get(const std::string& key, std::string& val) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load_locked();
+    const auto it = g_tune.find(key);
+    if (it == g_tune.end()) return false;
+    val = it->second;
+    return true;
+}
+
+// Records key -> val and rewrites the file (parent directories created, a
+// sibling temporary renamed over it). The cache only saves timing, so a file
+// that cannot be written leaves the in-memory entry alone.
+void tune_put(const std::string& key, const std::string& val) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load_locked();
+    g_tune[key] = val;
+    if (g_tune_path.empty()) return;
+    for (size_t k = 1; (k = g_tune_path.find('/', k)) != std::string::npos; ++k)
+        (void)mkdir(g_tune_path.substr(0, k).c_str(), 0755);
+    const std::string tmp = g_tune_path + ".tmp." + std::to_string((long)getpid());
+    FILE* f = fopen(tmp.c_str(), "w");
+    if (!f) return;
+    fprintf(f, "# ppl tuning cache: key value...\n");
+    for (const auto& e : g_tune) fprintf(f, "%s %s\n", e.first.c_str(), e.second.c_str());
+    const bool ok = fclose(f) == 0;
+    if (!ok || rename(tmp.c_str(), g_tune_path.c_str()) != 0) (void)remove(tmp.c_str());
+}
+
+// Architecture of the current device ("gfx950"), without feature suffixes.
+std::string gpu_arch() {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    static std::mutex mu;
+    static std::map<int, std::string> names;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = names.find(dev);
+    if (it != names.end()) return it->second;
+    hipDeviceProp_t prop;
+    std::string a = "unknown";
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        a = prop.gcnArchName;
+        const size_t c = a.find(':');
+        if (c != std::string::npos) a.resize(c);
+    }
+    names[dev] = a;
+    return a;
+}
+
+std::string hex64(unsigned long long v) {
+    char b[24];
+    snprintf(b, sizeof b, "%016llx", v);
+    return b;
+}
+
+std::string collide_key(const ppl_robot_s* r, int max_obbs) {
+    return "collide/" + hex64(r->digest) + "/" + std::to_string(max_obbs) + "/" + gpu_arch();
+}
+
+std::string cost_key(const ppl_robot_s* r, bool grad, int max_obbs, long n_states) {
+    int lg = 0;
+    while ((1L << (lg + 1)) <= n_states) ++lg;
+    return std::string(grad ? "adjoint/" : "cost/") + hex64(r->digest) + "/" + std::to_string(max_obbs) + "/" +
+           std::to_string(lg) + "/" + gpu_arch();
+}
+
 int group_index(int G) { return G >= 32 ? 5 : G >= 16 ? 4 : G >= 8 ? 3 : G >= 4 ? 2 : G >= 2 ? 1 : 0; }
 
 // Chunk table of a plan for `G` lanes per state (64 / G states per chunk),
@@ -293,6 +408,45 @@ DChunks plan_chunks(const ppl_plan_s* P, KernelKind k, int max_obbs, int G, int*
     C.chunk = kWave / G;
     C.obb_off = P->d_obb_off;
     *grid_out = std::max(1, std::min(grid_cap, C.n_chunks));
+    return C;
+}
+
+// The any-collision launch shape of plan P for problems of up to max_obbs
+// OBBs: this plan's tuned or set shape, else the tuning cache's entry for the
+// robot, OBB count and GPU, else the built-in one (every OBB of a problem in
+// one staging, persistent grid at occupancy). tile: OBB records per LDS
+// staging; gmode 0 persistent grid at occupancy, 1 twice that, 2 one
+// workgroup per chunk.
+void collide_shape(ppl_plan_s* P, int max_obbs, int* tile, int* gmode) {
+    if (P->shape_src != 2 && P->shape_obbs != max_obbs) {
+        P->shape_obbs = max_obbs;
+        P->ctile = P->cgrid = P->shape_src = 0;
+        std::string v;
+        int t = 0, g = 0;
+        if (tune_get(collide_key(P->robot, max_obbs), v) && sscanf(v.c_str(), "%d %d", &t, &g) == 2 && t >= 0 &&
+            t <= kMaxObbLds && g >= 0 && g <= 2) {
+            P->ctile = t;
+            P->cgrid = g;
+            P->shape_src = 1;
+        }
+    }
+    *tile = P->ctile > 0 ? std::min(P->ctile, obb_tile(max_obbs)) : obb_tile(max_obbs);
+    *gmode = P->cgrid;
+}
+
+DChunks plan_collide_chunks(ppl_plan_s* P, int max_obbs, int* grid_out) {
+    int tile = 0, gm = 0;
+    collide_shape(P, max_obbs, &tile, &gm);
+    DChunks C{};
+    C.obb_cap = tile;
+    C.tiled = max_obbs > tile;
+    C.tab = P->d_tab[0];
+    C.n_chunks = P->n_chunks[0];
+    C.chunk = kWave;
+    C.obb_off = P->d_obb_off;
+    const int occ = grid_collide(lds_bytes(tile, P->robot->R.n_dof));
+    const int g = gm == 2 ? C.n_chunks : (gm == 1 ? 2 * occ : occ);
+    *grid_out = std::max(1, std::min(g, C.n_chunks));
     return C;
 }
 
@@ -711,6 +865,8 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
 
     ppl_robot_s* r = new (std::nothrow) ppl_robot_s();
     if (!r) return fail(PPL_ERR_ALLOC, "out of host memory");
+    r->digest = 1469598103934665603ull;
+    for (char c : host) r->digest = (r->digest ^ (unsigned char)c) * 1099511628211ull;
     if (hipMalloc(&r->blob, host.size()) != hipSuccess) {
         delete r;
         return fail(PPL_ERR_ALLOC, "hipMalloc of %zu bytes for robot tables failed", host.size());
@@ -1266,10 +1422,25 @@ ppl_status hinge_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_
     a.S = S;
     a.X = X;
     int grid = 0;
-    const int tuned = P->tuned_group[d_grad ? 1 : 0];
-    // default: the grouped layout for the adjoint, the lane-group heuristic for the cost alone
     // problems with more OBBs than one LDS staging take the grouped kernel (OBB tiles)
     const int max_obbs = plan_obb_cap(P, S);
+    const int gi = d_grad ? 1 : 0;
+    int tuned = P->tuned_group[gi];
+    if (tuned == 0 && force_G == 0) {
+        // the tuning cache's layout for this robot, OBB count, state count and GPU
+        if (P->cost_obbs[gi] != max_obbs) {
+            P->cost_obbs[gi] = max_obbs;
+            P->cost_cached[gi] = 0;
+            std::string v;
+            int g = 0;
+            if (tune_get(cost_key(P->robot, gi != 0, max_obbs, P->n_states), v) && sscanf(v.c_str(), "%d", &g) == 1 &&
+                (g == PPL_COST_GROUPED || (g >= 1 && g <= kMaxGroup && (g & (g - 1)) == 0)) &&
+                group_fits(P, g, gi != 0, max_obbs))
+                P->cost_cached[gi] = g;
+        }
+        tuned = P->cost_cached[gi];
+    }
+    // default: the grouped layout for the adjoint, the lane-group heuristic for the cost alone
     const int G = max_obbs > kMaxObbLds ? PPL_COST_GROUPED
                 : force_G != 0          ? force_G
                 : tuned != 0            ? tuned
@@ -2031,7 +2202,7 @@ ppl_status ppl_check_collision_batched(ppl_plan P, const ppl_scene_view* view, c
     a.S = S;
     a.X = X;
     int grid = 0;
-    a.C = plan_chunks(P, kKCollide, plan_obb_cap(P, S), 1, &grid);
+    a.C = plan_collide_chunks(P, plan_obb_cap(P, S), &grid);
     a.margin = margin;
     a.mode = 0;
     a.out = d_out;
@@ -2069,7 +2240,7 @@ ppl_status ppl_edge_check_batched(ppl_plan P, const ppl_scene_view* view, const 
     a.S = S;
     a.X = X;
     int grid = 0;
-    a.C = plan_chunks(P, kKCollide, plan_obb_cap(P, S), 1, &grid);
+    a.C = plan_collide_chunks(P, plan_obb_cap(P, S), &grid);
     a.margin = prm->collision_margin;
     a.mode = 1;
     a.skip_resolved = prm->skip_resolved;
@@ -2449,8 +2620,119 @@ ppl_status ppl_plan_autotune_cost(ppl_plan P, const ppl_scene_view* view, const 
     (void)hipEventDestroy(e1);
     if (st != PPL_OK) return st;
     P->tuned_group[grad ? 1 : 0] = best;
+    {
+        char v[64];
+        snprintf(v, sizeof v, "%d %.6f", best, best_t / reps);
+        tune_put(cost_key(P->robot, grad != 0, cap, P->n_states), v);
+    }
     if (best_group) *best_group = best;
     if (best_ms) *best_ms = best_t / reps;
+    return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
+}
+
+ppl_status ppl_tune_cache_file(const char* path) try {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_path = path ? path : "";
+    g_tune_path_set = true;
+    g_tune.clear();
+    g_tune_loaded = false;
+    return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
+}
+
+ppl_status ppl_plan_autotune_collide(ppl_plan P, const ppl_scene_view* view, const float* d_x, const float* d_x2,
+                                     int reps, int* best_tile, int* best_grid, float* best_ms, ppl_stream stream) try {
+    CHECK_ARG(P && view && d_x, "NULL argument");
+    CHECK_ARG(P->kind == PPL_PLAN_CONFIGS || P->kind == PPL_PLAN_EDGES, "any-collision tuning needs a configuration or edge plan");
+    CHECK_ARG(P->kind != PPL_PLAN_EDGES || d_x2, "edge plans take the edges' start (d_x) and end (d_x2) configurations");
+    CHECK_ARG(reps >= 1 && reps <= 1000, "reps must be in [1, 1000]");
+    if (P->n_states == 0) return PPL_OK;
+    DScene S{};
+    ppl_status st = plan_scene(P, view, S);
+    if (st != PPL_OK) return st;
+    const int max_obbs = plan_obb_cap(P, S);
+    const bool edges = P->kind == PPL_PLAN_EDGES;
+    HIP_TRY(P->d_tune.ensure(edges ? (size_t)P->n_units * 4 : (size_t)P->n_states));
+    hipStream_t s = (hipStream_t)stream;
+    ppl_edge_batch_params ep{};
+    ep.skip_resolved = 1;
+    auto run = [&]() {
+        return edges ? ppl_edge_check_batched(P, view, d_x, d_x2, &ep, P->d_tune.as<int>(), nullptr, stream)
+                     : ppl_check_collision_batched(P, view, d_x, 0.0f, 0, P->d_tune.as<uint8_t>(), stream);
+    };
+    // candidates: the whole problem in one staging, then tiles of 128, 64 and
+    // 32 records below it; each with the three grid forms
+    std::vector<int> tiles{obb_tile(max_obbs)};
+    for (int t : {128, 64, 32})
+        if (t < tiles[0]) tiles.push_back(t);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    hipError_t he = hipEventCreate(&e1);
+    if (he != hipSuccess) { (void)hipEventDestroy(e0); return fail(PPL_ERR_HIP, "hipEventCreate failed"); }
+    const int keep_tile = P->ctile, keep_grid = P->cgrid, keep_src = P->shape_src, keep_obbs = P->shape_obbs;
+    int bt = -1, bg = 0;
+    float bms = 0.0f;
+    for (int t : tiles)
+        for (int g = 0; g <= 2 && st == PPL_OK && he == hipSuccess; ++g) {
+            P->ctile = t;
+            P->cgrid = g;
+            P->shape_src = 2;
+            st = run();   // warm-up
+            if (st == PPL_OK) he = hipEventRecord(e0, s);
+            for (int k = 0; k < reps && st == PPL_OK && he == hipSuccess; ++k) st = run();
+            if (st == PPL_OK && he == hipSuccess) he = hipEventRecord(e1, s);
+            if (st == PPL_OK && he == hipSuccess) he = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (st == PPL_OK && he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+            if (st == PPL_OK && he == hipSuccess && (bt < 0 || ms < bms)) { bt = t; bg = g; bms = ms; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st == PPL_OK && he != hipSuccess) st = fail(PPL_ERR_HIP, "collide autotune timing: %s", hipGetErrorString(he));
+    if (st != PPL_OK) {
+        P->ctile = keep_tile; P->cgrid = keep_grid; P->shape_src = keep_src; P->shape_obbs = keep_obbs;
+        return st;
+    }
+    // the whole-problem tile is stored as 0 (the built-in choice for any OBB count)
+    P->ctile = bt == tiles[0] ? 0 : bt;
+    P->cgrid = bg;
+    P->shape_src = 2;
+    P->shape_obbs = max_obbs;
+    char v[64];
+    snprintf(v, sizeof v, "%d %d %.6f", P->ctile, bg, bms / reps);
+    tune_put(collide_key(P->robot, max_obbs), v);
+    if (best_tile) *best_tile = bt;
+    if (best_grid) *best_grid = bg;
+    if (best_ms) *best_ms = bms / reps;
+    return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
+}
+
+ppl_status ppl_plan_collide_shape(ppl_plan P, int max_obbs, int* tile, int* grid, int* source) try {
+    CHECK_ARG(P, "NULL argument");
+    CHECK_ARG(max_obbs >= 0, "negative OBB count");
+    int t = 0, g = 0;
+    collide_shape(P, max_obbs, &t, &g);
+    if (tile) *tile = t;
+    if (grid) *grid = g;
+    if (source) *source = P->shape_src;
+    return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
+}
+
+ppl_status ppl_plan_set_collide_shape(ppl_plan P, int tile, int grid) try {
+    CHECK_ARG(P, "NULL argument");
+    CHECK_ARG(tile >= 0 && tile <= kMaxObbLds, "tile must be in [0, %d]", kMaxObbLds);
+    CHECK_ARG(grid >= 0 && grid <= 2, "grid must be 0, 1 or 2");
+    P->ctile = tile;
+    P->cgrid = grid;
+    P->shape_src = tile == 0 && grid == 0 ? 0 : 2;
+    P->shape_obbs = -1;   // built-in again: the tuning cache is consulted at the next launch
     return PPL_OK;
 } catch (const std::bad_alloc&) {
     return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");

@@ -290,6 +290,31 @@ class Plan:
         choice (grouped for the adjoint)."""
         call("ppl_plan_set_cost_group", self.handle, int(bool(grad)), int(lanes_per_state))
 
+    def autotune_collide(self, scene: "DeviceScene", x: torch.Tensor, x2: Optional[torch.Tensor] = None, reps=5,
+                         stream=None):
+        """Time the any-collision kernel's launch shapes (OBB staging tile x
+        grid form) on these inputs (configurations; or edge starts x and ends
+        x2), keep the fastest for this plan and record it in the tuning cache;
+        returns (tile, grid form, ms per call)."""
+        _dev(x)
+        if x2 is not None:
+            _dev(x2)
+        t, g, ms = C.c_int(0), C.c_int(0), C.c_float(0.0)
+        v = scene.view()
+        call("ppl_plan_autotune_collide", self.handle, C.byref(v), _dp(x), _dp(x2), int(reps), C.byref(t), C.byref(g),
+             C.byref(ms), _stream(stream))
+        return t.value, g.value, ms.value
+
+    def collide_shape(self, max_obbs: int):
+        """(tile, grid form, source) of the any-collision launches for problems
+        of up to max_obbs OBBs; source 0 built-in, 1 tuning cache, 2 this plan."""
+        t, g, src = C.c_int(0), C.c_int(0), C.c_int(0)
+        call("ppl_plan_collide_shape", self.handle, int(max_obbs), C.byref(t), C.byref(g), C.byref(src))
+        return t.value, g.value, src.value
+
+    def set_collide_shape(self, tile: int, grid: int):
+        call("ppl_plan_set_collide_shape", self.handle, int(tile), int(grid))
+
     def close(self):
         if getattr(self, "handle", None):
             lib().ppl_plan_destroy(self.handle)
@@ -300,6 +325,11 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def tune_cache_file(path: Optional[str]):
+    """Where the autotuners' choices are kept (None: in memory only)."""
+    call("ppl_tune_cache_file", None if path is None else str(path).encode())
 
 
 def _dev(t: torch.Tensor, dtype=torch.float32) -> torch.Tensor:

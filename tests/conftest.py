@@ -1,3 +1,4 @@
+import os
 import sys
 from pathlib import Path
 
@@ -6,6 +7,8 @@ import pytest
 ROOT = Path(__file__).resolve().parents[1]
 sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "oracle"))
+# autotuner choices stay in memory unless a test names a cache file
+os.environ.setdefault("PPL_TUNE_CACHE", "")
 
 
 def pytest_configure(config):
