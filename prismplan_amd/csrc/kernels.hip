@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <climits>
+#include <mutex>
 
 namespace ppl {
 namespace {
@@ -940,26 +941,29 @@ __device__ __forceinline__ void obb_argmin(const float4* __restrict__ s_obb, con
 }
 
 // Hinge terms of one sphere at world position p given its OBB argmin (best
-// key; rec: the argmin OBB's record, NULL when there is none): the summed
-// hinge (OBB + ESDF) and its point gradient d hinge / d p in g*; in
-// clearance-only mode the ESDF clearance with radius + margin.
-template <bool GRAD>
-__device__ __forceinline__ float sphere_hinge_at(const CostArgs& a, const float4* rec, float best, float px, float py,
-                                                 float pz, float r, int sph_id, bool use_obb, bool use_esdf, float& gx,
-                                                 float& gy, float& gz) {
+// key; has_arg: there is one; rec(r0, r1, r2, r3) fetches the argmin OBB's
+// record, called only for the gradient of an active hinge): the summed hinge
+// (OBB + ESDF) and its point gradient d hinge / d p in g*; in clearance-only
+// mode the ESDF clearance with radius + margin.
+template <bool GRAD, class Rec>
+__device__ __forceinline__ float sphere_hinge_at(const CostArgs& a, bool has_arg, const Rec& rec, float best, float px,
+                                                 float py, float pz, float r, int sph_id, bool use_obb, bool use_esdf,
+                                                 float& gx, float& gy, float& gz) {
     const float m = a.P.world_margin;
     float world = 0.0f;
     gx = gy = gz = 0.0f;
     // clearance mode is ESDF-only; no layer gives sphere_esdf_clearance's 1000
     if (a.clearance_only && !use_esdf) return 1000.0f;
     if (use_obb) {
-        const float clr = rec ? key_to_sdf(best) - r : 1000.0f;
+        const float clr = has_arg ? key_to_sdf(best) - r : 1000.0f;
         const float hh = m - clr;
         if (hh > 0.0f) {
             world += hh;
             if (GRAD) {
                 float dx, dy, dz;
-                box_sdf_grad(rec[0], rec[1], rec[2], rec[3], px, py, pz, dx, dy, dz);
+                float4 r0, r1, r2, r3;
+                rec(r0, r1, r2, r3);
+                box_sdf_grad(r0, r1, r2, r3, px, py, pz, dx, dy, dz);
                 gx -= dx; gy -= dy; gz -= dz;
             }
         }
@@ -1058,9 +1062,11 @@ __global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
                         a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)R.sph_id[s] * a.S.n_obbs : nullptr;
                     if (use_obb) obb_argmin(s_obb, mk, px, py, pz, acm_row, best, arg);
                     float gx, gy, gz;
-                    const float hs = sphere_hinge_at<GRAD>(a, arg >= 0 ? s_obb + 4 * arg : nullptr, best, px, py, pz,
-                                                           sp.w, R.sph_id[s], use_obb,
-                                                           use_esdf, gx, gy, gz);
+                    auto rec = [&](float4& r0, float4& r1, float4& r2, float4& r3) {
+                        r0 = s_obb[4 * arg]; r1 = s_obb[4 * arg + 1]; r2 = s_obb[4 * arg + 2]; r3 = s_obb[4 * arg + 3];
+                    };
+                    const float hs = sphere_hinge_at<GRAD>(a, arg >= 0, rec, best, px, py, pz, sp.w, R.sph_id[s],
+                                                           use_obb, use_esdf, gx, gy, gz);
                     if (a.clearance_only) minclr = fminf(minclr, hs);
                     else world += hs;
                     if (GRAD) {
@@ -1328,9 +1334,11 @@ __device__ __forceinline__ void grp_argmin(const float4* __restrict__ s_obb, con
 
 // WORLD: bit 0 the scene's boxes may enter (OBB hinge / argmin), bit 1 its
 // ESDF layers may; the launch picks the smallest variant the call needs, which
-// keeps the other path's registers out of the sphere pass.
-template <bool GRAD, int WORLD>
-__global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
+// keeps the other path's registers out of the sphere pass. TILED: some
+// problem's OBBs pass through LDS in tiles (a variant of its own, so the
+// folding across tiles costs the one-staging kernels no registers).
+template <bool GRAD, int WORLD, bool TILED>
+__device__ __forceinline__ void cost_grp(const CostArgs& a) {
     extern __shared__ float4 s_lds[];
     const DRobot& R = a.R;
     const int J = R.n_joints, D = R.n_dof, S = R.n_sph;
@@ -1366,14 +1374,14 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
     while (next_chunk(a.C, n_states, n_chunks, cur, ch)) {
         if ((WORLD & 1) && ch.x != cur_prob) {
             problem_obbs(a.C, a.S, ch.x, o0, o1);
-            n_tiles = uniform(obb_tiles(o0, o1, cap));
+            n_tiles = TILED ? uniform(obb_tiles(o0, o1, cap)) : 1;
             if (n_tiles == 1) {
                 __syncthreads();
                 n_obb = uniform(stage_obbs(a.S, o0, o1, s_obb, cap));
             }
             cur_prob = ch.x;
         }
-        const bool tiled = use_obb && n_tiles > 1;
+        const bool tiled = TILED && use_obb && n_tiles > 1;
         const long g = (long)ch.y + slot;
         const bool active = g < ch.z;
         stage_unit_ends(a.X, ch.w, s_end);
@@ -1432,6 +1440,24 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
         // candidate mask for the argmin box (tiled: reads the folded argmin
         // and rebuilds its record from the scene when the gradient needs it).
         float world = 0.0f, minclr = 1000.0f;
+        // wrench of a sphere's point gradient g at p about its joint's origin o
+        auto add_wrench = [&](int jj, int sid, float px, float py, float pz, float ox, float oy, float oz, float gx,
+                              float gy, float gz) {
+            if (si >= 0) {
+                if (sid == si) { gx -= snx; gy -= sny; gz -= snz; }
+                if (sid == sj) { gx += snx; gy += sny; gz += snz; }
+            }
+            if (gx != 0.0f || gy != 0.0f || gz != 0.0f) {
+                const float rx = px - ox, ry = py - oy, rz = pz - oz;
+                float* w = s_w + (slot * J + jj) * 6;
+                atomicAdd(w + 0, gx);
+                atomicAdd(w + 1, gy);
+                atomicAdd(w + 2, gz);
+                atomicAdd(w + 3, ry * gz - rz * gy);
+                atomicAdd(w + 4, rz * gx - rx * gz);
+                atomicAdd(w + 5, rx * gy - ry * gx);
+            }
+        };
         for (int r0 = 0; r0 < S; r0 += kGrp) {
             const int s = r0 + sub;
             const bool valid = active && s < S;
@@ -1445,44 +1471,32 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
             const float py = fmaf(t1.x, sp.x, fmaf(t1.y, sp.y, fmaf(t1.z, sp.z, t1.w)));
             const float pz = fmaf(t2.x, sp.x, fmaf(t2.y, sp.y, fmaf(t2.z, sp.z, t2.w)));
             float best = FLT_MAX;
-            const float4* rec = nullptr;
-            float4 grec[4];
+            int arg = -1;   // tiled: the argmin's original OBB index; else its LDS record
             if (tiled) {
                 if (valid) {
                     best = s_best[slot * S + s];
-                    const int orig = s_barg[slot * S + s];
-                    if (orig >= 0) {
-                        if (want_grad && a.P.world_margin - (key_to_sdf(best) - sp.w) > 0.0f) obb_record(a.S, orig, grec);
-                        rec = grec;
-                    }
+                    arg = s_barg[slot * S + s];
                 }
             } else if (use_obb) {
                 const uint8_t* acm_row = a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)sid * a.S.n_obbs : nullptr;
-                int arg;
                 grp_argmin(s_obb, s_mask + jj * kMaskWords, valid, px, py, pz, acm_row, best, arg);
-                if (arg >= 0) rec = s_obb + 4 * arg;
             }
             if (!valid) continue;
+            auto rec = [&](float4& r0, float4& r1, float4& r2, float4& r3) {
+                if (tiled) {
+                    float4 t[4];
+                    obb_record(a.S, arg, t);
+                    r0 = t[0]; r1 = t[1]; r2 = t[2]; r3 = t[3];
+                } else {
+                    r0 = s_obb[4 * arg]; r1 = s_obb[4 * arg + 1]; r2 = s_obb[4 * arg + 2]; r3 = s_obb[4 * arg + 3];
+                }
+            };
             float gx, gy, gz;
-            const float hs = sphere_hinge_at<GRAD>(a, rec, best, px, py, pz, sp.w, sid, use_obb, use_esdf, gx, gy, gz);
+            const float hs =
+                sphere_hinge_at<GRAD>(a, arg >= 0, rec, best, px, py, pz, sp.w, sid, use_obb, use_esdf, gx, gy, gz);
             if (a.clearance_only) minclr = fminf(minclr, hs);
             else world += hs;
-            if (want_grad) {
-                if (si >= 0) {
-                    if (sid == si) { gx -= snx; gy -= sny; gz -= snz; }
-                    if (sid == sj) { gx += snx; gy += sny; gz += snz; }
-                }
-                if (gx != 0.0f || gy != 0.0f || gz != 0.0f) {
-                    const float rx = px - t0.w, ry = py - t1.w, rz = pz - t2.w;
-                    float* w = s_w + (slot * J + jj) * 6;
-                    atomicAdd(w + 0, gx);
-                    atomicAdd(w + 1, gy);
-                    atomicAdd(w + 2, gz);
-                    atomicAdd(w + 3, ry * gz - rz * gy);
-                    atomicAdd(w + 4, rz * gx - rx * gz);
-                    atomicAdd(w + 5, rx * gy - ry * gx);
-                }
-            }
+            if (want_grad) add_wrench(jj, sid, px, py, pz, t0.w, t1.w, t2.w, gx, gy, gz);
         }
         // 3. adjoint: subtree wrench of joint `sub` -> d cost / d q
         if (want_grad) {
@@ -1551,6 +1565,18 @@ __global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
         }
         __syncthreads();   // frames, masks and wrenches of this chunk are dead before the next one
     }
+}
+
+template <bool GRAD, int WORLD, bool TILED>
+__global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
+    cost_grp<GRAD, WORLD, TILED>(a);
+}
+
+// ESDF-only launches (no OBB path): five waves per SIMD fit in the register
+// budget, which the lookups' memory latency rewards.
+template <bool GRAD>
+__global__ __launch_bounds__(64, 5) void k_cost_grp_esdf(CostArgs a) {
+    cost_grp<GRAD, 2, false>(a);
 }
 
 // ===========================================================================
@@ -2493,12 +2519,16 @@ size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, int n_sph, bool grad,
     return lds_floats_grp(obb_cap, n_dof, n_joints, n_sph, grad, tiled) * sizeof(float);
 }
 
-// Variant table: [grad][world - 1].
-const void* grp_kernel(bool grad, int world) {
-    static const void* const tab[2][3] = {
-        {(const void*)k_cost_grp<false, 1>, (const void*)k_cost_grp<false, 2>, (const void*)k_cost_grp<false, 3>},
-        {(const void*)k_cost_grp<true, 1>, (const void*)k_cost_grp<true, 2>, (const void*)k_cost_grp<true, 3>}};
-    return tab[grad ? 1 : 0][world - 1];
+// Variant table: [grad][world - 1][tiled] (ESDF-only launches stage no OBBs).
+const void* grp_kernel(bool grad, int world, bool tiled) {
+    static const void* const tab[2][3][2] = {
+        {{(const void*)k_cost_grp<false, 1, false>, (const void*)k_cost_grp<false, 1, true>},
+         {(const void*)k_cost_grp_esdf<false>, (const void*)k_cost_grp_esdf<false>},
+         {(const void*)k_cost_grp<false, 3, false>, (const void*)k_cost_grp<false, 3, true>}},
+        {{(const void*)k_cost_grp<true, 1, false>, (const void*)k_cost_grp<true, 1, true>},
+         {(const void*)k_cost_grp_esdf<true>, (const void*)k_cost_grp_esdf<true>},
+         {(const void*)k_cost_grp<true, 3, false>, (const void*)k_cost_grp<true, 3, true>}}};
+    return tab[grad ? 1 : 0][world - 1][tiled ? 1 : 0];
 }
 
 int grp_world(const CostArgs& a) {
@@ -2507,29 +2537,32 @@ int grp_world(const CostArgs& a) {
     return obb && esdf ? 3 : esdf ? 2 : 1;
 }
 
-int grid_cost_grp(bool grad, int world, size_t lds) {
-    static size_t last[2][3] = {{(size_t)-1, (size_t)-1, (size_t)-1}, {(size_t)-1, (size_t)-1, (size_t)-1}};
-    static int g[2][3] = {};
-    const int i = grad ? 1 : 0, w = world - 1;
-    if (lds != last[i][w]) {
-        g[i][w] = occupancy_grid(grp_kernel(grad, world), lds);
-        last[i][w] = lds;
+int grid_cost_grp(bool grad, int world, bool tiled, size_t lds) {
+    static std::mutex mu;
+    static size_t last[2][3][2];
+    static int g[2][3][2];
+    static bool init = false;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!init) {
+        for (auto& x : last) for (auto& y : x) for (auto& z : y) z = (size_t)-1;
+        init = true;
     }
-    return g[i][w];
+    const int i = grad ? 1 : 0, w = world - 1, t = tiled ? 1 : 0;
+    if (lds != last[i][w][t]) {
+        g[i][w][t] = occupancy_grid(grp_kernel(grad, world, tiled), lds);
+        last[i][w][t] = lds;
+    }
+    return g[i][w][t];
 }
 
 hipError_t launch_cost_grp(const CostArgs& a, int grid, hipStream_t s) {
     const size_t lds = lds_bytes_grp(a.C.obb_cap, a.R.n_dof, a.R.n_joints, a.R.n_sph, a.grad != 0, a.C.tiled != 0);
     const int world = grp_world(a);
-    grid = std::max(1, std::min(grid, grid_cost_grp(a.grad != 0, world, lds)));   // persistent: occupancy x CUs
-#define PPL_GRP_LAUNCH(G_, W_) hipLaunchKernelGGL((k_cost_grp<G_, W_>), dim3(grid), dim3(64), lds, s, a)
-    if (a.grad) {
-        if (world == 1) PPL_GRP_LAUNCH(true, 1); else if (world == 2) PPL_GRP_LAUNCH(true, 2); else PPL_GRP_LAUNCH(true, 3);
-    } else {
-        if (world == 1) PPL_GRP_LAUNCH(false, 1); else if (world == 2) PPL_GRP_LAUNCH(false, 2); else PPL_GRP_LAUNCH(false, 3);
-    }
-#undef PPL_GRP_LAUNCH
-    return hipGetLastError();
+    const bool tiled = a.C.tiled != 0 && world != 2;
+    grid = std::max(1, std::min(grid, grid_cost_grp(a.grad != 0, world, tiled, lds)));   // persistent: occupancy x CUs
+    CostArgs arg = a;
+    void* params[] = {&arg};
+    return hipLaunchKernel(grp_kernel(a.grad != 0, world, tiled), dim3(grid), dim3(64), params, lds, s);
 }
 
 hipError_t launch_self(const SelfArgs& a, hipStream_t s) {

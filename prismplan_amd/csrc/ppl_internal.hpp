@@ -259,7 +259,7 @@ int device_cus();
 int grid_collide(size_t lds);
 int grid_cost(int n_joints, bool grad, size_t lds);
 size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, int n_sph, bool grad, bool tiled);
-int grid_cost_grp(bool grad, int world, size_t lds);
+int grid_cost_grp(bool grad, int world, bool tiled, size_t lds);
 int grp_world(const CostArgs& a);
 
 }  // namespace ppl
