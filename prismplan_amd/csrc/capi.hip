@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <new>
@@ -725,6 +726,90 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     std::vector<float4> sph, apx;
     emit(S, d->spheres, d->sphere_to_joint, sph_off, sph, sph_id);
     emit(A, d->approx_spheres, d->approx_sphere_to_joint, apx_off, apx, apx_id);
+    // Sphere clusters for the cost kernels' candidate masks: each joint's
+    // spheres are bisected along their principal axis (FP64, median split)
+    // while a part holds more than 16 spheres, or more than 4 with an
+    // enclosing radius above 0.15 m; the leaves become contiguous runs of
+    // emission slots (the joint's spheres are re-emitted in leaf order).
+    // A mask built around a cluster's enclosing sphere holds far fewer
+    // boxes than one around a large link's.
+    std::vector<int> clu_off(J + 1, 0), clu_beg;
+    std::vector<float4> clu_enc;
+    {
+        auto enclose = [&](const std::vector<int>& ix, double* c) {
+            c[0] = c[1] = c[2] = 0.0;
+            for (int t : ix) { c[0] += sph[t].x; c[1] += sph[t].y; c[2] += sph[t].z; }
+            for (int k = 0; k < 3; ++k) c[k] /= (double)ix.size();
+            double m = 0.0;
+            for (int t : ix) {
+                const double dx = sph[t].x - c[0], dy = sph[t].y - c[1], dz = sph[t].z - c[2];
+                m = std::max(m, std::sqrt(dx * dx + dy * dy + dz * dz) + sph[t].w);
+            }
+            return m;
+        };
+        std::vector<float4> sph2(sph);
+        std::vector<int> id2(sph_id);
+        std::vector<std::vector<int>> leaves;
+        std::function<void(std::vector<int>)> split = [&](std::vector<int> ix) {
+            double c[3];
+            const double rad = enclose(ix, c);
+            if (ix.size() <= 1 || (ix.size() <= 16 && (ix.size() <= 4 || rad <= 0.15))) {
+                leaves.push_back(ix);
+                return;
+            }
+            // principal axis by power iteration on the 3x3 covariance
+            double C[9] = {0};
+            for (int t : ix) {
+                const double v[3] = {sph[t].x - c[0], sph[t].y - c[1], sph[t].z - c[2]};
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) C[3 * a + b] += v[a] * v[b];
+            }
+            double u[3] = {1.0, 0.7, 0.4};
+            for (int it = 0; it < 64; ++it) {
+                double w[3] = {0, 0, 0};
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) w[a] += C[3 * a + b] * u[b];
+                const double n = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+                if (!(n > 0.0)) break;
+                for (int a = 0; a < 3; ++a) u[a] = w[a] / n;
+            }
+            std::vector<std::pair<double, int>> key;
+            for (int t : ix) key.push_back({(sph[t].x - c[0]) * u[0] + (sph[t].y - c[1]) * u[1] + (sph[t].z - c[2]) * u[2], t});
+            std::stable_sort(key.begin(), key.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+            const size_t h = ix.size() / 2;
+            std::vector<int> lo, hi;
+            for (size_t k = 0; k < key.size(); ++k) (k < h ? lo : hi).push_back(key[k].second);
+            split(lo);
+            split(hi);
+        };
+        for (int k = 0; k < J; ++k) {
+            clu_off[k] = (int)clu_beg.size();
+            if (sph_off[k] == sph_off[k + 1]) continue;
+            std::vector<int> ix;
+            for (int t = sph_off[k]; t < sph_off[k + 1]; ++t) ix.push_back(t);
+            leaves.clear();
+            split(ix);
+            int at = sph_off[k];
+            for (const auto& lf : leaves) {
+                clu_beg.push_back(at);
+                double c[3];
+                const double rad = enclose(lf, c);
+                // 1e-5 m larger, so FP32 rounding cannot make a sphere poke out
+                clu_enc.push_back(make_float4((float)c[0], (float)c[1], (float)c[2], (float)(rad + 1e-5)));
+                for (int t : lf) {
+                    sph2[at] = sph[t];
+                    id2[at] = sph_id[t];
+                    ++at;
+                }
+            }
+        }
+        clu_off[J] = (int)clu_beg.size();
+        clu_beg.push_back(S);
+        sph.swap(sph2);
+        sph_id.swap(id2);
+    }
+    const int n_clu = (int)clu_enc.size();
+    CHECK_ARG(n_clu <= kMaxClusters, "%d sphere clusters, at most %d supported", n_clu, kMaxClusters);
     // Self-pair blocks for the self pass: the pairs of DFS joints (A, B),
     // A <= B, cut into at most 64 x 64 sphere tiles in emission order, with a
     // 64-bit partner mask per row. Emission order keeps each joint's spheres
@@ -816,9 +901,14 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     std::vector<int> jend(J);
     for (int k = 0; k < J; ++k) jend[k] = k + 1;
     for (int k = J - 1; k >= 1; --k) jend[close[k]] = std::max(jend[close[k]], jend[k]);
-    std::vector<int> sph_jnt(S);
+    std::vector<int> sph_jnt(S), sph_clu(S), clu_jnt(n_clu);
     for (int k = 0; k < J; ++k)
         for (int t = sph_off[k]; t < sph_off[k + 1]; ++t) sph_jnt[t] = k;
+    for (int k = 0; k < J; ++k)
+        for (int c = clu_off[k]; c < clu_off[k + 1]; ++c) {
+            clu_jnt[c] = k;
+            for (int t = clu_beg[c]; t < clu_beg[c + 1]; ++t) sph_clu[t] = c;
+        }
     // One device blob for every table.
     size_t off = 0;
     auto place = [&](size_t bytes) {
@@ -834,7 +924,9 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
                  o_sblk = place(sblk.size() * 16), o_rmask = place(rowmask.size() * 8), o_jorig = place(J * 4),
                  o_lim = place((size_t)D * 8),
                  o_jenc = place((size_t)J * 16), o_sjnt = place((size_t)S * 4), o_jend = place((size_t)J * 4),
-                 o_aft = place(aft.size() * 4), o_atype = place(atype.size() * 4), o_ajnt = place(apx_jnt.size() * 4);
+                 o_aft = place(aft.size() * 4), o_atype = place(atype.size() * 4), o_ajnt = place(apx_jnt.size() * 4),
+                 o_coff = place(clu_off.size() * 4), o_cbeg = place(clu_beg.size() * 4), o_cenc = place(clu_enc.size() * 16),
+                 o_sclu = place(sph_clu.size() * 4), o_cjnt = place(clu_jnt.size() * 4);
     std::vector<char> host(off + 16, 0);
     auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) memcpy(host.data() + o, src, bytes); };
     put(o_ft, ft.data(), ft.size() * 4);
@@ -862,6 +954,11 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     put(o_aft, aft.data(), aft.size() * 4);
     put(o_atype, atype.data(), atype.size() * 4);
     put(o_ajnt, apx_jnt.data(), apx_jnt.size() * 4);
+    put(o_coff, clu_off.data(), clu_off.size() * 4);
+    put(o_cbeg, clu_beg.data(), clu_beg.size() * 4);
+    put(o_cenc, clu_enc.data(), clu_enc.size() * 16);
+    put(o_sclu, sph_clu.data(), sph_clu.size() * 4);
+    put(o_cjnt, clu_jnt.data(), clu_jnt.size() * 4);
 
     ppl_robot_s* r = new (std::nothrow) ppl_robot_s();
     if (!r) return fail(PPL_ERR_ALLOC, "out of host memory");
@@ -904,6 +1001,12 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     R.aft = own_chain ? reinterpret_cast<const float*>(b + o_aft) : nullptr;
     R.atype = own_chain ? reinterpret_cast<const int*>(b + o_atype) : nullptr;
     R.apx_jnt = reinterpret_cast<const int*>(b + o_ajnt);
+    R.n_clu = n_clu;
+    R.clu_off = reinterpret_cast<const int*>(b + o_coff);
+    R.clu_beg = reinterpret_cast<const int*>(b + o_cbeg);
+    R.clu_enc = reinterpret_cast<const float4*>(b + o_cenc);
+    R.sph_clu = reinterpret_cast<const int*>(b + o_sclu);
+    R.clu_jnt = reinterpret_cast<const int*>(b + o_cjnt);
     R.scene_gate_fused = scene_gate_fused ? 1 : 0;
     R.self_gate_implied = self_gate_implied ? 1 : 0;
     R.n_sblk = (int)sblk.size() / 2;
@@ -1396,7 +1499,7 @@ bool group_fits(const ppl_plan_s* P, int G, bool grad, int max_obbs) {
     const DRobot& R = P->robot->R;
     const bool tiled = max_obbs > kMaxObbLds;
     if (G == PPL_COST_GROUPED)
-        return lds_bytes_grp(obb_tile(max_obbs), R.n_dof, R.n_joints, R.n_sph, grad, tiled) <= 65536;
+        return lds_bytes_grp(obb_tile(max_obbs), R.n_dof, R.n_joints, R.n_sph, R.n_clu, grad, tiled) <= 65536;
     return !tiled && lds_bytes_cost(max_obbs, R.n_dof, R.n_joints, G, grad) <= 65536;
 }
 

@@ -1049,10 +1049,11 @@ __global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
             fk_joint(R, jj, s_q + lane, sl, T, meta);
             float fx = 0.0f, fy = 0.0f, fz = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
             const int s0 = R.sph_off[jj], s1 = R.sph_off[jj + 1];
-            if (s1 > s0) {
+            // one candidate mask per sphere cluster of the joint
+            for (int c = (s1 > s0 ? R.clu_off[jj] : 0); c < (s1 > s0 ? R.clu_off[jj + 1] : 0); ++c) {
                 ObbMask mk;
-                if (use_obb) near_obbs(s_obb, n_obb, T, R.jenc[jj], a.P.world_margin, active, mk);
-                for (int s = s0 + sub; s < s1; s += G) {
+                if (use_obb) near_obbs(s_obb, n_obb, T, R.clu_enc[c], a.P.world_margin, active, mk);
+                for (int s = R.clu_beg[c] + sub; s < R.clu_beg[c + 1]; s += G) {
                     const float4 sp = R.sph[s];
                     float px, py, pz;
                     aff_point(T, sp.x, sp.y, sp.z, px, py, pz);
@@ -1150,11 +1151,12 @@ __global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
 //  1. Forward kinematics in parallel: lane jj of the group computes joint
 //     jj's step (one sin/cos per lane) and the frames are composed by
 //     pointer jumping over the DFS tree in ceil(log2(depth)) rounds through
-//     LDS. Then every joint's approximate sphere is placed at once and one
-//     wave pass per joint over the OBBs gives its candidate mask (the broad
-//     phase of near_obbs over the wave's four states).
-//  2. Sphere pass: emission slot k (spheres are emitted joint by joint) goes
-//     to lane k mod 16, which reads its joint's frame and mask from LDS,
+//     LDS. Then every sphere cluster's enclosing sphere is placed at once
+//     and one wave pass per cluster over the OBBs gives its candidate mask
+//     (the broad phase of near_obbs over the wave's four states).
+//  2. Sphere pass: emission slot k (spheres are emitted joint by joint,
+//     cluster by cluster) goes to lane k mod 16, which reads its joint's
+//     frame and its cluster's mask from LDS,
 //     finds the argmin OBB and the hinge terms. Panda's 59 spheres take four
 //     rounds instead of one round per joint with most lanes idle.
 //  3. Adjoint: every sphere with a non-zero point gradient g adds its force g
@@ -1232,67 +1234,65 @@ __device__ __forceinline__ void group_frames(const DRobot& R, const float* qs, i
 
 static_assert(kGrp >= kMaxJoints, "the grouped kernel gives every joint its own lane of the group");
 
-size_t lds_floats_grp(int obb_cap, int n_dof, int n_joints, int n_sph, bool grad, bool tiled) {
-    size_t f = (size_t)obb_cap * 16 + (size_t)n_joints * kMaskWords * 2 +
+size_t lds_floats_grp(int obb_cap, int n_dof, int n_joints, int n_sph, int n_clu, bool grad, bool tiled) {
+    size_t f = (size_t)obb_cap * 16 + (size_t)n_clu * kMaskWords * 2 +
                (((size_t)n_dof * kGrpStates + 3) & ~(size_t)3) + 2 * kWave + (size_t)kGrpStates * (n_joints * 13 + 4);
     if (grad) f += (size_t)kGrpStates * ((size_t)n_joints * 6 + kMaxDof + 1);
     if (tiled) f += (size_t)kGrpStates * n_sph * 2;
     return f;
 }
 
-// Candidate masks of every joint of the wave's four states over the staged
-// OBBs (the broad phase of near_obbs, all joints' enclosing spheres placed at
-// once): lane jj of each group places joint jj's enclosing sphere; the four
-// states' spread about state slot 0's centre sets the joint's reach; one pass
-// of 64 lanes over the OBBs per joint gives its mask.
+// Candidate masks of every sphere cluster of the wave's four states over the
+// staged OBBs (the broad phase of near_obbs, all clusters' enclosing spheres
+// placed at once, 16 clusters per pass): lane k of each group places cluster
+// c0 + k's enclosing sphere; the four states' spread about state slot 0's
+// centre sets the cluster's reach; one pass of 64 lanes over the OBBs per
+// cluster gives its mask.
 __device__ __forceinline__ void grp_masks(const DRobot& R, const float4* __restrict__ s_obb, int n_obb, const float* Ts,
                                           float band, bool active, int sub, unsigned long long* s_mask) {
-    const int J = R.n_joints, lane = lane_id();
-    float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    if (sub < J && active) {
-        const float4 je = R.jenc[sub];
-        if (je.w >= 0.0f) {
-            const float4* t = reinterpret_cast<const float4*>(Ts + 12 * sub);
+    const int C = R.n_clu, lane = lane_id();
+    for (int c0 = 0; c0 < C; c0 += kGrp) {
+        const int cl = c0 + sub;
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        if (cl < C && active) {
+            const float4 ce = R.clu_enc[cl];
+            const float4* t = reinterpret_cast<const float4*>(Ts + 12 * R.clu_jnt[cl]);
             const float4 t0 = t[0], t1 = t[1], t2 = t[2];
-            ax = fmaf(t0.x, je.x, fmaf(t0.y, je.y, fmaf(t0.z, je.z, t0.w)));
-            ay = fmaf(t1.x, je.x, fmaf(t1.y, je.y, fmaf(t1.z, je.z, t1.w)));
-            az = fmaf(t2.x, je.x, fmaf(t2.y, je.y, fmaf(t2.z, je.z, t2.w)));
+            ax = fmaf(t0.x, ce.x, fmaf(t0.y, ce.y, fmaf(t0.z, ce.z, t0.w)));
+            ay = fmaf(t1.x, ce.x, fmaf(t1.y, ce.y, fmaf(t1.z, ce.z, t1.w)));
+            az = fmaf(t2.x, ce.x, fmaf(t2.y, ce.y, fmaf(t2.z, ce.z, t2.w)));
         }
-    }
-    const float cx = __shfl(ax, sub), cy = __shfl(ay, sub), cz = __shfl(az, sub);   // slot 0's centre
-    const float dx = ax - cx, dy = ay - cy, dz = az - cz;
-    float d2 = (sub < J && active) ? fmaf(dz, dz, fmaf(dy, dy, dx * dx)) : 0.0f;
-    d2 = fmaxf(d2, __shfl_xor(d2, 16));
-    d2 = fmaxf(d2, __shfl_xor(d2, 32));
-    for (int jj = 0; jj < J; ++jj) {
-        const float4 je = R.jenc[jj];
-        ObbMask mk;
-        if (je.w < 0.0f) {
-#pragma unroll
-            for (int k = 0; k < kMaskWords; ++k) mk.w[k] = 0ull;   // no spheres: no candidates
-        } else {
-            const float c0x = read_lane(cx, jj), c0y = read_lane(cy, jj), c0z = read_lane(cz, jj);
-            const float reach = band + je.w + sqrtf(read_lane(d2, jj)) * 1.0001f + 1e-5f;
+        const float cx = __shfl(ax, sub), cy = __shfl(ay, sub), cz = __shfl(az, sub);   // slot 0's centre
+        const float dx = ax - cx, dy = ay - cy, dz = az - cz;
+        float d2 = (cl < C && active) ? fmaf(dz, dz, fmaf(dy, dy, dx * dx)) : 0.0f;
+        d2 = fmaxf(d2, __shfl_xor(d2, 16));
+        d2 = fmaxf(d2, __shfl_xor(d2, 32));
+        const int nk = min(kGrp, C - c0);
+        for (int k = 0; k < nk; ++k) {
+            const float4 ce = R.clu_enc[c0 + k];
+            ObbMask mk;
+            const float c0x = read_lane(cx, k), c0y = read_lane(cy, k), c0z = read_lane(cz, k);
+            const float reach = band + ce.w + sqrtf(read_lane(d2, k)) * 1.0001f + 1e-5f;
             const float thr = uniform(reach * fabsf(reach));
 #pragma unroll
-            for (int k = 0; k < kMaskWords; ++k) {
+            for (int w = 0; w < kMaskWords; ++w) {
                 bool c = false;
-                if (k * 64 < n_obb) {
-                    const int o = k * 64 + lane;
+                if (w * 64 < n_obb) {
+                    const int o = w * 64 + lane;
                     if (o < n_obb) {
                         float lx, ly, lz;
                         to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], c0x, c0y, c0z, lx, ly, lz);
                         c = box_key(lx, ly, lz, s_obb[4 * o + 3]) < thr;
                     }
                 }
-                mk.w[k] = __ballot(c);
+                mk.w[w] = __ballot(c);
             }
-        }
-        if (lane < kMaskWords) {
-            unsigned long long v = 0ull;
+            if (lane < kMaskWords) {
+                unsigned long long v = 0ull;
 #pragma unroll
-            for (int k = 0; k < kMaskWords; ++k) if (k == lane) v = mk.w[k];
-            s_mask[jj * kMaskWords + lane] = v;
+                for (int w = 0; w < kMaskWords; ++w) if (w == lane) v = mk.w[w];
+                s_mask[(c0 + k) * kMaskWords + lane] = v;
+            }
         }
     }
 }
@@ -1344,7 +1344,7 @@ __device__ __forceinline__ void cost_grp(const CostArgs& a) {
     const int J = R.n_joints, D = R.n_dof, S = R.n_sph;
     float4* s_obb = s_lds;
     unsigned long long* s_mask = reinterpret_cast<unsigned long long*>(s_lds + 4 * a.C.obb_cap);
-    float* s_q = reinterpret_cast<float*>(s_mask + (size_t)J * kMaskWords);
+    float* s_q = reinterpret_cast<float*>(s_mask + (size_t)R.n_clu * kMaskWords);
     int* s_end = reinterpret_cast<int*>(s_q + D * kGrpStates);   // s_q: [D][slot]
     // frames [slot][J][12], slot stride padded by 4 floats so the four
     // slots' frames of one joint start in different LDS banks
@@ -1392,7 +1392,7 @@ __device__ __forceinline__ void cost_grp(const CostArgs& a) {
         __syncthreads();
         // 1a. joint frames (pointer jumping over the DFS tree)
         group_frames(R, s_q + slot, kGrpStates, Ts, s_par + slot * J, sub);
-        // 1b. candidate masks per joint; a tiled problem's OBBs pass through
+        // 1b. candidate masks per sphere cluster; a tiled problem's OBBs pass through
         // LDS tile by tile, each tile's argmins folded into s_best / s_barg
         // (tiles in index order and a strict < keep the lowest index on ties)
         if (use_obb && !tiled) grp_masks(R, s_obb, n_obb, Ts, a.P.world_margin, active, sub, s_mask);
@@ -1417,7 +1417,7 @@ __device__ __forceinline__ void cost_grp(const CostArgs& a) {
                 const uint8_t* acm_row = a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)sid * a.S.n_obbs : nullptr;
                 float best;
                 int arg;
-                grp_argmin(s_obb, s_mask + jj * kMaskWords, valid, px, py, pz, acm_row, best, arg);
+                grp_argmin(s_obb, s_mask + R.sph_clu[sc] * kMaskWords, valid, px, py, pz, acm_row, best, arg);
                 if (valid) {
                     float* bk = s_best + slot * S + s;
                     int* ba = s_barg + slot * S + s;
@@ -1479,7 +1479,7 @@ __device__ __forceinline__ void cost_grp(const CostArgs& a) {
                 }
             } else if (use_obb) {
                 const uint8_t* acm_row = a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)sid * a.S.n_obbs : nullptr;
-                grp_argmin(s_obb, s_mask + jj * kMaskWords, valid, px, py, pz, acm_row, best, arg);
+                grp_argmin(s_obb, s_mask + R.sph_clu[sc] * kMaskWords, valid, px, py, pz, acm_row, best, arg);
             }
             if (!valid) continue;
             auto rec = [&](float4& r0, float4& r1, float4& r2, float4& r3) {
@@ -2515,8 +2515,8 @@ hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s) {
     return hipGetLastError();
 }
 
-size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, int n_sph, bool grad, bool tiled) {
-    return lds_floats_grp(obb_cap, n_dof, n_joints, n_sph, grad, tiled) * sizeof(float);
+size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, int n_sph, int n_clu, bool grad, bool tiled) {
+    return lds_floats_grp(obb_cap, n_dof, n_joints, n_sph, n_clu, grad, tiled) * sizeof(float);
 }
 
 // Variant table: [grad][world - 1][tiled] (ESDF-only launches stage no OBBs).
@@ -2556,7 +2556,8 @@ int grid_cost_grp(bool grad, int world, bool tiled, size_t lds) {
 }
 
 hipError_t launch_cost_grp(const CostArgs& a, int grid, hipStream_t s) {
-    const size_t lds = lds_bytes_grp(a.C.obb_cap, a.R.n_dof, a.R.n_joints, a.R.n_sph, a.grad != 0, a.C.tiled != 0);
+    const size_t lds = lds_bytes_grp(a.C.obb_cap, a.R.n_dof, a.R.n_joints, a.R.n_sph, a.R.n_clu, a.grad != 0,
+                                     a.C.tiled != 0);
     const int world = grp_world(a);
     const bool tiled = a.C.tiled != 0 && world != 2;
     grid = std::max(1, std::min(grid, grid_cost_grp(a.grad != 0, world, tiled, lds)));   // persistent: occupancy x CUs

@@ -32,6 +32,7 @@ constexpr int kMaxGroup = 16;       // lanes per state of the cost kernel (power
 constexpr int kGrp = 16;            // lanes per state of the grouped cost kernel (>= kMaxJoints)
 constexpr int kGrpStates = kWave / kGrp;
 constexpr int kGroupKinds = 6;      // chunk tables for 64 / G states, G = 1 .. 32
+constexpr int kMaxClusters = 64;    // sphere clusters per robot (cost-kernel masks in LDS)
 
 enum JointType { kFixed = -1, kXPrism = 0, kYPrism = 1, kZPrism = 2, kXRot = 3, kYRot = 4, kZRot = 5 };
 
@@ -70,6 +71,15 @@ struct DRobot {
     // hit of that OBB and the any-collision kernel gates each (joint, OBB)
     // pair by itself; 0: the gated form (joint flags from the approximate
     // pass over the whole scene, then the flagged joints' exact spheres).
+    // Sphere clusters (contiguous emission runs inside a joint): the cost
+    // kernels build one candidate-box mask per cluster around its enclosing
+    // sphere (link frame, 1e-5 m padded).
+    int n_clu;
+    const int* clu_off;         // [J+1] clusters of DFS joint k: [clu_off[k], clu_off[k+1])
+    const int* clu_beg;         // [C+1] emission slots of cluster c: [clu_beg[c], clu_beg[c+1])
+    const float4* clu_enc;      // [C] enclosing sphere of cluster c
+    const int* sph_clu;         // [S] cluster of emission slot k
+    const int* clu_jnt;         // [C] DFS joint of cluster c
     int scene_gate_fused;
     // 1: an exact self hit implies an approximate one (every exact pair lies in
     // an approximate pair); 0: the self verdict needs the approximate pass too.
@@ -258,7 +268,7 @@ __host__ __device__ inline int dq_stride(int n_dof) { return n_dof | 1; }
 int device_cus();
 int grid_collide(size_t lds);
 int grid_cost(int n_joints, bool grad, size_t lds);
-size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, int n_sph, bool grad, bool tiled);
+size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, int n_sph, int n_clu, bool grad, bool tiled);
 int grid_cost_grp(bool grad, int world, bool tiled, size_t lds);
 int grp_world(const CostArgs& a);
 
