@@ -170,6 +170,7 @@ struct ppl_plan_s {
     int n_chunks[kGroupKinds] = {};
     int* d_obb_off = nullptr;
     int wmajor = 0;                  // waypoint-major trajectory plan
+    int t_max = 0;                   // states of the longest unit (trajectory length)
     int tuned_group[2] = {0, 0};     // cost kernel lanes per state chosen by ppl_plan_autotune_cost (0: heuristic)
     // any-collision launch shape (OBB records per LDS staging, grid form) for
     // problems of up to shape_obbs OBBs; shape_src 0 built-in, 1 tuning cache,
@@ -1391,6 +1392,7 @@ ppl_status ppl_plan_create(ppl_robot r, const ppl_plan_desc* d, ppl_plan* out) t
     }
     P->h_off.assign(uoff.size(), 0);
     for (size_t i = 0; i < uoff.size(); ++i) P->h_off[i] = (int)uoff[i];
+    for (size_t i = 0; i + 1 < uoff.size(); ++i) P->t_max = std::max(P->t_max, (int)(uoff[i + 1] - uoff[i]));
     hipError_t e = ensure_table(P, 1);
     if (e == hipSuccess && !prob.empty()) {
         e = hipMalloc((void**)&P->d_prob, prob.size() * sizeof(int4));
@@ -1654,9 +1656,15 @@ ppl_status stomp_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_
                       float dt, float* d_state, float* d_total, hipStream_t s) {
     CHECK_ARG(P && prm && d_roll && d_total, "NULL argument");
     CHECK_ARG(P->kind == PPL_PLAN_TRAJ, "rollout costs need a trajectory plan");
-    ppl_status st = hinge_impl(P, view, d_roll, prm, d_state, d_total, nullptr, s);
+    float* state = d_state;
+    if (!state) {
+        HIP_TRY(P->d_cost.ensure((size_t)P->n_states * 4));
+        state = P->d_cost.as<float>();
+    }
+    ppl_status st = hinge_impl(P, view, d_roll, prm, state, nullptr, nullptr, s);
     if (st != PPL_OK) return st;
-    HIP_TRY(launch_control_cost(d_roll, P->d_off, P->n_units, P->robot->R.n_dof, dt, w_u, d_total, 1, s));
+    // rollout totals: the state costs' sum and the control cost in one launch
+    HIP_TRY(launch_control_cost(d_roll, P->d_off, P->n_units, P->robot->R.n_dof, dt, w_u, d_total, 0, s, P->t_max, state));
     return PPL_OK;
 }
 
@@ -2077,7 +2085,8 @@ ppl_status ppl_chomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_
         if (e != PPL_OK) return e;
         e = hinge_impl(P, view, d_theta, &cp, state, total, cs->fd_gradient ? nullptr : grad, q);
         if (e != PPL_OK) return e;
-        HIP_TRY(launch_control_cost(d_theta, P->d_off, P->n_units, D, cs->dt, cs->smoothness_weight, total, 1, q));
+        HIP_TRY(launch_control_cost(d_theta, P->d_off, P->n_units, D, cs->dt, cs->smoothness_weight, total, 1, q,
+                                    P->t_max));
         return dense_check_run(r, a, dc, q);
     };
     auto iteration = [&](hipStream_t q) -> ppl_status {
@@ -2167,7 +2176,8 @@ ppl_status ppl_stomp_optimize(ppl_plan P, const ppl_scene_view* view, const ppl_
     auto evaluate = [&](hipStream_t q) -> ppl_status {
         ppl_status e = hinge_impl(P, view, d_theta, &cp, z->theta_cost, z->totals, nullptr, q);
         if (e != PPL_OK) return e;
-        HIP_TRY(launch_control_cost(d_theta, P->d_off, B, D, ss->dt, ss->control_cost_weight, z->totals, 1, q));
+        HIP_TRY(launch_control_cost(d_theta, P->d_off, B, D, ss->dt, ss->control_cost_weight, z->totals, 1, q,
+                                    P->t_max));
         return dense_check_run(r, a, dc, q);
     };
     if (first_call) {

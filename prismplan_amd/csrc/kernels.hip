@@ -2359,29 +2359,67 @@ __global__ __launch_bounds__(64) void k_segment_sum(const float* x, const int* o
 // metric R = dt * (A^T A)[pad..pad+T] (chomp_precompute.cpp:20-38), evaluated
 // as dt * sum_i acc_i^2 over the stencil rows that touch the trajectory,
 // acc_i = (-x[i-2] + 16x[i-1] - 30x[i] + 16x[i+1] - x[i+2]) / (12 dt^2) with
-// x = 0 outside [0, T). One wave per trajectory.
+// x = 0 outside [0, T). One wave per trajectory; with `stage` the wave first
+// copies its trajectory's D x T block into LDS rows padded by four zeros on
+// each side (eight loads per lane in flight at a time), then evaluates the
+// stencil from LDS. state (nullable): the trajectory's per-waypoint state
+// costs, whose sum joins the total (the STOMP rollout total in one launch).
 __global__ __launch_bounds__(64) void k_control(const float* theta, const int* traj_off, int n_traj, int D, float dt,
-                                                float w, float* out, int accumulate) {
+                                                float w, const float* state, float* out, int accumulate, int stage) {
+    extern __shared__ float s_x[];
     const int b = blockIdx.x;
     if (b >= n_traj) return;
     const int lane = lane_id();
     const int t0 = traj_off[b], Tn = traj_off[b + 1] - t0;
     const float* th = theta + (size_t)t0 * D;
     const float inv = 1.0f / (12.0f * dt * dt);
+    float ssum = 0.0f;
+    if (state)
+        for (int i = lane; i < Tn; i += kWave) ssum += state[t0 + i];
     float acc = 0.0f;
-    for (int d = 0; d < D; ++d) {
-        const float* x = th + (size_t)d * Tn;
-        for (int i = lane - 2; i < Tn + 2; i += kWave) {
-            auto X = [&](int k) { return (k >= 0 && k < Tn) ? x[k] : 0.0f; };
-            const float v = (-X(i - 2) + 16.0f * X(i - 1) - 30.0f * X(i) + 16.0f * X(i + 1) - X(i + 2)) * inv;
-            acc = fmaf(v, v, acc);
+    if (stage) {
+        const int rs = Tn + 8, n = D * rs;
+        for (int k0 = 0; k0 < n; k0 += 8 * kWave) {
+            float v[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int k = k0 + c * kWave + lane;
+                const int d = k / rs, i = k - d * rs - 4;
+                v[c] = (k < n && i >= 0 && i < Tn) ? th[(size_t)d * Tn + i] : 0.0f;
+            }
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int k = k0 + c * kWave + lane;
+                if (k < n) s_x[k] = v[c];
+            }
+        }
+        __syncthreads();
+        for (int d = 0; d < D; ++d) {
+            const float* x = s_x + d * rs + 4;
+            for (int i = lane - 2; i < Tn + 2; i += kWave) {
+                const float v = (-x[i - 2] + 16.0f * x[i - 1] - 30.0f * x[i] + 16.0f * x[i + 1] - x[i + 2]) * inv;
+                acc = fmaf(v, v, acc);
+            }
+        }
+    } else {
+        for (int d = 0; d < D; ++d) {
+            const float* x = th + (size_t)d * Tn;
+            for (int i = lane - 2; i < Tn + 2; i += kWave) {
+                auto X = [&](int k) { return (k >= 0 && k < Tn) ? x[k] : 0.0f; };
+                const float v = (-X(i - 2) + 16.0f * X(i - 1) - 30.0f * X(i) + 16.0f * X(i + 1) - X(i + 2)) * inv;
+                acc = fmaf(v, v, acc);
+            }
         }
     }
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
+    for (int o = 32; o >= 1; o >>= 1) {
+        acc += __shfl_xor(acc, o);
+        ssum += __shfl_xor(ssum, o);
+    }
     if (lane == 0) {
         const float c = w * dt * acc;
-        out[b] = accumulate ? out[b] + c : c;
+        const float base = accumulate ? out[b] : 0.0f;
+        out[b] = state ? base + (ssum + c) : base + c;
     }
 }
 
@@ -2690,9 +2728,13 @@ hipError_t launch_segment_sum(const float* x, const int* off, int n_seg, float, 
 }
 
 hipError_t launch_control_cost(const float* theta, const int* traj_off, int n_traj, int D, float dt, float w, float* out,
-                               int accumulate, hipStream_t s) {
+                               int accumulate, hipStream_t s, int t_max, const float* state) {
     if (n_traj <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_control, dim3(n_traj), dim3(64), 0, s, theta, traj_off, n_traj, D, dt, w, out, accumulate);
+    // LDS staging when the longest trajectory's padded block fits 32 KiB
+    const size_t lds = t_max > 0 ? (size_t)D * (t_max + 8) * sizeof(float) : 0;
+    const int stage = lds > 0 && lds <= 32768 ? 1 : 0;
+    hipLaunchKernelGGL(k_control, dim3(n_traj), dim3(64), stage ? lds : 0, s, theta, traj_off, n_traj, D, dt, w, state,
+                       out, accumulate, stage);
     return hipGetLastError();
 }
 

@@ -254,8 +254,11 @@ hipError_t launch_scatter(const float* v, const int* idx, int base, int n, float
 hipError_t launch_fk_out(const DRobot& R, const DStates& X, float* frames, float* spheres, hipStream_t s);
 hipError_t launch_segment_sum(const float* x, const int* off, int n_seg, float scale_unused, float* out,
                               hipStream_t s);
+// t_max: the longest trajectory (0: unknown, no LDS staging); state
+// (nullable): per-waypoint state costs whose per-trajectory sum joins out.
 hipError_t launch_control_cost(const float* theta, const int* traj_off, int n_traj, int D, float dt,
-                               float w, float* out, int accumulate, hipStream_t s);
+                               float w, float* out, int accumulate, hipStream_t s, int t_max = 0,
+                               const float* state = nullptr);
 hipError_t launch_esdf_build(const float* obb_dims, const float* obb_pose, int n_obbs,
                              const float* voxel_params, const float* voxel_pose, float* out,
                              hipStream_t s);
