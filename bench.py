@@ -46,18 +46,34 @@ GATED_FLOPS_EDGES = 3191.0
 FLOPS_RULE = ("achieved = gated early-exit flops per state x states / kernel time; the gated count is the FP64 "
               "oracle's mean over a 20,000-state sample of the work its two-phase check does (FK, approximate "
               "spheres, exact spheres of flagged joints against every OBB; include/ppl/roofline.h item costs)")
-# The headline kernel's instruction counters (rocprofv3 --pmc, one launch;
-# profiles/edges_collide_instruction_mix.txt): VALU instructions and
+# The headline kernel's counters (rocprofv3 --pmc, median per dispatch over
+# the timed loop; profiles/edges_collide_counters.txt): VALU instructions and
 # SQ_INSTS_VALU_FLOPS_FP32 per dispatch, with the kernel time of that run.
-COUNTER_VALU_INSTS_EDGES = 15_144_020
-COUNTER_VALU_FLOPS_FP32_EDGES = 7_922_712
-COUNTER_KERNEL_US_EDGES = 44.98
+COUNTER_VALU_INSTS_EDGES = 15_578_828
+COUNTER_VALU_FLOPS_FP32_EDGES = 7_930_904
+COUNTER_KERNEL_US_EDGES = 44.91
 CUS, SIMDS_PER_CU, CLOCK_HZ = 256, 4, 2.4e9
 # HBM bytes one headline launch moves (k_collide), from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE scaled by a 256 MiB copy (profiles/edges_collide_hbm_traffic.txt);
-# the algorithmic bytes are 16,384 edges x 69 B.
-TRAFFIC_BYTES_EDGES = 2_101_488
+# WRITE_SIZE scaled by a 256 MiB copy (same profile); the algorithmic bytes are
+# 16,384 edges x 69 B.
+TRAFFIC_BYTES_EDGES = 2_132_185
 ALGORITHMIC_BYTES_EDGES = 16384 * 69
+
+
+def kernel_time(plan, fn, steps, torch):
+    """Mean duration (s) of the plan's main kernel over `steps` calls of fn,
+    from the HIP event pairs the library records on the kernel's own stream
+    around each launch (ppl_plan_time_kernels)."""
+    torch.cuda.synchronize()
+    plan.kernel_ms()
+    plan.time_kernels(True)
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    ms, n = plan.kernel_ms()
+    plan.time_kernels(False)
+    assert n == steps, f"{n} timed launches for {steps} calls"
+    return ms / n * 1e-3
 
 
 def log(msg):
@@ -130,13 +146,19 @@ def main():
     fh = torch.empty(len(e.n_states), dtype=torch.int32, device=dev)
     total_states = e.total_states
     log(f"edges: E={len(e.n_states)} states={total_states} sha256={e.sha256()[:16]}")
+    # launch shape (OBB staging tile x grid form) from the autotuner, before the timed region
+    c_tile, c_grid, c_ms = plan.autotune_collide(ds, frm, to, reps=5)
+    log(f"collide shape: tile {c_tile}, grid form {c_grid} ({c_ms * 1e3:.1f} us per first-hit call)")
 
     def step():
         S.edge_check_batched(plan, ds, frm, to, skip_resolved=False, first_hit=fh)
 
-    wall, ev = timed_loop(step, args.steps, args.warmup, torch)
+    wall, ev_step = timed_loop(step, args.steps, args.warmup, torch)
     value = total_states / wall
-    log(f"headline: {wall * 1e3:.3f} ms/step wall, {ev * 1e3:.3f} ms event -> {value:.3e} states/s")
+    # the roofline's kernel time: k_collide alone (the step also resets the first hits)
+    ev = kernel_time(plan, step, args.steps, torch)
+    log(f"headline: {wall * 1e3:.3f} ms/step wall, {ev_step * 1e3:.3f} ms step event, {ev * 1e3:.4f} ms k_collide "
+        f"-> {value:.3e} states/s")
     fh_host = fh.cpu().numpy()
     hit_edges = fh_host >= 0
     # the workload's 64 boxes of up to 0.6 m fill the Panda's reach: edges collide early
@@ -160,6 +182,8 @@ def main():
                 "flops_per_state": f_state, "flops_rule": FLOPS_RULE,
                 "exhaustive_flops_per_state": f_exh,
                 "exhaustive_frac": f_exh * total_states / ev / PEAK_FP32, "kernel_ms": ev * 1e3,
+                "kernel_ms_source": "HIP events the library records around each k_collide launch on its stream",
+                "step_event_ms": ev_step * 1e3,
                 # counter-derived, from the committed profile of this kernel: lane flops if every lane of
                 # each FP32 VALU instruction were active (an upper bound), and VALU issue occupancy
                 # (wave64 VALU instructions x 4 cycles over every SIMD's cycles at 2.4 GHz)
@@ -202,6 +226,7 @@ def main():
                    "edges": int(len(e.n_states)), "states": int(total_states), "obbs": int(e.scene.n_obbs),
                    **workload_stats},
         "roofline": roofline, "cpu_baseline": cpu_baseline, "parity": parity,
+        "collide_shape": {"obb_tile": c_tile, "grid_form": c_grid, "source": "ppl_plan_autotune_collide"},
         "device": props.name, **extras,
     }
     print(json.dumps(line), flush=True)
@@ -537,6 +562,7 @@ def bench_chomp(panda, S, W, torch, dev, args):
     prm = S.cost_params(world_margin=0.05, weight=1.0, use_self=False)
     tuned = plan.autotune_cost(ds, th, prm, grad=True)
     wall_a, ev_a = timed_loop(lambda: S.hinge_cost(plan, ds, th, prm, grad=True), args.steps, args.warmup, torch)
+    k_a = kernel_time(plan, lambda: S.hinge_cost(plan, ds, th, prm, grad=True), args.steps, torch)
     wall_f, ev_f = timed_loop(lambda: S.chomp_fd_grad(plan, ds, th, prm, eps=1e-3), max(5, args.steps // 5), 2, torch)
     # the reference's call pattern: one FD launch per trajectory (plain per-trajectory form)
     D, Tw = panda.n_dof, int(w.lengths()[0])
@@ -557,6 +583,7 @@ def bench_chomp(panda, S, W, torch, dev, args):
             "fd_per_trajectory_launch_traj_per_s": B / wall_fp, "adjoint_over_fd_per_trajectory": wall_fp / wall_a,
             "adjoint_flops_per_traj": f_adj, "fd_flops_per_traj": f_fd, "adjoint_lanes_per_state": tuned[0],
             "adjoint_roofline_frac": B * f_adj / ev_a / PEAK_FP32,
+            "adjoint_kernel_ms": k_a * 1e3, "adjoint_kernel_roofline_frac": B * f_adj / k_a / PEAK_FP32,
             "fd_roofline_frac": B * f_fd / ev_f / PEAK_FP32}
 
 
@@ -731,12 +758,14 @@ def bench_esdf(S, W, torch, dev, args):
     prm = S.cost_params(world_margin=0.02, weight=1.0, use_obb=False, use_self=False)
     tuned = plan.autotune_cost(ds, th, prm, grad=True)
     wall, ev = timed_loop(lambda: S.hinge_cost(plan, ds, th, prm, grad=True), args.steps, args.warmup, torch)
+    k_a = kernel_time(plan, lambda: S.hinge_cost(plan, ds, th, prm, grad=True), args.steps, torch)
     from prismplan_amd import _ppl
     f_wp = baxter.flops(_ppl.WORK_ESDF_WAYPOINT)
     return {"waypoints_per_s": wk.n_waypoints / wall, "trajectories_per_s": wk.n_traj / wall,
             "adjoint_ms": wall * 1e3, "esdf_build_ms": wall_b * 1e3, "waypoints": wk.n_waypoints,
             "flops_per_waypoint": f_wp, "lanes_per_state": tuned[0],
-            "roofline_frac": wk.n_waypoints * f_wp / ev / PEAK_FP32}
+            "roofline_frac": wk.n_waypoints * f_wp / ev / PEAK_FP32,
+            "adjoint_kernel_ms": k_a * 1e3, "kernel_roofline_frac": wk.n_waypoints * f_wp / k_a / PEAK_FP32}
 
 
 if __name__ == "__main__":

@@ -281,8 +281,9 @@ ppl_status ppl_hinge_cost_batched(ppl_plan plan, const ppl_scene_view* scene, co
  * candidate on `stream`; outputs go to plan scratch. The choice is also
  * recorded in the tuning cache (ppl_tune_cache_file) under the robot's
  * tables, the largest problem's OBB count, the state count's power of two
- * and the GPU architecture; untuned plans that match such an entry use it.
- * No counterpart in the reference. */
+ * and the GPU architecture; untuned plans that match such an entry use it,
+ * and a later autotuning call that finds it adopts it without timing
+ * (*best_ms: the time recorded with it). No counterpart in the reference. */
 ppl_status ppl_plan_autotune_cost(ppl_plan plan, const ppl_scene_view* scene, const float* d_x,
                                   const ppl_cost_params* params, int grad, int reps, int* best_group,
                                   float* best_ms, ppl_stream stream);
@@ -312,9 +313,11 @@ ppl_status ppl_tune_cache_file(const char* path);
  * (0: persistent at occupancy, 1: twice that, 2: one workgroup per 64-state
  * chunk), keeps the fastest for this plan's later any-collision calls and
  * records it in the tuning cache under the robot's tables, the largest
- * problem's OBB count and the GPU architecture. The fixed tile it replaces
- * is the reference's BATCH_SIZE (collision/scene_collision.cuh:9-11). Results
- * do not depend on the shape. No counterpart in the reference. */
+ * problem's OBB count and the GPU architecture; when the cache already holds
+ * a choice for that key it is adopted without timing (*best_ms: the time
+ * recorded with it). The fixed tile it replaces is the reference's
+ * BATCH_SIZE (collision/scene_collision.cuh:9-11). Results do not depend on
+ * the shape. No counterpart in the reference. */
 ppl_status ppl_plan_autotune_collide(ppl_plan plan, const ppl_scene_view* scene, const float* d_x, const float* d_x2,
                                      int reps, int* best_tile, int* best_grid, float* best_ms, ppl_stream stream);
 
@@ -328,6 +331,19 @@ ppl_status ppl_plan_collide_shape(ppl_plan plan, int max_obbs, int* tile, int* g
  * staging, up to 256; grid 0, 1, 2 as above); (0, 0) returns to the tuning
  * cache / built-in choice. */
 ppl_status ppl_plan_set_collide_shape(ppl_plan plan, int tile, int grid);
+
+/* Kernel timing (enable != 0: on): each later call of
+ * ppl_check_collision_batched, ppl_edge_check_batched or the batched cost
+ * evaluation on this plan records a HIP event pair on its stream around its
+ * main kernel (k_collide or the cost kernel; not the first-hit reset or the
+ * self pass), so the kernel's own duration can be read without a profiler.
+ * Calls under stream capture record nothing; at most 4096 launches are kept
+ * until read. No counterpart in the reference. */
+ppl_status ppl_plan_time_kernels(ppl_plan plan, int enable);
+
+/* Waits for the timed launches recorded since the last read, returns their
+ * summed duration (*total_ms) and count (*launches), and clears them. */
+ppl_status ppl_plan_kernel_ms(ppl_plan plan, double* total_ms, int* launches);
 
 /* Central-FD obstacle gradient over a trajectory plan (chomp_obs_grad_kernel
  * semantics), the baseline for the fused adjoint. */

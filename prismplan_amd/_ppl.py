@@ -158,6 +158,8 @@ SIGNATURES = [
                                             C.POINTER(C.c_int), C.POINTER(C.c_float), _vp]),
     ("ppl_plan_collide_shape", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("ppl_plan_set_collide_shape", C.c_int, [_vp, C.c_int, C.c_int]),
+    ("ppl_plan_time_kernels", C.c_int, [_vp, C.c_int]),
+    ("ppl_plan_kernel_ms", C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     ("ppl_plan_set_optimizer_graphs", C.c_int, [_vp, C.c_int]),
     ("ppl_chomp_fd_grad_batched", C.c_int, [_vp, C.POINTER(SceneView), _vp, C.POINTER(CostParams), C.c_float, _vp,
                                             _vp]),

@@ -192,6 +192,10 @@ struct ppl_plan_s {
     ppl_graph chomp_graph = nullptr, stomp_graph = nullptr;
     unsigned long long chomp_key = 0, stomp_key = 0;
     unsigned* d_iter = nullptr;      // STOMP graphs: the iteration word of the sampling counter
+    // kernel timing (ppl_plan_time_kernels): an event pair per timed launch
+    bool ktime = false;
+    int kused = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> kev;
 };
 
 struct ppl_graph_s {
@@ -260,6 +264,39 @@ DChunks implicit_chunks(long n_states, KernelKind k, const DRobot& R, int max_ob
 // The cost kernel over n_states implicit chunks: k_cost with one lane per
 // state when the scene's OBBs fit one LDS staging, else the grouped kernel,
 // which streams them through LDS in tiles.
+// Kernel timing: with the plan's timing on, the batched checks and the cost
+// evaluation record an event pair on their stream around their main kernel
+// (not under stream capture; at most kMaxTimedLaunches pairs until read).
+constexpr int kMaxTimedLaunches = 4096;
+
+bool timing_now(ppl_plan P, hipStream_t s) {
+    if (!P->ktime || P->kused >= kMaxTimedLaunches) return false;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone;
+}
+
+hipError_t time_begin(ppl_plan P, hipStream_t s, bool on) {
+    if (!on) return hipSuccess;
+    if ((int)P->kev.size() == P->kused) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipEventCreate(&e0);
+        if (e != hipSuccess) return e;
+        if ((e = hipEventCreate(&e1)) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            return e;
+        }
+        P->kev.emplace_back(e0, e1);
+    }
+    return hipEventRecord(P->kev[P->kused].first, s);
+}
+
+hipError_t time_end(ppl_plan P, hipStream_t s, bool on) {
+    if (!on) return hipSuccess;
+    const hipError_t e = hipEventRecord(P->kev[P->kused].second, s);
+    if (e == hipSuccess) ++P->kused;
+    return e;
+}
+
 hipError_t launch_cost_flat(CostArgs& a, long n_states, hipStream_t s) {
     int grid = 0;
     const bool tiled = a.S.n_obbs > kMaxObbLds;
@@ -1439,6 +1476,10 @@ ppl_status ppl_plan_destroy(ppl_plan P) try {
     }
     P->d_opt.release();
     P->d_self.release(); P->d_cost.release(); P->d_fdcfg.release(); P->d_fdcost.release();
+    for (auto& e : P->kev) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
     delete P;
     return PPL_OK;
 } catch (const std::bad_alloc&) {
@@ -1446,6 +1487,34 @@ ppl_status ppl_plan_destroy(ppl_plan P) try {
 }
 
 long ppl_plan_states(ppl_plan P) { return P ? P->n_states : -1; }
+
+ppl_status ppl_plan_time_kernels(ppl_plan P, int enable) try {
+    CHECK_ARG(P, "NULL plan");
+    P->ktime = enable != 0;
+    return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
+}
+
+ppl_status ppl_plan_kernel_ms(ppl_plan P, double* total_ms, int* launches) try {
+    CHECK_ARG(P && total_ms && launches, "NULL argument");
+    double sum = 0.0;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < P->kused && e == hipSuccess; ++i) {
+        float ms = 0.0f;
+        e = hipEventSynchronize(P->kev[i].second);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, P->kev[i].first, P->kev[i].second);
+        sum += ms;
+    }
+    const int n = P->kused;
+    P->kused = 0;
+    if (e != hipSuccess) return fail(PPL_ERR_HIP, "kernel timing: %s", hipGetErrorString(e));
+    *total_ms = sum;
+    *launches = n;
+    return PPL_OK;
+} catch (const std::bad_alloc&) {
+    return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
+}
 
 double ppl_algorithmic_flops(ppl_robot r, int kind, int n_obbs, int n_waypoints, int use_self) {
     if (!r || n_obbs < 0) return -1.0;
@@ -1561,7 +1630,10 @@ ppl_status hinge_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_
     a.self_in = self_in;
     a.cost_out = state;
     a.grad_out = d_grad;
+    const bool timed = timing_now(P, s);
+    HIP_TRY(time_begin(P, s, timed));
     HIP_TRY(grouped ? launch_cost_grp(a, grid, s) : launch_cost(a, grid, s));
+    HIP_TRY(time_end(P, s, timed));
     if (d_total && P->kind == PPL_PLAN_TRAJ) HIP_TRY(launch_segment_sum(state, P->d_off, P->n_units, 0.0f, d_total, s));
     return PPL_OK;
 }
@@ -2320,7 +2392,10 @@ ppl_status ppl_check_collision_batched(ppl_plan P, const ppl_scene_view* view, c
     a.mode = 0;
     a.out = d_out;
     a.self_hit = self_hit;
+    const bool timed = timing_now(P, s);
+    HIP_TRY(time_begin(P, s, timed));
     HIP_TRY(launch_collide(a, grid, s));
+    HIP_TRY(time_end(P, s, timed));
     return PPL_OK;
 } catch (const std::bad_alloc&) {
     return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
@@ -2360,7 +2435,10 @@ ppl_status ppl_edge_check_batched(ppl_plan P, const ppl_scene_view* view, const 
     a.out = d_state_hits;
     a.first_hit = d_first_hit;
     a.self_hit = self_hit;
+    const bool timed = timing_now(P, s);
+    HIP_TRY(time_begin(P, s, timed));
     HIP_TRY(launch_collide(a, grid, s));
+    HIP_TRY(time_end(P, s, timed));
     return PPL_OK;
 } catch (const std::bad_alloc&) {
     return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
@@ -2695,6 +2773,21 @@ ppl_status ppl_plan_autotune_cost(ppl_plan P, const ppl_scene_view* view, const 
     DScene S{};
     ppl_status st = plan_scene(P, view, S);
     if (st != PPL_OK) return st;
+    {
+        // a choice already in the tuning cache is adopted without timing
+        std::string v;
+        int g = 0;
+        float ms = 0.0f;
+        const int cap0 = plan_obb_cap(P, S);
+        if (tune_get(cost_key(P->robot, grad != 0, cap0, P->n_states), v) && sscanf(v.c_str(), "%d %f", &g, &ms) == 2 &&
+            (g == PPL_COST_GROUPED || (g >= 1 && g <= kMaxGroup && (g & (g - 1)) == 0)) &&
+            group_fits(P, g, grad != 0, cap0)) {
+            P->tuned_group[grad ? 1 : 0] = g;
+            if (best_group) *best_group = g;
+            if (best_ms) *best_ms = ms;
+            return PPL_OK;
+        }
+    }
     const int D = P->robot->R.n_dof;
     const size_t n = (size_t)P->n_states;
     HIP_TRY(P->d_tune.ensure((n * (grad ? D + 1 : 1) + (size_t)P->n_units) * sizeof(float)));
@@ -2768,6 +2861,23 @@ ppl_status ppl_plan_autotune_collide(ppl_plan P, const ppl_scene_view* view, con
     if (st != PPL_OK) return st;
     const int max_obbs = plan_obb_cap(P, S);
     const bool edges = P->kind == PPL_PLAN_EDGES;
+    {
+        // a choice already in the tuning cache is adopted without timing
+        std::string v;
+        int t = 0, g = 0;
+        float ms = 0.0f;
+        if (tune_get(collide_key(P->robot, max_obbs), v) && sscanf(v.c_str(), "%d %d %f", &t, &g, &ms) == 3 && t >= 0 &&
+            t <= kMaxObbLds && g >= 0 && g <= 2) {
+            P->ctile = t;
+            P->cgrid = g;
+            P->shape_src = 2;
+            P->shape_obbs = max_obbs;
+            if (best_tile) *best_tile = t > 0 ? std::min(t, obb_tile(max_obbs)) : obb_tile(max_obbs);
+            if (best_grid) *best_grid = g;
+            if (best_ms) *best_ms = ms;
+            return PPL_OK;
+        }
+    }
     HIP_TRY(P->d_tune.ensure(edges ? (size_t)P->n_units * 4 : (size_t)P->n_states));
     hipStream_t s = (hipStream_t)stream;
     ppl_edge_batch_params ep{};

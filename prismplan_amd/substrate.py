@@ -315,6 +315,18 @@ class Plan:
     def set_collide_shape(self, tile: int, grid: int):
         call("ppl_plan_set_collide_shape", self.handle, int(tile), int(grid))
 
+    def time_kernels(self, enable: bool = True):
+        """Record a HIP event pair around the main kernel of each later batched
+        check or cost evaluation on this plan (read with kernel_ms)."""
+        call("ppl_plan_time_kernels", self.handle, int(bool(enable)))
+
+    def kernel_ms(self):
+        """(summed ms, launches) of the kernels timed since the last read;
+        waits for them and clears the record."""
+        ms, n = C.c_double(0.0), C.c_int(0)
+        call("ppl_plan_kernel_ms", self.handle, C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
     def close(self):
         if getattr(self, "handle", None):
             lib().ppl_plan_destroy(self.handle)

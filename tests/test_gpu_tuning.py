@@ -63,6 +63,9 @@ def test_collide_autotune_edges_and_cache(cache_file):
     # after the file is read afresh (a new process); other robots do not
     plan2 = S.Plan.edges(panda, e.n_states[:500])
     assert plan2.collide_shape(64) == (tile, grid, 1)
+    # a later autotuning call adopts the cached choice (and its recorded time) without timing
+    t2, g2, ms2 = plan2.autotune_collide(ds, frm[:500], to[:500], reps=3)
+    assert (t2, g2) == (tile, grid) and abs(ms2 - ms) < 1e-6 * max(1.0, ms)
     S.tune_cache_file(cache_file)
     plan3 = S.Plan.configs(panda, 100)
     assert plan3.collide_shape(64) == (tile, grid, 1)
@@ -93,6 +96,8 @@ def test_cost_autotune_is_cached(cache_file):
     prm = S.cost_params(world_margin=0.05, weight=1.0, use_self=False)
     plan = S.Plan.trajectories(panda, w.lengths())
     G, ms = plan.autotune_cost(ds, th, prm, grad=True, reps=3)
+    G2, ms2 = S.Plan.trajectories(panda, w.lengths()).autotune_cost(ds, th, prm, grad=True, reps=3)
+    assert G2 == G and abs(ms2 - ms) < 1e-6   # adopted from the cache (time kept to 1 ns)
     ent = entries(cache_file)
     key = [k for k in ent if k.startswith("adjoint/")]
     assert len(key) == 1 and int(ent[key[0]][0]) == G
@@ -103,3 +108,59 @@ def test_cost_autotune_is_cached(cache_file):
     s2, t2, g2 = S.hinge_cost(plan2, ds, th, prm, grad=True)
     torch.testing.assert_close(s2, s1, rtol=0, atol=0)
     torch.testing.assert_close(g2, g1, rtol=0, atol=0)
+
+
+def test_kernel_timing_events():
+    """ppl_plan_time_kernels: one event pair per timed launch of the main
+    kernel, none when off or under graph capture, and the kernel alone takes
+    less than the whole call (which also resets the first hits)."""
+    panda = S.Robot("panda")
+    e = W.edges(n_edges=2048, seed=3)
+    ds = S.DeviceScene.from_host(e.scene)
+    plan = S.Plan.edges(panda, e.n_states)
+    frm, to = torch.from_numpy(e.frm).to(DEV), torch.from_numpy(e.to).to(DEV)
+    fh = torch.empty(len(e.n_states), dtype=torch.int32, device=DEV)
+
+    def call():
+        S.edge_check_batched(plan, ds, frm, to, skip_resolved=False, first_hit=fh)
+
+    call()
+    torch.cuda.synchronize()
+    assert plan.kernel_ms() == (0.0, 0)
+    plan.time_kernels(True)
+    for _ in range(5):
+        call()
+    ms, n = plan.kernel_ms()
+    assert n == 5 and ms > 0.0
+    assert plan.kernel_ms() == (0.0, 0)   # read clears
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(5):
+        call()
+    t1.record()
+    torch.cuda.synchronize()
+    ms, n = plan.kernel_ms()
+    banner(f"k_collide {ms / n * 1e3:.1f} us per launch; whole call {t0.elapsed_time(t1) / 5 * 1e3:.1f} us")
+    assert n == 5 and ms < t0.elapsed_time(t1)
+    # cost kernel of a trajectory plan; graph capture records nothing
+    w = W.chomp(n_traj=16, n_wp=32, n_obbs=20, seed=4)
+    dsc = S.DeviceScene.from_host(w.scene)
+    tp = S.Plan.trajectories(panda, w.lengths())
+    th = torch.from_numpy(w.theta).to(DEV)
+    prm = S.cost_params(world_margin=0.05, weight=1.0, use_self=False)
+    S.hinge_cost(tp, dsc, th, prm, grad=True)
+    tp.time_kernels(True)
+    S.hinge_cost(tp, dsc, th, prm, grad=True)
+    torch.cuda.synchronize()
+    state = torch.empty(tp.n_states, device=DEV)
+    total = torch.empty(len(w.lengths()), device=DEV)
+    grad = torch.empty_like(th)
+    g = S.capture_hinge(tp, dsc, th, prm, state, total, grad)   # one eager warm-up call, then the capture
+    torch.cuda.synchronize()
+    assert tp.kernel_ms()[1] == 2
+    g.launch()
+    torch.cuda.synchronize()
+    assert tp.kernel_ms() == (0.0, 0)
+    tp.time_kernels(False)
+    S.hinge_cost(tp, dsc, th, prm, grad=True)
+    assert tp.kernel_ms() == (0.0, 0)
