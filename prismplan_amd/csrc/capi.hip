@@ -848,42 +848,32 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     }
     const int n_clu = (int)clu_enc.size();
     CHECK_ARG(n_clu <= kMaxClusters, "%d sphere clusters, at most %d supported", n_clu, kMaxClusters);
-    // Self-pair blocks for the self pass: the pairs of DFS joints (A, B),
-    // A <= B, cut into at most 64 x 64 sphere tiles in emission order, with a
-    // 64-bit partner mask per row. Emission order keeps each joint's spheres
-    // contiguous, so a block names its rows and columns by first slot.
-    std::vector<int4> sblk;
-    std::vector<unsigned long long> rowmask;
+    // Self-pair blocks for the self pass: the exact pairs grouped by the
+    // sphere clusters (A, B), A <= B, of their two spheres, each block a list
+    // of (slot a | slot b << 16) in slot order. A state tests each block's two
+    // cluster enclosing spheres and walks the pair lists of the blocks that
+    // pass, 16 pairs per step.
+    CHECK_ARG(S <= 65535, "%d spheres, at most 65535 supported", S);
+    std::vector<int4> cpb;
+    std::vector<unsigned> cpl;
     {
-        std::vector<int> slot_of(S);
+        std::vector<int> slot_of(S), clu_of_slot(S);
         for (int k = 0; k < S; ++k) slot_of[sph_id[k]] = k;
-        std::vector<int> joint_of_slot(S);
-        for (int k = 0; k < J; ++k)
-            for (int t = sph_off[k]; t < sph_off[k + 1]; ++t) joint_of_slot[t] = k;
-        // key (A, B, row chunk, column chunk) -> (row slot, column slot) bits
-        std::vector<std::pair<std::array<int, 4>, std::pair<int, int>>> ent;
+        for (int c = 0; c + 1 < (int)clu_beg.size(); ++c)
+            for (int t = clu_beg[c]; t < clu_beg[c + 1]; ++t) clu_of_slot[t] = c;
+        std::vector<std::array<int, 4>> ent;   // (cluster A, cluster B, slot a, slot b)
         ent.reserve(pairs.size());
         for (const int2& p : pairs) {
             int u = slot_of[p.x], v = slot_of[p.y];
-            if (joint_of_slot[u] > joint_of_slot[v] || (joint_of_slot[u] == joint_of_slot[v] && u > v)) std::swap(u, v);
-            const int ja = joint_of_slot[u], jb = joint_of_slot[v];
-            const int ca = (u - sph_off[ja]) / 64, cb = (v - sph_off[jb]) / 64;
-            ent.push_back({{ja, jb, ca, cb}, {u, v}});
+            if (clu_of_slot[u] > clu_of_slot[v] || (clu_of_slot[u] == clu_of_slot[v] && u > v)) std::swap(u, v);
+            ent.push_back({clu_of_slot[u], clu_of_slot[v], u, v});
         }
-        std::stable_sort(ent.begin(), ent.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        std::sort(ent.begin(), ent.end());
         for (size_t k = 0; k < ent.size();) {
             size_t e = k;
-            while (e < ent.size() && ent[e].first == ent[k].first) ++e;
-            const auto& key = ent[k].first;
-            const int ja = key[0], jb = key[1];
-            const int a0 = sph_off[ja] + 64 * key[2], b0 = sph_off[jb] + 64 * key[3];
-            const int na = std::min(64, sph_off[ja + 1] - a0), nb = std::min(64, sph_off[jb + 1] - b0);
-            const int moff = (int)rowmask.size();
-            rowmask.resize(rowmask.size() + na, 0ull);
-            for (size_t t = k; t < e; ++t)
-                rowmask[moff + (ent[t].second.first - a0)] |= 1ull << (ent[t].second.second - b0);
-            sblk.push_back(make_int4(ja, jb, a0, na));
-            sblk.push_back(make_int4(b0, nb, moff, 0));
+            while (e < ent.size() && ent[e][0] == ent[k][0] && ent[e][1] == ent[k][1]) ++e;
+            cpb.push_back(make_int4(ent[k][0], ent[k][1], (int)cpl.size(), (int)(e - k)));
+            for (size_t t = k; t < e; ++t) cpl.push_back((unsigned)ent[t][2] | ((unsigned)ent[t][3] << 16));
             k = e;
         }
     }
@@ -959,7 +949,7 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
                  o_apx = place(apx.size() * 16), o_aid = place(apx_id.size() * 4), o_close = place(close.size() * 4),
                  o_sorig = place(sph_orig.size() * 16), o_pairs = place(pairs.size() * 8),
                  o_apairs = place(apairs.size() * 8), o_aorig = place(apx_orig.size() * 16),
-                 o_sblk = place(sblk.size() * 16), o_rmask = place(rowmask.size() * 8), o_jorig = place(J * 4),
+                 o_cpb = place(cpb.size() * 16), o_cpl = place(cpl.size() * 4), o_jorig = place(J * 4),
                  o_lim = place((size_t)D * 8),
                  o_jenc = place((size_t)J * 16), o_sjnt = place((size_t)S * 4), o_jend = place((size_t)J * 4),
                  o_aft = place(aft.size() * 4), o_atype = place(atype.size() * 4), o_ajnt = place(apx_jnt.size() * 4),
@@ -980,8 +970,8 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     put(o_pairs, pairs.data(), pairs.size() * 8);
     put(o_apairs, apairs.data(), apairs.size() * 8);
     put(o_aorig, apx_orig.data(), apx_orig.size() * 16);
-    put(o_sblk, sblk.data(), sblk.size() * 16);
-    put(o_rmask, rowmask.data(), rowmask.size() * 8);
+    put(o_cpb, cpb.data(), cpb.size() * 16);
+    put(o_cpl, cpl.data(), cpl.size() * 4);
     put(o_jorig, dfs.data(), (size_t)J * 4);
     std::vector<float2> lim(D);
     for (int k = 0; k < D; ++k) lim[k] = make_float2(d->joint_lower[k], d->joint_upper[k]);
@@ -1029,8 +1019,8 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     R.pairs = reinterpret_cast<const int2*>(b + o_pairs);
     R.apx_pairs = reinterpret_cast<const int2*>(b + o_apairs);
     R.apx_orig = reinterpret_cast<const float4*>(b + o_aorig);
-    R.sblk = reinterpret_cast<const int4*>(b + o_sblk);
-    R.rowmask = reinterpret_cast<const unsigned long long*>(b + o_rmask);
+    R.cpb = reinterpret_cast<const int4*>(b + o_cpb);
+    R.cpl = reinterpret_cast<const unsigned*>(b + o_cpl);
     R.jorig = reinterpret_cast<const int*>(b + o_jorig);
     R.lim = reinterpret_cast<const float2*>(b + o_lim);
     R.jenc = reinterpret_cast<const float4*>(b + o_jenc);
@@ -1047,7 +1037,7 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     R.clu_jnt = reinterpret_cast<const int*>(b + o_cjnt);
     R.scene_gate_fused = scene_gate_fused ? 1 : 0;
     R.self_gate_implied = self_gate_implied ? 1 : 0;
-    R.n_sblk = (int)sblk.size() / 2;
+    R.n_cpb = (int)cpb.size();
     r->n_slots = n_slots;
     r->n_pairs = (int)pairs.size();
     for (int i = 0; i < J; ++i) {
@@ -1614,11 +1604,14 @@ ppl_status hinge_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_
         }
         tuned = P->cost_cached[gi];
     }
-    // default: the grouped layout for the adjoint, the lane-group heuristic for the cost alone
+    // default: the grouped layout for the adjoint and for ESDF-only costs
+    // (its ESDF variant keeps five waves per SIMD on the grid lookups), the
+    // lane-group heuristic for the cost alone over boxes
+    const bool esdf_only = !(prm->use_obb && S.n_obbs > 0) && prm->use_esdf && S.n_voxel_layers > 0;
     const int G = max_obbs > kMaxObbLds ? PPL_COST_GROUPED
                 : force_G != 0          ? force_G
                 : tuned != 0            ? tuned
-                : d_grad                ? PPL_COST_GROUPED
+                : d_grad || esdf_only   ? PPL_COST_GROUPED
                                         : pick_group(P->n_states, P->robot->R, false);
     const bool grouped = G == PPL_COST_GROUPED;
     HIP_TRY(ensure_table(P, grouped ? kGrp : G));

@@ -1597,9 +1597,10 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
     const int st = lane / kSelfGroup;
     const int S = R.n_sph, D = R.n_dof, J = R.n_joints;
     float4* s_pos = s_dyn4;                                          // [kSelfStates][S] (x, y, z, r)
-    float4* s_jnt = s_dyn4 + kSelfStates * S;                         // [kSelfStates][J] enclosing sphere, r < 0: none
+    const int NC = R.n_clu;
+    float4* s_cenc = s_dyn4 + kSelfStates * S;                        // [kSelfStates][NC] cluster enclosing spheres
     const int fstride = J * 12 + 4;                                   // frame block per state, padded
-    float* s_T = reinterpret_cast<float*>(s_jnt + kSelfStates * J);   // [kSelfStates][J][12] joint frames
+    float* s_T = reinterpret_cast<float*>(s_cenc + kSelfStates * NC); // [kSelfStates][J][12] joint frames
     float* s_q = s_T + kSelfStates * fstride;                         // [kMaxDof][kSelfStates]
     int* s_par = reinterpret_cast<int*>(s_q + kMaxDof * kSelfStates); // [kSelfStates][J] pointer-jumping links
     int* s_end = s_par + kSelfStates * J;                             // [2][64] unit ends / starts of a chunk
@@ -1651,11 +1652,11 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
     }
     __syncthreads();
     float4* pos = s_pos + st * S;
-    float4* jnt = s_jnt + st * J;
+    float4* cenc = s_cenc + st * NC;
     float* Ts = s_T + st * fstride;
     group_frames(R, s_q + st, kSelfStates, Ts, s_par + st * J, sub);
     // sphere centres (emission slot k: a joint's spheres are contiguous) and
-    // each joint's enclosing sphere, from the frames
+    // each sphere cluster's enclosing sphere, from the frames
     for (int k = sub; k < S; k += kSelfGroup) {
         const float4 sp = R.sph[k];
         const float4* Tq = reinterpret_cast<const float4*>(Ts + 12 * R.sph_jnt[k]);
@@ -1664,13 +1665,13 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
                              fmaf(t1.x, sp.x, fmaf(t1.y, sp.y, fmaf(t1.z, sp.z, t1.w))),
                              fmaf(t2.x, sp.x, fmaf(t2.y, sp.y, fmaf(t2.z, sp.z, t2.w))), sp.w);
     }
-    if (sub < J) {
-        const float4 je = R.jenc[sub];
-        const float4* Tq = reinterpret_cast<const float4*>(Ts + 12 * sub);
+    for (int c = sub; c < NC; c += kSelfGroup) {
+        const float4 ce = R.clu_enc[c];
+        const float4* Tq = reinterpret_cast<const float4*>(Ts + 12 * R.clu_jnt[c]);
         const float4 t0 = Tq[0], t1 = Tq[1], t2 = Tq[2];
-        jnt[sub] = make_float4(fmaf(t0.x, je.x, fmaf(t0.y, je.y, fmaf(t0.z, je.z, t0.w))),
-                               fmaf(t1.x, je.x, fmaf(t1.y, je.y, fmaf(t1.z, je.z, t1.w))),
-                               fmaf(t2.x, je.x, fmaf(t2.y, je.y, fmaf(t2.z, je.z, t2.w))), je.w);
+        cenc[c] = make_float4(fmaf(t0.x, ce.x, fmaf(t0.y, ce.y, fmaf(t0.z, ce.z, t0.w))),
+                              fmaf(t1.x, ce.x, fmaf(t1.y, ce.y, fmaf(t1.z, ce.z, t1.w))),
+                              fmaf(t2.x, ce.x, fmaf(t2.y, ce.y, fmaf(t2.z, ce.z, t2.w))), ce.w);
     }
     __syncthreads();
     bool live = active;   // states whose pairs are checked (gated: an approximate pair collides)
@@ -1702,77 +1703,64 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
         }
         live = live && ((__ballot(ah) >> (st * kSelfGroup)) & 0xFFFFull) != 0ull;
     }
-    // Pairs come in blocks: rows = up to 64 spheres of joint A, columns = up
-    // to 64 spheres of joint B, a 64-bit mask of partners per row. Each
-    // joint's enclosing sphere contains all of its spheres, which culls at
-    // two levels for a state: a block whose two enclosing spheres are at
-    // clearance >= band holds no pair below band; within a block, a row
-    // sphere at clearance >= band from B's enclosing sphere (a column sphere
-    // from A's) has no partner below band. band = 0 for the any-collision
-    // check, where "below" is the strict d^2 < (r_i + r_j)^2 test. The 16
-    // lanes of a state test the spheres, then each surviving row walks its
-    // surviving partners.
+    // Pairs come in blocks by the sphere clusters of their two spheres; a
+    // cluster's enclosing sphere contains all of its spheres, so a block
+    // whose two enclosing spheres are at clearance >= band holds no pair
+    // below band (band = 0 for the any-collision check, where "below" is the
+    // strict d^2 < (r_i + r_j)^2 test). The 16 lanes of a state test 16
+    // blocks at once; then every block that passes for some state of the
+    // wave has its pair list walked, 16 pairs per step.
     const float band = a.mode == 0 ? 0.0f : a.band;
-    // near: clearance of (p, r) to the enclosing sphere e is below band
-    auto near = [&](float4 p, float4 e) {
-        const float dx = p.x - e.x, dy = p.y - e.y, dz = p.z - e.z;
+    auto near = [&](float4 e, float4 f) {
+        const float dx = e.x - f.x, dy = e.y - f.y, dz = e.z - f.z;
         const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        const float rs = p.w + e.w;
-        return e.w < 0.0f || (a.mode == 0 ? d2 < rs * rs : sqrtf(d2) - rs < band);
-    };
-    // 64-bit mask of the spheres [s0, s0 + n) of this state near e
-    auto near_mask = [&](int s0, int n, float4 e) {
-        unsigned long long m = 0ull;
-        for (int r0 = 0; r0 < n; r0 += kSelfGroup) {
-            const int r = r0 + sub;
-            const bool nb = r < n && near(pos[s0 + r], e);
-            m |= ((__ballot(nb) >> (st * kSelfGroup)) & 0xFFFFull) << r0;
-        }
-        return m;
+        const float rs = e.w + f.w;
+        return a.mode == 0 ? d2 < rs * rs : sqrtf(d2) - rs < band;
     };
     float best = 1000.0f;
-    int bkey = INT_MAX;   // (block, row, column) of the best pair: ties go to the first
+    int bkey = INT_MAX;   // (block << 16 | pair) of the best pair: ties go to the first
     bool hit = false;
-    for (int b = 0; b < R.n_sblk; ++b) {
-        const int4 h = R.sblk[2 * b], c = R.sblk[2 * b + 1];   // (jA, jB, a0, na), (b0, nb, mask offset, -)
-        const float4 ea = jnt[h.x], eb = jnt[h.y];
-        const bool gate = live && !hit && near(make_float4(ea.x, ea.y, ea.z, ea.w < 0.0f ? 0.0f : ea.w), eb);
-        if (!wave_any(gate)) continue;
-        const unsigned long long ma = gate ? near_mask(h.z, h.w, eb) : 0ull;
-        const unsigned long long mb = gate && ma ? near_mask(c.x, c.y, ea) : 0ull;
-        for (int r0 = 0; r0 < h.w; r0 += kSelfGroup) {
-            const int r = r0 + sub;
-            unsigned long long cols = 0ull;
-            if (r < h.w && ((ma >> r) & 1ull)) cols = R.rowmask[c.z + r] & mb;
-            if (!wave_any(cols != 0ull)) continue;
-            const float4 u = pos[h.z + (r < h.w ? r : 0)];
-            while (cols) {
-                const int j = __builtin_ctzll(cols);
-                cols &= cols - 1ull;
-                const float4 v = pos[c.x + j];
+    for (int b0 = 0; b0 < R.n_cpb; b0 += kSelfGroup) {
+        const int bb = b0 + sub;
+        bool g = false;
+        if (live && !hit && bb < R.n_cpb) {
+            const int4 hb = R.cpb[bb];
+            g = near(cenc[hb.x], cenc[hb.y]);
+        }
+        const unsigned long long bal = __ballot(g);
+        const unsigned mine = (unsigned)(bal >> (st * kSelfGroup)) & 0xFFFFu;
+        unsigned todo = (unsigned)(bal | (bal >> 16) | (bal >> 32) | (bal >> 48)) & 0xFFFFu;   // wave-uniform
+        while (todo) {
+            const int t = __builtin_ctz(todo);
+            todo &= todo - 1u;
+            const int b = b0 + t;
+            const int first = uniform(R.cpb[b].z), n = uniform(R.cpb[b].w);
+            const bool on = ((mine >> t) & 1u) != 0u && !hit;
+            for (int k0 = 0; k0 < n; k0 += kSelfGroup) {
+                const int k = k0 + sub;
+                if (!on || k >= n) continue;
+                const unsigned pr = R.cpl[first + k];
+                const float4 u = pos[pr & 0xFFFFu], v = pos[pr >> 16];
                 const float ux = u.x - v.x, uy = u.y - v.y, uz = u.z - v.z;
                 const float e2 = fmaf(uz, uz, fmaf(uy, uy, ux * ux));
                 if (a.mode == 0) {
                     const float r2 = u.w + v.w;
-                    if (e2 - r2 * r2 < 0.0f) {
-                        hit = true;
-                        cols = 0ull;
-                    }
+                    if (e2 - r2 * r2 < 0.0f) hit = true;
                 } else {
                     const float cl = sqrtf(e2) - (u.w + v.w);
-                    const int key = (b << 12) | (r << 6) | j;
+                    const int key = (b << 16) | k;
                     if (cl < best || (cl == best && key < bkey)) {
                         best = cl;
                         bkey = key;
                     }
                 }
             }
+            if (a.mode == 0) {
+                // share the state's verdict so its lanes stop
+                hit = ((__ballot(hit) >> (st * kSelfGroup)) & 0xFFFFull) != 0ull;
+            }
         }
-        if (a.mode == 0) {
-            // share the state's verdict so its lanes stop
-            hit = ((__ballot(hit) >> (st * kSelfGroup)) & 0xFFFFull) != 0ull;
-            if (__ballot(live && !hit) == 0ull) break;
-        }
+        if (a.mode == 0 && __ballot(live && !hit) == 0ull) break;
     }
     if (a.mode == 0) {
         if (active && sub == 0) {
@@ -1796,8 +1784,8 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
             float* o = a.out + 8 * (size_t)ref.gs;
             o[0] = best;
             if (bkey != INT_MAX) {
-                const int b = bkey >> 12, r = (bkey >> 6) & 63, j = bkey & 63;
-                const int si = R.sblk[2 * b].z + r, sj = R.sblk[2 * b + 1].x + j;
+                const unsigned pr = R.cpl[R.cpb[bkey >> 16].z + (bkey & 0xFFFF)];
+                const int si = (int)(pr & 0xFFFFu), sj = (int)(pr >> 16);
                 const float4 u = pos[si], v = pos[sj];
                 const float dx = u.x - v.x, dy = u.y - v.y, dz = u.z - v.z;
                 const float inv = 1.0f / sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
@@ -2611,7 +2599,7 @@ hipError_t launch_self(const SelfArgs& a, hipStream_t s) {
     if (a.X.n_dyn && blocks > 16384) blocks = 16384;
     if (blocks <= 0) return hipSuccess;
     const bool gate = a.mode == 0 && !a.R.self_gate_implied;   // approximate pair positions
-    const size_t smem = sizeof(float4) * (size_t)kSelfStates * (a.R.n_sph + a.R.n_joints) +
+    const size_t smem = sizeof(float4) * (size_t)kSelfStates * (a.R.n_sph + a.R.n_clu) +
                         sizeof(float) * (size_t)kSelfStates * (a.R.n_joints * 12 + 4) +
                         sizeof(float) * (size_t)kMaxDof * kSelfStates + sizeof(int) * (size_t)kSelfStates * a.R.n_joints +
                         sizeof(int) * 2 * kWave + (gate ? sizeof(float4) * (size_t)kSelfStates * a.R.n_apx : 0);

@@ -52,10 +52,9 @@ struct DRobot {
     const float2* lim;          // [D] joint limits (lower, upper)
     const float4* sph_orig;     // [S] spheres by original index
     const int2* pairs;          // [n_pairs] exact self pairs (original indices), grouped by joint pair
-    const int4* sblk;           // [n_sblk][2] self-pair blocks: (DFS joint A, DFS joint B, first slot a0,
-                                // rows na), (first slot b0, columns nb, row-mask offset, -); na, nb <= 64
-    const unsigned long long* rowmask;   // per block row: bit j = partner b0 + j (slots = emission order)
-    int n_sblk;
+    const int4* cpb;            // [n_cpb] self-pair blocks: (cluster A, cluster B, first pair, pairs), A <= B
+    const unsigned* cpl;        // [n_pairs] block pair lists: slot a | slot b << 16 (emission slots)
+    int n_cpb;
     const int2* apx_pairs;      // [n_apx_pairs] approx self pairs (approx original indices)
     const float4* apx_orig;     // [A] approx spheres by original index
     const float4* jenc;         // [J] link-frame sphere containing every sphere of DFS joint k (w < 0: none)
