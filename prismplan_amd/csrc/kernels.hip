@@ -1720,13 +1720,14 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
     float best = 1000.0f;
     int bkey = INT_MAX;   // (block << 16 | pair) of the best pair: ties go to the first
     bool hit = false;
+    // lane sub holds the entry of block b0 + sub, the next pass's loaded ahead;
+    // a walked block's list comes from its lane in the first state group
+    int4 nb4 = sub < R.n_cpb ? R.cpb[sub] : make_int4(0, 0, 0, 0);
     for (int b0 = 0; b0 < R.n_cpb; b0 += kSelfGroup) {
         const int bb = b0 + sub;
-        bool g = false;
-        if (live && !hit && bb < R.n_cpb) {
-            const int4 hb = R.cpb[bb];
-            g = near(cenc[hb.x], cenc[hb.y]);
-        }
+        const int4 hb = nb4;
+        if (bb + kSelfGroup < R.n_cpb) nb4 = R.cpb[bb + kSelfGroup];
+        const bool g = live && !hit && bb < R.n_cpb && near(cenc[hb.x], cenc[hb.y]);
         const unsigned long long bal = __ballot(g);
         const unsigned mine = (unsigned)(bal >> (st * kSelfGroup)) & 0xFFFFu;
         unsigned todo = (unsigned)(bal | (bal >> 16) | (bal >> 32) | (bal >> 48)) & 0xFFFFu;   // wave-uniform
@@ -1734,7 +1735,7 @@ __global__ __launch_bounds__(64) void k_self(SelfArgs a) {
             const int t = __builtin_ctz(todo);
             todo &= todo - 1u;
             const int b = b0 + t;
-            const int first = uniform(R.cpb[b].z), n = uniform(R.cpb[b].w);
+            const int first = __builtin_amdgcn_readlane(hb.z, t), n = __builtin_amdgcn_readlane(hb.w, t);
             const bool on = ((mine >> t) & 1u) != 0u && !hit;
             for (int k0 = 0; k0 < n; k0 += kSelfGroup) {
                 const int k = k0 + sub;
