@@ -557,7 +557,10 @@ __device__ __forceinline__ float wave_sum(float v) {
 // Any-collision: two-phase gate semantics (two_phase_cc.cuh:57-138 with the
 // check kernel's environment pass, pRRTC.cu:2337-2386).
 // ===========================================================================
-__global__ __launch_bounds__(64) void k_collide(CollideArgs a) {
+// Five waves per SIMD (95 VGPRs, no spills): the chunk loads and the joint
+// loop's LDS reads leave half the wave time waiting, which a fifth resident
+// wave hides better than the register room four waves would give.
+__global__ __launch_bounds__(64, 5) void k_collide(CollideArgs a) {
     extern __shared__ float4 s_lds[];
     float4* s_obb = s_lds;
     float* s_q = reinterpret_cast<float*>(s_lds + 4 * a.C.obb_cap);
