@@ -320,6 +320,35 @@ def test_batched_edges_first_hit(robots, n_obbs, seed):
             assert abs(clr[off[k] + lo]) < BAND, f"edge {k}: first hit {fh_all[k]} vs {want}"
 
 
+def test_first_hits_across_calls_and_plan_forms(robots):
+    """First hits with every state checked equal those of the skipping form
+    (which stops at a known hit) over consecutive calls on one plan with
+    alternating inputs and scenes, a three-problem plan with its own OBB
+    ranges, skip_first, the self pass, and the gated kernel (a model whose
+    approximate spheres do not enclose the exact ones)."""
+    from parity_common import model_variant
+    e1 = W.edges(n_edges=1500, n_obbs=40, seed=61)
+    e2 = W.edges(n_edges=1500, n_obbs=12, seed=62)
+    n_states = np.maximum(e1.n_states, 1)
+    for name, rob in (("panda", robots["panda"]), ("panda gated", S.Robot(model_variant("panda", "shrunk")))):
+        for skip_first in (False, True):
+            # three problems with their own OBB ranges
+            pu = np.array([0, 400, 900, len(n_states)], np.int32)
+            plan = S.Plan.edges(rob, n_states + int(skip_first), skip_first=skip_first, problem_units=pu,
+                                obb_offsets=np.array([0, 4, 8, 12], np.int32))
+            ds1, ds2 = S.DeviceScene.from_host(e1.scene), S.DeviceScene.from_host(e2.scene)
+            runs = [(ds1, e1), (ds2, e2), (ds1, e2), (ds2, e1), (ds1, e1)]
+            for k, (ds, e) in enumerate(runs):
+                frm, to = torch.from_numpy(e.frm).to(DEV), torch.from_numpy(e.to).to(DEV)
+                for check_self in (False, True):
+                    ref = S.edge_check_batched(plan, ds, frm, to, check_self=check_self, skip_resolved=True)
+                    got = S.edge_check_batched(plan, ds, frm, to, check_self=check_self, skip_resolved=False)
+                    torch.testing.assert_close(got, ref, rtol=0, atol=0,
+                                               msg=f"{name} skip_first={skip_first} call {k} self={check_self}")
+            banner(f"{name} skip_first={skip_first}: first hits equal with and without skipping over "
+                   f"{len(runs) * 4} calls (hit edges {float((ref >= 0).float().mean()):.2f})")
+
+
 def test_self_culling_without_enclosing_approx_spheres():
     """The self pass culls pair groups with spheres it builds around each
     joint's spheres, not with the approximate model: with approximate spheres
