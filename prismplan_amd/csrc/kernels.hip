@@ -362,6 +362,18 @@ __device__ __forceinline__ void stage_unit_ends(const DStates& X, int u0, int* s
     s_end[kWave + lane_id()] = in ? off[u] : INT_MAX;
 }
 
+// The same two loads into registers, for a chunk ahead.
+__device__ __forceinline__ void prefetch_unit_ends(const DStates& X, int u0, int& e_end, int& e_beg) {
+    if (X.source == kSrcConfig || u0 < 0) return;
+    const bool dense = X.source == kSrcDense || X.source == kSrcBfs;
+    const int* off = X.source == kSrcEdge ? X.edge_off : dense ? X.dense_off : X.traj_off;
+    const int n = X.source == kSrcEdge ? X.n_edges : X.n_traj;
+    const int u = u0 + lane_id();
+    const bool in = u < n;
+    e_end = in ? off[u + 1] : INT_MAX;
+    e_beg = in ? off[u] : INT_MAX;
+}
+
 // Unit holding state g and its state range [*u_begin, *u_end): the first of
 // units u0 .. u0+63 whose end exceeds g (bisection over the staged ends),
 // else a binary search over the offset table.
@@ -576,7 +588,15 @@ __global__ __launch_bounds__(64, 5) void k_collide(CollideArgs a) {
     ChunkCursor cur = chunk_cursor();
     const long n_states = state_count(a.X);
     const int n_chunks = chunk_count(a.C, n_states);
-    while (next_chunk(a.C, n_states, n_chunks, cur, ch)) {
+    // the next chunk's table entry is loaded at the top of each round and its
+    // unit ends once this chunk's states are in, so both arrive during the
+    // joint loop instead of heading the next round's chain of loads
+    bool have = next_chunk(a.C, n_states, n_chunks, cur, ch);
+    int pf_end = 0, pf_beg = 0;
+    if (have) prefetch_unit_ends(a.X, ch.w, pf_end, pf_beg);
+    while (have) {
+        int4 chn;
+        const bool have_n = next_chunk(a.C, n_states, n_chunks, cur, chn);
         if (ch.x != cur_prob) {
             problem_obbs(a.C, a.S, ch.x, o0, o1);
             n_tiles = uniform(obb_tiles(o0, o1, cap));
@@ -588,7 +608,10 @@ __global__ __launch_bounds__(64, 5) void k_collide(CollideArgs a) {
         }
         const long g = (long)ch.y + lane;
         bool active = g < ch.z;
-        stage_unit_ends(a.X, ch.w, s_end);
+        if (a.X.source != kSrcConfig && ch.w >= 0) {
+            s_end[lane] = pf_end;
+            s_end[kWave + lane] = pf_beg;
+        }
         __syncthreads();
         StateRef ref = load_state(a.X, D, g, active, s_q, ch.w, s_end, ch.x);
         bool hit = false;
@@ -597,6 +620,7 @@ __global__ __launch_bounds__(64, 5) void k_collide(CollideArgs a) {
             const unsigned fh = (unsigned)a.first_hit[ref.unit];
             if (fh < (unsigned)ref.k) active = false;
         }
+        if (have_n) prefetch_unit_ends(a.X, chn.w, pf_end, pf_beg);
         __syncthreads();
         // A problem with more OBBs than one staging holds is checked tile by
         // tile (FK redone per tile); bit jj of `flagged`: joint jj's
@@ -739,6 +763,8 @@ __global__ __launch_bounds__(64, 5) void k_collide(CollideArgs a) {
                 if ((hm & same) == 0ull) atomicMin((unsigned*)&a.first_hit[ref.unit], (unsigned)ref.k);
             }
         }
+        ch = chn;
+        have = have_n;
     }
 }
 
