@@ -950,6 +950,10 @@ __device__ __forceinline__ void near_obbs(const float4* __restrict__ s_obb, int 
 // OBB (LDS record index; -1 when there is none). Ties keep the lowest index.
 // A (sphere, OBB) pair the scene's ACM masks out is skipped
 // (collision/scene_collision.cuh:140-167: sphere_obb_clearance by sphere index).
+// PAIR: two candidates per step, both records read before either test, so
+// one record's LDS latency hides the other's (the cost-only kernel has the
+// registers for it; the adjoint instantiation would spill).
+template <bool PAIR>
 __device__ __forceinline__ void obb_argmin(const float4* __restrict__ s_obb, const ObbMask& mk, float px, float py,
                                            float pz, const uint8_t* __restrict__ acm_row, float& best, int& arg) {
     best = FLT_MAX;
@@ -957,14 +961,37 @@ __device__ __forceinline__ void obb_argmin(const float4* __restrict__ s_obb, con
 #pragma unroll
     for (int k = 0; k < kMaskWords; ++k) {
         unsigned long long bits = mk.w[k];
-        while (bits) {
-            const int o = k * 64 + (__ffsll(bits) - 1);
-            bits &= bits - 1ull;
-            if (acm_row && acm_row[__float_as_int(s_obb[4 * o + 3].w)] == 0) continue;
-            float lx, ly, lz;
-            to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], px, py, pz, lx, ly, lz);
-            const float key = box_key(lx, ly, lz, s_obb[4 * o + 3]);
-            if (key < best) { best = key; arg = o; }
+        if (PAIR) {
+            while (bits) {
+                const int oa = k * 64 + (__ffsll(bits) - 1);
+                bits &= bits - 1ull;
+                const bool two = bits != 0ull;
+                const int ob = two ? k * 64 + (__ffsll(bits) - 1) : oa;
+                if (two) bits &= bits - 1ull;
+                const float4 a0 = s_obb[4 * oa], a1 = s_obb[4 * oa + 1], a2 = s_obb[4 * oa + 2], ah = s_obb[4 * oa + 3];
+                const float4 b0 = s_obb[4 * ob], b1 = s_obb[4 * ob + 1], b2 = s_obb[4 * ob + 2], bh = s_obb[4 * ob + 3];
+                float lx, ly, lz;
+                to_box(a0, a1, a2, px, py, pz, lx, ly, lz);
+                float ka = box_key(lx, ly, lz, ah);
+                to_box(b0, b1, b2, px, py, pz, lx, ly, lz);
+                float kb = box_key(lx, ly, lz, bh);
+                if (acm_row) {
+                    if (acm_row[__float_as_int(ah.w)] == 0) ka = FLT_MAX;
+                    if (acm_row[__float_as_int(bh.w)] == 0) kb = FLT_MAX;
+                }
+                if (ka < best) { best = ka; arg = oa; }   // oa < ob: ties keep the lower index
+                if (two && kb < best) { best = kb; arg = ob; }
+            }
+        } else {
+            while (bits) {
+                const int o = k * 64 + (__ffsll(bits) - 1);
+                bits &= bits - 1ull;
+                if (acm_row && acm_row[__float_as_int(s_obb[4 * o + 3].w)] == 0) continue;
+                float lx, ly, lz;
+                to_box(s_obb[4 * o], s_obb[4 * o + 1], s_obb[4 * o + 2], px, py, pz, lx, ly, lz);
+                const float key = box_key(lx, ly, lz, s_obb[4 * o + 3]);
+                if (key < best) { best = key; arg = o; }
+            }
         }
     }
 }
@@ -1090,7 +1117,7 @@ __global__ __launch_bounds__(64, 4) void k_cost(CostArgs a) {
                     int arg = -1;
                     const uint8_t* acm_row =
                         a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)R.sph_id[s] * a.S.n_obbs : nullptr;
-                    if (use_obb) obb_argmin(s_obb, mk, px, py, pz, acm_row, best, arg);
+                    if (use_obb) obb_argmin<!GRAD>(s_obb, mk, px, py, pz, acm_row, best, arg);
                     float gx, gy, gz;
                     auto rec = [&](float4& r0, float4& r1, float4& r2, float4& r3) {
                         r0 = s_obb[4 * arg]; r1 = s_obb[4 * arg + 1]; r2 = s_obb[4 * arg + 2]; r3 = s_obb[4 * arg + 3];
