@@ -882,45 +882,6 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     for (int a = 0; a < A; ++a)
         apx_orig[a] = make_float4(d->approx_spheres[4 * a], d->approx_spheres[4 * a + 1], d->approx_spheres[4 * a + 2],
                                   d->approx_spheres[4 * a + 3]);
-    // Sphere around each joint's spheres for the self pass's group culling:
-    // the joint's single approximate sphere when it contains them all (to
-    // 1e-6 m), else a centroid sphere that does; 1e-5 m larger so rounding in
-    // the FP32 transforms cannot make a contained sphere poke out. w < 0: no
-    // spheres.
-    std::vector<float4> jenc(J, make_float4(0.0f, 0.0f, 0.0f, -1.0f));
-    for (int k = 0; k < J; ++k) {
-        const int s0 = sph_off[k], s1 = sph_off[k + 1];
-        if (s0 == s1) continue;
-        auto reach = [&](double cx, double cy, double cz) {
-            double m = 0.0;
-            for (int s = s0; s < s1; ++s) {
-                const double dx = sph[s].x - cx, dy = sph[s].y - cy, dz = sph[s].z - cz;
-                m = std::max(m, std::sqrt(dx * dx + dy * dy + dz * dz) + sph[s].w);
-            }
-            return m;
-        };
-        double cx = 0.0, cy = 0.0, cz = 0.0;
-        for (int s = s0; s < s1; ++s) {
-            cx += sph[s].x;
-            cy += sph[s].y;
-            cz += sph[s].z;
-        }
-        cx /= (s1 - s0);
-        cy /= (s1 - s0);
-        cz /= (s1 - s0);
-        double rad = reach(cx, cy, cz);
-        if (apx_off[k + 1] - apx_off[k] == 1) {
-            const float4 a = apx[apx_off[k]];
-            const double ra = reach(a.x, a.y, a.z);   // may exceed a.w by FP32 rounding of the tables
-            if (ra <= (double)a.w + 1e-6 && ra <= rad) {
-                cx = a.x;
-                cy = a.y;
-                cz = a.z;
-                rad = std::max(ra, (double)a.w);
-            }
-        }
-        jenc[k] = make_float4((float)cx, (float)cy, (float)cz, (float)(rad + 1e-5));
-    }
     // Parent of each DFS position, as a DFS position (root: itself).
     std::vector<int> close(J, 0);
     for (int k = 0; k < J; ++k) close[k] = k == 0 ? 0 : pos[parents[dfs[k]]];
@@ -951,7 +912,7 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
                  o_apairs = place(apairs.size() * 8), o_aorig = place(apx_orig.size() * 16),
                  o_cpb = place(cpb.size() * 16), o_cpl = place(cpl.size() * 4), o_jorig = place(J * 4),
                  o_lim = place((size_t)D * 8),
-                 o_jenc = place((size_t)J * 16), o_sjnt = place((size_t)S * 4), o_jend = place((size_t)J * 4),
+                 o_sjnt = place((size_t)S * 4), o_jend = place((size_t)J * 4),
                  o_aft = place(aft.size() * 4), o_atype = place(atype.size() * 4), o_ajnt = place(apx_jnt.size() * 4),
                  o_coff = place(clu_off.size() * 4), o_cbeg = place(clu_beg.size() * 4), o_cenc = place(clu_enc.size() * 16),
                  o_sclu = place(sph_clu.size() * 4), o_cjnt = place(clu_jnt.size() * 4);
@@ -976,7 +937,6 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     std::vector<float2> lim(D);
     for (int k = 0; k < D; ++k) lim[k] = make_float2(d->joint_lower[k], d->joint_upper[k]);
     put(o_lim, lim.data(), (size_t)D * 8);
-    put(o_jenc, jenc.data(), (size_t)J * 16);
     put(o_sjnt, sph_jnt.data(), (size_t)S * 4);
     put(o_jend, jend.data(), (size_t)J * 4);
     put(o_aft, aft.data(), aft.size() * 4);
@@ -1023,7 +983,6 @@ ppl_status ppl_robot_create(const ppl_robot_desc* d, ppl_robot* out) try {
     R.cpl = reinterpret_cast<const unsigned*>(b + o_cpl);
     R.jorig = reinterpret_cast<const int*>(b + o_jorig);
     R.lim = reinterpret_cast<const float2*>(b + o_lim);
-    R.jenc = reinterpret_cast<const float4*>(b + o_jenc);
     R.sph_jnt = reinterpret_cast<const int*>(b + o_sjnt);
     R.jend = reinterpret_cast<const int*>(b + o_jend);
     R.aft = own_chain ? reinterpret_cast<const float*>(b + o_aft) : nullptr;

@@ -907,14 +907,15 @@ __global__ __launch_bounds__(64) void k_collide_gated(CollideArgs a) {
 // recorded during the forward walk. Lever arms are taken relative to each
 // joint, so no large world-origin moments are summed and subtracted.
 // ===========================================================================
-// Candidate OBBs of one joint for the hinge: a wave-uniform bitmask of the
-// OBBs some active lane's spheres of this joint can come within the hinge
-// band of. Every sphere s lies inside the joint's enclosing sphere (centre
-// a, radius R; DRobot::jenc), so clr(s, o) >= sdf_o(a) - R; the active lanes' centres lie
-// within rw of the first active lane's centre c0, so sdf_o(a) >= sdf_o(c0) -
-// rw. An OBB with sdf_o(c0) >= band + R + rw therefore has clearance >= band
-// from every sphere: it cannot change a hinge, its argmin or its gradient.
-// One pass per 64 OBBs (lane i tests OBB base + i).
+// Candidate OBBs of one sphere cluster for the hinge: a wave-uniform bitmask
+// of the OBBs some active lane's spheres of the cluster can come within the
+// hinge band of. Every sphere s of the cluster lies inside its enclosing
+// sphere `as` (centre a, radius R; DRobot::clu_enc), so clr(s, o) >=
+// sdf_o(a) - R; the active lanes' centres lie within rw of the first active
+// lane's centre c0, so sdf_o(a) >= sdf_o(c0) - rw. An OBB with sdf_o(c0) >=
+// band + R + rw therefore has clearance >= band from every sphere: it cannot
+// change a hinge, its argmin or its gradient. One pass per 64 OBBs (lane i
+// tests OBB base + i).
 __device__ __forceinline__ void near_obbs(const float4* __restrict__ s_obb, int n_obb, const float* T, const float4& as,
                                           float band, bool active, ObbMask& mk) {
     float ax, ay, az;

@@ -57,7 +57,6 @@ struct DRobot {
     int n_cpb;
     const int2* apx_pairs;      // [n_apx_pairs] approx self pairs (approx original indices)
     const float4* apx_orig;     // [A] approx spheres by original index
-    const float4* jenc;         // [J] link-frame sphere containing every sphere of DFS joint k (w < 0: none)
     const int* sph_jnt;         // [S] DFS joint of emission slot k
     const int* jend;            // [J] end of DFS position k's subtree: its DFS positions are [k, jend[k])
     // The approximate spheres' own chain (DFS order), NULL when they ride the
