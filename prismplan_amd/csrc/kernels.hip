@@ -1492,21 +1492,32 @@ __device__ __forceinline__ void cost_grp(const CostArgs& a) {
             self_h = fmaxf(a.P.self_margin - so[0], 0.0f);
             if (self_h > 0.0f) { si = __float_as_int(so[1]); sj = __float_as_int(so[2]); snx = so[3]; sny = so[4]; snz = so[5]; }
         }
-        // 2. sphere pass, in rounds of 16 emission slots per group: each lane
-        // places its sphere with its joint's frame and walks its joint's
+        // 2. sphere pass, in rounds of 16 emission slots per state: each lane
+        // places its sphere with its joint's frame and walks its cluster's
         // candidate mask for the argmin box (tiled: reads the folded argmin
         // and rebuilds its record from the scene when the gradient needs it).
         float world = 0.0f, minclr = 1000.0f;
+        // In this pass lane = 4 * sphere slot + state: the four states of a
+        // sphere walk its cluster's mask together, so each quarter-wave of an
+        // LDS record read touches 4 records (broadcast) instead of 16 (bank
+        // conflicts). Each state's spheres keep their lanes' relative order,
+        // so sums and atomics run in the same order as with lane = 16 *
+        // state + slot.
+        const int rslot = lane & (kGrpStates - 1), rsub = lane >> 2;
+        const bool ractive = (long)ch.y + rslot < ch.z;
+        const float* Tr = s_T + rslot * fstride;
+        const int r_si = __shfl(si, rslot * kGrp), r_sj = __shfl(sj, rslot * kGrp);
+        const float r_snx = __shfl(snx, rslot * kGrp), r_sny = __shfl(sny, rslot * kGrp), r_snz = __shfl(snz, rslot * kGrp);
         // wrench of a sphere's point gradient g at p about its joint's origin o
         auto add_wrench = [&](int jj, int sid, float px, float py, float pz, float ox, float oy, float oz, float gx,
                               float gy, float gz) {
-            if (si >= 0) {
-                if (sid == si) { gx -= snx; gy -= sny; gz -= snz; }
-                if (sid == sj) { gx += snx; gy += sny; gz += snz; }
+            if (r_si >= 0) {
+                if (sid == r_si) { gx -= r_snx; gy -= r_sny; gz -= r_snz; }
+                if (sid == r_sj) { gx += r_snx; gy += r_sny; gz += r_snz; }
             }
             if (gx != 0.0f || gy != 0.0f || gz != 0.0f) {
                 const float rx = px - ox, ry = py - oy, rz = pz - oz;
-                float* w = s_w + (slot * J + jj) * 6;
+                float* w = s_w + (rslot * J + jj) * 6;
                 atomicAdd(w + 0, gx);
                 atomicAdd(w + 1, gy);
                 atomicAdd(w + 2, gz);
@@ -1516,11 +1527,11 @@ __device__ __forceinline__ void cost_grp(const CostArgs& a) {
             }
         };
         for (int r0 = 0; r0 < S; r0 += kGrp) {
-            const int s = r0 + sub;
-            const bool valid = active && s < S;
+            const int s = r0 + rsub;
+            const bool valid = ractive && s < S;
             const int sc = valid ? s : r0;
             const int jj = R.sph_jnt[sc];
-            const float4* Tq = reinterpret_cast<const float4*>(Ts + 12 * jj);
+            const float4* Tq = reinterpret_cast<const float4*>(Tr + 12 * jj);
             const float4 t0 = Tq[0], t1 = Tq[1], t2 = Tq[2];
             const float4 sp = R.sph[sc];
             const int sid = R.sph_id[sc];
@@ -1531,8 +1542,8 @@ __device__ __forceinline__ void cost_grp(const CostArgs& a) {
             int arg = -1;   // tiled: the argmin's original OBB index; else its LDS record
             if (tiled) {
                 if (valid) {
-                    best = s_best[slot * S + s];
-                    arg = s_barg[slot * S + s];
+                    best = s_best[rslot * S + s];
+                    arg = s_barg[rslot * S + s];
                 }
             } else if (use_obb) {
                 const uint8_t* acm_row = a.S.sphere_obb_acm ? a.S.sphere_obb_acm + (size_t)sid * a.S.n_obbs : nullptr;
@@ -1555,6 +1566,14 @@ __device__ __forceinline__ void cost_grp(const CostArgs& a) {
             else world += hs;
             if (want_grad) add_wrench(jj, sid, px, py, pz, t0.w, t1.w, t2.w, gx, gy, gz);
         }
+        // each state's partial sums sit in the lanes with its lane & 3; the
+        // pairwise tree matches a 16-lane group sum's
+        for (int o = kGrpStates; o < kWave; o <<= 1) {
+            world += __shfl_xor(world, o);
+            minclr = fminf(minclr, __shfl_xor(minclr, o));
+        }
+        world = __shfl(world, slot);
+        minclr = __shfl(minclr, slot);
         // 3. adjoint: subtree wrench of joint `sub` -> d cost / d q
         if (want_grad) {
             __syncthreads();
@@ -1614,18 +1633,18 @@ __device__ __forceinline__ void cost_grp(const CostArgs& a) {
             }
         }
         if (a.clearance_only) {
-            minclr = group_min(minclr, kGrp);
             if (active && sub == 0) a.cost_out[ref.gs] = minclr;
         } else {
-            world = group_sum(world, kGrp);
             if (active && sub == 0 && a.cost_out) a.cost_out[ref.gs] = wt * (world + self_h);
         }
         __syncthreads();   // frames, masks and wrenches of this chunk are dead before the next one
     }
 }
 
+// Four waves per SIMD for the one-staging variants (the adjoint's 128 VGPRs);
+// the tiled ones would spill there.
 template <bool GRAD, int WORLD, bool TILED>
-__global__ __launch_bounds__(64) void k_cost_grp(CostArgs a) {
+__global__ __launch_bounds__(64, TILED ? 1 : 4) void k_cost_grp(CostArgs a) {
     cost_grp<GRAD, WORLD, TILED>(a);
 }
 
