@@ -1325,12 +1325,30 @@ __device__ __forceinline__ void grp_masks(const DRobot& R, const float4* __restr
         d2 = fmaxf(d2, __shfl_xor(d2, 16));
         d2 = fmaxf(d2, __shfl_xor(d2, 32));
         const int nk = min(kGrp, C - c0);
+        // up to 64 staged boxes: each lane reads its box's record once for
+        // every cluster of the pass
+        float4 q0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), q1 = q0, q2 = q0, qh = q0;
+        const bool one_word = n_obb <= 64;
+        if (one_word && lane < n_obb) {
+            q0 = s_obb[4 * lane]; q1 = s_obb[4 * lane + 1]; q2 = s_obb[4 * lane + 2]; qh = s_obb[4 * lane + 3];
+        }
         for (int k = 0; k < nk; ++k) {
             const float4 ce = R.clu_enc[c0 + k];
             ObbMask mk;
             const float c0x = read_lane(cx, k), c0y = read_lane(cy, k), c0z = read_lane(cz, k);
             const float reach = band + ce.w + sqrtf(read_lane(d2, k)) * 1.0001f + 1e-5f;
             const float thr = uniform(reach * fabsf(reach));
+            if (one_word) {
+                bool c = false;
+                if (lane < n_obb) {
+                    float lx, ly, lz;
+                    to_box(q0, q1, q2, c0x, c0y, c0z, lx, ly, lz);
+                    c = box_key(lx, ly, lz, qh) < thr;
+                }
+                mk.w[0] = __ballot(c);
+#pragma unroll
+                for (int w = 1; w < kMaskWords; ++w) mk.w[w] = 0ull;
+            } else
 #pragma unroll
             for (int w = 0; w < kMaskWords; ++w) {
                 bool c = false;
