@@ -81,20 +81,22 @@ def log(msg):
 
 
 def timed_loop(fn, steps, warmup, torch):
-    """Wall time per step (synchronized both sides) and mean HIP-event time of
-    each step on the current stream."""
+    """Wall time per step over exactly `steps` back-to-back steps, synchronized
+    on both sides, and the HIP-event time per step of the same loop (one event
+    pair around all of it: events recorded around every step would add their
+    own GPU work to each step)."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    for a, b in evs:
-        a.record()
+    a.record()
+    for _ in range(steps):
         fn()
-        b.record()
+    b.record()
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / steps
-    ev = sum(a.elapsed_time(b) for a, b in evs) / steps / 1e3
+    ev = a.elapsed_time(b) / steps / 1e3
     return wall, ev
 
 
