@@ -51,12 +51,12 @@ FLOPS_RULE = ("achieved = gated early-exit flops per state x states / kernel tim
 # SQ_INSTS_VALU_FLOPS_FP32 per dispatch, with the kernel time of that run.
 COUNTER_VALU_INSTS_EDGES = 15_633_400
 COUNTER_VALU_FLOPS_FP32_EDGES = 7_971_864
-COUNTER_KERNEL_US_EDGES = 41.35
+COUNTER_KERNEL_US_EDGES = 40.67
 CUS, SIMDS_PER_CU, CLOCK_HZ = 256, 4, 2.4e9
 # HBM bytes one headline launch moves (k_collide), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE scaled by a 256 MiB copy (same profile); the algorithmic bytes are
 # 16,384 edges x 69 B.
-TRAFFIC_BYTES_EDGES = 2_189_008
+TRAFFIC_BYTES_EDGES = 2_189_004
 ALGORITHMIC_BYTES_EDGES = 16384 * 69
 
 
