@@ -62,8 +62,8 @@ ALGORITHMIC_BYTES_EDGES = 16384 * 69
 
 def kernel_time(plan, fn, steps, torch):
     """Mean duration (s) of the plan's main kernel over `steps` calls of fn,
-    from the HIP event pairs the library records on the kernel's own stream
-    around each launch (ppl_plan_time_kernels)."""
+    from the HIP event pairs the runtime records at each launch's start and
+    end (ppl_plan_time_kernels)."""
     torch.cuda.synchronize()
     plan.kernel_ms()
     plan.time_kernels(True)
@@ -184,7 +184,8 @@ def main():
                 "flops_per_state": f_state, "flops_rule": FLOPS_RULE,
                 "exhaustive_flops_per_state": f_exh,
                 "exhaustive_frac": f_exh * total_states / ev / PEAK_FP32, "kernel_ms": ev * 1e3,
-                "kernel_ms_source": "HIP events the library records around each k_collide launch on its stream",
+                "kernel_ms_source": "HIP events the runtime records at each k_collide launch's start and end "
+                                    "(hipExtLaunchKernel, ppl_plan_time_kernels)",
                 "step_event_ms": ev_step * 1e3,
                 # counter-derived, from the committed profile of this kernel: lane flops if every lane of
                 # each FP32 VALU instruction were active (an upper bound), and VALU issue occupancy

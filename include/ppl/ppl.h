@@ -334,9 +334,10 @@ ppl_status ppl_plan_set_collide_shape(ppl_plan plan, int tile, int grid);
 
 /* Kernel timing (enable != 0: on): each later call of
  * ppl_check_collision_batched, ppl_edge_check_batched or the batched cost
- * evaluation on this plan records a HIP event pair on its stream around its
- * main kernel (k_collide or the cost kernel; not the first-hit reset or the
- * self pass), so the kernel's own duration can be read without a profiler.
+ * evaluation on this plan times its main kernel (k_collide or the cost
+ * kernel; not the first-hit reset or the self pass) with a HIP event pair
+ * that the runtime records at the kernel's own start and end
+ * (hipExtLaunchKernel), so its duration can be read without a profiler.
  * Calls under stream capture record nothing; at most 4096 launches are kept
  * until read. No counterpart in the reference. */
 ppl_status ppl_plan_time_kernels(ppl_plan plan, int enable);

@@ -265,8 +265,9 @@ DChunks implicit_chunks(long n_states, KernelKind k, const DRobot& R, int max_ob
 // state when the scene's OBBs fit one LDS staging, else the grouped kernel,
 // which streams them through LDS in tiles.
 // Kernel timing: with the plan's timing on, the batched checks and the cost
-// evaluation record an event pair on their stream around their main kernel
-// (not under stream capture; at most kMaxTimedLaunches pairs until read).
+// evaluation time their main kernel with an event pair the runtime records
+// at the kernel's start and end (not under stream capture; at most
+// kMaxTimedLaunches pairs until read).
 constexpr int kMaxTimedLaunches = 4096;
 
 bool timing_now(ppl_plan P, hipStream_t s) {
@@ -275,26 +276,23 @@ bool timing_now(ppl_plan P, hipStream_t s) {
     return hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone;
 }
 
-hipError_t time_begin(ppl_plan P, hipStream_t s, bool on) {
-    if (!on) return hipSuccess;
+// The plan's next event pair for a timed launch (created on first use); the
+// launch passes it to the runtime, which records it at the kernel's own
+// start and end.
+hipError_t timing_pair(ppl_plan P, hipEvent_t* e0, hipEvent_t* e1) {
     if ((int)P->kev.size() == P->kused) {
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        hipError_t e = hipEventCreate(&e0);
+        hipEvent_t a = nullptr, b = nullptr;
+        hipError_t e = hipEventCreate(&a);
         if (e != hipSuccess) return e;
-        if ((e = hipEventCreate(&e1)) != hipSuccess) {
-            (void)hipEventDestroy(e0);
+        if ((e = hipEventCreate(&b)) != hipSuccess) {
+            (void)hipEventDestroy(a);
             return e;
         }
-        P->kev.emplace_back(e0, e1);
+        P->kev.emplace_back(a, b);
     }
-    return hipEventRecord(P->kev[P->kused].first, s);
-}
-
-hipError_t time_end(ppl_plan P, hipStream_t s, bool on) {
-    if (!on) return hipSuccess;
-    const hipError_t e = hipEventRecord(P->kev[P->kused].second, s);
-    if (e == hipSuccess) ++P->kused;
-    return e;
+    *e0 = P->kev[P->kused].first;
+    *e1 = P->kev[P->kused].second;
+    return hipSuccess;
 }
 
 hipError_t launch_cost_flat(CostArgs& a, long n_states, hipStream_t s) {
@@ -1582,10 +1580,10 @@ ppl_status hinge_impl(ppl_plan_s* P, const ppl_scene_view* view, const float* d_
     a.self_in = self_in;
     a.cost_out = state;
     a.grad_out = d_grad;
-    const bool timed = timing_now(P, s);
-    HIP_TRY(time_begin(P, s, timed));
-    HIP_TRY(grouped ? launch_cost_grp(a, grid, s) : launch_cost(a, grid, s));
-    HIP_TRY(time_end(P, s, timed));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (timing_now(P, s)) HIP_TRY(timing_pair(P, &t0, &t1));
+    HIP_TRY(grouped ? launch_cost_grp(a, grid, s, t0, t1) : launch_cost(a, grid, s, t0, t1));
+    if (t0) ++P->kused;
     if (d_total && P->kind == PPL_PLAN_TRAJ) HIP_TRY(launch_segment_sum(state, P->d_off, P->n_units, 0.0f, d_total, s));
     return PPL_OK;
 }
@@ -2344,10 +2342,10 @@ ppl_status ppl_check_collision_batched(ppl_plan P, const ppl_scene_view* view, c
     a.mode = 0;
     a.out = d_out;
     a.self_hit = self_hit;
-    const bool timed = timing_now(P, s);
-    HIP_TRY(time_begin(P, s, timed));
-    HIP_TRY(launch_collide(a, grid, s));
-    HIP_TRY(time_end(P, s, timed));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (timing_now(P, s)) HIP_TRY(timing_pair(P, &t0, &t1));
+    HIP_TRY(launch_collide(a, grid, s, t0, t1));
+    if (t0) ++P->kused;
     return PPL_OK;
 } catch (const std::bad_alloc&) {
     return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");
@@ -2387,10 +2385,10 @@ ppl_status ppl_edge_check_batched(ppl_plan P, const ppl_scene_view* view, const 
     a.out = d_state_hits;
     a.first_hit = d_first_hit;
     a.self_hit = self_hit;
-    const bool timed = timing_now(P, s);
-    HIP_TRY(time_begin(P, s, timed));
-    HIP_TRY(launch_collide(a, grid, s));
-    HIP_TRY(time_end(P, s, timed));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (timing_now(P, s)) HIP_TRY(timing_pair(P, &t0, &t1));
+    HIP_TRY(launch_collide(a, grid, s, t0, t1));
+    if (t0) ++P->kused;
     return PPL_OK;
 } catch (const std::bad_alloc&) {
     return ppl::fail_msg(PPL_ERR_ALLOC, "out of host memory");

@@ -18,6 +18,8 @@
 //    the argmin OBB of each active sphere in the same pass.
 #include "ppl_internal.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cfloat>
 #include <climits>
@@ -2620,20 +2622,28 @@ int grid_cost(int n_joints, bool grad, size_t lds) {
     return g[idx];
 }
 
-hipError_t launch_collide(const CollideArgs& a, int grid, hipStream_t s) {
-    if (a.R.scene_gate_fused)
-        hipLaunchKernelGGL(k_collide, dim3(grid), dim3(64), lds_bytes(a.C.obb_cap, a.R.n_dof), s, a);
-    else
-        hipLaunchKernelGGL(k_collide_gated, dim3(grid), dim3(64), lds_bytes(a.C.obb_cap, a.R.n_dof), s, a);
-    return hipGetLastError();
+// A launch with its argument block; with t0/t1 the runtime records the two
+// events at the kernel's own start and end (timing without event packets of
+// their own around the launch).
+static hipError_t launch_args(const void* k, int grid, size_t lds, hipStream_t s, void* arg, hipEvent_t t0,
+                              hipEvent_t t1) {
+    void* params[] = {arg};
+    if (t0) return hipExtLaunchKernel(k, dim3(grid), dim3(64), params, lds, s, t0, t1, 0);
+    return hipLaunchKernel(k, dim3(grid), dim3(64), params, lds, s);
 }
 
-hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s) {
+hipError_t launch_collide(const CollideArgs& a, int grid, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+    CollideArgs arg = a;
+    const void* k = a.R.scene_gate_fused ? (const void*)k_collide : (const void*)k_collide_gated;
+    return launch_args(k, grid, lds_bytes(a.C.obb_cap, a.R.n_dof), s, &arg, t0, t1);
+}
+
+hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
     if (a.C.tiled) return hipErrorInvalidValue;   // k_cost stages one OBB tile: tiled problems take k_cost_grp
     const size_t lds = lds_bytes_cost(a.C.obb_cap, a.R.n_dof, a.R.n_joints, a.group, a.grad != 0);
-    if (a.grad) hipLaunchKernelGGL(k_cost<true>, dim3(grid), dim3(64), lds, s, a);
-    else hipLaunchKernelGGL(k_cost<false>, dim3(grid), dim3(64), lds, s, a);
-    return hipGetLastError();
+    CostArgs arg = a;
+    const void* k = a.grad ? (const void*)k_cost<true> : (const void*)k_cost<false>;
+    return launch_args(k, grid, lds, s, &arg, t0, t1);
 }
 
 size_t lds_bytes_grp(int obb_cap, int n_dof, int n_joints, int n_sph, int n_clu, bool grad, bool tiled) {
@@ -2676,15 +2686,14 @@ int grid_cost_grp(bool grad, int world, bool tiled, size_t lds) {
     return g[i][w][t];
 }
 
-hipError_t launch_cost_grp(const CostArgs& a, int grid, hipStream_t s) {
+hipError_t launch_cost_grp(const CostArgs& a, int grid, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
     const size_t lds = lds_bytes_grp(a.C.obb_cap, a.R.n_dof, a.R.n_joints, a.R.n_sph, a.R.n_clu, a.grad != 0,
                                      a.C.tiled != 0);
     const int world = grp_world(a);
     const bool tiled = a.C.tiled != 0 && world != 2;
     grid = std::max(1, std::min(grid, grid_cost_grp(a.grad != 0, world, tiled, lds)));   // persistent: occupancy x CUs
     CostArgs arg = a;
-    void* params[] = {&arg};
-    return hipLaunchKernel(grp_kernel(a.grad != 0, world, tiled), dim3(grid), dim3(64), params, lds, s);
+    return launch_args(grp_kernel(a.grad != 0, world, tiled), grid, lds, s, &arg, t0, t1);
 }
 
 hipError_t launch_self(const SelfArgs& a, hipStream_t s) {

@@ -187,9 +187,12 @@ struct SelfArgs {
                                 // indexed by plan order either way
 };
 
-hipError_t launch_collide(const CollideArgs& a, int grid, hipStream_t s);
-hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s);
-hipError_t launch_cost_grp(const CostArgs& a, int grid, hipStream_t s);
+// t0 / t1 (optional): events the runtime records at the kernel's start and end
+hipError_t launch_collide(const CollideArgs& a, int grid, hipStream_t s, hipEvent_t t0 = nullptr,
+                          hipEvent_t t1 = nullptr);
+hipError_t launch_cost(const CostArgs& a, int grid, hipStream_t s, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
+hipError_t launch_cost_grp(const CostArgs& a, int grid, hipStream_t s, hipEvent_t t0 = nullptr,
+                           hipEvent_t t1 = nullptr);
 hipError_t launch_self(const SelfArgs& a, hipStream_t s);
 hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t s);
 hipError_t launch_add_u32(unsigned* p, unsigned v, hipStream_t s);   // *p += v (one thread)
